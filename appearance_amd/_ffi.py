@@ -1,0 +1,178 @@
+"""ctypes binding of libapt (include/apt.h).
+
+The library is loaded from this package directory; there is no fallback.  If
+libapt.so is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libapt.so"
+
+APT_OK = 0
+APT_MISS = 0xFFFFFFFF
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "BUSY",
+                6: "NOT_FOUND"}
+
+
+class AptError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Ray(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("tmin", C.c_float), ("dir", C.c_float * 3),
+                ("tmax", C.c_float)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("prim", C.c_uint32)]
+
+
+class BvhOpts(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("max_leaf", C.c_uint32), ("bins", C.c_uint32),
+                ("builder", C.c_uint32)]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("n_tris", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
+                ("width", C.c_uint32), ("depth", C.c_uint32), ("max_stack", C.c_uint32),
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("sah_cost", C.c_double)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_uint32),
+                ("wavefront_size", C.c_uint32), ("hbm_bytes", C.c_uint64),
+                ("lds_bytes_per_block", C.c_uint64)]
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("color", C.c_float * 3), ("color_texture", C.c_uint32),
+        ("metallic", C.c_float), ("roughness", C.c_float),
+        ("metallic_roughness_texture", C.c_uint32), ("normal_scale", C.c_float),
+        ("emission", C.c_float * 3), ("normal_texture", C.c_uint32),
+        ("emission_texture", C.c_uint32), ("transmission", C.c_float), ("eta", C.c_float),
+        ("subsurface", C.c_float),
+        ("absorption", C.c_float * 3), ("specular", C.c_float),
+        ("specular_tint", C.c_float * 3), ("anisotropic", C.c_float),
+        ("sheen", C.c_float), ("sheen_texture", C.c_uint32), ("clearcoat", C.c_float),
+        ("clearcoat_texture", C.c_uint32),
+        ("clearcoat_roughness", C.c_float), ("clearcoat_roughness_texture", C.c_uint32),
+        ("alpha_cutoff", C.c_float), ("sheen_tint_texture", C.c_uint32),
+        ("clearcoat_normal_texture", C.c_uint32), ("bsdf_model", C.c_uint32),
+        ("_padding1", C.c_uint32), ("_padding2", C.c_uint32),
+        ("sheen_tint", C.c_float * 3), ("transmission_texture", C.c_uint32),
+    ]
+
+
+class Sky(C.Structure):
+    _fields_ = [("sun_direction", C.c_float * 3), ("sun_size", C.c_float),
+                ("sun_color", C.c_float * 3), ("sun_intensity", C.c_float),
+                ("zenith", C.c_float * 3), ("sun_enabled", C.c_float),
+                ("horizon", C.c_float * 3), ("_pad0", C.c_float),
+                ("ground", C.c_float * 3), ("_pad1", C.c_float)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("material_ids", C.POINTER(C.c_uint32)), ("n_tris", C.c_uint32),
+                ("n_materials", C.c_uint32), ("materials", C.POINTER(Material))]
+
+
+class Camera(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("fov_deg", C.c_float), ("near_plane", C.c_float),
+                ("far_plane", C.c_float), ("_padding", C.c_uint32)]
+
+
+class Config(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "max_bounces", "sample_count", "accum_frames", "rr_start_bounce", "integrator",
+        "restir_di", "restir_gi", "svgf", "firefly_filter", "taa", "di_candidates", "di_mcap",
+        "di_spatial_neighbours")] + [("di_spatial_radius", C.c_float)] + [
+        (n, C.c_uint32) for n in ("gi_mcap", "svgf_iterations", "svgf_history", "bvh_width",
+                                  "queue_chunk", "lds_stack_depth", "seed", "frame_idx")]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("segments", C.c_uint64),
+                ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
+                ("ms_shadow", C.c_double), ("ms_other", C.c_double)]
+
+
+assert C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
+assert C.sizeof(Material) == 160 and C.sizeof(Camera) == 80
+
+RESULT_CB = C.CFUNCTYPE(None, C.POINTER(C.c_uint8), C.c_size_t, C.c_void_p)
+
+_P = C.c_void_p
+_SIGS = {
+    "apt_last_error": (C.c_char_p, []),
+    "apt_abi_version": (C.c_uint32, []),
+    "apt_device_query": (C.c_int, [C.POINTER(DeviceInfo)]),
+    "apt_bvh_build": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(BvhOpts), C.POINTER(_P)]),
+    "apt_bvh_build_device": (C.c_int, [_P, C.c_uint32, C.POINTER(BvhOpts), C.POINTER(_P)]),
+    "apt_bvh_destroy": (None, [_P]),
+    "apt_bvh_get_stats": (C.c_int, [_P, C.POINTER(BvhStats)]),
+    "apt_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "apt_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "apt_trace_closest_host": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "apt_trace_any_host": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "apt_material_default": (None, [C.POINTER(Material)]),
+    "apt_scene_generate": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
+    "apt_scene_data_view": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(Camera), C.POINTER(Sky)]),
+    "apt_scene_data_destroy": (None, [_P]),
+    "apt_rays_bvh_stress": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "apt_config_default": (None, [C.POINTER(Config)]),
+    "apt_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
+    "apt_destroy": (None, [_P]),
+    "apt_set_config": (C.c_int, [_P, C.POINTER(Config)]),
+    "apt_world_action": (C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    "apt_scene_set": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(Sky)]),
+    "apt_render": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_size_t,
+                             _P, C.c_size_t]),
+    "apt_render_cb": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, RESULT_CB,
+                                _P]),
+    "apt_get_render_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libapt.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libapt.so is not built ({LIB_PATH}); run appearance_amd.build")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, where: str) -> None:
+    if status != APT_OK:
+        msg = lib().apt_last_error()
+        raise AptError(status, where, msg.decode() if msg else "")
+
+
+def ptr(a: np.ndarray, ctype=C.c_float):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def vptr(a) -> C.c_void_p:
+    """Pointer of a numpy array (host) or a torch tensor (device)."""
+    if isinstance(a, np.ndarray):
+        return C.c_void_p(a.ctypes.data)
+    return C.c_void_p(a.data_ptr())

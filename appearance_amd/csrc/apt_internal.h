@@ -1,0 +1,117 @@
+// Internal declarations of libapt shared by the host and kernel translation
+// units: error reporting, the wide-BVH device layout and the BVH object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/apt.h"
+#include "apt_math.h"
+
+namespace apt {
+
+// Thread-local last-error message behind apt_last_error().
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define APT_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ::apt::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                     \
+      return e_ == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;                \
+    }                                                                                 \
+  } while (0)
+
+#define APT_REQUIRE(cond, status, ...) \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::apt::set_error(__VA_ARGS__);   \
+      return status;                   \
+    }                                  \
+  } while (0)
+
+// The library's own stream (default priority) used when callers pass NULL.
+hipStream_t lib_stream();
+
+// ---------------------------------------------------------------------------
+// Wide BVH layout in HBM.
+//
+// A node of width W stores its W child boxes as six float[W] rows (SoA, so a
+// lane reads its W boxes as W/4 float4 per row) followed by W child refs:
+//   ref <  0x80000000          internal node index
+//   ref == REF_EMPTY           unused slot (its box is inverted and never hit)
+//   ref &  0x80000000          leaf: first triangle in bits 0..27, count-1 in
+//                              bits 28..30 (1..8 triangles)
+// Triangles are stored in leaf order as three float4 (v0 | prim bits, v1, v2),
+// 48 B each, so a leaf is one contiguous run.
+// ---------------------------------------------------------------------------
+constexpr uint32_t REF_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t REF_LEAF = 0x80000000u;
+
+APT_HD bool ref_is_leaf(uint32_t r) { return (r & REF_LEAF) != 0u; }
+APT_HD uint32_t ref_leaf_first(uint32_t r) { return r & 0x0FFFFFFFu; }
+APT_HD uint32_t ref_leaf_count(uint32_t r) { return ((r >> 28) & 7u) + 1u; }
+APT_HD uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+  return REF_LEAF | ((count - 1u) << 28) | first;
+}
+
+template <int W>
+struct alignas(16) WNode {
+  float lo_x[W], lo_y[W], lo_z[W];
+  float hi_x[W], hi_y[W], hi_z[W];
+  uint32_t child[W];
+  // pad to a multiple of 64 B: W=2 -> 64, W=4 -> 128, W=8 -> 256
+  uint32_t pad[(W == 2) ? 2 : W];
+};
+static_assert(sizeof(WNode<2>) == 64, "WNode<2> size");
+static_assert(sizeof(WNode<4>) == 128, "WNode<4> size");
+static_assert(sizeof(WNode<8>) == 256, "WNode<8> size");
+
+// Host-side binary BVH produced by the builders before collapse.
+struct Node2 {
+  float lo[3], hi[3];
+  uint32_t left, right;   // children (internal only)
+  uint32_t first, count;  // triangle range in the reordered index list (leaf only)
+};
+
+struct BvhHostResult {
+  std::vector<Node2> nodes;     // node 0 is the root
+  std::vector<uint32_t> order;  // reordered triangle indices
+  double sah_cost = 0.0;
+};
+
+// Binned-SAH build on the host (bvh_host.cpp).
+apt_status build_bvh2_host(const float* tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
+                           BvhHostResult& out);
+// Scene-scale padding used for every node box (see box_pad in bvh_host.cpp).
+float scene_pad(const float* tri_xyz, uint32_t n);
+
+struct apt_bvh_impl;
+
+}  // namespace apt
+
+// The opaque handle of the C ABI.
+struct apt_bvh {
+  uint32_t width = 4;
+  uint32_t n_tris = 0;
+  uint32_t n_nodes = 0;
+  uint32_t root = 0;        // root ref (an internal node or a single leaf)
+  void* d_nodes = nullptr;  // WNode<width>[n_nodes]
+  float4* d_tris = nullptr; // 3 * n_tris float4 in leaf order
+  apt_bvh_stats stats{};
+};
+
+namespace apt {
+// Collapse a binary BVH into a wide one and upload it (bvh_host.cpp).
+apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2,
+                           uint32_t width, float pad, apt_bvh* out);
+// Launch the traversal kernels (trace.hip).
+apt_status launch_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
+                                apt_hit* d_hits, hipStream_t s);
+apt_status launch_trace_any(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
+                            uint32_t* d_occ, hipStream_t s);
+}  // namespace apt

@@ -1,0 +1,343 @@
+// Host binned-SAH BVH build and the collapse of the binary tree into the
+// 2/4/8-wide device layout.
+//
+// This replaces the driver BLAS build of the reference
+// (src/scene_resources/scene_model.rs:53-88, PREFER_FAST_TRACE) with an
+// explicit, deterministic builder: the split of every node depends only on the
+// set of triangles in it, so the tree topology does not depend on thread
+// timing, and the breadth-first collapse gives a canonical node order.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "apt_internal.h"
+
+namespace apt {
+namespace {
+
+struct Prim {
+  float lo[3], hi[3], c[3];
+};
+
+struct Box {
+  float lo[3] = {INFINITY, INFINITY, INFINITY};
+  float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  void grow(const float* l, const float* h) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], l[k]);
+      hi[k] = std::max(hi[k], h[k]);
+    }
+  }
+  void grow(const Box& b) { grow(b.lo, b.hi); }
+  float area() const {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0.0f || dy < 0.0f || dz < 0.0f) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+  }
+};
+
+struct Builder {
+  const Prim* prims;
+  uint32_t* order;
+  Node2* nodes;
+  std::atomic<uint32_t> next_node{1};
+  uint32_t max_leaf;
+  uint32_t bins;
+  static constexpr uint32_t PAR_MIN = 1u << 15;  // spawn a thread above this size
+  static constexpr int PAR_DEPTH = 4;            // ... and this shallow (<= 16 threads)
+  static constexpr int MAX_DEPTH = 56;           // beyond this, median splits only
+
+  void build(uint32_t node, uint32_t begin, uint32_t end, int depth) {
+    Box bb, cb;
+    for (uint32_t i = begin; i < end; ++i) {
+      const Prim& p = prims[order[i]];
+      bb.grow(p.lo, p.hi);
+      cb.grow(p.c, p.c);
+    }
+    Node2& nd = nodes[node];
+    std::memcpy(nd.lo, bb.lo, sizeof(nd.lo));
+    std::memcpy(nd.hi, bb.hi, sizeof(nd.hi));
+    const uint32_t n = end - begin;
+    if (n <= max_leaf) {
+      nd.first = begin;
+      nd.count = n;
+      nd.left = nd.right = 0;
+      return;
+    }
+
+    uint32_t mid = begin + n / 2;
+    int best_axis = -1;
+    uint32_t best_split = 0;
+    if (depth < MAX_DEPTH) {
+      float best_cost = INFINITY;
+      std::vector<Box> bin_box(bins);
+      std::vector<uint32_t> bin_cnt(bins);
+      std::vector<float> right_area(bins);
+      std::vector<uint32_t> right_cnt(bins);
+      for (int axis = 0; axis < 3; ++axis) {
+        float ext = cb.hi[axis] - cb.lo[axis];
+        if (!(ext > 0.0f)) continue;
+        float scale = (float)bins / ext;
+        std::fill(bin_box.begin(), bin_box.end(), Box());
+        std::fill(bin_cnt.begin(), bin_cnt.end(), 0u);
+        for (uint32_t i = begin; i < end; ++i) {
+          const Prim& p = prims[order[i]];
+          int b = (int)((p.c[axis] - cb.lo[axis]) * scale);
+          b = std::min(std::max(b, 0), (int)bins - 1);
+          bin_box[b].grow(p.lo, p.hi);
+          bin_cnt[b]++;
+        }
+        Box acc;
+        uint32_t cnt = 0;
+        for (int b = (int)bins - 1; b > 0; --b) {
+          acc.grow(bin_box[b]);
+          cnt += bin_cnt[b];
+          right_area[b] = acc.area();
+          right_cnt[b] = cnt;
+        }
+        acc = Box();
+        cnt = 0;
+        for (uint32_t b = 1; b < bins; ++b) {
+          acc.grow(bin_box[b - 1]);
+          cnt += bin_cnt[b - 1];
+          if (cnt == 0 || right_cnt[b] == 0) continue;
+          float cost = acc.area() * (float)cnt + right_area[b] * (float)right_cnt[b];
+          if (cost < best_cost) {
+            best_cost = cost;
+            best_axis = axis;
+            best_split = b;
+          }
+        }
+      }
+    }
+    if (best_axis >= 0) {
+      const int axis = best_axis;
+      const float lo = cb.lo[axis];
+      const float scale = (float)bins / (cb.hi[axis] - cb.lo[axis]);
+      const int bins_i = (int)bins;
+      const uint32_t split = best_split;
+      uint32_t* m = std::partition(order + begin, order + end, [&](uint32_t idx) {
+        int b = (int)((prims[idx].c[axis] - lo) * scale);
+        b = std::min(std::max(b, 0), bins_i - 1);
+        return (uint32_t)b < split;
+      });
+      mid = (uint32_t)(m - order);
+    } else {
+      // Coincident centroids (or too deep): object median on the input order,
+      // which is deterministic because the range was partitioned deterministically.
+      std::sort(order + begin, order + end);
+    }
+
+    uint32_t l = next_node.fetch_add(2);
+    nd.left = l;
+    nd.right = l + 1;
+    nd.first = 0;
+    nd.count = 0;
+    if (n >= PAR_MIN && depth < PAR_DEPTH) {
+      std::thread t([this, l, begin, mid, depth] { build(l, begin, mid, depth + 1); });
+      build(l + 1, mid, end, depth + 1);
+      t.join();
+    } else {
+      build(l, begin, mid, depth + 1);
+      build(l + 1, mid, end, depth + 1);
+    }
+  }
+};
+
+float box_area(const float* lo, const float* hi) {
+  float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+}  // namespace
+
+// Padding added to every node box: 2^-14 of the largest vertex coordinate
+// magnitude.  It covers the rounding of the slab test and of the triangle t
+// (tens of ulps for rays starting within ~100x the scene extent), so a
+// triangle whose rounded hit lies on or just outside its exact box is never
+// culled.
+float scene_pad(const float* tri_xyz, uint32_t n) {
+  float m = 0.0f;
+  for (uint64_t i = 0; i < (uint64_t)n * 9; ++i) m = std::max(m, std::fabs(tri_xyz[i]));
+  return std::max(m * (1.0f / 16384.0f), 1e-30f);
+}
+
+apt_status build_bvh2_host(const float* tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
+                           BvhHostResult& out) {
+  std::vector<Prim> prims(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* t = tri_xyz + (uint64_t)i * 9;
+    Prim& p = prims[i];
+    for (int k = 0; k < 3; ++k) {
+      p.lo[k] = std::min(t[k], std::min(t[3 + k], t[6 + k]));
+      p.hi[k] = std::max(t[k], std::max(t[3 + k], t[6 + k]));
+      p.c[k] = (p.lo[k] + p.hi[k]) * 0.5f;
+      if (!std::isfinite(t[k]) || !std::isfinite(t[3 + k]) || !std::isfinite(t[6 + k])) {
+        set_error("triangle %u has a non-finite coordinate", i);
+        return APT_ERR_INVALID_ARG;
+      }
+    }
+  }
+  out.order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) out.order[i] = i;
+  out.nodes.assign(std::max<size_t>(1, 2 * (size_t)n), Node2{});
+  Builder b;
+  b.prims = prims.data();
+  b.order = out.order.data();
+  b.nodes = out.nodes.data();
+  b.max_leaf = max_leaf;
+  b.bins = bins;
+  b.build(0, 0, n, 0);
+  out.nodes.resize(b.next_node.load());
+
+  // SAH cost (C_trav = 1, C_isect = 1) relative to the root area.
+  double root_area = std::max(1e-30f, box_area(out.nodes[0].lo, out.nodes[0].hi));
+  double cost = 0.0;
+  for (const Node2& nd : out.nodes) {
+    double a = box_area(nd.lo, nd.hi) / root_area;
+    cost += nd.count ? a * nd.count : a;
+  }
+  out.sah_cost = cost;
+  return APT_OK;
+}
+
+namespace {
+
+template <int W>
+apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad,
+                           apt_bvh* out) {
+  const std::vector<Node2>& bn = b2.nodes;
+  std::vector<WNode<W>> wide;
+  std::vector<uint32_t> queue;      // BVH2 node of each wide node, breadth first
+  std::vector<uint32_t> wdepth;     // depth of each wide node
+  std::vector<uint32_t> wstack;     // worst-case stack on the way to each wide node
+  uint32_t max_depth = 1, max_stack = 1, n_leaves = 0;
+
+  auto leaf_ref = [&](const Node2& nd) { return make_leaf_ref(nd.first, nd.count); };
+
+  uint32_t root;
+  if (bn[0].count) {
+    root = leaf_ref(bn[0]);
+    n_leaves = 1;
+  } else {
+    root = 0;
+    queue.push_back(0);
+    wdepth.push_back(1);
+    wstack.push_back(0);
+  }
+  for (size_t qi = 0; qi < queue.size(); ++qi) {
+    // Open the binary node into up to W children, greedily expanding the
+    // internal child of largest surface area.
+    uint32_t kids[W];
+    int nk = 0;
+    kids[nk++] = bn[queue[qi]].left;
+    kids[nk++] = bn[queue[qi]].right;
+    while (nk < W) {
+      int best = -1;
+      float best_a = -1.0f;
+      for (int k = 0; k < nk; ++k) {
+        const Node2& c = bn[kids[k]];
+        if (c.count) continue;
+        float a = box_area(c.lo, c.hi);
+        if (a > best_a) {
+          best_a = a;
+          best = k;
+        }
+      }
+      if (best < 0) break;
+      uint32_t x = kids[best];
+      kids[best] = bn[x].left;
+      kids[nk++] = bn[x].right;
+    }
+    WNode<W> w;
+    std::memset(&w, 0, sizeof(w));
+    // Worst-case stack: every sibling but one may be pushed at each level.
+    uint32_t here = wstack[qi] + (uint32_t)(nk - 1);
+    max_stack = std::max(max_stack, here + 1);
+    for (int k = 0; k < W; ++k) {
+      if (k >= nk) {
+        w.lo_x[k] = w.lo_y[k] = w.lo_z[k] = 1e30f;
+        w.hi_x[k] = w.hi_y[k] = w.hi_z[k] = -1e30f;
+        w.child[k] = REF_EMPTY;
+        continue;
+      }
+      const Node2& c = bn[kids[k]];
+      w.lo_x[k] = c.lo[0] - pad;
+      w.lo_y[k] = c.lo[1] - pad;
+      w.lo_z[k] = c.lo[2] - pad;
+      w.hi_x[k] = c.hi[0] + pad;
+      w.hi_y[k] = c.hi[1] + pad;
+      w.hi_z[k] = c.hi[2] + pad;
+      if (c.count) {
+        w.child[k] = leaf_ref(c);
+        n_leaves++;
+      } else {
+        w.child[k] = (uint32_t)queue.size();
+        queue.push_back(kids[k]);
+        wdepth.push_back(wdepth[qi] + 1);
+        wstack.push_back(here);
+        max_depth = std::max(max_depth, wdepth[qi] + 1);
+      }
+    }
+    wide.push_back(w);
+  }
+
+  // Triangles in leaf order: v0 | prim, v1, v2 as float4.
+  std::vector<float4> tris((size_t)n * 3);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t p = b2.order[i];
+    const float* t = tri_xyz + (uint64_t)p * 9;
+    tris[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], u2f(p));
+    tris[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+    tris[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+  }
+
+  size_t node_bytes = std::max<size_t>(1, wide.size()) * sizeof(WNode<W>);
+  size_t tri_bytes = std::max<size_t>(1, tris.size()) * sizeof(float4);
+  void* d_nodes = nullptr;
+  void* d_tris = nullptr;
+  APT_HIP(hipMalloc(&d_nodes, node_bytes));
+  hipError_t e = hipMalloc(&d_tris, tri_bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(d_nodes);
+    set_error("hipMalloc of %zu triangle bytes failed: %s", tri_bytes, hipGetErrorString(e));
+    return APT_ERR_OOM;
+  }
+  if (!wide.empty()) APT_HIP(hipMemcpy(d_nodes, wide.data(), wide.size() * sizeof(WNode<W>), hipMemcpyHostToDevice));
+  if (!tris.empty()) APT_HIP(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+
+  out->width = W;
+  out->n_tris = n;
+  out->n_nodes = (uint32_t)wide.size();
+  out->root = root;
+  out->d_nodes = d_nodes;
+  out->d_tris = (float4*)d_tris;
+  out->stats.n_tris = n;
+  out->stats.n_nodes = (uint32_t)wide.size();
+  out->stats.n_leaves = n_leaves;
+  out->stats.width = W;
+  out->stats.depth = wide.empty() ? 0 : max_depth;
+  out->stats.max_stack = max_stack;
+  out->stats.device_bytes = node_bytes + tri_bytes;
+  out->stats.sah_cost = b2.sah_cost;
+  return APT_OK;
+}
+
+}  // namespace
+
+apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2,
+                           uint32_t width, float pad, apt_bvh* out) {
+  switch (width) {
+    case 2: return collapse_upload<2>(tri_xyz, n, b2, pad, out);
+    case 4: return collapse_upload<4>(tri_xyz, n, b2, pad, out);
+    case 8: return collapse_upload<8>(tri_xyz, n, b2, pad, out);
+    default:
+      set_error("BVH width %u is not one of 2, 4, 8", width);
+      return APT_ERR_INVALID_ARG;
+  }
+}
+
+}  // namespace apt

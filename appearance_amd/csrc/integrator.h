@@ -1,0 +1,69 @@
+// Device views shared by the renderer (host) and the integrator kernels.
+#pragma once
+
+#include "apt_internal.h"
+
+namespace apt {
+
+enum Counter : uint32_t {
+  CNT_QUEUE0 = 0,
+  CNT_QUEUE1 = 1,
+  CNT_SHADOW = 2,
+  CNT_STAT_EXT = 3,     // extension rays traced (running total)
+  CNT_STAT_SHADOW = 4,  // shadow rays traced (running total)
+  CNT_COUNT = 8
+};
+
+// The frame being rendered: a slab of rows [row_start, row_start + rows) of a
+// width x height image.
+struct FrameView {
+  uint32_t width, height, row_start, rows;
+  uint32_t spp;          // samples per pixel of this render call
+  uint32_t frame_idx;
+  uint32_t seed;
+  uint32_t max_bounces;
+  uint32_t rr_start;
+};
+
+// Scene resident in HBM.  Triangle attributes are in input (prim) order; the
+// BVH keeps its own copy of the positions in leaf order.
+struct SceneView {
+  const void* nodes;            // WNode<width>[]
+  const float4* bvh_tris;
+  uint32_t root;
+  uint32_t width;
+  const float4* tri_pos;        // 3 per triangle
+  const float4* tri_nrm;        // 3 per triangle, or null (geometric normals)
+  const uint32_t* tri_mat;
+  const apt_material* mats;
+  const uint32_t* emissive;     // prims with emission
+  uint32_t n_emissive;
+  apt_sky sky;
+  v3 cam_pos, cam_right, cam_up, cam_back;
+  float cam_tan_half, cam_aspect;
+};
+
+// Path state, structure of arrays over path slots.
+struct PathView {
+  float4* o_pdf;    // origin, pdf of the BSDF sample that made the ray (0: camera)
+  float4* d_flags;  // direction, flag bits
+  float4* thr;      // throughput
+  float4* rad;      // radiance gathered so far
+  float4* hit;      // t, u, v, prim
+  float4* sh_o;     // shadow ray origin, tmax
+  float4* sh_d;     // shadow ray direction
+  float4* sh_c;     // unoccluded contribution
+  uint32_t* rng;
+  uint32_t* queue[2];
+  uint32_t* shadow_queue;
+  uint32_t* counters;  // Counter
+};
+
+apt_status integrate_pass(const FrameView& fv, const SceneView& sc, const PathView& ps,
+                          uint32_t n_paths, uint32_t sample_base, uint32_t s_count,
+                          float4* frame_sum, hipStream_t s);
+apt_status resolve_frame(const FrameView& fv, const float4* frame_sum, float4* accum_tm,
+                         float4* accum_lin, uint32_t accum_frames, uint32_t* out_rgba,
+                         hipStream_t s);
+
+}  // namespace apt

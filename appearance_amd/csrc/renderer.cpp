@@ -1,0 +1,587 @@
+// Renderer object of the C ABI: the host side of PathTracerGpu
+// (crates/appearance-path-tracer-gpu/src/lib.rs:250-615) and of the
+// NodeRenderer job interface (crates/appearance-render-loop/src/node.rs:14-25).
+//
+// Host responsibilities: world actions (camera, procedural models), scene
+// flattening and upload (BVH build, triangle attributes, emitter list), frame
+// buffers for the requested slab, the sample/bounce schedule of the wavefront
+// integrator, progressive accumulation and the blocking readback.
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "apt_internal.h"
+#include "integrator.h"
+
+namespace {
+
+using namespace apt;
+
+struct Model {
+  uint8_t uuid[16];
+  float xform[16];  // column-major
+  apt_scene_data* data;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  apt_status ensure(size_t count) {
+    if (count <= n && p) return APT_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc(%zu B) failed: %s", count * sizeof(T), hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
+    }
+    n = count;
+    return APT_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+void mat_point(const float* m, const float* p, float* out) {
+  for (int r = 0; r < 3; ++r) out[r] = m[r] * p[0] + m[4 + r] * p[1] + m[8 + r] * p[2] + m[12 + r];
+}
+
+// Normal transform: inverse transpose of the upper 3x3.
+bool normal_matrix(const float* m, float* n) {
+  const double a = m[0], b = m[4], c = m[8], d = m[1], e = m[5], f = m[9], g = m[2], h = m[6], i = m[10];
+  const double A = e * i - f * h, B = -(d * i - f * g), Cc = d * h - e * g;
+  const double D = -(b * i - c * h), E = a * i - c * g, F = -(a * h - b * g);
+  const double G = b * f - c * e, H = -(a * f - c * d), I = a * e - b * d;
+  const double det = a * A + b * B + c * Cc;
+  if (std::fabs(det) < 1e-30) return false;
+  // rows of the inverse transpose = cofactor rows / det
+  n[0] = (float)(A / det); n[1] = (float)(B / det); n[2] = (float)(Cc / det);
+  n[3] = (float)(D / det); n[4] = (float)(E / det); n[5] = (float)(F / det);
+  n[6] = (float)(G / det); n[7] = (float)(H / det); n[8] = (float)(I / det);
+  return true;
+}
+
+}  // namespace
+
+struct apt_renderer {
+  apt_config cfg{};
+  // camera (CameraUpdate); the reference's default camera until one arrives
+  apt_camera cam{};
+  uint64_t cam_version = 1, accum_cam_version = 0;
+  // scene
+  std::vector<Model> models;
+  std::vector<float> pos, nrm;
+  std::vector<uint32_t> mat_ids;
+  std::vector<apt_material> mats;
+  apt_sky sky{};
+  bool scene_dirty = true, explicit_scene = false, has_normals = false;
+  uint64_t scene_version = 1, accum_scene_version = 0;
+  // device scene
+  apt_bvh* bvh = nullptr;
+  DevBuf<float4> d_pos, d_nrm;
+  DevBuf<uint32_t> d_mat, d_emissive;
+  DevBuf<apt_material> d_mats;
+  uint32_t n_emissive = 0;
+  // frame
+  uint32_t fw = 0, fh = 0, frow0 = 0, frows = 0;
+  DevBuf<float4> d_frame_sum, d_accum_tm, d_accum_lin;
+  DevBuf<uint32_t> d_rgba;
+  uint32_t accum_frames = 0;
+  std::vector<float4> h_lin;
+  // paths
+  uint32_t path_cap = 0;
+  DevBuf<float4> p_o, p_d, p_t, p_l, p_h, p_so, p_sd, p_sc;
+  DevBuf<uint32_t> p_rng, p_q0, p_q1, p_qs, p_cnt;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  apt_render_stats stats{};
+  uint32_t frame_idx = 0;
+  std::vector<uint8_t> cb_pixels;
+
+  ~apt_renderer() {
+    for (Model& m : models) apt_scene_data_destroy(m.data);
+    if (bvh) apt_bvh_destroy(bvh);
+    d_pos.release(); d_nrm.release(); d_mat.release(); d_emissive.release(); d_mats.release();
+    d_frame_sum.release(); d_accum_tm.release(); d_accum_lin.release(); d_rgba.release();
+    p_o.release(); p_d.release(); p_t.release(); p_l.release(); p_h.release(); p_so.release();
+    p_sd.release(); p_sc.release(); p_rng.release(); p_q0.release(); p_q1.release();
+    p_qs.release(); p_cnt.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+apt_status validate_config(const apt_config& c) {
+  APT_REQUIRE(c.max_bounces >= 1 && c.max_bounces <= 64, APT_ERR_INVALID_ARG,
+              "config: max_bounces %u outside 1..64", c.max_bounces);
+  APT_REQUIRE(c.sample_count >= 1 && c.sample_count <= 65536, APT_ERR_INVALID_ARG,
+              "config: sample_count %u outside 1..65536", c.sample_count);
+  APT_REQUIRE(c.bvh_width == 2 || c.bvh_width == 4 || c.bvh_width == 8, APT_ERR_INVALID_ARG,
+              "config: bvh_width %u is not 2, 4 or 8", c.bvh_width);
+  APT_REQUIRE(c.integrator == APT_INTEGRATOR_PLAIN, APT_ERR_UNSUPPORTED,
+              "config: integrator %u is not available (only APT_INTEGRATOR_PLAIN)", c.integrator);
+  APT_REQUIRE(!c.restir_di && !c.restir_gi, APT_ERR_UNSUPPORTED,
+              "config: ReSTIR DI/GI is not available in this build");
+  APT_REQUIRE(!c.svgf && !c.taa && !c.firefly_filter, APT_ERR_UNSUPPORTED,
+              "config: the denoise/post chain (svgf, taa, firefly) is not available in this build");
+  return APT_OK;
+}
+
+apt_status upload_scene(apt_renderer* r) {
+  if (!r->scene_dirty) return APT_OK;
+  if (!r->explicit_scene) {
+    // Flatten the spawned models into one triangle soup (world space).
+    r->pos.clear();
+    r->nrm.clear();
+    r->mat_ids.clear();
+    r->mats.clear();
+    bool all_normals = !r->models.empty();
+    for (const Model& m : r->models) {
+      apt_mesh_desc md;
+      apt_camera cam;
+      apt_sky sky;
+      apt_scene_data_view(m.data, &md, &cam, &sky);
+      all_normals = all_normals && md.normals != nullptr;
+    }
+    for (const Model& m : r->models) {
+      apt_mesh_desc md;
+      apt_scene_data_view(m.data, &md, nullptr, nullptr);
+      const uint32_t mat_base = (uint32_t)r->mats.size();
+      r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
+      float nm[9];
+      const bool nm_ok = normal_matrix(m.xform, nm);
+      for (uint32_t t = 0; t < md.n_tris; ++t) {
+        float v[9];
+        for (int k = 0; k < 3; ++k) mat_point(m.xform, md.positions + 9 * (size_t)t + 3 * k, v + 3 * k);
+        r->pos.insert(r->pos.end(), v, v + 9);
+        r->mat_ids.push_back(mat_base + (md.material_ids ? md.material_ids[t] : 0u));
+        if (all_normals) {
+          for (int k = 0; k < 3; ++k) {
+            const float* n = md.normals + 9 * (size_t)t + 3 * k;
+            float o[3];
+            for (int q = 0; q < 3; ++q)
+              o[q] = nm_ok ? nm[3 * q] * n[0] + nm[3 * q + 1] * n[1] + nm[3 * q + 2] * n[2] : n[q];
+            float l = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+            if (l > 0.0f) for (int q = 0; q < 3; ++q) o[q] /= l;
+            r->nrm.insert(r->nrm.end(), o, o + 3);
+          }
+        }
+      }
+    }
+    r->has_normals = all_normals;
+  }
+  const uint32_t n = (uint32_t)r->mat_ids.size();
+  if (r->bvh) {
+    apt_bvh_destroy(r->bvh);
+    r->bvh = nullptr;
+  }
+  r->n_emissive = 0;
+  if (n == 0) {
+    r->scene_dirty = false;
+    r->scene_version++;
+    return APT_OK;
+  }
+  apt_bvh_opts o{r->cfg.bvh_width, 4, 32, APT_BUILDER_HOST};
+  apt_status st = apt_bvh_build(r->pos.data(), n, &o, &r->bvh);
+  if (st != APT_OK) return st;
+  std::vector<float4> p4((size_t)n * 3), n4;
+  for (size_t i = 0; i < (size_t)n * 3; ++i)
+    p4[i] = make_float4(r->pos[3 * i], r->pos[3 * i + 1], r->pos[3 * i + 2], 0.0f);
+  if ((st = r->d_pos.ensure(p4.size())) != APT_OK) return st;
+  APT_HIP(hipMemcpy(r->d_pos.p, p4.data(), p4.size() * sizeof(float4), hipMemcpyHostToDevice));
+  if (r->has_normals) {
+    n4.resize((size_t)n * 3);
+    for (size_t i = 0; i < (size_t)n * 3; ++i)
+      n4[i] = make_float4(r->nrm[3 * i], r->nrm[3 * i + 1], r->nrm[3 * i + 2], 0.0f);
+    if ((st = r->d_nrm.ensure(n4.size())) != APT_OK) return st;
+    APT_HIP(hipMemcpy(r->d_nrm.p, n4.data(), n4.size() * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  for (uint32_t m : r->mat_ids)
+    APT_REQUIRE(m < r->mats.size(), APT_ERR_INVALID_ARG, "scene: material id %u >= %zu", m, r->mats.size());
+  if ((st = r->d_mat.ensure(n)) != APT_OK) return st;
+  APT_HIP(hipMemcpy(r->d_mat.p, r->mat_ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if ((st = r->d_mats.ensure(r->mats.size())) != APT_OK) return st;
+  APT_HIP(hipMemcpy(r->d_mats.p, r->mats.data(), r->mats.size() * sizeof(apt_material), hipMemcpyHostToDevice));
+  std::vector<uint32_t> em;
+  for (uint32_t t = 0; t < n; ++t) {
+    const apt_material& m = r->mats[r->mat_ids[t]];
+    if (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f) {
+      const float* p = &r->pos[9 * (size_t)t];
+      float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+      float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 0.0f) em.push_back(t);  // degenerate emitters emit nothing
+    }
+  }
+  r->n_emissive = (uint32_t)em.size();
+  if ((st = r->d_emissive.ensure(em.size())) != APT_OK) return st;
+  if (!em.empty())
+    APT_HIP(hipMemcpy(r->d_emissive.p, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  r->scene_dirty = false;
+  r->scene_version++;
+  return APT_OK;
+}
+
+apt_status ensure_paths(apt_renderer* r, uint32_t n) {
+  if (n <= r->path_cap) return APT_OK;
+  apt_status st;
+  for (DevBuf<float4>* b : {&r->p_o, &r->p_d, &r->p_t, &r->p_l, &r->p_h, &r->p_so, &r->p_sd, &r->p_sc})
+    if ((st = b->ensure(n)) != APT_OK) return st;
+  for (DevBuf<uint32_t>* b : {&r->p_rng, &r->p_q0, &r->p_q1, &r->p_qs})
+    if ((st = b->ensure(n)) != APT_OK) return st;
+  if ((st = r->p_cnt.ensure(CNT_COUNT)) != APT_OK) return st;
+  r->path_cap = n;
+  return APT_OK;
+}
+
+SceneView scene_view(const apt_renderer* r, uint32_t width, uint32_t height) {
+  SceneView sc{};
+  sc.nodes = r->bvh ? r->bvh->d_nodes : nullptr;
+  sc.bvh_tris = r->bvh ? r->bvh->d_tris : nullptr;
+  sc.root = r->bvh ? r->bvh->root : 0;
+  sc.width = r->bvh ? r->bvh->width : r->cfg.bvh_width;
+  sc.tri_pos = r->d_pos.p;
+  sc.tri_nrm = r->has_normals ? r->d_nrm.p : nullptr;
+  sc.tri_mat = r->d_mat.p;
+  sc.mats = r->d_mats.p;
+  sc.emissive = r->d_emissive.p;
+  sc.n_emissive = r->n_emissive;
+  sc.sky = r->sky;
+  // CameraUpdate: rotation and translation of the world transform (scale
+  // dropped, as to_scale_rotation_translation does in lib.rs:289-296), a
+  // vertical-fov pinhole with aspect = full width / height (lib.rs:333-334).
+  const float* m = r->cam.transform;
+  v3 c0 = normalize(mk(m[0], m[1], m[2])), c1 = normalize(mk(m[4], m[5], m[6]));
+  v3 c2 = normalize(mk(m[8], m[9], m[10]));
+  sc.cam_right = c0;
+  sc.cam_up = c1;
+  sc.cam_back = c2;
+  sc.cam_pos = mk(m[12], m[13], m[14]);
+  sc.cam_tan_half = std::tan(0.5f * r->cam.fov_deg * PI / 180.0f);
+  sc.cam_aspect = (float)width / (float)height;
+  return sc;
+}
+
+apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
+                       uint32_t row_end, uint8_t* out_rgba, size_t out_len, float* out_accum,
+                       size_t accum_len) {
+  APT_REQUIRE(width > 0 && height > 0 && row_start < row_end && row_end <= height,
+              APT_ERR_INVALID_ARG, "render: bad frame %ux%u rows [%u,%u)", width, height, row_start, row_end);
+  const uint32_t rows = row_end - row_start;
+  APT_REQUIRE(width % 64 == 0 && rows % 64 == 0, APT_ERR_INVALID_ARG,
+              "render: width (%u) and slab height (%u) must be multiples of 64 (host.rs:476)", width, rows);
+  const size_t npix = (size_t)width * rows;
+  APT_REQUIRE(npix <= (1u << 26), APT_ERR_INVALID_ARG, "render: %zu pixels exceed 2^26", npix);
+  APT_REQUIRE(!out_rgba || out_len >= npix * 4, APT_ERR_INVALID_ARG,
+              "render: out_rgba holds %zu bytes, needs %zu", out_len, npix * 4);
+  APT_REQUIRE(!out_accum || accum_len >= npix * 3, APT_ERR_INVALID_ARG,
+              "render: out_accum_rgb holds %zu floats, needs %zu", accum_len, npix * 3);
+  apt_status st = upload_scene(r);
+  if (st != APT_OK) return st;
+  APT_REQUIRE(r->bvh != nullptr, APT_ERR_INVALID_ARG, "render: the scene is empty");
+
+  hipStream_t s = r->stream;
+  const bool resized = r->fw != width || r->fh != height || r->frow0 != row_start || r->frows != rows;
+  if (resized) {
+    if ((st = r->d_frame_sum.ensure(npix)) != APT_OK) return st;
+    if ((st = r->d_accum_tm.ensure(npix)) != APT_OK) return st;
+    if ((st = r->d_accum_lin.ensure(npix)) != APT_OK) return st;
+    if ((st = r->d_rgba.ensure(npix)) != APT_OK) return st;
+    r->fw = width;
+    r->fh = height;
+    r->frow0 = row_start;
+    r->frows = rows;
+  }
+  // Accumulation restarts on any change of view, scene or slab, or when
+  // accumulation is off (lib.rs:349-352).
+  if (resized || !r->cfg.accum_frames || r->accum_cam_version != r->cam_version ||
+      r->accum_scene_version != r->scene_version) {
+    APT_HIP(hipMemsetAsync(r->d_accum_tm.p, 0, npix * sizeof(float4), s));
+    APT_HIP(hipMemsetAsync(r->d_accum_lin.p, 0, npix * sizeof(float4), s));
+    r->accum_frames = 0;
+    r->accum_cam_version = r->cam_version;
+    r->accum_scene_version = r->scene_version;
+  }
+  APT_HIP(hipMemsetAsync(r->d_frame_sum.p, 0, npix * sizeof(float4), s));
+
+  const uint32_t spp = r->cfg.sample_count;
+  const uint64_t budget = r->cfg.queue_chunk ? r->cfg.queue_chunk : (1u << 21);
+  const uint32_t per_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(spp, budget / npix));
+  if ((st = ensure_paths(r, (uint32_t)(per_pass * npix))) != APT_OK) return st;
+  APT_HIP(hipMemsetAsync(r->p_cnt.p, 0, CNT_COUNT * sizeof(uint32_t), s));
+
+  FrameView fv{width, height, row_start, rows, spp, r->frame_idx, r->cfg.seed, r->cfg.max_bounces,
+               r->cfg.rr_start_bounce};
+  SceneView sc = scene_view(r, width, height);
+  PathView ps{};
+  ps.o_pdf = r->p_o.p; ps.d_flags = r->p_d.p; ps.thr = r->p_t.p; ps.rad = r->p_l.p;
+  ps.hit = r->p_h.p; ps.sh_o = r->p_so.p; ps.sh_d = r->p_sd.p; ps.sh_c = r->p_sc.p;
+  ps.rng = r->p_rng.p; ps.queue[0] = r->p_q0.p; ps.queue[1] = r->p_q1.p;
+  ps.shadow_queue = r->p_qs.p; ps.counters = r->p_cnt.p;
+
+  APT_HIP(hipEventRecord(r->ev0, s));
+  for (uint32_t base = 0; base < spp; base += per_pass) {
+    const uint32_t cnt = std::min(per_pass, spp - base);
+    st = integrate_pass(fv, sc, ps, (uint32_t)(cnt * npix), base, cnt, r->d_frame_sum.p, s);
+    if (st != APT_OK) return st;
+  }
+  st = resolve_frame(fv, r->d_frame_sum.p, r->d_accum_tm.p, r->d_accum_lin.p, r->accum_frames,
+                     r->d_rgba.p, s);
+  if (st != APT_OK) return st;
+  APT_HIP(hipEventRecord(r->ev1, s));
+  uint32_t cnt[CNT_COUNT];
+  APT_HIP(hipMemcpyAsync(cnt, r->p_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  if (out_rgba) APT_HIP(hipMemcpyAsync(out_rgba, r->d_rgba.p, npix * 4, hipMemcpyDeviceToHost, s));
+  if (out_accum) {
+    r->h_lin.resize(npix);
+    APT_HIP(hipMemcpyAsync(r->h_lin.data(), r->d_accum_lin.p, npix * sizeof(float4), hipMemcpyDeviceToHost, s));
+  }
+  APT_HIP(hipStreamSynchronize(s));
+  if (out_accum) {
+    for (size_t i = 0; i < npix; ++i) {
+      const float4 v = r->h_lin[i];
+      const float inv = v.w > 0.0f ? 1.0f / v.w : 0.0f;
+      out_accum[3 * i] = v.x * inv;
+      out_accum[3 * i + 1] = v.y * inv;
+      out_accum[3 * i + 2] = v.z * inv;
+    }
+  }
+  float ms = 0.0f;
+  APT_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  r->stats = apt_render_stats{};
+  r->stats.paths = (uint64_t)npix * spp;
+  r->stats.extension_rays = cnt[CNT_STAT_EXT];
+  r->stats.shadow_rays = cnt[CNT_STAT_SHADOW];
+  r->stats.segments = cnt[CNT_STAT_EXT];
+  r->stats.ms_total = ms;
+  r->accum_frames++;
+  r->frame_idx++;
+  return APT_OK;
+}
+
+bool parse_procedural(const char* path, std::string& name, uint32_t& seed) {
+  const std::string p(path);
+  const std::string pre = "procedural://";
+  if (p.rfind(pre, 0) != 0) return false;
+  std::string rest = p.substr(pre.size());
+  seed = 0;
+  const size_t q = rest.find('?');
+  if (q != std::string::npos) {
+    const std::string args = rest.substr(q + 1);
+    rest = rest.substr(0, q);
+    const size_t k = args.find("seed=");
+    if (k != std::string::npos) seed = (uint32_t)std::strtoul(args.c_str() + k + 5, nullptr, 10);
+  }
+  name = rest;
+  return !name.empty();
+}
+
+}  // namespace
+
+extern "C" {
+
+void apt_config_default(apt_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->max_bounces = 2;        // PathTracerGpuConfig::default (lib.rs:230-245)
+  c->sample_count = 1;
+  c->accum_frames = 0;
+  c->rr_start_bounce = 3;    // trace.wgsl:281: roulette once bounce > 1
+  c->integrator = APT_INTEGRATOR_PLAIN;
+  c->restir_di = 0;
+  c->restir_gi = 0;
+  c->svgf = 0;
+  c->firefly_filter = 0;
+  c->taa = 0;
+  c->di_candidates = 32;
+  c->di_mcap = 20;
+  c->di_spatial_neighbours = 5;
+  c->di_spatial_radius = 30.0f;
+  c->gi_mcap = 30;
+  c->svgf_iterations = 5;
+  c->svgf_history = 4;
+  c->bvh_width = 4;
+  c->queue_chunk = 0;
+  c->lds_stack_depth = 24;
+  c->seed = 0;
+  c->frame_idx = 0;
+}
+
+apt_status apt_create(const apt_config* cfg, apt_renderer** out) {
+  clear_error();
+  APT_REQUIRE(out, APT_ERR_INVALID_ARG, "apt_create: out is NULL");
+  apt_config c;
+  if (cfg) c = *cfg;
+  else apt_config_default(&c);
+  apt_status st = validate_config(c);
+  if (st != APT_OK) return st;
+  apt_renderer* r = new apt_renderer();
+  r->cfg = c;
+  r->frame_idx = c.frame_idx;
+  // Default camera: at the origin looking down -Z, 60 degree fov.
+  const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  std::memcpy(r->cam.transform, id, sizeof(id));
+  r->cam.fov_deg = 60.0f;
+  r->cam.near_plane = 0.01f;
+  r->cam.far_plane = 1000.0f;
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&r->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&r->ev1);
+  if (e != hipSuccess) {
+    set_error("apt_create: %s", hipGetErrorString(e));
+    delete r;
+    return APT_ERR_DEVICE;
+  }
+  *out = r;
+  return APT_OK;
+}
+
+void apt_destroy(apt_renderer* r) { delete r; }
+
+apt_status apt_set_config(apt_renderer* r, const apt_config* cfg) {
+  clear_error();
+  APT_REQUIRE(r && cfg, APT_ERR_INVALID_ARG, "apt_set_config: null argument");
+  apt_status st = validate_config(*cfg);
+  if (st != APT_OK) return st;
+  if (cfg->bvh_width != r->cfg.bvh_width) r->scene_dirty = true;
+  if (cfg->sample_count != r->cfg.sample_count || cfg->max_bounces != r->cfg.max_bounces ||
+      cfg->rr_start_bounce != r->cfg.rr_start_bounce || cfg->seed != r->cfg.seed)
+    r->cam_version++;  // restart accumulation
+  r->cfg = *cfg;
+  return APT_OK;
+}
+
+apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, size_t len) {
+  clear_error();
+  APT_REQUIRE(r && (bytes || len == 0), APT_ERR_INVALID_ARG, "apt_world_action: null argument");
+  const uint8_t* b = static_cast<const uint8_t*>(bytes);
+  switch (ty) {
+    case 0: {  // CameraUpdate
+      APT_REQUIRE(len == 80, APT_ERR_INVALID_ARG, "CameraUpdate carries 80 bytes, got %zu", len);
+      std::memcpy(&r->cam, b, 80);
+      r->cam_version++;
+      return APT_OK;
+    }
+    case 1: {  // SpawnModel: Mat4, Uuid, 256-byte NUL-terminated asset path
+      APT_REQUIRE(len == 336, APT_ERR_INVALID_ARG, "SpawnModel carries 336 bytes, got %zu", len);
+      char path[257];
+      std::memcpy(path, b + 80, 256);
+      path[256] = 0;
+      std::string name;
+      uint32_t seed;
+      APT_REQUIRE(parse_procedural(path, name, seed), APT_ERR_UNSUPPORTED,
+                  "SpawnModel: '%s' is not a procedural:// asset (glTF ingestion is not available)", path);
+      Model m;
+      std::memcpy(m.xform, b, 64);
+      std::memcpy(m.uuid, b + 64, 16);
+      apt_status st = apt_scene_generate(name.c_str(), seed, &m.data);
+      if (st != APT_OK) return st;
+      apt_sky sky;
+      apt_scene_data_view(m.data, nullptr, nullptr, &sky);
+      if (sky.sun_enabled > 0.0f || sky.zenith[0] > 0.0f || sky.zenith[1] > 0.0f || sky.zenith[2] > 0.0f)
+        r->sky = sky;
+      r->models.push_back(m);
+      r->explicit_scene = false;
+      r->scene_dirty = true;
+      return APT_OK;
+    }
+    case 2: {  // TransformModel
+      APT_REQUIRE(len == 80, APT_ERR_INVALID_ARG, "TransformModel carries 80 bytes, got %zu", len);
+      for (Model& m : r->models) {
+        if (std::memcmp(m.uuid, b + 64, 16) == 0) {
+          std::memcpy(m.xform, b, 64);
+          r->scene_dirty = true;
+          return APT_OK;
+        }
+      }
+      set_error("TransformModel: unknown entity");
+      return APT_ERR_NOT_FOUND;
+    }
+    case 3: {  // DestroyModel
+      APT_REQUIRE(len == 16, APT_ERR_INVALID_ARG, "DestroyModel carries 16 bytes, got %zu", len);
+      for (size_t i = 0; i < r->models.size(); ++i) {
+        if (std::memcmp(r->models[i].uuid, b, 16) == 0) {
+          apt_scene_data_destroy(r->models[i].data);
+          r->models.erase(r->models.begin() + (long)i);
+          r->scene_dirty = true;
+          return APT_OK;
+        }
+      }
+      set_error("DestroyModel: unknown entity");
+      return APT_ERR_NOT_FOUND;
+    }
+    case 4: {  // Clear
+      APT_REQUIRE(len == 4, APT_ERR_INVALID_ARG, "Clear carries 4 bytes, got %zu", len);
+      for (Model& m : r->models) apt_scene_data_destroy(m.data);
+      r->models.clear();
+      r->explicit_scene = false;
+      r->scene_dirty = true;
+      return APT_OK;
+    }
+    default:
+      set_error("apt_world_action: unknown action type %u", ty);
+      return APT_ERR_INVALID_ARG;
+  }
+}
+
+apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_sky* sky) {
+  clear_error();
+  APT_REQUIRE(r && mesh, APT_ERR_INVALID_ARG, "apt_scene_set: null argument");
+  APT_REQUIRE(mesh->n_tris == 0 || mesh->positions, APT_ERR_INVALID_ARG, "apt_scene_set: no positions");
+  APT_REQUIRE(mesh->n_materials > 0 && mesh->materials, APT_ERR_INVALID_ARG,
+              "apt_scene_set: at least one material is required");
+  const size_t n = mesh->n_tris;
+  r->pos.assign(mesh->positions, mesh->positions + 9 * n);
+  r->has_normals = mesh->normals != nullptr;
+  if (r->has_normals) r->nrm.assign(mesh->normals, mesh->normals + 9 * n);
+  else r->nrm.clear();
+  if (mesh->material_ids) r->mat_ids.assign(mesh->material_ids, mesh->material_ids + n);
+  else r->mat_ids.assign(n, 0u);
+  r->mats.assign(mesh->materials, mesh->materials + mesh->n_materials);
+  if (sky) r->sky = *sky;
+  else std::memset(&r->sky, 0, sizeof(r->sky));
+  r->explicit_scene = true;
+  r->scene_dirty = true;
+  return APT_OK;
+}
+
+apt_status apt_render(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
+                      uint32_t row_end, uint8_t* out_rgba, size_t out_len, float* out_accum_rgb,
+                      size_t accum_len) {
+  clear_error();
+  APT_REQUIRE(r, APT_ERR_INVALID_ARG, "apt_render: renderer is NULL");
+  return render_impl(r, width, height, row_start, row_end, out_rgba, out_len, out_accum_rgb, accum_len);
+}
+
+apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
+                         uint32_t row_end, apt_result_cb cb, void* user) {
+  clear_error();
+  APT_REQUIRE(r && cb, APT_ERR_INVALID_ARG, "apt_render_cb: null argument");
+  APT_REQUIRE(row_end > row_start, APT_ERR_INVALID_ARG, "apt_render_cb: empty slab");
+  r->cb_pixels.resize((size_t)width * (row_end - row_start) * 4);
+  apt_status st = render_impl(r, width, height, row_start, row_end, r->cb_pixels.data(),
+                              r->cb_pixels.size(), nullptr, 0);
+  if (st != APT_OK) return st;
+  cb(r->cb_pixels.data(), r->cb_pixels.size(), user);
+  return APT_OK;
+}
+
+apt_status apt_get_render_stats(const apt_renderer* r, apt_render_stats* out) {
+  APT_REQUIRE(r && out, APT_ERR_INVALID_ARG, "apt_get_render_stats: null argument");
+  *out = r->stats;
+  return APT_OK;
+}
+
+}  // extern "C"

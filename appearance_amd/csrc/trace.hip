@@ -1,0 +1,85 @@
+// Wide-BVH traversal kernels for gfx950: closest hit and any hit.
+//
+// These replace the reference's hardware ray query (rayQueryInitialize /
+// rayQueryProceed / rayQueryGetCommittedIntersection, trace.wgsl:159-162 and
+// helpers/trace.wgsl:20-26).  One lane traces one ray.  The traversal stack
+// keeps its bottom LDS_STACK entries in LDS (laid out [entry][lane] so lanes at
+// the same depth hit distinct banks) and spills deeper entries to a small
+// private array; the host checks that the BVH's worst-case stack fits.
+#include "apt_internal.h"
+#include "trace_device.h"
+
+namespace apt {
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+template <int W, bool ANY>
+__global__ __launch_bounds__(BLOCK) void k_trace(const WNode<W>* __restrict__ nodes,
+                                                 const float4* __restrict__ tris, uint32_t root,
+                                                 const apt_ray* __restrict__ rays, uint32_t n,
+                                                 apt_hit* __restrict__ hits,
+                                                 uint32_t* __restrict__ occluded) {
+  __shared__ uint32_t s_stack[TRACE_LDS_STACK * BLOCK];
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t i = blk * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float4* rp = reinterpret_cast<const float4*>(rays) + 2 * (size_t)i;
+  const float4 r0 = rp[0];
+  const float4 r1 = rp[1];
+  TraceRay ray = make_trace_ray(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w);
+  LdsStack st{s_stack + threadIdx.x, BLOCK};
+  if (ANY) {
+    occluded[i] = trace_any<W>(nodes, tris, root, ray, st) ? 1u : 0u;
+  } else {
+    HitRec h = trace_closest<W>(nodes, tris, root, ray, st);
+    float4 out = make_float4(h.prim == INVALID ? INFINITY : h.t, h.u, h.v, u2f(h.prim));
+    reinterpret_cast<float4*>(hits)[i] = out;
+  }
+}
+
+template <int W>
+apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
+                  uint32_t* d_occ, bool any, hipStream_t s) {
+  if (n == 0) return APT_OK;
+  const WNode<W>* nodes = static_cast<const WNode<W>*>(bvh->d_nodes);
+  dim3 grid((n + BLOCK - 1) / BLOCK);
+  if (any)
+    hipLaunchKernelGGL((k_trace<W, true>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris, bvh->root,
+                       d_rays, n, d_hits, d_occ);
+  else
+    hipLaunchKernelGGL((k_trace<W, false>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
+                       bvh->root, d_rays, n, d_hits, d_occ);
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+apt_status dispatch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
+                    uint32_t* d_occ, bool any, hipStream_t s) {
+  APT_REQUIRE(bvh && d_rays && (any ? d_occ != nullptr : d_hits != nullptr),
+              APT_ERR_INVALID_ARG, "trace: null argument");
+  APT_REQUIRE(bvh->stats.max_stack <= TRACE_LDS_STACK + TRACE_OVF_STACK, APT_ERR_UNSUPPORTED,
+              "trace: BVH needs a %u-entry stack, kernel holds %d", bvh->stats.max_stack,
+              TRACE_LDS_STACK + TRACE_OVF_STACK);
+  switch (bvh->width) {
+    case 2: return launch<2>(bvh, d_rays, n, d_hits, d_occ, any, s);
+    case 4: return launch<4>(bvh, d_rays, n, d_hits, d_occ, any, s);
+    case 8: return launch<8>(bvh, d_rays, n, d_hits, d_occ, any, s);
+    default: set_error("trace: unsupported width %u", bvh->width); return APT_ERR_INVALID_ARG;
+  }
+}
+
+}  // namespace
+
+apt_status launch_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
+                                apt_hit* d_hits, hipStream_t s) {
+  return dispatch(bvh, d_rays, n, d_hits, nullptr, false, s);
+}
+
+apt_status launch_trace_any(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
+                            uint32_t* d_occ, hipStream_t s) {
+  return dispatch(bvh, d_rays, n, nullptr, d_occ, true, s);
+}
+
+}  // namespace apt

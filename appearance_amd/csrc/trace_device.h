@@ -1,0 +1,212 @@
+// Device-side wide-BVH traversal shared by the ray-query kernels (trace.hip)
+// and the wavefront integrator's extend and shadow stages.
+#pragma once
+
+#include "apt_internal.h"
+
+namespace apt {
+
+// Stack entries kept in LDS per lane, and in the private overflow array.
+constexpr int TRACE_LDS_STACK = 24;
+constexpr int TRACE_OVF_STACK = 64;
+
+// Work-group -> tile mapping that gives each of the 8 XCDs of an MI355X one
+// contiguous range of work-groups.  The dispatcher deals work-groups to XCDs
+// round robin (group i runs on XCD i % 8), so consecutive logical groups
+// (neighbouring rays or pixels) share an XCD's L2.  The mapping is a bijection
+// on [0, G) for any G.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t G) {
+  const uint32_t q = G >> 3, r = G & 7u;
+  const uint32_t xcd = i & 7u, k = i >> 3;
+  return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+struct TraceRay {
+  RayPre pre;
+  v3 inv;
+};
+
+__device__ __forceinline__ TraceRay make_trace_ray(v3 o, v3 d, float tmin, float tmax) {
+  TraceRay r;
+  r.pre = ray_pre(o, d, tmin, tmax);
+  r.inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+  return r;
+}
+
+struct HitRec {
+  float t, u, v;
+  uint32_t prim;
+};
+
+// Per-lane view of the LDS stack: entry e of this lane lives at base[e*stride].
+struct LdsStack {
+  uint32_t* base;
+  int stride;
+};
+
+template <int W>
+__device__ __forceinline__ void load_node(const WNode<W>* __restrict__ nodes, uint32_t idx,
+                                          WNode<W>& nd) {
+  const float4* src = reinterpret_cast<const float4*>(nodes + idx);
+  float4* dst = reinterpret_cast<float4*>(&nd);
+  constexpr int N4 = (int)((6 * W + W) * 4 + 15) / 16;  // boxes + child refs, no padding
+#pragma unroll
+  for (int k = 0; k < N4; ++k) dst[k] = src[k];
+}
+
+// Entry distance of each child box, +inf for a missed or empty child.
+template <int W>
+__device__ __forceinline__ void child_dists(const WNode<W>& nd, const TraceRay& ray, float tcap,
+                                            float (&dist)[W]) {
+  const v3 o = ray.pre.o;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    float t0x = (nd.lo_x[k] - o.x) * ray.inv.x, t1x = (nd.hi_x[k] - o.x) * ray.inv.x;
+    float t0y = (nd.lo_y[k] - o.y) * ray.inv.y, t1y = (nd.hi_y[k] - o.y) * ray.inv.y;
+    float t0z = (nd.lo_z[k] - o.z) * ray.inv.z, t1z = (nd.hi_z[k] - o.z) * ray.inv.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), ray.pre.tmin));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcap));
+    dist[k] = (tn <= tf && nd.child[k] != REF_EMPTY) ? tn : INFINITY;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void cswap(float (&d)[W], uint32_t (&r)[W], int a, int b) {
+  if (d[b] < d[a]) {
+    float td = d[a];
+    d[a] = d[b];
+    d[b] = td;
+    uint32_t tr = r[a];
+    r[a] = r[b];
+    r[b] = tr;
+  }
+}
+
+// Ascending sort of (dist, ref) by dist: optimal networks for 2, 4 and 8.
+template <int W>
+__device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&r)[W]) {
+  if constexpr (W == 2) {
+    cswap<W>(d, r, 0, 1);
+  } else if constexpr (W == 4) {
+    cswap<W>(d, r, 0, 1); cswap<W>(d, r, 2, 3); cswap<W>(d, r, 0, 2);
+    cswap<W>(d, r, 1, 3); cswap<W>(d, r, 1, 2);
+  } else {
+    cswap<W>(d, r, 0, 2); cswap<W>(d, r, 1, 3); cswap<W>(d, r, 4, 6); cswap<W>(d, r, 5, 7);
+    cswap<W>(d, r, 0, 4); cswap<W>(d, r, 1, 5); cswap<W>(d, r, 2, 6); cswap<W>(d, r, 3, 7);
+    cswap<W>(d, r, 0, 1); cswap<W>(d, r, 2, 3); cswap<W>(d, r, 4, 5); cswap<W>(d, r, 6, 7);
+    cswap<W>(d, r, 2, 4); cswap<W>(d, r, 3, 5);
+    cswap<W>(d, r, 1, 4); cswap<W>(d, r, 3, 6);
+    cswap<W>(d, r, 1, 2); cswap<W>(d, r, 3, 4); cswap<W>(d, r, 5, 6);
+  }
+}
+
+__device__ __forceinline__ bool leaf_tri(const float4* __restrict__ tris, uint32_t j,
+                                         const TraceRay& ray, TriHit& h, uint32_t& prim) {
+  const float4 a = tris[3 * (size_t)j + 0];
+  const float4 b = tris[3 * (size_t)j + 1];
+  const float4 c = tris[3 * (size_t)j + 2];
+  prim = f2u(a.w);
+  return intersect_tri(ray.pre, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h);
+}
+
+// Closest hit: the lexicographic minimum of (t, prim) over all triangles that
+// pass the watertight test with tmin < t < tmax.  A node is entered while its
+// padded box starts no later than the best t so far, so equal-t candidates are
+// still visited and the tie goes to the lower prim.
+template <int W>
+__device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
+                                                const float4* __restrict__ tris, uint32_t root,
+                                                const TraceRay& ray, LdsStack st) {
+  uint32_t ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  HitRec best{ray.pre.tmax, 0.0f, 0.0f, INVALID};
+  uint32_t ref = root;
+  while (true) {
+    if (ref_is_leaf(ref)) {
+      const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
+      for (uint32_t j = first; j < first + cnt; ++j) {
+        TriHit h;
+        uint32_t prim;
+        if (leaf_tri(tris, j, ray, h, prim) && hit_less(h.t, prim, best.t, best.prim)) {
+          best.t = h.t;
+          best.u = h.u;
+          best.v = h.v;
+          best.prim = prim;
+        }
+      }
+    } else {
+      WNode<W> nd;
+      load_node<W>(nodes, ref, nd);
+      float dist[W];
+      uint32_t refs[W];
+      child_dists<W>(nd, ray, best.t, dist);
+#pragma unroll
+      for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
+      sort_children<W>(dist, refs);
+      if (dist[0] != INFINITY) {
+#pragma unroll
+        for (int k = W - 1; k >= 1; --k) {
+          if (dist[k] != INFINITY && sp < TRACE_LDS_STACK + TRACE_OVF_STACK) {
+            if (sp < TRACE_LDS_STACK) st.base[sp * st.stride] = refs[k];
+            else ovf[sp - TRACE_LDS_STACK] = refs[k];
+            ++sp;
+          }
+        }
+        ref = refs[0];
+        continue;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = sp < TRACE_LDS_STACK ? st.base[sp * st.stride] : ovf[sp - TRACE_LDS_STACK];
+  }
+  return best;
+}
+
+// Any hit: true as soon as one triangle passes with tmin < t < tmax
+// (the reference's terminate-on-first-hit shadow query, helpers/trace.wgsl:20-26).
+template <int W>
+__device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
+                                          const float4* __restrict__ tris, uint32_t root,
+                                          const TraceRay& ray, LdsStack st) {
+  uint32_t ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  uint32_t ref = root;
+  while (true) {
+    if (ref_is_leaf(ref)) {
+      const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
+      for (uint32_t j = first; j < first + cnt; ++j) {
+        TriHit h;
+        uint32_t prim;
+        if (leaf_tri(tris, j, ray, h, prim)) return true;
+      }
+    } else {
+      WNode<W> nd;
+      load_node<W>(nodes, ref, nd);
+      float dist[W];
+      child_dists<W>(nd, ray, ray.pre.tmax, dist);
+      uint32_t next = REF_EMPTY;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        if (dist[k] == INFINITY) continue;
+        if (next == REF_EMPTY) {
+          next = nd.child[k];
+        } else if (sp < TRACE_LDS_STACK + TRACE_OVF_STACK) {
+          if (sp < TRACE_LDS_STACK) st.base[sp * st.stride] = nd.child[k];
+          else ovf[sp - TRACE_LDS_STACK] = nd.child[k];
+          ++sp;
+        }
+      }
+      if (next != REF_EMPTY) {
+        ref = next;
+        continue;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = sp < TRACE_LDS_STACK ? st.base[sp * st.stride] : ovf[sp - TRACE_LDS_STACK];
+  }
+  return false;
+}
+
+}  // namespace apt

@@ -1,0 +1,98 @@
+"""Ray-query level API: BVH build and closest/any-hit traversal on the GPU.
+
+Mirrors what the reference obtains from the Vulkan driver: the BLAS build
+(src/scene_resources/scene_model.rs:53-88) and the ray query
+(rayQueryInitialize/Proceed/GetCommittedIntersection, trace.wgsl:159-162;
+terminate-on-first-hit shadow queries, helpers/trace.wgsl:20-26).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import BvhOpts, BvhStats, check, lib, vptr
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4")])
+
+BUILDER_HOST = 0
+BUILDER_GPU = 1
+
+
+class Bvh:
+    """A wide BVH resident in device memory (libapt's apt_bvh)."""
+
+    def __init__(self, tris: np.ndarray, width: int = 4, max_leaf: int = 4, bins: int = 32,
+                 builder: int = BUILDER_HOST):
+        tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        self.n_tris = tris.shape[0]
+        opts = BvhOpts(width, max_leaf, bins, builder)
+        h = C.c_void_p()
+        check(lib().apt_bvh_build(_ffi.ptr(tris), self.n_tris, C.byref(opts), C.byref(h)),
+              "apt_bvh_build")
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def stats(self) -> dict:
+        s = BvhStats()
+        check(lib().apt_bvh_get_stats(self._h, C.byref(s)), "apt_bvh_get_stats")
+        return {f: getattr(s, f) for f, _ in BvhStats._fields_}
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        """Host arrays in and out (synchronous)."""
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)
+        hits = np.empty(rays.shape[0], dtype=HIT_DTYPE)
+        check(lib().apt_trace_closest_host(self._h, vptr(rays), rays.shape[0], vptr(hits)),
+              "apt_trace_closest_host")
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)
+        occ = np.empty(rays.shape[0], dtype=np.uint32)
+        check(lib().apt_trace_any_host(self._h, vptr(rays), rays.shape[0], vptr(occ)),
+              "apt_trace_any_host")
+        return occ
+
+    def trace_closest_device(self, d_rays, n: int, d_hits, stream=None) -> None:
+        """Device pointers (torch tensors or raw ints); asynchronous on stream."""
+        check(lib().apt_trace_closest(self._h, _dp(d_rays), n, _dp(d_hits), _dp(stream)),
+              "apt_trace_closest")
+
+    def trace_any_device(self, d_rays, n: int, d_occ, stream=None) -> None:
+        check(lib().apt_trace_any(self._h, _dp(d_rays), n, _dp(d_occ), _dp(stream)),
+              "apt_trace_any")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().apt_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dp(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    if hasattr(x, "cuda_stream"):
+        return C.c_void_p(x.cuda_stream)
+    return x
+
+
+def rays_bvh_stress(seed: int = 2, side: int = 1024, n_incoherent: int = 1 << 20) -> np.ndarray:
+    """The BVH-stress ray sets: side*side coherent + n_incoherent random rays."""
+    out = np.empty(side * side + n_incoherent, dtype=RAY_DTYPE)
+    check(lib().apt_rays_bvh_stress(seed, side, n_incoherent, vptr(out)), "apt_rays_bvh_stress")
+    return out

@@ -1,0 +1,161 @@
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+Default workload (bvh_stress): 1,048,576 random triangles (seed 1) and
+2,097,152 rays (1M coherent primary rays of a 1024x1024 pinhole camera at
+(0,0,-25), 1M incoherent rays, seed 2).  One step is one closest-hit pass
+over all rays (libapt's apt_trace_closest) with rays and hits resident in
+HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Msamples/s & Mrays/s at 1080p; served p50/p99 latency vs one-at-a-time"
+HBM_PEAK_GBS = 8000.0
+COUNTS = ROOT / "tests" / "golden" / "bvh_stress_counts.json"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline_trace(tris, rays, closest_bytes):
+    """The oracle's BVH2 traversal on the host cores, on a bounded sample."""
+    import numpy as np
+
+    from tests._oracle import load_oracle
+
+    o = load_oracle()
+    threads = max(1, os.cpu_count() or 1)
+    t0 = time.time()
+    b = o.bvh(tris)
+    build_s = time.time() - t0
+    n_half = 1 << 20
+    rng = np.random.default_rng(0)
+    # 1/4 of each ray set, so the sample has the workload's coherent/incoherent mix
+    k = n_half // 4
+    idx = np.concatenate([rng.choice(n_half, k, replace=False),
+                          n_half + rng.choice(n_half, k, replace=False)])
+    sample = np.ascontiguousarray(rays.view(np.float32).reshape(-1, 8)[np.sort(idx)])
+    b.trace(sample[:4096], threads=threads)  # warm-up
+    times = []
+    for _ in range(3):
+        t0 = time.time()
+        b.trace(sample, threads=threads)
+        times.append(time.time() - t0)
+    t = sorted(times)[1]
+    return {"value": sample.shape[0] / t / 1e6, "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{sample.shape[0]} rays (1/4 of the coherent and incoherent sets), "
+                      f"closest hit, oracle BVH2 built in {build_s:.1f} s, median of 3"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(tiles/rays shard per GPU with no exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from appearance_amd.raytrace import Bvh, rays_bvh_stress
+    from appearance_amd.scene import generate_scene
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    log("generating bvh_stress scene and rays")
+    scene = generate_scene("bvh_stress", seed=1)
+    rays = rays_bvh_stress(seed=2, side=1024, n_incoherent=1 << 20)
+    n = rays.shape[0]
+    log(f"building BVH (width {args.width})")
+    bvh = Bvh(scene.positions, width=args.width)
+    st = bvh.stats()
+    d_rays = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
+    d_hits = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+    d_occ = torch.empty(n, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        bvh.trace_closest_device(d_rays, n, d_hits, stream)
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} steps")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        bvh.trace_closest_device(d_rays, n, d_hits, stream)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = sum(launch_ms) / len(launch_ms)
+    hits = d_hits.view(torch.int32).view(n, 4)[:, 3].cpu().numpy().view(np.uint32)
+    n_hit = int((hits != 0xFFFFFFFF).sum())
+
+    # any-hit pass (reported beside the headline)
+    for _ in range(2):
+        bvh.trace_any_device(d_rays, n, d_occ, stream)
+    ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ea.record(stream)
+    for _ in range(args.steps):
+        bvh.trace_any_device(d_rays, n, d_occ, stream)
+    eb.record(stream)
+    torch.cuda.synchronize()
+    any_ms = ea.elapsed_time(eb) / args.steps
+
+    counts = json.loads(COUNTS.read_text())
+    bpr = counts["closest_all"]["alg_bytes_per_ray"]
+    achieved = bpr * n / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": n * args.steps / wall / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bvh_stress: 1,048,576 triangles, 2,097,152 rays "
+                               "(1M coherent 1024x1024 primary + 1M incoherent), closest hit",
+                   "n_tris": scene.n_tris, "n_rays": n, "bvh_width": args.width,
+                   "bvh_nodes": st["n_nodes"], "bvh_build_ms": round(st["build_ms"], 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_ray": bpr, "kernel_avg_ms": avg_ms},
+        "closest_hits": n_hit,
+        "any_hit_mrays_s": n / (any_ms * 1e-3) / 1e6,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle BVH2 on host cores)")
+        out["cpu_baseline"] = cpu_baseline_trace(scene.positions, rays, bpr)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,335 @@
+/*
+ * apt.h - C ABI of libapt, the MI355X (gfx950) path-tracing hot path.
+ *
+ * Drop-in for the render-node job interface of Appearance:
+ *   NodeRenderer              crates/appearance-render-loop/src/node.rs:14-25
+ *   DistributedRenderer       crates/appearance-distributed-renderer/src/lib.rs:60-82
+ *   PathTracerGpu::{new, handle_visible_world_action, render}
+ *                             crates/appearance-path-tracer-gpu/src/lib.rs:260-615
+ * and for the hardware ray query those kernels call
+ *   rayQueryInitialize/Proceed/GetCommittedIntersection
+ *                             crates/appearance-path-tracer-gpu/assets/shaders/trace.wgsl:159-162,
+ *                             helpers/trace.wgsl:20-26 (terminate-on-first-hit shadow query)
+ *   BLAS/TLAS build           src/scene_resources/scene_model.rs:53-88, scene_resources/mod.rs:218-276
+ *
+ * Conventions
+ *   - Every entry point returns apt_status and never aborts; on failure the
+ *     reason is readable through apt_last_error() on the calling thread.
+ *   - Structs are plain C PODs; sizes are fixed and checked by static asserts
+ *     in the implementation.  A Rust -sys crate mirrors them 1:1.
+ *   - Pointers named d_* are device (HBM) pointers; every other pointer is host
+ *     memory owned by the caller.
+ *   - One apt_renderer is not re-entrant (the reference's &mut self); callers
+ *     serialise calls on it.  Distinct renderers may be used from distinct
+ *     threads.
+ */
+#ifndef APT_H
+#define APT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APT_ABI_VERSION 1u
+
+typedef enum apt_status {
+  APT_OK = 0,
+  APT_ERR_INVALID_ARG = 1,
+  APT_ERR_OOM = 2,
+  APT_ERR_DEVICE = 3,
+  APT_ERR_UNSUPPORTED = 4,
+  APT_ERR_BUSY = 5,
+  APT_ERR_NOT_FOUND = 6
+} apt_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* apt_last_error(void);
+uint32_t apt_abi_version(void);
+
+typedef struct apt_device_info {
+  char name[64];
+  char arch[32];         /* e.g. "gfx950" */
+  uint32_t compute_units;
+  uint32_t wavefront_size;
+  uint64_t hbm_bytes;
+  uint64_t lds_bytes_per_block;
+} apt_device_info;
+apt_status apt_device_query(apt_device_info* out);
+
+/* ------------------------------------------------------------------------
+ * Ray-query level.  Replaces the driver BVH build and rayQuery* traversal.
+ * ------------------------------------------------------------------------ */
+
+/* 32 B ray: origin, tmin, direction (need not be normalised), tmax. */
+typedef struct apt_ray {
+  float org[3];
+  float tmin;
+  float dir[3];
+  float tmax;
+} apt_ray;
+
+/* 16 B closest hit.  prim = APT_MISS when nothing is hit; u, v are the weights
+ * of the second and third vertex (the reference's intersection.barycentrics,
+ * trace.wgsl:168).  The hit is the lexicographic minimum of (t, prim) over all
+ * triangles with tmin < t < tmax under the watertight test, so it does not
+ * depend on the BVH width or on traversal order. */
+typedef struct apt_hit {
+  float t;
+  float u;
+  float v;
+  uint32_t prim;
+} apt_hit;
+
+#define APT_MISS 0xFFFFFFFFu
+
+typedef enum apt_builder {
+  APT_BUILDER_HOST = 0,  /* binned SAH on the host threads of libapt */
+  APT_BUILDER_GPU = 1    /* binned SAH on the device */
+} apt_builder;
+
+typedef struct apt_bvh_opts {
+  uint32_t width;     /* 2, 4 or 8 children per node (0 = 4) */
+  uint32_t max_leaf;  /* triangles per leaf, 1..8 (0 = 4) */
+  uint32_t bins;      /* SAH bins per axis, 4..64 (0 = 32) */
+  uint32_t builder;   /* apt_builder */
+} apt_bvh_opts;
+
+typedef struct apt_bvh_stats {
+  uint32_t n_tris;
+  uint32_t n_nodes;
+  uint32_t n_leaves;
+  uint32_t width;
+  uint32_t depth;       /* levels of wide nodes */
+  uint32_t max_stack;   /* worst-case traversal stack entries */
+  uint64_t device_bytes;
+  double build_ms;
+  double sah_cost;      /* SAH cost of the binary tree before collapse */
+} apt_bvh_stats;
+
+typedef struct apt_bvh apt_bvh;
+
+/* Build from n_tris triangles, tri_xyz = n_tris * 9 floats (v0 v1 v2) in host
+ * memory.  The triangles are copied to the device; prim ids are their input
+ * indices. */
+apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
+                         apt_bvh** out);
+/* Same, with tri_xyz already resident in device memory. */
+apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris,
+                                const apt_bvh_opts* opts, apt_bvh** out);
+void apt_bvh_destroy(apt_bvh* bvh);
+apt_status apt_bvh_get_stats(const apt_bvh* bvh, apt_bvh_stats* out);
+
+/* Closest-hit query (ray flags 0 in the reference): d_rays/d_hits are device
+ * arrays of n entries; stream is a hipStream_t (NULL = the library stream).
+ * The call is asynchronous on that stream. */
+apt_status apt_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
+                             apt_hit* d_hits, void* stream);
+/* Any-hit query (RAY_FLAG_TERMINATE_ON_FIRST_HIT, helpers/trace.wgsl:22):
+ * d_occluded[i] = 1 when some triangle has tmin < t < tmax, else 0. */
+apt_status apt_trace_any(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
+                         uint32_t* d_occluded, void* stream);
+/* Synchronous variants on host arrays (copies in and out). */
+apt_status apt_trace_closest_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
+                                  apt_hit* hits);
+apt_status apt_trace_any_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
+                              uint32_t* occluded);
+
+/* ------------------------------------------------------------------------
+ * Scene data.  Layouts follow the reference's GPU pools.
+ * ------------------------------------------------------------------------ */
+
+#define APT_INVALID_TEXTURE 0xFFFFFFFFu
+
+/* 160 B; field for field the reference's MaterialDescriptor
+ * (src/scene_resources/material_pool.rs:13-46, shared/material/material_pool.wgsl:6-41).
+ * bsdf_model occupies the reference's _padding0: 0 = Disney BSDF (the
+ * reference's only model), 1 = Lambertian (the procedural Cornell box). */
+typedef struct apt_material {
+  float color[3];
+  uint32_t color_texture;
+  float metallic;
+  float roughness;
+  uint32_t metallic_roughness_texture;
+  float normal_scale;
+  float emission[3];
+  uint32_t normal_texture;
+  uint32_t emission_texture;
+  float transmission;
+  float eta;
+  float subsurface;
+  float absorption[3];
+  float specular;
+  float specular_tint[3];
+  float anisotropic;
+  float sheen;
+  uint32_t sheen_texture;
+  float clearcoat;
+  uint32_t clearcoat_texture;
+  float clearcoat_roughness;
+  uint32_t clearcoat_roughness_texture;
+  float alpha_cutoff;
+  uint32_t sheen_tint_texture;
+  uint32_t clearcoat_normal_texture;
+  uint32_t bsdf_model;
+  uint32_t _padding1;
+  uint32_t _padding2;
+  float sheen_tint[3];
+  uint32_t transmission_texture;
+} apt_material;
+
+#define APT_BSDF_DISNEY 0u
+#define APT_BSDF_LAMBERT 1u
+
+/* Sets every field to the reference's glTF defaults: white, metallic 0,
+ * roughness 1, eta 1/1.5 (crates/appearance-model/src/asset.rs:229), no
+ * textures, Disney model. */
+void apt_material_default(apt_material* m);
+
+/* Sun (the reference's SunInfo, src/scene_resources/sky.rs:11-22, 32 B)
+ * plus an analytic gradient sky that stands in for the HDR panorama. */
+typedef struct apt_sky {
+  float sun_direction[3]; /* direction the sunlight travels (reference convention) */
+  float sun_size;
+  float sun_color[3];
+  float sun_intensity;
+  float zenith[3];        /* sky radiance straight up */
+  float sun_enabled;      /* 0 or 1 */
+  float horizon[3];       /* sky radiance at the horizon */
+  float _pad0;
+  float ground[3];        /* radiance below the horizon */
+  float _pad1;
+} apt_sky;
+
+/* A flat triangle soup with per-triangle material ids. */
+typedef struct apt_mesh_desc {
+  const float* positions;        /* n_tris * 9 */
+  const float* normals;          /* n_tris * 9 vertex normals, or NULL = geometric */
+  const uint32_t* material_ids;  /* n_tris, or NULL = all 0 */
+  uint32_t n_tris;
+  uint32_t n_materials;
+  const apt_material* materials; /* n_materials */
+} apt_mesh_desc;
+
+/* The reference's CameraUpdateData (crates/appearance-world/src/visible_world_action.rs:9-15),
+ * 80 B: column-major world transform of the camera (it looks down its -Z),
+ * vertical fov in degrees, near, far. */
+typedef struct apt_camera {
+  float transform[16];
+  float fov_deg;
+  float near_plane;
+  float far_plane;
+  uint32_t _padding;
+} apt_camera;
+
+/* Procedural scenes generated inside the library from a seed:
+ *   "cornell"   Cornell box, 36 triangles, Lambertian, quad light (17,12,4)
+ *   "bvh_stress" 1,048,576 random triangles in [-10,10]^3
+ *   "disney"    256 UV spheres x 2048 triangles + ground, Disney materials
+ *   "restir"    "disney" + 4096 small emissive triangles
+ * The same names work as asset paths "procedural://<name>?seed=<n>" in a
+ * SpawnModel world action. */
+typedef struct apt_scene_data apt_scene_data;
+apt_status apt_scene_generate(const char* name, uint32_t seed, apt_scene_data** out);
+/* Views into the library-owned arrays; valid until apt_scene_data_destroy. */
+apt_status apt_scene_data_view(const apt_scene_data* s, apt_mesh_desc* mesh, apt_camera* camera,
+                               apt_sky* sky);
+void apt_scene_data_destroy(apt_scene_data* s);
+
+/* The C1 ray sets: n_coherent primary rays of a side x side pinhole camera at
+ * (0,0,-25) looking +z with a 40 degree fov (side*side == n_coherent), then
+ * n_incoherent rays with origins uniform in [-10,10]^3 and uniform-sphere
+ * directions, all with tmin 0 and tmax 1e30. */
+apt_status apt_rays_bvh_stress(uint32_t seed, uint32_t side, uint32_t n_incoherent, apt_ray* out);
+
+/* ------------------------------------------------------------------------
+ * Renderer (PathTracerGpu / NodeRenderer).
+ * ------------------------------------------------------------------------ */
+
+typedef enum apt_integrator {
+  APT_INTEGRATOR_PLAIN = 0,  /* unidirectional PT, NEE + MIS, Russian roulette */
+  APT_INTEGRATOR_RESTIR = 1  /* ReSTIR DI/GI (not yet available) */
+} apt_integrator;
+
+/* PathTracerGpuConfig (src/lib.rs:219-245), made public, plus the knobs the
+ * reference keeps as shader constants and the MI355X tuning parameters. */
+typedef struct apt_config {
+  uint32_t max_bounces;       /* path segments traced per sample */
+  uint32_t sample_count;      /* samples per pixel per render call */
+  uint32_t accum_frames;      /* progressive accumulation across calls */
+  uint32_t rr_start_bounce;   /* Russian roulette from this bounce on */
+  uint32_t integrator;        /* apt_integrator */
+  uint32_t restir_di;
+  uint32_t restir_gi;
+  uint32_t svgf;
+  uint32_t firefly_filter;
+  uint32_t taa;
+  uint32_t di_candidates;     /* light candidates per pixel (RIS) */
+  uint32_t di_mcap;           /* temporal M cap */
+  uint32_t di_spatial_neighbours;
+  float di_spatial_radius;    /* pixels */
+  uint32_t gi_mcap;
+  uint32_t svgf_iterations;
+  uint32_t svgf_history;
+  uint32_t bvh_width;         /* 2, 4, 8 */
+  uint32_t queue_chunk;       /* paths per wavefront queue chunk (0 = whole frame) */
+  uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS */
+  uint32_t seed;              /* base seed mixed into every path seed */
+  uint32_t frame_idx;         /* starting frame index */
+} apt_config;
+
+/* PathTracerGpuConfig::default (src/lib.rs:230-245) for the switches; the
+ * plain integrator; MI355X tuning defaults. */
+void apt_config_default(apt_config* cfg);
+
+typedef struct apt_renderer apt_renderer;
+
+apt_status apt_create(const apt_config* cfg, apt_renderer** out);
+void apt_destroy(apt_renderer* r);
+apt_status apt_set_config(apt_renderer* r, const apt_config* cfg);
+
+/* VisibleWorldActionType::from_ty_and_bytes (visible_world_action.rs:87-96):
+ * ty 0 CameraUpdate (80 B), 1 SpawnModel (336 B), 2 TransformModel (80 B),
+ * 3 DestroyModel (16 B), 4 Clear (4 B).  SpawnModel accepts
+ * "procedural://<name>?seed=<n>" asset paths; glTF files return
+ * APT_ERR_UNSUPPORTED. */
+apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, size_t len);
+
+/* Replace the whole scene with a triangle soup (procedural or caller-made). */
+apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_sky* sky);
+
+/* PathTracerGpu::render: renders rows [row_start, row_end) of a width x height
+ * frame (width and the slab height multiples of 64, host.rs:476).  Writes
+ * RGBA8 in 64x64 block-major order (resolve.wgsl:56-57, block.wgsl:3-15) to
+ * out_rgba (width*(row_end-row_start)*4 bytes, may be NULL) and, when
+ * out_accum_rgb is not NULL, the FP32 linear radiance mean per pixel in row
+ * order (3 floats per pixel). */
+apt_status apt_render(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
+                      uint32_t row_end, uint8_t* out_rgba, size_t out_len, float* out_accum_rgb,
+                      size_t accum_len);
+
+/* The closure form of render (NodeRenderer::render's result_callback). */
+typedef void (*apt_result_cb)(const uint8_t* pixels, size_t len, void* user);
+apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
+                         uint32_t row_end, apt_result_cb cb, void* user);
+
+typedef struct apt_render_stats {
+  uint64_t paths;            /* camera paths started in the last render */
+  uint64_t extension_rays;   /* closest-hit rays traced */
+  uint64_t shadow_rays;      /* any-hit rays traced */
+  uint64_t segments;         /* shaded path vertices */
+  double ms_total;           /* device time of the last render (hipEvents) */
+  double ms_extend;
+  double ms_shade;
+  double ms_shadow;
+  double ms_other;
+} apt_render_stats;
+apt_status apt_get_render_stats(const apt_renderer* r, apt_render_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APT_H */

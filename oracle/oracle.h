@@ -1,0 +1,32 @@
+// ORACLE - test infrastructure only (see oracle_core.cpp).  C entry points
+// loaded by tests/ and bench.py through ctypes.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+uint32_t or_pcg_hash(uint32_t x);
+uint32_t or_xor_shift(uint32_t x);
+uint32_t or_hash_combine(uint32_t x, uint32_t y);
+uint32_t or_wangh_hash(uint32_t x);
+void or_uniform_floats(uint32_t state, uint32_t n, float* out);
+void or_rgb9e5_pack(const float* rgb, uint32_t n, uint32_t* out);
+void or_rgb9e5_unpack(const uint32_t* d, uint32_t n, float* rgb);
+void or_oct30_pack(const float* v, uint32_t n, uint32_t* out);
+void or_oct30_unpack(const uint32_t* p, uint32_t n, float* v);
+int or_tri_test(const float* ray, const float* tri, float* out_tuv);
+void* or_bvh_build(const float* tris, uint32_t n);
+void or_bvh_destroy(void* b);
+uint32_t or_bvh_node_count(void* b);
+void or_trace(void* b, const float* rays, uint32_t n, int any, float* out, uint64_t* counts,
+              int threads);
+void or_brute(const float* tris, uint32_t ntris, const float* rays, uint32_t n, int any,
+              float* out);
+uint32_t or_block_index(uint32_t x, uint32_t y, uint32_t width);
+float or_aces(float x);
+
+#ifdef __cplusplus
+}
+#endif

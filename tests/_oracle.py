@@ -1,0 +1,130 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so).  Test-side only."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB = ROOT / "oracle" / "liboracle.so"
+
+_P = C.c_void_p
+_SIGS = {
+    "or_pcg_hash": (C.c_uint32, [C.c_uint32]),
+    "or_xor_shift": (C.c_uint32, [C.c_uint32]),
+    "or_hash_combine": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "or_wangh_hash": (C.c_uint32, [C.c_uint32]),
+    "or_uniform_floats": (None, [C.c_uint32, C.c_uint32, _P]),
+    "or_rgb9e5_pack": (None, [_P, C.c_uint32, _P]),
+    "or_rgb9e5_unpack": (None, [_P, C.c_uint32, _P]),
+    "or_oct30_pack": (None, [_P, C.c_uint32, _P]),
+    "or_oct30_unpack": (None, [_P, C.c_uint32, _P]),
+    "or_tri_test": (C.c_int, [_P, _P, _P]),
+    "or_bvh_build": (_P, [_P, C.c_uint32]),
+    "or_bvh_destroy": (None, [_P]),
+    "or_bvh_node_count": (C.c_uint32, [_P]),
+    "or_trace": (None, [_P, _P, C.c_uint32, C.c_int, _P, _P, C.c_int]),
+    "or_brute": (None, [_P, C.c_uint32, _P, C.c_uint32, C.c_int, _P]),
+    "or_block_index": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "or_aces": (C.c_float, [C.c_float]),
+}
+
+
+def _vp(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+class Oracle:
+    def __init__(self, lib: C.CDLL):
+        self.lib = lib
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name, None)
+            if fn is not None:
+                fn.restype = res
+                fn.argtypes = args
+
+    def uniform_floats(self, state: int, n: int) -> np.ndarray:
+        out = np.empty(n, np.float32)
+        self.lib.or_uniform_floats(state, n, _vp(out))
+        return out
+
+    def rgb9e5_pack(self, rgb: np.ndarray) -> np.ndarray:
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        out = np.empty(rgb.shape[0], np.uint32)
+        self.lib.or_rgb9e5_pack(_vp(rgb), rgb.shape[0], _vp(out))
+        return out
+
+    def rgb9e5_unpack(self, d: np.ndarray) -> np.ndarray:
+        d = np.ascontiguousarray(d, np.uint32).reshape(-1)
+        out = np.empty((d.shape[0], 3), np.float32)
+        self.lib.or_rgb9e5_unpack(_vp(d), d.shape[0], _vp(out))
+        return out
+
+    def oct30_pack(self, v: np.ndarray) -> np.ndarray:
+        v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+        out = np.empty(v.shape[0], np.uint32)
+        self.lib.or_oct30_pack(_vp(v), v.shape[0], _vp(out))
+        return out
+
+    def oct30_unpack(self, p: np.ndarray) -> np.ndarray:
+        p = np.ascontiguousarray(p, np.uint32).reshape(-1)
+        out = np.empty((p.shape[0], 3), np.float32)
+        self.lib.or_oct30_unpack(_vp(p), p.shape[0], _vp(out))
+        return out
+
+    def tri_test(self, ray: np.ndarray, tri: np.ndarray):
+        ray = np.ascontiguousarray(ray, np.float32).reshape(8)
+        tri = np.ascontiguousarray(tri, np.float32).reshape(9)
+        out = np.zeros(3, np.float32)
+        ok = self.lib.or_tri_test(_vp(ray), _vp(tri), _vp(out))
+        return (bool(ok), out)
+
+    def bvh(self, tris: np.ndarray) -> "OracleBvh":
+        return OracleBvh(self, tris)
+
+    def brute(self, tris: np.ndarray, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        rays = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        out = np.empty(n if any_hit else (n, 4), np.float32)
+        self.lib.or_brute(_vp(tris), tris.shape[0], _vp(rays), n, int(any_hit), _vp(out))
+        return out
+
+
+class OracleBvh:
+    def __init__(self, o: Oracle, tris: np.ndarray):
+        self.o = o
+        self.tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self.h = C.c_void_p(o.lib.or_bvh_build(_vp(self.tris), self.tris.shape[0]))
+        self.last_counts = (0, 0)
+
+    def trace(self, rays: np.ndarray, any_hit: bool = False, threads: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        out = np.empty(n if any_hit else (n, 4), np.float32)
+        counts = np.zeros(2, np.uint64)
+        self.o.lib.or_trace(self.h, _vp(rays), n, int(any_hit), _vp(out), _vp(counts), threads)
+        self.last_counts = (int(counts[0]), int(counts[1]))
+        return out
+
+    def __del__(self):
+        try:
+            self.o.lib.or_bvh_destroy(self.h)
+        except Exception:
+            pass
+
+
+def hits_as_struct(h: np.ndarray):
+    """(n,4) float32 oracle hits -> t, u, v (float32) and prim (uint32)."""
+    h = np.ascontiguousarray(h, np.float32)
+    return h[:, 0], h[:, 1], h[:, 2], h[:, 3].view(np.uint32)
+
+
+def load_oracle() -> Oracle:
+    if not LIB.exists():
+        import subprocess
+        import sys
+        subprocess.run([sys.executable, str(ROOT / "oracle" / "build.py")], check=True)
+    return Oracle(C.CDLL(str(LIB)))
