@@ -73,6 +73,9 @@ static apt_status resolve_opts(const apt_bvh_opts* opts, apt_bvh_opts& o) {
   if (o.width == 0) o.width = 4;
   if (o.max_leaf == 0) o.max_leaf = 4;
   if (o.bins == 0) o.bins = 32;
+  if (o.lds_stack == 0) o.lds_stack = 16;
+  APT_REQUIRE(o.lds_stack == 8 || o.lds_stack == 16 || o.lds_stack == 24, APT_ERR_INVALID_ARG,
+              "bvh: lds_stack %u is not 8, 16 or 24", o.lds_stack);
   APT_REQUIRE(o.width == 2 || o.width == 4 || o.width == 8, APT_ERR_INVALID_ARG,
               "bvh: width %u is not 2, 4 or 8", o.width);
   APT_REQUIRE(o.max_leaf >= 1 && o.max_leaf <= 8, APT_ERR_INVALID_ARG,
@@ -106,6 +109,7 @@ apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_op
     delete bvh;
     return st;
   }
+  bvh->lds_stack = o.lds_stack;
   bvh->stats.build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   *out = bvh;

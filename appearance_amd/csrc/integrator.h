@@ -32,6 +32,7 @@ struct SceneView {
   const float4* bvh_tris;
   uint32_t root;
   uint32_t width;
+  uint32_t lds_stack;           // traversal stack entries in LDS (8, 16, 24)
   const float4* tri_pos;        // 3 per triangle
   const float4* tri_nrm;        // 3 per triangle, or null (geometric normals)
   const uint32_t* tri_mat;

@@ -90,10 +90,10 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(FrameView fv, SceneView sc, Pa
   ps.queue[0][p] = p;
 }
 
-template <int W>
+template <int W, int S>
 __global__ __launch_bounds__(BLOCK) void k_extend(SceneView sc, PathView ps, int qin,
                                                   const WNode<W>* __restrict__ nodes) {
-  __shared__ uint32_t s_stack[TRACE_LDS_STACK * BLOCK];
+  __shared__ uint2 s_stack[S * BLOCK];
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneView sc, PathView ps, int
   const uint32_t p = ps.queue[qin][i];
   const float4 o = ps.o_pdf[p], d = ps.d_flags[p];
   TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f);
-  HitRec h = trace_closest<W>(nodes, sc.bvh_tris, sc.root, ray, LdsStack{s_stack + threadIdx.x, BLOCK});
+  HitRec h = trace_closest<W, S>(nodes, sc.bvh_tris, sc.root, ray, LdsStack{s_stack + threadIdx.x, BLOCK});
   ps.hit[p] = make_float4(h.t, h.u, h.v, u2f(h.prim));
 }
 
@@ -177,7 +177,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, Pat
       if (p_sun > 0.0f || sc.n_emissive > 0) {
         v3 wi, Le;
         float p_l = 0.0f, dist = 0.0f;
-        if (u_sel < p_sun) {
+        // rng_next can return exactly 1.0, so the sun branch is also taken
+        // whenever there is no emissive triangle to index.
+        if (u_sel < p_sun || sc.n_emissive == 0) {
           wi = normalize(perturb_direction(u_a, u_b, sun_to(sc.sky), sun_half_angle(sc.sky)));
           float si = sun_intensity(sc.sky, wi);
           Le = mk(sc.sky.sun_color[0], sc.sky.sun_color[1], sc.sky.sun_color[2]) * si;
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, Pat
           dist = SUN_DISTANCE;
         } else {
           const float us = (u_sel - p_sun) / (1.0f - p_sun);
-          uint32_t k = (uint32_t)(us * (float)sc.n_emissive);
+          uint32_t k = (uint32_t)fmaxf(0.0f, us * (float)sc.n_emissive);
           k = k < sc.n_emissive ? k : sc.n_emissive - 1;
           const uint32_t lp = sc.emissive[k];
           const TriGeom lg = load_tri(sc, lp);
@@ -253,10 +255,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, Pat
   if (sslot >= 0) ps.shadow_queue[sslot] = p;
 }
 
-template <int W>
+template <int W, int S>
 __global__ __launch_bounds__(BLOCK) void k_shadow(SceneView sc, PathView ps,
                                                   const WNode<W>* __restrict__ nodes) {
-  __shared__ uint32_t s_stack[TRACE_LDS_STACK * BLOCK];
+  __shared__ uint2 s_stack[S * BLOCK];
   const uint32_t count = ps.counters[CNT_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[CNT_STAT_SHADOW] += count;
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneView sc, PathView ps,
   const uint32_t p = ps.shadow_queue[i];
   const float4 o = ps.sh_o[p], d = ps.sh_d[p];
   TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, o.w);
-  if (!trace_any<W>(nodes, sc.bvh_tris, sc.root, ray, LdsStack{s_stack + threadIdx.x, BLOCK})) {
+  if (!trace_any<W, S>(nodes, sc.bvh_tris, sc.root, ray, LdsStack{s_stack + threadIdx.x, BLOCK})) {
     const float4 c = ps.sh_c[p];
     float4 l = ps.rad[p];
     ps.rad[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
@@ -334,9 +336,11 @@ void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_p
                            const FrameView& fv, uint32_t bounce, hipStream_t s) {
   const dim3 grid((n_paths + BLOCK - 1) / BLOCK);
   const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
-  hipLaunchKernelGGL((k_extend<W>), grid, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
-  hipLaunchKernelGGL(k_shade, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
-  hipLaunchKernelGGL((k_shadow<W>), grid, dim3(BLOCK), 0, s, sc, ps, nodes);
+  APT_FOR_STACK_DEPTH(sc.lds_stack, {
+    hipLaunchKernelGGL((k_extend<W, S_>), grid, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
+    hipLaunchKernelGGL(k_shade, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
+    hipLaunchKernelGGL((k_shadow<W, S_>), grid, dim3(BLOCK), 0, s, sc, ps, nodes);
+  })
 }
 
 }  // namespace

@@ -131,6 +131,8 @@ apt_status validate_config(const apt_config& c) {
               "config: sample_count %u outside 1..65536", c.sample_count);
   APT_REQUIRE(c.bvh_width == 2 || c.bvh_width == 4 || c.bvh_width == 8, APT_ERR_INVALID_ARG,
               "config: bvh_width %u is not 2, 4 or 8", c.bvh_width);
+  APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 16 || c.lds_stack_depth == 24,
+              APT_ERR_INVALID_ARG, "config: lds_stack_depth %u is not 8, 16 or 24", c.lds_stack_depth);
   APT_REQUIRE(c.integrator == APT_INTEGRATOR_PLAIN, APT_ERR_UNSUPPORTED,
               "config: integrator %u is not available (only APT_INTEGRATOR_PLAIN)", c.integrator);
   APT_REQUIRE(!c.restir_di && !c.restir_gi, APT_ERR_UNSUPPORTED,
@@ -194,7 +196,7 @@ apt_status upload_scene(apt_renderer* r) {
     r->scene_version++;
     return APT_OK;
   }
-  apt_bvh_opts o{r->cfg.bvh_width, 4, 32, APT_BUILDER_HOST};
+  apt_bvh_opts o{r->cfg.bvh_width, 4, 32, APT_BUILDER_HOST, r->cfg.lds_stack_depth};
   apt_status st = apt_bvh_build(r->pos.data(), n, &o, &r->bvh);
   if (st != APT_OK) return st;
   std::vector<float4> p4((size_t)n * 3), n4;
@@ -252,6 +254,7 @@ SceneView scene_view(const apt_renderer* r, uint32_t width, uint32_t height) {
   sc.bvh_tris = r->bvh ? r->bvh->d_tris : nullptr;
   sc.root = r->bvh ? r->bvh->root : 0;
   sc.width = r->bvh ? r->bvh->width : r->cfg.bvh_width;
+  sc.lds_stack = r->bvh ? r->bvh->lds_stack : 16;
   sc.tri_pos = r->d_pos.p;
   sc.tri_nrm = r->has_normals ? r->d_nrm.p : nullptr;
   sc.tri_mat = r->d_mat.p;
@@ -414,7 +417,7 @@ void apt_config_default(apt_config* c) {
   c->svgf_history = 4;
   c->bvh_width = 4;
   c->queue_chunk = 0;
-  c->lds_stack_depth = 24;
+  c->lds_stack_depth = 16;
   c->seed = 0;
   c->frame_idx = 0;
 }
@@ -456,6 +459,7 @@ apt_status apt_set_config(apt_renderer* r, const apt_config* cfg) {
   apt_status st = validate_config(*cfg);
   if (st != APT_OK) return st;
   if (cfg->bvh_width != r->cfg.bvh_width) r->scene_dirty = true;
+  if (r->bvh) r->bvh->lds_stack = cfg->lds_stack_depth;
   if (cfg->sample_count != r->cfg.sample_count || cfg->max_bounces != r->cfg.max_bounces ||
       cfg->rr_start_bounce != r->cfg.rr_start_bounce || cfg->seed != r->cfg.seed)
     r->cam_version++;  // restart accumulation

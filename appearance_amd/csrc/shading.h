@@ -363,36 +363,39 @@ struct ShadeFrame {
 };
 
 // DisneyBsdf::evaluate (:567-682).  Returns f; pdf receives the density with
-// which sample() would produce wi.  Two deviations make evaluate() and
+// which sample() would produce wi.  Deviations that make evaluate() and
 // sample() agree, which the MIS weights need: the reference multiplies the
 // specular and clearcoat lobe pdfs by their lobe weight twice (:658-675),
-// here once, as in sample(); and the transmission branch picks eta from the
-// back-face flag as sample() does (:401-405), where the reference tests wol.z
-// of a frame that is always front facing (:576-580).
+// here once, as in sample(); and the transmission branch uses sample()'s
+// relative index eta = n_i/n_t (back face: 1/eta, :401-405) with the Fresnel
+// term, refraction half vector and jacobian of sample() (:418, :436-438),
+// where the reference tests wol.z of an always front-facing frame (:576-580)
+// and uses the reciprocal index (:596, :602-609).
 APT_HD v3 disney_eval(const Disney& b, const ShadeFrame& sf, v3 wo, v3 wi, bool back_face,
                       float& pdf) {
   pdf = 0.0f;
   if (b.transmission > 0.5f) {
     v3 wol = to_local(sf.f, wo), wil = to_local(sf.f, wi);
-    float eta = back_face ? 1.0f / b.eta : b.eta;
+    const float eta = back_face ? 1.0f / b.eta : b.eta;  // n_i / n_t
     if (eta == 1.0f) return mk(0.0f, 0.0f, 0.0f);
+    const float ieta = 1.0f / eta;
     Alpha2 a = alpha_from_roughness(b.roughness, b.anisotropic);
     v3 val, m;
     float jac, ct;
     if (wil.z * wol.z >= 0.0f) {
       m = half_vec(wol, wil);
       float cwm = dot(wol, m);
-      float f = fresnel_reflectance(cwm, 1.0f / eta, ct);
+      float f = fresnel_reflectance(cwm, eta, ct);
       val = eval_reflection(b.color, wol, wil, m, a, f);
       pdf = f;  // choose_reflection_probability(1, 1, f)
       jac = reflection_jacobian(cwm);
     } else {
-      m = half_vec(wol, wil * eta);
+      m = half_vec(wol, wil * ieta);
       float cwm = dot(wol, m);
-      float f = fresnel_reflectance(cwm, 1.0f / eta, ct);
-      val = eval_refraction(eta, b.color, wol, wil, m, a, 1.0f - f);
+      float f = fresnel_reflectance(cwm, eta, ct);
+      val = eval_refraction(ieta, b.color, wol, wil, m, a, 1.0f - f);
       pdf = 1.0f - f;
-      jac = refraction_jacobian(wol, wil, m, eta);
+      jac = refraction_jacobian(wol, wil, m, ieta);
     }
     pdf *= jac * ggx_pdf(a, wol, m);
     return val;

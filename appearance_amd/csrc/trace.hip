@@ -3,9 +3,10 @@
 // These replace the reference's hardware ray query (rayQueryInitialize /
 // rayQueryProceed / rayQueryGetCommittedIntersection, trace.wgsl:159-162 and
 // helpers/trace.wgsl:20-26).  One lane traces one ray.  The traversal stack
-// keeps its bottom LDS_STACK entries in LDS (laid out [entry][lane] so lanes at
-// the same depth hit distinct banks) and spills deeper entries to a small
-// private array; the host checks that the BVH's worst-case stack fits.
+// keeps its bottom S entries in LDS (laid out [entry][lane] so lanes at the
+// same depth hit distinct banks; S = 8, 16 or 24 per BVH, trading LDS for
+// occupancy) and spills deeper entries to a small private array; the host
+// checks that the BVH's worst-case stack fits.
 #include "apt_internal.h"
 #include "trace_device.h"
 
@@ -15,13 +16,13 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-template <int W, bool ANY>
+template <int W, int S, bool ANY>
 __global__ __launch_bounds__(BLOCK) void k_trace(const WNode<W>* __restrict__ nodes,
                                                  const float4* __restrict__ tris, uint32_t root,
                                                  const apt_ray* __restrict__ rays, uint32_t n,
                                                  apt_hit* __restrict__ hits,
                                                  uint32_t* __restrict__ occluded) {
-  __shared__ uint32_t s_stack[TRACE_LDS_STACK * BLOCK];
+  __shared__ uint2 s_stack[S * BLOCK];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t i = blk * BLOCK + threadIdx.x;
   if (i >= n) return;
@@ -31,9 +32,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const WNode<W>* __restrict__ no
   TraceRay ray = make_trace_ray(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w);
   LdsStack st{s_stack + threadIdx.x, BLOCK};
   if (ANY) {
-    occluded[i] = trace_any<W>(nodes, tris, root, ray, st) ? 1u : 0u;
+    occluded[i] = trace_any<W, S>(nodes, tris, root, ray, st) ? 1u : 0u;
   } else {
-    HitRec h = trace_closest<W>(nodes, tris, root, ray, st);
+    HitRec h = trace_closest<W, S>(nodes, tris, root, ray, st);
     float4 out = make_float4(h.prim == INVALID ? INFINITY : h.t, h.u, h.v, u2f(h.prim));
     reinterpret_cast<float4*>(hits)[i] = out;
   }
@@ -45,12 +46,14 @@ apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit
   if (n == 0) return APT_OK;
   const WNode<W>* nodes = static_cast<const WNode<W>*>(bvh->d_nodes);
   dim3 grid((n + BLOCK - 1) / BLOCK);
-  if (any)
-    hipLaunchKernelGGL((k_trace<W, true>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris, bvh->root,
-                       d_rays, n, d_hits, d_occ);
-  else
-    hipLaunchKernelGGL((k_trace<W, false>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
-                       bvh->root, d_rays, n, d_hits, d_occ);
+  APT_FOR_STACK_DEPTH(bvh->lds_stack, {
+    if (any)
+      hipLaunchKernelGGL((k_trace<W, S_, true>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
+                         bvh->root, d_rays, n, d_hits, d_occ);
+    else
+      hipLaunchKernelGGL((k_trace<W, S_, false>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
+                         bvh->root, d_rays, n, d_hits, d_occ);
+  })
   APT_HIP(hipGetLastError());
   return APT_OK;
 }
@@ -59,9 +62,9 @@ apt_status dispatch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_h
                     uint32_t* d_occ, bool any, hipStream_t s) {
   APT_REQUIRE(bvh && d_rays && (any ? d_occ != nullptr : d_hits != nullptr),
               APT_ERR_INVALID_ARG, "trace: null argument");
-  APT_REQUIRE(bvh->stats.max_stack <= TRACE_LDS_STACK + TRACE_OVF_STACK, APT_ERR_UNSUPPORTED,
+  APT_REQUIRE(bvh->stats.max_stack <= 8 + TRACE_OVF_STACK, APT_ERR_UNSUPPORTED,
               "trace: BVH needs a %u-entry stack, kernel holds %d", bvh->stats.max_stack,
-              TRACE_LDS_STACK + TRACE_OVF_STACK);
+              8 + TRACE_OVF_STACK);
   switch (bvh->width) {
     case 2: return launch<2>(bvh, d_rays, n, d_hits, d_occ, any, s);
     case 4: return launch<4>(bvh, d_rays, n, d_hits, d_occ, any, s);

@@ -6,9 +6,11 @@
 
 namespace apt {
 
-// Stack entries kept in LDS per lane, and in the private overflow array.
-constexpr int TRACE_LDS_STACK = 24;
+// Traversal stack: the bottom S entries of each lane live in LDS (S is a
+// template parameter, tuned per scene), deeper entries in a private overflow
+// array.  An entry is (node ref, entry distance of its box).
 constexpr int TRACE_OVF_STACK = 64;
+constexpr int TRACE_MAX_LDS_STACK = 32;
 
 // Work-group -> tile mapping that gives each of the 8 XCDs of an MI355X one
 // contiguous range of work-groups.  The dispatcher deals work-groups to XCDs
@@ -40,8 +42,26 @@ struct HitRec {
 
 // Per-lane view of the LDS stack: entry e of this lane lives at base[e*stride].
 struct LdsStack {
-  uint32_t* base;
+  uint2* base;
   int stride;
+};
+
+template <int S>
+struct TraversalStack {
+  LdsStack lds;
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  __device__ __forceinline__ void push(uint32_t ref, float t) {
+    const uint2 e = make_uint2(ref, f2u(t));
+    if (sp < S) lds.base[sp * lds.stride] = e;
+    else if (sp < S + TRACE_OVF_STACK) ovf[sp - S] = e;
+    else return;  // unreachable: the host checks the BVH's worst-case depth
+    ++sp;
+  }
+  __device__ __forceinline__ uint2 pop() {
+    --sp;
+    return sp < S ? lds.base[sp * lds.stride] : ovf[sp - S];
+  }
 };
 
 template <int W>
@@ -112,29 +132,21 @@ __device__ __forceinline__ bool leaf_tri(const float4* __restrict__ tris, uint32
 // Closest hit: the lexicographic minimum of (t, prim) over all triangles that
 // pass the watertight test with tmin < t < tmax.  A node is entered while its
 // padded box starts no later than the best t so far, so equal-t candidates are
-// still visited and the tie goes to the lower prim.
-template <int W>
+// still visited and the tie goes to the lower prim.  The loop runs inner nodes
+// until the lane reaches a leaf, then the leaf, so lanes of a wave tend to do
+// the same kind of work together; popped entries whose box starts beyond the
+// best t are dropped without being fetched.
+template <int W, int S>
 __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
                                                 const float4* __restrict__ tris, uint32_t root,
-                                                const TraceRay& ray, LdsStack st) {
-  uint32_t ovf[TRACE_OVF_STACK];
-  int sp = 0;
+                                                const TraceRay& ray, LdsStack lds) {
+  TraversalStack<S> st;
+  st.lds = lds;
   HitRec best{ray.pre.tmax, 0.0f, 0.0f, INVALID};
   uint32_t ref = root;
   while (true) {
-    if (ref_is_leaf(ref)) {
-      const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
-      for (uint32_t j = first; j < first + cnt; ++j) {
-        TriHit h;
-        uint32_t prim;
-        if (leaf_tri(tris, j, ray, h, prim) && hit_less(h.t, prim, best.t, best.prim)) {
-          best.t = h.t;
-          best.u = h.u;
-          best.v = h.v;
-          best.prim = prim;
-        }
-      }
-    } else {
+    bool have = true;
+    while (!ref_is_leaf(ref)) {
       WNode<W> nd;
       load_node<W>(nodes, ref, nd);
       float dist[W];
@@ -145,42 +157,60 @@ __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nod
       sort_children<W>(dist, refs);
       if (dist[0] != INFINITY) {
 #pragma unroll
-        for (int k = W - 1; k >= 1; --k) {
-          if (dist[k] != INFINITY && sp < TRACE_LDS_STACK + TRACE_OVF_STACK) {
-            if (sp < TRACE_LDS_STACK) st.base[sp * st.stride] = refs[k];
-            else ovf[sp - TRACE_LDS_STACK] = refs[k];
-            ++sp;
-          }
-        }
+        for (int k = W - 1; k >= 1; --k)
+          if (dist[k] != INFINITY) st.push(refs[k], dist[k]);
         ref = refs[0];
         continue;
       }
+      have = false;
+      while (st.sp > 0) {
+        const uint2 e = st.pop();
+        if (u2f(e.y) <= best.t) {
+          ref = e.x;
+          have = true;
+          break;
+        }
+      }
+      if (!have) break;
     }
-    if (sp == 0) break;
-    --sp;
-    ref = sp < TRACE_LDS_STACK ? st.base[sp * st.stride] : ovf[sp - TRACE_LDS_STACK];
+    if (!have) break;
+    const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
+    for (uint32_t j = first; j < first + cnt; ++j) {
+      TriHit h;
+      uint32_t prim;
+      if (leaf_tri(tris, j, ray, h, prim) && hit_less(h.t, prim, best.t, best.prim)) {
+        best.t = h.t;
+        best.u = h.u;
+        best.v = h.v;
+        best.prim = prim;
+      }
+    }
+    have = false;
+    while (st.sp > 0) {
+      const uint2 e = st.pop();
+      if (u2f(e.y) <= best.t) {
+        ref = e.x;
+        have = true;
+        break;
+      }
+    }
+    if (!have) break;
   }
   return best;
 }
 
 // Any hit: true as soon as one triangle passes with tmin < t < tmax
 // (the reference's terminate-on-first-hit shadow query, helpers/trace.wgsl:20-26).
-template <int W>
+template <int W, int S>
 __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
                                           const float4* __restrict__ tris, uint32_t root,
-                                          const TraceRay& ray, LdsStack st) {
-  uint32_t ovf[TRACE_OVF_STACK];
-  int sp = 0;
+                                          const TraceRay& ray, LdsStack lds) {
+  TraversalStack<S> st;
+  st.lds = lds;
   uint32_t ref = root;
   while (true) {
-    if (ref_is_leaf(ref)) {
-      const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
-      for (uint32_t j = first; j < first + cnt; ++j) {
-        TriHit h;
-        uint32_t prim;
-        if (leaf_tri(tris, j, ray, h, prim)) return true;
-      }
-    } else {
+    bool have = true;
+    while (!ref_is_leaf(ref)) {
       WNode<W> nd;
       load_node<W>(nodes, ref, nd);
       float dist[W];
@@ -189,24 +219,38 @@ __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         if (dist[k] == INFINITY) continue;
-        if (next == REF_EMPTY) {
-          next = nd.child[k];
-        } else if (sp < TRACE_LDS_STACK + TRACE_OVF_STACK) {
-          if (sp < TRACE_LDS_STACK) st.base[sp * st.stride] = nd.child[k];
-          else ovf[sp - TRACE_LDS_STACK] = nd.child[k];
-          ++sp;
-        }
+        if (next == REF_EMPTY) next = nd.child[k];
+        else st.push(nd.child[k], dist[k]);
       }
       if (next != REF_EMPTY) {
         ref = next;
         continue;
       }
+      if (st.sp == 0) {
+        have = false;
+        break;
+      }
+      ref = st.pop().x;
     }
-    if (sp == 0) break;
-    --sp;
-    ref = sp < TRACE_LDS_STACK ? st.base[sp * st.stride] : ovf[sp - TRACE_LDS_STACK];
+    if (!have) break;
+    const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
+    for (uint32_t j = first; j < first + cnt; ++j) {
+      TriHit h;
+      uint32_t prim;
+      if (leaf_tri(tris, j, ray, h, prim)) return true;
+    }
+    if (st.sp == 0) break;
+    ref = st.pop().x;
   }
   return false;
 }
+
+// Host-side dispatch over the supported LDS stack depths.
+#define APT_FOR_STACK_DEPTH(S, ...) \
+  switch (S) {                      \
+    case 8: { constexpr int S_ = 8; __VA_ARGS__; } break;   \
+    case 16: { constexpr int S_ = 16; __VA_ARGS__; } break; \
+    default: { constexpr int S_ = 24; __VA_ARGS__; } break; \
+  }
 
 }  // namespace apt

@@ -1,0 +1,147 @@
+"""Python view of libapt's renderer, mirroring the reference's interface.
+
+  PathTracerGpu        crates/appearance-path-tracer-gpu/src/lib.rs:250-615
+                       (new, handle_visible_world_action, render)
+  NodeRenderer         crates/appearance-render-loop/src/node.rs:14-25
+  DistributedRenderer  crates/appearance-distributed-renderer/src/lib.rs:11-82
+
+render() hands the RGBA8 64x64-block-layout pixels to result_callback, as the
+reference does, and returns nothing; render_accum() additionally returns the
+FP32 linear radiance mean used for parity checks.  Every call goes through the
+C ABI; there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import world
+from ._ffi import RESULT_CB, Config, MeshDesc, RenderStats, Sky, check, lib, vptr
+
+
+def default_config(**overrides) -> Config:
+    c = Config()
+    lib().apt_config_default(C.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise AttributeError(f"unknown config field {k}")
+        setattr(c, k, v)
+    return c
+
+
+class PathTracerGpu:
+    def __init__(self, config: Config | None = None, **overrides):
+        cfg = config if config is not None else default_config(**overrides)
+        h = C.c_void_p()
+        check(lib().apt_create(C.byref(cfg), C.byref(h)), "apt_create")
+        self._h = h
+        self.config = cfg
+        self._keep = []
+
+    def set_config(self, **overrides) -> None:
+        for k, v in overrides.items():
+            setattr(self.config, k, v)
+        check(lib().apt_set_config(self._h, C.byref(self.config)), "apt_set_config")
+
+    def handle_visible_world_action(self, action) -> None:
+        data = action.to_bytes()
+        buf = C.create_string_buffer(data, len(data))
+        check(lib().apt_world_action(self._h, action.ty, buf, len(data)), "apt_world_action")
+
+    def set_scene(self, scene, sky=None) -> None:
+        """Replace the scene by a SceneData (appearance_amd.scene) or arrays."""
+        pos = np.ascontiguousarray(scene.positions, np.float32)
+        nrm = None if scene.normals is None else np.ascontiguousarray(scene.normals, np.float32)
+        mids = np.ascontiguousarray(scene.material_ids, np.uint32)
+        from ._ffi import Material
+        mats = (Material * len(scene.materials))(*scene.materials)
+        md = MeshDesc()
+        md.positions = pos.ctypes.data_as(C.POINTER(C.c_float))
+        md.normals = nrm.ctypes.data_as(C.POINTER(C.c_float)) if nrm is not None else None
+        md.material_ids = mids.ctypes.data_as(C.POINTER(C.c_uint32))
+        md.n_tris = pos.shape[0]
+        md.n_materials = len(scene.materials)
+        md.materials = mats
+        sk = sky if sky is not None else getattr(scene, "sky", None)
+        check(lib().apt_scene_set(self._h, C.byref(md), C.byref(sk) if sk is not None else None),
+              "apt_scene_set")
+
+    def set_camera(self, camera) -> None:
+        """Apply a CameraUpdate (world.CameraUpdate or the ABI Camera struct)."""
+        if isinstance(camera, world.CameraUpdate):
+            self.handle_visible_world_action(camera)
+            return
+        data = bytes(camera)
+        buf = C.create_string_buffer(data, len(data))
+        check(lib().apt_world_action(self._h, world.CAMERA_UPDATE, buf, len(data)), "apt_world_action")
+
+    def render(self, resolution, start_row: int, end_row: int, result_callback) -> None:
+        w, h = int(resolution[0]), int(resolution[1])
+
+        def _cb(ptr, n, _user):
+            result_callback(C.string_at(ptr, n))
+
+        cb = RESULT_CB(_cb)
+        check(lib().apt_render_cb(self._h, w, h, start_row, end_row, cb, None), "apt_render_cb")
+
+    def render_accum(self, resolution, start_row: int = 0, end_row: int | None = None):
+        """Render; return (rgba8 block layout bytes, FP32 mean radiance (rows, w, 3))."""
+        w, h = int(resolution[0]), int(resolution[1])
+        end_row = h if end_row is None else end_row
+        rows = end_row - start_row
+        rgba = np.empty(w * rows * 4, np.uint8)
+        acc = np.empty((rows, w, 3), np.float32)
+        check(lib().apt_render(self._h, w, h, start_row, end_row, vptr(rgba), rgba.size, vptr(acc),
+                               acc.size), "apt_render")
+        return rgba, acc
+
+    def stats(self) -> dict:
+        s = RenderStats()
+        check(lib().apt_get_render_stats(self._h, C.byref(s)), "apt_get_render_stats")
+        return {f: getattr(s, f) for f, _ in RenderStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().apt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NodeRenderer:
+    """The trait a render node drives (node.rs:14-25)."""
+
+    def visible_world_action(self, action) -> None:
+        raise NotImplementedError
+
+    def render(self, resolution, start_row: int, end_row: int, result_callback) -> None:
+        raise NotImplementedError
+
+
+class DistributedRenderer(NodeRenderer):
+    """NodeRenderer backed by libapt (distributed-renderer/src/lib.rs:60-82)."""
+
+    def __init__(self, config: Config | None = None, **overrides):
+        self.path_tracer = PathTracerGpu(config, **overrides)
+
+    def visible_world_action(self, action) -> None:
+        self.path_tracer.handle_visible_world_action(action)
+
+    def render(self, resolution, start_row: int, end_row: int, result_callback) -> None:
+        self.path_tracer.render(resolution, start_row, end_row, result_callback)
+
+
+def block_to_linear(pixels: bytes | np.ndarray, width: int, rows: int) -> np.ndarray:
+    """Undo the 64x64 block layout (crates/appearance-render-loop/assets/shaders/block.wgsl)."""
+    px = np.frombuffer(bytes(pixels), np.uint8).reshape(-1, 4)
+    B = 64
+    y, x = np.mgrid[0:rows, 0:width]
+    bid = (y // B) * ((width + B - 1) // B) + x // B
+    src = bid * B * B + (y % B) * B + (x % B)
+    return px[src.ravel()].reshape(rows, width, 4)

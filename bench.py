@@ -6,7 +6,12 @@ Default workload (bvh_stress): 1,048,576 random triangles (seed 1) and
 over all rays (libapt's apt_trace_closest) with rays and hits resident in
 HBM.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+--workload disney: the Disney-BSDF scene (256 UV spheres, 524,290 triangles)
+at 1920x1080, 4 spp, depth 8; one step is one apt_render call (one frame);
+value is Msamples/s, Mrays/s is reported beside it.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bvh_stress|disney]
+                  [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -66,12 +71,17 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
+    ap.add_argument("--lds-stack", type=int, default=16)
+    ap.add_argument("--workload", default="bvh_stress", choices=["bvh_stress", "disney"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(tiles/rays shard per GPU with no exchange; see DESIGN.md)", file=sys.stderr)
         return 2
+
+    if args.workload == "disney":
+        return bench_disney(args)
 
     import numpy as np
     import torch
@@ -86,12 +96,16 @@ def main() -> int:
     rays = rays_bvh_stress(seed=2, side=1024, n_incoherent=1 << 20)
     n = rays.shape[0]
     log(f"building BVH (width {args.width})")
-    bvh = Bvh(scene.positions, width=args.width)
+    bvh = Bvh(scene.positions, width=args.width, lds_stack=args.lds_stack)
     st = bvh.stats()
     d_rays = torch.from_numpy(rays.view(np.uint8).copy()).cuda()
     d_hits = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
     d_occ = torch.empty(n, dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
+    # An explicit stream (default priority): the kernels and the timing events
+    # must share it, and torch's default stream has the null handle, which
+    # libapt would replace by its own stream.
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
 
     for _ in range(args.warmup):
         bvh.trace_closest_device(d_rays, n, d_hits, stream)
@@ -143,6 +157,7 @@ def main() -> int:
         "config": {"workload": "bvh_stress: 1,048,576 triangles, 2,097,152 rays "
                                "(1M coherent 1024x1024 primary + 1M incoherent), closest hit",
                    "n_tris": scene.n_tris, "n_rays": n, "bvh_width": args.width,
+                   "lds_stack": args.lds_stack,
                    "bvh_nodes": st["n_nodes"], "bvh_build_ms": round(st["build_ms"], 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -153,6 +168,77 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle BVH2 on host cores)")
         out["cpu_baseline"] = cpu_baseline_trace(scene.positions, rays, bpr)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline_render(scene, width, height, spp, bounces, rr):
+    """The oracle path tracer on the host cores, on a bounded pixel sample."""
+    import numpy as np
+
+    from tests._oracle import load_oracle, oracle_render
+
+    o = load_oracle()
+    threads = max(1, os.cpu_count() or 1)
+    b = o.bvh(scene.positions)
+    pix = np.random.default_rng(0).choice(width * height, 16384, replace=False).astype(np.uint32)
+    oracle_render(o, scene, width, height, spp=spp, max_bounces=bounces, rr_start=rr,
+                  pixels=pix[:256], bvh=b, threads=threads)
+    t0 = time.time()
+    oracle_render(o, scene, width, height, spp=spp, max_bounces=bounces, rr_start=rr, pixels=pix,
+                  bvh=b, threads=threads)
+    t = time.time() - t0
+    return {"value": pix.shape[0] * spp / t / 1e6, "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{pix.shape[0]} random pixels of the {width}x{height} frame x {spp} spp, "
+                      f"FP32 oracle path tracer, same scene/camera/seeds"}
+
+
+def bench_disney(args) -> int:
+    import torch
+
+    from appearance_amd.renderer import PathTracerGpu, default_config
+    from appearance_amd.scene import generate_scene
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    W, H, SPP, DEPTH, RR = 1920, 1088, 4, 8, 3  # 1088: slab heights are multiples of 64
+    log("generating disney scene")
+    scene = generate_scene("disney", seed=3)
+    pt = PathTracerGpu(default_config(sample_count=SPP, max_bounces=DEPTH, rr_start_bounce=RR,
+                                      bvh_width=args.width, lds_stack_depth=args.lds_stack))
+    pt.set_scene(scene)
+    pt.set_camera(scene.camera)
+    for _ in range(args.warmup):
+        pt.render_accum((W, H))
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} frames")
+    dev_ms, rays = [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pt.render_accum((W, H))
+        st = pt.stats()
+        dev_ms.append(st["ms_total"])
+        rays += st["extension_rays"] + st["shadow_rays"]
+    wall = time.perf_counter() - t0
+    samples = W * H * SPP * args.steps
+    out = {
+        "metric": METRIC,
+        "value": samples / wall / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"disney: 256 UV spheres + ground (524,290 triangles), "
+                               f"{W}x{H}, {SPP} spp, depth {DEPTH}, RR from {RR}",
+                   "bvh_width": args.width, "lds_stack": args.lds_stack},
+        "mrays_s": rays / wall / 1e6,
+        "device_ms_per_frame": sum(dev_ms) / len(dev_ms),
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle path tracer on host cores)")
+        out["cpu_baseline"] = cpu_baseline_render(scene, W, H, SPP, DEPTH, RR)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -95,6 +95,7 @@ typedef struct apt_bvh_opts {
   uint32_t max_leaf;  /* triangles per leaf, 1..8 (0 = 4) */
   uint32_t bins;      /* SAH bins per axis, 4..64 (0 = 32) */
   uint32_t builder;   /* apt_builder */
+  uint32_t lds_stack; /* traversal stack entries kept in LDS: 8, 16 or 24 (0 = 16) */
 } apt_bvh_opts;
 
 typedef struct apt_bvh_stats {
@@ -275,7 +276,7 @@ typedef struct apt_config {
   uint32_t svgf_history;
   uint32_t bvh_width;         /* 2, 4, 8 */
   uint32_t queue_chunk;       /* paths per wavefront queue chunk (0 = whole frame) */
-  uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS */
+  uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS: 8, 16, 24 */
   uint32_t seed;              /* base seed mixed into every path seed */
   uint32_t frame_idx;         /* starting frame index */
 } apt_config;

@@ -26,6 +26,14 @@ void or_brute(const float* tris, uint32_t ntris, const float* rays, uint32_t n, 
               float* out);
 uint32_t or_block_index(uint32_t x, uint32_t y, uint32_t width);
 float or_aces(float x);
+int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
+              uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg9,
+              const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
+              uint64_t* counts);
+double or_disney_eval(const void* mat, const double* n, const double* wo, const double* wi, int back,
+                      double* f);
+double or_disney_sample(const void* mat, const double* n, const double* wo, int back, double r0, double r1,
+                        double r2, double* wi, double* f);
 
 #ifdef __cplusplus
 }

@@ -395,6 +395,113 @@ done:
   std::memcpy(&hit[3], &bp, 4);
 }
 
+
+// ---- hooks for the integrator (oracle_render.cpp) --------------------------
+bool oracle_closest_f(const Bvh* b, const float* ray8, float* hit4) {
+  uint64_t nb = 0, nt = 0;
+  trace1(*b, ray8, false, hit4, nb, nt);
+  uint32_t p;
+  std::memcpy(&p, &hit4[3], 4);
+  return p != 0xFFFFFFFFu;
+}
+
+bool oracle_any_f(const Bvh* b, const float* ray8) {
+  uint64_t nb = 0, nt = 0;
+  float h[4];
+  trace1(*b, ray8, true, h, nb, nt);
+  return h[0] != 0.0f;
+}
+
+// FP64 watertight test (the same algorithm evaluated in double).
+static bool tri_test_d(const double* o, const int* ax, const double* S, const float* v, double tmin,
+                       double tmax, double& t, double& u, double& w) {
+  double X[3], Y[3], Z[3];
+  for (int j = 0; j < 3; ++j) {
+    double r[3] = {(double)v[3 * j] - o[0], (double)v[3 * j + 1] - o[1], (double)v[3 * j + 2] - o[2]};
+    X[j] = r[ax[0]] - S[0] * r[ax[2]];
+    Y[j] = r[ax[1]] - S[1] * r[ax[2]];
+    Z[j] = S[2] * r[ax[2]];
+  }
+  double e0 = X[2] * Y[1] - Y[2] * X[1], e1 = X[0] * Y[2] - Y[0] * X[2], e2 = X[1] * Y[0] - Y[1] * X[0];
+  if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return false;
+  double det = e0 + e1 + e2;
+  if (det == 0.0) return false;
+  double tt = (e0 * Z[0] + e1 * Z[1] + e2 * Z[2]) / det;
+  if (!(tt > tmin && tt < tmax)) return false;
+  t = tt;
+  u = e1 / det;
+  w = e2 / det;
+  return true;
+}
+
+static bool trace_d(const Bvh* b, const double* o, const double* d, double tmin, double tmax, bool any,
+                    double* ot, double* ou, double* ov, uint32_t* op) {
+  int kz = 0;
+  double a0 = std::fabs(d[0]), a1 = std::fabs(d[1]), a2 = std::fabs(d[2]);
+  if (a0 > a1) kz = a0 > a2 ? 0 : 2;
+  else kz = a1 > a2 ? 1 : 2;
+  int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+  if (d[kz] < 0.0) std::swap(kx, ky);
+  int ax[3] = {kx, ky, kz};
+  double S[3] = {d[kx] / d[kz], d[ky] / d[kz], 1.0 / d[kz]};
+  double inv[3];
+  for (int k = 0; k < 3; ++k) inv[k] = std::fabs(d[k]) < 1e-300 ? (d[k] < 0 ? -1e300 : 1e300) : 1.0 / d[k];
+  auto enter = [&](const BNode& nd, double cap) {
+    double tn = tmin, tf = cap;
+    for (int k = 0; k < 3; ++k) {
+      double p = ((double)nd.lo[k] - o[k]) * inv[k], q = ((double)nd.hi[k] - o[k]) * inv[k];
+      tn = std::max(tn, std::min(p, q));
+      tf = std::min(tf, std::max(p, q));
+    }
+    return tn <= tf;
+  };
+  double bt = tmax, bu = 0, bv = 0;
+  uint32_t bp = 0xFFFFFFFFu;
+  uint32_t stack[256];
+  int sp = 0;
+  if (!enter(b->nodes[0], bt)) return false;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const BNode& nd = b->nodes[stack[--sp]];
+    if (!enter(nd, bt)) continue;
+    if (nd.count) {
+      for (uint32_t i = nd.a; i < nd.a + nd.count; ++i) {
+        uint32_t p = b->idx[i];
+        double t, u, w;
+        if (tri_test_d(o, ax, S, &b->tris[9 * (size_t)p], tmin, tmax, t, u, w)) {
+          if (any) return true;
+          if (t < bt || (t == bt && p < bp)) {
+            bt = t;
+            bu = u;
+            bv = w;
+            bp = p;
+          }
+        }
+      }
+    } else if (sp + 2 <= 256) {
+      stack[sp++] = nd.a + 1;
+      stack[sp++] = nd.a;
+    }
+  }
+  if (bp == 0xFFFFFFFFu) return false;
+  *ot = bt;
+  *ou = bu;
+  *ov = bv;
+  *op = bp;
+  return true;
+}
+
+bool oracle_closest_d(const Bvh* b, const double* o, const double* d, double tmin, double tmax,
+                      double* t, double* u, double* v, uint32_t* prim) {
+  return trace_d(b, o, d, tmin, tmax, false, t, u, v, prim);
+}
+
+bool oracle_any_d(const Bvh* b, const double* o, const double* d, double tmin, double tmax) {
+  double t, u, v;
+  uint32_t p;
+  return trace_d(b, o, d, tmin, tmax, true, &t, &u, &v, &p);
+}
+
 }  // namespace orc
 
 using namespace orc;

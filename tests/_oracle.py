@@ -128,3 +128,40 @@ def load_oracle() -> Oracle:
         import sys
         subprocess.run([sys.executable, str(ROOT / "oracle" / "build.py")], check=True)
     return Oracle(C.CDLL(str(LIB)))
+
+
+def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounces: int,
+                  rr_start: int, seed: int = 0, frame_idx: int = 0, row_start: int = 0,
+                  rows: int | None = None, pixels=None, precision: int = 32, threads: int = 0,
+                  bvh=None, camera=None, sky=None):
+    """Run the oracle path tracer on a SceneData; returns (mean radiance, counts)."""
+    import ctypes as C
+    from appearance_amd._ffi import Material
+    rows = height - row_start if rows is None else rows
+    pos = np.ascontiguousarray(scene.positions, np.float32)
+    nrm = None if scene.normals is None else np.ascontiguousarray(scene.normals, np.float32)
+    mids = np.ascontiguousarray(scene.material_ids, np.uint32)
+    mats = (Material * len(scene.materials))(*scene.materials)
+    b = bvh if bvh is not None else o.bvh(pos)
+    cam = camera if camera is not None else scene.camera
+    sk = sky if sky is not None else scene.sky
+    cfg = np.array([width, height, row_start, rows, spp, max_bounces, rr_start, seed, frame_idx],
+                   np.uint32)
+    if pixels is not None:
+        pixels = np.ascontiguousarray(pixels, np.uint32)
+        out = np.empty((pixels.shape[0], 3), np.float32)
+    else:
+        out = np.empty((rows, width, 3), np.float32)
+    counts = np.zeros(2, np.uint64)
+    fn = o.lib.or_render
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p] * 6 + [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
+                   C.c_void_p, C.c_void_p]
+    fn(_vp(pos), _vp(nrm) if nrm is not None else None, _vp(mids), C.cast(mats, C.c_void_p),
+       pos.shape[0], b.h, C.cast(C.pointer(sk), C.c_void_p), C.cast(C.pointer(cam), C.c_void_p),
+       _vp(cfg), _vp(pixels) if pixels is not None else None,
+       0 if pixels is None else pixels.shape[0], precision, threads, _vp(out), _vp(counts))
+    return out, (int(counts[0]), int(counts[1]))
