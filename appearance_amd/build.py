@@ -21,7 +21,11 @@ SOURCES = [
     "scenes.cpp",
     "trace.hip",
     "renderer.cpp",
+    "scene_gpu.cpp",
     "integrator.hip",
+    "server.cpp",
+    "server.hip",
+    "tuner.cpp",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

@@ -73,7 +73,7 @@ static apt_status resolve_opts(const apt_bvh_opts* opts, apt_bvh_opts& o) {
   if (o.width == 0) o.width = 4;
   if (o.max_leaf == 0) o.max_leaf = 4;
   if (o.bins == 0) o.bins = 32;
-  if (o.lds_stack == 0) o.lds_stack = 16;
+  if (o.lds_stack == 0) o.lds_stack = 8;
   APT_REQUIRE(o.lds_stack == 8 || o.lds_stack == 16 || o.lds_stack == 24, APT_ERR_INVALID_ARG,
               "bvh: lds_stack %u is not 8, 16 or 24", o.lds_stack);
   APT_REQUIRE(o.width == 2 || o.width == 4 || o.width == 8, APT_ERR_INVALID_ARG,

@@ -15,6 +15,7 @@
 
 #include "apt_internal.h"
 #include "integrator.h"
+#include "scene_gpu.h"
 
 namespace {
 
@@ -87,11 +88,7 @@ struct apt_renderer {
   bool scene_dirty = true, explicit_scene = false, has_normals = false;
   uint64_t scene_version = 1, accum_scene_version = 0;
   // device scene
-  apt_bvh* bvh = nullptr;
-  DevBuf<float4> d_pos, d_nrm;
-  DevBuf<uint32_t> d_mat, d_emissive;
-  DevBuf<apt_material> d_mats;
-  uint32_t n_emissive = 0;
+  SceneGpu gpu;
   // frame
   uint32_t fw = 0, fh = 0, frow0 = 0, frows = 0;
   DevBuf<float4> d_frame_sum, d_accum_tm, d_accum_lin;
@@ -110,8 +107,7 @@ struct apt_renderer {
 
   ~apt_renderer() {
     for (Model& m : models) apt_scene_data_destroy(m.data);
-    if (bvh) apt_bvh_destroy(bvh);
-    d_pos.release(); d_nrm.release(); d_mat.release(); d_emissive.release(); d_mats.release();
+    scene_gpu_free(gpu);
     d_frame_sum.release(); d_accum_tm.release(); d_accum_lin.release(); d_rgba.release();
     p_o.release(); p_d.release(); p_t.release(); p_l.release(); p_h.release(); p_so.release();
     p_sd.release(); p_sc.release(); p_rng.release(); p_q0.release(); p_q1.release();
@@ -186,51 +182,13 @@ apt_status upload_scene(apt_renderer* r) {
     r->has_normals = all_normals;
   }
   const uint32_t n = (uint32_t)r->mat_ids.size();
-  if (r->bvh) {
-    apt_bvh_destroy(r->bvh);
-    r->bvh = nullptr;
+  scene_gpu_free(r->gpu);
+  if (n > 0) {
+    apt_status st = scene_gpu_upload(r->pos.data(), r->has_normals ? r->nrm.data() : nullptr,
+                                     r->mat_ids.data(), n, r->mats.data(), (uint32_t)r->mats.size(),
+                                     &r->sky, r->cfg.bvh_width, r->cfg.lds_stack_depth, r->gpu);
+    if (st != APT_OK) return st;
   }
-  r->n_emissive = 0;
-  if (n == 0) {
-    r->scene_dirty = false;
-    r->scene_version++;
-    return APT_OK;
-  }
-  apt_bvh_opts o{r->cfg.bvh_width, 4, 32, APT_BUILDER_HOST, r->cfg.lds_stack_depth};
-  apt_status st = apt_bvh_build(r->pos.data(), n, &o, &r->bvh);
-  if (st != APT_OK) return st;
-  std::vector<float4> p4((size_t)n * 3), n4;
-  for (size_t i = 0; i < (size_t)n * 3; ++i)
-    p4[i] = make_float4(r->pos[3 * i], r->pos[3 * i + 1], r->pos[3 * i + 2], 0.0f);
-  if ((st = r->d_pos.ensure(p4.size())) != APT_OK) return st;
-  APT_HIP(hipMemcpy(r->d_pos.p, p4.data(), p4.size() * sizeof(float4), hipMemcpyHostToDevice));
-  if (r->has_normals) {
-    n4.resize((size_t)n * 3);
-    for (size_t i = 0; i < (size_t)n * 3; ++i)
-      n4[i] = make_float4(r->nrm[3 * i], r->nrm[3 * i + 1], r->nrm[3 * i + 2], 0.0f);
-    if ((st = r->d_nrm.ensure(n4.size())) != APT_OK) return st;
-    APT_HIP(hipMemcpy(r->d_nrm.p, n4.data(), n4.size() * sizeof(float4), hipMemcpyHostToDevice));
-  }
-  for (uint32_t m : r->mat_ids)
-    APT_REQUIRE(m < r->mats.size(), APT_ERR_INVALID_ARG, "scene: material id %u >= %zu", m, r->mats.size());
-  if ((st = r->d_mat.ensure(n)) != APT_OK) return st;
-  APT_HIP(hipMemcpy(r->d_mat.p, r->mat_ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  if ((st = r->d_mats.ensure(r->mats.size())) != APT_OK) return st;
-  APT_HIP(hipMemcpy(r->d_mats.p, r->mats.data(), r->mats.size() * sizeof(apt_material), hipMemcpyHostToDevice));
-  std::vector<uint32_t> em;
-  for (uint32_t t = 0; t < n; ++t) {
-    const apt_material& m = r->mats[r->mat_ids[t]];
-    if (m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f) {
-      const float* p = &r->pos[9 * (size_t)t];
-      float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
-      float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-      if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 0.0f) em.push_back(t);  // degenerate emitters emit nothing
-    }
-  }
-  r->n_emissive = (uint32_t)em.size();
-  if ((st = r->d_emissive.ensure(em.size())) != APT_OK) return st;
-  if (!em.empty())
-    APT_HIP(hipMemcpy(r->d_emissive.p, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   r->scene_dirty = false;
   r->scene_version++;
   return APT_OK;
@@ -248,43 +206,15 @@ apt_status ensure_paths(apt_renderer* r, uint32_t n) {
   return APT_OK;
 }
 
-SceneView scene_view(const apt_renderer* r, uint32_t width, uint32_t height) {
-  SceneView sc{};
-  sc.nodes = r->bvh ? r->bvh->d_nodes : nullptr;
-  sc.bvh_tris = r->bvh ? r->bvh->d_tris : nullptr;
-  sc.root = r->bvh ? r->bvh->root : 0;
-  sc.width = r->bvh ? r->bvh->width : r->cfg.bvh_width;
-  sc.lds_stack = r->bvh ? r->bvh->lds_stack : 16;
-  sc.tri_pos = r->d_pos.p;
-  sc.tri_nrm = r->has_normals ? r->d_nrm.p : nullptr;
-  sc.tri_mat = r->d_mat.p;
-  sc.mats = r->d_mats.p;
-  sc.emissive = r->d_emissive.p;
-  sc.n_emissive = r->n_emissive;
-  sc.sky = r->sky;
-  // CameraUpdate: rotation and translation of the world transform (scale
-  // dropped, as to_scale_rotation_translation does in lib.rs:289-296), a
-  // vertical-fov pinhole with aspect = full width / height (lib.rs:333-334).
-  const float* m = r->cam.transform;
-  v3 c0 = normalize(mk(m[0], m[1], m[2])), c1 = normalize(mk(m[4], m[5], m[6]));
-  v3 c2 = normalize(mk(m[8], m[9], m[10]));
-  sc.cam_right = c0;
-  sc.cam_up = c1;
-  sc.cam_back = c2;
-  sc.cam_pos = mk(m[12], m[13], m[14]);
-  sc.cam_tan_half = std::tan(0.5f * r->cam.fov_deg * PI / 180.0f);
-  sc.cam_aspect = (float)width / (float)height;
-  return sc;
-}
-
 apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
                        uint32_t row_end, uint8_t* out_rgba, size_t out_len, float* out_accum,
                        size_t accum_len) {
   APT_REQUIRE(width > 0 && height > 0 && row_start < row_end && row_end <= height,
               APT_ERR_INVALID_ARG, "render: bad frame %ux%u rows [%u,%u)", width, height, row_start, row_end);
   const uint32_t rows = row_end - row_start;
-  APT_REQUIRE(width % 64 == 0 && rows % 64 == 0, APT_ERR_INVALID_ARG,
-              "render: width (%u) and slab height (%u) must be multiples of 64 (host.rs:476)", width, rows);
+  APT_REQUIRE(!out_rgba || (width % 64 == 0 && rows % 64 == 0), APT_ERR_INVALID_ARG,
+              "render: block-layout output needs width (%u) and slab height (%u) to be multiples "
+              "of 64 (host.rs:476)", width, rows);
   const size_t npix = (size_t)width * rows;
   APT_REQUIRE(npix <= (1u << 26), APT_ERR_INVALID_ARG, "render: %zu pixels exceed 2^26", npix);
   APT_REQUIRE(!out_rgba || out_len >= npix * 4, APT_ERR_INVALID_ARG,
@@ -293,7 +223,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
               "render: out_accum_rgb holds %zu floats, needs %zu", accum_len, npix * 3);
   apt_status st = upload_scene(r);
   if (st != APT_OK) return st;
-  APT_REQUIRE(r->bvh != nullptr, APT_ERR_INVALID_ARG, "render: the scene is empty");
+  APT_REQUIRE(r->gpu.bvh != nullptr, APT_ERR_INVALID_ARG, "render: the scene is empty");
 
   hipStream_t s = r->stream;
   const bool resized = r->fw != width || r->fh != height || r->frow0 != row_start || r->frows != rows;
@@ -327,7 +257,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
 
   FrameView fv{width, height, row_start, rows, spp, r->frame_idx, r->cfg.seed, r->cfg.max_bounces,
                r->cfg.rr_start_bounce};
-  SceneView sc = scene_view(r, width, height);
+  SceneView sc = scene_gpu_view(r->gpu, r->cam, width, height);
   PathView ps{};
   ps.o_pdf = r->p_o.p; ps.d_flags = r->p_d.p; ps.thr = r->p_t.p; ps.rad = r->p_l.p;
   ps.hit = r->p_h.p; ps.sh_o = r->p_so.p; ps.sh_d = r->p_sd.p; ps.sh_c = r->p_sc.p;
@@ -341,7 +271,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
     if (st != APT_OK) return st;
   }
   st = resolve_frame(fv, r->d_frame_sum.p, r->d_accum_tm.p, r->d_accum_lin.p, r->accum_frames,
-                     r->d_rgba.p, s);
+                     out_rgba ? r->d_rgba.p : nullptr, s);
   if (st != APT_OK) return st;
   APT_HIP(hipEventRecord(r->ev1, s));
   uint32_t cnt[CNT_COUNT];
@@ -417,7 +347,7 @@ void apt_config_default(apt_config* c) {
   c->svgf_history = 4;
   c->bvh_width = 4;
   c->queue_chunk = 0;
-  c->lds_stack_depth = 16;
+  c->lds_stack_depth = 8;
   c->seed = 0;
   c->frame_idx = 0;
 }
@@ -459,7 +389,7 @@ apt_status apt_set_config(apt_renderer* r, const apt_config* cfg) {
   apt_status st = validate_config(*cfg);
   if (st != APT_OK) return st;
   if (cfg->bvh_width != r->cfg.bvh_width) r->scene_dirty = true;
-  if (r->bvh) r->bvh->lds_stack = cfg->lds_stack_depth;
+  if (r->gpu.bvh) r->gpu.bvh->lds_stack = cfg->lds_stack_depth;
   if (cfg->sample_count != r->cfg.sample_count || cfg->max_bounces != r->cfg.max_bounces ||
       cfg->rr_start_bounce != r->cfg.rr_start_bounce || cfg->seed != r->cfg.seed)
     r->cam_version++;  // restart accumulation
@@ -579,6 +509,14 @@ apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint3
                               r->cb_pixels.size(), nullptr, 0);
   if (st != APT_OK) return st;
   cb(r->cb_pixels.data(), r->cb_pixels.size(), user);
+  return APT_OK;
+}
+
+apt_status apt_set_frame_index(apt_renderer* r, uint32_t frame_idx) {
+  clear_error();
+  APT_REQUIRE(r, APT_ERR_INVALID_ARG, "apt_set_frame_index: renderer is NULL");
+  r->frame_idx = frame_idx;
+  r->cam_version++;  // a new sequence restarts accumulation
   return APT_OK;
 }
 

@@ -1,0 +1,194 @@
+// Serving-loop kernels: one wavefront batch over path slots drawn from the
+// tiles of many admitted requests (different scenes, cameras, resolutions,
+// sample counts and depths in the same launch).
+//
+// A batch is raygen -> (extend, shade, shadow) x max_bounces -> film, all
+// over the same slot pool, captured once into a hipGraph and replayed; the
+// per-batch inputs (tile list, path count) live in device memory so the
+// captured kernel arguments never change.  Film adds each finished path to its
+// request's accumulator with 64-bit fixed-point atomics: integer addition is
+// associative, so a request's sum does not depend on the order its paths
+// finish in or on what else shared the batch.
+#include "pathtrace_device.h"
+#include "server.h"
+
+namespace apt {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr float FIXED_SCALE = 16777216.0f;  // 2^24
+
+// Tile of the batch containing batch path p (tiles are sorted by offset).
+__device__ __forceinline__ uint32_t find_tile(const SrvTile* __restrict__ tiles, uint32_t n,
+                                              uint32_t p) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tiles[mid].offset <= p) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_srv_raygen(const SrvBatch* __restrict__ batch,
+                                                      const SrvReq* __restrict__ reqs, PathView ps,
+                                                      uint32_t* __restrict__ path_req,
+                                                      uint32_t* __restrict__ path_pix) {
+  const uint32_t n_paths = batch->n_paths;
+  const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ps.counters[CNT_QUEUE0] = n_paths;
+    ps.counters[CNT_QUEUE1] = 0;
+    ps.counters[CNT_SHADOW] = 0;
+  }
+  if (p >= n_paths) return;
+  const SrvTile& t = batch->tiles[find_tile(batch->tiles, batch->n_tiles, p)];
+  const SrvReq& r = reqs[t.req_slot];
+  const uint32_t idx = t.first + (p - t.offset);  // request path index, sample-major
+  const uint32_t npix = r.width * r.height;
+  const uint32_t sample = idx / npix, pixel = idx - sample * npix;
+  const uint32_t x = pixel % r.width, y = pixel / r.width;
+  start_path(r.sc, ps, p, x, y, r.width, r.height, sample + r.seed);
+  path_req[p] = t.req_slot;
+  path_pix[p] = pixel;
+  ps.queue[0][p] = p;
+}
+
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_srv_extend(const SrvReq* __restrict__ reqs, PathView ps,
+                                                      const uint32_t* __restrict__ path_req,
+                                                      int qin) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
+    ps.counters[CNT_SHADOW] = 0;
+    ps.counters[CNT_STAT_EXT] += count;
+  }
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t p = ps.queue[qin][i];
+  const SceneView& sc = reqs[path_req[p]].sc;
+  const float4 o = ps.o_pdf[p], d = ps.d_flags[p];
+  TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f);
+  HitRec h = trace_closest<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray,
+                                 LdsStack{s_stack + threadIdx.x, BLOCK});
+  ps.hit[p] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_srv_shade(const SrvReq* __restrict__ reqs, PathView ps,
+                                                     const uint32_t* __restrict__ path_req,
+                                                     int qin, uint32_t bounce) {
+  const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  bool alive = false, want_shadow = false;
+  uint32_t p = 0;
+  if (i < count) {
+    p = ps.queue[qin][i];
+    const SrvReq& r = reqs[path_req[p]];
+    shade_path(r.sc, ps, p, bounce, r.max_bounces, r.rr_start, alive, want_shadow);
+  }
+  const int slot = wave_append(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)]);
+  if (slot >= 0) ps.queue[qin ^ 1][slot] = p;
+  const int sslot = wave_append(want_shadow, &ps.counters[CNT_SHADOW]);
+  if (sslot >= 0) ps.shadow_queue[sslot] = p;
+}
+
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_srv_shadow(const SrvReq* __restrict__ reqs, PathView ps,
+                                                      const uint32_t* __restrict__ path_req) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  const uint32_t count = ps.counters[CNT_SHADOW];
+  if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[CNT_STAT_SHADOW] += count;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t p = ps.shadow_queue[i];
+  const SceneView& sc = reqs[path_req[p]].sc;
+  const float4 o = ps.sh_o[p], d = ps.sh_d[p];
+  TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, o.w);
+  if (!trace_any<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray,
+                       LdsStack{s_stack + threadIdx.x, BLOCK})) {
+    const float4 c = ps.sh_c[p];
+    float4 l = ps.rad[p];
+    ps.rad[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+  }
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  // radiance is >= 0; non-finite or absurd values are dropped rather than
+  // allowed to wrap the accumulator
+  if (!(v > 0.0f)) return 0ull;
+  const float s = v * FIXED_SCALE;
+  return s < 1.0e18f ? (unsigned long long)rintf(s) : 0ull;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_srv_film(const SrvBatch* __restrict__ batch,
+                                                    const SrvReq* __restrict__ reqs, PathView ps,
+                                                    const uint32_t* __restrict__ path_req,
+                                                    const uint32_t* __restrict__ path_pix) {
+  const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (p >= batch->n_paths) return;
+  const float4 l = ps.rad[p];
+  unsigned long long* a = reqs[path_req[p]].accum + 3 * (size_t)path_pix[p];
+  atomicAdd(a + 0, to_fixed(l.x));
+  atomicAdd(a + 1, to_fixed(l.y));
+  atomicAdd(a + 2, to_fixed(l.z));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_srv_resolve(const unsigned long long* __restrict__ accum,
+                                                       uint32_t npix, uint32_t spp,
+                                                       float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= npix * 3) return;
+  const double scale = 1.0 / ((double)FIXED_SCALE * (double)spp);
+  out[i] = (float)((double)accum[i] * scale);
+}
+
+__global__ void k_srv_zero(unsigned long long* __restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) a[i] = 0ull;
+}
+
+template <int W, int S>
+void enqueue_batch_t(const SrvDevice& d, hipStream_t s) {
+  const dim3 grid((d.path_slots + BLOCK - 1) / BLOCK), block(BLOCK);
+  hipLaunchKernelGGL(k_srv_raygen, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);
+  for (uint32_t b = 0; b < d.max_bounces; ++b) {
+    const int qin = (int)(b & 1u);
+    hipLaunchKernelGGL((k_srv_extend<W, S>), grid, block, 0, s, d.reqs, d.ps, d.path_req, qin);
+    hipLaunchKernelGGL(k_srv_shade, grid, block, 0, s, d.reqs, d.ps, d.path_req, qin, b);
+    hipLaunchKernelGGL((k_srv_shadow<W, S>), grid, block, 0, s, d.reqs, d.ps, d.path_req);
+  }
+  hipLaunchKernelGGL(k_srv_film, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);
+}
+
+}  // namespace
+
+apt_status srv_enqueue_batch(const SrvDevice& d, hipStream_t s) {
+  switch (d.width) {
+    case 2: APT_FOR_STACK_DEPTH(d.lds_stack, enqueue_batch_t<2, S_>(d, s)); break;
+    case 4: APT_FOR_STACK_DEPTH(d.lds_stack, enqueue_batch_t<4, S_>(d, s)); break;
+    case 8: APT_FOR_STACK_DEPTH(d.lds_stack, enqueue_batch_t<8, S_>(d, s)); break;
+    default: set_error("server: unsupported BVH width %u", d.width); return APT_ERR_INVALID_ARG;
+  }
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+apt_status srv_resolve(const unsigned long long* accum, uint32_t npix, uint32_t spp, float* out,
+                       hipStream_t s) {
+  const uint32_t n = npix * 3;
+  hipLaunchKernelGGL(k_srv_resolve, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, accum, npix,
+                     spp, out);
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+apt_status srv_zero(unsigned long long* a, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_srv_zero, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, a, n);
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+}  // namespace apt

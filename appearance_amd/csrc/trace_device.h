@@ -46,24 +46,6 @@ struct LdsStack {
   int stride;
 };
 
-template <int S>
-struct TraversalStack {
-  LdsStack lds;
-  uint2 ovf[TRACE_OVF_STACK];
-  int sp = 0;
-  __device__ __forceinline__ void push(uint32_t ref, float t) {
-    const uint2 e = make_uint2(ref, f2u(t));
-    if (sp < S) lds.base[sp * lds.stride] = e;
-    else if (sp < S + TRACE_OVF_STACK) ovf[sp - S] = e;
-    else return;  // unreachable: the host checks the BVH's worst-case depth
-    ++sp;
-  }
-  __device__ __forceinline__ uint2 pop() {
-    --sp;
-    return sp < S ? lds.base[sp * lds.stride] : ovf[sp - S];
-  }
-};
-
 template <int W>
 __device__ __forceinline__ void load_node(const WNode<W>* __restrict__ nodes, uint32_t idx,
                                           WNode<W>& nd) {
@@ -129,6 +111,33 @@ __device__ __forceinline__ bool leaf_tri(const float4* __restrict__ tris, uint32
   return intersect_tri(ray.pre, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h);
 }
 
+// Push/pop on a lane's stack: entries below S live in LDS at lds[e * stride],
+// deeper ones in the private array `ovf`.  sp, lds and ovf are separate
+// locals of the traversal loop so the stack pointer stays in a register and
+// the LDS accesses compile to ds_* instructions.
+#define APT_STACK_PUSH(ref_, t_)                                                 \
+  do {                                                                           \
+    const uint2 e_ = make_uint2((ref_), f2u(t_));                                \
+    if (sp < S) lds[sp * stride] = e_;                                           \
+    else if (sp < S + TRACE_OVF_STACK) ovf[sp - S] = e_;                         \
+    if (sp < S + TRACE_OVF_STACK) ++sp;                                          \
+  } while (0)
+// The empty asm keeps the two branches distinct so the compiler does not fuse
+// them into one generic (flat) load through a selected pointer.
+__device__ __forceinline__ uint2 stack_pop(int& sp, const uint2* lds, int stride, const uint2* ovf,
+                                           int S) {
+  --sp;
+  uint2 e;
+  if (sp < S) {
+    e = lds[sp * stride];
+  } else {
+    e = ovf[sp - S];
+    asm volatile("" : "+v"(e.x), "+v"(e.y));
+  }
+  return e;
+}
+#define APT_STACK_POP() stack_pop(sp, lds, stride, ovf, S)
+
 // Closest hit: the lexicographic minimum of (t, prim) over all triangles that
 // pass the watertight test with tmin < t < tmax.  A node is entered while its
 // padded box starts no later than the best t so far, so equal-t candidates are
@@ -139,13 +148,15 @@ __device__ __forceinline__ bool leaf_tri(const float4* __restrict__ tris, uint32
 template <int W, int S>
 __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
                                                 const float4* __restrict__ tris, uint32_t root,
-                                                const TraceRay& ray, LdsStack lds) {
-  TraversalStack<S> st;
-  st.lds = lds;
+                                                const TraceRay& ray, LdsStack st) {
+  uint2* const lds = st.base;
+  const int stride = st.stride;
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
   HitRec best{ray.pre.tmax, 0.0f, 0.0f, INVALID};
   uint32_t ref = root;
-  while (true) {
-    bool have = true;
+  bool have = true;
+  while (have) {
     while (!ref_is_leaf(ref)) {
       WNode<W> nd;
       load_node<W>(nodes, ref, nd);
@@ -158,13 +169,13 @@ __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nod
       if (dist[0] != INFINITY) {
 #pragma unroll
         for (int k = W - 1; k >= 1; --k)
-          if (dist[k] != INFINITY) st.push(refs[k], dist[k]);
+          if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
         ref = refs[0];
         continue;
       }
       have = false;
-      while (st.sp > 0) {
-        const uint2 e = st.pop();
+      while (sp > 0) {
+        const uint2 e = APT_STACK_POP();
         if (u2f(e.y) <= best.t) {
           ref = e.x;
           have = true;
@@ -186,15 +197,14 @@ __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nod
       }
     }
     have = false;
-    while (st.sp > 0) {
-      const uint2 e = st.pop();
+    while (sp > 0) {
+      const uint2 e = APT_STACK_POP();
       if (u2f(e.y) <= best.t) {
         ref = e.x;
         have = true;
         break;
       }
     }
-    if (!have) break;
   }
   return best;
 }
@@ -204,9 +214,11 @@ __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nod
 template <int W, int S>
 __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
                                           const float4* __restrict__ tris, uint32_t root,
-                                          const TraceRay& ray, LdsStack lds) {
-  TraversalStack<S> st;
-  st.lds = lds;
+                                          const TraceRay& ray, LdsStack st) {
+  uint2* const lds = st.base;
+  const int stride = st.stride;
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
   uint32_t ref = root;
   while (true) {
     bool have = true;
@@ -220,17 +232,17 @@ __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
       for (int k = 0; k < W; ++k) {
         if (dist[k] == INFINITY) continue;
         if (next == REF_EMPTY) next = nd.child[k];
-        else st.push(nd.child[k], dist[k]);
+        else APT_STACK_PUSH(nd.child[k], dist[k]);
       }
       if (next != REF_EMPTY) {
         ref = next;
         continue;
       }
-      if (st.sp == 0) {
+      if (sp == 0) {
         have = false;
         break;
       }
-      ref = st.pop().x;
+      ref = APT_STACK_POP().x;
     }
     if (!have) break;
     const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
@@ -239,11 +251,14 @@ __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
       uint32_t prim;
       if (leaf_tri(tris, j, ray, h, prim)) return true;
     }
-    if (st.sp == 0) break;
-    ref = st.pop().x;
+    if (sp == 0) break;
+    ref = APT_STACK_POP().x;
   }
   return false;
 }
+
+#undef APT_STACK_PUSH
+#undef APT_STACK_POP
 
 // Host-side dispatch over the supported LDS stack depths.
 #define APT_FOR_STACK_DEPTH(S, ...) \

@@ -25,7 +25,7 @@ class Bvh:
     """A wide BVH resident in device memory (libapt's apt_bvh)."""
 
     def __init__(self, tris: np.ndarray, width: int = 4, max_leaf: int = 4, bins: int = 32,
-                 builder: int = BUILDER_HOST, lds_stack: int = 16):
+                 builder: int = BUILDER_HOST, lds_stack: int = 8):
         tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
         self.n_tris = tris.shape[0]
         opts = BvhOpts(width, max_leaf, bins, builder, lds_stack)

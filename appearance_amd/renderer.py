@@ -86,15 +86,17 @@ class PathTracerGpu:
         cb = RESULT_CB(_cb)
         check(lib().apt_render_cb(self._h, w, h, start_row, end_row, cb, None), "apt_render_cb")
 
-    def render_accum(self, resolution, start_row: int = 0, end_row: int | None = None):
-        """Render; return (rgba8 block layout bytes, FP32 mean radiance (rows, w, 3))."""
+    def render_accum(self, resolution, start_row: int = 0, end_row: int | None = None,
+                     with_rgba: bool = True):
+        """Render; return (rgba8 block layout bytes or None, FP32 mean radiance (rows, w, 3))."""
         w, h = int(resolution[0]), int(resolution[1])
         end_row = h if end_row is None else end_row
         rows = end_row - start_row
-        rgba = np.empty(w * rows * 4, np.uint8)
+        rgba = np.empty(w * rows * 4, np.uint8) if with_rgba else None
         acc = np.empty((rows, w, 3), np.float32)
-        check(lib().apt_render(self._h, w, h, start_row, end_row, vptr(rgba), rgba.size, vptr(acc),
-                               acc.size), "apt_render")
+        check(lib().apt_render(self._h, w, h, start_row, end_row,
+                               vptr(rgba) if with_rgba else None, rgba.size if with_rgba else 0,
+                               vptr(acc), acc.size), "apt_render")
         return rgba, acc
 
     def stats(self) -> dict:

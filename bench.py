@@ -71,8 +71,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
-    ap.add_argument("--lds-stack", type=int, default=16)
-    ap.add_argument("--workload", default="bvh_stress", choices=["bvh_stress", "disney"])
+    ap.add_argument("--lds-stack", type=int, default=8)
+    ap.add_argument("--workload", default="bvh_stress", choices=["bvh_stress", "disney", "serving"])
+    ap.add_argument("--requests", type=int, default=96)
+    ap.add_argument("--rates", default="20,200")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -82,6 +84,8 @@ def main() -> int:
 
     if args.workload == "disney":
         return bench_disney(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import numpy as np
     import torch
@@ -239,6 +243,139 @@ def bench_disney(args) -> int:
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle path tracer on host cores)")
         out["cpu_baseline"] = cpu_baseline_render(scene, W, H, SPP, DEPTH, RR)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def make_requests(n, rate, seed=5):
+    """Requests drawn as in the serving configuration: resolution from
+    {256^2, 512^2, 1280x720, 1920x1080}, spp from {1, 4, 16, 64}, depth 4..8,
+    scene 0..3, a seeded orbit camera; Poisson arrivals at `rate` req/s."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        w, h = res[rng.integers(4)]
+        out.append({"t": t, "scene": int(rng.integers(4)), "w": w, "h": h,
+                    "spp": [1, 4, 16, 64][rng.integers(4)], "depth": int(rng.integers(4, 9)),
+                    "angle": float(rng.uniform(0, 2 * np.pi)), "seed": i})
+    return out
+
+
+def orbit_camera(scene, angle):
+    import numpy as np
+
+    from tests.scenes_util import camera
+
+    cam = scene.camera
+    m = np.array(cam.transform[:], np.float32).reshape(4, 4)  # column-major rows = columns
+    eye = m[3, :3].astype(np.float64)
+    back = m[2, :3].astype(np.float64)
+    dist = 5.0
+    target = eye - back * dist
+    off = eye - target
+    c, s_ = np.cos(angle * 0.25 - 0.125), np.sin(angle * 0.25 - 0.125)
+    rot = np.array([off[0] * c + off[2] * s_, off[1], -off[0] * s_ + off[2] * c])
+    return camera(tuple(target + rot), tuple(target), cam.fov_deg)
+
+
+def bench_serving(args) -> int:
+    import threading
+
+    import numpy as np
+    import torch
+
+    from appearance_amd.renderer import PathTracerGpu, default_config
+    from appearance_amd.scene import generate_scene
+    from appearance_amd.serving import DONE, Server
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    log("generating the four resident scenes")
+    names = ["cornell", "bvh_stress", "disney", "restir"]
+    scenes = [generate_scene(n, seed=[0, 1, 3, 3][i]) for i, n in enumerate(names)]
+    sky = scenes[2].sky
+    scenes[1].sky = sky  # the triangle soup has no emitters: light it with the sky
+    srv = Server(max_active=64, policy=0)
+    ids = [srv.add_scene(sc) for sc in scenes]
+    plain = []
+    for sc in scenes:
+        pt = PathTracerGpu(default_config())
+        pt.set_scene(sc)
+        plain.append(pt)
+    report = {"rates": {}}
+    total_samples = None
+    rates = [float(r) for r in args.rates.split(",")]
+    for rate in rates:
+        reqs = make_requests(args.requests, rate)
+        for r in reqs:
+            r["cam"] = orbit_camera(scenes[r["scene"]], r["angle"])
+        samples = sum(r["w"] * r["h"] * r["spp"] for r in reqs)
+        total_samples = samples
+        # served: a client thread submits at the arrival times
+        log(f"served stream: {len(reqs)} requests at {rate} req/s, {samples/1e6:.0f} Msamples")
+        rids = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def client():
+            for i, r in enumerate(reqs):
+                d = r["t"] - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                rids[i] = srv.submit(ids[r["scene"]], r["w"], r["h"], r["spp"], r["depth"], r["cam"],
+                                     seed=r["seed"])
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        lat = []
+        last = 0.0
+        for rid in rids:
+            res = srv.wait(rid)
+            assert res.state == DONE
+            lat.append(res.done_ms - res.submit_ms)
+            last = max(last, res.done_ms)
+        first = min(srv.poll(r).submit_ms for r in rids)
+        span_s = (last - first) / 1e3
+        served = {"msamples_s": samples / span_s / 1e6, "p50_ms": float(np.percentile(lat, 50)),
+                  "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span_s}
+        # one at a time: FIFO through the plain renderer
+        log("one-at-a-time baseline")
+        lat_b = []
+        t0 = time.perf_counter()
+        for r in reqs:
+            d = r["t"] - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            pt = plain[r["scene"]]
+            pt.set_config(sample_count=r["spp"], max_bounces=r["depth"], seed=r["seed"], frame_idx=0)
+            pt.set_camera(r["cam"])
+            pt.render_accum((r["w"], r["h"]), with_rgba=False)
+            lat_b.append((time.perf_counter() - t0 - r["t"]) * 1e3)
+        span_b = time.perf_counter() - t0 - reqs[0]["t"]
+        base = {"msamples_s": samples / span_b / 1e6, "p50_ms": float(np.percentile(lat_b, 50)),
+                "p99_ms": float(np.percentile(lat_b, 99)), "makespan_s": span_b}
+        report["rates"][str(rate)] = {"served": served, "one_at_a_time": base,
+                                      "throughput_ratio": served["msamples_s"] / base["msamples_s"]}
+        log(f"rate {rate}: served {served}, baseline {base}")
+    top = report["rates"][str(rates[-1])]
+    out = {
+        "metric": METRIC,
+        "value": top["served"]["msamples_s"],
+        "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.requests, "warmup": 0,
+        "ms_per_step": top["served"]["makespan_s"] * 1e3 / args.requests,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"serving: {args.requests} requests, Poisson arrivals, 4 resident "
+                               f"scenes, res in {{256^2,512^2,1280x720,1920x1080}}, spp in "
+                               f"{{1,4,16,64}}, depth 4..8", "total_samples": total_samples},
+        "serving": report,
+    }
     print(json.dumps(out), flush=True)
     return 0
 

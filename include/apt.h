@@ -95,7 +95,7 @@ typedef struct apt_bvh_opts {
   uint32_t max_leaf;  /* triangles per leaf, 1..8 (0 = 4) */
   uint32_t bins;      /* SAH bins per axis, 4..64 (0 = 32) */
   uint32_t builder;   /* apt_builder */
-  uint32_t lds_stack; /* traversal stack entries kept in LDS: 8, 16 or 24 (0 = 16) */
+  uint32_t lds_stack; /* traversal stack entries kept in LDS: 8, 16 or 24 (0 = 8) */
 } apt_bvh_opts;
 
 typedef struct apt_bvh_stats {
@@ -302,11 +302,11 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
 apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_sky* sky);
 
 /* PathTracerGpu::render: renders rows [row_start, row_end) of a width x height
- * frame (width and the slab height multiples of 64, host.rs:476).  Writes
- * RGBA8 in 64x64 block-major order (resolve.wgsl:56-57, block.wgsl:3-15) to
- * out_rgba (width*(row_end-row_start)*4 bytes, may be NULL) and, when
+ * frame.  Writes RGBA8 in 64x64 block-major order (resolve.wgsl:56-57,
+ * block.wgsl:3-15) to out_rgba (width*(row_end-row_start)*4 bytes; width and
+ * slab height must then be multiples of 64, host.rs:476) and, when
  * out_accum_rgb is not NULL, the FP32 linear radiance mean per pixel in row
- * order (3 floats per pixel). */
+ * order (3 floats per pixel).  Either output may be NULL. */
 apt_status apt_render(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
                       uint32_t row_end, uint8_t* out_rgba, size_t out_len, float* out_accum_rgb,
                       size_t accum_len);
@@ -315,6 +315,9 @@ apt_status apt_render(apt_renderer* r, uint32_t width, uint32_t height, uint32_t
 typedef void (*apt_result_cb)(const uint8_t* pixels, size_t len, void* user);
 apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
                          uint32_t row_end, apt_result_cb cb, void* user);
+
+/* Restarts the frame sequence at frame_idx (seeds are frame*spp + sample). */
+apt_status apt_set_frame_index(apt_renderer* r, uint32_t frame_idx);
 
 typedef struct apt_render_stats {
   uint64_t paths;            /* camera paths started in the last render */
@@ -328,6 +331,113 @@ typedef struct apt_render_stats {
   double ms_other;
 } apt_render_stats;
 apt_status apt_get_render_stats(const apt_renderer* r, apt_render_stats* out);
+
+/* ------------------------------------------------------------------------
+ * Autotuner: per-scene choice of BVH width (2/4/8), traversal stack entries
+ * kept in LDS (8/16/24) and wavefront path budget, by timing a probe frame of
+ * the integrator on the device for each candidate.  Every candidate must give
+ * the identical image (checked); the fastest is returned with a JSON report.
+ * ------------------------------------------------------------------------ */
+typedef struct apt_tune_opts {
+  uint32_t probe_width;    /* 0 = 512 */
+  uint32_t probe_height;   /* 0 = 256 */
+  uint32_t probe_spp;      /* 0 = 2 */
+  uint32_t probe_bounces;  /* 0 = 4 */
+  uint32_t repeats;        /* timed renders per candidate, 0 = 2 */
+} apt_tune_opts;
+
+typedef struct apt_tune_result {
+  uint32_t bvh_width;
+  uint32_t lds_stack_depth;
+  uint32_t queue_chunk;
+  uint32_t trials;
+  double best_ms;
+} apt_tune_result;
+
+apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt_camera* camera,
+                        const apt_tune_opts* opts, apt_tune_result* out, char* json,
+                        size_t json_len);
+
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching of render requests over resident scenes.
+ *
+ * Replaces running one render-node job per request (NodeRenderer::render,
+ * node.rs:44-105) with a server that keeps scene BVHs resident in HBM, admits
+ * requests by memory footprint as earlier ones finish, and packs tiles of all
+ * admitted requests (any scene, camera, resolution, spp and depth) into shared
+ * wavefront batches replayed from a captured hipGraph.  Each request's result
+ * does not depend on what it was batched with: paths are seeded by (request
+ * seed, pixel, sample) and radiance is summed in 64-bit fixed point.
+ * All apt_server_* / apt_submit / apt_poll / apt_wait / apt_cancel calls are
+ * thread-safe.
+ * ------------------------------------------------------------------------ */
+
+typedef struct apt_server apt_server;
+
+typedef struct apt_server_config {
+  uint32_t path_slots;        /* paths per wavefront batch (0 = 2^21) */
+  uint32_t max_active;        /* admitted requests at a time (0 = 64) */
+  uint64_t accum_budget;      /* bytes of per-request accumulators admitted (0 = 4 GiB) */
+  uint32_t max_bounces;       /* deepest path any request may ask for (0 = 8) */
+  uint32_t tile_paths;        /* paths per scheduling tile (0 = 65536) */
+  uint32_t bvh_width;         /* 2, 4, 8 (0 = 4) */
+  uint32_t lds_stack_depth;   /* 8, 16, 24 (0 = 8) */
+  uint32_t policy;            /* 0 = shortest remaining work first, 1 = FIFO */
+} apt_server_config;
+
+/* A render request: scene, camera, frame size, samples, depth and seed. */
+typedef struct apt_request {
+  uint32_t scene_id;
+  uint32_t width;
+  uint32_t height;
+  uint32_t spp;
+  uint32_t max_bounces;
+  uint32_t rr_start_bounce;
+  uint32_t seed;
+  uint32_t _pad;
+  apt_camera camera;
+} apt_request;
+
+typedef enum apt_request_state {
+  APT_REQ_QUEUED = 0,
+  APT_REQ_ACTIVE = 1,
+  APT_REQ_DONE = 2,
+  APT_REQ_CANCELLED = 3,
+  APT_REQ_FAILED = 4
+} apt_request_state;
+
+typedef struct apt_result {
+  uint32_t state;             /* apt_request_state */
+  uint32_t status;            /* apt_status of a failed request */
+  double submit_ms;           /* server clock at apt_submit */
+  double admit_ms;            /* ... when admitted (footprint reserved) */
+  double done_ms;             /* ... when the result was in out_rgb */
+  uint64_t paths;             /* camera paths traced */
+} apt_result;
+
+apt_status apt_server_create(const apt_server_config* cfg, apt_server** out);
+void apt_server_destroy(apt_server* s);
+/* Uploads a scene (BVH + attributes) and keeps it resident. */
+apt_status apt_server_add_scene(apt_server* s, const apt_mesh_desc* mesh, const apt_sky* sky,
+                                uint32_t* scene_id);
+/* Queues a request; out_rgb (width*height*3 floats, host memory owned by the
+ * caller) receives the linear radiance mean when the request is done. */
+apt_status apt_submit(apt_server* s, const apt_request* req, float* out_rgb, uint64_t* id);
+apt_status apt_poll(apt_server* s, uint64_t id, apt_result* out);
+/* Blocks until the request is done, cancelled or failed, or timeout_ms elapses
+ * (APT_ERR_BUSY). */
+apt_status apt_wait(apt_server* s, uint64_t id, uint32_t timeout_ms, apt_result* out);
+apt_status apt_cancel(apt_server* s, uint64_t id);
+
+typedef struct apt_server_stats {
+  uint64_t batches;
+  uint64_t paths;
+  uint64_t requests_done;
+  uint32_t active;
+  uint32_t queued;
+  double gpu_busy_ms;         /* summed device time of the batches */
+} apt_server_stats;
+apt_status apt_server_get_stats(apt_server* s, apt_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -153,3 +153,14 @@ def test_invalid_requests_rejected(gpu_available):
     for args in [((100, 64),), ((64, 64), 0, 32), ((64, 64), 64, 128)]:
         with pytest.raises(Exception):
             pt.render_accum(*args)
+
+
+def test_autotune_cornell(gpu_available, tmp_path):
+    from appearance_amd.autotune import tune
+    s = generate_scene("cornell")
+    rep = tune(s, probe=(128, 64), spp=2, bounces=4, repeats=1, json_path=tmp_path / "t.json")
+    assert len(rep["candidates"]) == 18
+    assert all(c["identical"] for c in rep["candidates"])
+    best = rep["best"]
+    assert best["bvh_width"] in (2, 4, 8) and best["lds_stack_depth"] in (8, 16, 24)
+    assert (tmp_path / "t.json").exists()
