@@ -26,6 +26,7 @@ SOURCES = [
     "server.cpp",
     "server.hip",
     "tuner.cpp",
+    "restir.hip",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

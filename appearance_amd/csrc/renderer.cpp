@@ -7,6 +7,7 @@
 // buffers for the requested slab, the sample/bounce schedule of the wavefront
 // integrator, progressive accumulation and the blocking readback.
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -15,6 +16,7 @@
 
 #include "apt_internal.h"
 #include "integrator.h"
+#include "restir.h"
 #include "scene_gpu.h"
 
 namespace {
@@ -52,6 +54,62 @@ struct DevBuf {
     n = 0;
   }
 };
+
+// Per-pixel state of the ReSTIR integrator for one slab.
+struct RestirState {
+  DevBuf<float4> ray_o, ray_d, thr, radiance, vtx_p, vtx_n, vtx_t, demod[2], work, col[2], hist;
+  DevBuf<uint32_t> rng, count, counters;
+  DevBuf<DiRes> di, di_tmp, di_hist[2];
+  DevBuf<GiRes> gi, gi_hist[2];
+  DevBuf<GTexel> gbuf[2];
+  DevBuf<float2> mom[2];
+  DevBuf<float> var[2];
+  size_t npix = 0;
+
+  template <typename F>
+  void each(F&& f) {
+    f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(demod[0]); f(demod[1]);
+    f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
+    f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
+    f(var[0]); f(var[1]);
+  }
+  // Sizes every buffer for n pixels and clears them (no history).
+  apt_status reset(size_t n, hipStream_t s) {
+    apt_status st = APT_OK;
+    each([&](auto& b) {
+      if (st == APT_OK) st = b.ensure(n);
+      if (st == APT_OK && hipMemsetAsync(b.p, 0, n * sizeof(*b.p), s) != hipSuccess) {
+        set_error("restir: clearing state failed");
+        st = APT_ERR_DEVICE;
+      }
+    });
+    if (st == APT_OK) st = counters.ensure(2);
+    npix = st == APT_OK ? n : 0;
+    return st;
+  }
+  void release() {
+    each([](auto& b) { b.release(); });
+    counters.release();
+    npix = 0;
+  }
+  RestirBuffers view() {
+    RestirBuffers b{};
+    b.ray_o = ray_o.p; b.ray_d = ray_d.p; b.thr = thr.p; b.rng = rng.p; b.radiance = radiance.p;
+    b.vtx_p = vtx_p.p; b.vtx_n = vtx_n.p; b.vtx_t = vtx_t.p;
+    b.di = di.p; b.di_tmp = di_tmp.p; b.di_hist[0] = di_hist[0].p; b.di_hist[1] = di_hist[1].p;
+    b.gi = gi.p; b.gi_hist[0] = gi_hist[0].p; b.gi_hist[1] = gi_hist[1].p;
+    b.gbuf[0] = gbuf[0].p; b.gbuf[1] = gbuf[1].p;
+    b.demod[0] = demod[0].p; b.demod[1] = demod[1].p; b.work = work.p;
+    b.svgf_col[0] = col[0].p; b.svgf_col[1] = col[1].p; b.svgf_mom[0] = mom[0].p; b.svgf_mom[1] = mom[1].p;
+    b.svgf_var[0] = var[0].p; b.svgf_var[1] = var[1].p; b.svgf_count = count.p; b.svgf_hist = hist.p;
+    b.counters = counters.p;
+    return b;
+  }
+};
+
+CamView cam_view(const SceneView& sc) {
+  return CamView{sc.cam_pos, sc.cam_right, sc.cam_up, sc.cam_back, sc.cam_tan_half, sc.cam_aspect};
+}
 
 void mat_point(const float* m, const float* p, float* out) {
   for (int r = 0; r < 3; ++r) out[r] = m[r] * p[0] + m[4 + r] * p[1] + m[8 + r] * p[2] + m[12 + r];
@@ -104,6 +162,13 @@ struct apt_renderer {
   apt_render_stats stats{};
   uint32_t frame_idx = 0;
   std::vector<uint8_t> cb_pixels;
+  // ReSTIR integrator: per-pixel state, frame parity and the previous camera
+  RestirState rs;
+  uint32_t rs_cur = 0;
+  bool rs_history = false;
+  CamView rs_prev_cam{};
+  uint64_t rs_scene_version = 0;
+  uint32_t rs_w = 0, rs_h = 0, rs_row0 = 0;
 
   ~apt_renderer() {
     for (Model& m : models) apt_scene_data_destroy(m.data);
@@ -112,6 +177,7 @@ struct apt_renderer {
     p_o.release(); p_d.release(); p_t.release(); p_l.release(); p_h.release(); p_so.release();
     p_sd.release(); p_sc.release(); p_rng.release(); p_q0.release(); p_q1.release();
     p_qs.release(); p_cnt.release();
+    rs.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -129,12 +195,22 @@ apt_status validate_config(const apt_config& c) {
               "config: bvh_width %u is not 2, 4 or 8", c.bvh_width);
   APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 16 || c.lds_stack_depth == 24,
               APT_ERR_INVALID_ARG, "config: lds_stack_depth %u is not 8, 16 or 24", c.lds_stack_depth);
-  APT_REQUIRE(c.integrator == APT_INTEGRATOR_PLAIN, APT_ERR_UNSUPPORTED,
-              "config: integrator %u is not available (only APT_INTEGRATOR_PLAIN)", c.integrator);
-  APT_REQUIRE(!c.restir_di && !c.restir_gi, APT_ERR_UNSUPPORTED,
-              "config: ReSTIR DI/GI is not available in this build");
-  APT_REQUIRE(!c.svgf && !c.taa && !c.firefly_filter, APT_ERR_UNSUPPORTED,
-              "config: the denoise/post chain (svgf, taa, firefly) is not available in this build");
+  APT_REQUIRE(c.integrator == APT_INTEGRATOR_PLAIN || c.integrator == APT_INTEGRATOR_RESTIR,
+              APT_ERR_INVALID_ARG, "config: unknown integrator %u", c.integrator);
+  if (c.integrator == APT_INTEGRATOR_PLAIN) {
+    APT_REQUIRE(!c.restir_di && !c.restir_gi, APT_ERR_INVALID_ARG,
+                "config: restir_di/restir_gi need integrator APT_INTEGRATOR_RESTIR");
+    APT_REQUIRE(!c.svgf && !c.taa && !c.firefly_filter, APT_ERR_INVALID_ARG,
+                "config: svgf, taa and firefly_filter need the G-buffer of APT_INTEGRATOR_RESTIR");
+    return APT_OK;
+  }
+  APT_REQUIRE(c.di_candidates <= 64, APT_ERR_INVALID_ARG, "config: di_candidates %u above 64", c.di_candidates);
+  APT_REQUIRE(c.di_spatial_neighbours <= 16, APT_ERR_INVALID_ARG, "config: di_spatial_neighbours %u above 16",
+              c.di_spatial_neighbours);
+  APT_REQUIRE(c.di_spatial_radius >= 0.0f && c.di_spatial_radius <= 4096.0f, APT_ERR_INVALID_ARG,
+              "config: di_spatial_radius %g outside [0, 4096]", (double)c.di_spatial_radius);
+  APT_REQUIRE(c.svgf_iterations <= 8, APT_ERR_INVALID_ARG, "config: svgf_iterations %u above 8", c.svgf_iterations);
+  APT_REQUIRE(c.svgf_history >= 1, APT_ERR_INVALID_ARG, "config: svgf_history must be at least 1");
   return APT_OK;
 }
 
@@ -206,6 +282,101 @@ apt_status ensure_paths(apt_renderer* r, uint32_t n) {
   return APT_OK;
 }
 
+// Copies the frame out (RGBA8 block layout and/or the FP32 mean) and waits.
+apt_status finish_frame(apt_renderer* r, size_t npix, uint8_t* out_rgba, float* out_accum) {
+  hipStream_t s = r->stream;
+  if (out_rgba) APT_HIP(hipMemcpyAsync(out_rgba, r->d_rgba.p, npix * 4, hipMemcpyDeviceToHost, s));
+  if (out_accum) {
+    r->h_lin.resize(npix);
+    APT_HIP(hipMemcpyAsync(r->h_lin.data(), r->d_accum_lin.p, npix * sizeof(float4), hipMemcpyDeviceToHost, s));
+  }
+  APT_HIP(hipStreamSynchronize(s));
+  if (out_accum) {
+    for (size_t i = 0; i < npix; ++i) {
+      const float4 v = r->h_lin[i];
+      const float inv = v.w > 0.0f ? 1.0f / v.w : 0.0f;
+      out_accum[3 * i] = v.x * inv;
+      out_accum[3 * i + 1] = v.y * inv;
+      out_accum[3 * i + 2] = v.z * inv;
+    }
+  }
+  return APT_OK;
+}
+
+// The ReSTIR frame (PathTracerGpu::render, src/lib.rs:354-576): samples at
+// pixel centres with reuse against the previous frame's reservoirs and
+// G-buffer, the post chain, then the shared resolve.
+apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
+                         uint32_t rows, uint8_t* out_rgba, float* out_accum) {
+  hipStream_t s = r->stream;
+  const size_t npix = (size_t)width * rows;
+  apt_status st;
+  if (r->rs.npix != npix || r->rs_scene_version != r->scene_version || r->rs_w != width || r->rs_h != height ||
+      r->rs_row0 != row_start) {
+    if ((st = r->rs.reset(npix, s)) != APT_OK) return st;
+    r->rs_history = false;
+    r->rs_cur = 0;
+    r->rs_scene_version = r->scene_version;
+    r->rs_w = width;
+    r->rs_h = height;
+    r->rs_row0 = row_start;
+  }
+  APT_HIP(hipMemsetAsync(r->rs.radiance.p, 0, npix * sizeof(float4), s));
+  APT_HIP(hipMemsetAsync(r->rs.counters.p, 0, 2 * sizeof(uint32_t), s));
+  const SceneView sc = scene_gpu_view(r->gpu, r->cam, width, height);
+  const CamView cur = cam_view(sc);
+  const apt_config& c = r->cfg;
+  RestirParams p{};
+  p.width = width;
+  p.rows = rows;
+  p.row_start = row_start;
+  p.height = height;
+  p.max_bounces = c.max_bounces;
+  p.di_candidates = c.di_candidates;
+  p.di_mcap = c.di_mcap;
+  p.neighbours = c.di_spatial_neighbours;
+  p.radius = c.di_spatial_radius;
+  p.gi_mcap = c.gi_mcap;
+  p.restir_di = c.restir_di ? 1u : 0u;
+  p.restir_gi = c.restir_gi ? 1u : 0u;
+  p.cur = r->rs_cur;
+  p.svgf_history = c.svgf_history;
+  p.atrous_passes = c.svgf_iterations;
+  p.first_frame = r->rs_history ? 0u : 1u;
+  p.prev_cam = r->rs_history ? r->rs_prev_cam : cur;
+  const RestirBuffers b = r->rs.view();
+  const uint32_t spp = c.sample_count;
+  APT_HIP(hipEventRecord(r->ev0, s));
+  for (uint32_t k = 0; k < spp; ++k) {
+    p.seed = r->frame_idx * spp + k + c.seed;  // lib.rs:374
+    if ((st = restir_sample(p, sc, b, s)) != APT_OK) return st;
+  }
+  if ((st = restir_post(p, b, c.firefly_filter != 0, c.svgf != 0, c.taa != 0, r->d_frame_sum.p, s)) != APT_OK)
+    return st;
+  FrameView fv{width, height, row_start, rows, spp, r->frame_idx, c.seed, c.max_bounces, 2};
+  if ((st = resolve_frame(fv, r->d_frame_sum.p, r->d_accum_tm.p, r->d_accum_lin.p, r->accum_frames,
+                          out_rgba ? r->d_rgba.p : nullptr, s)) != APT_OK)
+    return st;
+  APT_HIP(hipEventRecord(r->ev1, s));
+  uint32_t cnt[2];
+  APT_HIP(hipMemcpyAsync(cnt, r->rs.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  if ((st = finish_frame(r, npix, out_rgba, out_accum)) != APT_OK) return st;
+  float ms = 0.0f;
+  APT_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  r->stats = apt_render_stats{};
+  r->stats.paths = (uint64_t)npix * spp;
+  r->stats.extension_rays = cnt[0];
+  r->stats.shadow_rays = cnt[1];
+  r->stats.segments = cnt[0];
+  r->stats.ms_total = ms;
+  r->accum_frames++;
+  r->frame_idx++;
+  r->rs_cur ^= 1u;
+  r->rs_prev_cam = cur;
+  r->rs_history = true;
+  return APT_OK;
+}
+
 apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
                        uint32_t row_end, uint8_t* out_rgba, size_t out_len, float* out_accum,
                        size_t accum_len) {
@@ -248,6 +419,8 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
     r->accum_scene_version = r->scene_version;
   }
   APT_HIP(hipMemsetAsync(r->d_frame_sum.p, 0, npix * sizeof(float4), s));
+  if (r->cfg.integrator == APT_INTEGRATOR_RESTIR)
+    return render_restir(r, width, height, row_start, rows, out_rgba, out_accum);
 
   const uint32_t spp = r->cfg.sample_count;
   const uint64_t budget = r->cfg.queue_chunk ? r->cfg.queue_chunk : (1u << 21);
@@ -276,21 +449,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
   APT_HIP(hipEventRecord(r->ev1, s));
   uint32_t cnt[CNT_COUNT];
   APT_HIP(hipMemcpyAsync(cnt, r->p_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-  if (out_rgba) APT_HIP(hipMemcpyAsync(out_rgba, r->d_rgba.p, npix * 4, hipMemcpyDeviceToHost, s));
-  if (out_accum) {
-    r->h_lin.resize(npix);
-    APT_HIP(hipMemcpyAsync(r->h_lin.data(), r->d_accum_lin.p, npix * sizeof(float4), hipMemcpyDeviceToHost, s));
-  }
-  APT_HIP(hipStreamSynchronize(s));
-  if (out_accum) {
-    for (size_t i = 0; i < npix; ++i) {
-      const float4 v = r->h_lin[i];
-      const float inv = v.w > 0.0f ? 1.0f / v.w : 0.0f;
-      out_accum[3 * i] = v.x * inv;
-      out_accum[3 * i + 1] = v.y * inv;
-      out_accum[3 * i + 2] = v.z * inv;
-    }
-  }
+  if ((st = finish_frame(r, npix, out_rgba, out_accum)) != APT_OK) return st;
   float ms = 0.0f;
   APT_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
   r->stats = apt_render_stats{};
@@ -331,20 +490,20 @@ void apt_config_default(apt_config* c) {
   c->max_bounces = 2;        // PathTracerGpuConfig::default (lib.rs:230-245)
   c->sample_count = 1;
   c->accum_frames = 0;
-  c->rr_start_bounce = 3;    // trace.wgsl:281: roulette once bounce > 1
-  c->integrator = APT_INTEGRATOR_PLAIN;
-  c->restir_di = 0;
-  c->restir_gi = 0;
+  c->rr_start_bounce = 3;    // plain integrator; trace.wgsl:221: roulette once bounce > 1
+  c->integrator = APT_INTEGRATOR_RESTIR;
+  c->restir_di = 1;
+  c->restir_gi = 1;
   c->svgf = 0;
-  c->firefly_filter = 0;
+  c->firefly_filter = 1;
   c->taa = 0;
-  c->di_candidates = 32;
-  c->di_mcap = 20;
-  c->di_spatial_neighbours = 5;
-  c->di_spatial_radius = 30.0f;
-  c->gi_mcap = 30;
-  c->svgf_iterations = 5;
-  c->svgf_history = 4;
+  c->di_candidates = 4;      // NUM_AREA_SAMPLES (nee.wgsl:125)
+  c->di_mcap = 20;           // restir_di_temporal.wgsl:159
+  c->di_spatial_neighbours = 3;  // NUM_SAMPLES (restir_di_spatial.wgsl:15)
+  c->di_spatial_radius = 0.0f;   // 5% of (width + height) / 2 (restir_di_spatial.wgsl:96)
+  c->gi_mcap = 30;           // restir_gi_temporal.wgsl:100
+  c->svgf_iterations = 5;    // lib.rs:523
+  c->svgf_history = 24;      // lib.rs:522
   c->bvh_width = 4;
   c->queue_chunk = 0;
   c->lds_stack_depth = 8;
@@ -391,8 +550,11 @@ apt_status apt_set_config(apt_renderer* r, const apt_config* cfg) {
   if (cfg->bvh_width != r->cfg.bvh_width) r->scene_dirty = true;
   if (r->gpu.bvh) r->gpu.bvh->lds_stack = cfg->lds_stack_depth;
   if (cfg->sample_count != r->cfg.sample_count || cfg->max_bounces != r->cfg.max_bounces ||
-      cfg->rr_start_bounce != r->cfg.rr_start_bounce || cfg->seed != r->cfg.seed)
+      cfg->rr_start_bounce != r->cfg.rr_start_bounce || cfg->seed != r->cfg.seed ||
+      std::memcmp(&cfg->integrator, &r->cfg.integrator,
+                  offsetof(apt_config, bvh_width) - offsetof(apt_config, integrator)) != 0)
     r->cam_version++;  // restart accumulation
+  if (cfg->integrator != r->cfg.integrator) r->rs.npix = 0;  // no ReSTIR history across a switch
   r->cfg = *cfg;
   return APT_OK;
 }

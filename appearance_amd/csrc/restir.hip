@@ -1,0 +1,1117 @@
+// ReSTIR DI / GI integrator and the denoise/post chain for gfx950.
+//
+// Restates the reference's frame (crates/appearance-path-tracer-gpu/src/lib.rs:
+// 354-576) as per-pixel kernels over 16x16 tiles, one sample at a time:
+//   raygen                 pixel-centre camera rays (raygen.wgsl:21-40)
+//   per bounce:
+//     extend               closest hit, bounce-0 emission, sky on miss,
+//                          G-buffer at bounce 0 (trace.wgsl:62-263)
+//     di_ris               RIS over light candidates plus one BSDF candidate,
+//                          visibility of the winner (helpers/nee.wgsl:122-237)
+//     gi_ris               roulette, one BSDF-sampled candidate whose target is
+//                          an approximate one-bounce estimate
+//                          (helpers/inline_path_tracer.wgsl:51-296)
+//     bounce 0 only:       DI temporal + spatial reuse (restir_di_temporal.wgsl,
+//                          restir_di_spatial.wgsl), GI temporal reuse
+//                          (restir_gi_temporal.wgsl)
+//     apply_di             shade the DI reservoir (apply_di.wgsl:48-111)
+//     apply_gi             continue the path toward the GI sample
+//                          (apply_gi.wgsl:44-99)
+//   post                   demodulate, firefly filter, SVGF temporal + a-trous,
+//                          TAA, remodulate (demodulate_radiance.wgsl,
+//                          firefly_filter.wgsl, svgf_temporal.wgsl,
+//                          svgf_atrous.wgsl, taa.wgsl)
+// Each pixel carries one RNG chain through all of its kernels in the
+// reference's draw order.  Deviations from the reference (light pdfs in solid
+// angle with plain emission, balance-heuristic MIS on each candidate's own
+// pdfs, uniform barycentrics, one-sided emitters, the GI target floor, the
+// roulette probability, out-of-place filters, clamped neighbour indices) are
+// noted where they occur.
+#include <hip/hip_fp16.h>
+
+#include "pathtrace_device.h"
+#include "restir.h"
+
+namespace apt {
+
+namespace {
+
+constexpr int TILE = 16;
+constexpr int BLOCK = TILE * TILE;
+constexpr float RESTIR_DI_EPSILON = 1e-6f;            // di_reservoir.wgsl:4
+constexpr float GOLDEN_ANGLE = 2.39996322972865332f;  // pi (3 - sqrt 5)
+constexpr float GI_SKY_DISTANCE = 1000.0f;  // first hit of an escaped GI ray (inline_path_tracer.wgsl:246)
+constexpr float GI_OFFSET = 1e-4f;          // GiReservoir::phat origin offset (inline_path_tracer.wgsl:323)
+// Target floor of the GI reservoir, times luma(T f cos): keeps p^ > 0 wherever
+// the continuation can carry light.  The reference's p^ is zero whenever its
+// one-sample estimate is occluded, and such paths are dropped (a bias).
+constexpr float GI_PHAT_FLOOR = 1e-3f;
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool tile_pixel(const RestirParams& p, uint32_t& x, uint32_t& y,
+                                           uint32_t& pix) {
+  const uint32_t tiles_x = (p.width + TILE - 1) / TILE;
+  const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+  x = (t % tiles_x) * TILE + threadIdx.x % TILE;
+  y = (t / tiles_x) * TILE + threadIdx.x / TILE;
+  pix = y * p.width + x;
+  return x < p.width && y < p.rows;
+}
+
+__device__ __forceinline__ void count_rays(bool traced, uint32_t* ctr) {
+  const unsigned long long m = __ballot(traced);
+  if ((int)__lane_id() == __ffsll(m) - 1) atomicAdd(ctr, (uint32_t)__popcll(m));
+}
+
+__device__ __forceinline__ uint32_t h16(float a) { return (uint32_t)__half_as_ushort(__float2half_rn(a)); }
+__device__ __forceinline__ float f16lo(uint32_t v) { return __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu))); }
+__device__ __forceinline__ float f16hi(uint32_t v) { return __half2float(__ushort_as_half((unsigned short)(v >> 16))); }
+
+__device__ __forceinline__ float safe_div(float a, float b) { return b > 0.0f ? a / b : 0.0f; }  // math.wgsl:20-26
+__device__ __forceinline__ v3 safe_div3(v3 a, v3 b) { return mk(safe_div(a.x, b.x), safe_div(a.y, b.y), safe_div(a.z, b.z)); }
+__device__ __forceinline__ v3 f4v(float4 a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ float4 v4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+__device__ __forceinline__ bool finite3(v3 a) { return a.x == a.x && a.y == a.y && a.z == a.z; }
+__device__ __forceinline__ v3 lerp3(v3 a, v3 b, float t) { return a * (1.0f - t) + b * t; }
+
+// G-buffer texel as the reuse passes see it: FP16 depth and normal.
+struct Tex {
+  v3 pos;
+  float depth;
+  v3 n;
+  v3 albedo;
+};
+
+__device__ __forceinline__ GTexel pack_texel(v3 pos, float depth, v3 n, v3 albedo, uint32_t matbits) {
+  GTexel t;
+  t.pos_depth = make_float4(pos.x, pos.y, pos.z, depth);
+  t.packed = make_uint4(h16(depth) | (h16(n.x) << 16), h16(n.y) | (h16(n.z) << 16), pack_rgb9e5(albedo), matbits);
+  return t;
+}
+
+__device__ __forceinline__ Tex unpack_texel(const GTexel& g) {
+  Tex t;
+  t.pos = f4v(g.pos_depth);
+  t.depth = f16lo(g.packed.x);
+  t.n = mk(f16hi(g.packed.x), f16lo(g.packed.y), f16hi(g.packed.y));
+  t.albedo = unpack_rgb9e5(g.packed.z);
+  return t;
+}
+
+// GBuffer::sample_prev_gbuffer (gbuffer_bindings.wgsl:20-51): bilinear texel
+// of the previous frame at a pixel-index position inside the slab.
+__device__ __forceinline__ Tex sample_prev(const GTexel* g, const RestirParams& p, float px, float py) {
+  const uint32_t ix = (uint32_t)floorf(px), iy = (uint32_t)floorf(py);
+  const float fx = px - floorf(px), fy = py - floorf(py);
+  const uint32_t x1 = min(ix + 1, p.width - 1), y1 = min(iy + 1, p.rows - 1);
+  const Tex t00 = unpack_texel(g[iy * p.width + ix]), t10 = unpack_texel(g[iy * p.width + x1]);
+  const Tex t01 = unpack_texel(g[y1 * p.width + ix]), t11 = unpack_texel(g[y1 * p.width + x1]);
+  Tex r;
+  r.pos = lerp3(lerp3(t00.pos, t10.pos, fx), lerp3(t01.pos, t11.pos, fx), fy);
+  r.depth = lerpf(lerpf(t00.depth, t10.depth, fx), lerpf(t01.depth, t11.depth, fx), fy);
+  r.n = normalize(lerp3(lerp3(t00.n, t10.n, fx), lerp3(t01.n, t11.n, fx), fy));
+  r.albedo = lerp3(lerp3(t00.albedo, t10.albedo, fx), lerp3(t01.albedo, t11.albedo, fx), fy);
+  return r;
+}
+
+// GBufferTexel::is_disoccluded (gbuffer_bindings.wgsl:53-57); true when the
+// texels match.
+__device__ __forceinline__ bool texel_similar(const Tex& a, const Tex& prev) {
+  return fabsf(a.depth - prev.depth) <= 0.1f * prev.depth && dot(a.n, prev.n) > 0.906f;
+}
+
+// PackedGBufferTexel::depth_cs (gbuffer.wgsl:43-46) with near 0.001, far 10000.
+__device__ __forceinline__ float depth_cs(float depth) {
+  const float zn = 0.001f, zf = 10000.0f;
+  const float zl = (depth - zn) / zf;
+  return (zn * zf) / (zf - zl * (zf - zn));
+}
+
+// Pixel-index position (pixel centres at integers, slab rows) of a world point
+// in the previous camera; stands in for the velocity texture
+// (gbuffer.wgsl:25-53): prev = id - resolution * velocity.
+__device__ __forceinline__ bool reproject(const RestirParams& p, v3 x, float& px, float& py) {
+  const CamView& c = p.prev_cam;
+  const v3 d = x - c.pos;
+  const float lz = dot(d, c.back);
+  if (!(lz < 0.0f)) return false;
+  const float nx = dot(d, c.right) / (-lz * c.tan_half * c.aspect);
+  const float ny = dot(d, c.up) / (-lz * c.tan_half);
+  px = (nx + 1.0f) * 0.5f * (float)p.width - 0.5f;
+  py = (1.0f - ny) * 0.5f * (float)p.height - 0.5f - (float)p.row_start;
+  return px >= 0.0f && py >= 0.0f && px <= (float)(p.width - 1) && py <= (float)(p.rows - 1);
+}
+
+// ---------------------------------------------------------------------------
+// shading point and BSDF
+// ---------------------------------------------------------------------------
+struct Vtx {
+  v3 x, n, wo, T;
+  float t;
+  uint32_t mat;
+  bool back;
+};
+
+__device__ __forceinline__ bool load_vtx(const RestirBuffers& b, uint32_t i, Vtx& v) {
+  const float4 p4 = b.vtx_p[i];
+  if (!(p4.w > 0.0f)) return false;
+  const float4 n4 = b.vtx_n[i];
+  v.x = f4v(p4);
+  v.t = p4.w;
+  v.n = f4v(n4);
+  const uint32_t bits = f2u(n4.w);
+  v.mat = bits & 0x7FFFFFFFu;
+  v.back = (bits >> 31) != 0u;
+  v.wo = -f4v(b.ray_d[i]);
+  v.T = f4v(b.vtx_t[i]);
+  return true;
+}
+
+struct Bsdf {
+  Disney d;
+  v3 albedo;
+  ShadeFrame sf;
+  bool lambert, back;
+  float t;
+};
+
+__device__ __forceinline__ Bsdf make_bsdf(const SceneView& sc, uint32_t mat, v3 n, bool back, float t) {
+  const apt_material& m = sc.mats[mat];
+  Bsdf b;
+  b.lambert = m.bsdf_model == APT_BSDF_LAMBERT;
+  b.albedo = mk(m.color[0], m.color[1], m.color[2]);
+  b.d = disney_from_material(m);
+  b.sf.f = onb(n);
+  b.sf.cc = b.sf.f;
+  b.back = back;
+  b.t = t;
+  return b;
+}
+
+__device__ __forceinline__ v3 bsdf_eval(const Bsdf& b, v3 wo, v3 wi, float& pdf) {
+  return b.lambert ? lambert_eval(b.albedo, b.sf.f, wo, wi, pdf) : disney_eval(b.d, b.sf, wo, wi, b.back, pdf);
+}
+
+__device__ __forceinline__ BsdfSample bsdf_sample(const Bsdf& b, v3 wo, float r0, float r1, float r2) {
+  return b.lambert ? lambert_sample(b.albedo, b.sf.f, wo, r0, r1)
+                   : disney_sample(b.d, b.sf, wo, b.t, b.back, r0, r1, r2);
+}
+
+// ---------------------------------------------------------------------------
+// light samples (light_sample.wgsl, helpers/nee.wgsl:16-115)
+// ---------------------------------------------------------------------------
+// Nee::sample_uniform: r0 picks the sun (probability sun_pick) or the
+// emissive triangles, r2 the triangle (r1 would pick an instance; the scene
+// here is one instance), (r3, r4) the point: cone coordinates for the sun,
+// uniform barycentrics (sqrt warp) for a triangle.  The reference uses (r3, r4)
+// as barycentrics directly, which leaves the triangle when r3 + r4 > 1.
+__device__ __forceinline__ void sample_light(const SceneView& sc, float r0, float r2, float r3, float r4,
+                                             uint32_t& light, float& u, float& v) {
+  const float p_sun = sun_pick(sc);
+  if (p_sun > 0.0f && (r0 < p_sun || sc.n_emissive == 0)) {
+    light = LIGHT_SUN;
+    u = r3;
+    v = r4;
+  } else if (sc.n_emissive > 0) {
+    uint32_t k = (uint32_t)(r2 * (float)sc.n_emissive);
+    k = k < sc.n_emissive ? k : sc.n_emissive - 1;
+    light = sc.emissive[k];
+    const float su = sqrtf(r3);
+    u = su * (1.0f - r4);
+    v = su * r4;
+  } else {
+    light = LIGHT_EMPTY;
+    u = v = 0.0f;
+  }
+}
+
+// Sky::inverse_direction_to_sun (sky_bindings.wgsl:57-72).
+__device__ __forceinline__ void sun_uv(const SceneView& sc, v3 w, float& u, float& v) {
+  const v3 s = sun_to(sc.sky);
+  const v3 bt = perpendicular(s);
+  const v3 tg = cross(bt, s);
+  const float z = dot(w, s), x = dot(w, bt), y = dot(w, tg);
+  const float h = cosf(sun_half_angle(sc.sky));
+  u = fmodf(atan2f(y, x) / TWO_PI, 1.0f);
+  v = (z - h) / (1.0f - h);
+}
+
+// LightSample::load_eval_data (nee.wgsl:16-57) evaluated from shading point x:
+// the radiance toward x (triangles emit from their front side only; plain
+// emission instead of the reference's solid-angle x 10 scale), direction,
+// distance and the solid-angle density with which sample_light produces it.
+struct LightHit {
+  v3 Le, wi;
+  float dist, pdf;
+};
+
+__device__ __forceinline__ bool eval_light(const SceneView& sc, uint32_t light, float u, float v, v3 x,
+                                           LightHit& e) {
+  const float p_sun = sun_pick(sc);
+  if (light == LIGHT_SUN) {
+    const v3 dir = normalize(perturb_direction(u, v, sun_to(sc.sky), sun_half_angle(sc.sky)));
+    const v3 dv = dir * SUN_DISTANCE - x;
+    e.dist = length(dv);
+    e.wi = normalize(dv);
+    e.Le = mk(sc.sky.sun_color[0], sc.sky.sun_color[1], sc.sky.sun_color[2]) * sun_intensity(sc.sky, dir);
+    e.pdf = p_sun > 0.0f ? p_sun / sun_solid_angle(sc.sky) : 0.0f;
+    return true;
+  }
+  if (light == LIGHT_EMPTY || sc.n_emissive == 0) return false;
+  const TriGeom g = load_tri(sc, light);
+  const v3 pt = g.p0 * (1.0f - u - v) + g.p1 * u + g.p2 * v;
+  const v3 nl = normalize(cross(g.p1 - g.p0, g.p2 - g.p0));
+  const v3 dv = pt - x;
+  e.dist = length(dv);
+  e.wi = normalize(dv);
+  const float cos_l = -dot(nl, e.wi);
+  const apt_material& lm = sc.mats[sc.tri_mat[light]];
+  if (cos_l > 0.0f) {
+    e.Le = mk(lm.emission[0], lm.emission[1], lm.emission[2]);
+    e.pdf = (1.0f - p_sun) / (float)sc.n_emissive / tri_area(g) * (e.dist * e.dist) / cos_l;
+  } else {
+    e.Le = mk(0.0f, 0.0f, 0.0f);
+    e.pdf = 0.0f;
+  }
+  return true;
+}
+
+// trace_shadow_ray (helpers/trace.wgsl:20-84): opaque any-hit from the point
+// offset along the normal, tmax = distance - 2 eps.
+template <int W, int S>
+__device__ __forceinline__ bool visible(const SceneView& sc, const LdsStack& st, v3 x, v3 n, v3 wi, float dist,
+                                        uint32_t* ctr) {
+  TraceRay ray = make_trace_ray(x + n * TRACE_EPSILON, wi, 0.0f, dist - 2.0f * TRACE_EPSILON);
+  count_rays(true, ctr + 1);
+  return !trace_any<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st);
+}
+
+template <int W, int S>
+__device__ __forceinline__ HitRec closest(const SceneView& sc, const LdsStack& st, v3 o, v3 d, uint32_t* ctr) {
+  TraceRay ray = make_trace_ray(o, d, 0.0f, 1e30f);
+  count_rays(true, ctr);
+  return trace_closest<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st);
+}
+
+// LightSample::phat (di_reservoir.wgsl:58-97): luma(f Le cos), with the
+// visibility test when `vis`.
+template <int W, int S>
+__device__ __forceinline__ float di_phat(const SceneView& sc, const LdsStack& st, const Vtx& v, const Bsdf& bs,
+                                         uint32_t light, float u, float w, bool vis, uint32_t* ctr) {
+  LightHit e;
+  if (!eval_light(sc, light, u, w, v.x, e)) return 0.0f;
+  const float c = dot(e.wi, v.n);
+  if (!(c > 0.0f)) return 0.0f;
+  if (vis && !visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, ctr)) return 0.0f;
+  float pdf;
+  return luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * c);
+}
+
+// DiReservoir::update (di_reservoir.wgsl:46-56); one draw per call.
+__device__ __forceinline__ bool res_update(float& wsum, float& M, float w, uint32_t& rng) {
+  wsum += w;
+  M += 1.0f;
+  return rng_next(rng) <= w / wsum;
+}
+
+__device__ __forceinline__ DiRes di_empty() { return DiRes{LIGHT_EMPTY, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}; }
+
+// The two updates of DiReservoir::combine / combine_unbiased (:99-133).
+__device__ __forceinline__ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng) {
+  DiRes c = di_empty();
+  if (res_update(c.wsum, c.M, r1.phat * r1.W * r1.M, rng)) {
+    c.light = r1.light; c.u = r1.u; c.v = r1.v; c.phat = r1.phat;
+  }
+  if (res_update(c.wsum, c.M, r2.phat * r2.W * r2.M, rng)) {
+    c.light = r2.light; c.u = r2.u; c.v = r2.v; c.phat = r2.phat;
+  }
+  c.M = r1.M + r2.M;
+  return c;
+}
+
+// InlinePathTracer::trace with the approx config (max_bounces 1, no RIS):
+// radiance toward x carried by direction wi with path throughput Tf, and the
+// first hit (or a far point on an escaped ray).  Next-event estimation at the
+// hit takes one light sample (5 draws), visibility tested.
+template <int W, int S>
+__device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& st, v3 x, v3 wi, v3 Tf, v3& first,
+                                           uint32_t& rng, uint32_t* ctr) {
+  const v3 o = x + wi * TRACE_EPSILON;
+  const HitRec h = closest<W, S>(sc, st, o, wi, ctr);
+  if (h.prim == INVALID) {
+    first = x + wi * GI_SKY_DISTANCE;
+    return Tf * sky_gradient(sc.sky, wi);
+  }
+  const TriGeom g = load_tri(sc, h.prim);
+  const v3 x2 = o + wi * h.t;
+  first = x2;
+  v3 n2 = normalize(cross(g.p1 - g.p0, g.p2 - g.p0));
+  if (sc.tri_nrm) {
+    const float4* tn = sc.tri_nrm + 3 * (size_t)h.prim;
+    n2 = normalize(ld3(tn[0]) * (1.0f - h.u - h.v) + ld3(tn[1]) * h.u + ld3(tn[2]) * h.v);
+  }
+  const v3 wo2 = -wi;
+  const bool back = dot(wo2, n2) < 0.0f;
+  if (back) n2 = -n2;
+  const float r0 = rng_next(rng);
+  rng_next(rng);  // r1: instance pick
+  const float r2 = rng_next(rng), r3 = rng_next(rng), r4 = rng_next(rng);
+  uint32_t light;
+  float lu, lv;
+  sample_light(sc, r0, r2, r3, r4, light, lu, lv);
+  LightHit e;
+  if (!eval_light(sc, light, lu, lv, x2, e) || !(e.pdf > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
+  const float c = dot(e.wi, n2);
+  if (!(c > 0.0f) || !(maxc(e.Le) > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
+  if (!visible<W, S>(sc, st, x2, n2, e.wi, e.dist, ctr)) return mk(0.0f, 0.0f, 0.0f);
+  const Bsdf b2 = make_bsdf(sc, sc.tri_mat[h.prim], n2, back, h.t);
+  float pdf;
+  return Tf * bsdf_eval(b2, wo2, e.wi, pdf) * e.Le * (c / e.pdf);
+}
+
+// GiReservoir::phat (inline_path_tracer.wgsl:299-331): replays the approximate
+// estimate of sample x2 from shading point v with the stored RNG state.
+template <int W, int S>
+__device__ __forceinline__ float gi_phat(const SceneView& sc, const LdsStack& st, const Vtx& v, const Bsdf& bs,
+                                         v3 T, const GiRes& g, uint32_t* ctr) {
+  const v3 wi = normalize(f4v(g.x2_rng) - v.x);
+  float pdf;
+  const v3 f = bsdf_eval(bs, v.wo, wi, pdf);
+  const float c = fabsf(dot(wi, v.n));
+  const v3 Tf = T * f * c;
+  uint32_t r = f2u(g.x2_rng.w);
+  v3 first;
+  const v3 L = approx_trace<W, S>(sc, st, v.x + wi * GI_OFFSET, wi, Tf, first, r, ctr);
+  return luma(L) + GI_PHAT_FLOOR * luma(Tf);
+}
+
+// ---------------------------------------------------------------------------
+// kernels: one sample
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_rs_raygen(RestirParams p, SceneView sc, RestirBuffers b) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  v3 o, d;
+  camera_ray(sc, x, y + p.row_start, p.width, p.height, 0.5f, 0.5f, o, d);
+  b.ray_o[pix] = v4(o, 0.0f);
+  b.ray_d[pix] = v4(d, 0.0f);
+  b.thr[pix] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  b.rng[pix] = path_seed(pix, p.seed);  // trace.wgsl:76
+}
+
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const float4 t4 = b.thr[pix];
+  if (!(t4.w > 0.0f)) {
+    b.vtx_p[pix].w = -1.0f;
+    return;
+  }
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  const v3 T = f4v(t4);
+  const v3 d = f4v(b.ray_d[pix]);
+  const v3 o = f4v(b.ray_o[pix]) + d * TRACE_EPSILON;  // safe_origin along the ray (trace.wgsl:91-98)
+  const HitRec h = closest<W, S>(sc, st, o, d, b.counters);
+  const float4 L = b.radiance[pix];
+  if (h.prim == INVALID) {
+    // sky without the sun disk, which the light samples cover (trace.wgsl:237-243)
+    const v3 sky = T * sky_gradient(sc.sky, d);
+    b.radiance[pix] = make_float4(L.x + sky.x, L.y + sky.y, L.z + sky.z, 0.0f);
+    b.thr[pix].w = 0.0f;
+    b.vtx_p[pix].w = -1.0f;
+    if (bounce == 0)
+      b.gbuf[p.cur][pix] = pack_texel(mk(0.0f, 0.0f, 0.0f), 0.0f, mk(0.0f, 0.0f, 0.0f), mk(1.0f, 1.0f, 1.0f), 0u);
+    return;
+  }
+  const TriGeom g = load_tri(sc, h.prim);
+  const v3 hp = o + d * h.t;
+  const v3 ng = normalize(cross(g.p1 - g.p0, g.p2 - g.p0));
+  v3 ns = ng;
+  if (sc.tri_nrm) {
+    const float4* tn = sc.tri_nrm + 3 * (size_t)h.prim;
+    ns = normalize(ld3(tn[0]) * (1.0f - h.u - h.v) + ld3(tn[1]) * h.u + ld3(tn[2]) * h.v);
+  }
+  const bool back = dot(-d, ns) < 0.0f;
+  if (back) ns = -ns;
+  const uint32_t mid = sc.tri_mat[h.prim];
+  const apt_material& m = sc.mats[mid];
+  // emission at the first vertex only (trace.wgsl:139-141), front side only
+  if (bounce == 0 && has_emission(m) && dot(ng, d) < 0.0f) {
+    const v3 e = T * mk(m.emission[0], m.emission[1], m.emission[2]);
+    b.radiance[pix] = make_float4(L.x + e.x, L.y + e.y, L.z + e.z, 0.0f);
+  }
+  const uint32_t bits = mid | (back ? 0x80000000u : 0u);
+  b.vtx_p[pix] = v4(hp, h.t);
+  b.vtx_n[pix] = v4(ns, u2f(bits));
+  b.vtx_t[pix] = v4(T, 0.0f);
+  if (bounce == 0) {
+    // albedo = colour + emission (trace.wgsl:218-223)
+    const v3 alb = mk(m.color[0] + m.emission[0], m.color[1] + m.emission[1], m.color[2] + m.emission[2]);
+    b.gbuf[p.cur][pix] = pack_texel(hp, h.t, ns, alb, bits);
+  }
+}
+
+// Nee::sample_ris (nee.wgsl:122-237) with di_candidates area candidates and
+// one BSDF candidate.  MIS is the balance heuristic over each candidate's own
+// densities under both strategies (the reference mixes the densities of two
+// different candidates), so W = wsum / p^ is unbiased.
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_di_ris(RestirParams p, SceneView sc, RestirBuffers b) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  if (!load_vtx(b, pix, v)) return;
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+  uint32_t rng = b.rng[pix];
+  const float p_sun = sun_pick(sc);
+  const float na = (float)p.di_candidates, nb = 1.0f;
+  const uint32_t iters = p.di_candidates > 1u ? p.di_candidates : 1u;
+  DiRes r = di_empty();
+  for (uint32_t i = 0; i < iters; ++i) {
+    uint32_t a_light = LIGHT_EMPTY;
+    float a_u = 0.0f, a_v = 0.0f, a_phat = 0.0f, a_pl = 0.0f, a_pb = 0.0f;
+    if (i < p.di_candidates) {
+      const float r0 = rng_next(rng);
+      rng_next(rng);  // r1: instance pick
+      const float r2 = rng_next(rng), r3 = rng_next(rng), r4 = rng_next(rng);
+      sample_light(sc, r0, r2, r3, r4, a_light, a_u, a_v);
+      LightHit e;
+      if (eval_light(sc, a_light, a_u, a_v, v.x, e)) {
+        a_pl = e.pdf;
+        const float c = dot(e.wi, v.n);
+        if (c > 0.0f && a_pl > 0.0f) {
+          const v3 f = bsdf_eval(bs, v.wo, e.wi, a_pb);
+          if (a_pb > 0.0f) a_phat = luma(f * e.Le * c);
+        }
+      }
+    }
+    uint32_t b_light = LIGHT_EMPTY;
+    float b_u = 0.0f, b_v = 0.0f, b_phat = 0.0f, b_pl = 0.0f, b_pb = 0.0f;
+    if (i == 0) {
+      const float r0 = rng_next(rng), r1 = rng_next(rng), r2 = rng_next(rng);
+      const BsdfSample s = bsdf_sample(bs, v.wo, r0, r1, r2);
+      b_pb = s.pdf;
+      const v3 contrib = s.f * fabsf(dot(s.wi, v.n));
+      if (s.pdf > 0.0f && dot(contrib, contrib) > 0.0f) {
+        const HitRec h = closest<W, S>(sc, st, v.x + v.n * TRACE_EPSILON, s.wi, b.counters);
+        if (h.prim != INVALID) {
+          if (has_emission(sc.mats[sc.tri_mat[h.prim]])) {
+            b_light = h.prim;
+            b_u = h.u;
+            b_v = h.v;
+          }
+        } else if (p_sun > 0.0f) {
+          b_light = LIGHT_SUN;
+          sun_uv(sc, s.wi, b_u, b_v);
+        }
+        LightHit e;
+        if (b_light != LIGHT_EMPTY && eval_light(sc, b_light, b_u, b_v, v.x, e)) {
+          b_pl = e.pdf;
+          b_phat = luma(contrib * e.Le);
+        }
+      }
+    }
+    if (i < p.di_candidates) {
+      const float w = a_pl > 0.0f ? a_phat / (na * a_pl + nb * a_pb) : 0.0f;
+      if (res_update(r.wsum, r.M, w, rng)) {
+        r.light = a_light; r.u = a_u; r.v = a_v; r.phat = a_phat;
+      }
+    }
+    if (i == 0) {
+      const float w = b_pb > 0.0f ? b_phat / (na * b_pl + nb * b_pb) : 0.0f;
+      if (res_update(r.wsum, r.M, w, rng)) {
+        r.light = b_light; r.u = b_u; r.v = b_v; r.phat = b_phat;
+      }
+    }
+  }
+  r.W = 0.0f;
+  if (r.phat > RESTIR_DI_EPSILON) {
+    LightHit e;
+    if (eval_light(sc, r.light, r.u, r.v, v.x, e) && visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, b.counters))
+      r.W = (1.0f / r.phat) * r.wsum;
+  }
+  b.di[pix] = r;
+  b.rng[pix] = rng;
+}
+
+// Roulette and InlinePathTracer::sample_ris (inline_path_tracer.wgsl:258-296):
+// one BSDF candidate whose target is the approximate one-bounce estimate.
+// Roulette keeps the path with probability min(1, max T) (the reference
+// divides by max T even above 1), and a path it ends keeps this vertex's DI.
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_gi_ris(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  if (!load_vtx(b, pix, v)) return;
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  uint32_t rng = b.rng[pix];
+  v3 T = v.T;
+  GiRes g;
+  g.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  g.mwsp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool alive = true;
+  if (bounce > 1) {  // trace.wgsl:221-230
+    const float q = fminf(1.0f, maxc(T));
+    if (q < rng_next(rng)) alive = false;
+    else T = T * (1.0f / q);
+  }
+  if (alive) {
+    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    const float r0 = rng_next(rng), r1 = rng_next(rng), r2 = rng_next(rng);
+    const BsdfSample s = bsdf_sample(bs, v.wo, r0, r1, r2);
+    float M = 0.0f, wsum = 0.0f, phat = 0.0f;
+    if (s.pdf > 1e-6f) {
+      const float c = fabsf(dot(s.wi, v.n));
+      const v3 Tf = T * s.f * c;
+      const uint32_t phat_rng = rng;
+      v3 first;
+      const v3 L = approx_trace<W, S>(sc, st, v.x, s.wi, Tf, first, rng, b.counters);
+      const float ph = luma(L) + GI_PHAT_FLOOR * luma(Tf);
+      if (res_update(wsum, M, ph / s.pdf, rng)) {
+        g.x2_rng = v4(first, u2f(phat_rng));
+        phat = ph;
+      }
+    }
+    const float Wc = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
+    g.mwsp = make_float4(M, Wc, wsum, phat);
+  }
+  b.thr[pix] = v4(T, alive ? 1.0f : 0.0f);
+  b.gi[pix] = g;
+  b.rng[pix] = rng;
+}
+
+// restir_di_temporal.wgsl:68-176 (nearest previous pixel, stochastic
+// rounding), with the visibility-tested target.
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_di_temporal(RestirParams p, SceneView sc, RestirBuffers b) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  if (!load_vtx(b, pix, v)) {
+    b.di_tmp[pix] = di_empty();
+    b.di_hist[p.cur][pix] = di_empty();
+    return;
+  }
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  DiRes r = b.di[pix];
+  uint32_t rng = b.rng[pix];
+  float px, py;
+  if (reproject(p, v.x, px, py)) {
+    const Tex cur = unpack_texel(b.gbuf[p.cur][pix]);
+    const Tex prev = sample_prev(b.gbuf[p.cur ^ 1], p, px, py);
+    if (texel_similar(cur, prev)) {
+      const float ux = rng_next(rng), uy = rng_next(rng);
+      const uint32_t qx = min((uint32_t)floorf(px + ux), p.width - 1);
+      const uint32_t qy = min((uint32_t)floorf(py + uy), p.rows - 1);
+      DiRes pr = b.di_hist[p.cur ^ 1][qy * p.width + qx];
+      pr.M = fminf(pr.M, (float)p.di_mcap * r.M);
+      const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+      pr.phat = di_phat<W, S>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
+      DiRes c = di_merge(r, pr, rng);
+      if (c.phat > RESTIR_DI_EPSILON && c.M * c.wsum > RESTIR_DI_EPSILON)
+        c.W = (1.0f / c.phat) * (1.0f / c.M * c.wsum);
+      r = c;
+    }
+  }
+  b.di_tmp[pix] = r;
+  if (p.neighbours == 0) {
+    b.di[pix] = r;
+    b.di_hist[p.cur][pix] = r;
+  }
+  b.rng[pix] = rng;
+}
+
+// interleaved_gradient_noise_animated (shared/sampling.wgsl:64-74).
+__device__ __forceinline__ float fractf(float a) { return a - floorf(a); }
+__device__ __forceinline__ float ign_animated(uint32_t x, uint32_t y, uint32_t frame) {
+  const uint32_t id = frame % 64u;
+  const float fx = (float)x + 5.588238f * (float)id, fy = (float)y + 5.588238f * (float)id;
+  return fractf(52.9829189f * fractf(0.06711056f * fx + 0.00583715f * fy));
+}
+
+// mirror (restir_di_spatial.wgsl:59-61), clamped into the frame (the
+// reference's returns `max` for x = +-max).
+__device__ __forceinline__ uint32_t mirror_px(int x, int m) {
+  const int r = ((x + m) % (2 * m)) - m;
+  const int a = r < 0 ? -r : r;
+  return (uint32_t)(a < m ? a : m - 1);
+}
+
+// restir_di_spatial.wgsl:70-162 (one pass, unbiased combination).
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_di_spatial(RestirParams p, SceneView sc, RestirBuffers b) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  if (!load_vtx(b, pix, v)) {
+    b.di[pix] = di_empty();
+    return;
+  }
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  DiRes r = b.di_tmp[pix];
+  uint32_t rng = b.rng[pix];
+  const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+  const Tex ct = unpack_texel(b.gbuf[p.cur][pix]);
+  const float cd = depth_cs(ct.depth);
+  float radius = p.radius > 0.0f ? p.radius : (float)(p.width + p.rows) / 2.0f * 0.05f;
+  const uint32_t pass_seed = p.seed;  // seed * spatial_pass_count + pass index, one pass
+  const float r_off = ign_animated(x, y, pass_seed);
+  const uint32_t aseed = hash_combine(x >> 2, hash_combine(y >> 2, pass_seed));
+  const float a_off = (float)aseed * (1.0f / (float)0xFFFFFFFFu) * TWO_PI;
+  const uint32_t n = p.neighbours;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float ang = (float)i * GOLDEN_ANGLE + a_off;
+    const float rad = powf((float)i / (float)n, 0.5f) * radius + r_off;
+    const int ox = (int)(rad * cosf(ang)), oy = (int)(rad * sinf(ang));
+    const uint32_t nx = mirror_px((int)x + ox, (int)p.width), ny = mirror_px((int)y + oy, (int)p.rows);
+    const uint32_t q = ny * p.width + nx;
+    if (q == pix) continue;
+    const GTexel ng = b.gbuf[p.cur][q];
+    bool valid = ng.pos_depth.w != 0.0f;
+    if (valid) {
+      const Tex nt = unpack_texel(ng);
+      valid = fabsf(cd - depth_cs(nt.depth)) / cd < 0.1f && dot(ct.n, nt.n) > 0.906f;
+    }
+    if (valid) {
+      DiRes nr = b.di_tmp[q];
+      nr.phat = di_phat<W, S>(sc, st, v, bs, nr.light, nr.u, nr.v, true, b.counters);
+      valid = nr.phat > 0.0f;
+      // DiReservoir::combine_unbiased: Z counts the M of the domains where the
+      // chosen sample has a (visibility-tested) target > 0
+      DiRes c = di_merge(r, nr, rng);
+      float z = 0.0f;
+      if (di_phat<W, S>(sc, st, v, bs, c.light, c.u, c.v, true, b.counters) > 0.0f) z += r.M;
+      Vtx vn;
+      if (load_vtx(b, q, vn)) {
+        vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
+        const Bsdf bn = make_bsdf(sc, vn.mat, vn.n, vn.back, vn.t);
+        if (di_phat<W, S>(sc, st, vn, bn, c.light, c.u, c.v, true, b.counters) > 0.0f) z += nr.M;
+      }
+      if (c.phat > RESTIR_DI_EPSILON && z * c.wsum > RESTIR_DI_EPSILON) c.W = (1.0f / c.phat) * (1.0f / z * c.wsum);
+      r = c;
+    }
+    if (!valid) radius = fmaxf(radius * 0.5f, 3.0f);
+  }
+  b.di[pix] = r;
+  b.di_hist[p.cur][pix] = r;
+  b.rng[pix] = rng;
+}
+
+// restir_gi_temporal.wgsl:69-115 (M capped at gi_mcap, biased combination).
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_gi_temporal(RestirParams p, SceneView sc, RestirBuffers b) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  GiRes r;
+  r.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  r.mwsp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (!load_vtx(b, pix, v)) {
+    b.gi_hist[p.cur][pix] = r;
+    return;
+  }
+  r = b.gi[pix];
+  const float4 t4 = b.thr[pix];
+  if (t4.w > 0.0f) {
+    const LdsStack st{s_stack + threadIdx.x, BLOCK};
+    uint32_t rng = b.rng[pix];
+    float px, py;
+    if (reproject(p, v.x, px, py)) {
+      const float ux = rng_next(rng), uy = rng_next(rng);
+      const uint32_t qx = min((uint32_t)floorf(px + ux), p.width - 1);
+      const uint32_t qy = min((uint32_t)floorf(py + uy), p.rows - 1);
+      const Tex cur = unpack_texel(b.gbuf[p.cur][pix]);
+      const Tex prev = sample_prev(b.gbuf[p.cur ^ 1], p, px, py);
+      if (texel_similar(cur, prev)) {
+        const GiRes pr = b.gi_hist[p.cur ^ 1][qy * p.width + qx];
+        const float pM = fminf(pr.mwsp.x, (float)p.gi_mcap);
+        const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+        const float pp = gi_phat<W, S>(sc, st, v, bs, f4v(t4), pr, b.counters);
+        // GiReservoir::combine (gi_reservoir.wgsl:66-78)
+        float wsum = 0.0f, M = 0.0f, phat = 0.0f;
+        GiRes c;
+        c.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (res_update(wsum, M, r.mwsp.w * r.mwsp.y * r.mwsp.x, rng)) {
+          c.x2_rng = r.x2_rng;
+          phat = r.mwsp.w;
+        }
+        if (res_update(wsum, M, pp * pr.mwsp.y * pM, rng)) {
+          c.x2_rng = pr.x2_rng;
+          phat = pp;
+        }
+        M = r.mwsp.x + pM;
+        const float Wc = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
+        c.mwsp = make_float4(M, Wc, wsum, phat);
+        r = c;
+      }
+    }
+    b.rng[pix] = rng;
+  }
+  b.gi[pix] = r;
+  b.gi_hist[p.cur][pix] = r;
+}
+
+// apply_di.wgsl:48-111: the reservoir's sample, visibility tested, weighted by
+// W, with the throughput at the vertex.
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK) void k_rs_apply_di(RestirParams p, SceneView sc, RestirBuffers b) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  if (!load_vtx(b, pix, v)) return;
+  const DiRes r = b.di[pix];
+  if (r.W == 0.0f) return;
+  LightHit e;
+  if (!eval_light(sc, r.light, r.u, r.v, v.x, e)) return;
+  const float c = dot(e.wi, v.n);
+  if (!(c > 0.0f)) return;
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  if (!visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, b.counters)) return;
+  const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+  float pdf;
+  const v3 add = v.T * bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * (c * r.W);
+  if (!finite3(add)) return;
+  const float4 L = b.radiance[pix];
+  b.radiance[pix] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.0f);
+}
+
+// apply_gi.wgsl:44-99: the next ray heads for the GI sample point.
+__global__ __launch_bounds__(BLOCK) void k_rs_apply_gi(RestirParams p, SceneView sc, RestirBuffers b) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  if (!load_vtx(b, pix, v)) return;
+  const float4 t4 = b.thr[pix];
+  if (!(t4.w > 0.0f)) return;
+  const GiRes g = b.gi[pix];
+  if (g.mwsp.y > 0.0f) {
+    const v3 wi = normalize(f4v(g.x2_rng) - v.x);
+    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    float pdf;
+    const v3 T = f4v(t4) * bsdf_eval(bs, v.wo, wi, pdf) * (fabsf(dot(v.n, wi)) * g.mwsp.y);
+    const bool ok = finite3(T) && maxc(T) > 0.0f;
+    b.thr[pix] = v4(T, ok ? 1.0f : 0.0f);
+    b.ray_o[pix] = v4(v.x, 0.0f);
+    b.ray_d[pix] = v4(wi, 0.0f);
+  } else {
+    b.thr[pix].w = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernels: post chain
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_rs_demod(RestirParams p, RestirBuffers b) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const v3 alb = unpack_rgb9e5(b.gbuf[p.cur][pix].packed.z);
+  b.demod[p.cur][pix] = v4(safe_div3(f4v(b.radiance[pix]), alb), 0.0f);
+}
+
+// firefly_filter.wgsl: clamp to the luma range of the 3x3 neighbourhood
+// (out of place; the reference filters in place while neighbours read).
+__global__ __launch_bounds__(BLOCK) void k_rs_firefly(RestirParams p, RestirBuffers b, const float4* in,
+                                                      float4* out) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const float4 c4 = in[pix];
+  if (b.gbuf[p.cur][pix].pos_depth.w == 0.0f) {
+    out[pix] = c4;
+    return;
+  }
+  const v3 cur = f4v(c4);
+  float min_l = 3.402823466e38f, max_l = 0.0f;
+  v3 min_c = cur, max_c = mk(0.0f, 0.0f, 0.0f);
+  const float cl = luma(cur);
+  for (int dx = -1; dx <= 1; ++dx) {
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (dx == 0 && dy == 0) continue;
+      const int sx = (int)x + dx, sy = (int)y + dy;
+      if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
+      const uint32_t q = (uint32_t)sy * p.width + (uint32_t)sx;
+      if (b.gbuf[p.cur][q].pos_depth.w == 0.0f) continue;
+      const v3 nc = f4v(in[q]);
+      const float nl = luma(nc);
+      if (nl < min_l) {
+        min_l = nl;
+        min_c = nc;
+      } else if (nl > max_l) {
+        max_l = nl;
+        max_c = nc;
+      }
+    }
+  }
+  v3 rec = cur;
+  if (cl < min_l) rec = min_c;
+  else if (cl > max_l) rec = max_c;
+  if (min_l > max_l) rec = cur;
+  out[pix] = v4(rec, 0.0f);
+}
+
+// svgf_temporal.wgsl:87-139.
+__global__ __launch_bounds__(BLOCK) void k_svgf_temporal(RestirParams p, RestirBuffers b, const float4* in) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const float4 rad4 = in[pix];
+  const GTexel cg = b.gbuf[p.cur][pix];
+  if (cg.pos_depth.w == 0.0f) {
+    b.svgf_col[0][pix] = rad4;
+    b.svgf_var[0][pix] = 0.0f;
+    b.svgf_mom[p.cur][pix] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  const Tex cur = unpack_texel(cg);
+  const v3 rad = f4v(rad4);
+  const float l = luma(rad);
+  v3 tc = mk(0.0f, 0.0f, 0.0f);
+  float m1 = 0.0f, m2 = 0.0f;
+  uint32_t count = 0;
+  float px, py;
+  if (reproject(p, cur.pos, px, py)) {
+    const Tex prev = sample_prev(b.gbuf[p.cur ^ 1], p, px, py);
+    if (texel_similar(cur, prev)) {
+      const uint32_t ix = (uint32_t)floorf(px), iy = (uint32_t)floorf(py);
+      const float fx = px - floorf(px), fy = py - floorf(py);
+      const uint32_t x1 = min(ix + 1, p.width - 1), y1 = min(iy + 1, p.rows - 1);
+      const uint32_t i00 = iy * p.width + ix, i10 = iy * p.width + x1, i01 = y1 * p.width + ix, i11 = y1 * p.width + x1;
+      const float4* h = b.svgf_hist;
+      tc = lerp3(lerp3(f4v(h[i00]), f4v(h[i10]), fx), lerp3(f4v(h[i01]), f4v(h[i11]), fx), fy);
+      const float2* hm = b.svgf_mom[p.cur ^ 1];
+      m1 = lerpf(lerpf(hm[i00].x, hm[i10].x, fx), lerpf(hm[i01].x, hm[i11].x, fx), fy);
+      m2 = lerpf(lerpf(hm[i00].y, hm[i10].y, fx), lerpf(hm[i01].y, hm[i11].y, fx), fy);
+      count = min(b.svgf_count[pix] + 1, p.svgf_history);  // this pixel's count (:126)
+    }
+  }
+  const float alpha = 1.0f / (1.0f + (float)count);
+  m1 = lerpf(m1, l, alpha);
+  m2 = lerpf(m2, l * l, alpha);
+  tc = lerp3(tc, rad, alpha);
+  b.svgf_var[0][pix] = fmaxf(m2 - m1 * m1, 0.0f);
+  b.svgf_col[0][pix] = v4(tc, 0.0f);
+  b.svgf_mom[p.cur][pix] = make_float2(m1, m2);
+  b.svgf_count[pix] = count;
+}
+
+// svgf_atrous.wgsl:48-117: pass `pass` reads ping-pong slot pass & 1.  Sky
+// pixels pass through (the reference leaves them unwritten).
+__global__ __launch_bounds__(BLOCK) void k_svgf_atrous(RestirParams p, RestirBuffers b, uint32_t pass) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const uint32_t src = pass & 1u, dst = src ^ 1u;
+  const float4 c4 = b.svgf_col[src][pix];
+  const GTexel cg = b.gbuf[p.cur][pix];
+  if (cg.pos_depth.w == 0.0f) {
+    b.svgf_col[dst][pix] = c4;
+    b.svgf_var[dst][pix] = 0.0f;
+    if (pass == 0) b.svgf_hist[pix] = c4;
+    return;
+  }
+  const Tex ct = unpack_texel(cg);
+  const float KW[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};  // svgf_atrous.wgsl:11-15
+  const int step = 1 << pass;
+  v3 sum = f4v(c4);
+  float vsum = b.svgf_var[src][pix];
+  float wsum = 1.0f;
+  const float cl = luma(sum);
+  const float cv = vsum;
+  for (int dy = -2; dy <= 2; ++dy) {
+    for (int dx = -2; dx <= 2; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const int sx = (int)x + dx * step, sy = (int)y + dy * step;
+      if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
+      const uint32_t q = (uint32_t)sy * p.width + (uint32_t)sx;
+      const GTexel g = b.gbuf[p.cur][q];
+      if (g.pos_depth.w == 0.0f) continue;
+      const Tex tq = unpack_texel(g);
+      const v3 cq = f4v(b.svgf_col[src][q]);
+      const float wn = powf(fmaxf(dot(ct.n, tq.n), 0.0f), 128.0f);  // PHI_NORMAL
+      const float dd = fabsf(ct.depth - tq.depth);
+      const float wd = expf(-(dd * dd) / (2.0f * (0.1f * 0.1f)));  // PHI_DEPTH
+      const float wl = expf(-fabsf(cl - luma(cq)) / fmaxf(4.0f * sqrtf(cv), 1e-8f));  // PHI_LUMA
+      float w = wn * wd * wl;
+      w *= KW[dx < 0 ? -dx : dx] * KW[dy < 0 ? -dy : dy];
+      sum += cq * w;
+      vsum += b.svgf_var[src][q] * (w * w);
+      wsum += w;
+    }
+  }
+  sum = sum / wsum;
+  vsum = vsum / (wsum * wsum);
+  b.svgf_col[dst][pix] = v4(sum, 0.0f);
+  b.svgf_var[dst][pix] = vsum;
+  if (pass == 0) b.svgf_hist[pix] = v4(sum, 0.0f);
+}
+
+// taa.wgsl: Mitchell-weighted neighbourhood, history clipped to the
+// neighbourhood's mean +- deviation (out of place).
+__device__ __forceinline__ float mitchell_1d(float x0, float B, float C) {
+  const float x = fabsf(2.0f * x0);
+  const float k = 1.0f / 6.0f;
+  if (x > 1.0f)
+    return ((-B - 6.0f * C) * x * x * x + (6.0f * B + 30.0f * C) * x * x + (-12.0f * B - 48.0f * C) * x +
+            (8.0f * B + 24.0f * C)) * k;
+  return ((12.0f - 9.0f * B - 6.0f * C) * x * x * x + (-18.0f + 12.0f * B + 6.0f * C) * x * x + (6.0f - 2.0f * B)) * k;
+}
+
+__device__ __forceinline__ v3 clip_aabb(v3 lo, v3 hi, v3 h) {
+  const v3 c = (hi + lo) * 0.5f;
+  const v3 e = (hi - lo) * 0.5f;
+  const v3 r = h - c;
+  const v3 ru = mk(fabsf(r.x / e.x), fabsf(r.y / e.y), fabsf(r.z / e.z));
+  const float m = fmaxf(ru.x, fmaxf(ru.y, ru.z));
+  return m > 1.0f ? c + r / m : h;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rs_taa(RestirParams p, RestirBuffers b, const float4* in,
+                                                  const float4* prev, float4* out) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const GTexel cg = b.gbuf[p.cur][pix];
+  if (cg.pos_depth.w == 0.0f) {
+    out[pix] = in[pix];
+    return;
+  }
+  const v3 cur = f4v(in[pix]);
+  const float m0 = mitchell_1d(0.0f, 0.33f, 0.33f);
+  float wsum = m0 * m0;
+  v3 rec = cur * wsum;
+  v3 fm = cur, sm = cur * cur;
+  float n = 1.0f;
+  for (int dx = -1; dx <= 1; ++dx) {
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (dx == 0 && dy == 0) continue;
+      const int sx = (int)x + dx, sy = (int)y + dy;
+      if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
+      const float4 q4 = in[(uint32_t)sy * p.width + (uint32_t)sx];
+      const v3 sc = mk(fmaxf(q4.x, 0.0f), fmaxf(q4.y, 0.0f), fmaxf(q4.z, 0.0f));
+      float w = mitchell_1d((float)dx, 0.33f, 0.33f) * mitchell_1d((float)dy, 0.33f, 0.33f);
+      w *= 1.0f / (1.0f + luma(sc));
+      rec += sc * w;
+      wsum += w;
+      fm += sc;
+      sm += sc * sc;
+      n += 1.0f;
+    }
+  }
+  rec = rec / fmaxf(wsum, 1e-5f);
+  v3 hist;
+  uint32_t prev_id;
+  float px, py;
+  const Tex ct = unpack_texel(cg);
+  if (reproject(p, ct.pos, px, py)) {
+    const uint32_t fx0 = min((uint32_t)floorf(px), p.width - 1), fy0 = min((uint32_t)floorf(py), p.rows - 1);
+    const uint32_t cx = min((uint32_t)ceilf(px), p.width - 1), cy = min((uint32_t)ceilf(py), p.rows - 1);
+    prev_id = (uint32_t)floorf(py) * p.width + (uint32_t)floorf(px);
+    // the reference's history00/10/01/11 index rows by the first letter
+    const v3 h00 = f4v(prev[fy0 * p.width + fx0]), h10 = f4v(prev[cy * p.width + fx0]);
+    const v3 h01 = f4v(prev[fy0 * p.width + cx]), h11 = f4v(prev[cy * p.width + cx]);
+    const float tx = px - floorf(px), ty = py - floorf(py);
+    const v3 a = h00 * (1.0f - tx) + h10 * tx;  // bilinear(tx, ty, c00, c10, c01, c11) (taa.wgsl:48-52)
+    const v3 c = h01 * (1.0f - tx) + h11 * tx;
+    hist = a * (1.0f - ty) + c * ty;
+  } else {
+    hist = f4v(prev[pix]);
+    prev_id = pix;
+  }
+  const Tex pt = unpack_texel(b.gbuf[p.cur ^ 1][prev_id]);
+  const float cdc = depth_cs(ct.depth), pdc = depth_cs(pt.depth);
+  const bool valid_depth = fabsf(cdc - pdc) / cdc < 0.1f;
+  const v3 mean = fm / n;
+  v3 sd = mk(fabsf(sm.x - fm.x * fm.x / n), fabsf(sm.y - fm.y * fm.y / n), fabsf(sm.z - fm.z * fm.z / n));
+  sd = sd / (n - 1.0f);
+  sd = mk(sqrtf(sd.x), sqrtf(sd.y), sqrtf(sd.z));
+  const v3 clipped = clip_aabb(mean - sd, mean + sd, hist);
+  const float blend = 1.0f - 0.8f * (valid_depth ? 1.0f : 0.0f);  // history_influence 0.8 (lib.rs:537)
+  const float cw = clampf(blend * (1.0f / (1.0f + luma(rec))), 0.0f, 1.0f);
+  const float hw = clampf((1.0f - blend) * (1.0f / (1.0f + luma(clipped))), 0.0f, 1.0f);
+  out[pix] = v4((rec * cw + clipped * hw) / (cw + hw), 0.0f);
+}
+
+// Remodulation (demodulate_radiance.wgsl with remodulate) into the frame sum;
+// the final demodulated image stays in demod[cur] for the next frame's TAA.
+__global__ __launch_bounds__(BLOCK) void k_rs_remod(RestirParams p, RestirBuffers b, const float4* fin,
+                                                    float4* frame_sum) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const float4 c = fin[pix];
+  const v3 alb = unpack_rgb9e5(b.gbuf[p.cur][pix].packed.z);
+  const v3 r = f4v(c) * alb;
+  const float4 s = frame_sum[pix];
+  frame_sum[pix] = make_float4(s.x + r.x, s.y + r.y, s.z + r.z, s.w);
+  if (fin != b.demod[p.cur]) b.demod[p.cur][pix] = c;
+}
+
+template <int W, int S>
+apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
+  const uint32_t tiles = ((p.width + TILE - 1) / TILE) * ((p.rows + TILE - 1) / TILE);
+  const dim3 grid(tiles), block(BLOCK);
+  hipLaunchKernelGGL(k_rs_raygen, grid, block, 0, s, p, sc, b);
+  for (uint32_t bounce = 0; bounce < p.max_bounces; ++bounce) {
+    const bool more = bounce + 1 < p.max_bounces;
+    hipLaunchKernelGGL((k_rs_extend<W, S>), grid, block, 0, s, p, sc, b, bounce);
+    hipLaunchKernelGGL((k_rs_di_ris<W, S>), grid, block, 0, s, p, sc, b);
+    if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S>), grid, block, 0, s, p, sc, b, bounce);
+    if (bounce == 0 && p.restir_di) {
+      hipLaunchKernelGGL((k_rs_di_temporal<W, S>), grid, block, 0, s, p, sc, b);
+      if (p.neighbours > 0) hipLaunchKernelGGL((k_rs_di_spatial<W, S>), grid, block, 0, s, p, sc, b);
+    }
+    if (bounce == 0 && more && p.restir_gi)
+      hipLaunchKernelGGL((k_rs_gi_temporal<W, S>), grid, block, 0, s, p, sc, b);
+    hipLaunchKernelGGL((k_rs_apply_di<W, S>), grid, block, 0, s, p, sc, b);
+    if (more) hipLaunchKernelGGL(k_rs_apply_gi, grid, block, 0, s, p, sc, b);
+  }
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+}  // namespace
+
+apt_status restir_sample(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
+  switch (sc.width) {
+    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_>(p, sc, b, s)));
+    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_>(p, sc, b, s)));
+    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_>(p, sc, b, s)));
+    default: break;
+  }
+  set_error("restir: unsupported BVH width %u", sc.width);
+  return APT_ERR_INVALID_ARG;
+}
+
+apt_status restir_post(const RestirParams& p, const RestirBuffers& b, bool firefly, bool svgf, bool taa,
+                       float4* frame_sum, hipStream_t s) {
+  const uint32_t tiles = ((p.width + TILE - 1) / TILE) * ((p.rows + TILE - 1) / TILE);
+  const dim3 grid(tiles), block(BLOCK);
+  hipLaunchKernelGGL(k_rs_demod, grid, block, 0, s, p, b);
+  const float4* img = b.demod[p.cur];
+  if (firefly) {
+    hipLaunchKernelGGL(k_rs_firefly, grid, block, 0, s, p, b, img, b.work);
+    img = b.work;
+  }
+  if (svgf) {
+    hipLaunchKernelGGL(k_svgf_temporal, grid, block, 0, s, p, b, img);
+    for (uint32_t i = 0; i < p.atrous_passes; ++i) hipLaunchKernelGGL(k_svgf_atrous, grid, block, 0, s, p, b, i);
+    img = b.svgf_col[p.atrous_passes & 1u];
+  }
+  if (taa && !p.first_frame) {
+    float4* out = img == b.work ? b.demod[p.cur] : b.work;
+    hipLaunchKernelGGL(k_rs_taa, grid, block, 0, s, p, b, img, b.demod[p.cur ^ 1], out);
+    img = out;
+  }
+  hipLaunchKernelGGL(k_rs_remod, grid, block, 0, s, p, b, img, frame_sum);
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+}  // namespace apt

@@ -39,6 +39,9 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
   for (uint32_t w : widths) {
     apt_config cfg;
     apt_config_default(&cfg);
+    // the wavefront path tracer: the candidates schedule its passes
+    cfg.integrator = APT_INTEGRATOR_PLAIN;
+    cfg.restir_di = cfg.restir_gi = cfg.firefly_filter = cfg.svgf = cfg.taa = 0;
     cfg.bvh_width = w;
     cfg.sample_count = o.probe_spp;
     cfg.max_bounces = o.probe_bounces;

@@ -31,6 +31,18 @@ def default_config(**overrides) -> Config:
     return c
 
 
+INTEGRATOR_PLAIN = 0
+INTEGRATOR_RESTIR = 1
+
+
+def plain_config(**overrides) -> Config:
+    """default_config with the wavefront path tracer (APT_INTEGRATOR_PLAIN)
+    and the ReSTIR switches and post chain off."""
+    kw = dict(integrator=INTEGRATOR_PLAIN, restir_di=0, restir_gi=0, firefly_filter=0, svgf=0, taa=0)
+    kw.update(overrides)
+    return default_config(**kw)
+
+
 class PathTracerGpu:
     def __init__(self, config: Config | None = None, **overrides):
         cfg = config if config is not None else default_config(**overrides)
@@ -98,6 +110,15 @@ class PathTracerGpu:
                                vptr(rgba) if with_rgba else None, rgba.size if with_rgba else 0,
                                vptr(acc), acc.size), "apt_render")
         return rgba, acc
+
+    def render_rgba(self, resolution, start_row: int = 0, end_row: int | None = None) -> np.ndarray:
+        """Render; return only the RGBA8 block-layout bytes (the render-node output)."""
+        w, h = int(resolution[0]), int(resolution[1])
+        end_row = h if end_row is None else end_row
+        rgba = np.empty(w * (end_row - start_row) * 4, np.uint8)
+        check(lib().apt_render(self._h, w, h, start_row, end_row, vptr(rgba), rgba.size, None, 0),
+              "apt_render")
+        return rgba
 
     def stats(self) -> dict:
         s = RenderStats()

@@ -30,6 +30,21 @@ HBM_PEAK_GBS = 8000.0
 COUNTS = ROOT / "tests" / "golden" / "bvh_stress_counts.json"
 
 
+COPY_BYTES = 256 << 20
+TRAFFIC = ROOT / "profiles" / "trace_closest_bvh_stress_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per closest-hit launch from the committed FETCH_SIZE/WRITE_SIZE
+    counter summary (already scaled by the copy-kernel calibration), or None."""
+    import re
+
+    if not TRAFFIC.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC.read_text())
+    return float(m.group(1)) if m else None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -72,10 +87,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--lds-stack", type=int, default=8)
-    ap.add_argument("--workload", default="bvh_stress", choices=["bvh_stress", "disney", "serving"])
+    ap.add_argument("--workload", default="bvh_stress", choices=["bvh_stress", "disney", "restir", "serving"])
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--rates", default="20,200")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device-to-device copy before timing (the known-size "
+                         "kernel that scales the HBM counters of a --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -86,6 +104,8 @@ def main() -> int:
         return bench_disney(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "restir":
+        return bench_restir(args)
 
     import numpy as np
     import torch
@@ -110,6 +130,13 @@ def main() -> int:
     # libapt would replace by its own stream.
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         bvh.trace_closest_device(d_rays, n, d_hits, stream)
@@ -164,7 +191,7 @@ def main() -> int:
                    "lds_stack": args.lds_stack,
                    "bvh_nodes": st["n_nodes"], "bvh_build_ms": round(st["build_ms"], 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "alg_bytes_per_ray": bpr, "kernel_avg_ms": avg_ms},
         "closest_hits": n_hit,
         "any_hit_mrays_s": n / (any_ms * 1e-3) / 1e6,
@@ -201,7 +228,7 @@ def cpu_baseline_render(scene, width, height, spp, bounces, rr):
 def bench_disney(args) -> int:
     import torch
 
-    from appearance_amd.renderer import PathTracerGpu, default_config
+    from appearance_amd.renderer import PathTracerGpu, plain_config
     from appearance_amd.scene import generate_scene
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -209,7 +236,7 @@ def bench_disney(args) -> int:
     W, H, SPP, DEPTH, RR = 1920, 1088, 4, 8, 3  # 1088: slab heights are multiples of 64
     log("generating disney scene")
     scene = generate_scene("disney", seed=3)
-    pt = PathTracerGpu(default_config(sample_count=SPP, max_bounces=DEPTH, rr_start_bounce=RR,
+    pt = PathTracerGpu(plain_config(sample_count=SPP, max_bounces=DEPTH, rr_start_bounce=RR,
                                       bvh_width=args.width, lds_stack_depth=args.lds_stack))
     pt.set_scene(scene)
     pt.set_camera(scene.camera)
@@ -243,6 +270,94 @@ def bench_disney(args) -> int:
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle path tracer on host cores)")
         out["cpu_baseline"] = cpu_baseline_render(scene, W, H, SPP, DEPTH, RR)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def restir_orbit(k, frames=60):
+    """Camera k of the 60-frame orbit around the many-light scene."""
+    import math
+
+    from tests.scenes_util import camera
+
+    a = 2.0 * math.pi * (k % frames) / 240.0
+    return camera((13.5 * math.sin(a), 7.5, 0.5 + 13.5 * math.cos(a)), (0.0, 0.0, 0.5), 40.0)
+
+
+def restir_config(**kw):
+    from appearance_amd.renderer import default_config
+
+    return default_config(di_candidates=32, di_mcap=20, di_spatial_neighbours=5, di_spatial_radius=30.0,
+                          restir_di=1, restir_gi=1, svgf=1, svgf_iterations=5, svgf_history=4, **kw)
+
+
+def cpu_baseline_restir(scene, frames=3, w=480, h=270):
+    """The oracle's ReSTIR integrator on the host cores on a bounded sample:
+    a few frames of the orbit at 1/16 of the 1080p pixel count."""
+    from tests._oracle import OracleRestir, load_oracle
+
+    o = load_oracle()
+    threads = max(1, os.cpu_count() or 1)
+    r = OracleRestir(o, scene, w, h)
+    cfg = restir_config()
+    r.frame(cfg, camera=restir_orbit(0), frame_idx=0, threads=threads)  # warm-up, builds history
+    t0 = time.time()
+    for f in range(1, frames + 1):
+        r.frame(cfg, camera=restir_orbit(f), frame_idx=f, threads=threads)
+    t = time.time() - t0
+    return {"value": w * h * frames / t / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} frames of the orbit at {w}x{h} (1/16 of the 1080p pixels), 1 spp, "
+                      f"same settings, FP32 oracle ReSTIR integrator"}
+
+
+def bench_restir(args) -> int:
+    import torch
+
+    from appearance_amd.renderer import PathTracerGpu
+    from appearance_amd.scene import generate_scene
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    W, H = 1920, 1088  # RGBA8 block layout: slab heights are multiples of 64
+    log("generating the many-light scene")
+    scene = generate_scene("restir", seed=3)
+    pt = PathTracerGpu(restir_config(bvh_width=args.width, lds_stack_depth=args.lds_stack))
+    pt.set_scene(scene)
+    k = 0
+    for _ in range(max(1, args.warmup)):
+        pt.set_camera(restir_orbit(k))
+        pt.render_rgba((W, H))
+        k += 1
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} frames of the 60-frame orbit")
+    dev_ms, rays = [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pt.set_camera(restir_orbit(k))
+        pt.render_rgba((W, H))
+        k += 1
+        st = pt.stats()
+        dev_ms.append(st["ms_total"])
+        rays += st["extension_rays"] + st["shadow_rays"]
+    wall = time.perf_counter() - t0
+    out = {
+        "metric": METRIC,
+        "value": W * H * args.steps / wall / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"restir: disney scene + 4096 emissive triangles (528,386 triangles), {W}x{H}, "
+                               "1 spp per frame, camera orbit, ReSTIR DI (32 candidates, M cap 20, 5 taps in "
+                               "30 px) + GI reuse, depth 2, firefly filter, SVGF 5 passes / 4 frames",
+                   "bvh_width": args.width, "lds_stack": args.lds_stack},
+        "mrays_s": rays / wall / 1e6,
+        "device_ms_per_frame": sum(dev_ms) / len(dev_ms),
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle ReSTIR integrator on host cores)")
+        out["cpu_baseline"] = cpu_baseline_restir(scene)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -289,7 +404,7 @@ def bench_serving(args) -> int:
     import numpy as np
     import torch
 
-    from appearance_amd.renderer import PathTracerGpu, default_config
+    from appearance_amd.renderer import PathTracerGpu, plain_config
     from appearance_amd.scene import generate_scene
     from appearance_amd.serving import DONE, Server
 
@@ -304,7 +419,7 @@ def bench_serving(args) -> int:
     ids = [srv.add_scene(sc) for sc in scenes]
     plain = []
     for sc in scenes:
-        pt = PathTracerGpu(default_config())
+        pt = PathTracerGpu(plain_config())
         pt.set_scene(sc)
         plain.append(pt)
     report = {"rates": {}}

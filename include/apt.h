@@ -239,7 +239,7 @@ apt_status apt_scene_data_view(const apt_scene_data* s, apt_mesh_desc* mesh, apt
                                apt_sky* sky);
 void apt_scene_data_destroy(apt_scene_data* s);
 
-/* The C1 ray sets: n_coherent primary rays of a side x side pinhole camera at
+/* The bvh_stress ray sets: n_coherent primary rays of a side x side pinhole camera at
  * (0,0,-25) looking +z with a 40 degree fov (side*side == n_coherent), then
  * n_incoherent rays with origins uniform in [-10,10]^3 and uniform-sphere
  * directions, all with tmin 0 and tmax 1e30. */
@@ -250,8 +250,14 @@ apt_status apt_rays_bvh_stress(uint32_t seed, uint32_t side, uint32_t n_incohere
  * ------------------------------------------------------------------------ */
 
 typedef enum apt_integrator {
-  APT_INTEGRATOR_PLAIN = 0,  /* unidirectional PT, NEE + MIS, Russian roulette */
-  APT_INTEGRATOR_RESTIR = 1  /* ReSTIR DI/GI (not yet available) */
+  /* Wavefront unidirectional path tracer: jittered samples, NEE + MIS,
+   * Russian roulette, many samples per call; no G-buffer, so no post chain. */
+  APT_INTEGRATOR_PLAIN = 0,
+  /* The reference's integrator (src/lib.rs:354-576): pixel-centre samples,
+   * DI by RIS over light + BSDF candidates, the continuation chosen by a GI
+   * reservoir, spatiotemporal reuse at the first vertex (restir_di,
+   * restir_gi), then demodulation, firefly filter, SVGF and TAA. */
+  APT_INTEGRATOR_RESTIR = 1
 } apt_integrator;
 
 /* PathTracerGpuConfig (src/lib.rs:219-245), made public, plus the knobs the
@@ -260,29 +266,32 @@ typedef struct apt_config {
   uint32_t max_bounces;       /* path segments traced per sample */
   uint32_t sample_count;      /* samples per pixel per render call */
   uint32_t accum_frames;      /* progressive accumulation across calls */
-  uint32_t rr_start_bounce;   /* Russian roulette from this bounce on */
+  uint32_t rr_start_bounce;   /* PLAIN: Russian roulette from this bounce on (RESTIR: bounce 2, trace.wgsl:220) */
   uint32_t integrator;        /* apt_integrator */
-  uint32_t restir_di;
-  uint32_t restir_gi;
-  uint32_t svgf;
-  uint32_t firefly_filter;
-  uint32_t taa;
-  uint32_t di_candidates;     /* light candidates per pixel (RIS) */
-  uint32_t di_mcap;           /* temporal M cap */
-  uint32_t di_spatial_neighbours;
-  float di_spatial_radius;    /* pixels */
-  uint32_t gi_mcap;
-  uint32_t svgf_iterations;
-  uint32_t svgf_history;
+  uint32_t restir_di;         /* RESTIR: temporal + spatial reuse of DI reservoirs */
+  uint32_t restir_gi;         /* RESTIR: temporal reuse of GI reservoirs */
+  uint32_t svgf;              /* RESTIR: SVGF denoiser */
+  uint32_t firefly_filter;    /* RESTIR: 3x3 luma clamp before SVGF */
+  uint32_t taa;               /* RESTIR: temporal antialiasing */
+  uint32_t di_candidates;     /* RESTIR: light candidates of the DI RIS (plus one BSDF candidate), <= 64 */
+  uint32_t di_mcap;           /* RESTIR: temporal M cap, times the current M */
+  uint32_t di_spatial_neighbours; /* RESTIR: spatial taps, <= 16 */
+  float di_spatial_radius;    /* RESTIR: spatial radius in pixels (0 = 5% of (width + rows) / 2) */
+  uint32_t gi_mcap;           /* RESTIR: GI temporal M cap */
+  uint32_t svgf_iterations;   /* RESTIR: a-trous passes, <= 8 */
+  uint32_t svgf_history;      /* RESTIR: max frames of SVGF moment history */
   uint32_t bvh_width;         /* 2, 4, 8 */
-  uint32_t queue_chunk;       /* paths per wavefront queue chunk (0 = whole frame) */
+  uint32_t queue_chunk;       /* PLAIN: paths per wavefront pass (0 = 2^21) */
   uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS: 8, 16, 24 */
   uint32_t seed;              /* base seed mixed into every path seed */
   uint32_t frame_idx;         /* starting frame index */
 } apt_config;
 
-/* PathTracerGpuConfig::default (src/lib.rs:230-245) for the switches; the
- * plain integrator; MI355X tuning defaults. */
+/* PathTracerGpuConfig::default (src/lib.rs:230-245): the RESTIR integrator,
+ * 2 bounces, 1 sample, no accumulation, restir_di, restir_gi and the firefly
+ * filter on, SVGF and TAA off; the shader constants of the reference
+ * (4 light candidates, M caps 20 and 30, 3 spatial taps, 5 a-trous passes,
+ * 24 history frames) and MI355X tuning defaults. */
 void apt_config_default(apt_config* cfg);
 
 typedef struct apt_renderer apt_renderer;

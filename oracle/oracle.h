@@ -27,9 +27,17 @@ void or_brute(const float* tris, uint32_t ntris, const float* rays, uint32_t n, 
 uint32_t or_block_index(uint32_t x, uint32_t y, uint32_t width);
 float or_aces(float x);
 int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
-              uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg9,
+              uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
               const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
               uint64_t* counts);
+void* or_restir_create(uint32_t width, uint32_t rows);
+void or_restir_destroy(void* st);
+int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
+                    uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg20,
+                    int threads, float* out, uint64_t* rays);
+void or_f16_round(const float* in, uint32_t n, float* out);
+void or_atrous_probe(uint32_t width, uint32_t height, uint32_t pass, const float* col, const float* var,
+                     const float* depth, const float* nrm, float* out);
 double or_disney_eval(const void* mat, const double* n, const double* wo, const double* wi, int back,
                       double* f);
 double or_disney_sample(const void* mat, const double* n, const double* wo, int back, double r0, double r1,

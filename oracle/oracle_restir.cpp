@@ -1,0 +1,1279 @@
+// ORACLE - test infrastructure only (see oracle_core.cpp).
+//
+// CPU restatement of libapt's ReSTIR integrator and post chain, pass by pass
+// over the whole slab, in FP32 with the device code's draw order, epsilons and
+// rounding points (FP16 G-buffer depth and normal, rgb9e5 albedo).  It follows
+// the reference's frame (crates/appearance-path-tracer-gpu/src/lib.rs:354-576)
+// and shaders: trace.wgsl, helpers/nee.wgsl (Nee::sample_ris,
+// LightSample::load_eval_data), helpers/inline_path_tracer.wgsl,
+// shared/restir/{di,gi}_reservoir.wgsl, restir_di_temporal.wgsl,
+// restir_di_spatial.wgsl, restir_gi_temporal.wgsl, apply_di.wgsl,
+// apply_gi.wgsl, shared/gbuffer_bindings.wgsl, demodulate_radiance.wgsl,
+// firefly_filter.wgsl, svgf_temporal.wgsl, svgf_atrous.wgsl, taa.wgsl; with
+// the same documented deviations as the product (solid-angle light pdfs and
+// plain emission, balance-heuristic MIS on each candidate's own pdfs, uniform
+// barycentrics, one-sided emitters, the GI target floor, min(1, max T)
+// roulette that keeps the vertex's DI, out-of-place filters, clamped
+// neighbour indices).
+//
+// Pins (tests/test_oracle_restir.py): with reuse off, the frame mean over many
+// seeds equals the plain oracle path tracer (itself pinned by the furnace and
+// FP64 checks) at pixel centres; reservoir update/combine closed forms
+// (single candidate W = 1/p); M caps; SVGF keeps a constant image constant,
+// reproduces a linear ramp inside a flat region, and its first pass equals a
+// direct evaluation of the 5x5 edge-stopping sum; the FP16 conversion equals
+// numpy's.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oracle_pt.h"
+
+namespace orc {
+uint32_t hash_combine(uint32_t x, uint32_t y);
+uint32_t rgb9e5_pack(const float* rgb);
+void rgb9e5_unpack(uint32_t d, float* rgb);
+}  // namespace orc
+
+namespace ors {
+
+using ocr::Cam;
+using ocr::Mat;
+using ocr::Sky;
+using F = float;
+using V3 = ocr::V<float>;
+using ocr::dotp;
+using ocr::crossp;
+using ocr::len;
+using ocr::unit;
+using ocr::mx;
+using ocr::f3;
+using ocr::lerp;
+
+constexpr F PI = 3.14159265358979323846f;
+constexpr F TWO_PI = 2.0f * PI;
+constexpr F EPS = 1e-4f;                  // TRACE_EPSILON
+constexpr F SUN_DISTANCE = 1e6f;
+constexpr F RESTIR_DI_EPSILON = 1e-6f;
+constexpr F GOLDEN_ANGLE = 2.39996322972865332f;
+constexpr F GI_SKY_DISTANCE = 1000.0f;
+constexpr F GI_OFFSET = 1e-4f;
+constexpr F GI_PHAT_FLOOR = 1e-3f;
+constexpr uint32_t LIGHT_SUN = 0xFFFFFFFEu, LIGHT_EMPTY = 0xFFFFFFFFu, NONE = 0xFFFFFFFFu;
+
+// ---- IEEE binary16 round trip (round to nearest even) -----------------------
+uint16_t f2h(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u));
+  if (ax < 0x33000000u) return (uint16_t)sign;  // below 2^-25: rounds to zero
+  const int e = (int)(ax >> 23) - 127;
+  const uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+  if (e < -14) {  // half subnormal
+    const int shift = -14 - e + 13;
+    uint32_t r = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (r & 1u))) r++;
+    return (uint16_t)(sign | r);
+  }
+  uint32_t r = ((uint32_t)(e + 15) << 10) | ((m >> 13) & 0x3FFu);
+  const uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) r++;  // a carry moves into the exponent
+  if (r >= 0x7C00u) r = 0x7C00u;
+  return (uint16_t)(sign | r);
+}
+
+float h2f(uint16_t h) {
+  const uint32_t s = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  if (e == 0) {
+    const float v = std::ldexp((float)m, -24);
+    return s ? -v : v;
+  }
+  uint32_t x;
+  if (e == 31) x = s | 0x7F800000u | (m << 13);
+  else x = s | ((e - 15 + 127) << 23) | (m << 13);
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+F r16(F v) { return h2f(f2h(v)); }
+V3 rgb9e5_round(const V3& c) {
+  float in[3] = {c.x, c.y, c.z}, out[3];
+  orc::rgb9e5_unpack(orc::rgb9e5_pack(in), out);
+  return V3(out[0], out[1], out[2]);
+}
+
+F luma(const V3& c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+V3 lerp3(const V3& a, const V3& b, F t) { return a * (1.0f - t) + b * t; }
+bool finite3(const V3& a) { return a.x == a.x && a.y == a.y && a.z == a.z; }
+F safe_div(F a, F b) { return b > 0.0f ? a / b : 0.0f; }
+F fractf(F a) { return a - std::floor(a); }
+V3 divs(const V3& a, F d) { return V3(a.x / d, a.y / d, a.z / d); }
+
+struct Scene {
+  ocr::Scene s;
+  std::vector<uint32_t> emissive;
+  uint32_t ne = 0;
+  F p_sun = 0;
+};
+
+struct CamV {
+  V3 pos, right, up, back;
+  F tan_half, aspect;
+};
+
+CamV cam_view(const Cam& c, uint32_t width, uint32_t height) {
+  CamV v;
+  v.right = unit(V3(c.m[0], c.m[1], c.m[2]));
+  v.up = unit(V3(c.m[4], c.m[5], c.m[6]));
+  v.back = unit(V3(c.m[8], c.m[9], c.m[10]));
+  v.pos = V3(c.m[12], c.m[13], c.m[14]);
+  v.tan_half = std::tan(0.5f * c.fov_deg * PI / 180.0f);
+  v.aspect = (F)width / (F)height;
+  return v;
+}
+
+struct P {
+  uint32_t width, rows, row_start, height, spp, max_bounces, seed_base, frame_idx;
+  uint32_t di_candidates, di_mcap, neighbours, gi_mcap, restir_di, restir_gi;
+  uint32_t firefly, svgf, taa, svgf_history, atrous;
+  F radius;
+  uint32_t seed, cur, first_frame;
+  CamV cam, prev_cam;
+};
+
+struct Tex {
+  V3 pos;
+  F depth32;
+  F depth;  // FP16-rounded
+  V3 n;     // FP16-rounded
+  V3 albedo;
+  uint32_t matbits;
+};
+
+struct DiRes {
+  uint32_t light = LIGHT_EMPTY;
+  F u = 0, v = 0, wsum = 0, M = 0, W = 0, phat = 0;
+};
+
+struct GiRes {
+  V3 x2;
+  uint32_t rng = 0;
+  F M = 0, W = 0, wsum = 0, phat = 0;
+};
+
+struct Vtx {
+  V3 x, n, wo, T;
+  F t = -1;
+  uint32_t mat = 0;
+  bool back = false;
+};
+
+struct State {
+  uint32_t npix = 0;
+  std::vector<V3> ray_o, ray_d, thr_T, radiance;
+  std::vector<uint8_t> alive;
+  std::vector<uint32_t> rng;
+  std::vector<Vtx> vtx;
+  std::vector<DiRes> di, di_tmp, di_hist[2];
+  std::vector<GiRes> gi, gi_hist[2];
+  std::vector<Tex> gbuf[2];
+  std::vector<V3> demod[2], work, col[2], hist;
+  std::vector<F> var[2];
+  std::vector<std::pair<F, F>> mom[2];
+  std::vector<uint32_t> count;
+  uint32_t cur = 0;
+  bool history = false;
+  CamV prev_cam{};
+  uint64_t rays[2] = {0, 0};
+
+  explicit State(uint32_t n) : npix(n) {
+    ray_o.assign(n, V3()); ray_d.assign(n, V3()); thr_T.assign(n, V3()); radiance.assign(n, V3());
+    alive.assign(n, 0); rng.assign(n, 0); vtx.assign(n, Vtx());
+    di.assign(n, DiRes()); di_tmp.assign(n, DiRes());
+    gi.assign(n, GiRes());
+    for (int k = 0; k < 2; ++k) {
+      di_hist[k].assign(n, DiRes());
+      gi_hist[k].assign(n, GiRes());
+      gbuf[k].assign(n, Tex{V3(), 0, 0, V3(), V3(), 0});
+      demod[k].assign(n, V3());
+      col[k].assign(n, V3());
+      var[k].assign(n, 0.0f);
+      mom[k].assign(n, {0.0f, 0.0f});
+    }
+    work.assign(n, V3()); hist.assign(n, V3()); count.assign(n, 0);
+  }
+};
+
+// ---- parallel pixel loop ------------------------------------------------------
+template <typename Fn>
+void each_pixel(uint32_t n, int threads, Fn&& fn) {
+  std::atomic<uint32_t> next{0};
+  auto work = [&]() {
+    while (true) {
+      const uint32_t b = next.fetch_add(1024);
+      if (b >= n) break;
+      for (uint32_t i = b; i < std::min(n, b + 1024); ++i) fn(i);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+}
+
+// ---- geometry, BSDF, lights ---------------------------------------------------
+bool closest(const Scene& sc, const V3& o, const V3& d, F& t, F& u, F& v, uint32_t& prim, std::atomic<uint64_t>& c) {
+  c.fetch_add(1, std::memory_order_relaxed);
+  return ocr::closest<F>(sc.s, o, d, t, u, v, prim);
+}
+
+bool visible(const Scene& sc, const V3& x, const V3& n, const V3& wi, F dist, std::atomic<uint64_t>& c) {
+  c.fetch_add(1, std::memory_order_relaxed);
+  return !ocr::occluded<F>(sc.s, x + n * EPS, wi, dist - 2.0f * EPS);
+}
+
+V3 shading_normal(const Scene& sc, uint32_t prim, F bu, F bv, V3& ng) {
+  V3 a, b, c;
+  ocr::tri_pts<F>(sc.s, prim, a, b, c);
+  ng = unit(crossp(b - a, c - a));
+  if (!sc.s.nrm) return ng;
+  const float* n = sc.s.nrm + 9 * (size_t)prim;
+  return unit(f3<F>(n) * (1.0f - bu - bv) + f3<F>(n + 3) * bu + f3<F>(n + 6) * bv);
+}
+
+const Mat& mat_of(const Scene& sc, uint32_t prim) { return sc.s.mats[sc.s.mat_ids ? sc.s.mat_ids[prim] : 0]; }
+bool emits(const Mat& m) { return m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f; }
+
+struct Bsdf {
+  ocr::Disney<F> d;
+  V3 albedo;
+  ocr::Frame<F> fr;
+  bool lambert, back;
+  F t;
+};
+
+Bsdf make_bsdf(const Scene& sc, uint32_t mat, const V3& n, bool back, F t) {
+  const Mat& m = sc.s.mats[mat];
+  Bsdf b;
+  b.lambert = m.bsdf_model == 1u;
+  b.albedo = f3<F>(m.color);
+  b.d = ocr::from_mat<F>(m);
+  b.fr = ocr::basis(n);
+  b.back = back;
+  b.t = t;
+  return b;
+}
+
+V3 bsdf_eval(const Bsdf& b, const V3& wo, const V3& wi, F& pdf) {
+  if (b.lambert) {
+    const F co = dotp(b.fr.n, wo), ci = dotp(b.fr.n, wi);
+    if (co * ci <= 0.0f) {
+      pdf = 0.0f;
+      return V3();
+    }
+    pdf = std::fabs(ci) * (1.0f / PI);
+    return b.albedo * (1.0f / PI);
+  }
+  return ocr::disney_f(b.d, b.fr, wo, wi, b.back, pdf);
+}
+
+ocr::Smp<F> bsdf_sample(const Bsdf& b, const V3& wo, F r0, F r1, F r2) {
+  if (b.lambert) {
+    ocr::Smp<F> s;
+    V3 l = ocr::cos_hemi(r0, r1);
+    if (dotp(b.fr.n, wo) < 0.0f) l.z = -l.z;
+    s.wi = b.fr.world(l);
+    s.pdf = std::fabs(l.z) * (1.0f / PI);
+    s.f = b.albedo * (1.0f / PI);
+    if (!(s.pdf > 0.0f)) s.pdf = 0.0f;
+    return s;
+  }
+  return ocr::disney_sample(b.d, b.fr, wo, b.t, b.back, r0, r1, r2);
+}
+
+V3 sun_to(const Sky& k) { return V3(-k.sun_direction[0], -k.sun_direction[1], -k.sun_direction[2]); }
+F sun_half(const Sky& k) { return k.sun_size * 0.1f; }
+F sun_sa(const Sky& k) { return TWO_PI * (1.0f - std::cos(sun_half(k))); }
+
+void sample_light(const Scene& sc, F r0, F r2, F r3, F r4, uint32_t& light, F& u, F& v) {
+  if (sc.p_sun > 0.0f && (r0 < sc.p_sun || sc.ne == 0)) {
+    light = LIGHT_SUN;
+    u = r3;
+    v = r4;
+  } else if (sc.ne > 0) {
+    uint32_t k = (uint32_t)(r2 * (F)sc.ne);
+    k = k < sc.ne ? k : sc.ne - 1;
+    light = sc.emissive[k];
+    const F su = std::sqrt(r3);
+    u = su * (1.0f - r4);
+    v = su * r4;
+  } else {
+    light = LIGHT_EMPTY;
+    u = v = 0.0f;
+  }
+}
+
+void sun_uv(const Scene& sc, const V3& w, F& u, F& v) {
+  const V3 s = sun_to(*sc.s.sky);
+  const V3 bt = ocr::perp(s);
+  const V3 tg = crossp(bt, s);
+  const F z = dotp(w, s), x = dotp(w, bt), y = dotp(w, tg);
+  const F h = std::cos(sun_half(*sc.s.sky));
+  u = std::fmod(std::atan2(y, x) / TWO_PI, 1.0f);
+  v = (z - h) / (1.0f - h);
+}
+
+struct LightHit {
+  V3 Le, wi;
+  F dist = 0, pdf = 0;
+};
+
+bool eval_light(const Scene& sc, uint32_t light, F u, F v, const V3& x, LightHit& e) {
+  const Sky& k = *sc.s.sky;
+  if (light == LIGHT_SUN) {
+    const V3 dir = unit(ocr::cone_dir(u, v, sun_to(k), sun_half(k)));
+    const V3 dv = dir * SUN_DISTANCE - x;
+    e.dist = len(dv);
+    e.wi = unit(dv);
+    e.Le = f3<F>(k.sun_color) * ocr::sun_I<F>(k, dir);
+    e.pdf = sc.p_sun > 0.0f ? sc.p_sun / sun_sa(k) : 0.0f;
+    return true;
+  }
+  if (light == LIGHT_EMPTY || sc.ne == 0) return false;
+  V3 a, b, c;
+  ocr::tri_pts<F>(sc.s, light, a, b, c);
+  const V3 pt = a * (1.0f - u - v) + b * u + c * v;
+  const V3 nl = unit(crossp(b - a, c - a));
+  const V3 dv = pt - x;
+  e.dist = len(dv);
+  e.wi = unit(dv);
+  const F cos_l = -dotp(nl, e.wi);
+  if (cos_l > 0.0f) {
+    e.Le = f3<F>(mat_of(sc, light).emission);
+    e.pdf = (1.0f - sc.p_sun) / (F)sc.ne / ocr::area_of(a, b, c) * (e.dist * e.dist) / cos_l;
+  } else {
+    e.Le = V3();
+    e.pdf = 0.0f;
+  }
+  return true;
+}
+
+F draw(uint32_t& st) { return ocr::draw<F>(st); }
+
+bool res_update(F& wsum, F& M, F w, uint32_t& rng) {
+  wsum += w;
+  M += 1.0f;
+  return draw(rng) <= w / wsum;
+}
+
+F di_phat(const Scene& sc, const Vtx& v, const Bsdf& bs, uint32_t light, F u, F w, bool vis, std::atomic<uint64_t>* c) {
+  LightHit e;
+  if (!eval_light(sc, light, u, w, v.x, e)) return 0.0f;
+  const F cs = dotp(e.wi, v.n);
+  if (!(cs > 0.0f)) return 0.0f;
+  if (vis && !visible(sc, v.x, v.n, e.wi, e.dist, c[1])) return 0.0f;
+  F pdf;
+  return luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * cs);
+}
+
+DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng) {
+  DiRes c;
+  if (res_update(c.wsum, c.M, r1.phat * r1.W * r1.M, rng)) {
+    c.light = r1.light; c.u = r1.u; c.v = r1.v; c.phat = r1.phat;
+  }
+  if (res_update(c.wsum, c.M, r2.phat * r2.W * r2.M, rng)) {
+    c.light = r2.light; c.u = r2.u; c.v = r2.v; c.phat = r2.phat;
+  }
+  c.M = r1.M + r2.M;
+  return c;
+}
+
+V3 approx_trace(const Scene& sc, const V3& x, const V3& wi, const V3& Tf, V3& first, uint32_t& rng,
+                std::atomic<uint64_t>* c) {
+  const V3 o = x + wi * EPS;
+  F t, hu, hv;
+  uint32_t prim;
+  if (!closest(sc, o, wi, t, hu, hv, prim, c[0])) {
+    first = x + wi * GI_SKY_DISTANCE;
+    return Tf * ocr::sky_rad<F>(*sc.s.sky, wi);
+  }
+  const V3 x2 = o + wi * t;
+  first = x2;
+  V3 ng;
+  V3 n2 = shading_normal(sc, prim, hu, hv, ng);
+  const V3 wo2 = -wi;
+  const bool back = dotp(wo2, n2) < 0.0f;
+  if (back) n2 = -n2;
+  const F r0 = draw(rng);
+  draw(rng);
+  const F r2 = draw(rng), r3 = draw(rng), r4 = draw(rng);
+  uint32_t light;
+  F lu, lv;
+  sample_light(sc, r0, r2, r3, r4, light, lu, lv);
+  LightHit e;
+  if (!eval_light(sc, light, lu, lv, x2, e) || !(e.pdf > 0.0f)) return V3();
+  const F cs = dotp(e.wi, n2);
+  if (!(cs > 0.0f) || !(mx(e.Le) > 0.0f)) return V3();
+  if (!visible(sc, x2, n2, e.wi, e.dist, c[1])) return V3();
+  const Bsdf b2 = make_bsdf(sc, sc.s.mat_ids ? sc.s.mat_ids[prim] : 0, n2, back, t);
+  F pdf;
+  return Tf * bsdf_eval(b2, wo2, e.wi, pdf) * e.Le * (cs / e.pdf);
+}
+
+F gi_phat(const Scene& sc, const Vtx& v, const Bsdf& bs, const V3& T, const GiRes& g, std::atomic<uint64_t>* c) {
+  const V3 wi = unit(g.x2 - v.x);
+  F pdf;
+  const V3 f = bsdf_eval(bs, v.wo, wi, pdf);
+  const F cs = std::fabs(dotp(wi, v.n));
+  const V3 Tf = T * f * cs;
+  uint32_t r = g.rng;
+  V3 first;
+  const V3 L = approx_trace(sc, v.x + wi * GI_OFFSET, wi, Tf, first, r, c);
+  return luma(L) + GI_PHAT_FLOOR * luma(Tf);
+}
+
+// ---- G-buffer helpers ---------------------------------------------------------
+Tex pack_texel(const V3& pos, F depth, const V3& n, const V3& alb, uint32_t bits) {
+  Tex t;
+  t.pos = pos;
+  t.depth32 = depth;
+  t.depth = r16(depth);
+  t.n = V3(r16(n.x), r16(n.y), r16(n.z));
+  t.albedo = rgb9e5_round(alb);
+  t.matbits = bits;
+  return t;
+}
+
+Tex sample_prev(const std::vector<Tex>& g, const P& p, F px, F py) {
+  const uint32_t ix = (uint32_t)std::floor(px), iy = (uint32_t)std::floor(py);
+  const F fx = px - std::floor(px), fy = py - std::floor(py);
+  const uint32_t x1 = std::min(ix + 1, p.width - 1), y1 = std::min(iy + 1, p.rows - 1);
+  const Tex &t00 = g[iy * p.width + ix], &t10 = g[iy * p.width + x1];
+  const Tex &t01 = g[y1 * p.width + ix], &t11 = g[y1 * p.width + x1];
+  Tex r;
+  r.pos = lerp3(lerp3(t00.pos, t10.pos, fx), lerp3(t01.pos, t11.pos, fx), fy);
+  r.depth = lerp(lerp(t00.depth, t10.depth, fx), lerp(t01.depth, t11.depth, fx), fy);
+  r.n = unit(lerp3(lerp3(t00.n, t10.n, fx), lerp3(t01.n, t11.n, fx), fy));
+  r.albedo = lerp3(lerp3(t00.albedo, t10.albedo, fx), lerp3(t01.albedo, t11.albedo, fx), fy);
+  return r;
+}
+
+bool texel_similar(const Tex& a, const Tex& prev) {
+  return std::fabs(a.depth - prev.depth) <= 0.1f * prev.depth && dotp(a.n, prev.n) > 0.906f;
+}
+
+F depth_cs(F depth) {
+  const F zn = 0.001f, zf = 10000.0f;
+  const F zl = (depth - zn) / zf;
+  return (zn * zf) / (zf - zl * (zf - zn));
+}
+
+bool reproject(const P& p, const V3& x, F& px, F& py) {
+  const CamV& c = p.prev_cam;
+  const V3 d = x - c.pos;
+  const F lz = dotp(d, c.back);
+  if (!(lz < 0.0f)) return false;
+  const F nx = dotp(d, c.right) / (-lz * c.tan_half * c.aspect);
+  const F ny = dotp(d, c.up) / (-lz * c.tan_half);
+  px = (nx + 1.0f) * 0.5f * (F)p.width - 0.5f;
+  py = (1.0f - ny) * 0.5f * (F)p.height - 0.5f - (F)p.row_start;
+  return px >= 0.0f && py >= 0.0f && px <= (F)(p.width - 1) && py <= (F)(p.rows - 1);
+}
+
+bool load_vtx(const State& s, uint32_t i, Vtx& v) {
+  v = s.vtx[i];
+  return v.t > 0.0f;
+}
+
+F ign_animated(uint32_t x, uint32_t y, uint32_t frame) {
+  const uint32_t id = frame % 64u;
+  const F fx = (F)x + 5.588238f * (F)id, fy = (F)y + 5.588238f * (F)id;
+  return fractf(52.9829189f * fractf(0.06711056f * fx + 0.00583715f * fy));
+}
+
+uint32_t mirror_px(int x, int m) {
+  const int r = ((x + m) % (2 * m)) - m;
+  const int a = r < 0 ? -r : r;
+  return (uint32_t)(a < m ? a : m - 1);
+}
+
+// ---- passes -------------------------------------------------------------------
+struct Ctx {
+  const Scene& sc;
+  State& s;
+  P& p;
+  int threads;
+  std::atomic<uint64_t>* cnt;
+};
+
+void pass_raygen(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    const uint32_t x = pix % c.p.width, y = pix / c.p.width + c.p.row_start;
+    const F ux = ((F)x + 0.5f) / (F)c.p.width * 2.0f - 1.0f;
+    const F uy = -(((F)y + 0.5f) / (F)c.p.height * 2.0f - 1.0f);
+    const CamV& cv = c.p.cam;
+    const V3 dl = unit(V3(ux * cv.tan_half * cv.aspect, uy * cv.tan_half, -1.0f));
+    c.s.ray_o[pix] = cv.pos;
+    c.s.ray_d[pix] = cv.right * dl.x + cv.up * dl.y + cv.back * dl.z;
+    c.s.thr_T[pix] = V3(1, 1, 1);
+    c.s.alive[pix] = 1;
+    c.s.rng[pix] = orc::pcg_hash(pix ^ orc::xor_shift(c.p.seed));
+  });
+}
+
+void pass_extend(Ctx& c, uint32_t bounce) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    if (!s.alive[pix]) {
+      s.vtx[pix].t = -1.0f;
+      return;
+    }
+    const V3 T = s.thr_T[pix], d = s.ray_d[pix];
+    const V3 o = s.ray_o[pix] + d * EPS;
+    F t, hu, hv;
+    uint32_t prim;
+    if (!closest(c.sc, o, d, t, hu, hv, prim, c.cnt[0])) {
+      s.radiance[pix] = s.radiance[pix] + T * ocr::sky_rad<F>(*c.sc.s.sky, d);
+      s.alive[pix] = 0;
+      s.vtx[pix].t = -1.0f;
+      if (bounce == 0) s.gbuf[c.p.cur][pix] = pack_texel(V3(), 0.0f, V3(), V3(1, 1, 1), 0u);
+      return;
+    }
+    const V3 hp = o + d * t;
+    V3 ng;
+    V3 ns = shading_normal(c.sc, prim, hu, hv, ng);
+    const bool back = dotp(-d, ns) < 0.0f;
+    if (back) ns = -ns;
+    const uint32_t mid = c.sc.s.mat_ids ? c.sc.s.mat_ids[prim] : 0;
+    const Mat& m = c.sc.s.mats[mid];
+    if (bounce == 0 && emits(m) && dotp(ng, d) < 0.0f) s.radiance[pix] = s.radiance[pix] + T * f3<F>(m.emission);
+    const uint32_t bits = mid | (back ? 0x80000000u : 0u);
+    Vtx v;
+    v.x = hp;
+    v.t = t;
+    v.n = ns;
+    v.mat = mid;
+    v.back = back;
+    v.wo = -d;
+    v.T = T;
+    s.vtx[pix] = v;
+    if (bounce == 0) {
+      const V3 alb(m.color[0] + m.emission[0], m.color[1] + m.emission[1], m.color[2] + m.emission[2]);
+      s.gbuf[c.p.cur][pix] = pack_texel(hp, t, ns, alb, bits);
+    }
+  });
+}
+
+void pass_di_ris(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) return;
+    const Scene& sc = c.sc;
+    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    uint32_t rng = s.rng[pix];
+    const F na = (F)c.p.di_candidates, nb = 1.0f;
+    const uint32_t iters = c.p.di_candidates > 1u ? c.p.di_candidates : 1u;
+    DiRes r;
+    for (uint32_t i = 0; i < iters; ++i) {
+      uint32_t a_light = LIGHT_EMPTY;
+      F a_u = 0, a_v = 0, a_phat = 0, a_pl = 0, a_pb = 0;
+      if (i < c.p.di_candidates) {
+        const F r0 = draw(rng);
+        draw(rng);
+        const F r2 = draw(rng), r3 = draw(rng), r4 = draw(rng);
+        sample_light(sc, r0, r2, r3, r4, a_light, a_u, a_v);
+        LightHit e;
+        if (eval_light(sc, a_light, a_u, a_v, v.x, e)) {
+          a_pl = e.pdf;
+          const F cs = dotp(e.wi, v.n);
+          if (cs > 0.0f && a_pl > 0.0f) {
+            const V3 f = bsdf_eval(bs, v.wo, e.wi, a_pb);
+            if (a_pb > 0.0f) a_phat = luma(f * e.Le * cs);
+          }
+        }
+      }
+      uint32_t b_light = LIGHT_EMPTY;
+      F b_u = 0, b_v = 0, b_phat = 0, b_pl = 0, b_pb = 0;
+      if (i == 0) {
+        const F r0 = draw(rng), r1 = draw(rng), r2 = draw(rng);
+        const ocr::Smp<F> sm = bsdf_sample(bs, v.wo, r0, r1, r2);
+        b_pb = sm.pdf;
+        const V3 contrib = sm.f * std::fabs(dotp(sm.wi, v.n));
+        if (sm.pdf > 0.0f && dotp(contrib, contrib) > 0.0f) {
+          F t, hu, hv;
+          uint32_t prim;
+          if (closest(sc, v.x + v.n * EPS, sm.wi, t, hu, hv, prim, c.cnt[0])) {
+            if (emits(mat_of(sc, prim))) {
+              b_light = prim;
+              b_u = hu;
+              b_v = hv;
+            }
+          } else if (sc.p_sun > 0.0f) {
+            b_light = LIGHT_SUN;
+            sun_uv(sc, sm.wi, b_u, b_v);
+          }
+          LightHit e;
+          if (b_light != LIGHT_EMPTY && eval_light(sc, b_light, b_u, b_v, v.x, e)) {
+            b_pl = e.pdf;
+            b_phat = luma(contrib * e.Le);
+          }
+        }
+      }
+      if (i < c.p.di_candidates) {
+        const F w = a_pl > 0.0f ? a_phat / (na * a_pl + nb * a_pb) : 0.0f;
+        if (res_update(r.wsum, r.M, w, rng)) {
+          r.light = a_light; r.u = a_u; r.v = a_v; r.phat = a_phat;
+        }
+      }
+      if (i == 0) {
+        const F w = b_pb > 0.0f ? b_phat / (na * b_pl + nb * b_pb) : 0.0f;
+        if (res_update(r.wsum, r.M, w, rng)) {
+          r.light = b_light; r.u = b_u; r.v = b_v; r.phat = b_phat;
+        }
+      }
+    }
+    r.W = 0.0f;
+    if (r.phat > RESTIR_DI_EPSILON) {
+      LightHit e;
+      if (eval_light(sc, r.light, r.u, r.v, v.x, e) && visible(sc, v.x, v.n, e.wi, e.dist, c.cnt[1]))
+        r.W = (1.0f / r.phat) * r.wsum;
+    }
+    s.di[pix] = r;
+    s.rng[pix] = rng;
+  });
+}
+
+void pass_gi_ris(Ctx& c, uint32_t bounce) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) return;
+    uint32_t rng = s.rng[pix];
+    V3 T = v.T;
+    GiRes g;
+    bool alive = true;
+    if (bounce > 1) {
+      const F q = std::min(1.0f, mx(T));
+      if (q < draw(rng)) alive = false;
+      else T = T * (1.0f / q);
+    }
+    if (alive) {
+      const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+      const F r0 = draw(rng), r1 = draw(rng), r2 = draw(rng);
+      const ocr::Smp<F> sm = bsdf_sample(bs, v.wo, r0, r1, r2);
+      F M = 0, wsum = 0, phat = 0;
+      if (sm.pdf > 1e-6f) {
+        const F cs = std::fabs(dotp(sm.wi, v.n));
+        const V3 Tf = T * sm.f * cs;
+        const uint32_t phat_rng = rng;
+        V3 first;
+        const V3 L = approx_trace(c.sc, v.x, sm.wi, Tf, first, rng, c.cnt);
+        const F ph = luma(L) + GI_PHAT_FLOOR * luma(Tf);
+        if (res_update(wsum, M, ph / sm.pdf, rng)) {
+          g.x2 = first;
+          g.rng = phat_rng;
+          phat = ph;
+        }
+      }
+      g.M = M;
+      g.W = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
+      g.wsum = wsum;
+      g.phat = phat;
+    }
+    s.thr_T[pix] = T;
+    s.alive[pix] = alive ? 1 : 0;
+    s.gi[pix] = g;
+    s.rng[pix] = rng;
+  });
+}
+
+void pass_di_temporal(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    const P& p = c.p;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) {
+      s.di_tmp[pix] = DiRes();
+      s.di_hist[p.cur][pix] = DiRes();
+      return;
+    }
+    DiRes r = s.di[pix];
+    uint32_t rng = s.rng[pix];
+    F px, py;
+    if (reproject(p, v.x, px, py)) {
+      const Tex& cur = s.gbuf[p.cur][pix];
+      const Tex prev = sample_prev(s.gbuf[p.cur ^ 1], p, px, py);
+      if (texel_similar(cur, prev)) {
+        const F ux = draw(rng), uy = draw(rng);
+        const uint32_t qx = std::min((uint32_t)std::floor(px + ux), p.width - 1);
+        const uint32_t qy = std::min((uint32_t)std::floor(py + uy), p.rows - 1);
+        DiRes pr = s.di_hist[p.cur ^ 1][qy * p.width + qx];
+        pr.M = std::min(pr.M, (F)p.di_mcap * r.M);
+        const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+        pr.phat = di_phat(c.sc, v, bs, pr.light, pr.u, pr.v, true, c.cnt);
+        DiRes m = di_merge(r, pr, rng);
+        if (m.phat > RESTIR_DI_EPSILON && m.M * m.wsum > RESTIR_DI_EPSILON)
+          m.W = (1.0f / m.phat) * (1.0f / m.M * m.wsum);
+        r = m;
+      }
+    }
+    s.di_tmp[pix] = r;
+    if (p.neighbours == 0) {
+      s.di[pix] = r;
+      s.di_hist[p.cur][pix] = r;
+    }
+    s.rng[pix] = rng;
+  });
+}
+
+void pass_di_spatial(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    const P& p = c.p;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) {
+      s.di[pix] = DiRes();
+      return;
+    }
+    const uint32_t x = pix % p.width, y = pix / p.width;
+    DiRes r = s.di_tmp[pix];
+    uint32_t rng = s.rng[pix];
+    const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+    const Tex& ct = s.gbuf[p.cur][pix];
+    const F cd = depth_cs(ct.depth);
+    F radius = p.radius > 0.0f ? p.radius : (F)(p.width + p.rows) / 2.0f * 0.05f;
+    const uint32_t pass_seed = p.seed;
+    const F r_off = ign_animated(x, y, pass_seed);
+    const uint32_t aseed = orc::hash_combine(x >> 2, orc::hash_combine(y >> 2, pass_seed));
+    const F a_off = (F)aseed * (1.0f / (F)0xFFFFFFFFu) * TWO_PI;
+    const uint32_t n = p.neighbours;
+    for (uint32_t i = 0; i < n; ++i) {
+      const F ang = (F)i * GOLDEN_ANGLE + a_off;
+      const F rad = std::pow((F)i / (F)n, 0.5f) * radius + r_off;
+      const int ox = (int)(rad * std::cos(ang)), oy = (int)(rad * std::sin(ang));
+      const uint32_t nx = mirror_px((int)x + ox, (int)p.width), ny = mirror_px((int)y + oy, (int)p.rows);
+      const uint32_t q = ny * p.width + nx;
+      if (q == pix) continue;
+      const Tex& nt = s.gbuf[p.cur][q];
+      bool valid = nt.depth32 != 0.0f;
+      if (valid) valid = std::fabs(cd - depth_cs(nt.depth)) / cd < 0.1f && dotp(ct.n, nt.n) > 0.906f;
+      if (valid) {
+        DiRes nr = s.di_tmp[q];
+        nr.phat = di_phat(c.sc, v, bs, nr.light, nr.u, nr.v, true, c.cnt);
+        valid = nr.phat > 0.0f;
+        DiRes m = di_merge(r, nr, rng);
+        F z = 0.0f;
+        if (di_phat(c.sc, v, bs, m.light, m.u, m.v, true, c.cnt) > 0.0f) z += r.M;
+        Vtx vn;
+        if (load_vtx(s, q, vn)) {
+          vn.x = nt.pos;
+          const Bsdf bn = make_bsdf(c.sc, vn.mat, vn.n, vn.back, vn.t);
+          if (di_phat(c.sc, vn, bn, m.light, m.u, m.v, true, c.cnt) > 0.0f) z += nr.M;
+        }
+        if (m.phat > RESTIR_DI_EPSILON && z * m.wsum > RESTIR_DI_EPSILON) m.W = (1.0f / m.phat) * (1.0f / z * m.wsum);
+        r = m;
+      }
+      if (!valid) radius = std::max(radius * 0.5f, 3.0f);
+    }
+    s.di[pix] = r;
+    s.di_hist[p.cur][pix] = r;
+    s.rng[pix] = rng;
+  });
+}
+
+void pass_gi_temporal(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    const P& p = c.p;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) {
+      s.gi_hist[p.cur][pix] = GiRes();
+      return;
+    }
+    GiRes r = s.gi[pix];
+    if (s.alive[pix]) {
+      uint32_t rng = s.rng[pix];
+      F px, py;
+      if (reproject(p, v.x, px, py)) {
+        const F ux = draw(rng), uy = draw(rng);
+        const uint32_t qx = std::min((uint32_t)std::floor(px + ux), p.width - 1);
+        const uint32_t qy = std::min((uint32_t)std::floor(py + uy), p.rows - 1);
+        const Tex& cur = s.gbuf[p.cur][pix];
+        const Tex prev = sample_prev(s.gbuf[p.cur ^ 1], p, px, py);
+        if (texel_similar(cur, prev)) {
+          const GiRes pr = s.gi_hist[p.cur ^ 1][qy * p.width + qx];
+          const F pM = std::min(pr.M, (F)p.gi_mcap);
+          const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+          const F pp = gi_phat(c.sc, v, bs, s.thr_T[pix], pr, c.cnt);
+          F wsum = 0, M = 0, phat = 0;
+          GiRes m;
+          if (res_update(wsum, M, r.phat * r.W * r.M, rng)) {
+            m.x2 = r.x2;
+            m.rng = r.rng;
+            phat = r.phat;
+          }
+          if (res_update(wsum, M, pp * pr.W * pM, rng)) {
+            m.x2 = pr.x2;
+            m.rng = pr.rng;
+            phat = pp;
+          }
+          M = r.M + pM;
+          m.M = M;
+          m.W = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
+          m.wsum = wsum;
+          m.phat = phat;
+          r = m;
+        }
+      }
+      s.rng[pix] = rng;
+    }
+    s.gi[pix] = r;
+    s.gi_hist[p.cur][pix] = r;
+  });
+}
+
+void pass_apply_di(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) return;
+    const DiRes r = s.di[pix];
+    if (r.W == 0.0f) return;
+    LightHit e;
+    if (!eval_light(c.sc, r.light, r.u, r.v, v.x, e)) return;
+    const F cs = dotp(e.wi, v.n);
+    if (!(cs > 0.0f)) return;
+    if (!visible(c.sc, v.x, v.n, e.wi, e.dist, c.cnt[1])) return;
+    const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+    F pdf;
+    const V3 add = v.T * bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * (cs * r.W);
+    if (!finite3(add)) return;
+    s.radiance[pix] = s.radiance[pix] + add;
+  });
+}
+
+void pass_apply_gi(Ctx& c) {
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) return;
+    if (!s.alive[pix]) return;
+    const GiRes g = s.gi[pix];
+    if (g.W > 0.0f) {
+      const V3 wi = unit(g.x2 - v.x);
+      const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+      F pdf;
+      const V3 T = s.thr_T[pix] * bsdf_eval(bs, v.wo, wi, pdf) * (std::fabs(dotp(v.n, wi)) * g.W);
+      const bool ok = finite3(T) && mx(T) > 0.0f;
+      s.thr_T[pix] = T;
+      s.alive[pix] = ok ? 1 : 0;
+      s.ray_o[pix] = v.x;
+      s.ray_d[pix] = wi;
+    } else {
+      s.alive[pix] = 0;
+    }
+  });
+}
+
+// ---- post chain ---------------------------------------------------------------
+bool is_sky(const Tex& t) { return t.depth32 == 0.0f; }
+
+void pass_firefly(Ctx& c, const std::vector<V3>& in, std::vector<V3>& out) {
+  const P& p = c.p;
+  const std::vector<Tex>& g = c.s.gbuf[p.cur];
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    const uint32_t x = pix % p.width, y = pix / p.width;
+    const V3 cur = in[pix];
+    if (is_sky(g[pix])) {
+      out[pix] = cur;
+      return;
+    }
+    F min_l = 3.402823466e38f, max_l = 0.0f;
+    V3 min_c = cur, max_c;
+    const F cl = luma(cur);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        if (dx == 0 && dy == 0) continue;
+        const int sx = (int)x + dx, sy = (int)y + dy;
+        if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
+        const uint32_t q = (uint32_t)sy * p.width + (uint32_t)sx;
+        if (is_sky(g[q])) continue;
+        const V3 nc = in[q];
+        const F nl = luma(nc);
+        if (nl < min_l) {
+          min_l = nl;
+          min_c = nc;
+        } else if (nl > max_l) {
+          max_l = nl;
+          max_c = nc;
+        }
+      }
+    V3 rec = cur;
+    if (cl < min_l) rec = min_c;
+    else if (cl > max_l) rec = max_c;
+    if (min_l > max_l) rec = cur;
+    out[pix] = rec;
+  });
+}
+
+void pass_svgf_temporal(Ctx& c, const std::vector<V3>& in) {
+  State& s = c.s;
+  const P& p = c.p;
+  each_pixel(s.npix, c.threads, [&](uint32_t pix) {
+    const Tex& cur = s.gbuf[p.cur][pix];
+    if (is_sky(cur)) {
+      s.col[0][pix] = in[pix];
+      s.var[0][pix] = 0.0f;
+      s.mom[p.cur][pix] = {0.0f, 0.0f};
+      return;
+    }
+    const V3 rad = in[pix];
+    const F l = luma(rad);
+    V3 tc;
+    F m1 = 0, m2 = 0;
+    uint32_t count = 0;
+    F px, py;
+    if (reproject(p, cur.pos, px, py)) {
+      const Tex prev = sample_prev(s.gbuf[p.cur ^ 1], p, px, py);
+      if (texel_similar(cur, prev)) {
+        const uint32_t ix = (uint32_t)std::floor(px), iy = (uint32_t)std::floor(py);
+        const F fx = px - std::floor(px), fy = py - std::floor(py);
+        const uint32_t x1 = std::min(ix + 1, p.width - 1), y1 = std::min(iy + 1, p.rows - 1);
+        const uint32_t i00 = iy * p.width + ix, i10 = iy * p.width + x1, i01 = y1 * p.width + ix, i11 = y1 * p.width + x1;
+        const std::vector<V3>& h = s.hist;
+        tc = lerp3(lerp3(h[i00], h[i10], fx), lerp3(h[i01], h[i11], fx), fy);
+        const auto& hm = s.mom[p.cur ^ 1];
+        m1 = lerp(lerp(hm[i00].first, hm[i10].first, fx), lerp(hm[i01].first, hm[i11].first, fx), fy);
+        m2 = lerp(lerp(hm[i00].second, hm[i10].second, fx), lerp(hm[i01].second, hm[i11].second, fx), fy);
+        count = std::min(s.count[pix] + 1, p.svgf_history);
+      }
+    }
+    const F alpha = 1.0f / (1.0f + (F)count);
+    m1 = lerp(m1, l, alpha);
+    m2 = lerp(m2, l * l, alpha);
+    tc = lerp3(tc, rad, alpha);
+    s.var[0][pix] = std::max(m2 - m1 * m1, 0.0f);
+    s.col[0][pix] = tc;
+    s.mom[p.cur][pix] = {m1, m2};
+    s.count[pix] = count;
+  });
+}
+
+void pass_svgf_atrous(Ctx& c, uint32_t pass) {
+  State& s = c.s;
+  const P& p = c.p;
+  const uint32_t src = pass & 1u, dst = src ^ 1u;
+  const std::vector<Tex>& g = s.gbuf[p.cur];
+  each_pixel(s.npix, c.threads, [&](uint32_t pix) {
+    const uint32_t x = pix % p.width, y = pix / p.width;
+    const V3 c0 = s.col[src][pix];
+    if (is_sky(g[pix])) {
+      s.col[dst][pix] = c0;
+      s.var[dst][pix] = 0.0f;
+      if (pass == 0) s.hist[pix] = c0;
+      return;
+    }
+    const Tex& ct = g[pix];
+    const F KW[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    const int step = 1 << pass;
+    V3 sum = c0;
+    F vsum = s.var[src][pix];
+    F wsum = 1.0f;
+    const F cl = luma(sum);
+    const F cv = vsum;
+    for (int dy = -2; dy <= 2; ++dy)
+      for (int dx = -2; dx <= 2; ++dx) {
+        if (dx == 0 && dy == 0) continue;
+        const int sx = (int)x + dx * step, sy = (int)y + dy * step;
+        if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
+        const uint32_t q = (uint32_t)sy * p.width + (uint32_t)sx;
+        if (is_sky(g[q])) continue;
+        const Tex& tq = g[q];
+        const V3 cq = s.col[src][q];
+        const F wn = std::pow(std::max(dotp(ct.n, tq.n), 0.0f), 128.0f);
+        const F dd = std::fabs(ct.depth - tq.depth);
+        const F wd = std::exp(-(dd * dd) / (2.0f * (0.1f * 0.1f)));
+        const F wl = std::exp(-std::fabs(cl - luma(cq)) / std::max(4.0f * std::sqrt(cv), 1e-8f));
+        F w = wn * wd * wl;
+        w *= KW[dx < 0 ? -dx : dx] * KW[dy < 0 ? -dy : dy];
+        sum = sum + cq * w;
+        vsum += s.var[src][q] * (w * w);
+        wsum += w;
+      }
+    sum = divs(sum, wsum);
+    vsum = vsum / (wsum * wsum);
+    s.col[dst][pix] = sum;
+    s.var[dst][pix] = vsum;
+    if (pass == 0) s.hist[pix] = sum;
+  });
+}
+
+F mitchell_1d(F x0, F B, F C) {
+  const F x = std::fabs(2.0f * x0);
+  const F k = 1.0f / 6.0f;
+  if (x > 1.0f)
+    return ((-B - 6.0f * C) * x * x * x + (6.0f * B + 30.0f * C) * x * x + (-12.0f * B - 48.0f * C) * x +
+            (8.0f * B + 24.0f * C)) * k;
+  return ((12.0f - 9.0f * B - 6.0f * C) * x * x * x + (-18.0f + 12.0f * B + 6.0f * C) * x * x + (6.0f - 2.0f * B)) * k;
+}
+
+V3 clip_aabb(const V3& lo, const V3& hi, const V3& h) {
+  const V3 c = (hi + lo) * 0.5f;
+  const V3 e = (hi - lo) * 0.5f;
+  const V3 r = h - c;
+  const V3 ru(std::fabs(r.x / e.x), std::fabs(r.y / e.y), std::fabs(r.z / e.z));
+  const F m = std::max(ru.x, std::max(ru.y, ru.z));
+  return m > 1.0f ? c + divs(r, m) : h;
+}
+
+void pass_taa(Ctx& c, const std::vector<V3>& in, const std::vector<V3>& prev, std::vector<V3>& out) {
+  State& s = c.s;
+  const P& p = c.p;
+  const std::vector<Tex>& g = s.gbuf[p.cur];
+  each_pixel(s.npix, c.threads, [&](uint32_t pix) {
+    const uint32_t x = pix % p.width, y = pix / p.width;
+    if (is_sky(g[pix])) {
+      out[pix] = in[pix];
+      return;
+    }
+    const V3 cur = in[pix];
+    const F m0 = mitchell_1d(0.0f, 0.33f, 0.33f);
+    F wsum = m0 * m0;
+    V3 rec = cur * wsum;
+    V3 fm = cur, sm = cur * cur;
+    F n = 1.0f;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        if (dx == 0 && dy == 0) continue;
+        const int sx = (int)x + dx, sy = (int)y + dy;
+        if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
+        const V3 q4 = in[(uint32_t)sy * p.width + (uint32_t)sx];
+        const V3 sc(std::max(q4.x, 0.0f), std::max(q4.y, 0.0f), std::max(q4.z, 0.0f));
+        F w = mitchell_1d((F)dx, 0.33f, 0.33f) * mitchell_1d((F)dy, 0.33f, 0.33f);
+        w *= 1.0f / (1.0f + luma(sc));
+        rec = rec + sc * w;
+        wsum += w;
+        fm = fm + sc;
+        sm = sm + sc * sc;
+        n += 1.0f;
+      }
+    rec = divs(rec, std::max(wsum, 1e-5f));
+    V3 hist;
+    uint32_t prev_id;
+    F px, py;
+    const Tex& ct = g[pix];
+    if (reproject(p, ct.pos, px, py)) {
+      const uint32_t fx0 = std::min((uint32_t)std::floor(px), p.width - 1), fy0 = std::min((uint32_t)std::floor(py), p.rows - 1);
+      const uint32_t cx = std::min((uint32_t)std::ceil(px), p.width - 1), cy = std::min((uint32_t)std::ceil(py), p.rows - 1);
+      prev_id = (uint32_t)std::floor(py) * p.width + (uint32_t)std::floor(px);
+      const V3 h00 = prev[fy0 * p.width + fx0], h10 = prev[cy * p.width + fx0];
+      const V3 h01 = prev[fy0 * p.width + cx], h11 = prev[cy * p.width + cx];
+      const F tx = px - std::floor(px), ty = py - std::floor(py);
+      const V3 a = h00 * (1.0f - tx) + h10 * tx;
+      const V3 cc = h01 * (1.0f - tx) + h11 * tx;
+      hist = a * (1.0f - ty) + cc * ty;
+    } else {
+      hist = prev[pix];
+      prev_id = pix;
+    }
+    const Tex& pt = s.gbuf[p.cur ^ 1][prev_id];
+    const F cdc = depth_cs(ct.depth), pdc = depth_cs(pt.depth);
+    const bool valid_depth = std::fabs(cdc - pdc) / cdc < 0.1f;
+    const V3 mean = divs(fm, n);
+    V3 sd(std::fabs(sm.x - fm.x * fm.x / n), std::fabs(sm.y - fm.y * fm.y / n), std::fabs(sm.z - fm.z * fm.z / n));
+    sd = divs(sd, n - 1.0f);
+    sd = V3(std::sqrt(sd.x), std::sqrt(sd.y), std::sqrt(sd.z));
+    const V3 clipped = clip_aabb(mean - sd, mean + sd, hist);
+    const F blend = 1.0f - 0.8f * (valid_depth ? 1.0f : 0.0f);
+    const F cw = ocr::cl(blend * (1.0f / (1.0f + luma(rec))), 0.0f, 1.0f);
+    const F hw = ocr::cl((1.0f - blend) * (1.0f / (1.0f + luma(clipped))), 0.0f, 1.0f);
+    out[pix] = divs(rec * cw + clipped * hw, cw + hw);
+  });
+}
+
+// ---- one frame ------------------------------------------------------------------
+void frame(const Scene& sc, State& s, P& p, int threads, float* out) {
+  std::atomic<uint64_t> cnt[2];
+  cnt[0] = 0;
+  cnt[1] = 0;
+  Ctx c{sc, s, p, threads, cnt};
+  p.cur = s.cur;
+  p.first_frame = s.history ? 0u : 1u;
+  p.prev_cam = s.history ? s.prev_cam : p.cam;
+  std::fill(s.radiance.begin(), s.radiance.end(), V3());
+  for (uint32_t k = 0; k < p.spp; ++k) {
+    p.seed = p.frame_idx * p.spp + k + p.seed_base;
+    pass_raygen(c);
+    for (uint32_t b = 0; b < p.max_bounces; ++b) {
+      const bool more = b + 1 < p.max_bounces;
+      pass_extend(c, b);
+      pass_di_ris(c);
+      if (more) pass_gi_ris(c, b);
+      if (b == 0 && p.restir_di) {
+        pass_di_temporal(c);
+        if (p.neighbours > 0) pass_di_spatial(c);
+      }
+      if (b == 0 && more && p.restir_gi) pass_gi_temporal(c);
+      pass_apply_di(c);
+      if (more) pass_apply_gi(c);
+    }
+  }
+  // post chain
+  const std::vector<Tex>& g = s.gbuf[p.cur];
+  std::vector<V3>& dm = s.demod[p.cur];
+  each_pixel(s.npix, threads, [&](uint32_t i) {
+    const V3 r = s.radiance[i], a = g[i].albedo;
+    dm[i] = V3(safe_div(r.x, a.x), safe_div(r.y, a.y), safe_div(r.z, a.z));
+  });
+  const std::vector<V3>* img = &dm;
+  if (p.firefly) {
+    pass_firefly(c, *img, s.work);
+    img = &s.work;
+  }
+  if (p.svgf) {
+    pass_svgf_temporal(c, *img);
+    for (uint32_t i = 0; i < p.atrous; ++i) pass_svgf_atrous(c, i);
+    img = &s.col[p.atrous & 1u];
+  }
+  if (p.taa && !p.first_frame) {
+    std::vector<V3>* o = img == &s.work ? &dm : &s.work;
+    pass_taa(c, *img, s.demod[p.cur ^ 1], *o);
+    img = o;
+  }
+  const std::vector<V3> fin = *img;
+  const F inv = 1.0f / (F)p.spp;
+  for (uint32_t i = 0; i < s.npix; ++i) {
+    const V3 r = fin[i] * g[i].albedo;
+    out[3 * (size_t)i] = r.x * inv;
+    out[3 * (size_t)i + 1] = r.y * inv;
+    out[3 * (size_t)i + 2] = r.z * inv;
+    dm[i] = fin[i];
+  }
+  s.rays[0] = cnt[0];
+  s.rays[1] = cnt[1];
+  s.cur ^= 1u;
+  s.prev_cam = p.cam;
+  s.history = true;
+}
+
+}  // namespace ors
+
+extern "C" {
+
+// Per-pixel state of the ReSTIR oracle for a width x rows slab; it carries the
+// reservoirs, G-buffer and denoiser history from frame to frame.
+void* or_restir_create(uint32_t width, uint32_t rows) { return new ors::State(width * rows); }
+void or_restir_destroy(void* st) { delete static_cast<ors::State*>(st); }
+
+// One frame.  cfg20: width, height, row_start, rows, spp, max_bounces, seed,
+// frame_idx, di_candidates, di_mcap, neighbours, gi_mcap, restir_di,
+// restir_gi, firefly, svgf, taa, svgf_history, atrous passes, radius (float
+// bits).  out: the frame's linear mean radiance (rows x width x 3).
+// rays[0..1]: closest-hit and shadow rays traced.
+int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
+                    uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg20,
+                    int threads, float* out, uint64_t* rays) {
+  ors::State& s = *static_cast<ors::State*>(st);
+  ors::Scene sc;
+  sc.s.pos = pos;
+  sc.s.nrm = nrm;
+  sc.s.mat_ids = mat_ids;
+  sc.s.mats = static_cast<const ocr::Mat*>(mats);
+  sc.s.n_tris = n_tris;
+  sc.s.sky = static_cast<const ocr::Sky*>(sky);
+  sc.s.cam = static_cast<const ocr::Cam*>(cam);
+  sc.s.bvh = static_cast<const orc::Bvh*>(bvh);
+  for (uint32_t t = 0; t < n_tris; ++t) {
+    const ocr::Mat& m = sc.s.mats[mat_ids ? mat_ids[t] : 0];
+    if (ors::emits(m)) {
+      const float* p = pos + 9 * (size_t)t;
+      float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+      float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 0.0f) sc.emissive.push_back(t);
+    }
+  }
+  sc.ne = (uint32_t)sc.emissive.size();
+  sc.p_sun = sc.s.sky->sun_enabled > 0.0f ? (sc.ne > 0 ? 0.5f : 1.0f) : 0.0f;
+  ors::P p{};
+  p.width = cfg20[0];
+  p.height = cfg20[1];
+  p.row_start = cfg20[2];
+  p.rows = cfg20[3];
+  p.spp = cfg20[4];
+  p.max_bounces = cfg20[5];
+  p.seed_base = cfg20[6];
+  p.frame_idx = cfg20[7];
+  p.di_candidates = cfg20[8];
+  p.di_mcap = cfg20[9];
+  p.neighbours = cfg20[10];
+  p.gi_mcap = cfg20[11];
+  p.restir_di = cfg20[12];
+  p.restir_gi = cfg20[13];
+  p.firefly = cfg20[14];
+  p.svgf = cfg20[15];
+  p.taa = cfg20[16];
+  p.svgf_history = cfg20[17];
+  p.atrous = cfg20[18];
+  std::memcpy(&p.radius, &cfg20[19], 4);
+  if (s.npix != p.width * p.rows) return -1;
+  p.cam = ors::cam_view(*sc.s.cam, p.width, p.height);
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  ors::frame(sc, s, p, threads, out);
+  if (rays) {
+    rays[0] = s.rays[0];
+    rays[1] = s.rays[1];
+  }
+  return 0;
+}
+
+// FP16 round trip of n floats (the G-buffer's depth and normal precision).
+void or_f16_round(const float* in, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = ors::r16(in[i]);
+}
+
+// One a-trous pass over a caller-supplied image (width x height, no sky):
+// colour, variance, FP16-rounded depth and normal per pixel; writes the
+// filtered colour.  The probe the SVGF pins evaluate.
+void or_atrous_probe(uint32_t width, uint32_t height, uint32_t pass, const float* col, const float* var,
+                     const float* depth, const float* nrm, float* out) {
+  ors::State s(width * height);
+  for (uint32_t i = 0; i < width * height; ++i) {
+    ors::Tex t;
+    t.pos = ors::V3(0, 0, 0);
+    t.depth32 = 1.0f;
+    t.depth = depth[i];
+    t.n = ors::V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    t.albedo = ors::V3(1, 1, 1);
+    t.matbits = 0;
+    s.gbuf[0][i] = t;
+    s.col[pass & 1u][i] = ors::V3(col[3 * i], col[3 * i + 1], col[3 * i + 2]);
+    s.var[pass & 1u][i] = var[i];
+  }
+  std::atomic<uint64_t> cnt[2];
+  cnt[0] = 0;
+  cnt[1] = 0;
+  ors::Scene sc;
+  ors::P p{};
+  p.width = width;
+  p.rows = height;
+  p.height = height;
+  p.cur = 0;
+  ors::Ctx c{sc, s, p, 1, cnt};
+  ors::pass_svgf_atrous(c, pass);
+  const uint32_t dst = (pass & 1u) ^ 1u;
+  for (uint32_t i = 0; i < width * height; ++i) {
+    out[3 * i] = s.col[dst][i].x;
+    out[3 * i + 1] = s.col[dst][i].y;
+    out[3 * i + 2] = s.col[dst][i].z;
+  }
+}
+
+}  // extern "C"

@@ -133,7 +133,7 @@ def load_oracle() -> Oracle:
 def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounces: int,
                   rr_start: int, seed: int = 0, frame_idx: int = 0, row_start: int = 0,
                   rows: int | None = None, pixels=None, precision: int = 32, threads: int = 0,
-                  bvh=None, camera=None, sky=None):
+                  bvh=None, camera=None, sky=None, jitter: bool = True):
     """Run the oracle path tracer on a SceneData; returns (mean radiance, counts)."""
     import ctypes as C
     from appearance_amd._ffi import Material
@@ -145,8 +145,8 @@ def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounc
     b = bvh if bvh is not None else o.bvh(pos)
     cam = camera if camera is not None else scene.camera
     sk = sky if sky is not None else scene.sky
-    cfg = np.array([width, height, row_start, rows, spp, max_bounces, rr_start, seed, frame_idx],
-                   np.uint32)
+    cfg = np.array([width, height, row_start, rows, spp, max_bounces, rr_start, seed, frame_idx,
+                    1 if jitter else 0], np.uint32)
     if pixels is not None:
         pixels = np.ascontiguousarray(pixels, np.uint32)
         out = np.empty((pixels.shape[0], 3), np.float32)
@@ -165,3 +165,81 @@ def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounc
        _vp(cfg), _vp(pixels) if pixels is not None else None,
        0 if pixels is None else pixels.shape[0], precision, threads, _vp(out), _vp(counts))
     return out, (int(counts[0]), int(counts[1]))
+
+
+class OracleRestir:
+    """The oracle's ReSTIR integrator with its own per-pixel state (reservoirs,
+    G-buffer, denoiser history) carried from frame to frame."""
+
+    FIELDS = ["di_candidates", "di_mcap", "di_spatial_neighbours", "gi_mcap", "restir_di", "restir_gi",
+              "firefly_filter", "svgf", "taa", "svgf_history", "svgf_iterations"]
+
+    def __init__(self, o: Oracle, scene, width: int, height: int, row_start: int = 0,
+                 rows: int | None = None, bvh=None):
+        self.o = o
+        self.scene = scene
+        self.width, self.height, self.row_start = width, height, row_start
+        self.rows = height - row_start if rows is None else rows
+        self.pos = np.ascontiguousarray(scene.positions, np.float32)
+        self.nrm = None if scene.normals is None else np.ascontiguousarray(scene.normals, np.float32)
+        self.mids = np.ascontiguousarray(scene.material_ids, np.uint32)
+        from appearance_amd._ffi import Material
+        self.mats = (Material * len(scene.materials))(*scene.materials)
+        self.bvh = bvh if bvh is not None else o.bvh(self.pos)
+        fn = o.lib.or_restir_create
+        fn.restype = C.c_void_p
+        fn.argtypes = [C.c_uint32, C.c_uint32]
+        self.h = C.c_void_p(fn(width, self.rows))
+        self.rays = (0, 0)
+
+    def frame(self, cfg, camera=None, sky=None, spp: int = 1, max_bounces: int = 2, seed: int = 0,
+              frame_idx: int = 0, threads: int = 0) -> np.ndarray:
+        """cfg: an appearance_amd Config (or any object with the ReSTIR fields)."""
+        cam = camera if camera is not None else self.scene.camera
+        sk = sky if sky is not None else self.scene.sky
+        radius = np.array([cfg.di_spatial_radius], np.float32).view(np.uint32)[0]
+        c = np.array([self.width, self.height, self.row_start, self.rows, spp, max_bounces, seed, frame_idx,
+                      cfg.di_candidates, cfg.di_mcap, cfg.di_spatial_neighbours, cfg.gi_mcap,
+                      cfg.restir_di, cfg.restir_gi, cfg.firefly_filter, cfg.svgf, cfg.taa,
+                      cfg.svgf_history, cfg.svgf_iterations, radius], np.uint32)
+        out = np.empty((self.rows, self.width, 3), np.float32)
+        rays = np.zeros(2, np.uint64)
+        fn = self.o.lib.or_restir_frame
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, _P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P, C.c_int, _P, _P]
+        r = fn(self.h, _vp(self.pos), _vp(self.nrm) if self.nrm is not None else None, _vp(self.mids),
+               C.cast(self.mats, C.c_void_p), self.pos.shape[0], self.bvh.h, C.cast(C.pointer(sk), C.c_void_p),
+               C.cast(C.pointer(cam), C.c_void_p), _vp(c), threads, _vp(out), _vp(rays))
+        assert r == 0
+        self.rays = (int(rays[0]), int(rays[1]))
+        return out
+
+    def __del__(self):
+        try:
+            fn = self.o.lib.or_restir_destroy
+            fn.argtypes = [C.c_void_p]
+            fn(self.h)
+        except Exception:
+            pass
+
+
+def f16_round(o: Oracle, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.empty_like(x)
+    fn = o.lib.or_f16_round
+    fn.argtypes = [_P, C.c_uint32, _P]
+    fn(_vp(x), x.shape[0], _vp(out))
+    return out
+
+
+def atrous_probe(o: Oracle, col, var, depth, nrm, pass_idx: int) -> np.ndarray:
+    h, w = depth.shape
+    col = np.ascontiguousarray(col, np.float32)
+    var = np.ascontiguousarray(var, np.float32)
+    depth = np.ascontiguousarray(depth, np.float32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    out = np.empty((h, w, 3), np.float32)
+    fn = o.lib.or_atrous_probe
+    fn.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P]
+    fn(w, h, pass_idx, _vp(col), _vp(var), _vp(depth), _vp(nrm), _vp(out))
+    return out

@@ -4,7 +4,7 @@ cancellation and bad requests behave."""
 import numpy as np
 import pytest
 
-from appearance_amd.renderer import PathTracerGpu, default_config
+from appearance_amd.renderer import PathTracerGpu, plain_config
 from appearance_amd.scene import generate_scene
 from appearance_amd.serving import CANCELLED, DONE, Server
 from tests.scenes_util import camera
@@ -18,11 +18,11 @@ def scenes():
 
 
 def plain(scene, w, h, spp, bounces, rr=3, seed=0):
-    pt = PathTracerGpu(default_config(sample_count=spp, max_bounces=bounces, rr_start_bounce=rr,
+    pt = PathTracerGpu(plain_config(sample_count=spp, max_bounces=bounces, rr_start_bounce=rr,
                                       seed=seed))
     pt.set_scene(scene)
     pt.set_camera(scene.camera)
-    return pt.render_accum((w, h))[1]
+    return pt.render_accum((w, h), with_rgba=False)[1]
 
 
 def test_matches_plain_renderer(gpu_available, scenes):
