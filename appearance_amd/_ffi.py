@@ -119,6 +119,7 @@ _SIGS = {
     "apt_bvh_build_device": (C.c_int, [_P, C.c_uint32, C.POINTER(BvhOpts), C.POINTER(_P)]),
     "apt_bvh_destroy": (None, [_P]),
     "apt_bvh_get_stats": (C.c_int, [_P, C.POINTER(BvhStats)]),
+    "apt_bvh_export": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t]),
     "apt_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "apt_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "apt_trace_closest_host": (C.c_int, [_P, _P, C.c_uint32, _P]),

@@ -27,6 +27,7 @@ SOURCES = [
     "server.hip",
     "tuner.cpp",
     "restir.hip",
+    "bvh_gpu.hip",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

@@ -87,21 +87,37 @@ static apt_status resolve_opts(const apt_bvh_opts* opts, apt_bvh_opts& o) {
   return APT_OK;
 }
 
-apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
-                         apt_bvh** out) {
-  clear_error();
-  APT_REQUIRE(out && (tri_xyz || n_tris == 0), APT_ERR_INVALID_ARG, "apt_bvh_build: null argument");
+// Builds the binary tree (host or device builder), collapses it to the wide
+// layout and uploads it.  d_tri_xyz, when given, holds the same triangles on
+// the device.
+static apt_status build_impl(const float* tri_xyz, const float* d_tri_xyz, uint32_t n_tris,
+                             const apt_bvh_opts* opts, apt_bvh** out) {
   APT_REQUIRE(n_tris > 0, APT_ERR_INVALID_ARG, "apt_bvh_build: empty triangle list");
   APT_REQUIRE(n_tris < (1u << 28), APT_ERR_INVALID_ARG, "apt_bvh_build: %u triangles exceed 2^28",
               n_tris);
   apt_bvh_opts o;
   apt_status st = resolve_opts(opts, o);
   if (st != APT_OK) return st;
-  APT_REQUIRE(o.builder == APT_BUILDER_HOST, APT_ERR_UNSUPPORTED,
-              "apt_bvh_build: the device builder is not available in this build");
   auto t0 = std::chrono::steady_clock::now();
   BvhHostResult b2;
-  st = build_bvh2_host(tri_xyz, n_tris, o.max_leaf, o.bins, b2);
+  if (o.builder == APT_BUILDER_GPU) {
+    float* d_tmp = nullptr;
+    if (!d_tri_xyz) {
+      APT_HIP(hipMalloc(&d_tmp, (size_t)n_tris * 9 * sizeof(float)));
+      hipError_t e = hipMemcpyAsync(d_tmp, tri_xyz, (size_t)n_tris * 9 * sizeof(float),
+                                    hipMemcpyHostToDevice, lib_stream());
+      if (e != hipSuccess) {
+        (void)hipFree(d_tmp);
+        set_error("apt_bvh_build: upload failed: %s", hipGetErrorString(e));
+        return APT_ERR_DEVICE;
+      }
+      d_tri_xyz = d_tmp;
+    }
+    st = build_bvh2_device(d_tri_xyz, n_tris, o.max_leaf, o.bins, lib_stream(), b2);
+    if (d_tmp) (void)hipFree(d_tmp);
+  } else {
+    st = build_bvh2_host(tri_xyz, n_tris, o.max_leaf, o.bins, b2);
+  }
   if (st != APT_OK) return st;
   apt_bvh* bvh = new apt_bvh();
   st = upload_wide_bvh(tri_xyz, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh);
@@ -116,14 +132,22 @@ apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_op
   return APT_OK;
 }
 
+apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
+                         apt_bvh** out) {
+  clear_error();
+  APT_REQUIRE(out && (tri_xyz || n_tris == 0), APT_ERR_INVALID_ARG, "apt_bvh_build: null argument");
+  return build_impl(tri_xyz, nullptr, n_tris, opts, out);
+}
+
 apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
                                 apt_bvh** out) {
   clear_error();
   APT_REQUIRE(out && d_tri_xyz && n_tris > 0, APT_ERR_INVALID_ARG,
               "apt_bvh_build_device: null or empty argument");
+  // the collapse writes the triangles in leaf order from a host copy
   std::vector<float> h((size_t)n_tris * 9);
   APT_HIP(hipMemcpy(h.data(), d_tri_xyz, h.size() * sizeof(float), hipMemcpyDeviceToHost));
-  return apt_bvh_build(h.data(), n_tris, opts, out);
+  return build_impl(h.data(), d_tri_xyz, n_tris, opts, out);
 }
 
 void apt_bvh_destroy(apt_bvh* bvh) {
@@ -136,6 +160,19 @@ void apt_bvh_destroy(apt_bvh* bvh) {
 apt_status apt_bvh_get_stats(const apt_bvh* bvh, apt_bvh_stats* out) {
   APT_REQUIRE(bvh && out, APT_ERR_INVALID_ARG, "apt_bvh_get_stats: null argument");
   *out = bvh->stats;
+  return APT_OK;
+}
+
+apt_status apt_bvh_export(const apt_bvh* bvh, void* nodes, size_t nodes_len, void* tris,
+                          size_t tris_len) {
+  clear_error();
+  APT_REQUIRE(bvh, APT_ERR_INVALID_ARG, "apt_bvh_export: null bvh");
+  const size_t nb = (size_t)bvh->n_nodes * (16 + 24 * (size_t)bvh->width);
+  const size_t tb = (size_t)bvh->n_tris * 3 * sizeof(float4);
+  APT_REQUIRE(!nodes || nodes_len >= nb, APT_ERR_INVALID_ARG, "apt_bvh_export: nodes need %zu bytes", nb);
+  APT_REQUIRE(!tris || tris_len >= tb, APT_ERR_INVALID_ARG, "apt_bvh_export: triangles need %zu bytes", tb);
+  if (nodes && nb) APT_HIP(hipMemcpy(nodes, bvh->d_nodes, nb, hipMemcpyDeviceToHost));
+  if (tris && tb) APT_HIP(hipMemcpy(tris, bvh->d_tris, tb, hipMemcpyDeviceToHost));
   return APT_OK;
 }
 

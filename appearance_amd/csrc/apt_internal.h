@@ -87,6 +87,11 @@ struct BvhHostResult {
 // Binned-SAH build on the host (bvh_host.cpp).
 apt_status build_bvh2_host(const float* tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
                            BvhHostResult& out);
+// The same tree built on the device from device-resident triangles
+// (bvh_gpu.hip); synchronous on stream s.
+apt_status build_bvh2_device(const float* d_tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
+                             hipStream_t s, BvhHostResult& out);
+double bvh2_sah_cost(const std::vector<Node2>& nodes);
 // Scene-scale padding used for every node box (see box_pad in bvh_host.cpp).
 float scene_pad(const float* tri_xyz, uint32_t n);
 

@@ -119,15 +119,17 @@ struct Builder {
       const float scale = (float)bins / (cb.hi[axis] - cb.lo[axis]);
       const int bins_i = (int)bins;
       const uint32_t split = best_split;
-      uint32_t* m = std::partition(order + begin, order + end, [&](uint32_t idx) {
+      // stable: both halves keep the input order, so the tree is a function of
+      // the input alone and the device builder reproduces it exactly
+      uint32_t* m = std::stable_partition(order + begin, order + end, [&](uint32_t idx) {
         int b = (int)((prims[idx].c[axis] - lo) * scale);
         b = std::min(std::max(b, 0), bins_i - 1);
         return (uint32_t)b < split;
       });
       mid = (uint32_t)(m - order);
     } else {
-      // Coincident centroids (or too deep): object median on the input order,
-      // which is deterministic because the range was partitioned deterministically.
+      // Coincident centroids (or too deep): object median on the input order
+      // (stable partitions keep every range sorted; the sort only restates it).
       std::sort(order + begin, order + end);
     }
 
@@ -193,15 +195,20 @@ apt_status build_bvh2_host(const float* tri_xyz, uint32_t n, uint32_t max_leaf, 
   b.build(0, 0, n, 0);
   out.nodes.resize(b.next_node.load());
 
-  // SAH cost (C_trav = 1, C_isect = 1) relative to the root area.
-  double root_area = std::max(1e-30f, box_area(out.nodes[0].lo, out.nodes[0].hi));
+  out.sah_cost = bvh2_sah_cost(out.nodes);
+  return APT_OK;
+}
+
+// SAH cost (C_trav = 1, C_isect = 1) relative to the root area.
+double bvh2_sah_cost(const std::vector<Node2>& nodes) {
+  if (nodes.empty()) return 0.0;
+  double root_area = std::max(1e-30f, box_area(nodes[0].lo, nodes[0].hi));
   double cost = 0.0;
-  for (const Node2& nd : out.nodes) {
+  for (const Node2& nd : nodes) {
     double a = box_area(nd.lo, nd.hi) / root_area;
     cost += nd.count ? a * nd.count : a;
   }
-  out.sah_cost = cost;
-  return APT_OK;
+  return cost;
 }
 
 namespace {

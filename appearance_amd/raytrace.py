@@ -43,6 +43,15 @@ class Bvh:
         check(lib().apt_bvh_get_stats(self._h, C.byref(s)), "apt_bvh_get_stats")
         return {f: getattr(s, f) for f, _ in BvhStats._fields_}
 
+    def export(self):
+        """(node bytes, triangle records (n_tris*3, 4) float32) of the device layout."""
+        st = self.stats()
+        nodes = np.empty(st["n_nodes"] * (16 + 24 * st["width"]), np.uint8)
+        tris = np.empty((st["n_tris"] * 3, 4), np.float32)
+        check(lib().apt_bvh_export(self._h, vptr(nodes), nodes.size, vptr(tris), tris.nbytes),
+              "apt_bvh_export")
+        return nodes, tris
+
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
         """Host arrays in and out (synchronous)."""
         rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)

@@ -122,6 +122,13 @@ apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris,
                                 const apt_bvh_opts* opts, apt_bvh** out);
 void apt_bvh_destroy(apt_bvh* bvh);
 apt_status apt_bvh_get_stats(const apt_bvh* bvh, apt_bvh_stats* out);
+/* Copies the device layout out: n_nodes wide nodes (16 + 24 * width bytes
+ * each: six float[width] box rows, then width child refs) and 3 * n_tris
+ * float4 triangle records in leaf order (prim id in the first record's w).
+ * No counterpart in the reference (its BVH is opaque inside the driver); for
+ * callers that cache builds, and for comparing builders. */
+apt_status apt_bvh_export(const apt_bvh* bvh, void* nodes, size_t nodes_len, void* tris,
+                          size_t tris_len);
 
 /* Closest-hit query (ray flags 0 in the reference): d_rays/d_hits are device
  * arrays of n entries; stream is a hipStream_t (NULL = the library stream).
