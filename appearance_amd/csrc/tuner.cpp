@@ -79,8 +79,16 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
         js += buf;
         trials++;
         if (!same) {
+          size_t nd = 0, first = 0;
+          for (size_t i = 0; i < img.size(); ++i) {
+            if (std::memcmp(&ref[i], &img[i], sizeof(float)) != 0) {
+              if (nd++ == 0) first = i;
+            }
+          }
           apt_destroy(r);
-          apt::set_error("apt_autotune: candidate width %u stack %u chunk %u changed the image", w, s, q);
+          apt::set_error("apt_autotune: candidate width %u stack %u chunk %u changed the image: %zu of %zu "
+                         "values differ, first at %zu: %.9g vs %.9g",
+                         w, s, q, nd, img.size(), first, (double)img[first], (double)ref[first]);
           return APT_ERR_DEVICE;
         }
         if (ms_best < best) {
