@@ -35,7 +35,7 @@ class Hit(C.Structure):
 
 class BvhOpts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("max_leaf", C.c_uint32), ("bins", C.c_uint32),
-                ("builder", C.c_uint32), ("lds_stack", C.c_uint32)]
+                ("builder", C.c_uint32), ("lds_stack", C.c_uint32), ("refill", C.c_uint32)]
 
 
 class BvhStats(C.Structure):

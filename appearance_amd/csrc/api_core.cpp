@@ -74,6 +74,8 @@ static apt_status resolve_opts(const apt_bvh_opts* opts, apt_bvh_opts& o) {
   if (o.max_leaf == 0) o.max_leaf = 4;
   if (o.bins == 0) o.bins = 32;
   if (o.lds_stack == 0) o.lds_stack = 8;
+  if (o.refill == 0) o.refill = 32;
+  APT_REQUIRE(o.refill <= 65, APT_ERR_INVALID_ARG, "bvh: refill %u outside 1..65", o.refill);
   APT_REQUIRE(o.lds_stack == 8 || o.lds_stack == 16 || o.lds_stack == 24, APT_ERR_INVALID_ARG,
               "bvh: lds_stack %u is not 8, 16 or 24", o.lds_stack);
   APT_REQUIRE(o.width == 2 || o.width == 4 || o.width == 8, APT_ERR_INVALID_ARG,
@@ -126,6 +128,7 @@ static apt_status build_impl(const float* tri_xyz, const float* d_tri_xyz, uint3
     return st;
   }
   bvh->lds_stack = o.lds_stack;
+  bvh->refill = o.refill;
   bvh->stats.build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   *out = bvh;
@@ -154,6 +157,7 @@ void apt_bvh_destroy(apt_bvh* bvh) {
   if (!bvh) return;
   if (bvh->d_nodes) (void)hipFree(bvh->d_nodes);
   if (bvh->d_tris) (void)hipFree(bvh->d_tris);
+  if (bvh->d_counters) (void)hipFree(bvh->d_counters);
   delete bvh;
 }
 

@@ -11,6 +11,8 @@ enum Counter : uint32_t {
   CNT_SHADOW = 2,
   CNT_STAT_EXT = 3,     // extension rays traced (running total)
   CNT_STAT_SHADOW = 4,  // shadow rays traced (running total)
+  CNT_POOL_EXT = 5,     // ray-pool counter of the extend stage (zeroed by the kernel before it)
+  CNT_POOL_SHADOW = 6,  // ray-pool counter of the shadow stage
   CNT_COUNT = 8
 };
 
@@ -33,6 +35,7 @@ struct SceneView {
   uint32_t root;
   uint32_t width;
   uint32_t lds_stack;           // traversal stack entries in LDS (8, 16, 24)
+  uint32_t refill;              // persistent traversal refill threshold (65: one ray per thread)
   const float4* tri_pos;        // 3 per triangle
   const float4* tri_nrm;        // 3 per triangle, or null (geometric normals)
   const uint32_t* tri_mat;

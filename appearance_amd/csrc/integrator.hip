@@ -9,6 +9,8 @@
 // ballot, a popcount prefix and one atomic per wave.  Each path owns its RNG
 // state and its radiance sum, so queue order never changes a result; the film
 // adds samples to a pixel in sample order.
+#include <algorithm>
+
 #include "pathtrace_device.h"
 
 namespace apt {
@@ -25,6 +27,7 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(FrameView fv, SceneView sc, Pa
     ps.counters[CNT_QUEUE0] = n_paths;
     ps.counters[CNT_QUEUE1] = 0;
     ps.counters[CNT_SHADOW] = 0;
+    ps.counters[CNT_POOL_EXT] = 0;
   }
   if (p >= n_paths) return;
   const uint32_t npix = fv.width * fv.rows;
@@ -35,23 +38,39 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(FrameView fv, SceneView sc, Pa
   ps.queue[0][p] = p;
 }
 
+// Extension rays of the queued paths (persistent ray pool over the queue).
+template <int W>
+struct ExtendSrc {
+  static constexpr bool kOneScene = true;
+  const WNode<W>* nodes;
+  const float4* tris;
+  uint32_t root;
+  PathView ps;
+  const uint32_t* queue;
+  __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray) const {
+    const uint32_t p = queue[i];
+    const float4 o = ps.o_pdf[p], d = ps.d_flags[p];
+    ray = make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f);
+  }
+  __device__ __forceinline__ void closest(uint32_t i, const HitRec& h) const {
+    ps.hit[queue[i]] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+  }
+};
+
 template <int W, int S>
-__global__ __launch_bounds__(BLOCK) void k_extend(SceneView sc, PathView ps, int qin,
-                                                  const WNode<W>* __restrict__ nodes) {
+__global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, int qin,
+                                                     const WNode<W>* __restrict__ nodes) {
   __shared__ uint2 s_stack[S * BLOCK];
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
     ps.counters[CNT_SHADOW] = 0;
+    ps.counters[CNT_POOL_SHADOW] = 0;
     ps.counters[CNT_STAT_EXT] += count;
   }
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t p = ps.queue[qin][i];
-  const float4 o = ps.o_pdf[p], d = ps.d_flags[p];
-  TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f);
-  HitRec h = trace_closest<W, S>(nodes, sc.bvh_tris, sc.root, ray, LdsStack{s_stack + threadIdx.x, BLOCK});
-  ps.hit[p] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+  ExtendSrc<W> src{nodes, sc.bvh_tris, sc.root, ps, ps.queue[qin]};
+  trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], min(sc.refill, 64u), s_stack + threadIdx.x,
+                          BLOCK);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, PathView ps, int qin,
@@ -70,22 +89,41 @@ __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, Pat
   if (sslot >= 0) ps.shadow_queue[sslot] = p;
 }
 
-template <int W, int S>
-__global__ __launch_bounds__(BLOCK) void k_shadow(SceneView sc, PathView ps,
-                                                  const WNode<W>* __restrict__ nodes) {
-  __shared__ uint2 s_stack[S * BLOCK];
-  const uint32_t count = ps.counters[CNT_SHADOW];
-  if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[CNT_STAT_SHADOW] += count;
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t p = ps.shadow_queue[i];
-  const float4 o = ps.sh_o[p], d = ps.sh_d[p];
-  TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, o.w);
-  if (!trace_any<W, S>(nodes, sc.bvh_tris, sc.root, ray, LdsStack{s_stack + threadIdx.x, BLOCK})) {
+// Shadow rays of the paths that asked for one; unoccluded ones add their
+// contribution.
+template <int W>
+struct ShadowSrc {
+  static constexpr bool kOneScene = true;
+  const WNode<W>* nodes;
+  const float4* tris;
+  uint32_t root;
+  PathView ps;
+  __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray) const {
+    const uint32_t p = ps.shadow_queue[i];
+    const float4 o = ps.sh_o[p], d = ps.sh_d[p];
+    ray = make_trace_ray(ld3(o), ld3(d), 0.0f, o.w);
+  }
+  __device__ __forceinline__ void any(uint32_t i, bool hit) const {
+    if (hit) return;
+    const uint32_t p = ps.shadow_queue[i];
     const float4 c = ps.sh_c[p];
-    float4 l = ps.rad[p];
+    const float4 l = ps.rad[p];
     ps.rad[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
   }
+};
+
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK, 8) void k_shadow(SceneView sc, PathView ps,
+                                                     const WNode<W>* __restrict__ nodes) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  const uint32_t count = ps.counters[CNT_SHADOW];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ps.counters[CNT_STAT_SHADOW] += count;
+    ps.counters[CNT_POOL_EXT] = 0;  // for the next bounce's extend stage
+  }
+  ShadowSrc<W> src{nodes, sc.bvh_tris, sc.root, ps};
+  trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], min(sc.refill, 64u), s_stack + threadIdx.x,
+                         BLOCK);
 }
 
 // Adds this pass's samples of every pixel to its accumulator, one sample at a
@@ -150,11 +188,13 @@ template <int W>
 void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_paths, int qin,
                            const FrameView& fv, uint32_t bounce, hipStream_t s) {
   const dim3 grid((n_paths + BLOCK - 1) / BLOCK);
+  // the traversal stages pull from their queue: a grid that fills the device
+  const dim3 pool(std::min<uint32_t>(pool_blocks(), (n_paths + 63) / 64));
   const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
   APT_FOR_STACK_DEPTH(sc.lds_stack, {
-    hipLaunchKernelGGL((k_extend<W, S_>), grid, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
+    hipLaunchKernelGGL((k_extend<W, S_>), pool, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
     hipLaunchKernelGGL(k_shade, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
-    hipLaunchKernelGGL((k_shadow<W, S_>), grid, dim3(BLOCK), 0, s, sc, ps, nodes);
+    hipLaunchKernelGGL((k_shadow<W, S_>), pool, dim3(BLOCK), 0, s, sc, ps, nodes);
   })
 }
 

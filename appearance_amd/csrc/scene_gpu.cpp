@@ -81,7 +81,8 @@ SceneView scene_gpu_view(const SceneGpu& s, const apt_camera& cam, uint32_t widt
   sc.bvh_tris = s.bvh ? s.bvh->d_tris : nullptr;
   sc.root = s.bvh ? s.bvh->root : 0;
   sc.width = s.bvh ? s.bvh->width : 4;
-  sc.lds_stack = s.bvh ? s.bvh->lds_stack : 16;
+  sc.lds_stack = s.bvh ? s.bvh->lds_stack : 8;
+  sc.refill = s.bvh ? s.bvh->refill : 32;
   sc.tri_pos = s.d_pos;
   sc.tri_nrm = s.d_nrm;
   sc.tri_mat = s.d_mat;

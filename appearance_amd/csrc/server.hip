@@ -9,6 +9,8 @@
 // request's accumulator with 64-bit fixed-point atomics: integer addition is
 // associative, so a request's sum does not depend on the order its paths
 // finish in or on what else shared the batch.
+#include <algorithm>
+
 #include "pathtrace_device.h"
 #include "server.h"
 
@@ -41,6 +43,7 @@ __global__ __launch_bounds__(BLOCK) void k_srv_raygen(const SrvBatch* __restrict
     ps.counters[CNT_QUEUE0] = n_paths;
     ps.counters[CNT_QUEUE1] = 0;
     ps.counters[CNT_SHADOW] = 0;
+    ps.counters[CNT_POOL_EXT] = 0;
   }
   if (p >= n_paths) return;
   const SrvTile& t = batch->tiles[find_tile(batch->tiles, batch->n_tiles, p)];
@@ -55,26 +58,44 @@ __global__ __launch_bounds__(BLOCK) void k_srv_raygen(const SrvBatch* __restrict
   ps.queue[0][p] = p;
 }
 
+constexpr uint32_t SRV_REFILL = 32;  // persistent traversal refill threshold
+
+// Rays of paths from many requests (and scenes) in one launch.
+template <int W>
+struct SrvExtendSrc {
+  static constexpr bool kOneScene = false;
+  const SrvReq* reqs;
+  PathView ps;
+  const uint32_t* path_req;
+  const uint32_t* queue;
+  __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray, const WNode<W>*& nodes, const float4*& tris,
+                                        uint32_t& root) const {
+    const uint32_t p = queue[i];
+    const SceneView& sc = reqs[path_req[p]].sc;
+    nodes = static_cast<const WNode<W>*>(sc.nodes);
+    tris = sc.bvh_tris;
+    root = sc.root;
+    const float4 o = ps.o_pdf[p], d = ps.d_flags[p];
+    ray = make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f);
+  }
+  __device__ __forceinline__ void closest(uint32_t i, const HitRec& h) const {
+    ps.hit[queue[i]] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+  }
+};
+
 template <int W, int S>
-__global__ __launch_bounds__(BLOCK) void k_srv_extend(const SrvReq* __restrict__ reqs, PathView ps,
-                                                      const uint32_t* __restrict__ path_req,
-                                                      int qin) {
+__global__ __launch_bounds__(BLOCK, 8) void k_srv_extend(const SrvReq* __restrict__ reqs, PathView ps,
+                                                         const uint32_t* __restrict__ path_req, int qin) {
   __shared__ uint2 s_stack[S * BLOCK];
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
     ps.counters[CNT_SHADOW] = 0;
+    ps.counters[CNT_POOL_SHADOW] = 0;
     ps.counters[CNT_STAT_EXT] += count;
   }
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t p = ps.queue[qin][i];
-  const SceneView& sc = reqs[path_req[p]].sc;
-  const float4 o = ps.o_pdf[p], d = ps.d_flags[p];
-  TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f);
-  HitRec h = trace_closest<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray,
-                                 LdsStack{s_stack + threadIdx.x, BLOCK});
-  ps.hit[p] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+  SrvExtendSrc<W> src{reqs, ps, path_req, ps.queue[qin]};
+  trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], SRV_REFILL, s_stack + threadIdx.x, BLOCK);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_srv_shade(const SrvReq* __restrict__ reqs, PathView ps,
@@ -95,24 +116,42 @@ __global__ __launch_bounds__(BLOCK) void k_srv_shade(const SrvReq* __restrict__ 
   if (sslot >= 0) ps.shadow_queue[sslot] = p;
 }
 
-template <int W, int S>
-__global__ __launch_bounds__(BLOCK) void k_srv_shadow(const SrvReq* __restrict__ reqs, PathView ps,
-                                                      const uint32_t* __restrict__ path_req) {
-  __shared__ uint2 s_stack[S * BLOCK];
-  const uint32_t count = ps.counters[CNT_SHADOW];
-  if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[CNT_STAT_SHADOW] += count;
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t p = ps.shadow_queue[i];
-  const SceneView& sc = reqs[path_req[p]].sc;
-  const float4 o = ps.sh_o[p], d = ps.sh_d[p];
-  TraceRay ray = make_trace_ray(ld3(o), ld3(d), 0.0f, o.w);
-  if (!trace_any<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray,
-                       LdsStack{s_stack + threadIdx.x, BLOCK})) {
+template <int W>
+struct SrvShadowSrc {
+  static constexpr bool kOneScene = false;
+  const SrvReq* reqs;
+  PathView ps;
+  const uint32_t* path_req;
+  __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray, const WNode<W>*& nodes, const float4*& tris,
+                                        uint32_t& root) const {
+    const uint32_t p = ps.shadow_queue[i];
+    const SceneView& sc = reqs[path_req[p]].sc;
+    nodes = static_cast<const WNode<W>*>(sc.nodes);
+    tris = sc.bvh_tris;
+    root = sc.root;
+    const float4 o = ps.sh_o[p], d = ps.sh_d[p];
+    ray = make_trace_ray(ld3(o), ld3(d), 0.0f, o.w);
+  }
+  __device__ __forceinline__ void any(uint32_t i, bool hit) const {
+    if (hit) return;
+    const uint32_t p = ps.shadow_queue[i];
     const float4 c = ps.sh_c[p];
-    float4 l = ps.rad[p];
+    const float4 l = ps.rad[p];
     ps.rad[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
   }
+};
+
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK, 8) void k_srv_shadow(const SrvReq* __restrict__ reqs, PathView ps,
+                                                         const uint32_t* __restrict__ path_req) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  const uint32_t count = ps.counters[CNT_SHADOW];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ps.counters[CNT_STAT_SHADOW] += count;
+    ps.counters[CNT_POOL_EXT] = 0;  // for the next bounce's extend stage
+  }
+  SrvShadowSrc<W> src{reqs, ps, path_req};
+  trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], SRV_REFILL, s_stack + threadIdx.x, BLOCK);
 }
 
 __device__ __forceinline__ unsigned long long to_fixed(float v) {
@@ -153,12 +192,13 @@ __global__ void k_srv_zero(unsigned long long* __restrict__ a, size_t n) {
 template <int W, int S>
 void enqueue_batch_t(const SrvDevice& d, hipStream_t s) {
   const dim3 grid((d.path_slots + BLOCK - 1) / BLOCK), block(BLOCK);
+  const dim3 pool(std::min<uint32_t>(pool_blocks(), (d.path_slots + 63) / 64));
   hipLaunchKernelGGL(k_srv_raygen, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);
   for (uint32_t b = 0; b < d.max_bounces; ++b) {
     const int qin = (int)(b & 1u);
-    hipLaunchKernelGGL((k_srv_extend<W, S>), grid, block, 0, s, d.reqs, d.ps, d.path_req, qin);
+    hipLaunchKernelGGL((k_srv_extend<W, S>), pool, block, 0, s, d.reqs, d.ps, d.path_req, qin);
     hipLaunchKernelGGL(k_srv_shade, grid, block, 0, s, d.reqs, d.ps, d.path_req, qin, b);
-    hipLaunchKernelGGL((k_srv_shadow<W, S>), grid, block, 0, s, d.reqs, d.ps, d.path_req);
+    hipLaunchKernelGGL((k_srv_shadow<W, S>), pool, block, 0, s, d.reqs, d.ps, d.path_req);
   }
   hipLaunchKernelGGL(k_srv_film, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);
 }

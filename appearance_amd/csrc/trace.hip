@@ -1,12 +1,22 @@
 // Wide-BVH traversal kernels for gfx950: closest hit and any hit.
 //
 // These replace the reference's hardware ray query (rayQueryInitialize /
-// rayQueryProceed / rayQueryGetCommittedIntersection, trace.wgsl:159-162 and
-// helpers/trace.wgsl:20-26).  One lane traces one ray.  The traversal stack
-// keeps its bottom S entries in LDS (laid out [entry][lane] so lanes at the
-// same depth hit distinct banks; S = 8, 16 or 24 per BVH, trading LDS for
-// occupancy) and spills deeper entries to a small private array; the host
-// checks that the BVH's worst-case stack fits.
+// rayQueryProceed / rayQueryGetCommittedIntersection, trace.wgsl:98-102 and
+// helpers/trace.wgsl:20-26).  The traversal stack keeps its bottom S entries
+// in LDS (laid out [entry][lane] so lanes at the same depth hit distinct
+// banks; S = 8, 16 or 24 per BVH, trading LDS for occupancy) and spills
+// deeper entries to a small private array; the host checks that the BVH's
+// worst-case stack fits.
+//
+// Two schedules.  k_trace gives each lane one ray.  k_trace_pool is
+// persistent: a grid that just fills the device pulls rays from a shared
+// counter, and a wave refetches as soon as `refill` of its 64 lanes have
+// finished their rays, so lanes no longer idle behind the slowest ray of
+// their wave (persistent threads with dynamic ray fetch, Aila and Laine,
+// HPG 2009).  Both return the same hits: each ray's traversal is the same
+// while-while loop.
+#include <algorithm>
+
 #include "apt_internal.h"
 #include "trace_device.h"
 
@@ -40,20 +50,94 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const WNode<W>* __restrict__ no
   }
 }
 
+__device__ __forceinline__ TraceRay load_ray(const apt_ray* __restrict__ rays, uint32_t i) {
+  const float4* rp = reinterpret_cast<const float4*>(rays) + 2 * (size_t)i;
+  const float4 r0 = rp[0];
+  const float4 r1 = rp[1];
+  return make_trace_ray(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w);
+}
+
+__global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
+
+// Rays and results of the ray-query entry points for trace_pool.
+template <int W>
+struct RaySrc {
+  static constexpr bool kOneScene = true;
+  const WNode<W>* nodes;
+  const float4* tris;
+  uint32_t root;
+  const apt_ray* rays;
+  apt_hit* hits;
+  uint32_t* occluded;
+  __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray) const { ray = load_ray(rays, i); }
+  __device__ __forceinline__ void closest(uint32_t i, const HitRec& h) const {
+    reinterpret_cast<float4*>(hits)[i] =
+        make_float4(h.prim == INVALID ? INFINITY : h.t, h.u, h.v, u2f(h.prim));
+  }
+  __device__ __forceinline__ void any(uint32_t i, bool hit) const { occluded[i] = hit ? 1u : 0u; }
+};
+
+template <int W, int S, bool ANY>
+__global__ __launch_bounds__(BLOCK, 8) void k_trace_pool(const WNode<W>* __restrict__ nodes,
+                                                         const float4* __restrict__ tris, uint32_t root,
+                                                         const apt_ray* __restrict__ rays, uint32_t n,
+                                                         apt_hit* __restrict__ hits,
+                                                         uint32_t* __restrict__ occluded,
+                                                         uint32_t* __restrict__ counter, uint32_t refill) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  RaySrc<W> src{nodes, tris, root, rays, hits, occluded};
+  trace_pool<W, S, ANY>(src, n, counter, refill, s_stack + threadIdx.x, BLOCK);
+}
+
+// Blocks of k_trace_pool that fit on the device at once.
+template <int W, int S, bool ANY>
+uint32_t pool_grid() {
+  static uint32_t g = 0;
+  if (!g) {
+    int per_cu = 0, dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_pool<W, S, ANY>, BLOCK, 0) == hipSuccess &&
+        per_cu > 0)
+      g = (uint32_t)per_cu * (uint32_t)p.multiProcessorCount;
+    else
+      g = 2048;
+  }
+  return g;
+}
+
 template <int W>
 apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
                   uint32_t* d_occ, bool any, hipStream_t s) {
   if (n == 0) return APT_OK;
   const WNode<W>* nodes = static_cast<const WNode<W>*>(bvh->d_nodes);
-  dim3 grid((n + BLOCK - 1) / BLOCK);
-  APT_FOR_STACK_DEPTH(bvh->lds_stack, {
-    if (any)
-      hipLaunchKernelGGL((k_trace<W, S_, true>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
-                         bvh->root, d_rays, n, d_hits, d_occ);
-    else
-      hipLaunchKernelGGL((k_trace<W, S_, false>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
-                         bvh->root, d_rays, n, d_hits, d_occ);
-  })
+  if (bvh->refill > 64) {
+    dim3 grid((n + BLOCK - 1) / BLOCK);
+    APT_FOR_STACK_DEPTH(bvh->lds_stack, {
+      if (any)
+        hipLaunchKernelGGL((k_trace<W, S_, true>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
+                           bvh->root, d_rays, n, d_hits, d_occ);
+      else
+        hipLaunchKernelGGL((k_trace<W, S_, false>), grid, dim3(BLOCK), 0, s, nodes, bvh->d_tris,
+                           bvh->root, d_rays, n, d_hits, d_occ);
+    })
+  } else {
+    // one counter of the BVH's ring per call: calls in flight on different
+    // streams (up to the ring size) do not share a counter
+    const uint32_t slot = __atomic_fetch_add(&bvh->counter_next, 1u, __ATOMIC_RELAXED) % APT_BVH_COUNTERS;
+    uint32_t* counter = bvh->d_counters + slot;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
+    APT_FOR_STACK_DEPTH(bvh->lds_stack, {
+      if (any)
+        hipLaunchKernelGGL((k_trace_pool<W, S_, true>), dim3(std::min(pool_grid<W, S_, true>(), (n + 63) / 64)),
+                           dim3(BLOCK), 0, s, nodes, bvh->d_tris, bvh->root, d_rays, n, d_hits, d_occ,
+                           counter, bvh->refill);
+      else
+        hipLaunchKernelGGL((k_trace_pool<W, S_, false>), dim3(std::min(pool_grid<W, S_, false>(), (n + 63) / 64)),
+                           dim3(BLOCK), 0, s, nodes, bvh->d_tris, bvh->root, d_rays, n, d_hits, d_occ,
+                           counter, bvh->refill);
+    })
+  }
   APT_HIP(hipGetLastError());
   return APT_OK;
 }

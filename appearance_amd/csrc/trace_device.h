@@ -257,8 +257,6 @@ __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
   return false;
 }
 
-#undef APT_STACK_PUSH
-#undef APT_STACK_POP
 
 // Host-side dispatch over the supported LDS stack depths.
 #define APT_FOR_STACK_DEPTH(S, ...) \
@@ -267,5 +265,165 @@ __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
     case 16: { constexpr int S_ = 16; __VA_ARGS__; } break; \
     default: { constexpr int S_ = 24; __VA_ARGS__; } break; \
   }
+
+// 256-thread blocks that fill the device at 8 waves per SIMD (8 blocks per
+// CU), the grid of the persistent traversal kernels; blocks that do not fit
+// start late, find the ray pool drained and exit.
+inline uint32_t pool_blocks() {
+  static uint32_t g = 0;
+  if (!g) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    g = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            ? 8u * (uint32_t)p.multiProcessorCount
+            : 2048u;
+  }
+  return g;
+}
+
+// Persistent traversal with dynamic ray fetch (Aila and Laine, HPG 2009),
+// shared by the ray-query kernels and the integrators' extend and shadow
+// stages.  A grid that just fills the device pulls ray indices [0, n) from
+// *counter (zeroed by an earlier kernel on the stream); a wave refetches once
+// `refill` of its 64 lanes are idle, so lanes do not wait for the slowest ray
+// of their wave.  Each lane runs one round of the while-while loop per
+// iteration (inner nodes to a leaf, the leaf, the next stack entry), so every
+// ray gets exactly the traversal of trace_closest / trace_any.
+//
+// Src supplies the rays: with kOneScene, members nodes/tris/root and
+// `void fetch(uint32_t i, TraceRay& ray)`; otherwise `void fetch(uint32_t i,
+// TraceRay& ray, const WNode<W>*& nodes, const float4*& tris, uint32_t& root)`
+// (rays of different scenes in one launch); and `void closest(uint32_t i,
+// const HitRec&)` or `void any(uint32_t i, bool hit)`.
+template <int W, int S, bool ANY, class Src>
+__device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
+                                           uint2* const lds, const int stride) {
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  bool active = false, drained = false;
+  uint32_t rid = 0, ref = 0, root = 0;
+  const WNode<W>* nodes = nullptr;
+  const float4* tris = nullptr;
+  if constexpr (Src::kOneScene) {  // one BVH for every ray: keep its pointers scalar
+    nodes = src.nodes;
+    tris = src.tris;
+    root = src.root;
+  }
+  TraceRay ray;
+  HitRec best{0.0f, 0.0f, 0.0f, INVALID};
+  while (true) {
+    const unsigned long long idle = __ballot(!active);
+    if (!drained && idle != 0ull) {
+      const uint32_t nidle = (uint32_t)__popcll(idle);
+      if (nidle >= refill || __ballot(active) == 0ull) {
+        const int leader = __ffsll(idle) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(counter, nidle);
+        base = __shfl(base, leader);
+        if (!active) {
+          const uint32_t r = base + (uint32_t)__popcll(idle & below);
+          if (r < n) {
+            rid = r;
+            if constexpr (Src::kOneScene) src.fetch(r, ray);
+            else src.fetch(r, ray, nodes, tris, root);
+            best = HitRec{ray.pre.tmax, 0.0f, 0.0f, INVALID};
+            ref = root;
+            sp = 0;
+            active = true;
+          }
+        }
+        if (base + nidle >= n) drained = true;
+      }
+    }
+    if (__ballot(active) == 0ull) break;
+    if (!active) continue;
+    bool have = true;
+    while (!ref_is_leaf(ref)) {
+      WNode<W> nd;
+      load_node<W>(nodes, ref, nd);
+      float dist[W];
+      child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best.t, dist);
+      if (ANY) {
+        uint32_t next = REF_EMPTY;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+          if (dist[k] == INFINITY) continue;
+          if (next == REF_EMPTY) next = nd.child[k];
+          else APT_STACK_PUSH(nd.child[k], dist[k]);
+        }
+        if (next != REF_EMPTY) {
+          ref = next;
+          continue;
+        }
+        if (sp == 0) {
+          have = false;
+          break;
+        }
+        ref = APT_STACK_POP().x;
+      } else {
+        uint32_t refs[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
+        sort_children<W>(dist, refs);
+        if (dist[0] != INFINITY) {
+#pragma unroll
+          for (int k = W - 1; k >= 1; --k)
+            if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
+          ref = refs[0];
+          continue;
+        }
+        have = false;
+        while (sp > 0) {
+          const uint2 e = APT_STACK_POP();
+          if (u2f(e.y) <= best.t) {
+            ref = e.x;
+            have = true;
+            break;
+          }
+        }
+        if (!have) break;
+      }
+    }
+    bool done = !have;
+    bool occluded = false;
+    if (have) {
+      const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
+      for (uint32_t j = first; j < first + cnt; ++j) {
+        TriHit h;
+        uint32_t prim;
+        if (leaf_tri(tris, j, ray, h, prim)) {
+          if (ANY) {
+            occluded = true;
+            break;
+          }
+          if (hit_less(h.t, prim, best.t, best.prim)) {
+            best.t = h.t;
+            best.u = h.u;
+            best.v = h.v;
+            best.prim = prim;
+          }
+        }
+      }
+      done = true;
+      if (!occluded) {
+        while (sp > 0) {
+          const uint2 e = APT_STACK_POP();
+          if (ANY || u2f(e.y) <= best.t) {
+            ref = e.x;
+            done = false;
+            break;
+          }
+        }
+      }
+    }
+    if (done) {
+      if constexpr (ANY) src.any(rid, occluded);
+      else src.closest(rid, best);
+      active = false;
+    }
+  }
+}
 
 }  // namespace apt

@@ -25,10 +25,10 @@ class Bvh:
     """A wide BVH resident in device memory (libapt's apt_bvh)."""
 
     def __init__(self, tris: np.ndarray, width: int = 4, max_leaf: int = 4, bins: int = 32,
-                 builder: int = BUILDER_HOST, lds_stack: int = 8):
+                 builder: int = BUILDER_HOST, lds_stack: int = 8, refill: int = 0):
         tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
         self.n_tris = tris.shape[0]
-        opts = BvhOpts(width, max_leaf, bins, builder, lds_stack)
+        opts = BvhOpts(width, max_leaf, bins, builder, lds_stack, refill)
         h = C.c_void_p()
         check(lib().apt_bvh_build(_ffi.ptr(tris), self.n_tris, C.byref(opts), C.byref(h)),
               "apt_bvh_build")

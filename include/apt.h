@@ -96,6 +96,8 @@ typedef struct apt_bvh_opts {
   uint32_t bins;      /* SAH bins per axis, 4..64 (0 = 32) */
   uint32_t builder;   /* apt_builder */
   uint32_t lds_stack; /* traversal stack entries kept in LDS: 8, 16 or 24 (0 = 8) */
+  uint32_t refill;    /* ray queries: persistent waves refetch rays once this many of their
+                         64 lanes are idle, 1..64; 65 = one ray per thread (0 = 32) */
 } apt_bvh_opts;
 
 typedef struct apt_bvh_stats {
