@@ -163,30 +163,32 @@ APT_HD v3 unpack_oct30(uint32_t p) {
 // ---------------------------------------------------------------------------
 struct RayPre {
   v3 o;
-  int kx, ky, kz;
+  int kz;  // dominant axis of d; the other two follow it cyclically
   float Sx, Sy, Sz;
   float tmin, tmax;
 };
 
+// Components of a in the ray's frame: (a[kx], a[ky], a[kz]) with
+// kx = kz+1, ky = kz+2 (mod 3).
+APT_HD v3 ray_frame(v3 a, int kz) {
+  const bool z0 = kz == 0, z1 = kz == 1;
+  return mk(z0 ? a.y : (z1 ? a.z : a.x), z0 ? a.z : (z1 ? a.x : a.y), z0 ? a.x : (z1 ? a.y : a.z));
+}
+
+// Woop et al. swap kx and ky when d[kz] < 0 to keep the triangles' winding.
+// That swap negates U, V, W, det and T exactly (every product and sum changes
+// sign, and round-to-nearest is symmetric), so t = T/det, u and v are the
+// same without it; it is left out, which keeps kx and ky implied by kz.
 APT_HD RayPre ray_pre(v3 o, v3 d, float tmin, float tmax) {
   RayPre r;
   r.o = o;
   float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-  int kx = kz + 1 == 3 ? 0 : kz + 1;
-  int ky = kx + 1 == 3 ? 0 : kx + 1;
-  float dz = comp(d, kz);
-  if (dz < 0.0f) {
-    int tmp = kx;
-    kx = ky;
-    ky = tmp;
-  }
-  r.kx = kx;
-  r.ky = ky;
+  const v3 f = ray_frame(d, kz);
   r.kz = kz;
-  r.Sx = comp(d, kx) / dz;
-  r.Sy = comp(d, ky) / dz;
-  r.Sz = 1.0f / dz;
+  r.Sx = f.x / f.z;
+  r.Sy = f.y / f.z;
+  r.Sz = 1.0f / f.z;
   r.tmin = tmin;
   r.tmax = tmax;
   return r;
@@ -197,14 +199,14 @@ struct TriHit {
 };
 
 APT_HD bool intersect_tri(const RayPre& r, v3 p0, v3 p1, v3 p2, TriHit& h) {
-  v3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
-  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-  float Ax = comp(A, r.kx) - r.Sx * Akz;
-  float Ay = comp(A, r.ky) - r.Sy * Akz;
-  float Bx = comp(B, r.kx) - r.Sx * Bkz;
-  float By = comp(B, r.ky) - r.Sy * Bkz;
-  float Cx = comp(C, r.kx) - r.Sx * Ckz;
-  float Cy = comp(C, r.ky) - r.Sy * Ckz;
+  const v3 A = ray_frame(p0 - r.o, r.kz), B = ray_frame(p1 - r.o, r.kz), C = ray_frame(p2 - r.o, r.kz);
+  const float Akz = A.z, Bkz = B.z, Ckz = C.z;
+  float Ax = A.x - r.Sx * Akz;
+  float Ay = A.y - r.Sy * Akz;
+  float Bx = B.x - r.Sx * Bkz;
+  float By = B.y - r.Sy * Bkz;
+  float Cx = C.x - r.Sx * Ckz;
+  float Cy = C.y - r.Sy * Ckz;
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;

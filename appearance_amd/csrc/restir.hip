@@ -318,14 +318,17 @@ __device__ __forceinline__ bool res_update(float& wsum, float& M, float w, uint3
 
 __device__ __forceinline__ DiRes di_empty() { return DiRes{LIGHT_EMPTY, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}; }
 
-// The two updates of DiReservoir::combine / combine_unbiased (:99-133).
-__device__ __forceinline__ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng) {
+// The two updates of DiReservoir::combine / combine_unbiased (:99-133);
+// `second` tells whether r2's sample was kept.
+__device__ __forceinline__ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng, bool& second) {
   DiRes c = di_empty();
+  second = false;
   if (res_update(c.wsum, c.M, r1.phat * r1.W * r1.M, rng)) {
     c.light = r1.light; c.u = r1.u; c.v = r1.v; c.phat = r1.phat;
   }
   if (res_update(c.wsum, c.M, r2.phat * r2.W * r2.M, rng)) {
     c.light = r2.light; c.u = r2.u; c.v = r2.v; c.phat = r2.phat;
+    second = true;
   }
   c.M = r1.M + r2.M;
   return c;
@@ -616,7 +619,8 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_temporal(RestirParams p, SceneV
       pr.M = fminf(pr.M, (float)p.di_mcap * r.M);
       const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
       pr.phat = di_phat<W, S>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
-      DiRes c = di_merge(r, pr, rng);
+      bool second;
+      DiRes c = di_merge(r, pr, rng, second);
       if (c.phat > RESTIR_DI_EPSILON && c.M * c.wsum > RESTIR_DI_EPSILON)
         c.W = (1.0f / c.phat) * (1.0f / c.M * c.wsum);
       r = c;
@@ -688,9 +692,14 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_spatial(RestirParams p, SceneVi
       valid = nr.phat > 0.0f;
       // DiReservoir::combine_unbiased: Z counts the M of the domains where the
       // chosen sample has a (visibility-tested) target > 0
-      DiRes c = di_merge(r, nr, rng);
+      bool second;
+      DiRes c = di_merge(r, nr, rng, second);
       float z = 0.0f;
-      if (di_phat<W, S>(sc, st, v, bs, c.light, c.u, c.v, true, b.counters) > 0.0f) z += r.M;
+      // When the neighbour's sample was kept, its visibility-tested target at
+      // this pixel is nr.phat, computed just above from the same inputs: the
+      // shadow ray is not traced twice.
+      const float here = second ? nr.phat : di_phat<W, S>(sc, st, v, bs, c.light, c.u, c.v, true, b.counters);
+      if (here > 0.0f) z += r.M;
       Vtx vn;
       if (load_vtx(b, q, vn)) {
         vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position

@@ -56,32 +56,38 @@ __device__ __forceinline__ void load_node(const WNode<W>* __restrict__ nodes, ui
   for (int k = 0; k < N4; ++k) dst[k] = src[k];
 }
 
-// Entry distance of each child box, +inf for a missed or empty child.
+// Entry distance of each child box, +inf for a missed or empty child.  The
+// plane distances are fma(plane, inv, -o*inv): one rounding of o*inv more
+// than (plane - o) * inv, i.e. an error of about 2^-24 |o| in space, far below
+// the boxes' padding of 2^-14 max|coord| for origins within ~100x the scene
+// extent, so the test stays conservative.  o*inv is formed per node (three
+// multiplies) rather than kept per ray, which would cost three registers.
 template <int W>
 __device__ __forceinline__ void child_dists(const WNode<W>& nd, const TraceRay& ray, float tcap,
                                             float (&dist)[W]) {
-  const v3 o = ray.pre.o;
+  const v3 inv = ray.inv, o = ray.pre.o;
+  const v3 noi = mk(-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z));
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    float t0x = (nd.lo_x[k] - o.x) * ray.inv.x, t1x = (nd.hi_x[k] - o.x) * ray.inv.x;
-    float t0y = (nd.lo_y[k] - o.y) * ray.inv.y, t1y = (nd.hi_y[k] - o.y) * ray.inv.y;
-    float t0z = (nd.lo_z[k] - o.z) * ray.inv.z, t1z = (nd.hi_z[k] - o.z) * ray.inv.z;
+    float t0x = __builtin_fmaf(nd.lo_x[k], inv.x, noi.x), t1x = __builtin_fmaf(nd.hi_x[k], inv.x, noi.x);
+    float t0y = __builtin_fmaf(nd.lo_y[k], inv.y, noi.y), t1y = __builtin_fmaf(nd.hi_y[k], inv.y, noi.y);
+    float t0z = __builtin_fmaf(nd.lo_z[k], inv.z, noi.z), t1z = __builtin_fmaf(nd.hi_z[k], inv.z, noi.z);
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), ray.pre.tmin));
     float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcap));
     dist[k] = (tn <= tf && nd.child[k] != REF_EMPTY) ? tn : INFINITY;
   }
 }
 
+// Compare-exchange without a branch: two selects per array.
 template <int W>
 __device__ __forceinline__ void cswap(float (&d)[W], uint32_t (&r)[W], int a, int b) {
-  if (d[b] < d[a]) {
-    float td = d[a];
-    d[a] = d[b];
-    d[b] = td;
-    uint32_t tr = r[a];
-    r[a] = r[b];
-    r[b] = tr;
-  }
+  const bool sw = d[b] < d[a];
+  const float da = d[a], db = d[b];
+  const uint32_t ra = r[a], rb = r[b];
+  d[a] = sw ? db : da;
+  d[b] = sw ? da : db;
+  r[a] = sw ? rb : ra;
+  r[b] = sw ? ra : rb;
 }
 
 // Ascending sort of (dist, ref) by dist: optimal networks for 2, 4 and 8.
