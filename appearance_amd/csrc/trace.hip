@@ -13,8 +13,11 @@
 // counter, and a wave refetches as soon as `refill` of its 64 lanes have
 // finished their rays, so lanes no longer idle behind the slowest ray of
 // their wave (persistent threads with dynamic ray fetch, Aila and Laine,
-// HPG 2009).  Both return the same hits: each ray's traversal is the same
-// while-while loop.
+// HPG 2009).  For 2- and 4-wide BVHs each iteration of the pool kernel
+// fetches one item per lane, a node or two triangles, so the wave pays one
+// memory round trip per iteration whatever its lanes are doing.  Both
+// schedules return the same hits: every ray visits children nearest first
+// and the closest hit is the lexicographic (t, prim) minimum.
 #include <algorithm>
 
 #include "apt_internal.h"

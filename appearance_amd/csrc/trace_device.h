@@ -302,7 +302,7 @@ inline uint32_t pool_blocks() {
 // (rays of different scenes in one launch); and `void closest(uint32_t i,
 // const HitRec&)` or `void any(uint32_t i, bool hit)`.
 template <int W, int S, bool ANY, class Src>
-__device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
+__device__ __forceinline__ void trace_pool_ww(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
                                            uint2* const lds, const int stride) {
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
@@ -430,6 +430,167 @@ __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* count
       active = false;
     }
   }
+}
+
+
+// Single-step schedule of the persistent traversal.  Every iteration each
+// active lane fetches one item and processes it: a node (its boxes and child
+// refs), or the next one or two triangles of the leaf it is in.  A node is
+// seven float4 for W = 4 and two triangles are six, so both land in the same
+// registers, and every lane's fetch of an iteration is in flight at once: one
+// memory round trip per iteration, whatever mix of nodes and leaves the wave
+// holds.  (The while-while loop of trace_pool_ww runs inner nodes until every
+// lane of the wave has reached a leaf, so lanes that get there early wait a
+// round trip per step of the others.)  The per-ray traversal is the same:
+// nearest child first, the others pushed far to near, entries popped while
+// their box starts beyond the best hit are dropped, so the hits are those of
+// trace_closest / trace_any.
+template <int W, int S, bool ANY, class Src>
+__device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
+                                                uint2* const lds, const int stride) {
+  static_assert(W == 2 || W == 4, "single-step traversal: node and two triangles share registers");
+  constexpr int NQ = 7;  // float4 per fetch: a W=4 node (112 B) or two triangles (96 B)
+  constexpr int N4 = (int)((6 * W + W) * 4 + 15) / 16;
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  bool active = false, drained = false, in_leaf = false;
+  uint32_t rid = 0, cur = 0, end = 0, root = 0;
+  const WNode<W>* nodes = nullptr;
+  const float4* tris = nullptr;
+  if constexpr (Src::kOneScene) {
+    nodes = src.nodes;
+    tris = src.tris;
+    root = src.root;
+  }
+  TraceRay ray;
+  HitRec best{0.0f, 0.0f, 0.0f, INVALID};
+  while (true) {
+    const unsigned long long idle = __ballot(!active);
+    if (!drained && idle != 0ull) {
+      const uint32_t nidle = (uint32_t)__popcll(idle);
+      if (nidle >= refill || __ballot(active) == 0ull) {
+        const int leader = __ffsll(idle) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(counter, nidle);
+        base = __shfl(base, leader);
+        if (!active) {
+          const uint32_t r = base + (uint32_t)__popcll(idle & below);
+          if (r < n) {
+            rid = r;
+            if constexpr (Src::kOneScene) src.fetch(r, ray);
+            else src.fetch(r, ray, nodes, tris, root);
+            best = HitRec{ray.pre.tmax, 0.0f, 0.0f, INVALID};
+            sp = 0;
+            in_leaf = ref_is_leaf(root);
+            cur = in_leaf ? ref_leaf_first(root) : root;
+            end = in_leaf ? cur + ref_leaf_count(root) : 0u;
+            active = true;
+          }
+        }
+        if (base + nidle >= n) drained = true;
+      }
+    }
+    if (__ballot(active) == 0ull) break;
+    if (!active) continue;
+    // Fetch: a node, or triangles cur and (if in the leaf) cur + 1.
+    float4 q[NQ];
+    const bool two = in_leaf && cur + 1u < end;
+    {
+      const float4* p = in_leaf ? tris + 3 * (size_t)cur : reinterpret_cast<const float4*>(nodes + cur);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) q[k] = p[k];
+      if (!in_leaf || two) {
+#pragma unroll
+        for (int k = 3; k < 6; ++k) q[k] = (k < N4 || two) ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      if (!in_leaf && N4 > 6) q[6] = p[6];
+    }
+    uint32_t next = REF_EMPTY;  // ref to enter after this item; REF_EMPTY: pop
+    bool stay = false, occluded = false;
+    if (!in_leaf) {
+      WNode<W> nd;
+      __builtin_memcpy(&nd, q, sizeof(float4) * N4);
+      float dist[W];
+      child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best.t, dist);
+      if (ANY) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+          if (dist[k] == INFINITY) continue;
+          if (next == REF_EMPTY) next = nd.child[k];
+          else APT_STACK_PUSH(nd.child[k], dist[k]);
+        }
+      } else {
+        uint32_t refs[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
+        sort_children<W>(dist, refs);
+        if (dist[0] != INFINITY) {
+#pragma unroll
+          for (int k = W - 1; k >= 1; --k)
+            if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
+          next = refs[0];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int h2 = 0; h2 < 2; ++h2) {
+        if (h2 == 1 && !two) break;
+        const float4 a = q[3 * h2], b = q[3 * h2 + 1], c = q[3 * h2 + 2];
+        TriHit h;
+        if (intersect_tri(ray.pre, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h)) {
+          if (ANY) {
+            occluded = true;
+          } else {
+            const uint32_t prim = f2u(a.w);
+            if (hit_less(h.t, prim, best.t, best.prim)) {
+              best.t = h.t;
+              best.u = h.u;
+              best.v = h.v;
+              best.prim = prim;
+            }
+          }
+        }
+      }
+      cur += two ? 2u : 1u;
+      stay = cur < end;
+    }
+    bool done = occluded;
+    if (!done && !stay) {
+      if (next == REF_EMPTY) {
+        while (sp > 0) {
+          const uint2 e = APT_STACK_POP();
+          if (ANY || u2f(e.y) <= best.t) {
+            next = e.x;
+            break;
+          }
+        }
+      }
+      if (next == REF_EMPTY) {
+        done = true;
+      } else {
+        in_leaf = ref_is_leaf(next);
+        cur = in_leaf ? ref_leaf_first(next) : next;
+        end = in_leaf ? cur + ref_leaf_count(next) : 0u;
+      }
+    }
+    if (done) {
+      if constexpr (ANY) src.any(rid, occluded);
+      else src.closest(rid, best);
+      active = false;
+    }
+  }
+}
+
+// Persistent traversal: the single-step schedule for 2- and 4-wide BVHs, the
+// while-while one for 8-wide nodes (224 B, too large to share registers with
+// triangles).
+template <int W, int S, bool ANY, class Src>
+__device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
+                                           uint2* const lds, const int stride) {
+  if constexpr (W <= 4) trace_pool_step<W, S, ANY, Src>(src, n, counter, refill, lds, stride);
+  else trace_pool_ww<W, S, ANY, Src>(src, n, counter, refill, lds, stride);
 }
 
 }  // namespace apt
