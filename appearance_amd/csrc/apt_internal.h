@@ -113,7 +113,13 @@ struct apt_bvh {
   mutable uint32_t counter_next = 0;  // next counter slot (atomic)
   apt_bvh_stats stats{};
 };
-constexpr uint32_t APT_BVH_COUNTERS = 64;
+constexpr uint32_t APT_BVH_COUNTERS = 64;  // ray pools in a BVH's ring
+
+// A persistent launch's ray pool: POOL_PARTS counters POOL_STRIDE words apart
+// (one range of rays per XCD, see pool_take in trace_device.h).
+constexpr uint32_t POOL_PARTS = 8;
+constexpr uint32_t POOL_STRIDE = 16;
+constexpr uint32_t POOL_WORDS = POOL_PARTS * POOL_STRIDE;
 
 namespace apt {
 // Collapse a binary BVH into a wide one and upload it (bvh_host.cpp).

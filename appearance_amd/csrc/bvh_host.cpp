@@ -322,7 +322,7 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
   out->root = root;
   out->d_nodes = d_nodes;
   out->d_tris = (float4*)d_tris;
-  e = hipMalloc(&out->d_counters, APT_BVH_COUNTERS * sizeof(uint32_t));
+  e = hipMalloc(&out->d_counters, APT_BVH_COUNTERS * POOL_WORDS * sizeof(uint32_t));
   if (e != hipSuccess) {
     out->d_counters = nullptr;
     set_error("hipMalloc of the ray-pool counters failed: %s", hipGetErrorString(e));

@@ -11,9 +11,10 @@ enum Counter : uint32_t {
   CNT_SHADOW = 2,
   CNT_STAT_EXT = 3,     // extension rays traced (running total)
   CNT_STAT_SHADOW = 4,  // shadow rays traced (running total)
-  CNT_POOL_EXT = 5,     // ray-pool counter of the extend stage (zeroed by the kernel before it)
-  CNT_POOL_SHADOW = 6,  // ray-pool counter of the shadow stage
-  CNT_COUNT = 8
+  CNT_POOL_EXT = 16,                              // ray pool of the extend stage (POOL_WORDS,
+                                                  // zeroed by the kernel before it)
+  CNT_POOL_SHADOW = CNT_POOL_EXT + POOL_WORDS,    // ray pool of the shadow stage
+  CNT_COUNT = CNT_POOL_SHADOW + POOL_WORDS
 };
 
 // The frame being rendered: a slab of rows [row_start, row_start + rows) of a

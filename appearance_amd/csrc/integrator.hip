@@ -27,8 +27,8 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(FrameView fv, SceneView sc, Pa
     ps.counters[CNT_QUEUE0] = n_paths;
     ps.counters[CNT_QUEUE1] = 0;
     ps.counters[CNT_SHADOW] = 0;
-    ps.counters[CNT_POOL_EXT] = 0;
   }
+  if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);
   if (p >= n_paths) return;
   const uint32_t npix = fv.width * fv.rows;
   const uint32_t s_local = p / npix, pix_local = p - s_local * npix;
@@ -60,14 +60,14 @@ struct ExtendSrc {
 template <int W, int S>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, int qin,
                                                      const WNode<W>* __restrict__ nodes) {
-  __shared__ uint2 s_stack[S * BLOCK];
+  __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
     ps.counters[CNT_SHADOW] = 0;
-    ps.counters[CNT_POOL_SHADOW] = 0;
     ps.counters[CNT_STAT_EXT] += count;
   }
+  if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_SHADOW], threadIdx.x);
   ExtendSrc<W> src{nodes, sc.bvh_tris, sc.root, ps, ps.queue[qin]};
   trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], min(sc.refill, 64u), s_stack + threadIdx.x,
                           BLOCK);
@@ -115,12 +115,12 @@ struct ShadowSrc {
 template <int W, int S>
 __global__ __launch_bounds__(BLOCK, 8) void k_shadow(SceneView sc, PathView ps,
                                                      const WNode<W>* __restrict__ nodes) {
-  __shared__ uint2 s_stack[S * BLOCK];
+  __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   const uint32_t count = ps.counters[CNT_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_STAT_SHADOW] += count;
-    ps.counters[CNT_POOL_EXT] = 0;  // for the next bounce's extend stage
   }
+  if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);  // next bounce's extend
   ShadowSrc<W> src{nodes, sc.bvh_tris, sc.root, ps};
   trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], min(sc.refill, 64u), s_stack + threadIdx.x,
                          BLOCK);

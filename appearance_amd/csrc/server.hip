@@ -43,8 +43,8 @@ __global__ __launch_bounds__(BLOCK) void k_srv_raygen(const SrvBatch* __restrict
     ps.counters[CNT_QUEUE0] = n_paths;
     ps.counters[CNT_QUEUE1] = 0;
     ps.counters[CNT_SHADOW] = 0;
-    ps.counters[CNT_POOL_EXT] = 0;
   }
+  if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);
   if (p >= n_paths) return;
   const SrvTile& t = batch->tiles[find_tile(batch->tiles, batch->n_tiles, p)];
   const SrvReq& r = reqs[t.req_slot];
@@ -86,14 +86,14 @@ struct SrvExtendSrc {
 template <int W, int S>
 __global__ __launch_bounds__(BLOCK, 8) void k_srv_extend(const SrvReq* __restrict__ reqs, PathView ps,
                                                          const uint32_t* __restrict__ path_req, int qin) {
-  __shared__ uint2 s_stack[S * BLOCK];
+  __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
     ps.counters[CNT_SHADOW] = 0;
-    ps.counters[CNT_POOL_SHADOW] = 0;
     ps.counters[CNT_STAT_EXT] += count;
   }
+  if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_SHADOW], threadIdx.x);
   SrvExtendSrc<W> src{reqs, ps, path_req, ps.queue[qin]};
   trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], SRV_REFILL, s_stack + threadIdx.x, BLOCK);
 }
@@ -144,12 +144,12 @@ struct SrvShadowSrc {
 template <int W, int S>
 __global__ __launch_bounds__(BLOCK, 8) void k_srv_shadow(const SrvReq* __restrict__ reqs, PathView ps,
                                                          const uint32_t* __restrict__ path_req) {
-  __shared__ uint2 s_stack[S * BLOCK];
+  __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   const uint32_t count = ps.counters[CNT_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_STAT_SHADOW] += count;
-    ps.counters[CNT_POOL_EXT] = 0;  // for the next bounce's extend stage
   }
+  if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);  // next bounce's extend
   SrvShadowSrc<W> src{reqs, ps, path_req};
   trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], SRV_REFILL, s_stack + threadIdx.x, BLOCK);
 }

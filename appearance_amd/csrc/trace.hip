@@ -60,7 +60,7 @@ __device__ __forceinline__ TraceRay load_ray(const apt_ray* __restrict__ rays, u
   return make_trace_ray(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w);
 }
 
-__global__ void k_zero_counter(uint32_t* c) { *c = 0u; }
+__global__ void k_zero_pool(uint32_t* c) { pool_zero(c, threadIdx.x); }
 
 // Rays and results of the ray-query entry points for trace_pool.
 template <int W>
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_trace_pool(const WNode<W>* __restr
                                                          apt_hit* __restrict__ hits,
                                                          uint32_t* __restrict__ occluded,
                                                          uint32_t* __restrict__ counter, uint32_t refill) {
-  __shared__ uint2 s_stack[S * BLOCK];
+  __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   RaySrc<W> src{nodes, tris, root, rays, hits, occluded};
   trace_pool<W, S, ANY>(src, n, counter, refill, s_stack + threadIdx.x, BLOCK);
 }
@@ -125,11 +125,11 @@ apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit
                            bvh->root, d_rays, n, d_hits, d_occ);
     })
   } else {
-    // one counter of the BVH's ring per call: calls in flight on different
-    // streams (up to the ring size) do not share a counter
+    // one pool of the BVH's ring per call: calls in flight on different
+    // streams (up to the ring size) do not share counters
     const uint32_t slot = __atomic_fetch_add(&bvh->counter_next, 1u, __ATOMIC_RELAXED) % APT_BVH_COUNTERS;
-    uint32_t* counter = bvh->d_counters + slot;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
+    uint32_t* counter = bvh->d_counters + (size_t)slot * POOL_WORDS;
+    hipLaunchKernelGGL(k_zero_pool, dim3(1), dim3(POOL_PARTS), 0, s, counter);
     APT_FOR_STACK_DEPTH(bvh->lds_stack, {
       if (any)
         hipLaunchKernelGGL((k_trace_pool<W, S_, true>), dim3(std::min(pool_grid<W, S_, true>(), (n + 63) / 64)),

@@ -287,10 +287,69 @@ inline uint32_t pool_blocks() {
   return g;
 }
 
+// The ray pool of a persistent launch is split into POOL_PARTS contiguous
+// ranges, one per XCD, each with its own counter (POOL_STRIDE words apart, so
+// the counters sit on separate 64-byte lines).  A wave starts on the range of
+// its own XCD (work-groups are dealt to the 8 XCDs round robin, so block b
+// runs on XCD b % 8) and moves on to the next range when that one runs dry,
+// so neighbouring rays are traced by one XCD and share its L2 while no XCD
+// idles before the whole pool is drained.
+__device__ __forceinline__ uint32_t pool_part_lo(uint32_t n, uint32_t part) {
+  return (uint32_t)(((uint64_t)n * part) / POOL_PARTS);
+}
+
+// Zeroes the counters of a pool; call with tid = 0 .. POOL_PARTS-1.
+__device__ __forceinline__ void pool_zero(uint32_t* ctr, uint32_t tid) {
+  if (tid < POOL_PARTS) ctr[tid * POOL_STRIDE] = 0u;
+}
+
+struct PoolCursor {
+  uint32_t part;  // range the wave takes rays from
+  uint32_t left;  // ranges not yet found empty
+};
+
+__device__ __forceinline__ PoolCursor pool_start() {
+  return PoolCursor{blockIdx.x % POOL_PARTS, POOL_PARTS};
+}
+
+// Called by every lane of the wave: once at least `refill` lanes are idle (or
+// all are), the idle lanes take consecutive rays of the current range; a wave
+// with no ray left moves through the ranges until it gets rays or all are
+// empty.  Returns the lane's new ray index, or ~0u.
+__device__ __forceinline__ uint32_t pool_take(uint32_t* ctr, uint32_t n, PoolCursor& pc, bool active,
+                                              uint32_t refill) {
+  const unsigned long long idle = __ballot(!active);
+  if (idle == 0ull) return ~0u;
+  const uint32_t nidle = (uint32_t)__popcll(idle);
+  const bool all_idle = nidle == 64u;
+  if (nidle < refill && !all_idle) return ~0u;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll(idle) - 1;
+  uint32_t got = ~0u;
+  while (pc.left != 0u) {
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(ctr + pc.part * POOL_STRIDE, nidle);
+    // the cursor and the range are the same in every lane: keep them scalar
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    const uint32_t lo = pool_part_lo(n, pc.part), len = pool_part_lo(n, pc.part + 1u) - lo;
+    if (!active) {
+      const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+      if (r < len) got = lo + r;
+    }
+    if (base + nidle >= len) {
+      pc.part = (uint32_t)__builtin_amdgcn_readfirstlane((int)((pc.part + 1u) % POOL_PARTS));
+      pc.left = (uint32_t)__builtin_amdgcn_readfirstlane((int)(pc.left - 1u));
+    }
+    if (base < len || !all_idle) break;  // rays taken, or lanes still busy
+  }
+  return got;
+}
+
 // Persistent traversal with dynamic ray fetch (Aila and Laine, HPG 2009),
 // shared by the ray-query kernels and the integrators' extend and shadow
 // stages.  A grid that just fills the device pulls ray indices [0, n) from
-// *counter (zeroed by an earlier kernel on the stream); a wave refetches once
+// the pool counters at `counter` (POOL_WORDS words, zeroed by an earlier
+// kernel on the stream, see pool_take); a wave refetches once
 // `refill` of its 64 lanes are idle, so lanes do not wait for the slowest ray
 // of their wave.  Each lane runs one round of the while-while loop per
 // iteration (inner nodes to a leaf, the leaf, the next stack entry), so every
@@ -306,9 +365,8 @@ __device__ __forceinline__ void trace_pool_ww(Src& src, uint32_t n, uint32_t* co
                                            uint2* const lds, const int stride) {
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
-  const uint32_t lane = __lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
-  bool active = false, drained = false;
+  PoolCursor pc = pool_start();
+  bool active = false;
   uint32_t rid = 0, ref = 0, root = 0;
   const WNode<W>* nodes = nullptr;
   const float4* tris = nullptr;
@@ -320,28 +378,15 @@ __device__ __forceinline__ void trace_pool_ww(Src& src, uint32_t n, uint32_t* co
   TraceRay ray;
   HitRec best{0.0f, 0.0f, 0.0f, INVALID};
   while (true) {
-    const unsigned long long idle = __ballot(!active);
-    if (!drained && idle != 0ull) {
-      const uint32_t nidle = (uint32_t)__popcll(idle);
-      if (nidle >= refill || __ballot(active) == 0ull) {
-        const int leader = __ffsll(idle) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(counter, nidle);
-        base = __shfl(base, leader);
-        if (!active) {
-          const uint32_t r = base + (uint32_t)__popcll(idle & below);
-          if (r < n) {
-            rid = r;
-            if constexpr (Src::kOneScene) src.fetch(r, ray);
-            else src.fetch(r, ray, nodes, tris, root);
-            best = HitRec{ray.pre.tmax, 0.0f, 0.0f, INVALID};
-            ref = root;
-            sp = 0;
-            active = true;
-          }
-        }
-        if (base + nidle >= n) drained = true;
-      }
+    const uint32_t r = pool_take(counter, n, pc, active, refill);
+    if (r != ~0u) {
+      rid = r;
+      if constexpr (Src::kOneScene) src.fetch(r, ray);
+      else src.fetch(r, ray, nodes, tris, root);
+      best = HitRec{ray.pre.tmax, 0.0f, 0.0f, INVALID};
+      ref = root;
+      sp = 0;
+      active = true;
     }
     if (__ballot(active) == 0ull) break;
     if (!active) continue;
@@ -433,6 +478,11 @@ __device__ __forceinline__ void trace_pool_ww(Src& src, uint32_t n, uint32_t* co
 }
 
 
+// LDS entries per lane of a pool kernel: the S-entry stack plus one entry
+// that holds the (u, v) of the best hit so far (rarely written, read once per
+// ray, and kept out of the registers the fetch needs).
+constexpr int pool_lds_entries(int S) { return S + 1; }
+
 // Single-step schedule of the persistent traversal.  Every iteration each
 // active lane fetches one item and processes it: a node (its boxes and child
 // refs), or the next one or two triangles of the leaf it is in.  A node is
@@ -453,10 +503,11 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
   constexpr int N4 = (int)((6 * W + W) * 4 + 15) / 16;
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
-  const uint32_t lane = __lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
-  bool active = false, drained = false, in_leaf = false;
-  uint32_t rid = 0, cur = 0, end = 0, root = 0;
+  PoolCursor pc = pool_start();
+  bool active = false;
+  // cur: the node to fetch, or the leaf ref of the triangles still to test
+  // (first advanced and count lowered as the leaf is consumed)
+  uint32_t rid = 0, cur = 0, root = 0;
   const WNode<W>* nodes = nullptr;
   const float4* tris = nullptr;
   if constexpr (Src::kOneScene) {
@@ -465,40 +516,31 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
     root = src.root;
   }
   TraceRay ray;
-  HitRec best{0.0f, 0.0f, 0.0f, INVALID};
+  float best_t = 0.0f;
+  uint32_t best_prim = INVALID;
+  uint2* const uv = lds + S * stride;  // (u, v) of the best hit
   while (true) {
-    const unsigned long long idle = __ballot(!active);
-    if (!drained && idle != 0ull) {
-      const uint32_t nidle = (uint32_t)__popcll(idle);
-      if (nidle >= refill || __ballot(active) == 0ull) {
-        const int leader = __ffsll(idle) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(counter, nidle);
-        base = __shfl(base, leader);
-        if (!active) {
-          const uint32_t r = base + (uint32_t)__popcll(idle & below);
-          if (r < n) {
-            rid = r;
-            if constexpr (Src::kOneScene) src.fetch(r, ray);
-            else src.fetch(r, ray, nodes, tris, root);
-            best = HitRec{ray.pre.tmax, 0.0f, 0.0f, INVALID};
-            sp = 0;
-            in_leaf = ref_is_leaf(root);
-            cur = in_leaf ? ref_leaf_first(root) : root;
-            end = in_leaf ? cur + ref_leaf_count(root) : 0u;
-            active = true;
-          }
-        }
-        if (base + nidle >= n) drained = true;
-      }
+    const uint32_t r = pool_take(counter, n, pc, active, refill);
+    if (r != ~0u) {
+      rid = r;
+      if constexpr (Src::kOneScene) src.fetch(r, ray);
+      else src.fetch(r, ray, nodes, tris, root);
+      best_t = ray.pre.tmax;
+      best_prim = INVALID;
+      sp = 0;
+      cur = root;
+      active = true;
     }
     if (__ballot(active) == 0ull) break;
     if (!active) continue;
-    // Fetch: a node, or triangles cur and (if in the leaf) cur + 1.
+    // Fetch: a node, or the leaf's next one or two triangles.
     float4 q[NQ];
-    const bool two = in_leaf && cur + 1u < end;
+    const bool in_leaf = ref_is_leaf(cur);
+    const uint32_t left = ref_leaf_count(cur);
+    const bool two = in_leaf && left >= 2u;
     {
-      const float4* p = in_leaf ? tris + 3 * (size_t)cur : reinterpret_cast<const float4*>(nodes + cur);
+      const float4* p = in_leaf ? tris + 3 * (size_t)ref_leaf_first(cur)
+                                : reinterpret_cast<const float4*>(nodes + cur);
 #pragma unroll
       for (int k = 0; k < 3; ++k) q[k] = p[k];
       if (!in_leaf || two) {
@@ -506,6 +548,9 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
         for (int k = 3; k < 6; ++k) q[k] = (k < N4 || two) ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
       if (!in_leaf && N4 > 6) q[6] = p[6];
+      // keep all of the fetch's loads ahead of the processing below, so they
+      // are in flight together
+      asm volatile("" ::: "memory");
     }
     uint32_t next = REF_EMPTY;  // ref to enter after this item; REF_EMPTY: pop
     bool stay = false, occluded = false;
@@ -513,7 +558,7 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
       WNode<W> nd;
       __builtin_memcpy(&nd, q, sizeof(float4) * N4);
       float dist[W];
-      child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best.t, dist);
+      child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best_t, dist);
       if (ANY) {
 #pragma unroll
         for (int k = 0; k < W; ++k) {
@@ -544,40 +589,39 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
             occluded = true;
           } else {
             const uint32_t prim = f2u(a.w);
-            if (hit_less(h.t, prim, best.t, best.prim)) {
-              best.t = h.t;
-              best.u = h.u;
-              best.v = h.v;
-              best.prim = prim;
+            if (hit_less(h.t, prim, best_t, best_prim)) {
+              best_t = h.t;
+              best_prim = prim;
+              *uv = make_uint2(f2u(h.u), f2u(h.v));
             }
           }
         }
       }
-      cur += two ? 2u : 1u;
-      stay = cur < end;
+      stay = left > 2u;
+      if (stay) cur += 2u - (2u << 28);  // first + 2, count - 2
     }
     bool done = occluded;
     if (!done && !stay) {
       if (next == REF_EMPTY) {
         while (sp > 0) {
           const uint2 e = APT_STACK_POP();
-          if (ANY || u2f(e.y) <= best.t) {
+          if (ANY || u2f(e.y) <= best_t) {
             next = e.x;
             break;
           }
         }
       }
-      if (next == REF_EMPTY) {
-        done = true;
-      } else {
-        in_leaf = ref_is_leaf(next);
-        cur = in_leaf ? ref_leaf_first(next) : next;
-        end = in_leaf ? cur + ref_leaf_count(next) : 0u;
-      }
+      if (next == REF_EMPTY) done = true;
+      else cur = next;
     }
     if (done) {
-      if constexpr (ANY) src.any(rid, occluded);
-      else src.closest(rid, best);
+      if constexpr (ANY) {
+        src.any(rid, occluded);
+      } else {
+        const uint2 e = *uv;
+        src.closest(rid, HitRec{best_t, best_prim == INVALID ? 0.0f : u2f(e.x),
+                                best_prim == INVALID ? 0.0f : u2f(e.y), best_prim});
+      }
       active = false;
     }
   }
