@@ -171,7 +171,7 @@ apt_status apt_bvh_export(const apt_bvh* bvh, void* nodes, size_t nodes_len, voi
                           size_t tris_len) {
   clear_error();
   APT_REQUIRE(bvh, APT_ERR_INVALID_ARG, "apt_bvh_export: null bvh");
-  const size_t nb = (size_t)bvh->n_nodes * (16 + 24 * (size_t)bvh->width);
+  const size_t nb = (size_t)bvh->n_nodes * wide_node_bytes(bvh->width);
   const size_t tb = (size_t)bvh->n_tris * 3 * sizeof(float4);
   APT_REQUIRE(!nodes || nodes_len >= nb, APT_ERR_INVALID_ARG, "apt_bvh_export: nodes need %zu bytes", nb);
   APT_REQUIRE(!tris || tris_len >= tb, APT_ERR_INVALID_ARG, "apt_bvh_export: triangles need %zu bytes", tb);

@@ -40,12 +40,16 @@ hipStream_t lib_stream();
 // ---------------------------------------------------------------------------
 // Wide BVH layout in HBM.
 //
-// A node of width W stores its W child boxes as six float[W] rows (SoA, so a
-// lane reads its W boxes as W/4 float4 per row) followed by W child refs:
+// A node of width W quantizes its W child boxes to an 8-bit grid over the
+// node's own box: grid origin (ox, oy, oz) = the box's lower corner, grid step
+// 2^(e - 127) per axis, and for each child six bytes (lo and hi per axis, in
+// steps) rounded outwards, so every decoded box contains the child's padded
+// box.  Then W child refs:
 //   ref <  0x80000000          internal node index
-//   ref == REF_EMPTY           unused slot (its box is inverted and never hit)
+//   ref == REF_EMPTY           unused slot (never entered)
 //   ref &  0x80000000          leaf: first triangle in bits 0..27, count-1 in
 //                              bits 28..30 (1..8 triangles)
+// 48 / 64 / 96 B for W = 2 / 4 / 8, against 64 / 128 / 256 B for float boxes.
 // Triangles are stored in leaf order as three float4 (v0 | prim bits, v1, v2),
 // 48 B each, so a leaf is one contiguous run.
 // ---------------------------------------------------------------------------
@@ -59,17 +63,33 @@ APT_HD uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
   return REF_LEAF | ((count - 1u) << 28) | first;
 }
 
+enum { QLO_X = 0, QLO_Y = 1, QLO_Z = 2, QHI_X = 3, QHI_Y = 4, QHI_Z = 5 };
+
 template <int W>
 struct alignas(16) WNode {
-  float lo_x[W], lo_y[W], lo_z[W];
-  float hi_x[W], hi_y[W], hi_z[W];
+  float ox, oy, oz;       // grid origin
+  uint8_t ex, ey, ez;     // grid step 2^(e - 127) per axis, e in 1..254
+  uint8_t pad0;
   uint32_t child[W];
-  // pad to a multiple of 64 B: W=2 -> 64, W=4 -> 128, W=8 -> 256
-  uint32_t pad[(W == 2) ? 2 : W];
+  uint8_t q[6][W];        // QLO_X .. QHI_Z rows, one byte per child
 };
-static_assert(sizeof(WNode<2>) == 64, "WNode<2> size");
-static_assert(sizeof(WNode<4>) == 128, "WNode<4> size");
-static_assert(sizeof(WNode<8>) == 256, "WNode<8> size");
+static_assert(sizeof(WNode<2>) == 48, "WNode<2> size");
+static_assert(sizeof(WNode<4>) == 64, "WNode<4> size");
+static_assert(sizeof(WNode<8>) == 96, "WNode<8> size");
+
+inline size_t wide_node_bytes(uint32_t width) {
+  return width == 2 ? sizeof(WNode<2>) : width == 4 ? sizeof(WNode<4>) : sizeof(WNode<8>);
+}
+
+// 2^(e - 127) for a biased exponent e in 1..254 (a normal float, exact).
+APT_HD float qstep(uint32_t e) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.u = e << 23;
+  return c.f;
+}
 
 // Host-side binary BVH produced by the builders before collapse.
 struct Node2 {

@@ -213,6 +213,40 @@ double bvh2_sah_cost(const std::vector<Node2>& nodes) {
 
 namespace {
 
+// Grid of one node: per axis the smallest power-of-two step that spans the
+// node box in 255 steps, then each child box rounded outwards to the grid
+// (checked in double, so origin + q * step never cuts into the padded box).
+template <int W>
+void quantize_node(WNode<W>& w, int nk, const float (&clo)[W][3], const float (&chi)[W][3],
+                   const float (&nlo)[3], const float (&nhi)[3]) {
+  float* org[3] = {&w.ox, &w.oy, &w.oz};
+  uint8_t* ex[3] = {&w.ex, &w.ey, &w.ez};
+  for (int a = 0; a < 3; ++a) {
+    const double lo = nk ? (double)nlo[a] : 0.0, ext = nk ? (double)nhi[a] - lo : 0.0;
+    int e = ext > 0.0 ? std::max(-126, std::min(127, (int)std::ceil(std::log2(ext / 255.0)))) : -126;
+    while (e < 127 && ext > 255.0 * std::ldexp(1.0, e)) ++e;
+    while (e > -126 && ext <= 255.0 * std::ldexp(1.0, e - 1)) --e;
+    *org[a] = (float)lo;
+    *ex[a] = (uint8_t)(e + 127);
+    const double step = std::ldexp(1.0, e);
+    for (int k = 0; k < W; ++k) {
+      if (k >= nk) {
+        w.q[a][k] = 255;
+        w.q[a + 3][k] = 0;
+        continue;
+      }
+      double ql = std::floor(((double)clo[k][a] - lo) / step);
+      double qh = std::ceil(((double)chi[k][a] - lo) / step);
+      ql = std::min(255.0, std::max(0.0, ql));
+      qh = std::min(255.0, std::max(0.0, qh));
+      while (ql > 0.0 && lo + ql * step > (double)clo[k][a]) ql -= 1.0;
+      while (qh < 255.0 && lo + qh * step < (double)chi[k][a]) qh += 1.0;
+      w.q[a][k] = (uint8_t)ql;
+      w.q[a + 3][k] = (uint8_t)qh;
+    }
+  }
+}
+
 template <int W>
 apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad,
                            apt_bvh* out) {
@@ -264,20 +298,24 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
     // Worst-case stack: every sibling but one may be pushed at each level.
     uint32_t here = wstack[qi] + (uint32_t)(nk - 1);
     max_stack = std::max(max_stack, here + 1);
+    // Padded child boxes and the node box that holds them.
+    float clo[W][3], chi[W][3], nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < nk; ++k) {
+      const Node2& c = bn[kids[k]];
+      for (int a = 0; a < 3; ++a) {
+        clo[k][a] = c.lo[a] - pad;
+        chi[k][a] = c.hi[a] + pad;
+        nlo[a] = std::min(nlo[a], clo[k][a]);
+        nhi[a] = std::max(nhi[a], chi[k][a]);
+      }
+    }
+    quantize_node<W>(w, nk, clo, chi, nlo, nhi);
     for (int k = 0; k < W; ++k) {
       if (k >= nk) {
-        w.lo_x[k] = w.lo_y[k] = w.lo_z[k] = 1e30f;
-        w.hi_x[k] = w.hi_y[k] = w.hi_z[k] = -1e30f;
         w.child[k] = REF_EMPTY;
         continue;
       }
       const Node2& c = bn[kids[k]];
-      w.lo_x[k] = c.lo[0] - pad;
-      w.lo_y[k] = c.lo[1] - pad;
-      w.lo_z[k] = c.lo[2] - pad;
-      w.hi_x[k] = c.hi[0] + pad;
-      w.hi_y[k] = c.hi[1] + pad;
-      w.hi_z[k] = c.hi[2] + pad;
       if (c.count) {
         w.child[k] = leaf_ref(c);
         n_leaves++;

@@ -34,6 +34,7 @@ struct SceneView {
   const void* nodes;            // WNode<width>[]
   const float4* bvh_tris;
   uint32_t root;
+  uint32_t n_nodes;
   uint32_t width;
   uint32_t lds_stack;           // traversal stack entries in LDS (8, 16, 24)
   uint32_t refill;              // persistent traversal refill threshold (65: one ray per thread)

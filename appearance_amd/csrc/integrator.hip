@@ -61,6 +61,8 @@ template <int W, int S>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, int qin,
                                                      const WNode<W>* __restrict__ nodes) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
+  __shared__ float4 s_top[TOP_BYTES / 16];
+  const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0 + (qin ^ 1)] = 0;
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, 
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_SHADOW], threadIdx.x);
   ExtendSrc<W> src{nodes, sc.bvh_tris, sc.root, ps, ps.queue[qin]};
   trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], min(sc.refill, 64u), s_stack + threadIdx.x,
-                          BLOCK);
+                          BLOCK, s_top, ktop);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, PathView ps, int qin,
@@ -116,6 +118,8 @@ template <int W, int S>
 __global__ __launch_bounds__(BLOCK, 8) void k_shadow(SceneView sc, PathView ps,
                                                      const WNode<W>* __restrict__ nodes) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
+  __shared__ float4 s_top[TOP_BYTES / 16];
+  const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
   const uint32_t count = ps.counters[CNT_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_STAT_SHADOW] += count;
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_shadow(SceneView sc, PathView ps,
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);  // next bounce's extend
   ShadowSrc<W> src{nodes, sc.bvh_tris, sc.root, ps};
   trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], min(sc.refill, 64u), s_stack + threadIdx.x,
-                         BLOCK);
+                         BLOCK, s_top, ktop);
 }
 
 // Adds this pass's samples of every pixel to its accumulator, one sample at a

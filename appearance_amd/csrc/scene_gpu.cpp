@@ -80,6 +80,7 @@ SceneView scene_gpu_view(const SceneGpu& s, const apt_camera& cam, uint32_t widt
   sc.nodes = s.bvh ? s.bvh->d_nodes : nullptr;
   sc.bvh_tris = s.bvh ? s.bvh->d_tris : nullptr;
   sc.root = s.bvh ? s.bvh->root : 0;
+  sc.n_nodes = s.bvh ? s.bvh->n_nodes : 0;
   sc.width = s.bvh ? s.bvh->width : 4;
   sc.lds_stack = s.bvh ? s.bvh->lds_stack : 8;
   sc.refill = s.bvh ? s.bvh->refill : 32;

@@ -95,7 +95,8 @@ __global__ __launch_bounds__(BLOCK, 8) void k_srv_extend(const SrvReq* __restric
   }
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_SHADOW], threadIdx.x);
   SrvExtendSrc<W> src{reqs, ps, path_req, ps.queue[qin]};
-  trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], SRV_REFILL, s_stack + threadIdx.x, BLOCK);
+  trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], SRV_REFILL, s_stack + threadIdx.x, BLOCK,
+                          nullptr, 0u);  // rays of several scenes: no shared top levels
 }
 
 __global__ __launch_bounds__(BLOCK) void k_srv_shade(const SrvReq* __restrict__ reqs, PathView ps,
@@ -151,7 +152,8 @@ __global__ __launch_bounds__(BLOCK, 8) void k_srv_shadow(const SrvReq* __restric
   }
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);  // next bounce's extend
   SrvShadowSrc<W> src{reqs, ps, path_req};
-  trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], SRV_REFILL, s_stack + threadIdx.x, BLOCK);
+  trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], SRV_REFILL, s_stack + threadIdx.x, BLOCK,
+                         nullptr, 0u);  // rays of several scenes: no shared top levels
 }
 
 __device__ __forceinline__ unsigned long long to_fixed(float v) {

@@ -86,10 +86,13 @@ __global__ __launch_bounds__(BLOCK, 8) void k_trace_pool(const WNode<W>* __restr
                                                          const apt_ray* __restrict__ rays, uint32_t n,
                                                          apt_hit* __restrict__ hits,
                                                          uint32_t* __restrict__ occluded,
-                                                         uint32_t* __restrict__ counter, uint32_t refill) {
+                                                         uint32_t* __restrict__ counter, uint32_t refill,
+                                                         uint32_t n_nodes) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
+  __shared__ float4 s_top[TOP_BYTES / 16];
+  const uint32_t ktop = stage_top<W>(nodes, n_nodes, s_top);
   RaySrc<W> src{nodes, tris, root, rays, hits, occluded};
-  trace_pool<W, S, ANY>(src, n, counter, refill, s_stack + threadIdx.x, BLOCK);
+  trace_pool<W, S, ANY>(src, n, counter, refill, s_stack + threadIdx.x, BLOCK, s_top, ktop);
 }
 
 // Blocks of k_trace_pool that fit on the device at once.
@@ -134,11 +137,11 @@ apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit
       if (any)
         hipLaunchKernelGGL((k_trace_pool<W, S_, true>), dim3(std::min(pool_grid<W, S_, true>(), (n + 63) / 64)),
                            dim3(BLOCK), 0, s, nodes, bvh->d_tris, bvh->root, d_rays, n, d_hits, d_occ,
-                           counter, bvh->refill);
+                           counter, bvh->refill, bvh->n_nodes);
       else
         hipLaunchKernelGGL((k_trace_pool<W, S_, false>), dim3(std::min(pool_grid<W, S_, false>(), (n + 63) / 64)),
                            dim3(BLOCK), 0, s, nodes, bvh->d_tris, bvh->root, d_rays, n, d_hits, d_occ,
-                           counter, bvh->refill);
+                           counter, bvh->refill, bvh->n_nodes);
     })
   }
   APT_HIP(hipGetLastError());

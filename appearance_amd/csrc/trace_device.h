@@ -51,27 +51,29 @@ __device__ __forceinline__ void load_node(const WNode<W>* __restrict__ nodes, ui
                                           WNode<W>& nd) {
   const float4* src = reinterpret_cast<const float4*>(nodes + idx);
   float4* dst = reinterpret_cast<float4*>(&nd);
-  constexpr int N4 = (int)((6 * W + W) * 4 + 15) / 16;  // boxes + child refs, no padding
+  constexpr int N4 = (int)(sizeof(WNode<W>) / 16);
 #pragma unroll
   for (int k = 0; k < N4; ++k) dst[k] = src[k];
 }
 
-// Entry distance of each child box, +inf for a missed or empty child.  The
-// plane distances are fma(plane, inv, -o*inv): one rounding of o*inv more
-// than (plane - o) * inv, i.e. an error of about 2^-24 |o| in space, far below
-// the boxes' padding of 2^-14 max|coord| for origins within ~100x the scene
-// extent, so the test stays conservative.  o*inv is formed per node (three
-// multiplies) rather than kept per ray, which would cost three registers.
+// Entry distance of each child box, +inf for a missed or empty child.  A
+// grid plane origin + q * step lies at distance q * (step * inv) + (origin -
+// o) * inv along the ray: one FMA per plane after three multiplies per node
+// (step * inv is exact, step being a power of two).  The rounding of
+// (origin - o) * inv and of the FMA is about 2^-23 of the node's distance
+// from the origin, far below the boxes' padding of 2^-14 max|coord| for
+// origins within ~100x the scene extent, so the test stays conservative.
 template <int W>
 __device__ __forceinline__ void child_dists(const WNode<W>& nd, const TraceRay& ray, float tcap,
                                             float (&dist)[W]) {
   const v3 inv = ray.inv, o = ray.pre.o;
-  const v3 noi = mk(-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z));
+  const float ax = qstep(nd.ex) * inv.x, ay = qstep(nd.ey) * inv.y, az = qstep(nd.ez) * inv.z;
+  const float bx = (nd.ox - o.x) * inv.x, by = (nd.oy - o.y) * inv.y, bz = (nd.oz - o.z) * inv.z;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    float t0x = __builtin_fmaf(nd.lo_x[k], inv.x, noi.x), t1x = __builtin_fmaf(nd.hi_x[k], inv.x, noi.x);
-    float t0y = __builtin_fmaf(nd.lo_y[k], inv.y, noi.y), t1y = __builtin_fmaf(nd.hi_y[k], inv.y, noi.y);
-    float t0z = __builtin_fmaf(nd.lo_z[k], inv.z, noi.z), t1z = __builtin_fmaf(nd.hi_z[k], inv.z, noi.z);
+    float t0x = __builtin_fmaf((float)nd.q[QLO_X][k], ax, bx), t1x = __builtin_fmaf((float)nd.q[QHI_X][k], ax, bx);
+    float t0y = __builtin_fmaf((float)nd.q[QLO_Y][k], ay, by), t1y = __builtin_fmaf((float)nd.q[QHI_Y][k], ay, by);
+    float t0z = __builtin_fmaf((float)nd.q[QLO_Z][k], az, bz), t1z = __builtin_fmaf((float)nd.q[QHI_Z][k], az, bz);
     float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), ray.pre.tmin));
     float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcap));
     dist[k] = (tn <= tf && nd.child[k] != REF_EMPTY) ? tn : INFINITY;
@@ -345,162 +347,66 @@ __device__ __forceinline__ uint32_t pool_take(uint32_t* ctr, uint32_t n, PoolCur
   return got;
 }
 
-// Persistent traversal with dynamic ray fetch (Aila and Laine, HPG 2009),
-// shared by the ray-query kernels and the integrators' extend and shadow
-// stages.  A grid that just fills the device pulls ray indices [0, n) from
-// the pool counters at `counter` (POOL_WORDS words, zeroed by an earlier
-// kernel on the stream, see pool_take); a wave refetches once
-// `refill` of its 64 lanes are idle, so lanes do not wait for the slowest ray
-// of their wave.  Each lane runs one round of the while-while loop per
-// iteration (inner nodes to a leaf, the leaf, the next stack entry), so every
-// ray gets exactly the traversal of trace_closest / trace_any.
-//
-// Src supplies the rays: with kOneScene, members nodes/tris/root and
-// `void fetch(uint32_t i, TraceRay& ray)`; otherwise `void fetch(uint32_t i,
-// TraceRay& ray, const WNode<W>*& nodes, const float4*& tris, uint32_t& root)`
-// (rays of different scenes in one launch); and `void closest(uint32_t i,
-// const HitRec&)` or `void any(uint32_t i, bool hit)`.
-template <int W, int S, bool ANY, class Src>
-__device__ __forceinline__ void trace_pool_ww(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
-                                           uint2* const lds, const int stride) {
-  uint2 ovf[TRACE_OVF_STACK];
-  int sp = 0;
-  PoolCursor pc = pool_start();
-  bool active = false;
-  uint32_t rid = 0, ref = 0, root = 0;
-  const WNode<W>* nodes = nullptr;
-  const float4* tris = nullptr;
-  if constexpr (Src::kOneScene) {  // one BVH for every ray: keep its pointers scalar
-    nodes = src.nodes;
-    tris = src.tris;
-    root = src.root;
-  }
-  TraceRay ray;
-  HitRec best{0.0f, 0.0f, 0.0f, INVALID};
-  while (true) {
-    const uint32_t r = pool_take(counter, n, pc, active, refill);
-    if (r != ~0u) {
-      rid = r;
-      if constexpr (Src::kOneScene) src.fetch(r, ray);
-      else src.fetch(r, ray, nodes, tris, root);
-      best = HitRec{ray.pre.tmax, 0.0f, 0.0f, INVALID};
-      ref = root;
-      sp = 0;
-      active = true;
-    }
-    if (__ballot(active) == 0ull) break;
-    if (!active) continue;
-    bool have = true;
-    while (!ref_is_leaf(ref)) {
-      WNode<W> nd;
-      load_node<W>(nodes, ref, nd);
-      float dist[W];
-      child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best.t, dist);
-      if (ANY) {
-        uint32_t next = REF_EMPTY;
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-          if (dist[k] == INFINITY) continue;
-          if (next == REF_EMPTY) next = nd.child[k];
-          else APT_STACK_PUSH(nd.child[k], dist[k]);
-        }
-        if (next != REF_EMPTY) {
-          ref = next;
-          continue;
-        }
-        if (sp == 0) {
-          have = false;
-          break;
-        }
-        ref = APT_STACK_POP().x;
-      } else {
-        uint32_t refs[W];
-#pragma unroll
-        for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
-        sort_children<W>(dist, refs);
-        if (dist[0] != INFINITY) {
-#pragma unroll
-          for (int k = W - 1; k >= 1; --k)
-            if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
-          ref = refs[0];
-          continue;
-        }
-        have = false;
-        while (sp > 0) {
-          const uint2 e = APT_STACK_POP();
-          if (u2f(e.y) <= best.t) {
-            ref = e.x;
-            have = true;
-            break;
-          }
-        }
-        if (!have) break;
-      }
-    }
-    bool done = !have;
-    bool occluded = false;
-    if (have) {
-      const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
-      for (uint32_t j = first; j < first + cnt; ++j) {
-        TriHit h;
-        uint32_t prim;
-        if (leaf_tri(tris, j, ray, h, prim)) {
-          if (ANY) {
-            occluded = true;
-            break;
-          }
-          if (hit_less(h.t, prim, best.t, best.prim)) {
-            best.t = h.t;
-            best.u = h.u;
-            best.v = h.v;
-            best.prim = prim;
-          }
-        }
-      }
-      done = true;
-      if (!occluded) {
-        while (sp > 0) {
-          const uint2 e = APT_STACK_POP();
-          if (ANY || u2f(e.y) <= best.t) {
-            ref = e.x;
-            done = false;
-            break;
-          }
-        }
-      }
-    }
-    if (done) {
-      if constexpr (ANY) src.any(rid, occluded);
-      else src.closest(rid, best);
-      active = false;
-    }
-  }
+// The first nodes in breadth-first order -- the top levels, which every ray
+// visits -- are staged in LDS by each work-group of a pool kernel: 2 KB, i.e.
+// 42 / 32 / 21 nodes of width 2 / 4 / 8, which with the 8-entry stack keeps
+// eight 256-thread work-groups per CU within the 160 KB of LDS.
+constexpr uint32_t TOP_BYTES = 2048;
+template <int W>
+constexpr uint32_t top_nodes() {
+  return TOP_BYTES / (uint32_t)sizeof(WNode<W>);
 }
 
+// Copies min(n_nodes, top_nodes<W>()) nodes into `top` (TOP_BYTES of LDS)
+// and returns that count; every thread of the work-group must call it.
+template <int W>
+__device__ __forceinline__ uint32_t stage_top(const WNode<W>* __restrict__ nodes, uint32_t n_nodes,
+                                              float4* top) {
+  const uint32_t k = n_nodes < top_nodes<W>() ? n_nodes : top_nodes<W>();
+  const float4* src = reinterpret_cast<const float4*>(nodes);
+  const uint32_t n4 = k * (uint32_t)(sizeof(WNode<W>) / 16);
+  for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) top[i] = src[i];
+  __syncthreads();
+  return k;
+}
 
 // LDS entries per lane of a pool kernel: the S-entry stack plus one entry
 // that holds the (u, v) of the best hit so far (rarely written, read once per
 // ray, and kept out of the registers the fetch needs).
 constexpr int pool_lds_entries(int S) { return S + 1; }
 
-// Single-step schedule of the persistent traversal.  Every iteration each
-// active lane fetches one item and processes it: a node (its boxes and child
-// refs), or the next one or two triangles of the leaf it is in.  A node is
-// seven float4 for W = 4 and two triangles are six, so both land in the same
-// registers, and every lane's fetch of an iteration is in flight at once: one
-// memory round trip per iteration, whatever mix of nodes and leaves the wave
-// holds.  (The while-while loop of trace_pool_ww runs inner nodes until every
-// lane of the wave has reached a leaf, so lanes that get there early wait a
-// round trip per step of the others.)  The per-ray traversal is the same:
-// nearest child first, the others pushed far to near, entries popped while
-// their box starts beyond the best hit are dropped, so the hits are those of
-// trace_closest / trace_any.
+// Persistent traversal with dynamic ray fetch (Aila and Laine, HPG 2009),
+// shared by the ray-query kernels and the integrators' extend and shadow
+// stages.  A grid that just fills the device pulls ray indices [0, n) from
+// the pool counters at `counter` (POOL_WORDS words, zeroed by an earlier
+// kernel on the stream, see pool_take); a wave refetches once `refill` of
+// its 64 lanes are idle, so lanes do not wait for the slowest ray of their
+// wave.
+//
+// Every iteration each active lane fetches one item and processes it: a node
+// (its quantized boxes and child refs, at most 96 B), or the next one or two
+// triangles of the leaf it is in (48 or 96 B).  Both land in the same six
+// float4 registers and every lane's fetch of an iteration is in flight at
+// once: one memory round trip per iteration, whatever mix of nodes and leaves
+// the wave holds.  (A while-while loop runs inner nodes until every lane of
+// the wave has reached a leaf, so lanes that get there early wait a round
+// trip per step of the others.)  Each ray visits children nearest first,
+// pushes the others far to near and drops popped entries whose box starts
+// beyond its best hit, so the hits are those of trace_closest / trace_any.
+//
+// Src supplies the rays: with kOneScene, members nodes/tris/root and
+// `void fetch(uint32_t i, TraceRay& ray)`; otherwise `void fetch(uint32_t i,
+// TraceRay& ray, const WNode<W>*& nodes, const float4*& tris, uint32_t& root)`
+// (rays of different scenes in one launch); and `void closest(uint32_t i,
+// const HitRec&)` or `void any(uint32_t i, bool hit)`.  Nodes below `ktop`
+// are read from the LDS copy `top` (stage_top; ktop = 0 with several scenes).
 template <int W, int S, bool ANY, class Src>
-__device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
-                                                uint2* const lds, const int stride) {
-  static_assert(W == 2 || W == 4, "single-step traversal: node and two triangles share registers");
-  constexpr int NQ = 7;  // float4 per fetch: a W=4 node (112 B) or two triangles (96 B)
-  constexpr int N4 = (int)((6 * W + W) * 4 + 15) / 16;
+__device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
+                                           uint2* const lds, const int stride, const float4* top,
+                                           uint32_t ktop) {
+  constexpr int N4 = (int)(sizeof(WNode<W>) / 16);  // float4 per node: 3, 4, 6
+  constexpr int NQ = 6;                              // float4 per fetch: a node or two triangles
+  static_assert(N4 <= NQ, "a node must fit the fetch registers");
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
   PoolCursor pc = pool_start();
@@ -538,16 +444,18 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
     const bool in_leaf = ref_is_leaf(cur);
     const uint32_t left = ref_leaf_count(cur);
     const bool two = in_leaf && left >= 2u;
-    {
+    if (!in_leaf && cur < ktop) {
+#pragma unroll
+      for (int k = 0; k < N4; ++k) q[k] = top[cur * N4 + k];
+    } else {
       const float4* p = in_leaf ? tris + 3 * (size_t)ref_leaf_first(cur)
                                 : reinterpret_cast<const float4*>(nodes + cur);
 #pragma unroll
       for (int k = 0; k < 3; ++k) q[k] = p[k];
-      if (!in_leaf || two) {
+      if ((!in_leaf && N4 > 3) || two) {
 #pragma unroll
-        for (int k = 3; k < 6; ++k) q[k] = (k < N4 || two) ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int k = 3; k < NQ; ++k) q[k] = (k < N4 || two) ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
-      if (!in_leaf && N4 > 6) q[6] = p[6];
       // keep all of the fetch's loads ahead of the processing below, so they
       // are in flight together
       asm volatile("" ::: "memory");
@@ -625,16 +533,6 @@ __device__ __forceinline__ void trace_pool_step(Src& src, uint32_t n, uint32_t* 
       active = false;
     }
   }
-}
-
-// Persistent traversal: the single-step schedule for 2- and 4-wide BVHs, the
-// while-while one for 8-wide nodes (224 B, too large to share registers with
-// triangles).
-template <int W, int S, bool ANY, class Src>
-__device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
-                                           uint2* const lds, const int stride) {
-  if constexpr (W <= 4) trace_pool_step<W, S, ANY, Src>(src, n, counter, refill, lds, stride);
-  else trace_pool_ww<W, S, ANY, Src>(src, n, counter, refill, lds, stride);
 }
 
 }  // namespace apt

@@ -16,6 +16,8 @@ from ._ffi import BvhOpts, BvhStats, check, lib, vptr
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4")])
+# bytes of one wide node (quantized child boxes, include/apt.h apt_bvh_export)
+NODE_BYTES = {2: 48, 4: 64, 8: 96}
 
 BUILDER_HOST = 0
 BUILDER_GPU = 1
@@ -46,7 +48,7 @@ class Bvh:
     def export(self):
         """(node bytes, triangle records (n_tris*3, 4) float32) of the device layout."""
         st = self.stats()
-        nodes = np.empty(st["n_nodes"] * (16 + 24 * st["width"]), np.uint8)
+        nodes = np.empty(st["n_nodes"] * NODE_BYTES[st["width"]], np.uint8)
         tris = np.empty((st["n_tris"] * 3, 4), np.float32)
         check(lib().apt_bvh_export(self._h, vptr(nodes), nodes.size, vptr(tris), tris.nbytes),
               "apt_bvh_export")

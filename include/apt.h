@@ -124,9 +124,12 @@ apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris,
                                 const apt_bvh_opts* opts, apt_bvh** out);
 void apt_bvh_destroy(apt_bvh* bvh);
 apt_status apt_bvh_get_stats(const apt_bvh* bvh, apt_bvh_stats* out);
-/* Copies the device layout out: n_nodes wide nodes (16 + 24 * width bytes
- * each: six float[width] box rows, then width child refs) and 3 * n_tris
- * float4 triangle records in leaf order (prim id in the first record's w).
+/* Copies the device layout out: n_nodes wide nodes of 48 / 64 / 96 bytes for
+ * width 2 / 4 / 8 (float grid origin[3]; uint8 biased grid exponent[3] and one
+ * pad byte; uint32 child ref[width]; uint8 quantized box rows lo_x, lo_y,
+ * lo_z, hi_x, hi_y, hi_z of [width] bytes each; zero padding to 16 bytes) and
+ * 3 * n_tris float4 triangle records in leaf order (prim id in the first
+ * record's w).
  * No counterpart in the reference (its BVH is opaque inside the driver); for
  * callers that cache builds, and for comparing builders. */
 apt_status apt_bvh_export(const apt_bvh* bvh, void* nodes, size_t nodes_len, void* tris,
