@@ -149,122 +149,18 @@ __device__ __forceinline__ uint2 stack_pop(int& sp, const uint2* lds, int stride
 // Closest hit: the lexicographic minimum of (t, prim) over all triangles that
 // pass the watertight test with tmin < t < tmax.  A node is entered while its
 // padded box starts no later than the best t so far, so equal-t candidates are
-// still visited and the tie goes to the lower prim.  The loop runs inner nodes
-// until the lane reaches a leaf, then the leaf, so lanes of a wave tend to do
-// the same kind of work together; popped entries whose box starts beyond the
-// best t are dropped without being fetched.
+// still visited and the tie goes to the lower prim.  One ray per lane, in the
+// single-step schedule (step_fetch / step_item, defined below).
 template <int W, int S>
 __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
                                                 const float4* __restrict__ tris, uint32_t root,
-                                                const TraceRay& ray, LdsStack st) {
-  uint2* const lds = st.base;
-  const int stride = st.stride;
-  uint2 ovf[TRACE_OVF_STACK];
-  int sp = 0;
-  HitRec best{ray.pre.tmax, 0.0f, 0.0f, INVALID};
-  uint32_t ref = root;
-  bool have = true;
-  while (have) {
-    while (!ref_is_leaf(ref)) {
-      WNode<W> nd;
-      load_node<W>(nodes, ref, nd);
-      float dist[W];
-      uint32_t refs[W];
-      child_dists<W>(nd, ray, best.t, dist);
-#pragma unroll
-      for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
-      sort_children<W>(dist, refs);
-      if (dist[0] != INFINITY) {
-#pragma unroll
-        for (int k = W - 1; k >= 1; --k)
-          if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
-        ref = refs[0];
-        continue;
-      }
-      have = false;
-      while (sp > 0) {
-        const uint2 e = APT_STACK_POP();
-        if (u2f(e.y) <= best.t) {
-          ref = e.x;
-          have = true;
-          break;
-        }
-      }
-      if (!have) break;
-    }
-    if (!have) break;
-    const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
-    for (uint32_t j = first; j < first + cnt; ++j) {
-      TriHit h;
-      uint32_t prim;
-      if (leaf_tri(tris, j, ray, h, prim) && hit_less(h.t, prim, best.t, best.prim)) {
-        best.t = h.t;
-        best.u = h.u;
-        best.v = h.v;
-        best.prim = prim;
-      }
-    }
-    have = false;
-    while (sp > 0) {
-      const uint2 e = APT_STACK_POP();
-      if (u2f(e.y) <= best.t) {
-        ref = e.x;
-        have = true;
-        break;
-      }
-    }
-  }
-  return best;
-}
-
+                                                const TraceRay& ray, LdsStack st);
 // Any hit: true as soon as one triangle passes with tmin < t < tmax
 // (the reference's terminate-on-first-hit shadow query, helpers/trace.wgsl:20-26).
 template <int W, int S>
 __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
                                           const float4* __restrict__ tris, uint32_t root,
-                                          const TraceRay& ray, LdsStack st) {
-  uint2* const lds = st.base;
-  const int stride = st.stride;
-  uint2 ovf[TRACE_OVF_STACK];
-  int sp = 0;
-  uint32_t ref = root;
-  while (true) {
-    bool have = true;
-    while (!ref_is_leaf(ref)) {
-      WNode<W> nd;
-      load_node<W>(nodes, ref, nd);
-      float dist[W];
-      child_dists<W>(nd, ray, ray.pre.tmax, dist);
-      uint32_t next = REF_EMPTY;
-#pragma unroll
-      for (int k = 0; k < W; ++k) {
-        if (dist[k] == INFINITY) continue;
-        if (next == REF_EMPTY) next = nd.child[k];
-        else APT_STACK_PUSH(nd.child[k], dist[k]);
-      }
-      if (next != REF_EMPTY) {
-        ref = next;
-        continue;
-      }
-      if (sp == 0) {
-        have = false;
-        break;
-      }
-      ref = APT_STACK_POP().x;
-    }
-    if (!have) break;
-    const uint32_t first = ref_leaf_first(ref), cnt = ref_leaf_count(ref);
-    for (uint32_t j = first; j < first + cnt; ++j) {
-      TriHit h;
-      uint32_t prim;
-      if (leaf_tri(tris, j, ray, h, prim)) return true;
-    }
-    if (sp == 0) break;
-    ref = APT_STACK_POP().x;
-  }
-  return false;
-}
-
+                                          const TraceRay& ray, LdsStack st);
 
 // Host-side dispatch over the supported LDS stack depths.
 #define APT_FOR_STACK_DEPTH(S, ...) \
@@ -375,6 +271,127 @@ __device__ __forceinline__ uint32_t stage_top(const WNode<W>* __restrict__ nodes
 // ray, and kept out of the registers the fetch needs).
 constexpr int pool_lds_entries(int S) { return S + 1; }
 
+// Single-step traversal.  Every iteration a lane fetches one item and
+// processes it: a node (its quantized boxes and child refs, at most 96 B), or
+// the next one or two triangles of the leaf it is in (48 or 96 B).  Both land
+// in the same six float4 registers and every lane's fetch of an iteration is
+// in flight at once: one memory round trip per iteration, whatever mix of
+// nodes and leaves the wave holds.  (A while-while loop runs inner nodes
+// until every lane of the wave has reached a leaf, so lanes that get there
+// early wait a round trip per step of the others.)  A ray visits children
+// nearest first, pushes the others far to near and drops popped entries whose
+// box starts beyond its best hit; `cur` is the node to fetch next or the leaf
+// ref of the triangles still to test (first advanced and count lowered as the
+// leaf is consumed).
+constexpr int STEP_NQ = 6;  // float4 per fetch: a node or two triangles
+
+template <int W>
+__device__ __forceinline__ void step_fetch(const WNode<W>* __restrict__ nodes, const float4* __restrict__ tris,
+                                           const float4* top, uint32_t ktop, uint32_t cur,
+                                           float4 (&q)[STEP_NQ]) {
+  constexpr int N4 = (int)(sizeof(WNode<W>) / 16);  // float4 per node: 3, 4, 6
+  static_assert(N4 <= STEP_NQ, "a node must fit the fetch registers");
+  const bool in_leaf = ref_is_leaf(cur);
+  const bool two = in_leaf && ref_leaf_count(cur) >= 2u;
+  if (!in_leaf && cur < ktop) {
+#pragma unroll
+    for (int k = 0; k < N4; ++k) q[k] = top[cur * N4 + k];
+  } else {
+    const float4* p = in_leaf ? tris + 3 * (size_t)ref_leaf_first(cur)
+                              : reinterpret_cast<const float4*>(nodes + cur);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q[k] = p[k];
+    if ((!in_leaf && N4 > 3) || two) {
+#pragma unroll
+      for (int k = 3; k < STEP_NQ; ++k) q[k] = (k < N4 || two) ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // keep all of the fetch's loads ahead of the processing, so they are in
+    // flight together
+    asm volatile("" ::: "memory");
+  }
+}
+
+// Processes the item step_fetch loaded for `cur` and moves `cur` on; returns
+// true when the ray is finished (stack empty, or an any-hit occluder found).
+// The (u, v) of an improved closest hit go to *uv when UV_LDS, else to hu, hv.
+template <int W, int S, bool ANY, bool UV_LDS>
+__device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], const TraceRay& ray, uint32_t& cur,
+                                          float& best_t, uint32_t& best_prim, float& hu, float& hv, uint2* uv,
+                                          bool& occluded, uint2* const lds, const int stride,
+                                          uint2 (&ovf)[TRACE_OVF_STACK], int& sp) {
+  constexpr int N4 = (int)(sizeof(WNode<W>) / 16);
+  const bool in_leaf = ref_is_leaf(cur);
+  const uint32_t left = ref_leaf_count(cur);
+  const bool two = in_leaf && left >= 2u;
+  uint32_t next = REF_EMPTY;  // ref to enter after this item; REF_EMPTY: pop
+  bool stay = false;
+  if (!in_leaf) {
+    WNode<W> nd;
+    __builtin_memcpy(&nd, q, sizeof(float4) * N4);
+    float dist[W];
+    child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best_t, dist);
+    if (ANY) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        if (dist[k] == INFINITY) continue;
+        if (next == REF_EMPTY) next = nd.child[k];
+        else APT_STACK_PUSH(nd.child[k], dist[k]);
+      }
+    } else {
+      uint32_t refs[W];
+#pragma unroll
+      for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
+      sort_children<W>(dist, refs);
+      if (dist[0] != INFINITY) {
+#pragma unroll
+        for (int k = W - 1; k >= 1; --k)
+          if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
+        next = refs[0];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      if (h2 == 1 && !two) break;
+      const float4 a = q[3 * h2], b = q[3 * h2 + 1], c = q[3 * h2 + 2];
+      TriHit h;
+      if (intersect_tri(ray.pre, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h)) {
+        if (ANY) {
+          occluded = true;
+        } else {
+          const uint32_t prim = f2u(a.w);
+          if (hit_less(h.t, prim, best_t, best_prim)) {
+            best_t = h.t;
+            best_prim = prim;
+            if (UV_LDS) {
+              *uv = make_uint2(f2u(h.u), f2u(h.v));
+            } else {
+              hu = h.u;
+              hv = h.v;
+            }
+          }
+        }
+      }
+    }
+    stay = left > 2u;
+    if (stay) cur += 2u - (2u << 28);  // first + 2, count - 2
+  }
+  if (occluded) return true;
+  if (stay) return false;
+  if (next == REF_EMPTY) {
+    while (sp > 0) {
+      const uint2 e = APT_STACK_POP();
+      if (ANY || u2f(e.y) <= best_t) {
+        next = e.x;
+        break;
+      }
+    }
+  }
+  if (next == REF_EMPTY) return true;
+  cur = next;
+  return false;
+}
+
 // Persistent traversal with dynamic ray fetch (Aila and Laine, HPG 2009),
 // shared by the ray-query kernels and the integrators' extend and shadow
 // stages.  A grid that just fills the device pulls ray indices [0, n) from
@@ -383,16 +400,7 @@ constexpr int pool_lds_entries(int S) { return S + 1; }
 // its 64 lanes are idle, so lanes do not wait for the slowest ray of their
 // wave.
 //
-// Every iteration each active lane fetches one item and processes it: a node
-// (its quantized boxes and child refs, at most 96 B), or the next one or two
-// triangles of the leaf it is in (48 or 96 B).  Both land in the same six
-// float4 registers and every lane's fetch of an iteration is in flight at
-// once: one memory round trip per iteration, whatever mix of nodes and leaves
-// the wave holds.  (A while-while loop runs inner nodes until every lane of
-// the wave has reached a leaf, so lanes that get there early wait a round
-// trip per step of the others.)  Each ray visits children nearest first,
-// pushes the others far to near and drops popped entries whose box starts
-// beyond its best hit, so the hits are those of trace_closest / trace_any.
+// Every iteration each active lane runs one step (step_fetch, step_item).
 //
 // Src supplies the rays: with kOneScene, members nodes/tris/root and
 // `void fetch(uint32_t i, TraceRay& ray)`; otherwise `void fetch(uint32_t i,
@@ -404,9 +412,6 @@ template <int W, int S, bool ANY, class Src>
 __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
                                            uint2* const lds, const int stride, const float4* top,
                                            uint32_t ktop) {
-  constexpr int N4 = (int)(sizeof(WNode<W>) / 16);  // float4 per node: 3, 4, 6
-  constexpr int NQ = 6;                              // float4 per fetch: a node or two triangles
-  static_assert(N4 <= NQ, "a node must fit the fetch registers");
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
   PoolCursor pc = pool_start();
@@ -439,89 +444,12 @@ __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* count
     }
     if (__ballot(active) == 0ull) break;
     if (!active) continue;
-    // Fetch: a node, or the leaf's next one or two triangles.
-    float4 q[NQ];
-    const bool in_leaf = ref_is_leaf(cur);
-    const uint32_t left = ref_leaf_count(cur);
-    const bool two = in_leaf && left >= 2u;
-    if (!in_leaf && cur < ktop) {
-#pragma unroll
-      for (int k = 0; k < N4; ++k) q[k] = top[cur * N4 + k];
-    } else {
-      const float4* p = in_leaf ? tris + 3 * (size_t)ref_leaf_first(cur)
-                                : reinterpret_cast<const float4*>(nodes + cur);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) q[k] = p[k];
-      if ((!in_leaf && N4 > 3) || two) {
-#pragma unroll
-        for (int k = 3; k < NQ; ++k) q[k] = (k < N4 || two) ? p[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      }
-      // keep all of the fetch's loads ahead of the processing below, so they
-      // are in flight together
-      asm volatile("" ::: "memory");
-    }
-    uint32_t next = REF_EMPTY;  // ref to enter after this item; REF_EMPTY: pop
-    bool stay = false, occluded = false;
-    if (!in_leaf) {
-      WNode<W> nd;
-      __builtin_memcpy(&nd, q, sizeof(float4) * N4);
-      float dist[W];
-      child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best_t, dist);
-      if (ANY) {
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-          if (dist[k] == INFINITY) continue;
-          if (next == REF_EMPTY) next = nd.child[k];
-          else APT_STACK_PUSH(nd.child[k], dist[k]);
-        }
-      } else {
-        uint32_t refs[W];
-#pragma unroll
-        for (int k = 0; k < W; ++k) refs[k] = nd.child[k];
-        sort_children<W>(dist, refs);
-        if (dist[0] != INFINITY) {
-#pragma unroll
-          for (int k = W - 1; k >= 1; --k)
-            if (dist[k] != INFINITY) APT_STACK_PUSH(refs[k], dist[k]);
-          next = refs[0];
-        }
-      }
-    } else {
-#pragma unroll
-      for (int h2 = 0; h2 < 2; ++h2) {
-        if (h2 == 1 && !two) break;
-        const float4 a = q[3 * h2], b = q[3 * h2 + 1], c = q[3 * h2 + 2];
-        TriHit h;
-        if (intersect_tri(ray.pre, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h)) {
-          if (ANY) {
-            occluded = true;
-          } else {
-            const uint32_t prim = f2u(a.w);
-            if (hit_less(h.t, prim, best_t, best_prim)) {
-              best_t = h.t;
-              best_prim = prim;
-              *uv = make_uint2(f2u(h.u), f2u(h.v));
-            }
-          }
-        }
-      }
-      stay = left > 2u;
-      if (stay) cur += 2u - (2u << 28);  // first + 2, count - 2
-    }
-    bool done = occluded;
-    if (!done && !stay) {
-      if (next == REF_EMPTY) {
-        while (sp > 0) {
-          const uint2 e = APT_STACK_POP();
-          if (ANY || u2f(e.y) <= best_t) {
-            next = e.x;
-            break;
-          }
-        }
-      }
-      if (next == REF_EMPTY) done = true;
-      else cur = next;
-    }
+    float4 q[STEP_NQ];
+    step_fetch<W>(nodes, tris, top, ktop, cur, q);
+    bool occluded = false;
+    float hu, hv;
+    const bool done = step_item<W, S, ANY, true>(q, ray, cur, best_t, best_prim, hu, hv, uv, occluded, lds, stride,
+                                                 ovf, sp);
     if (done) {
       if constexpr (ANY) {
         src.any(rid, occluded);
@@ -533,6 +461,50 @@ __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* count
       active = false;
     }
   }
+}
+
+template <int W, int S>
+__device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
+                                                const float4* __restrict__ tris, uint32_t root,
+                                                const TraceRay& ray, LdsStack st) {
+  uint2* const lds = st.base;
+  const int stride = st.stride;
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  uint32_t cur = root;
+  float best_t = ray.pre.tmax, hu = 0.0f, hv = 0.0f;
+  uint32_t best_prim = INVALID;
+  bool occluded = false;
+  while (true) {
+    float4 q[STEP_NQ];
+    step_fetch<W>(nodes, tris, nullptr, 0u, cur, q);
+    if (step_item<W, S, false, false>(q, ray, cur, best_t, best_prim, hu, hv, nullptr, occluded, lds, stride,
+                                      ovf, sp))
+      break;
+  }
+  return HitRec{best_t, hu, hv, best_prim};
+}
+
+template <int W, int S>
+__device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
+                                          const float4* __restrict__ tris, uint32_t root,
+                                          const TraceRay& ray, LdsStack st) {
+  uint2* const lds = st.base;
+  const int stride = st.stride;
+  uint2 ovf[TRACE_OVF_STACK];
+  int sp = 0;
+  uint32_t cur = root;
+  float best_t = ray.pre.tmax, hu, hv;
+  uint32_t best_prim = INVALID;
+  bool occluded = false;
+  while (true) {
+    float4 q[STEP_NQ];
+    step_fetch<W>(nodes, tris, nullptr, 0u, cur, q);
+    if (step_item<W, S, true, false>(q, ray, cur, best_t, best_prim, hu, hv, nullptr, occluded, lds, stride,
+                                     ovf, sp))
+      break;
+  }
+  return occluded;
 }
 
 }  // namespace apt
