@@ -102,6 +102,7 @@ static apt_status build_impl(const float* tri_xyz, const float* d_tri_xyz, uint3
   if (st != APT_OK) return st;
   auto t0 = std::chrono::steady_clock::now();
   BvhHostResult b2;
+  apt_bvh* bvh = nullptr;
   if (o.builder == APT_BUILDER_GPU) {
     float* d_tmp = nullptr;
     if (!d_tri_xyz) {
@@ -116,13 +117,18 @@ static apt_status build_impl(const float* tri_xyz, const float* d_tri_xyz, uint3
       d_tri_xyz = d_tmp;
     }
     st = build_bvh2_device(d_tri_xyz, n_tris, o.max_leaf, o.bins, lib_stream(), b2);
+    if (st == APT_OK) {
+      bvh = new apt_bvh();
+      st = upload_wide_bvh(tri_xyz, d_tri_xyz, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh);
+    }
     if (d_tmp) (void)hipFree(d_tmp);
   } else {
     st = build_bvh2_host(tri_xyz, n_tris, o.max_leaf, o.bins, b2);
+    if (st == APT_OK) {
+      bvh = new apt_bvh();
+      st = upload_wide_bvh(tri_xyz, nullptr, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh);
+    }
   }
-  if (st != APT_OK) return st;
-  apt_bvh* bvh = new apt_bvh();
-  st = upload_wide_bvh(tri_xyz, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh);
   if (st != APT_OK) {
     delete bvh;
     return st;

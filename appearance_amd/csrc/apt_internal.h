@@ -142,9 +142,14 @@ constexpr uint32_t POOL_STRIDE = 16;
 constexpr uint32_t POOL_WORDS = POOL_PARTS * POOL_STRIDE;
 
 namespace apt {
-// Collapse a binary BVH into a wide one and upload it (bvh_host.cpp).
-apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2,
+// Collapse a binary BVH into a wide one and upload it (bvh_host.cpp).  The
+// leaf-order triangle records are gathered on the device from d_tri_xyz when
+// it is not null, else on the host from tri_xyz.
+apt_status upload_wide_bvh(const float* tri_xyz, const float* d_tri_xyz, uint32_t n, const BvhHostResult& b2,
                            uint32_t width, float pad, apt_bvh* out);
+// tris[3i..3i+2] = triangle order[i] of d_tri_xyz (v0 | prim, v1, v2) (bvh_gpu.hip).
+apt_status gather_tris_device(const float* d_tri_xyz, const uint32_t* h_order, uint32_t n, float4* d_tris,
+                              hipStream_t s);
 // Launch the traversal kernels (trace.hip).
 apt_status launch_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
                                 apt_hit* d_hits, hipStream_t s);

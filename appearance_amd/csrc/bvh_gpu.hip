@@ -11,11 +11,13 @@
 //   3. partitions the node's index range stably with wave ballots (left
 //      triangles keep their order, then right ones), reducing both children's
 //      triangle and centroid bounds on the way.
-// A single-block scan then numbers the children of the splitting nodes and
-// forms the next level.  Every choice depends only on the set of triangles in
+// Nodes of more than LARGE_NODE triangles (the top levels) run the same three
+// steps over many blocks instead of one wave.  A single-block scan then
+// numbers the children of the splitting nodes and forms the next level.  Every choice depends only on the set of triangles in
 // a node and their input order, so the binary tree, its leaves and their
 // triangle order equal the host builder's (which uses a stable partition too);
 // the breadth-first collapse to 2/4/8-wide nodes is shared with it.
+#include <algorithm>
 #include <chrono>
 #include <vector>
 
@@ -106,6 +108,7 @@ struct Level {
   uint32_t n_active;
   uint32_t depth;
   uint32_t max_leaf, bins;
+  uint32_t large;             // nodes above this many triangles use the multi-block path
   const float* prim;
   const uint32_t* order_in;
   uint32_t* order_out;
@@ -129,6 +132,72 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+// SAH sweep of the host builder over three axes of bins (lo/hi box keys and
+// counts): right-to-left areas and counts, then left-to-right costs; the
+// first strictly smaller cost wins.  One thread.
+__device__ void sah_sweep(const int (*blo)[MAX_BINS][3], const int (*bhi)[MAX_BINS][3],
+                          const uint32_t (*bcnt)[MAX_BINS], const bool* use, int bins, int& axis_out,
+                          int& split_out, uint32_t& nleft_out) {
+  float best_cost = INFINITY;
+  int best_axis = -1, best_split = 0;
+  uint32_t best_left = 0;
+  float right_area[MAX_BINS];
+  uint32_t right_cnt[MAX_BINS];
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!use[ax]) continue;
+    GBox acc;
+    for (int k = 0; k < 3; ++k) {
+      acc.lo[k] = INFINITY;
+      acc.hi[k] = -INFINITY;
+    }
+    uint32_t cnt = 0;
+    for (int b = bins - 1; b > 0; --b) {
+      for (int k = 0; k < 3; ++k) {
+        acc.lo[k] = smin(acc.lo[k], ofloat(blo[ax][b][k]));
+        acc.hi[k] = smax(acc.hi[k], ofloat(bhi[ax][b][k]));
+      }
+      cnt += bcnt[ax][b];
+      right_area[b] = box_area(acc.lo, acc.hi);
+      right_cnt[b] = cnt;
+    }
+    for (int k = 0; k < 3; ++k) {
+      acc.lo[k] = INFINITY;
+      acc.hi[k] = -INFINITY;
+    }
+    cnt = 0;
+    for (int b = 1; b < bins; ++b) {
+      for (int k = 0; k < 3; ++k) {
+        acc.lo[k] = smin(acc.lo[k], ofloat(blo[ax][b - 1][k]));
+        acc.hi[k] = smax(acc.hi[k], ofloat(bhi[ax][b - 1][k]));
+      }
+      cnt += bcnt[ax][b - 1];
+      if (cnt == 0 || right_cnt[b] == 0) continue;
+      const float cost = box_area(acc.lo, acc.hi) * (float)cnt + right_area[b] * (float)right_cnt[b];
+      if (cost < best_cost) {
+        best_cost = cost;
+        best_axis = ax;
+        best_split = b;
+        best_left = cnt;
+      }
+    }
+  }
+  axis_out = best_axis;
+  split_out = best_split;
+  nleft_out = best_left;
+}
+
+// Binning set-up of a node from its centroid box: the axes that can split
+// (non-flat, above the depth limit) and their bin scale.
+__device__ __forceinline__ void bin_setup(const float* box, uint32_t depth, int bins, float* cblo, float* scale,
+                                          bool* use) {
+  for (int k = 0; k < 3; ++k) {
+    cblo[k] = box[6 + k];
+    const float ext = box[9 + k] - box[6 + k];
+    use[k] = ext > 0.0f && depth < (uint32_t)MAX_DEPTH;
+    scale[k] = use[k] ? (float)bins / ext : 0.0f;
+  }
+}
+
 __global__ __launch_bounds__(WAVE) void k_level(Level L) {
   __shared__ int s_lo[3][MAX_BINS][3];
   __shared__ int s_hi[3][MAX_BINS][3];
@@ -148,6 +217,7 @@ __global__ __launch_bounds__(WAVE) void k_level(Level L) {
       nd.hi[k] = box[3 + k];
     }
   }
+  if (n > L.large) return;  // split by the multi-block kernels below
   if (n <= L.max_leaf) {
     for (uint32_t i = begin + lane; i < end; i += WAVE) L.order_out[i] = L.order_in[i];
     if (lane == 0) {
@@ -161,12 +231,7 @@ __global__ __launch_bounds__(WAVE) void k_level(Level L) {
   const int bins = (int)L.bins;
   float cblo[3], scale[3];
   bool use[3];
-  for (int k = 0; k < 3; ++k) {
-    cblo[k] = box[6 + k];
-    const float ext = box[9 + k] - box[6 + k];
-    use[k] = ext > 0.0f && L.depth < (uint32_t)MAX_DEPTH;
-    scale[k] = use[k] ? (float)bins / ext : 0.0f;
-  }
+  bin_setup(box, L.depth, bins, cblo, scale, use);
   if (lane == 0) s_axis = -1;
   if (use[0] || use[1] || use[2]) {
     for (int j = lane; j < 3 * MAX_BINS; j += WAVE) {
@@ -193,56 +258,7 @@ __global__ __launch_bounds__(WAVE) void k_level(Level L) {
       }
     }
     __syncthreads();
-    if (lane == 0) {
-      // SAH sweep of the host builder: right-to-left areas and counts, then
-      // left-to-right costs, first strictly smaller cost wins
-      float best_cost = INFINITY;
-      int best_axis = -1, best_split = 0;
-      uint32_t best_left = 0;
-      float right_area[MAX_BINS];
-      uint32_t right_cnt[MAX_BINS];
-      for (int ax = 0; ax < 3; ++ax) {
-        if (!use[ax]) continue;
-        GBox acc;
-        for (int k = 0; k < 3; ++k) {
-          acc.lo[k] = INFINITY;
-          acc.hi[k] = -INFINITY;
-        }
-        uint32_t cnt = 0;
-        for (int b = bins - 1; b > 0; --b) {
-          for (int k = 0; k < 3; ++k) {
-            acc.lo[k] = smin(acc.lo[k], ofloat(s_lo[ax][b][k]));
-            acc.hi[k] = smax(acc.hi[k], ofloat(s_hi[ax][b][k]));
-          }
-          cnt += s_cnt[ax][b];
-          right_area[b] = box_area(acc.lo, acc.hi);
-          right_cnt[b] = cnt;
-        }
-        for (int k = 0; k < 3; ++k) {
-          acc.lo[k] = INFINITY;
-          acc.hi[k] = -INFINITY;
-        }
-        cnt = 0;
-        for (int b = 1; b < bins; ++b) {
-          for (int k = 0; k < 3; ++k) {
-            acc.lo[k] = smin(acc.lo[k], ofloat(s_lo[ax][b - 1][k]));
-            acc.hi[k] = smax(acc.hi[k], ofloat(s_hi[ax][b - 1][k]));
-          }
-          cnt += s_cnt[ax][b - 1];
-          if (cnt == 0 || right_cnt[b] == 0) continue;
-          const float cost = box_area(acc.lo, acc.hi) * (float)cnt + right_area[b] * (float)right_cnt[b];
-          if (cost < best_cost) {
-            best_cost = cost;
-            best_axis = ax;
-            best_split = b;
-            best_left = cnt;
-          }
-        }
-      }
-      s_axis = best_axis;
-      s_split = best_split;
-      s_nleft = best_left;
-    }
+    if (lane == 0) sah_sweep(s_lo, s_hi, s_cnt, use, bins, s_axis, s_split, s_nleft);
   }
   __syncthreads();
   const int axis = s_axis, split = s_split;
@@ -297,28 +313,340 @@ __global__ __launch_bounds__(WAVE) void k_level(Level L) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-block path for large nodes (the top levels, where a level has few
+// nodes of many triangles and one wave per node would leave the GPU idle).
+// The same three steps, spread over blocks of LARGE_CHUNK triangles: the bins
+// and child boxes are merged across blocks with integer atomics on
+// order-preserving keys (exact and order-independent), the sweep is the same
+// function on the merged bins, and the stable partition places each block's
+// left and right triangles after those of the blocks before it.
+// ---------------------------------------------------------------------------
+constexpr uint32_t LARGE_NODE = 16384;  // triangles above which a node takes this path
+constexpr uint32_t LARGE_CHUNK = 4096;  // triangles per block
+constexpr int LBLOCK = 256;
+constexpr int BIN_WORDS = 3 * MAX_BINS * 3 * 2 + 3 * MAX_BINS;  // lo keys, hi keys, counts
+
+struct Large {
+  uint32_t n;                  // large nodes of this level
+  const uint32_t* act;         // [n] active index of each
+  const uint32_t* blk_node;    // [blocks] large node of each block
+  const uint32_t* blk_first;   // [n] first block of each large node
+  int* bins;                   // [n][BIN_WORDS]
+  int* info;                   // [n][4]: axis, split, n_left
+  uint32_t* blk_left;          // [blocks]: left triangles per block, then their exclusive prefix
+  int* cbox;                   // [n][24] child box keys
+  uint32_t blocks;
+};
+
+__device__ __forceinline__ int* bins_lo(int* b) { return b; }
+__device__ __forceinline__ int* bins_hi(int* b) { return b + 3 * MAX_BINS * 3; }
+__device__ __forceinline__ uint32_t* bins_cnt(int* b) { return reinterpret_cast<uint32_t*>(b + 2 * 3 * MAX_BINS * 3); }
+
+__global__ __launch_bounds__(LBLOCK) void k_large_init(Large G) {
+  int* b = G.bins + (size_t)blockIdx.x * BIN_WORDS;
+  for (int i = threadIdx.x; i < 3 * MAX_BINS * 3; i += LBLOCK) {
+    bins_lo(b)[i] = okey(INFINITY);
+    bins_hi(b)[i] = okey(-INFINITY);
+  }
+  for (int i = threadIdx.x; i < 3 * MAX_BINS; i += LBLOCK) bins_cnt(b)[i] = 0u;
+  if (threadIdx.x < 24) {
+    const int k = threadIdx.x % 12;
+    G.cbox[24 * blockIdx.x + threadIdx.x] = (k % 6) < 3 ? okey(INFINITY) : okey(-INFINITY);
+  }
+}
+
+// Triangle range of block `blk` of large node j.
+__device__ __forceinline__ void large_chunk(const Level& L, const Large& G, uint32_t blk, uint32_t& j,
+                                            uint32_t& a, uint32_t& c0, uint32_t& c1) {
+  j = G.blk_node[blk];
+  a = G.act[j];
+  const uint32_t begin = L.act_range[2 * a], end = L.act_range[2 * a + 1];
+  c0 = begin + (blk - G.blk_first[j]) * LARGE_CHUNK;
+  c1 = min(end, c0 + LARGE_CHUNK);
+}
+
+__global__ __launch_bounds__(LBLOCK) void k_large_bin(Level L, Large G) {
+  __shared__ int s_lo[3][MAX_BINS][3];
+  __shared__ int s_hi[3][MAX_BINS][3];
+  __shared__ uint32_t s_cnt[3][MAX_BINS];
+  uint32_t j, a, c0, c1;
+  large_chunk(L, G, blockIdx.x, j, a, c0, c1);
+  const int bins = (int)L.bins;
+  float cblo[3], scale[3];
+  bool use[3];
+  bin_setup(L.act_box + 12 * (size_t)a, L.depth, bins, cblo, scale, use);
+  if (!(use[0] || use[1] || use[2])) return;
+  for (int i = threadIdx.x; i < 3 * MAX_BINS; i += LBLOCK) {
+    const int ax = i / MAX_BINS, b = i % MAX_BINS;
+    for (int k = 0; k < 3; ++k) {
+      s_lo[ax][b][k] = okey(INFINITY);
+      s_hi[ax][b][k] = okey(-INFINITY);
+    }
+    s_cnt[ax][b] = 0;
+  }
+  __syncthreads();
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += LBLOCK) {
+    const float* p = L.prim + (size_t)L.order_in[i] * 9;
+    const int kl[3] = {okey(p[0]), okey(p[1]), okey(p[2])};
+    const int kh[3] = {okey(p[3]), okey(p[4]), okey(p[5])};
+    for (int ax = 0; ax < 3; ++ax) {
+      if (!use[ax]) continue;
+      const int b = bin_of(p[6 + ax], cblo[ax], scale[ax], bins);
+      for (int k = 0; k < 3; ++k) {
+        atomicMin(&s_lo[ax][b][k], kl[k]);
+        atomicMax(&s_hi[ax][b][k], kh[k]);
+      }
+      atomicAdd(&s_cnt[ax][b], 1u);
+    }
+  }
+  __syncthreads();
+  int* gb = G.bins + (size_t)j * BIN_WORDS;
+  for (int i = threadIdx.x; i < 3 * MAX_BINS; i += LBLOCK) {
+    const int ax = i / MAX_BINS, b = i % MAX_BINS;
+    if (!s_cnt[ax][b]) continue;
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&bins_lo(gb)[(ax * MAX_BINS + b) * 3 + k], s_lo[ax][b][k]);
+      atomicMax(&bins_hi(gb)[(ax * MAX_BINS + b) * 3 + k], s_hi[ax][b][k]);
+    }
+    atomicAdd(&bins_cnt(gb)[ax * MAX_BINS + b], s_cnt[ax][b]);
+  }
+}
+
+__global__ void k_large_sweep(Level L, Large G) {
+  const uint32_t j = blockIdx.x, a = G.act[j];
+  const uint32_t n = L.act_range[2 * a + 1] - L.act_range[2 * a];
+  float cblo[3], scale[3];
+  bool use[3];
+  bin_setup(L.act_box + 12 * (size_t)a, L.depth, (int)L.bins, cblo, scale, use);
+  int axis = -1, split = 0;
+  uint32_t nleft = 0;
+  int* gb = G.bins + (size_t)j * BIN_WORDS;
+  if (use[0] || use[1] || use[2])
+    sah_sweep(reinterpret_cast<const int (*)[MAX_BINS][3]>(bins_lo(gb)),
+              reinterpret_cast<const int (*)[MAX_BINS][3]>(bins_hi(gb)),
+              reinterpret_cast<const uint32_t (*)[MAX_BINS]>(bins_cnt(gb)), use, (int)L.bins, axis, split, nleft);
+  G.info[4 * j + 0] = axis;
+  G.info[4 * j + 1] = split;
+  G.info[4 * j + 2] = (int)(axis >= 0 ? nleft : n / 2);  // object median when nothing splits
+}
+
+// The split test of large node j (axis, split bin, or the object median).
+struct LargeSplit {
+  int axis, split;
+  uint32_t begin, n_left;
+  float lo, scale;
+};
+
+__device__ __forceinline__ LargeSplit large_split(const Level& L, const Large& G, uint32_t j, uint32_t a) {
+  LargeSplit t;
+  t.axis = G.info[4 * j];
+  t.split = G.info[4 * j + 1];
+  t.n_left = (uint32_t)G.info[4 * j + 2];
+  t.begin = L.act_range[2 * a];
+  float cblo[3], scale[3];
+  bool use[3];
+  bin_setup(L.act_box + 12 * (size_t)a, L.depth, (int)L.bins, cblo, scale, use);
+  t.lo = t.axis >= 0 ? cblo[t.axis] : 0.0f;
+  t.scale = t.axis >= 0 ? scale[t.axis] : 0.0f;
+  return t;
+}
+
+__device__ __forceinline__ bool large_left(const Level& L, const LargeSplit& t, uint32_t i, uint32_t idx) {
+  if (t.axis < 0) return i - t.begin < t.n_left;
+  return bin_of(L.prim[(size_t)idx * 9 + 6 + t.axis], t.lo, t.scale, (int)L.bins) < t.split;
+}
+
+__global__ __launch_bounds__(LBLOCK) void k_large_count(Level L, Large G) {
+  __shared__ uint32_t s_sum;
+  uint32_t j, a, c0, c1;
+  large_chunk(L, G, blockIdx.x, j, a, c0, c1);
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  const LargeSplit t = large_split(L, G, j, a);
+  uint32_t c = 0;
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += LBLOCK) c += large_left(L, t, i, L.order_in[i]) ? 1u : 0u;
+  atomicAdd(&s_sum, c);
+  __syncthreads();
+  if (threadIdx.x == 0) G.blk_left[blockIdx.x] = s_sum;
+}
+
+// Exclusive prefix of the per-block left counts within each large node.
+__global__ void k_large_scan(Large G) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t run = 0, node = 0xFFFFFFFFu;
+  for (uint32_t b = 0; b < G.blocks; ++b) {
+    if (G.blk_node[b] != node) {
+      node = G.blk_node[b];
+      run = 0;
+    }
+    const uint32_t c = G.blk_left[b];
+    G.blk_left[b] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(LBLOCK) void k_large_scatter(Level L, Large G) {
+  __shared__ uint32_t s_wl[LBLOCK / WAVE], s_wr[LBLOCK / WAVE];
+  __shared__ float s_box[LBLOCK / WAVE][24];
+  uint32_t j, a, c0, c1;
+  large_chunk(L, G, blockIdx.x, j, a, c0, c1);
+  const LargeSplit t = large_split(L, G, j, a);
+  const uint32_t begin = t.begin, n_left = t.n_left, end = L.act_range[2 * a + 1];
+  const uint32_t lane = threadIdx.x % WAVE, wv = threadIdx.x / WAVE;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t lbase = G.blk_left[blockIdx.x];
+  uint32_t rbase = (c0 - begin) - lbase;  // right triangles of the blocks before this one
+  float lb[12], rb[12];
+  for (int k = 0; k < 3; ++k) {
+    lb[k] = lb[6 + k] = rb[k] = rb[6 + k] = INFINITY;
+    lb[3 + k] = lb[9 + k] = rb[3 + k] = rb[9 + k] = -INFINITY;
+  }
+  for (uint32_t i0 = c0; i0 < c1; i0 += LBLOCK) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool valid = i < c1;
+    const uint32_t idx = valid ? L.order_in[i] : 0u;
+    const bool left = valid && large_left(L, t, i, idx);
+    const unsigned long long ml = __ballot(valid && left), mr = __ballot(valid && !left);
+    if (lane == 0) {
+      s_wl[wv] = (uint32_t)__popcll(ml);
+      s_wr[wv] = (uint32_t)__popcll(mr);
+    }
+    __syncthreads();
+    uint32_t wl = 0, wr = 0, tl = 0, tr = 0;
+    for (uint32_t w = 0; w < LBLOCK / WAVE; ++w) {
+      if (w < wv) {
+        wl += s_wl[w];
+        wr += s_wr[w];
+      }
+      tl += s_wl[w];
+      tr += s_wr[w];
+    }
+    if (valid) {
+      const float* p = L.prim + (size_t)idx * 9;
+      float* bx = left ? lb : rb;
+      for (int k = 0; k < 3; ++k) {
+        bx[k] = smin(bx[k], p[k]);
+        bx[3 + k] = smax(bx[3 + k], p[3 + k]);
+        bx[6 + k] = smin(bx[6 + k], p[6 + k]);
+        bx[9 + k] = smax(bx[9 + k], p[6 + k]);
+      }
+      const uint32_t dst = left ? begin + lbase + wl + (uint32_t)__popcll(ml & below)
+                                : begin + n_left + rbase + wr + (uint32_t)__popcll(mr & below);
+      if (dst < end) L.order_out[dst] = idx;  // always true: the count pass used the same test
+    }
+    lbase += tl;
+    rbase += tr;
+    __syncthreads();
+  }
+  for (int k = 0; k < 12; ++k) {
+    const bool is_min = (k % 6) < 3;
+    const float l = is_min ? wave_min(lb[k]) : wave_max(lb[k]);
+    const float r = is_min ? wave_min(rb[k]) : wave_max(rb[k]);
+    if (lane == 0) {
+      s_box[wv][k] = l;
+      s_box[wv][12 + k] = r;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 24) {
+    const int k = threadIdx.x;
+    const bool is_min = (k % 6) < 3;
+    int key = is_min ? okey(INFINITY) : okey(-INFINITY);
+    for (uint32_t w = 0; w < LBLOCK / WAVE; ++w) {
+      const int v = okey(s_box[w][k]);
+      key = is_min ? min(key, v) : max(key, v);
+    }
+    if (is_min) atomicMin(&G.cbox[24 * j + k], key);
+    else atomicMax(&G.cbox[24 * j + k], key);
+  }
+}
+
+__global__ void k_large_final(Level L, Large G) {
+  const uint32_t j = blockIdx.x, a = G.act[j];
+  const float* box = L.act_box + 12 * (size_t)a;
+  Node2& nd = L.nodes[L.act_node[a]];
+  for (int k = 0; k < 3; ++k) {
+    nd.lo[k] = box[k];
+    nd.hi[k] = box[3 + k];
+  }
+  nd.first = 0;
+  nd.count = 0;
+  float* cb = L.child_box + 24 * (size_t)a;
+  for (int k = 0; k < 24; ++k) cb[k] = ofloat(G.cbox[24 * j + k]);
+  L.split[a] = 1;
+  L.mid[a] = L.act_range[2 * a] + (uint32_t)G.info[4 * j + 2];
+}
+
 // Numbers the children of the split nodes in active order and writes the
-// next level's active list; counter[0] receives its size.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_next_level(Level L, uint32_t node_base, uint32_t* next_node,
-                                                          uint32_t* next_range, float* next_box,
-                                                          uint32_t* counter) {
+// next level's active list: per-block split counts, their exclusive scan
+// (one block; counter[0] receives the next level's size), then each block
+// writes its nodes' children from its offset.
+constexpr uint32_t NEXT_CHUNK = 1024;  // active nodes per block, at least (a multiple of NEXT_BLOCK)
+constexpr int NEXT_BLOCK = 256;
+constexpr uint32_t NEXT_MAX_BLOCKS = 4096;  // one scan block of SCAN_BLOCK threads, 4 each
+
+__global__ __launch_bounds__(NEXT_BLOCK) void k_next_count(Level L, uint32_t chunk, uint32_t* blk_sum) {
+  __shared__ uint32_t s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  const uint32_t a0 = blockIdx.x * chunk, a1 = min(L.n_active, a0 + chunk);
+  uint32_t c = 0;
+  for (uint32_t a = a0 + threadIdx.x; a < a1; a += NEXT_BLOCK) c += L.split[a];
+  atomicAdd(&s_sum, c);
+  __syncthreads();
+  if (threadIdx.x == 0) blk_sum[blockIdx.x] = s_sum;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_next_scan(uint32_t* blk_sum, uint32_t nb, uint32_t* counter) {
   __shared__ uint32_t s_part[SCAN_BLOCK];
   const uint32_t t = threadIdx.x;
-  const uint32_t per = (L.n_active + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  const uint32_t a0 = t * per, a1 = min(L.n_active, a0 + per);
-  uint32_t c = 0;
-  for (uint32_t a = a0; a < a1; ++a) c += L.split[a];
+  constexpr uint32_t PER = NEXT_MAX_BLOCKS / SCAN_BLOCK;
+  uint32_t v[PER], c = 0;
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint32_t i = t * PER + k;
+    v[k] = i < nb ? blk_sum[i] : 0u;
+    c += v[k];
+  }
   s_part[t] = c;
   __syncthreads();
   for (uint32_t off = 1; off < SCAN_BLOCK; off <<= 1) {
-    const uint32_t v = t >= off ? s_part[t - off] : 0u;
+    const uint32_t x = t >= off ? s_part[t - off] : 0u;
     __syncthreads();
-    s_part[t] += v;
+    s_part[t] += x;
     __syncthreads();
   }
-  uint32_t r = s_part[t] - c;  // exclusive prefix
-  for (uint32_t a = a0; a < a1; ++a) {
-    if (!L.split[a]) continue;
+  uint32_t r = s_part[t] - c;
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint32_t i = t * PER + k;
+    if (i < nb) blk_sum[i] = r;
+    r += v[k];
+  }
+  if (t == SCAN_BLOCK - 1) counter[0] = 2 * s_part[t];
+}
+
+__global__ __launch_bounds__(NEXT_BLOCK) void k_next_write(Level L, uint32_t chunk, const uint32_t* __restrict__ blk_off,
+                                                          uint32_t node_base, uint32_t* next_node,
+                                                          uint32_t* next_range, float* next_box) {
+  const uint32_t PER = chunk / NEXT_BLOCK;  // consecutive active nodes per thread
+  __shared__ uint32_t s_part[NEXT_BLOCK];
+  const uint32_t t = threadIdx.x;
+  const uint32_t a0 = blockIdx.x * chunk + t * PER;
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < PER; ++k)
+    if (a0 + k < L.n_active) c += L.split[a0 + k];
+  s_part[t] = c;
+  __syncthreads();
+  for (uint32_t off = 1; off < NEXT_BLOCK; off <<= 1) {
+    const uint32_t x = t >= off ? s_part[t - off] : 0u;
+    __syncthreads();
+    s_part[t] += x;
+    __syncthreads();
+  }
+  uint32_t r = blk_off[blockIdx.x] + s_part[t] - c;
+  for (uint32_t k = 0; k < PER; ++k) {
+    const uint32_t a = a0 + k;
+    if (a >= L.n_active || !L.split[a]) continue;
     const uint32_t l = node_base + 2 * r;
     Node2& nd = L.nodes[L.act_node[a]];
     nd.left = l;
@@ -331,10 +659,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_next_level(Level L, uint32_t nod
     next_range[4 * r + 2] = m;
     next_range[4 * r + 3] = e;
     const float* cb = L.child_box + 24 * (size_t)a;
-    for (int k = 0; k < 24; ++k) next_box[24 * (size_t)r + k] = cb[k];
+    for (int k2 = 0; k2 < 24; ++k2) next_box[24 * (size_t)r + k2] = cb[k2];
     ++r;
   }
-  if (t == SCAN_BLOCK - 1) counter[0] = 2 * s_part[t];
+}
+
+__global__ void k_gather_tris(const float* __restrict__ tri, const uint32_t* __restrict__ order, uint32_t n,
+                              float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = order[i];
+  const float* t = tri + (size_t)p * 9;
+  out[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], __uint_as_float(p));
+  out[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+  out[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
 }
 
 template <typename T>
@@ -347,6 +685,18 @@ struct DBuf {
 };
 
 }  // namespace
+
+apt_status gather_tris_device(const float* d_tri_xyz, const uint32_t* h_order, uint32_t n, float4* d_tris,
+                              hipStream_t s) {
+  if (n == 0) return APT_OK;
+  DBuf<uint32_t> order;
+  APT_HIP(order.alloc(n));
+  APT_HIP(hipMemcpyAsync(order.p, h_order, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_gather_tris, dim3((n + 255) / 256), dim3(256), 0, s, d_tri_xyz, order.p, n, d_tris);
+  APT_HIP(hipGetLastError());
+  APT_HIP(hipStreamSynchronize(s));
+  return APT_OK;
+}
 
 // Builds the binary tree on the device from device-resident triangles and
 // returns it (nodes, triangle order, SAH cost) like build_bvh2_host.
@@ -369,6 +719,9 @@ apt_status build_bvh2_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
   APT_HIP(mid.alloc(n));
   APT_HIP(child_box.alloc(24 * (size_t)n));
   APT_HIP(counter.alloc(1));
+  DBuf<uint32_t> blk_sum;
+  APT_HIP(blk_sum.alloc(NEXT_MAX_BLOCKS));
+
   APT_HIP(bad.alloc(1));
   APT_HIP(nodes.alloc(cap_nodes));
   APT_HIP(keys.alloc(12));
@@ -390,9 +743,46 @@ apt_status build_bvh2_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
   APT_HIP(hipStreamSynchronize(s));
   APT_REQUIRE(bad_h == 0xFFFFFFFFu, APT_ERR_INVALID_ARG, "triangle %u has a non-finite coordinate", bad_h);
 
+  // multi-block path buffers: at most n / LARGE_NODE large nodes per level
+  const uint32_t max_large = n / LARGE_NODE + 1, max_blocks = n / LARGE_CHUNK + max_large;
+  DBuf<uint32_t> l_act, l_blk_node, l_blk_first, l_blk_left;
+  DBuf<int> l_bins, l_info, l_cbox;
+  APT_HIP(l_act.alloc(max_large));
+  APT_HIP(l_blk_first.alloc(max_large));
+  APT_HIP(l_blk_node.alloc(max_blocks));
+  APT_HIP(l_blk_left.alloc(max_blocks));
+  APT_HIP(l_bins.alloc((size_t)max_large * BIN_WORDS));
+  APT_HIP(l_info.alloc(4 * (size_t)max_large));
+  APT_HIP(l_cbox.alloc(24 * (size_t)max_large));
+  std::vector<uint32_t> h_range, h_act, h_blk_node, h_blk_first;
+
   uint32_t n_active = 1, node_base = 1, depth = 0;
   int cur = 0, ord = 0;
   while (n_active > 0) {
+    // Large nodes of this level (looked up while a level has few nodes: a
+    // node above LARGE_NODE deeper down is split by k_level alone).
+    h_act.clear();
+    h_blk_node.clear();
+    h_blk_first.clear();
+    if (n_active <= 2 * max_large) {
+      h_range.resize(2 * (size_t)n_active);
+      APT_HIP(hipMemcpyAsync(h_range.data(), range[cur].p, h_range.size() * 4, hipMemcpyDeviceToHost, s));
+      APT_HIP(hipStreamSynchronize(s));
+      for (uint32_t a = 0; a < n_active && h_act.size() < max_large; ++a) {
+        const uint32_t len = h_range[2 * a + 1] - h_range[2 * a];
+        if (len <= LARGE_NODE) continue;
+        h_blk_first.push_back((uint32_t)h_blk_node.size());
+        for (uint32_t c = 0; c < (len + LARGE_CHUNK - 1) / LARGE_CHUNK; ++c)
+          h_blk_node.push_back((uint32_t)h_act.size());
+        h_act.push_back(a);
+      }
+    }
+    const uint32_t n_large = (uint32_t)h_act.size(), blocks = (uint32_t)h_blk_node.size();
+    if (n_large) {
+      APT_HIP(hipMemcpyAsync(l_act.p, h_act.data(), n_large * 4, hipMemcpyHostToDevice, s));
+      APT_HIP(hipMemcpyAsync(l_blk_first.p, h_blk_first.data(), n_large * 4, hipMemcpyHostToDevice, s));
+      APT_HIP(hipMemcpyAsync(l_blk_node.p, h_blk_node.data(), blocks * 4, hipMemcpyHostToDevice, s));
+    }
     Level L;
     L.act_node = node_ids[cur].p;
     L.act_range = range[cur].p;
@@ -408,9 +798,34 @@ apt_status build_bvh2_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
     L.split = split.p;
     L.mid = mid.p;
     L.child_box = child_box.p;
+    L.large = n_large ? LARGE_NODE : 0xFFFFFFFFu;
     hipLaunchKernelGGL(k_level, dim3(n_active), dim3(WAVE), 0, s, L);
-    hipLaunchKernelGGL(k_next_level, dim3(1), dim3(SCAN_BLOCK), 0, s, L, node_base, node_ids[cur ^ 1].p,
-                       range[cur ^ 1].p, box[cur ^ 1].p, counter.p);
+    if (n_large) {
+      Large G;
+      G.n = n_large;
+      G.act = l_act.p;
+      G.blk_node = l_blk_node.p;
+      G.blk_first = l_blk_first.p;
+      G.bins = l_bins.p;
+      G.info = l_info.p;
+      G.blk_left = l_blk_left.p;
+      G.cbox = l_cbox.p;
+      G.blocks = blocks;
+      hipLaunchKernelGGL(k_large_init, dim3(n_large), dim3(LBLOCK), 0, s, G);
+      hipLaunchKernelGGL(k_large_bin, dim3(blocks), dim3(LBLOCK), 0, s, L, G);
+      hipLaunchKernelGGL(k_large_sweep, dim3(n_large), dim3(1), 0, s, L, G);
+      hipLaunchKernelGGL(k_large_count, dim3(blocks), dim3(LBLOCK), 0, s, L, G);
+      hipLaunchKernelGGL(k_large_scan, dim3(1), dim3(1), 0, s, G);
+      hipLaunchKernelGGL(k_large_scatter, dim3(blocks), dim3(LBLOCK), 0, s, L, G);
+      hipLaunchKernelGGL(k_large_final, dim3(n_large), dim3(1), 0, s, L, G);
+    }
+    uint32_t chunk = (n_active + NEXT_MAX_BLOCKS - 1) / NEXT_MAX_BLOCKS;
+    chunk = std::max(NEXT_CHUNK, (chunk + NEXT_BLOCK - 1) / NEXT_BLOCK * NEXT_BLOCK);
+    const uint32_t nb_next = (n_active + chunk - 1) / chunk;  // <= NEXT_MAX_BLOCKS
+    hipLaunchKernelGGL(k_next_count, dim3(nb_next), dim3(NEXT_BLOCK), 0, s, L, chunk, blk_sum.p);
+    hipLaunchKernelGGL(k_next_scan, dim3(1), dim3(SCAN_BLOCK), 0, s, blk_sum.p, nb_next, counter.p);
+    hipLaunchKernelGGL(k_next_write, dim3(nb_next), dim3(NEXT_BLOCK), 0, s, L, chunk, blk_sum.p, node_base,
+                       node_ids[cur ^ 1].p, range[cur ^ 1].p, box[cur ^ 1].p);
     uint32_t next = 0;
     APT_HIP(hipMemcpyAsync(&next, counter.p, 4, hipMemcpyDeviceToHost, s));
     APT_HIP(hipStreamSynchronize(s));

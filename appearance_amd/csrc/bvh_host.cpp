@@ -213,6 +213,23 @@ double bvh2_sah_cost(const std::vector<Node2>& nodes) {
 
 namespace {
 
+// Runs f(begin, end) over [0, n) in contiguous chunks on the host's cores.
+template <class F>
+void parallel_for(uint32_t n, F f) {
+  const uint32_t hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  const uint32_t parts = n < 4096 ? 1u : hw;
+  if (parts == 1) {
+    f(0u, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (uint32_t t = 0; t < parts; ++t) {
+    const uint32_t i0 = (uint32_t)((uint64_t)n * t / parts), i1 = (uint32_t)((uint64_t)n * (t + 1) / parts);
+    pool.emplace_back(f, i0, i1);
+  }
+  for (std::thread& th : pool) th.join();
+}
+
 // Grid of one node: per axis the smallest power-of-two step that spans the
 // node box in 255 steps, then each child box rounded outwards to the grid
 // (checked in double, so origin + q * step never cuts into the padded box).
@@ -223,9 +240,13 @@ void quantize_node(WNode<W>& w, int nk, const float (&clo)[W][3], const float (&
   uint8_t* ex[3] = {&w.ex, &w.ey, &w.ez};
   for (int a = 0; a < 3; ++a) {
     const double lo = nk ? (double)nlo[a] : 0.0, ext = nk ? (double)nhi[a] - lo : 0.0;
-    int e = ext > 0.0 ? std::max(-126, std::min(127, (int)std::ceil(std::log2(ext / 255.0)))) : -126;
-    while (e < 127 && ext > 255.0 * std::ldexp(1.0, e)) ++e;
-    while (e > -126 && ext <= 255.0 * std::ldexp(1.0, e - 1)) --e;
+    int e = -126;
+    if (ext > 0.0) {
+      (void)std::frexp(ext / 255.0, &e);  // 2^(e-1) <= ext/255 < 2^e
+      e = std::max(-126, std::min(127, e));
+      while (e > -126 && ext <= 255.0 * std::ldexp(1.0, e - 1)) --e;
+      while (e < 127 && ext > 255.0 * std::ldexp(1.0, e)) ++e;
+    }
     *org[a] = (float)lo;
     *ex[a] = (uint8_t)(e + 127);
     const double step = std::ldexp(1.0, e);
@@ -248,13 +269,14 @@ void quantize_node(WNode<W>& w, int nk, const float (&clo)[W][3], const float (&
 }
 
 template <int W>
-apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad,
-                           apt_bvh* out) {
+apt_status collapse_upload(const float* tri_xyz, const float* d_tri_xyz, uint32_t n, const BvhHostResult& b2,
+                           float pad, apt_bvh* out) {
   const std::vector<Node2>& bn = b2.nodes;
   std::vector<WNode<W>> wide;
   std::vector<uint32_t> queue;      // BVH2 node of each wide node, breadth first
   std::vector<uint32_t> wdepth;     // depth of each wide node
   std::vector<uint32_t> wstack;     // worst-case stack on the way to each wide node
+  std::vector<uint32_t> wkids;      // the W BVH2 nodes under each wide node (~0u: empty slot)
   uint32_t max_depth = 1, max_stack = 1, n_leaves = 0;
 
   auto leaf_ref = [&](const Node2& nd) { return make_leaf_ref(nd.first, nd.count); };
@@ -298,18 +320,6 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
     // Worst-case stack: every sibling but one may be pushed at each level.
     uint32_t here = wstack[qi] + (uint32_t)(nk - 1);
     max_stack = std::max(max_stack, here + 1);
-    // Padded child boxes and the node box that holds them.
-    float clo[W][3], chi[W][3], nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int k = 0; k < nk; ++k) {
-      const Node2& c = bn[kids[k]];
-      for (int a = 0; a < 3; ++a) {
-        clo[k][a] = c.lo[a] - pad;
-        chi[k][a] = c.hi[a] + pad;
-        nlo[a] = std::min(nlo[a], clo[k][a]);
-        nhi[a] = std::max(nhi[a], chi[k][a]);
-      }
-    }
-    quantize_node<W>(w, nk, clo, chi, nlo, nhi);
     for (int k = 0; k < W; ++k) {
       if (k >= nk) {
         w.child[k] = REF_EMPTY;
@@ -328,20 +338,48 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
       }
     }
     wide.push_back(w);
+    for (int k = 0; k < W; ++k) wkids.push_back(k < nk ? kids[k] : 0xFFFFFFFFu);
   }
 
-  // Triangles in leaf order: v0 | prim, v1, v2 as float4.
-  std::vector<float4> tris((size_t)n * 3);
-  for (uint32_t i = 0; i < n; ++i) {
-    uint32_t p = b2.order[i];
-    const float* t = tri_xyz + (uint64_t)p * 9;
-    tris[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], u2f(p));
-    tris[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], 0.0f);
-    tris[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+  // Child grids of every wide node (independent per node: in parallel).
+  parallel_for((uint32_t)wide.size(), [&](uint32_t i0, uint32_t i1) {
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t* kids = &wkids[(size_t)i * W];
+      int nk = 0;
+      while (nk < W && kids[nk] != 0xFFFFFFFFu) ++nk;
+      float clo[W][3], chi[W][3], nlo[3] = {INFINITY, INFINITY, INFINITY};
+      float nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int k = 0; k < nk; ++k) {
+        const Node2& c = bn[kids[k]];
+        for (int a = 0; a < 3; ++a) {
+          clo[k][a] = c.lo[a] - pad;
+          chi[k][a] = c.hi[a] + pad;
+          nlo[a] = std::min(nlo[a], clo[k][a]);
+          nhi[a] = std::max(nhi[a], chi[k][a]);
+        }
+      }
+      quantize_node<W>(wide[i], nk, clo, chi, nlo, nhi);
+    }
+  });
+
+  // Triangles in leaf order: v0 | prim, v1, v2 as float4; gathered on the
+  // device when the caller's triangles are there.
+  std::vector<float4> tris;
+  if (!d_tri_xyz) {
+    tris.resize((size_t)n * 3);
+    parallel_for(n, [&](uint32_t i0, uint32_t i1) {
+      for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t p = b2.order[i];
+        const float* t = tri_xyz + (uint64_t)p * 9;
+        tris[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], u2f(p));
+        tris[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+        tris[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+      }
+    });
   }
 
   size_t node_bytes = std::max<size_t>(1, wide.size()) * sizeof(WNode<W>);
-  size_t tri_bytes = std::max<size_t>(1, tris.size()) * sizeof(float4);
+  size_t tri_bytes = std::max<size_t>(1, (size_t)n * 3) * sizeof(float4);
   void* d_nodes = nullptr;
   void* d_tris = nullptr;
   APT_HIP(hipMalloc(&d_nodes, node_bytes));
@@ -353,6 +391,14 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
   }
   if (!wide.empty()) APT_HIP(hipMemcpy(d_nodes, wide.data(), wide.size() * sizeof(WNode<W>), hipMemcpyHostToDevice));
   if (!tris.empty()) APT_HIP(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+  if (d_tri_xyz) {
+    apt_status gs = gather_tris_device(d_tri_xyz, b2.order.data(), n, static_cast<float4*>(d_tris), lib_stream());
+    if (gs != APT_OK) {
+      (void)hipFree(d_nodes);
+      (void)hipFree(d_tris);
+      return gs;
+    }
+  }
 
   out->width = W;
   out->n_tris = n;
@@ -379,12 +425,12 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
 
 }  // namespace
 
-apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2,
+apt_status upload_wide_bvh(const float* tri_xyz, const float* d_tri_xyz, uint32_t n, const BvhHostResult& b2,
                            uint32_t width, float pad, apt_bvh* out) {
   switch (width) {
-    case 2: return collapse_upload<2>(tri_xyz, n, b2, pad, out);
-    case 4: return collapse_upload<4>(tri_xyz, n, b2, pad, out);
-    case 8: return collapse_upload<8>(tri_xyz, n, b2, pad, out);
+    case 2: return collapse_upload<2>(tri_xyz, d_tri_xyz, n, b2, pad, out);
+    case 4: return collapse_upload<4>(tri_xyz, d_tri_xyz, n, b2, pad, out);
+    case 8: return collapse_upload<8>(tri_xyz, d_tri_xyz, n, b2, pad, out);
     default:
       set_error("BVH width %u is not one of 2, 4, 8", width);
       return APT_ERR_INVALID_ARG;

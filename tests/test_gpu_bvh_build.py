@@ -48,3 +48,52 @@ def test_device_build_rejects_bad_input(gpu_available):
     tris[2, 4] = np.nan
     with pytest.raises(Exception):
         Bvh(tris, builder=BUILDER_GPU)
+
+
+def decode_nodes(raw: np.ndarray, width: int):
+    """Exported wide nodes -> (child refs [n, W], decoded child boxes lo/hi [n, W, 3]) in float64."""
+    from appearance_amd.raytrace import NODE_BYTES
+    nb = NODE_BYTES[width]
+    n = raw.size // nb
+    rec = raw.reshape(n, nb)
+    org = rec[:, 0:12].copy().view(np.float32).astype(np.float64)          # [n, 3]
+    exp = rec[:, 12:15].astype(np.int64) - 127                              # [n, 3]
+    child = rec[:, 16:16 + 4 * width].copy().view(np.uint32)                # [n, W]
+    q = rec[:, 16 + 4 * width:16 + 10 * width].reshape(n, 6, width).astype(np.float64)
+    step = np.ldexp(1.0, exp)                                               # [n, 3]
+    lo = org[:, None, :] + np.transpose(q[:, 0:3, :], (0, 2, 1)) * step[:, None, :]
+    hi = org[:, None, :] + np.transpose(q[:, 3:6, :], (0, 2, 1)) * step[:, None, :]
+    return child, lo, hi
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_quantized_boxes_contain_their_subtrees(gpu_available, width):
+    """Every decoded child box (origin + q * 2^e, exactly) holds all triangles
+    below that child: the quantization rounds outwards."""
+    from tests.test_oracle_bvh import random_tris
+    tris = np.concatenate([random_tris(3000, 21), generate_scene("cornell").positions])
+    b = Bvh(tris, width=width)
+    raw, trec = b.export()
+    child, lo, hi = decode_nodes(raw, width)
+    v = trec.reshape(-1, 3, 4)[:, :, :3].astype(np.float64)   # leaf-order triangles
+    tlo, thi = v.min(axis=1), v.max(axis=1)
+    n = child.shape[0]
+    sub_lo = np.full((n, 3), np.inf)
+    sub_hi = np.full((n, 3), -np.inf)
+    checked = 0
+    for i in range(n - 1, -1, -1):   # breadth-first order: children after parents
+        for k in range(width):
+            r = int(child[i, k])
+            if r == 0xFFFFFFFF:
+                continue
+            if r & 0x80000000:
+                first, cnt = r & 0x0FFFFFFF, ((r >> 28) & 7) + 1
+                clo, chi = tlo[first:first + cnt].min(axis=0), thi[first:first + cnt].max(axis=0)
+            else:
+                assert r > i
+                clo, chi = sub_lo[r], sub_hi[r]
+            assert np.all(lo[i, k] <= clo) and np.all(hi[i, k] >= chi), (i, k)
+            sub_lo[i] = np.minimum(sub_lo[i], clo)
+            sub_hi[i] = np.maximum(sub_hi[i], chi)
+            checked += 1
+    assert checked == n + b.stats()["n_leaves"] - 1
