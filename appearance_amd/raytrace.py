@@ -36,6 +36,21 @@ class Bvh:
               "apt_bvh_build")
         self._h = h
 
+    @classmethod
+    def from_device(cls, d_tri, n_tris: int, width: int = 4, max_leaf: int = 4, bins: int = 32,
+                    builder: int = BUILDER_GPU, lds_stack: int = 8, refill: int = 0) -> "Bvh":
+        """Build from n_tris * 9 floats already in device memory (apt_bvh_build_device);
+        d_tri is a device pointer or an object with data_ptr() (a torch tensor)."""
+        ptr = d_tri.data_ptr() if hasattr(d_tri, "data_ptr") else int(d_tri)
+        self = cls.__new__(cls)
+        self.n_tris = int(n_tris)
+        opts = BvhOpts(width, max_leaf, bins, builder, lds_stack, refill)
+        h = C.c_void_p()
+        check(lib().apt_bvh_build_device(C.c_void_p(ptr), self.n_tris, C.byref(opts), C.byref(h)),
+              "apt_bvh_build_device")
+        self._h = h
+        return self
+
     @property
     def handle(self) -> C.c_void_p:
         return self._h

@@ -87,7 +87,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--lds-stack", type=int, default=8)
-    ap.add_argument("--workload", default="bvh_stress", choices=["bvh_stress", "disney", "restir", "serving"])
+    ap.add_argument("--workload", default="bvh_stress",
+                    choices=["bvh_stress", "disney", "restir", "serving", "bvh_build"])
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--rates", default="20,200")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,6 +107,8 @@ def main() -> int:
         return bench_serving(args)
     if args.workload == "restir":
         return bench_restir(args)
+    if args.workload == "bvh_build":
+        return bench_bvh_build(args)
 
     import numpy as np
     import torch
@@ -199,6 +202,60 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle BVH2 on host cores)")
         out["cpu_baseline"] = cpu_baseline_trace(scene.positions, rays, bpr)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_bvh_build(args) -> int:
+    """One step = one device binned-SAH build of the 1,048,576-triangle
+    bvh_stress soup (triangles resident in HBM) into a 4-wide BVH; the host
+    builder's result is checked byte for byte."""
+    import numpy as np
+    import torch
+
+    from appearance_amd.raytrace import BUILDER_GPU, BUILDER_HOST, Bvh
+    from appearance_amd.scene import generate_scene
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    log("generating bvh_stress scene")
+    scene = generate_scene("bvh_stress", seed=1)
+    n = scene.n_tris
+    d_tri = torch.from_numpy(np.ascontiguousarray(scene.positions, np.float32).reshape(-1)).cuda()
+    for _ in range(args.warmup):
+        Bvh.from_device(d_tri, n, width=args.width, builder=BUILDER_GPU)
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} device builds")
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b = Bvh.from_device(d_tri, n, width=args.width, builder=BUILDER_GPU)
+        ms.append(b.stats()["build_ms"])
+    wall = time.perf_counter() - t0
+    log("host build and byte comparison")
+    h = Bvh(scene.positions, width=args.width, builder=BUILDER_HOST)
+    nh, th = h.export()
+    ng, tg = b.export()
+    same = bool(np.array_equal(nh, ng) and np.array_equal(th.view(np.uint32), tg.view(np.uint32)))
+    out = {
+        "metric": METRIC, "value": n * args.steps / wall / 1e6, "unit": "Mtris/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"bvh_build: {n:,} triangles, device binned SAH (32 bins, leaf <= 4), "
+                               f"{args.width}-wide collapse", "bvh_width": args.width},
+        "device_build_ms": sorted(ms)[len(ms) // 2], "host_build_ms": h.stats()["build_ms"],
+        "bvh_nodes": h.stats()["n_nodes"], "identical_to_host_build": same,
+    }
+    if not args.no_cpu_baseline:
+        from tests._oracle import load_oracle
+        o = load_oracle()
+        t0 = time.time()
+        o.bvh(scene.positions)
+        dt = time.time() - t0
+        out["cpu_baseline"] = {"value": n / dt / 1e6, "unit": "Mtris/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's binned-SAH BVH2 build of all {n:,} triangles, "
+                                         f"one thread, {dt:.1f} s"}
     print(json.dumps(out), flush=True)
     return 0
 
