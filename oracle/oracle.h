@@ -38,6 +38,8 @@ int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t
 void or_f16_round(const float* in, uint32_t n, float* out);
 void or_atrous_probe(uint32_t width, uint32_t height, uint32_t pass, const float* col, const float* var,
                      const float* depth, const float* nrm, float* out);
+int or_post_probe(uint32_t kind, uint32_t width, uint32_t height, uint32_t frames, const float* imgs,
+                  const uint8_t* sky, uint32_t svgf_history, float* out, float* out_var);
 double or_disney_eval(const void* mat, const double* n, const double* wo, const double* wi, int back,
                       double* f);
 double or_disney_sample(const void* mat, const double* n, const double* wo, int back, double r0, double r1,

@@ -22,7 +22,11 @@
 // (single candidate W = 1/p); M caps; SVGF keeps a constant image constant,
 // reproduces a linear ramp inside a flat region, and its first pass equals a
 // direct evaluation of the 5x5 edge-stopping sum; the FP16 conversion equals
-// numpy's.
+// numpy's; the firefly filter clamps impulses to their neighbourhood and
+// leaves in-range pixels and sky alone, the SVGF temporal pass is a running
+// mean up to its history cap and an exponential average after it, and TAA
+// keeps a constant frame and clips an outlying history onto the
+// neighbourhood box (or_post_probe).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -1274,6 +1278,97 @@ void or_atrous_probe(uint32_t width, uint32_t height, uint32_t pass, const float
     out[3 * i + 1] = s.col[dst][i].y;
     out[3 * i + 2] = s.col[dst][i].z;
   }
+}
+
+
+// Post-chain probe on a static camera looking down -z at a flat wall (z = -4)
+// that fills the frame, with `sky` marking pixels that see the sky instead.
+// kind 0: the firefly filter on frame 0.  kind 1: the SVGF temporal pass
+// over `frames` frames, its output fed back as the next frame's history
+// colour (the chain feeds back the first a-trous output, which leaves a flat
+// image unchanged); out_var receives the variance.  kind 2: TAA over
+// `frames` frames (frame 0 passes through, as in the chain), each output the
+// next frame's history.  imgs: frames x height x width x 3; out: the last
+// frame's result.  The probe the post-chain pins evaluate.
+int or_post_probe(uint32_t kind, uint32_t width, uint32_t height, uint32_t frames, const float* imgs,
+                  const uint8_t* sky, uint32_t svgf_history, float* out, float* out_var) {
+  if (frames == 0 || kind > 2) return -1;
+  const uint32_t n = width * height;
+  ors::State s(n);
+  ors::P p{};
+  p.width = width;
+  p.rows = height;
+  p.height = height;
+  p.row_start = 0;
+  p.svgf_history = svgf_history;
+  p.atrous = 1;
+  ors::CamV cv;
+  cv.pos = ors::V3(0, 0, 0);
+  cv.right = ors::V3(1, 0, 0);
+  cv.up = ors::V3(0, 1, 0);
+  cv.back = ors::V3(0, 0, 1);
+  cv.tan_half = 1.0f;
+  cv.aspect = (ors::F)width / (ors::F)height;
+  p.cam = cv;
+  p.prev_cam = cv;
+  // the wall's G-buffer; a frame writes it into gbuf[cur], so frame 0 finds
+  // an empty previous G-buffer (no history), as the chain's first frame does
+  std::vector<ors::Tex> wall(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t x = i % width, y = i / width;
+    const ors::F nx = ((ors::F)x + 0.5f) / (ors::F)width * 2.0f - 1.0f;
+    const ors::F ny = 1.0f - ((ors::F)y + 0.5f) / (ors::F)height * 2.0f;
+    ors::Tex t;
+    t.pos = ors::V3(4.0f * nx * cv.aspect, 4.0f * ny, -4.0f);
+    t.depth32 = (sky && sky[i]) ? 0.0f : 4.0f;
+    t.depth = t.depth32;
+    t.n = ors::V3(0, 0, 1);
+    t.albedo = ors::V3(1, 1, 1);
+    t.matbits = 0;
+    wall[i] = t;
+    ors::Tex z;
+    z.pos = ors::V3(0, 0, 0);
+    z.depth32 = 0.0f;
+    z.depth = 0.0f;
+    z.n = ors::V3(0, 0, 0);
+    z.albedo = ors::V3(0, 0, 0);
+    z.matbits = 0;
+    s.gbuf[0][i] = z;
+    s.gbuf[1][i] = z;
+  }
+  std::atomic<uint64_t> cnt[2];
+  cnt[0] = 0;
+  cnt[1] = 0;
+  ors::Scene sc;
+  ors::Ctx c{sc, s, p, 1, cnt};
+  std::vector<ors::V3> in(n), res(n), prev(n);
+  for (uint32_t f = 0; f < frames; ++f) {
+    p.cur = f & 1u;
+    p.first_frame = f == 0 ? 1u : 0u;
+    s.gbuf[p.cur] = wall;
+    const float* im = imgs + (size_t)f * n * 3;
+    for (uint32_t i = 0; i < n; ++i) in[i] = ors::V3(im[3 * i], im[3 * i + 1], im[3 * i + 2]);
+    if (kind == 0) {
+      ors::pass_firefly(c, in, res);
+      break;
+    }
+    if (kind == 1) {
+      ors::pass_svgf_temporal(c, in);
+      res = s.col[0];
+      s.hist = s.col[0];
+      continue;
+    }
+    if (f == 0) res = in;
+    else ors::pass_taa(c, in, prev, res);
+    prev = res;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    out[3 * i] = res[i].x;
+    out[3 * i + 1] = res[i].y;
+    out[3 * i + 2] = res[i].z;
+    if (out_var) out_var[i] = kind == 1 ? s.var[0][i] : 0.0f;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -243,3 +243,20 @@ def atrous_probe(o: Oracle, col, var, depth, nrm, pass_idx: int) -> np.ndarray:
     fn.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P]
     fn(w, h, pass_idx, _vp(col), _vp(var), _vp(depth), _vp(nrm), _vp(out))
     return out
+
+
+def post_probe(o: Oracle, kind: int, imgs, sky=None, svgf_history: int = 4):
+    """kind 0 firefly (frame 0), 1 SVGF temporal, 2 TAA over the frames of imgs
+    (frames, h, w, 3) on a static camera facing a flat wall; returns (colour,
+    variance) of the last frame."""
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    f, h, w, _ = imgs.shape
+    skyb = None if sky is None else np.ascontiguousarray(sky, np.uint8)
+    out = np.empty((h, w, 3), np.float32)
+    var = np.empty((h, w), np.float32)
+    fn = o.lib.or_post_probe
+    fn.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, C.c_uint32, _P, _P]
+    fn.restype = C.c_int
+    rc = fn(kind, w, h, f, _vp(imgs), _vp(skyb) if skyb is not None else None, svgf_history, _vp(out), _vp(var))
+    assert rc == 0
+    return out, var

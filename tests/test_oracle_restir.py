@@ -143,3 +143,143 @@ def test_atrous_edge_stopping(oracle):
     wn = c ** 128
     tot = 1.0 + 55.0 / 64.0 - (1 / 4 * 3 / 8) * (1 - wn)
     np.testing.assert_allclose(out[8, 8, 0], (1 / 4 * 3 / 8) * wn / tot, rtol=1e-4)
+
+
+# ---- firefly filter, SVGF temporal pass and TAA: closed-form pins -------------
+# The oracle's restatements of firefly_filter.wgsl, svgf_temporal.wgsl and
+# taa.wgsl run through or_post_probe on a static camera facing a flat wall.
+
+def _luma(c):
+    return 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
+
+
+def test_firefly_constant_image_is_fixed(oracle):
+    from tests._oracle import post_probe
+    img = np.broadcast_to(np.array([0.3, 0.5, 0.2], np.float32), (1, 9, 11, 3)).copy()
+    out, _ = post_probe(oracle, 0, img)
+    np.testing.assert_array_equal(out, img[0])
+
+
+def test_firefly_clamps_an_impulse_to_its_neighbourhood(oracle):
+    from tests._oracle import post_probe
+    base = np.array([0.3, 0.5, 0.2], np.float32)
+    img = np.broadcast_to(base, (1, 9, 11, 3)).copy()
+    img[0, 4, 5] = [40.0, 35.0, 50.0]      # a firefly: replaced by the neighbourhood's brightest
+    img[0, 2, 2] = [0.0, 0.001, 0.0]       # a dark outlier: replaced by its darkest neighbour
+    out, _ = post_probe(oracle, 0, img)
+    np.testing.assert_array_equal(out[4, 5], base)
+    np.testing.assert_array_equal(out[2, 2], base)
+    mask = np.ones((9, 11), bool)
+    mask[4, 5] = mask[2, 2] = False
+    np.testing.assert_array_equal(out[mask], img[0][mask])  # in range of their neighbours: untouched
+
+
+def test_firefly_ignores_sky(oracle):
+    from tests._oracle import post_probe
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 5, (1, 7, 8, 3)).astype(np.float32)
+    sky = np.zeros((7, 8), np.uint8)
+    sky[:, :4] = 1
+    img[0, 3, 6] = 1e4                      # surrounded by sky on one side only
+    out, _ = post_probe(oracle, 0, img, sky=sky)
+    np.testing.assert_array_equal(out[:, :4], img[0][:, :4])   # sky passes through
+    sky2 = np.ones((7, 8), np.uint8)
+    sky2[3, 3] = 0                          # a surface pixel whose neighbours are all sky
+    out2, _ = post_probe(oracle, 0, img, sky=sky2)
+    np.testing.assert_array_equal(out2[3, 3], img[0, 3, 3])
+
+
+def test_svgf_temporal_first_frame_passes_through(oracle):
+    from tests._oracle import post_probe
+    rng = np.random.default_rng(4)
+    img = rng.uniform(0, 2, (1, 6, 9, 3)).astype(np.float32)
+    out, var = post_probe(oracle, 1, img)
+    np.testing.assert_array_equal(out, img[0])
+    assert np.all(var == 0.0)
+
+
+@pytest.mark.parametrize("cap", [4, 24])
+def test_svgf_temporal_is_a_running_mean_then_an_ema(oracle, cap):
+    """Frame-constant colours c_0..c_{F-1} on a static view: the history
+    length k grows 0, 1, .. up to the cap and alpha = 1 / (1 + k), so the
+    output is the mean of the frames seen while k <= cap and an exponential
+    average with weight 1 / (1 + cap) after that; the variance is that of the
+    luma moments."""
+    from tests._oracle import post_probe
+    rng = np.random.default_rng(5)
+    frames = 12
+    cols = rng.uniform(0.1, 3.0, (frames, 3))
+    imgs = np.broadcast_to(cols[:, None, None, :], (frames, 8, 10, 3)).astype(np.float32)
+    m = np.zeros(3)
+    m1 = m2 = 0.0
+    for f in range(frames):
+        a = 1.0 / (1.0 + min(f, cap))
+        c = imgs[f, 0, 0].astype(np.float64)
+        m = m + (c - m) * a
+        lum = _luma(c)
+        m1 = m1 + (lum - m1) * a
+        m2 = m2 + (lum * lum - m2) * a
+        out, var = post_probe(oracle, 1, imgs[:f + 1], svgf_history=cap)
+        inner = out[1:-1, 1:-1]
+        np.testing.assert_allclose(inner, np.broadcast_to(m, inner.shape), rtol=2e-6)
+        np.testing.assert_allclose(var[1:-1, 1:-1], max(m2 - m1 * m1, 0.0), rtol=1e-3, atol=1e-6)
+        if f == frames - 1 and cap >= frames:
+            np.testing.assert_allclose(m, imgs[:, 0, 0].astype(np.float64).mean(axis=0), rtol=1e-12)
+
+
+def test_svgf_temporal_sky_has_no_history(oracle):
+    from tests._oracle import post_probe
+    sky = np.zeros((6, 7), np.uint8)
+    sky[2:4, 2:5] = 1
+    imgs = np.stack([np.full((6, 7, 3), v, np.float32) for v in (1.0, 3.0, 5.0)])
+    out, var = post_probe(oracle, 1, imgs, sky=sky)
+    np.testing.assert_array_equal(out[2:4, 2:5], imgs[-1][2:4, 2:5])
+    assert np.all(var[2:4, 2:5] == 0.0)
+
+
+def test_taa_first_frame_passes_through_and_constant_frames_are_fixed(oracle):
+    from tests._oracle import post_probe
+    rng = np.random.default_rng(6)
+    img = rng.uniform(0, 2, (1, 6, 9, 3)).astype(np.float32)
+    out, _ = post_probe(oracle, 2, img)
+    np.testing.assert_array_equal(out, img[0])
+    # a constant frame after any history: the neighbourhood box has zero
+    # extent, so the history clips to its mean and the output is the frame
+    # (up to the box's FP32 width: the variance sum(x^2) - sum(x)^2 / n of
+    # equal values cancels to a few ulps, whose square root is ~3e-4 |c|)
+    c = np.array([0.7, 0.2, 1.3], np.float32)
+    imgs = np.concatenate([img, np.broadcast_to(c, (1, 6, 9, 3))])
+    out, _ = post_probe(oracle, 2, imgs)
+    np.testing.assert_allclose(out[1:-1, 1:-1], np.broadcast_to(c, (4, 7, 3)), rtol=2e-3)
+    # ... and with the same constant as history the box test leaves it alone
+    imgs = np.broadcast_to(c, (3, 6, 9, 3)).copy()
+    out, _ = post_probe(oracle, 2, imgs)
+    np.testing.assert_allclose(out, imgs[-1], rtol=1e-6)
+
+
+def test_taa_clips_history_to_the_neighbourhood_box(oracle):
+    """One interior pixel's 3x3 neighbourhood with known mean and deviation:
+    the history (a constant far outside the box) is clipped onto the box
+    surface along the line to the box centre, and blended with the current
+    pixel by luma-weighted factors 0.2 and 0.8 (Mitchell-Netravali weights
+    vanish at the neighbours: the filter is evaluated at 2 |dx| = 2, its
+    support's edge, so the reconstruction is the centre pixel itself)."""
+    from tests._oracle import post_probe
+    rng = np.random.default_rng(7)
+    cur = rng.uniform(0.2, 1.0, (7, 7, 3)).astype(np.float32)
+    hist_c = np.array([5.0, -3.0, 4.0], np.float32)
+    imgs = np.stack([np.broadcast_to(hist_c, (7, 7, 3)), cur])
+    out, _ = post_probe(oracle, 2, imgs)
+    y, x = 3, 3
+    nb = cur[y - 1:y + 2, x - 1:x + 2].reshape(9, 3).astype(np.float64)
+    mean = nb.mean(axis=0)
+    sd = nb.std(axis=0, ddof=1)
+    r = hist_c - mean
+    k = np.max(np.abs(r / sd))
+    clipped = mean + r / k if k > 1 else hist_c
+    assert np.isclose(np.max(np.abs((clipped - mean) / sd)), 1.0)   # on the box surface
+    rec = cur[y, x].astype(np.float64)
+    cw = 0.2 / (1.0 + _luma(rec))
+    hw = 0.8 / (1.0 + _luma(clipped))
+    expect = (rec * cw + clipped * hw) / (cw + hw)
+    np.testing.assert_allclose(out[y, x], expect, rtol=2e-5)
