@@ -319,16 +319,17 @@ __device__ __forceinline__ bool res_update(float& wsum, float& M, float w, uint3
 __device__ __forceinline__ DiRes di_empty() { return DiRes{LIGHT_EMPTY, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}; }
 
 // The two updates of DiReservoir::combine / combine_unbiased (:99-133);
-// `second` tells whether r2's sample was kept.
-__device__ __forceinline__ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng, bool& second) {
+// `kept` tells whose sample the result holds: 0 none (empty), 1 r1's, 2 r2's.
+__device__ __forceinline__ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng, int& kept) {
   DiRes c = di_empty();
-  second = false;
+  kept = 0;
   if (res_update(c.wsum, c.M, r1.phat * r1.W * r1.M, rng)) {
     c.light = r1.light; c.u = r1.u; c.v = r1.v; c.phat = r1.phat;
+    kept = 1;
   }
   if (res_update(c.wsum, c.M, r2.phat * r2.W * r2.M, rng)) {
     c.light = r2.light; c.u = r2.u; c.v = r2.v; c.phat = r2.phat;
-    second = true;
+    kept = 2;
   }
   c.M = r1.M + r2.M;
   return c;
@@ -626,8 +627,8 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_temporal(Restir
       pr.M = fminf(pr.M, (float)p.di_mcap * r.M);
       const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
       pr.phat = di_phat<W, S>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
-      bool second;
-      DiRes c = di_merge(r, pr, rng, second);
+      int kept;
+      DiRes c = di_merge(r, pr, rng, kept);
       if (c.phat > RESTIR_DI_EPSILON && c.M * c.wsum > RESTIR_DI_EPSILON)
         c.W = (1.0f / c.phat) * (1.0f / c.M * c.wsum);
       r = c;
@@ -680,6 +681,12 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
   const uint32_t aseed = hash_combine(x >> 2, hash_combine(y >> 2, pass_seed));
   const float a_off = (float)aseed * (1.0f / (float)0xFFFFFFFFu) * TWO_PI;
   const uint32_t n = p.neighbours;
+  // Whether r's sample has a visibility-tested target > 0 at this pixel:
+  // -1 not known yet, else 0 / 1.  It is a function of the sample alone, so
+  // it carries over while r keeps its sample, and a kept neighbour sample
+  // brings its own (nr.phat below): each shadow ray at this pixel is traced
+  // once.
+  int r_vis = -1;
   for (uint32_t i = 0; i < n; ++i) {
     const float ang = (float)i * GOLDEN_ANGLE + a_off;
     const float rad = powf((float)i / (float)n, 0.5f) * radius + r_off;
@@ -699,14 +706,17 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
       valid = nr.phat > 0.0f;
       // DiReservoir::combine_unbiased: Z counts the M of the domains where the
       // chosen sample has a (visibility-tested) target > 0
-      bool second;
-      DiRes c = di_merge(r, nr, rng, second);
+      int kept;
+      DiRes c = di_merge(r, nr, rng, kept);
       float z = 0.0f;
-      // When the neighbour's sample was kept, its visibility-tested target at
-      // this pixel is nr.phat, computed just above from the same inputs: the
-      // shadow ray is not traced twice.
-      const float here = second ? nr.phat : di_phat<W, S>(sc, st, v, bs, c.light, c.u, c.v, true, b.counters);
-      if (here > 0.0f) z += r.M;
+      int c_vis = 0;  // the merged (empty) reservoir's sample has no target
+      if (kept == 2) {
+        c_vis = nr.phat > 0.0f ? 1 : 0;
+      } else if (kept == 1) {
+        if (r_vis < 0) r_vis = di_phat<W, S>(sc, st, v, bs, r.light, r.u, r.v, true, b.counters) > 0.0f ? 1 : 0;
+        c_vis = r_vis;
+      }
+      if (c_vis) z += r.M;
       Vtx vn;
       if (load_vtx(b, q, vn)) {
         vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
@@ -715,6 +725,7 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
       }
       if (c.phat > RESTIR_DI_EPSILON && z * c.wsum > RESTIR_DI_EPSILON) c.W = (1.0f / c.phat) * (1.0f / z * c.wsum);
       r = c;
+      r_vis = c_vis;
     }
     if (!valid) radius = fmaxf(radius * 0.5f, 3.0f);
   }

@@ -46,16 +46,6 @@ struct LdsStack {
   int stride;
 };
 
-template <int W>
-__device__ __forceinline__ void load_node(const WNode<W>* __restrict__ nodes, uint32_t idx,
-                                          WNode<W>& nd) {
-  const float4* src = reinterpret_cast<const float4*>(nodes + idx);
-  float4* dst = reinterpret_cast<float4*>(&nd);
-  constexpr int N4 = (int)(sizeof(WNode<W>) / 16);
-#pragma unroll
-  for (int k = 0; k < N4; ++k) dst[k] = src[k];
-}
-
 // Entry distance of each child box, +inf for a missed or empty child.  A
 // grid plane origin + q * step lies at distance q * (step * inv) + (origin -
 // o) * inv along the ray: one FMA per plane after three multiplies per node
@@ -108,15 +98,6 @@ __device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&r)[W]) {
     cswap<W>(d, r, 1, 4); cswap<W>(d, r, 3, 6);
     cswap<W>(d, r, 1, 2); cswap<W>(d, r, 3, 4); cswap<W>(d, r, 5, 6);
   }
-}
-
-__device__ __forceinline__ bool leaf_tri(const float4* __restrict__ tris, uint32_t j,
-                                         const TraceRay& ray, TriHit& h, uint32_t& prim) {
-  const float4 a = tris[3 * (size_t)j + 0];
-  const float4 b = tris[3 * (size_t)j + 1];
-  const float4 c = tris[3 * (size_t)j + 2];
-  prim = f2u(a.w);
-  return intersect_tri(ray.pre, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z), h);
 }
 
 // Push/pop on a lane's stack: entries below S live in LDS at lds[e * stride],
