@@ -58,7 +58,8 @@ struct DevBuf {
 // Per-pixel state of the ReSTIR integrator for one slab.
 struct RestirState {
   DevBuf<float4> ray_o, ray_d, thr, radiance, vtx_p, vtx_n, vtx_t, demod[2], work, col[2], hist;
-  DevBuf<uint32_t> rng, count, counters;
+  DevBuf<float4> rq_o, rq_d, rq_hit;
+  DevBuf<uint32_t> rng, count, counters, rq_pix, rq_ctr;
   DevBuf<DiRes> di, di_tmp, di_hist[2];
   DevBuf<GiRes> gi, gi_hist[2];
   DevBuf<GTexel> gbuf[2];
@@ -71,7 +72,7 @@ struct RestirState {
     f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(demod[0]); f(demod[1]);
     f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
     f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
-    f(var[0]); f(var[1]);
+    f(var[0]); f(var[1]); f(rq_o); f(rq_d); f(rq_hit); f(rq_pix);
   }
   // Sizes every buffer for n pixels and clears them (no history).
   apt_status reset(size_t n, hipStream_t s) {
@@ -84,12 +85,14 @@ struct RestirState {
       }
     });
     if (st == APT_OK) st = counters.ensure(2);
+    if (st == APT_OK) st = rq_ctr.ensure(RQ_WORDS);
     npix = st == APT_OK ? n : 0;
     return st;
   }
   void release() {
     each([](auto& b) { b.release(); });
     counters.release();
+    rq_ctr.release();
     npix = 0;
   }
   RestirBuffers view() {
@@ -103,6 +106,7 @@ struct RestirState {
     b.svgf_col[0] = col[0].p; b.svgf_col[1] = col[1].p; b.svgf_mom[0] = mom[0].p; b.svgf_mom[1] = mom[1].p;
     b.svgf_var[0] = var[0].p; b.svgf_var[1] = var[1].p; b.svgf_count = count.p; b.svgf_hist = hist.p;
     b.counters = counters.p;
+    b.rq_o = rq_o.p; b.rq_d = rq_d.p; b.rq_pix = rq_pix.p; b.rq_hit = rq_hit.p; b.rq_ctr = rq_ctr.p;
     return b;
   }
 };

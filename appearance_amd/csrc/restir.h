@@ -90,7 +90,19 @@ struct RestirBuffers {
   uint32_t* svgf_count;
   float4* svgf_hist;    // a-trous pass-0 output kept for the next frame
   uint32_t* counters;   // [0] closest-hit rays, [1] shadow rays traced (running totals)
+  // ray queue of one tracing stage (traced by the persistent pool kernel):
+  // origin, direction | tmax, pixel per queued ray; the result per pixel
+  // (t, u, v, prim bits for a closest hit; w = 1 when occluded for any hit)
+  float4* rq_o;
+  float4* rq_d;
+  uint32_t* rq_pix;
+  float4* rq_hit;
+  uint32_t* rq_ctr;     // [RQ_COUNT] queued rays, [RQ_POOL ..] the pool's counters
 };
+
+constexpr uint32_t RQ_COUNT = 0;
+constexpr uint32_t RQ_POOL = 16;
+constexpr uint32_t RQ_WORDS = RQ_POOL + POOL_WORDS;
 
 // One sample of the ReSTIR integrator over the slab: camera rays, bounces with
 // DI/GI resampling and reuse at the first vertex; radiance accumulates into

@@ -412,9 +412,74 @@ __global__ __launch_bounds__(BLOCK) void k_rs_raygen(RestirParams p, SceneView s
 // two waves, 15% slower end to end on the 1080p ReSTIR frame.
 constexpr int RS_TRACE_WAVES = 4;
 
-template <int W, int S>
+// ---- ray queues ------------------------------------------------------------
+// The stages whose rays dominate the frame (the path's extension ray, the BSDF
+// candidate of the DI RIS and the visibility of its winner) queue their rays
+// with wave-level compaction, the persistent pool kernel traces the queue, and
+// the next kernel reads each pixel's result.  The traversal is the one every
+// other stage runs, so the results equal tracing in place.
+
+__global__ void k_rq_reset(RestirBuffers b) {
+  if (threadIdx.x == 0) b.rq_ctr[RQ_COUNT] = 0u;
+  pool_zero(b.rq_ctr + RQ_POOL, threadIdx.x);
+}
+
+// Queues a ray for pixel `pix` when `want` (every lane of the wave calls it).
+__device__ __forceinline__ void rq_push(const RestirBuffers& b, bool want, uint32_t pix, v3 o, v3 d, float tmax) {
+  const int slot = wave_append(want, &b.rq_ctr[RQ_COUNT]);
+  if (slot >= 0) {
+    b.rq_o[slot] = v4(o, 0.0f);
+    b.rq_d[slot] = v4(d, tmax);
+    b.rq_pix[slot] = pix;
+  }
+}
+
+template <int W>
+struct RqSrc {
+  static constexpr bool kOneScene = true;
+  const WNode<W>* nodes;
+  const float4* tris;
+  uint32_t root;
+  RestirBuffers b;
+  __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray) const {
+    const float4 o = b.rq_o[i], d = b.rq_d[i];
+    ray = make_trace_ray(f4v(o), f4v(d), 0.0f, d.w);
+  }
+  __device__ __forceinline__ void closest(uint32_t i, const HitRec& h) const {
+    b.rq_hit[b.rq_pix[i]] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+  }
+  __device__ __forceinline__ void any(uint32_t i, bool hit) const {
+    b.rq_hit[b.rq_pix[i]] = make_float4(0.0f, 0.0f, 0.0f, hit ? 1.0f : 0.0f);
+  }
+};
+
+template <int W, int S, bool ANY>
+__global__ __launch_bounds__(BLOCK, 8) void k_rs_trace(SceneView sc, RestirBuffers b) {
+  __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
+  __shared__ float4 s_top[TOP_BYTES / 16];
+  const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
+  const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
+  const uint32_t n = b.rq_ctr[RQ_COUNT];
+  RqSrc<W> src{nodes, sc.bvh_tris, sc.root, b};
+  trace_pool<W, S, ANY>(src, n, b.rq_ctr + RQ_POOL, min(sc.refill, 64u), s_stack + threadIdx.x, BLOCK, s_top,
+                        ktop);
+}
+
+// Extension rays of the live paths (safe_origin along the ray, trace.wgsl:91-98).
+__global__ __launch_bounds__(BLOCK) void k_rs_extend_ray(RestirParams p, RestirBuffers b) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  const bool live = b.thr[pix].w > 0.0f;
+  v3 o = mk(0.0f, 0.0f, 0.0f), d = o;
+  if (live) {
+    d = f4v(b.ray_d[pix]);
+    o = f4v(b.ray_o[pix]) + d * TRACE_EPSILON;
+  }
+  count_rays(live, b.counters);
+  rq_push(b, live, pix, o, d, 1e30f);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
-  __shared__ uint2 s_stack[S * BLOCK];
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   const float4 t4 = b.thr[pix];
@@ -422,11 +487,15 @@ __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView s
     b.vtx_p[pix].w = -1.0f;
     return;
   }
-  const LdsStack st{s_stack + threadIdx.x, BLOCK};
   const v3 T = f4v(t4);
   const v3 d = f4v(b.ray_d[pix]);
   const v3 o = f4v(b.ray_o[pix]) + d * TRACE_EPSILON;  // safe_origin along the ray (trace.wgsl:91-98)
-  const HitRec h = closest<W, S>(sc, st, o, d, b.counters);
+  const float4 hq = b.rq_hit[pix];                     // k_rs_extend_ray -> k_rs_trace
+  HitRec h;
+  h.t = hq.x;
+  h.u = hq.y;
+  h.v = hq.z;
+  h.prim = f2u(hq.w);
   const float4 L = b.radiance[pix];
   if (h.prim == INVALID) {
     // sky without the sun disk, which the light samples cover (trace.wgsl:237-243)
@@ -470,14 +539,11 @@ __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView s
 // one BSDF candidate.  MIS is the balance heuristic over each candidate's own
 // densities under both strategies (the reference mixes the densities of two
 // different candidates), so W = wsum / p^ is unbiased.
-template <int W, int S>
 __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_ris(RestirParams p, SceneView sc, RestirBuffers b) {
-  __shared__ uint2 s_stack[S * BLOCK];
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
   if (!load_vtx(b, pix, v)) return;
-  const LdsStack st{s_stack + threadIdx.x, BLOCK};
   const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
   uint32_t rng = b.rng[pix];
   const float p_sun = sun_pick(sc);
@@ -510,7 +576,11 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_ris(RestirParam
       b_pb = s.pdf;
       const v3 contrib = s.f * fabsf(dot(s.wi, v.n));
       if (s.pdf > 0.0f && dot(contrib, contrib) > 0.0f) {
-        const HitRec h = closest<W, S>(sc, st, v.x + v.n * TRACE_EPSILON, s.wi, b.counters);
+        const float4 hq = b.rq_hit[pix];  // the ray k_rs_di_bsdf_ray queued for this sample
+        HitRec h;
+        h.u = hq.y;
+        h.v = hq.z;
+        h.prim = f2u(hq.w);
         if (h.prim != INVALID) {
           if (has_emission(sc.mats[sc.tri_mat[h.prim]])) {
             b_light = h.prim;
@@ -541,14 +611,51 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_ris(RestirParam
       }
     }
   }
+  // W = wsum / phat once the winner's shadow ray (queued here) finds it
+  // visible: k_rs_di_vis
   r.W = 0.0f;
-  if (r.phat > RESTIR_DI_EPSILON) {
-    LightHit e;
-    if (eval_light(sc, r.light, r.u, r.v, v.x, e) && visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, b.counters))
-      r.W = (1.0f / r.phat) * r.wsum;
-  }
+  LightHit e{};
+  const bool shoot = r.phat > RESTIR_DI_EPSILON && eval_light(sc, r.light, r.u, r.v, v.x, e);
   b.di[pix] = r;
   b.rng[pix] = rng;
+  count_rays(shoot, b.counters + 1);
+  rq_push(b, shoot, pix, v.x + v.n * TRACE_EPSILON, e.wi, e.dist - 2.0f * TRACE_EPSILON);
+}
+
+// The BSDF candidate of k_rs_di_ris: the same draws (after the first area
+// candidate's five), the same sample, its closest-hit ray queued.
+__global__ __launch_bounds__(BLOCK) void k_rs_di_bsdf_ray(RestirParams p, SceneView sc, RestirBuffers b) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  const bool have = load_vtx(b, pix, v);
+  bool want = false;
+  v3 wi = mk(0.0f, 0.0f, 0.0f);
+  if (have) {
+    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    uint32_t rng = b.rng[pix];
+    if (p.di_candidates > 0u)
+      for (int k = 0; k < 5; ++k) rng_next(rng);
+    const float r0 = rng_next(rng), r1 = rng_next(rng), r2 = rng_next(rng);
+    const BsdfSample s = bsdf_sample(bs, v.wo, r0, r1, r2);
+    const v3 contrib = s.f * fabsf(dot(s.wi, v.n));
+    want = s.pdf > 0.0f && dot(contrib, contrib) > 0.0f;
+    wi = s.wi;
+  }
+  count_rays(want, b.counters);
+  rq_push(b, want, pix, have ? v.x + v.n * TRACE_EPSILON : wi, wi, 1e30f);
+}
+
+// trace_shadow_ray's verdict on the queued winners: visible ones get
+// W = wsum / phat (nee.wgsl:226-235).
+__global__ __launch_bounds__(BLOCK) void k_rs_di_vis(RestirBuffers b) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= b.rq_ctr[RQ_COUNT]) return;
+  const uint32_t pix = b.rq_pix[i];
+  if (b.rq_hit[pix].w != 0.0f) return;
+  DiRes r = b.di[pix];
+  r.W = (1.0f / r.phat) * r.wsum;
+  b.di[pix] = r;
 }
 
 // Roulette and InlinePathTracer::sample_ris (inline_path_tracer.wgsl:258-296):
@@ -1087,8 +1194,18 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
   hipLaunchKernelGGL(k_rs_raygen, grid, block, 0, s, p, sc, b);
   for (uint32_t bounce = 0; bounce < p.max_bounces; ++bounce) {
     const bool more = bounce + 1 < p.max_bounces;
-    hipLaunchKernelGGL((k_rs_extend<W, S>), grid, block, 0, s, p, sc, b, bounce);
-    hipLaunchKernelGGL((k_rs_di_ris<W, S>), grid, block, 0, s, p, sc, b);
+    const dim3 pool(pool_blocks());
+    hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_rs_extend_ray, grid, block, 0, s, p, b);
+    hipLaunchKernelGGL((k_rs_trace<W, S, false>), pool, block, 0, s, sc, b);
+    hipLaunchKernelGGL(k_rs_extend, grid, block, 0, s, p, sc, b, bounce);
+    hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_rs_di_bsdf_ray, grid, block, 0, s, p, sc, b);
+    hipLaunchKernelGGL((k_rs_trace<W, S, false>), pool, block, 0, s, sc, b);
+    hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_rs_di_ris, grid, block, 0, s, p, sc, b);
+    hipLaunchKernelGGL((k_rs_trace<W, S, true>), pool, block, 0, s, sc, b);
+    hipLaunchKernelGGL(k_rs_di_vis, dim3((p.width * p.rows + BLOCK - 1) / BLOCK), block, 0, s, b);
     if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S>), grid, block, 0, s, p, sc, b, bounce);
     if (bounce == 0 && p.restir_di) {
       hipLaunchKernelGGL((k_rs_di_temporal<W, S>), grid, block, 0, s, p, sc, b);
