@@ -85,9 +85,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, Pat
     p = ps.queue[qin][i];
     shade_path(sc, ps, p, bounce, fv.max_bounces, fv.rr_start, alive, want_shadow);
   }
-  const int slot = wave_append(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)]);
+  int slot, sslot;
+  block_append2<BLOCK>(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)], slot, want_shadow, &ps.counters[CNT_SHADOW],
+                       sslot);
   if (slot >= 0) ps.queue[qin ^ 1][slot] = p;
-  const int sslot = wave_append(want_shadow, &ps.counters[CNT_SHADOW]);
   if (sslot >= 0) ps.shadow_queue[sslot] = p;
 }
 

@@ -26,6 +26,42 @@ __device__ __forceinline__ int wave_append(bool want, uint32_t* counter) {
   return want ? (int)(base + below) : -1;
 }
 
+// Work-group-level compaction into two queues at once: one atomic per queue
+// per work-group instead of one per wave (a single counter that every wave of
+// a launch hits serialises those atomics).  Every thread of the work-group
+// must call it; returns the slots (or -1) through sa / sb.
+template <int BLOCK>
+__device__ __forceinline__ void block_append2(bool a, uint32_t* ca, int& sa, bool b, uint32_t* cb, int& sb) {
+  constexpr int NW = BLOCK / 64;
+  __shared__ uint32_t s_n[2][NW];
+  __shared__ uint32_t s_base[2];
+  const uint32_t lane = __lane_id(), w = threadIdx.x / 64;
+  const unsigned long long ma = __ballot(a), mb = __ballot(b);
+  if (lane == 0) {
+    s_n[0][w] = (uint32_t)__popcll(ma);
+    s_n[1][w] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ta = 0, tb = 0;
+    for (int k = 0; k < NW; ++k) {
+      ta += s_n[0][k];
+      tb += s_n[1][k];
+    }
+    s_base[0] = ta ? atomicAdd(ca, ta) : 0u;
+    s_base[1] = tb ? atomicAdd(cb, tb) : 0u;
+  }
+  __syncthreads();
+  uint32_t oa = s_base[0], ob = s_base[1];
+  for (uint32_t k = 0; k < w; ++k) {
+    oa += s_n[0][k];
+    ob += s_n[1][k];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  sa = a ? (int)(oa + (uint32_t)__popcll(ma & below)) : -1;
+  sb = b ? (int)(ob + (uint32_t)__popcll(mb & below)) : -1;
+}
+
 __device__ __forceinline__ v3 ld3(const float4& a) { return mk(a.x, a.y, a.z); }
 
 struct TriGeom {

@@ -59,7 +59,7 @@ struct DevBuf {
 struct RestirState {
   DevBuf<float4> ray_o, ray_d, thr, radiance, vtx_p, vtx_n, vtx_t, demod[2], work, col[2], hist;
   DevBuf<float4> rq_o, rq_d, rq_hit;
-  DevBuf<uint32_t> rng, count, counters, rq_pix, rq_ctr;
+  DevBuf<uint32_t> rng, count, counters, rq_ctr;
   DevBuf<DiRes> di, di_tmp, di_hist[2];
   DevBuf<GiRes> gi, gi_hist[2];
   DevBuf<GTexel> gbuf[2];
@@ -72,7 +72,7 @@ struct RestirState {
     f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(demod[0]); f(demod[1]);
     f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
     f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
-    f(var[0]); f(var[1]); f(rq_o); f(rq_d); f(rq_hit); f(rq_pix);
+    f(var[0]); f(var[1]); f(rq_o); f(rq_d); f(rq_hit);
   }
   // Sizes every buffer for n pixels and clears them (no history).
   apt_status reset(size_t n, hipStream_t s) {
@@ -84,7 +84,7 @@ struct RestirState {
         st = APT_ERR_DEVICE;
       }
     });
-    if (st == APT_OK) st = counters.ensure(2);
+    if (st == APT_OK) st = counters.ensure(2 * RS_CTR_SLOTS);
     if (st == APT_OK) st = rq_ctr.ensure(RQ_WORDS);
     npix = st == APT_OK ? n : 0;
     return st;
@@ -106,7 +106,7 @@ struct RestirState {
     b.svgf_col[0] = col[0].p; b.svgf_col[1] = col[1].p; b.svgf_mom[0] = mom[0].p; b.svgf_mom[1] = mom[1].p;
     b.svgf_var[0] = var[0].p; b.svgf_var[1] = var[1].p; b.svgf_count = count.p; b.svgf_hist = hist.p;
     b.counters = counters.p;
-    b.rq_o = rq_o.p; b.rq_d = rq_d.p; b.rq_pix = rq_pix.p; b.rq_hit = rq_hit.p; b.rq_ctr = rq_ctr.p;
+    b.rq_o = rq_o.p; b.rq_d = rq_d.p; b.rq_hit = rq_hit.p; b.rq_ctr = rq_ctr.p;
     return b;
   }
 };
@@ -326,7 +326,7 @@ apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint3
     r->rs_row0 = row_start;
   }
   APT_HIP(hipMemsetAsync(r->rs.radiance.p, 0, npix * sizeof(float4), s));
-  APT_HIP(hipMemsetAsync(r->rs.counters.p, 0, 2 * sizeof(uint32_t), s));
+  APT_HIP(hipMemsetAsync(r->rs.counters.p, 0, 2 * RS_CTR_SLOTS * sizeof(uint32_t), s));
   const SceneView sc = scene_gpu_view(r->gpu, r->cam, width, height);
   const CamView cur = cam_view(sc);
   const apt_config& c = r->cfg;
@@ -362,13 +362,15 @@ apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint3
                           out_rgba ? r->d_rgba.p : nullptr, s)) != APT_OK)
     return st;
   APT_HIP(hipEventRecord(r->ev1, s));
-  uint32_t cnt[2];
-  APT_HIP(hipMemcpyAsync(cnt, r->rs.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  uint32_t part[2 * RS_CTR_SLOTS];
+  APT_HIP(hipMemcpyAsync(part, r->rs.counters.p, sizeof(part), hipMemcpyDeviceToHost, s));
   if ((st = finish_frame(r, npix, out_rgba, out_accum)) != APT_OK) return st;
   float ms = 0.0f;
   APT_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
   r->stats = apt_render_stats{};
   r->stats.paths = (uint64_t)npix * spp;
+  uint64_t cnt[2] = {0, 0};
+  for (uint32_t k = 0; k < 2 * RS_CTR_SLOTS; ++k) cnt[k / RS_CTR_SLOTS] += part[k];
   r->stats.extension_rays = cnt[0];
   r->stats.shadow_rays = cnt[1];
   r->stats.segments = cnt[0];

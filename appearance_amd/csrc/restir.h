@@ -89,19 +89,18 @@ struct RestirBuffers {
   float* svgf_var[2];
   uint32_t* svgf_count;
   float4* svgf_hist;    // a-trous pass-0 output kept for the next frame
-  uint32_t* counters;   // [0] closest-hit rays, [1] shadow rays traced (running totals)
-  // ray queue of one tracing stage (traced by the persistent pool kernel):
-  // origin, direction | tmax, pixel per queued ray; the result per pixel
+  uint32_t* counters;   // closest-hit rays, then shadow rays traced: RS_CTR_SLOTS partial sums each
+  // rays of one tracing stage, per pixel, traced by the persistent pool kernel:
+  // origin | 1 when the pixel has a ray, direction | tmax; the result
   // (t, u, v, prim bits for a closest hit; w = 1 when occluded for any hit)
   float4* rq_o;
   float4* rq_d;
-  uint32_t* rq_pix;
   float4* rq_hit;
-  uint32_t* rq_ctr;     // [RQ_COUNT] queued rays, [RQ_POOL ..] the pool's counters
+  uint32_t* rq_ctr;     // [RQ_POOL ..] the pool's counters
 };
 
-constexpr uint32_t RQ_COUNT = 0;
-constexpr uint32_t RQ_POOL = 16;
+constexpr uint32_t RS_CTR_SLOTS = 64;
+constexpr uint32_t RQ_POOL = 0;
 constexpr uint32_t RQ_WORDS = RQ_POOL + POOL_WORDS;
 
 // One sample of the ReSTIR integrator over the slab: camera rays, bounces with

@@ -60,9 +60,13 @@ __device__ __forceinline__ bool tile_pixel(const RestirParams& p, uint32_t& x, u
   return x < p.width && y < p.rows;
 }
 
+// Adds the wave's traced rays to one of RS_CTR_SLOTS counters of a kind
+// (chosen by work-group): one address for every wave of a light kernel
+// serialises its atomics.
 __device__ __forceinline__ void count_rays(bool traced, uint32_t* ctr) {
   const unsigned long long m = __ballot(traced);
-  if ((int)__lane_id() == __ffsll(m) - 1) atomicAdd(ctr, (uint32_t)__popcll(m));
+  if (m != 0ull && (int)__lane_id() == __ffsll(m) - 1)
+    atomicAdd(ctr + (blockIdx.x % RS_CTR_SLOTS), (uint32_t)__popcll(m));
 }
 
 __device__ __forceinline__ uint32_t h16(float a) { return (uint32_t)__half_as_ushort(__float2half_rn(a)); }
@@ -284,7 +288,7 @@ template <int W, int S>
 __device__ __forceinline__ bool visible(const SceneView& sc, const LdsStack& st, v3 x, v3 n, v3 wi, float dist,
                                         uint32_t* ctr) {
   TraceRay ray = make_trace_ray(x + n * TRACE_EPSILON, wi, 0.0f, dist - 2.0f * TRACE_EPSILON);
-  count_rays(true, ctr + 1);
+  count_rays(true, ctr + RS_CTR_SLOTS);
   return !trace_any<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st);
 }
 
@@ -412,29 +416,26 @@ __global__ __launch_bounds__(BLOCK) void k_rs_raygen(RestirParams p, SceneView s
 // two waves, 15% slower end to end on the 1080p ReSTIR frame.
 constexpr int RS_TRACE_WAVES = 4;
 
-// ---- ray queues ------------------------------------------------------------
-// The stages whose rays dominate the frame (the path's extension ray, the BSDF
-// candidate of the DI RIS and the visibility of its winner) queue their rays
-// with wave-level compaction, the persistent pool kernel traces the queue, and
-// the next kernel reads each pixel's result.  The traversal is the one every
-// other stage runs, so the results equal tracing in place.
+// ---- ray stages --------------------------------------------------------------
+// The rays that dominate the frame (the path's extension ray, the BSDF
+// candidate of the DI RIS and the visibility of its winner) are written per
+// pixel, traced by the persistent pool kernel over the slab's pixels (pixels
+// without a ray hold an empty one, tmax < 0, which ends at the root), and the
+// next kernel reads each pixel's result.  The traversal is the one every other
+// stage runs, so the results equal tracing in place.
 
-__global__ void k_rq_reset(RestirBuffers b) {
-  if (threadIdx.x == 0) b.rq_ctr[RQ_COUNT] = 0u;
-  pool_zero(b.rq_ctr + RQ_POOL, threadIdx.x);
+__global__ void k_rq_reset(RestirBuffers b) { pool_zero(b.rq_ctr + RQ_POOL, threadIdx.x); }
+
+// Stores pixel pix's ray of the stage (rq_o.w = 1 marks a ray, else empty).
+__device__ __forceinline__ void rq_put(const RestirBuffers& b, bool want, uint32_t pix, v3 o, v3 d, float tmax) {
+  b.rq_o[pix] = v4(o, want ? 1.0f : 0.0f);
+  b.rq_d[pix] = v4(d, want ? tmax : -1.0f);
 }
 
-// Queues a ray for pixel `pix` when `want` (every lane of the wave calls it).
-__device__ __forceinline__ void rq_push(const RestirBuffers& b, bool want, uint32_t pix, v3 o, v3 d, float tmax) {
-  const int slot = wave_append(want, &b.rq_ctr[RQ_COUNT]);
-  if (slot >= 0) {
-    b.rq_o[slot] = v4(o, 0.0f);
-    b.rq_d[slot] = v4(d, tmax);
-    b.rq_pix[slot] = pix;
-  }
-}
-
-template <int W>
+// Rays of a stage: rq_o/rq_d per pixel (EXT: the live paths' extension rays,
+// read straight from the path state, safe_origin along the ray as in
+// trace.wgsl:91-98); results to rq_hit per pixel.
+template <int W, bool EXT>
 struct RqSrc {
   static constexpr bool kOneScene = true;
   const WNode<W>* nodes;
@@ -442,47 +443,39 @@ struct RqSrc {
   uint32_t root;
   RestirBuffers b;
   __device__ __forceinline__ void fetch(uint32_t i, TraceRay& ray) const {
-    const float4 o = b.rq_o[i], d = b.rq_d[i];
-    ray = make_trace_ray(f4v(o), f4v(d), 0.0f, d.w);
+    if (EXT) {
+      const bool live = b.thr[i].w > 0.0f;
+      const v3 d = f4v(b.ray_d[i]);
+      ray = make_trace_ray(f4v(b.ray_o[i]) + d * TRACE_EPSILON, d, 0.0f, live ? 1e30f : -1.0f);
+    } else {
+      const float4 o = b.rq_o[i], d = b.rq_d[i];
+      ray = make_trace_ray(f4v(o), f4v(d), 0.0f, d.w);
+    }
   }
   __device__ __forceinline__ void closest(uint32_t i, const HitRec& h) const {
-    b.rq_hit[b.rq_pix[i]] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+    b.rq_hit[i] = make_float4(h.t, h.u, h.v, u2f(h.prim));
   }
   __device__ __forceinline__ void any(uint32_t i, bool hit) const {
-    b.rq_hit[b.rq_pix[i]] = make_float4(0.0f, 0.0f, 0.0f, hit ? 1.0f : 0.0f);
+    b.rq_hit[i] = make_float4(0.0f, 0.0f, 0.0f, hit ? 1.0f : 0.0f);
   }
 };
 
-template <int W, int S, bool ANY>
-__global__ __launch_bounds__(BLOCK, 8) void k_rs_trace(SceneView sc, RestirBuffers b) {
+template <int W, int S, bool ANY, bool EXT>
+__global__ __launch_bounds__(BLOCK, 8) void k_rs_trace(SceneView sc, RestirBuffers b, uint32_t n) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   __shared__ float4 s_top[TOP_BYTES / 16];
   const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
   const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
-  const uint32_t n = b.rq_ctr[RQ_COUNT];
-  RqSrc<W> src{nodes, sc.bvh_tris, sc.root, b};
+  RqSrc<W, EXT> src{nodes, sc.bvh_tris, sc.root, b};
   trace_pool<W, S, ANY>(src, n, b.rq_ctr + RQ_POOL, min(sc.refill, 64u), s_stack + threadIdx.x, BLOCK, s_top,
                         ktop);
-}
-
-// Extension rays of the live paths (safe_origin along the ray, trace.wgsl:91-98).
-__global__ __launch_bounds__(BLOCK) void k_rs_extend_ray(RestirParams p, RestirBuffers b) {
-  uint32_t x, y, pix;
-  if (!tile_pixel(p, x, y, pix)) return;
-  const bool live = b.thr[pix].w > 0.0f;
-  v3 o = mk(0.0f, 0.0f, 0.0f), d = o;
-  if (live) {
-    d = f4v(b.ray_d[pix]);
-    o = f4v(b.ray_o[pix]) + d * TRACE_EPSILON;
-  }
-  count_rays(live, b.counters);
-  rq_push(b, live, pix, o, d, 1e30f);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   const float4 t4 = b.thr[pix];
+  count_rays(t4.w > 0.0f, b.counters);
   if (!(t4.w > 0.0f)) {
     b.vtx_p[pix].w = -1.0f;
     return;
@@ -490,7 +483,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView s
   const v3 T = f4v(t4);
   const v3 d = f4v(b.ray_d[pix]);
   const v3 o = f4v(b.ray_o[pix]) + d * TRACE_EPSILON;  // safe_origin along the ray (trace.wgsl:91-98)
-  const float4 hq = b.rq_hit[pix];                     // k_rs_extend_ray -> k_rs_trace
+  const float4 hq = b.rq_hit[pix];                     // k_rs_trace<.., EXT>
   HitRec h;
   h.t = hq.x;
   h.u = hq.y;
@@ -543,7 +536,10 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_ris(RestirParam
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) return;
+  if (!load_vtx(b, pix, v)) {
+    rq_put(b, false, pix, mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 1.0f), 0.0f);
+    return;
+  }
   const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
   uint32_t rng = b.rng[pix];
   const float p_sun = sun_pick(sc);
@@ -618,13 +614,14 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_ris(RestirParam
   const bool shoot = r.phat > RESTIR_DI_EPSILON && eval_light(sc, r.light, r.u, r.v, v.x, e);
   b.di[pix] = r;
   b.rng[pix] = rng;
-  count_rays(shoot, b.counters + 1);
-  rq_push(b, shoot, pix, v.x + v.n * TRACE_EPSILON, e.wi, e.dist - 2.0f * TRACE_EPSILON);
+  count_rays(shoot, b.counters + RS_CTR_SLOTS);
+  rq_put(b, shoot, pix, v.x + v.n * TRACE_EPSILON, e.wi, e.dist - 2.0f * TRACE_EPSILON);
 }
 
 // The BSDF candidate of k_rs_di_ris: the same draws (after the first area
 // candidate's five), the same sample, its closest-hit ray queued.
 __global__ __launch_bounds__(BLOCK) void k_rs_di_bsdf_ray(RestirParams p, SceneView sc, RestirBuffers b) {
+  if (blockIdx.x == 0) pool_zero(b.rq_ctr + RQ_POOL, threadIdx.x);  // for the trace that follows
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
@@ -643,16 +640,14 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_bsdf_ray(RestirParams p, SceneV
     wi = s.wi;
   }
   count_rays(want, b.counters);
-  rq_push(b, want, pix, have ? v.x + v.n * TRACE_EPSILON : wi, wi, 1e30f);
+  rq_put(b, want, pix, have ? v.x + v.n * TRACE_EPSILON : wi, wi, 1e30f);
 }
 
-// trace_shadow_ray's verdict on the queued winners: visible ones get
+// trace_shadow_ray's verdict on the winners' rays: visible ones get
 // W = wsum / phat (nee.wgsl:226-235).
-__global__ __launch_bounds__(BLOCK) void k_rs_di_vis(RestirBuffers b) {
-  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= b.rq_ctr[RQ_COUNT]) return;
-  const uint32_t pix = b.rq_pix[i];
-  if (b.rq_hit[pix].w != 0.0f) return;
+__global__ __launch_bounds__(BLOCK) void k_rs_di_vis(RestirBuffers b, uint32_t npix) {
+  const uint32_t pix = blockIdx.x * BLOCK + threadIdx.x;
+  if (pix >= npix || b.rq_o[pix].w == 0.0f || b.rq_hit[pix].w != 0.0f) return;
   DiRes r = b.di[pix];
   r.W = (1.0f / r.phat) * r.wsum;
   b.di[pix] = r;
@@ -1194,18 +1189,17 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
   hipLaunchKernelGGL(k_rs_raygen, grid, block, 0, s, p, sc, b);
   for (uint32_t bounce = 0; bounce < p.max_bounces; ++bounce) {
     const bool more = bounce + 1 < p.max_bounces;
-    const dim3 pool(pool_blocks());
+    const uint32_t npix = p.width * p.rows;
+    const dim3 pool(std::min<uint32_t>(pool_blocks(), (npix + 63) / 64));
     hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
-    hipLaunchKernelGGL(k_rs_extend_ray, grid, block, 0, s, p, b);
-    hipLaunchKernelGGL((k_rs_trace<W, S, false>), pool, block, 0, s, sc, b);
+    hipLaunchKernelGGL((k_rs_trace<W, S, false, true>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rs_extend, grid, block, 0, s, p, sc, b, bounce);
-    hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
-    hipLaunchKernelGGL(k_rs_di_bsdf_ray, grid, block, 0, s, p, sc, b);
-    hipLaunchKernelGGL((k_rs_trace<W, S, false>), pool, block, 0, s, sc, b);
+    hipLaunchKernelGGL(k_rs_di_bsdf_ray, grid, block, 0, s, p, sc, b);  // also resets the pool counters
+    hipLaunchKernelGGL((k_rs_trace<W, S, false, false>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
     hipLaunchKernelGGL(k_rs_di_ris, grid, block, 0, s, p, sc, b);
-    hipLaunchKernelGGL((k_rs_trace<W, S, true>), pool, block, 0, s, sc, b);
-    hipLaunchKernelGGL(k_rs_di_vis, dim3((p.width * p.rows + BLOCK - 1) / BLOCK), block, 0, s, b);
+    hipLaunchKernelGGL((k_rs_trace<W, S, true, false>), pool, block, 0, s, sc, b, npix);
+    hipLaunchKernelGGL(k_rs_di_vis, dim3((npix + BLOCK - 1) / BLOCK), block, 0, s, b, npix);
     if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S>), grid, block, 0, s, p, sc, b, bounce);
     if (bounce == 0 && p.restir_di) {
       hipLaunchKernelGGL((k_rs_di_temporal<W, S>), grid, block, 0, s, p, sc, b);
