@@ -393,6 +393,12 @@ template <int W, int S, bool ANY, class Src>
 __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
                                            uint2* const lds, const int stride, const float4* top,
                                            uint32_t ktop) {
+  // The first ceil(n / 64) waves take every ray on their first visits (64
+  // each); the rest of the grid would only add to the queue on the range
+  // counters and find them drained, which over a few thousand rays costs
+  // more than tracing them: those waves leave at once.
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if ((uint64_t)(blockIdx.x * (blockDim.x >> 6) + wave) * 64u >= n) return;
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
   PoolCursor pc = pool_start();
