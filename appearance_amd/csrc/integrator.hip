@@ -78,18 +78,23 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, 
 __global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, PathView ps, int qin,
                                                  uint32_t bounce) {
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  bool alive = false, want_shadow = false;
-  uint32_t p = 0;
-  if (i < count) {
-    p = ps.queue[qin][i];
-    shade_path(sc, ps, p, bounce, fv.max_bounces, fv.rr_start, alive, want_shadow);
+  // A grid of a few work-groups per CU strides over the queue: work-group b
+  // takes tiles b, b + G, ..., so a short queue (a late bounce) still spreads
+  // over every XCD, and the work-groups past it leave at once.
+  for (uint32_t tile = blockIdx.x; tile * BLOCK < count; tile += gridDim.x) {
+    const uint32_t i = tile * BLOCK + threadIdx.x;
+    bool alive = false, want_shadow = false;
+    uint32_t p = 0;
+    if (i < count) {
+      p = ps.queue[qin][i];
+      shade_path(sc, ps, p, bounce, fv.max_bounces, fv.rr_start, alive, want_shadow);
+    }
+    int slot, sslot;
+    block_append2<BLOCK>(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)], slot, want_shadow,
+                         &ps.counters[CNT_SHADOW], sslot);
+    if (slot >= 0) ps.queue[qin ^ 1][slot] = p;
+    if (sslot >= 0) ps.shadow_queue[sslot] = p;
   }
-  int slot, sslot;
-  block_append2<BLOCK>(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)], slot, want_shadow, &ps.counters[CNT_SHADOW],
-                       sslot);
-  if (slot >= 0) ps.queue[qin ^ 1][slot] = p;
-  if (sslot >= 0) ps.shadow_queue[sslot] = p;
 }
 
 // Shadow rays of the paths that asked for one; unoccluded ones add their
@@ -192,8 +197,9 @@ __global__ __launch_bounds__(BLOCK) void k_resolve(FrameView fv, const float4* _
 template <int W>
 void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_paths, int qin,
                            const FrameView& fv, uint32_t bounce, hipStream_t s) {
-  const dim3 grid((n_paths + BLOCK - 1) / BLOCK);
-  // the traversal stages pull from their queue: a grid that fills the device
+  // the traversal stages pull from their queue and the shade stage strides
+  // over it: grids that fill the device
+  const dim3 grid(std::min<uint32_t>(pool_blocks(), (n_paths + BLOCK - 1) / BLOCK));
   const dim3 pool(std::min<uint32_t>(pool_blocks(), (n_paths + 63) / 64));
   const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
   APT_FOR_STACK_DEPTH(sc.lds_stack, {

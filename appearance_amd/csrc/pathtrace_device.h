@@ -37,6 +37,7 @@ __device__ __forceinline__ void block_append2(bool a, uint32_t* ca, int& sa, boo
   __shared__ uint32_t s_base[2];
   const uint32_t lane = __lane_id(), w = threadIdx.x / 64;
   const unsigned long long ma = __ballot(a), mb = __ballot(b);
+  __syncthreads();  // a previous call's reads of s_n are done (loops call it repeatedly)
   if (lane == 0) {
     s_n[0][w] = (uint32_t)__popcll(ma);
     s_n[1][w] = (uint32_t)__popcll(mb);

@@ -29,6 +29,32 @@ struct Model {
   apt_scene_data* data;
 };
 
+// Page-locked host memory, so the frame's readback runs at the link's rate
+// with no staging copy.
+struct PinnedBytes {
+  uint8_t* p = nullptr;
+  size_t n = 0;
+  apt_status ensure(size_t count) {
+    if (count <= n && p) return APT_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipHostMalloc((void**)&p, count ? count : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipHostMalloc(%zu B) failed: %s", count, hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
+    }
+    n = count;
+    return APT_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -165,7 +191,7 @@ struct apt_renderer {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   apt_render_stats stats{};
   uint32_t frame_idx = 0;
-  std::vector<uint8_t> cb_pixels;
+  PinnedBytes cb_pixels;  // the callback's borrowed RGBA8 slab
   // ReSTIR integrator: per-pixel state, frame parity and the previous camera
   RestirState rs;
   uint32_t rs_cur = 0;
@@ -182,6 +208,7 @@ struct apt_renderer {
     p_sd.release(); p_sc.release(); p_rng.release(); p_q0.release(); p_q1.release();
     p_qs.release(); p_cnt.release();
     rs.release();
+    cb_pixels.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -429,7 +456,10 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
     return render_restir(r, width, height, row_start, rows, out_rgba, out_accum);
 
   const uint32_t spp = r->cfg.sample_count;
-  const uint64_t budget = r->cfg.queue_chunk ? r->cfg.queue_chunk : (1u << 21);
+  // Paths in flight per pass: with 2^24 (2.4 GB of path state) a 1080p frame
+  // runs up to 8 samples per pass, so the late bounces' short queues still
+  // hold enough rays to fill the device.
+  const uint64_t budget = r->cfg.queue_chunk ? r->cfg.queue_chunk : (1u << 24);
   const uint32_t per_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(spp, budget / npix));
   if ((st = ensure_paths(r, (uint32_t)(per_pass * npix))) != APT_OK) return st;
   APT_HIP(hipMemsetAsync(r->p_cnt.p, 0, CNT_COUNT * sizeof(uint32_t), s));
@@ -672,11 +702,12 @@ apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint3
   clear_error();
   APT_REQUIRE(r && cb, APT_ERR_INVALID_ARG, "apt_render_cb: null argument");
   APT_REQUIRE(row_end > row_start, APT_ERR_INVALID_ARG, "apt_render_cb: empty slab");
-  r->cb_pixels.resize((size_t)width * (row_end - row_start) * 4);
-  apt_status st = render_impl(r, width, height, row_start, row_end, r->cb_pixels.data(),
-                              r->cb_pixels.size(), nullptr, 0);
+  const size_t len = (size_t)width * (row_end - row_start) * 4;
+  apt_status st = r->cb_pixels.ensure(len);
   if (st != APT_OK) return st;
-  cb(r->cb_pixels.data(), r->cb_pixels.size(), user);
+  st = render_impl(r, width, height, row_start, row_end, r->cb_pixels.p, len, nullptr, 0);
+  if (st != APT_OK) return st;
+  cb(r->cb_pixels.p, len, user);
   return APT_OK;
 }
 

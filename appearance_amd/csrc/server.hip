@@ -103,19 +103,22 @@ __global__ __launch_bounds__(BLOCK) void k_srv_shade(const SrvReq* __restrict__ 
                                                      const uint32_t* __restrict__ path_req,
                                                      int qin, uint32_t bounce) {
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  bool alive = false, want_shadow = false;
-  uint32_t p = 0;
-  if (i < count) {
-    p = ps.queue[qin][i];
-    const SrvReq& r = reqs[path_req[p]];
-    shade_path(r.sc, ps, p, bounce, r.max_bounces, r.rr_start, alive, want_shadow);
+  // grid-stride over the queue, as the plain integrator's shade stage
+  for (uint32_t tile = blockIdx.x; tile * BLOCK < count; tile += gridDim.x) {
+    const uint32_t i = tile * BLOCK + threadIdx.x;
+    bool alive = false, want_shadow = false;
+    uint32_t p = 0;
+    if (i < count) {
+      p = ps.queue[qin][i];
+      const SrvReq& r = reqs[path_req[p]];
+      shade_path(r.sc, ps, p, bounce, r.max_bounces, r.rr_start, alive, want_shadow);
+    }
+    int slot, sslot;
+    block_append2<BLOCK>(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)], slot, want_shadow,
+                         &ps.counters[CNT_SHADOW], sslot);
+    if (slot >= 0) ps.queue[qin ^ 1][slot] = p;
+    if (sslot >= 0) ps.shadow_queue[sslot] = p;
   }
-  int slot, sslot;
-  block_append2<BLOCK>(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)], slot, want_shadow, &ps.counters[CNT_SHADOW],
-                       sslot);
-  if (slot >= 0) ps.queue[qin ^ 1][slot] = p;
-  if (sslot >= 0) ps.shadow_queue[sslot] = p;
 }
 
 template <int W>
@@ -196,11 +199,12 @@ template <int W, int S>
 void enqueue_batch_t(const SrvDevice& d, hipStream_t s) {
   const dim3 grid((d.path_slots + BLOCK - 1) / BLOCK), block(BLOCK);
   const dim3 pool(std::min<uint32_t>(pool_blocks(), (d.path_slots + 63) / 64));
+  const dim3 shade(std::min<uint32_t>(pool_blocks(), (d.path_slots + BLOCK - 1) / BLOCK));
   hipLaunchKernelGGL(k_srv_raygen, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);
   for (uint32_t b = 0; b < d.max_bounces; ++b) {
     const int qin = (int)(b & 1u);
     hipLaunchKernelGGL((k_srv_extend<W, S>), pool, block, 0, s, d.reqs, d.ps, d.path_req, qin);
-    hipLaunchKernelGGL(k_srv_shade, grid, block, 0, s, d.reqs, d.ps, d.path_req, qin, b);
+    hipLaunchKernelGGL(k_srv_shade, shade, block, 0, s, d.reqs, d.ps, d.path_req, qin, b);
     hipLaunchKernelGGL((k_srv_shadow<W, S>), pool, block, 0, s, d.reqs, d.ps, d.path_req);
   }
   hipLaunchKernelGGL(k_srv_film, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);

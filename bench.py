@@ -297,14 +297,17 @@ def bench_disney(args) -> int:
                                       bvh_width=args.width, lds_stack_depth=args.lds_stack))
     pt.set_scene(scene)
     pt.set_camera(scene.camera)
+    # render_rgba is the render-node entry: the frame resolved to RGBA8 in the
+    # 64x64 block layout and read back to host memory (the FP32 accumulator
+    # export that the parity tests read is not part of it)
     for _ in range(args.warmup):
-        pt.render_accum((W, H))
+        pt.render_rgba((W, H))
     torch.cuda.synchronize()
     log(f"timing {args.steps} frames")
     dev_ms, rays = [], 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        pt.render_accum((W, H))
+        pt.render_rgba((W, H))
         st = pt.stats()
         dev_ms.append(st["ms_total"])
         rays += st["extension_rays"] + st["shadow_rays"]

@@ -293,7 +293,7 @@ typedef struct apt_config {
   uint32_t svgf_iterations;   /* RESTIR: a-trous passes, <= 8 */
   uint32_t svgf_history;      /* RESTIR: max frames of SVGF moment history */
   uint32_t bvh_width;         /* 2, 4, 8 */
-  uint32_t queue_chunk;       /* PLAIN: paths per wavefront pass (0 = 2^21) */
+  uint32_t queue_chunk;       /* PLAIN: paths per wavefront pass (0 = 2^24) */
   uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS: 8, 16, 24 */
   uint32_t seed;              /* base seed mixed into every path seed */
   uint32_t frame_idx;         /* starting frame index */
