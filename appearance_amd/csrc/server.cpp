@@ -425,7 +425,7 @@ apt_status apt_server_create(const apt_server_config* cfg, apt_server** out) {
   clear_error();
   APT_REQUIRE(out, APT_ERR_INVALID_ARG, "apt_server_create: out is NULL");
   apt_server_config c = cfg ? *cfg : apt_server_config{};
-  if (!c.path_slots) c.path_slots = 1u << 21;
+  if (!c.path_slots) c.path_slots = 1u << 24;  // 2.6 GB of path state
   if (!c.max_active) c.max_active = 64;
   if (!c.accum_budget) c.accum_budget = 4ull << 30;
   if (!c.max_bounces) c.max_bounces = 8;

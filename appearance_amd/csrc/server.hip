@@ -38,24 +38,26 @@ __global__ __launch_bounds__(BLOCK) void k_srv_raygen(const SrvBatch* __restrict
                                                       uint32_t* __restrict__ path_req,
                                                       uint32_t* __restrict__ path_pix) {
   const uint32_t n_paths = batch->n_paths;
-  const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ps.counters[CNT_QUEUE0] = n_paths;
     ps.counters[CNT_QUEUE1] = 0;
     ps.counters[CNT_SHADOW] = 0;
   }
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);
-  if (p >= n_paths) return;
-  const SrvTile& t = batch->tiles[find_tile(batch->tiles, batch->n_tiles, p)];
-  const SrvReq& r = reqs[t.req_slot];
-  const uint32_t idx = t.first + (p - t.offset);  // request path index, sample-major
-  const uint32_t npix = r.width * r.height;
-  const uint32_t sample = idx / npix, pixel = idx - sample * npix;
-  const uint32_t x = pixel % r.width, y = pixel / r.width;
-  start_path(r.sc, ps, p, x, y, r.width, r.height, sample + r.seed);
-  path_req[p] = t.req_slot;
-  path_pix[p] = pixel;
-  ps.queue[0][p] = p;
+  // a device-filling grid strides over the batch, whose size the graph
+  // replay does not know
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n_paths; p += gridDim.x * BLOCK) {
+    const SrvTile& t = batch->tiles[find_tile(batch->tiles, batch->n_tiles, p)];
+    const SrvReq& r = reqs[t.req_slot];
+    const uint32_t idx = t.first + (p - t.offset);  // request path index, sample-major
+    const uint32_t npix = r.width * r.height;
+    const uint32_t sample = idx / npix, pixel = idx - sample * npix;
+    const uint32_t x = pixel % r.width, y = pixel / r.width;
+    start_path(r.sc, ps, p, x, y, r.width, r.height, sample + r.seed);
+    path_req[p] = t.req_slot;
+    path_pix[p] = pixel;
+    ps.queue[0][p] = p;
+  }
 }
 
 constexpr uint32_t SRV_REFILL = 32;  // persistent traversal refill threshold
@@ -172,13 +174,14 @@ __global__ __launch_bounds__(BLOCK) void k_srv_film(const SrvBatch* __restrict__
                                                     const SrvReq* __restrict__ reqs, PathView ps,
                                                     const uint32_t* __restrict__ path_req,
                                                     const uint32_t* __restrict__ path_pix) {
-  const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (p >= batch->n_paths) return;
-  const float4 l = ps.rad[p];
-  unsigned long long* a = reqs[path_req[p]].accum + 3 * (size_t)path_pix[p];
-  atomicAdd(a + 0, to_fixed(l.x));
-  atomicAdd(a + 1, to_fixed(l.y));
-  atomicAdd(a + 2, to_fixed(l.z));
+  const uint32_t n_paths = batch->n_paths;
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < n_paths; p += gridDim.x * BLOCK) {
+    const float4 l = ps.rad[p];
+    unsigned long long* a = reqs[path_req[p]].accum + 3 * (size_t)path_pix[p];
+    atomicAdd(a + 0, to_fixed(l.x));
+    atomicAdd(a + 1, to_fixed(l.y));
+    atomicAdd(a + 2, to_fixed(l.z));
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_srv_resolve(const unsigned long long* __restrict__ accum,
@@ -197,14 +200,14 @@ __global__ void k_srv_zero(unsigned long long* __restrict__ a, size_t n) {
 
 template <int W, int S>
 void enqueue_batch_t(const SrvDevice& d, hipStream_t s) {
-  const dim3 grid((d.path_slots + BLOCK - 1) / BLOCK), block(BLOCK);
+  // every stage strides over (or pulls from) the batch with a grid that fills the device
+  const dim3 grid(std::min<uint32_t>(pool_blocks(), (d.path_slots + BLOCK - 1) / BLOCK)), block(BLOCK);
   const dim3 pool(std::min<uint32_t>(pool_blocks(), (d.path_slots + 63) / 64));
-  const dim3 shade(std::min<uint32_t>(pool_blocks(), (d.path_slots + BLOCK - 1) / BLOCK));
   hipLaunchKernelGGL(k_srv_raygen, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);
   for (uint32_t b = 0; b < d.max_bounces; ++b) {
     const int qin = (int)(b & 1u);
     hipLaunchKernelGGL((k_srv_extend<W, S>), pool, block, 0, s, d.reqs, d.ps, d.path_req, qin);
-    hipLaunchKernelGGL(k_srv_shade, shade, block, 0, s, d.reqs, d.ps, d.path_req, qin, b);
+    hipLaunchKernelGGL(k_srv_shade, grid, block, 0, s, d.reqs, d.ps, d.path_req, qin, b);
     hipLaunchKernelGGL((k_srv_shadow<W, S>), pool, block, 0, s, d.reqs, d.ps, d.path_req);
   }
   hipLaunchKernelGGL(k_srv_film, grid, block, 0, s, d.batch, d.reqs, d.ps, d.path_req, d.path_pix);

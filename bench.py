@@ -91,6 +91,8 @@ def main() -> int:
                     choices=["bvh_stress", "disney", "restir", "serving", "bvh_build"])
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--rates", default="20,200")
+    ap.add_argument("--policy", type=int, default=0,
+                    help="serving: 0 = shortest remaining work first, 1 = FIFO")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy before timing (the known-size "
@@ -475,7 +477,7 @@ def bench_serving(args) -> int:
     scenes = [generate_scene(n, seed=[0, 1, 3, 3][i]) for i, n in enumerate(names)]
     sky = scenes[2].sky
     scenes[1].sky = sky  # the triangle soup has no emitters: light it with the sky
-    srv = Server(max_active=64, policy=0)
+    srv = Server(max_active=64, policy=args.policy)
     ids = [srv.add_scene(sc) for sc in scenes]
     plain = []
     for sc in scenes:
@@ -548,7 +550,7 @@ def bench_serving(args) -> int:
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"serving: {args.requests} requests, Poisson arrivals, 4 resident "
                                f"scenes, res in {{256^2,512^2,1280x720,1920x1080}}, spp in "
-                               f"{{1,4,16,64}}, depth 4..8", "total_samples": total_samples},
+                               f"{{1,4,16,64}}, depth 4..8", "total_samples": total_samples, "policy": args.policy},
         "serving": report,
     }
     print(json.dumps(out), flush=True)

@@ -396,7 +396,7 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
 typedef struct apt_server apt_server;
 
 typedef struct apt_server_config {
-  uint32_t path_slots;        /* paths per wavefront batch (0 = 2^21) */
+  uint32_t path_slots;        /* paths per wavefront batch (0 = 2^24) */
   uint32_t max_active;        /* admitted requests at a time (0 = 64) */
   uint64_t accum_budget;      /* bytes of per-request accumulators admitted (0 = 4 GiB) */
   uint32_t max_bounces;       /* deepest path any request may ask for (0 = 8) */
