@@ -75,7 +75,8 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, 
                           BLOCK, s_top, ktop);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_shade(FrameView fv, SceneView sc, PathView ps, int qin,
+// Four waves per SIMD: the compiler's own choice (136 VGPRs) allows three.
+__global__ __launch_bounds__(BLOCK, 4) void k_shade(FrameView fv, SceneView sc, PathView ps, int qin,
                                                  uint32_t bounce) {
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   // A grid of a few work-groups per CU strides over the queue: work-group b
