@@ -4,7 +4,8 @@
 // (crates/appearance-render-loop/src/node.rs:44-105).  Here a scheduler
 // thread owns the GPU: it admits queued requests while the accumulator budget
 // and the active-slot table allow, packs tiles of every admitted request into
-// one wavefront batch (shortest remaining work first), replays the batch from
+// one wavefront batch (requests that fit in one batch shortest-first, then the
+// larger ones in arrival order), replays the batch from
 // a captured hipGraph, keeps two batches in flight, and retires a request as
 // soon as the batch holding its last paths completes.  Requests that arrive
 // while others run join the next batch.
@@ -150,6 +151,18 @@ void build_batch(apt_server* s, Batch& b, SrvBatch& hb) {
   for (Req* q : s->slots)
     if (q && !q->cancel && q->issued < q->total) act.push_back(q);
   if (s->cfg.policy == 0) {
+    // Requests that fit in one batch go first, shortest remaining work
+    // first; the larger ones follow in arrival order, so a stream that
+    // outruns the device delays its large requests by the small requests'
+    // share of the work only, not by every later request smaller than they.
+    const uint64_t one = s->cfg.path_slots;
+    std::stable_sort(act.begin(), act.end(), [one](const Req* a, const Req* c) {
+      const bool sa = a->total <= one, sc = c->total <= one;
+      if (sa != sc) return sa;
+      if (sa) return a->total - a->issued < c->total - c->issued;
+      return a->id < c->id;
+    });
+  } else if (s->cfg.policy == 2) {
     std::stable_sort(act.begin(), act.end(), [](const Req* a, const Req* c) {
       return a->total - a->issued < c->total - c->issued;
     });
@@ -442,7 +455,7 @@ apt_status apt_server_create(const apt_server_config* cfg, apt_server** out) {
               "server: bvh_width %u is not 2, 4 or 8", c.bvh_width);
   APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 16 || c.lds_stack_depth == 24,
               APT_ERR_INVALID_ARG, "server: lds_stack_depth %u is not 8, 16 or 24", c.lds_stack_depth);
-  APT_REQUIRE(c.policy <= 1, APT_ERR_INVALID_ARG, "server: unknown policy %u", c.policy);
+  APT_REQUIRE(c.policy <= 2, APT_ERR_INVALID_ARG, "server: unknown policy %u", c.policy);
   apt_server* s = new apt_server();
   s->cfg = c;
   (void)hipGetDevice(&s->device);

@@ -92,7 +92,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--rates", default="20,200")
     ap.add_argument("--policy", type=int, default=0,
-                    help="serving: 0 = shortest remaining work first, 1 = FIFO")
+                    help="serving: 0 = one-batch requests shortest first, then the rest in "
+                         "arrival order; 1 = FIFO; 2 = shortest remaining work first")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device-to-device copy before timing (the known-size "

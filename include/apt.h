@@ -403,7 +403,9 @@ typedef struct apt_server_config {
   uint32_t tile_paths;        /* paths per scheduling tile (0 = 65536) */
   uint32_t bvh_width;         /* 2, 4, 8 (0 = 4) */
   uint32_t lds_stack_depth;   /* 8, 16, 24 (0 = 8) */
-  uint32_t policy;            /* 0 = shortest remaining work first, 1 = FIFO */
+  uint32_t policy;            /* 0 = requests of at most path_slots samples shortest remaining work
+                                 first, then the larger ones in arrival order; 1 = FIFO;
+                                 2 = shortest remaining work first */
 } apt_server_config;
 
 /* A render request: scene, camera, frame size, samples, depth and seed. */
