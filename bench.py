@@ -501,6 +501,8 @@ def bench_serving(args) -> int:
 
         def client():
             for i, r in enumerate(reqs):
+                if i % 128 == 127:
+                    log(f"served: {i + 1} of {len(reqs)} requests submitted")
                 d = r["t"] - (time.perf_counter() - t0)
                 if d > 0:
                     time.sleep(d)
@@ -512,7 +514,9 @@ def bench_serving(args) -> int:
         th.join()
         lat = []
         last = 0.0
-        for rid in rids:
+        for k, rid in enumerate(rids):
+            if k % 128 == 127:
+                log(f"served: {k + 1} of {len(rids)} requests done")
             res = srv.wait(rid)
             assert res.state == DONE
             lat.append(res.done_ms - res.submit_ms)
@@ -525,7 +529,9 @@ def bench_serving(args) -> int:
         log("one-at-a-time baseline")
         lat_b = []
         t0 = time.perf_counter()
-        for r in reqs:
+        for k, r in enumerate(reqs):
+            if k % 128 == 127:
+                log(f"one at a time: {k + 1} of {len(reqs)} requests done")
             d = r["t"] - (time.perf_counter() - t0)
             if d > 0:
                 time.sleep(d)
