@@ -308,9 +308,12 @@ __device__ __forceinline__ float di_phat(const SceneView& sc, const LdsStack& st
   if (!eval_light(sc, light, u, w, v.x, e)) return 0.0f;
   const float c = dot(e.wi, v.n);
   if (!(c > 0.0f)) return 0.0f;
-  if (vis && !visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, ctr)) return 0.0f;
+  // the BSDF first: nothing of it stays live through the shadow ray, and a
+  // zero target needs no ray
   float pdf;
-  return luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * c);
+  const float ph = luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * c);
+  if (vis && ph != 0.0f && !visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, ctr)) return 0.0f;
+  return ph;
 }
 
 // DiReservoir::update (di_reservoir.wgsl:46-56); one draw per call.
@@ -777,7 +780,13 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
   const LdsStack st{s_stack + threadIdx.x, BLOCK};
   DiRes r = b.di_tmp[pix];
   uint32_t rng = b.rng[pix];
-  const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+  // The pixel's BSDF (about 50 registers) is rebuilt from its material at
+  // each use instead of staying live through the taps' shadow rays.
+  const auto pixel_bsdf = [&]() {
+    uint32_t m = v.mat;
+    asm volatile("" : "+v"(m));
+    return make_bsdf(sc, m, v.n, v.back, v.t);
+  };
   const Tex ct = unpack_texel(b.gbuf[p.cur][pix]);
   const float cd = depth_cs(ct.depth);
   float radius = p.radius > 0.0f ? p.radius : (float)(p.width + p.rows) / 2.0f * 0.05f;
@@ -807,7 +816,7 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
     }
     if (valid) {
       DiRes nr = b.di_tmp[q];
-      nr.phat = di_phat<W, S>(sc, st, v, bs, nr.light, nr.u, nr.v, true, b.counters);
+      nr.phat = di_phat<W, S>(sc, st, v, pixel_bsdf(), nr.light, nr.u, nr.v, true, b.counters);
       valid = nr.phat > 0.0f;
       // DiReservoir::combine_unbiased: Z counts the M of the domains where the
       // chosen sample has a (visibility-tested) target > 0
@@ -818,7 +827,7 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
       if (kept == 2) {
         c_vis = nr.phat > 0.0f ? 1 : 0;
       } else if (kept == 1) {
-        if (r_vis < 0) r_vis = di_phat<W, S>(sc, st, v, bs, r.light, r.u, r.v, true, b.counters) > 0.0f ? 1 : 0;
+        if (r_vis < 0) r_vis = di_phat<W, S>(sc, st, v, pixel_bsdf(), r.light, r.u, r.v, true, b.counters) > 0.0f ? 1 : 0;
         c_vis = r_vis;
       }
       if (c_vis) z += r.M;

@@ -40,6 +40,21 @@ def test_random_scene_vs_brute(gpu_available, oracle, width):
                                   oracle.brute(tris, rays, True))
 
 
+def test_small_ray_counts(gpu_available, oracle):
+    """Ray counts around the 64-ray wave boundary: the pool kernel's waves
+    beyond ceil(n / 64) leave before taking rays, the others must still take
+    every ray."""
+    tris = random_tris(3000, 21)
+    rays_all = random_rays(5000, 22)
+    b = Bvh(tris, width=4)
+    for n in (1, 63, 64, 65, 257, 4097):
+        print(f"[trace] small ray count n={n}", flush=True)
+        rays = np.ascontiguousarray(rays_all[:n])
+        assert_hits_equal(b.trace_closest(as_rays(rays)), oracle.brute(tris, rays))
+        np.testing.assert_array_equal(b.trace_any(as_rays(rays)).astype(np.float32),
+                                      oracle.brute(tris, rays, True))
+
+
 def test_cornell_vs_brute(gpu_available, oracle):
     s = generate_scene("cornell")
     rng = np.random.default_rng(5)
