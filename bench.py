@@ -100,8 +100,8 @@ def main() -> int:
                          "kernel that scales the HBM counters of a --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(tiles/rays shard per GPU with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(slabs, rays and requests would shard per GPU with no exchange)", file=sys.stderr)
         return 2
 
     if args.workload == "disney":

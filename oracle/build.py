@@ -18,7 +18,7 @@ OUT = HERE / "liboracle.so"
 
 # -ffp-contract=off / no fast-math: the FP32 paths must round exactly like the
 # device code they are compared with.  -march is left generic so the library
-# built here also runs on the GPU box host.
+# built here also runs on another x86-64 host.
 FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off",
          "-fno-fast-math", "-Wall", "-Wno-unused-function"]
 

@@ -4,7 +4,7 @@ against the oracle path tracer on the same scene, camera and per-pixel seeds.
 Tolerances: relMSE = sum((gpu - ref)^2) / sum(ref^2) over the frame.  GPU and
 oracle draw the same random numbers, so paths only diverge where the device
 and host transcendental functions round differently; the bar is relMSE
-<= 1e-3 (north-star tolerance at 4096 spp) and tighter where stated."""
+<= 1e-3 (the radiance parity tolerance, stated at 4096 spp) and tighter where stated."""
 import numpy as np
 import pytest
 

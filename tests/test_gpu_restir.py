@@ -6,7 +6,7 @@ Tolerance: relMSE = sum((gpu - ref)^2) / sum(ref^2) per frame.  The two sides
 draw the same random numbers in the same order, so frames only differ where
 device and host transcendental functions round differently and a discrete
 choice (a reservoir pick, a similarity test) flips; the bar is relMSE <= 1e-3
-per frame, with the measured value printed."""
+per frame (the parity tolerance for radiance), with the measured value printed."""
 import math
 
 import numpy as np
