@@ -52,9 +52,10 @@ __device__ __forceinline__ float box_area(const float* lo, const float* hi) {
 
 // Per-triangle bounds and centroid (9 floats), and a flag for non-finite input.
 __global__ void k_prims(const float* __restrict__ tri, uint32_t n, float* __restrict__ prim,
-                        uint32_t* __restrict__ bad) {
+                        uint32_t* __restrict__ order, uint32_t* __restrict__ bad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  order[i] = i;  // the root's triangles in input order
   const float* t = tri + (size_t)i * 9;
   float* p = prim + (size_t)i * 9;
   bool ok = true;
@@ -729,10 +730,7 @@ apt_status build_bvh2_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
   const uint32_t nb = (n + 255) / 256;
   const uint32_t bad_init = 0xFFFFFFFFu;
   APT_HIP(hipMemcpyAsync(bad.p, &bad_init, 4, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_prims, dim3(nb), dim3(256), 0, s, d_tri, n, prim.p, bad.p);
-  std::vector<uint32_t> iota(n);
-  for (uint32_t i = 0; i < n; ++i) iota[i] = i;
-  APT_HIP(hipMemcpyAsync(order[0].p, iota.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_prims, dim3(nb), dim3(256), 0, s, d_tri, n, prim.p, order[0].p, bad.p);
   int kinit[12];
   for (int k = 0; k < 12; ++k) kinit[k] = ((k % 6) < 3) ? 0x7FFFFFFF : (int)0x80000000;
   APT_HIP(hipMemcpyAsync(keys.p, kinit, sizeof(kinit), hipMemcpyHostToDevice, s));

@@ -10,7 +10,10 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 #include "apt_internal.h"
@@ -213,21 +216,73 @@ double bvh2_sah_cost(const std::vector<Node2>& nodes) {
 
 namespace {
 
+// Host worker threads that stay alive between calls: the collapse runs two
+// parallel passes per tree level, and starting threads for each would cost
+// more than the passes.  One job at a time (callers hold `call_mu`).
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool* p = new HostPool();  // never destroyed: the workers just wait at exit
+    return *p;
+  }
+  uint32_t parts() const { return (uint32_t)workers_.size() + 1u; }
+  // Runs job(t) for t in [0, parts()) and returns when all have finished.
+  void run(const std::function<void(uint32_t)>& job) {
+    std::lock_guard<std::mutex> call(call_mu_);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      job_ = &job;
+      pending_ = (uint32_t)workers_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    job(0);
+    std::unique_lock<std::mutex> lock(mu_);
+    done_.wait(lock, [&] { return pending_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  HostPool() {
+    const uint32_t hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    for (uint32_t t = 1; t < hw; ++t) workers_.emplace_back([this, t] { loop(t); });
+  }
+  void loop(uint32_t t) {
+    uint64_t seen = 0;
+    while (true) {
+      const std::function<void(uint32_t)>* job;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [&] { return gen_ != seen; });
+        seen = gen_;
+        job = job_;
+      }
+      (*job)(t);
+      std::lock_guard<std::mutex> lock(mu_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_, call_mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(uint32_t)>* job_ = nullptr;
+  uint32_t pending_ = 0;
+  uint64_t gen_ = 0;
+};
+
 // Runs f(begin, end) over [0, n) in contiguous chunks on the host's cores.
 template <class F>
 void parallel_for(uint32_t n, F f) {
-  const uint32_t hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-  const uint32_t parts = n < 4096 ? 1u : hw;
-  if (parts == 1) {
+  if (n < 4096) {
     f(0u, n);
     return;
   }
-  std::vector<std::thread> pool;
-  for (uint32_t t = 0; t < parts; ++t) {
+  HostPool& pool = HostPool::get();
+  const uint32_t parts = pool.parts();
+  pool.run([&](uint32_t t) {
     const uint32_t i0 = (uint32_t)((uint64_t)n * t / parts), i1 = (uint32_t)((uint64_t)n * (t + 1) / parts);
-    pool.emplace_back(f, i0, i1);
-  }
-  for (std::thread& th : pool) th.join();
+    if (i0 < i1) f(i0, i1);
+  });
 }
 
 // Grid of one node: per axis the smallest power-of-two step that spans the
@@ -273,72 +328,98 @@ apt_status collapse_upload(const float* tri_xyz, const float* d_tri_xyz, uint32_
                            float pad, apt_bvh* out) {
   const std::vector<Node2>& bn = b2.nodes;
   std::vector<WNode<W>> wide;
-  std::vector<uint32_t> queue;      // BVH2 node of each wide node, breadth first
-  std::vector<uint32_t> wdepth;     // depth of each wide node
-  std::vector<uint32_t> wstack;     // worst-case stack on the way to each wide node
-  std::vector<uint32_t> wkids;      // the W BVH2 nodes under each wide node (~0u: empty slot)
+  std::vector<uint32_t> wkids;  // the W BVH2 nodes under each wide node (~0u: empty slot)
   uint32_t max_depth = 1, max_stack = 1, n_leaves = 0;
 
   auto leaf_ref = [&](const Node2& nd) { return make_leaf_ref(nd.first, nd.count); };
 
+  // Breadth first, one level of wide nodes at a time: the nodes of a level
+  // are opened in parallel, and a prefix sum over their internal children
+  // numbers the next level in the order a FIFO queue would (parent order,
+  // then child slot).
   uint32_t root;
+  std::vector<uint32_t> front, fstack;  // BVH2 node and worst-case stack of each wide node of the level
   if (bn[0].count) {
     root = leaf_ref(bn[0]);
     n_leaves = 1;
   } else {
     root = 0;
-    queue.push_back(0);
-    wdepth.push_back(1);
-    wstack.push_back(0);
+    front.push_back(0);
+    fstack.push_back(0);
   }
-  for (size_t qi = 0; qi < queue.size(); ++qi) {
-    // Open the binary node into up to W children, greedily expanding the
-    // internal child of largest surface area.
-    uint32_t kids[W];
-    int nk = 0;
-    kids[nk++] = bn[queue[qi]].left;
-    kids[nk++] = bn[queue[qi]].right;
-    while (nk < W) {
-      int best = -1;
-      float best_a = -1.0f;
-      for (int k = 0; k < nk; ++k) {
-        const Node2& c = bn[kids[k]];
-        if (c.count) continue;
-        float a = box_area(c.lo, c.hi);
-        if (a > best_a) {
-          best_a = a;
-          best = k;
+  for (uint32_t depth = 1; !front.empty(); ++depth) {
+    const uint32_t m = (uint32_t)front.size(), base = (uint32_t)wide.size();
+    wide.resize((size_t)base + m);
+    wkids.resize(((size_t)base + m) * W);
+    std::vector<uint8_t> nk_of(m), nint(m);
+    parallel_for(m, [&](uint32_t i0, uint32_t i1) {
+      for (uint32_t i = i0; i < i1; ++i) {
+        // Open the binary node into up to W children, greedily expanding the
+        // internal child of largest surface area.
+        uint32_t kids[W];
+        int nk = 0;
+        kids[nk++] = bn[front[i]].left;
+        kids[nk++] = bn[front[i]].right;
+        while (nk < W) {
+          int best = -1;
+          float best_a = -1.0f;
+          for (int k = 0; k < nk; ++k) {
+            const Node2& c = bn[kids[k]];
+            if (c.count) continue;
+            float a = box_area(c.lo, c.hi);
+            if (a > best_a) {
+              best_a = a;
+              best = k;
+            }
+          }
+          if (best < 0) break;
+          uint32_t x = kids[best];
+          kids[best] = bn[x].left;
+          kids[nk++] = bn[x].right;
         }
+        int ni = 0;
+        for (int k = 0; k < W; ++k) {
+          wkids[((size_t)base + i) * W + k] = k < nk ? kids[k] : 0xFFFFFFFFu;
+          if (k < nk && !bn[kids[k]].count) ++ni;
+        }
+        nk_of[i] = (uint8_t)nk;
+        nint[i] = (uint8_t)ni;
       }
-      if (best < 0) break;
-      uint32_t x = kids[best];
-      kids[best] = bn[x].left;
-      kids[nk++] = bn[x].right;
+    });
+    std::vector<uint32_t> off((size_t)m + 1, 0u);
+    for (uint32_t i = 0; i < m; ++i) {
+      off[i + 1] = off[i] + nint[i];
+      // Worst-case stack: every sibling but one may be pushed at each level.
+      max_stack = std::max(max_stack, fstack[i] + (uint32_t)nk_of[i]);
+      n_leaves += (uint32_t)(nk_of[i] - nint[i]);
     }
-    WNode<W> w;
-    std::memset(&w, 0, sizeof(w));
-    // Worst-case stack: every sibling but one may be pushed at each level.
-    uint32_t here = wstack[qi] + (uint32_t)(nk - 1);
-    max_stack = std::max(max_stack, here + 1);
-    for (int k = 0; k < W; ++k) {
-      if (k >= nk) {
-        w.child[k] = REF_EMPTY;
-        continue;
+    const uint32_t next_base = base + m;
+    std::vector<uint32_t> nfront(off[m]), nstack(off[m]);
+    parallel_for(m, [&](uint32_t i0, uint32_t i1) {
+      for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t here = fstack[i] + (uint32_t)nk_of[i] - 1u;
+        WNode<W> w;
+        std::memset(&w, 0, sizeof(w));
+        uint32_t j = off[i];
+        for (int k = 0; k < W; ++k) {
+          const uint32_t kid = wkids[((size_t)base + i) * W + k];
+          if (kid == 0xFFFFFFFFu) {
+            w.child[k] = REF_EMPTY;
+          } else if (bn[kid].count) {
+            w.child[k] = leaf_ref(bn[kid]);
+          } else {
+            w.child[k] = next_base + j;
+            nfront[j] = kid;
+            nstack[j] = here;
+            ++j;
+          }
+        }
+        wide[(size_t)base + i] = w;
       }
-      const Node2& c = bn[kids[k]];
-      if (c.count) {
-        w.child[k] = leaf_ref(c);
-        n_leaves++;
-      } else {
-        w.child[k] = (uint32_t)queue.size();
-        queue.push_back(kids[k]);
-        wdepth.push_back(wdepth[qi] + 1);
-        wstack.push_back(here);
-        max_depth = std::max(max_depth, wdepth[qi] + 1);
-      }
-    }
-    wide.push_back(w);
-    for (int k = 0; k < W; ++k) wkids.push_back(k < nk ? kids[k] : 0xFFFFFFFFu);
+    });
+    if (off[m]) max_depth = std::max(max_depth, depth + 1);
+    front.swap(nfront);
+    fstack.swap(nstack);
   }
 
   // Child grids of every wide node (independent per node: in parallel).
