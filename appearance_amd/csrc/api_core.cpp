@@ -1,6 +1,7 @@
 // C ABI: error reporting, device query, BVH build and the ray-query entry
 // points of include/apt.h.
 #include <chrono>
+#include <memory>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -101,8 +102,7 @@ static apt_status build_impl(const float* tri_xyz, const float* d_tri_xyz, uint3
   apt_status st = resolve_opts(opts, o);
   if (st != APT_OK) return st;
   auto t0 = std::chrono::steady_clock::now();
-  BvhHostResult b2;
-  apt_bvh* bvh = nullptr;
+  std::unique_ptr<apt_bvh> bvh(new apt_bvh());  // freed on every early return
   if (o.builder == APT_BUILDER_GPU) {
     float* d_tmp = nullptr;
     if (!d_tri_xyz) {
@@ -116,28 +116,25 @@ static apt_status build_impl(const float* tri_xyz, const float* d_tri_xyz, uint3
       }
       d_tri_xyz = d_tmp;
     }
-    st = build_bvh2_device(d_tri_xyz, n_tris, o.max_leaf, o.bins, lib_stream(), b2);
-    if (st == APT_OK) {
-      bvh = new apt_bvh();
-      st = upload_wide_bvh(tri_xyz, d_tri_xyz, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh);
-    }
+    st = build_wide_device(d_tri_xyz, n_tris, o.max_leaf, o.bins, o.width, lib_stream(), bvh.get());
     if (d_tmp) (void)hipFree(d_tmp);
   } else {
-    st = build_bvh2_host(tri_xyz, n_tris, o.max_leaf, o.bins, b2);
-    if (st == APT_OK) {
-      bvh = new apt_bvh();
-      st = upload_wide_bvh(tri_xyz, nullptr, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh);
+    std::vector<float> h;
+    if (!tri_xyz) {  // device triangles, host builder
+      h.resize((size_t)n_tris * 9);
+      APT_HIP(hipMemcpy(h.data(), d_tri_xyz, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+      tri_xyz = h.data();
     }
+    BvhHostResult b2;
+    st = build_bvh2_host(tri_xyz, n_tris, o.max_leaf, o.bins, b2);
+    if (st == APT_OK) st = upload_wide_bvh(tri_xyz, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), bvh.get());
   }
-  if (st != APT_OK) {
-    delete bvh;
-    return st;
-  }
+  if (st != APT_OK) return st;
   bvh->lds_stack = o.lds_stack;
   bvh->refill = o.refill;
   bvh->stats.build_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  *out = bvh;
+  *out = bvh.release();
   return APT_OK;
 }
 
@@ -153,10 +150,7 @@ apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris, const a
   clear_error();
   APT_REQUIRE(out && d_tri_xyz && n_tris > 0, APT_ERR_INVALID_ARG,
               "apt_bvh_build_device: null or empty argument");
-  // the collapse writes the triangles in leaf order from a host copy
-  std::vector<float> h((size_t)n_tris * 9);
-  APT_HIP(hipMemcpy(h.data(), d_tri_xyz, h.size() * sizeof(float), hipMemcpyDeviceToHost));
-  return build_impl(h.data(), d_tri_xyz, n_tris, opts, out);
+  return build_impl(nullptr, d_tri_xyz, n_tris, opts, out);
 }
 
 void apt_bvh_destroy(apt_bvh* bvh) {

@@ -107,10 +107,7 @@ struct BvhHostResult {
 // Binned-SAH build on the host (bvh_host.cpp).
 apt_status build_bvh2_host(const float* tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
                            BvhHostResult& out);
-// The same tree built on the device from device-resident triangles
-// (bvh_gpu.hip); synchronous on stream s.
-apt_status build_bvh2_device(const float* d_tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
-                             hipStream_t s, BvhHostResult& out);
+
 double bvh2_sah_cost(const std::vector<Node2>& nodes);
 // Scene-scale padding used for every node box (see box_pad in bvh_host.cpp).
 float scene_pad(const float* tri_xyz, uint32_t n);
@@ -142,14 +139,16 @@ constexpr uint32_t POOL_STRIDE = 16;
 constexpr uint32_t POOL_WORDS = POOL_PARTS * POOL_STRIDE;
 
 namespace apt {
-// Collapse a binary BVH into a wide one and upload it (bvh_host.cpp).  The
-// leaf-order triangle records are gathered on the device from d_tri_xyz when
-// it is not null, else on the host from tri_xyz.
-apt_status upload_wide_bvh(const float* tri_xyz, const float* d_tri_xyz, uint32_t n, const BvhHostResult& b2,
-                           uint32_t width, float pad, apt_bvh* out);
-// tris[3i..3i+2] = triangle order[i] of d_tri_xyz (v0 | prim, v1, v2) (bvh_gpu.hip).
-apt_status gather_tris_device(const float* d_tri_xyz, const uint32_t* h_order, uint32_t n, float4* d_tris,
-                              hipStream_t s);
+// Collapse a binary BVH into a wide one and upload it with the leaf-order
+// triangle records (bvh_host.cpp).
+apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, uint32_t width, float pad,
+                           apt_bvh* out);
+// The same binary tree built on the device from device-resident triangles
+// and collapsed there into `out` (bvh_gpu.hip): byte-identical to
+// build_bvh2_host + upload_wide_bvh (the SAH cost statistic is summed in
+// another order).  Synchronous on stream s.
+apt_status build_wide_device(const float* d_tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
+                             uint32_t width, hipStream_t s, apt_bvh* out);
 // Launch the traversal kernels (trace.hip).
 apt_status launch_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
                                 apt_hit* d_hits, hipStream_t s);

@@ -19,6 +19,7 @@
 // the breadth-first collapse to 2/4/8-wide nodes is shared with it.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "apt_internal.h"
@@ -676,6 +677,238 @@ __global__ void k_gather_tris(const float* __restrict__ tri, const uint32_t* __r
   out[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
 }
 
+// ---------------------------------------------------------------------------
+// Collapse to W-wide quantized nodes on the device: the host collapse
+// (bvh_host.cpp) restated level by level.  Each wide node of a level opens
+// its BVH2 node greedily (the internal child of largest surface area first,
+// the same float operations), a prefix sum over the internal children numbers
+// the next level in breadth-first queue order, and a last pass quantizes
+// every node's child boxes to its grid in double precision with the host's
+// floor/ceil and nudge loops, so the nodes are byte-identical to the host
+// collapse's.
+constexpr uint32_t COL_BLOCK = 256;
+
+// bvh_host.cpp's box_area (no empty-box test: collapse boxes are valid)
+__device__ __forceinline__ float col_area(const float* lo, const float* hi) {
+  const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// Opens wide node base + i (BVH2 node front[i]): its up to W BVH2 children
+// into wkids, nk | (internal children << 8) into nkni, and the block's count
+// of internal children into blk_sum.
+template <int W>
+__global__ __launch_bounds__(COL_BLOCK) void k_col_open(const Node2* __restrict__ bn,
+                                                        const uint32_t* __restrict__ front, uint32_t m,
+                                                        uint32_t base, uint32_t* __restrict__ wkids,
+                                                        uint32_t* __restrict__ nkni,
+                                                        uint32_t* __restrict__ blk_sum) {
+  __shared__ uint32_t s_sum[COL_BLOCK / WAVE];
+  const uint32_t i = blockIdx.x * COL_BLOCK + threadIdx.x;
+  uint32_t ni = 0;
+  if (i < m) {
+    uint32_t kids[W];
+    int nk = 0;
+    kids[nk++] = bn[front[i]].left;
+    kids[nk++] = bn[front[i]].right;
+    while (nk < W) {
+      int best = -1;
+      float best_a = -1.0f;
+      for (int k = 0; k < nk; ++k) {
+        const Node2& c = bn[kids[k]];
+        if (c.count) continue;
+        const float a = col_area(c.lo, c.hi);
+        if (a > best_a) {
+          best_a = a;
+          best = k;
+        }
+      }
+      if (best < 0) break;
+      const uint32_t x = kids[best];
+      kids[best] = bn[x].left;
+      kids[nk++] = bn[x].right;
+    }
+    for (int k = 0; k < W; ++k) {
+      const bool in = k < nk;
+      wkids[((size_t)base + i) * W + k] = in ? kids[k] : 0xFFFFFFFFu;
+      if (in && !bn[kids[k]].count) ++ni;
+    }
+    nkni[i] = (uint32_t)nk | (ni << 8);
+  }
+  uint32_t v = ni;
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_sum[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < COL_BLOCK / WAVE; ++w) t += s_sum[w];
+    blk_sum[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of blk[0, nb) in place by one work-group; the sum goes to *total.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_col_scan(uint32_t* __restrict__ blk, uint32_t nb,
+                                                         uint32_t* __restrict__ total) {
+  __shared__ uint32_t s[SCAN_BLOCK];
+  const uint32_t t = threadIdx.x;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nb; c0 += SCAN_BLOCK) {
+    const uint32_t i = c0 + t;
+    const uint32_t v = i < nb ? blk[i] : 0u;
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < SCAN_BLOCK; off <<= 1) {
+      const uint32_t x = t >= off ? s[t - off] : 0u;
+      __syncthreads();
+      s[t] += x;
+      __syncthreads();
+    }
+    if (i < nb) blk[i] = carry + s[t] - v;
+    carry += s[SCAN_BLOCK - 1];
+    __syncthreads();
+  }
+  if (t == 0) *total = carry;
+}
+
+// Writes the child refs of wide node base + i (internal children numbered
+// base + m + their rank among the level's internal children) and the next
+// level's BVH2 nodes and stack depths; stats[0] = max stack, stats[1] =
+// leaves (as the host collapse counts them).
+template <int W>
+__global__ __launch_bounds__(COL_BLOCK) void k_col_write(const Node2* __restrict__ bn,
+                                                         const uint32_t* __restrict__ fstack, uint32_t m,
+                                                         uint32_t base, const uint32_t* __restrict__ wkids,
+                                                         const uint32_t* __restrict__ nkni,
+                                                         const uint32_t* __restrict__ blk_off,
+                                                         WNode<W>* __restrict__ wide,
+                                                         uint32_t* __restrict__ nfront,
+                                                         uint32_t* __restrict__ nstack,
+                                                         uint32_t* __restrict__ stats) {
+  __shared__ uint32_t s[COL_BLOCK];
+  const uint32_t t = threadIdx.x, i = blockIdx.x * COL_BLOCK + t;
+  const uint32_t code = i < m ? nkni[i] : 0u;
+  const uint32_t nk = code & 0xFFu, ni = code >> 8;
+  const uint32_t fs = i < m ? fstack[i] : 0u;
+  s[t] = ni;
+  __syncthreads();
+  for (uint32_t off = 1; off < COL_BLOCK; off <<= 1) {
+    const uint32_t x = t >= off ? s[t - off] : 0u;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  uint32_t smax_ = i < m ? fs + nk : 0u, sleaf = nk - ni;
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    smax_ = max(smax_, (uint32_t)__shfl_xor(smax_, o, WAVE));
+    sleaf += __shfl_xor(sleaf, o, WAVE);
+  }
+  if ((t & (WAVE - 1)) == 0) {
+    atomicMax(&stats[0], smax_);
+    if (sleaf) atomicAdd(&stats[1], sleaf);
+  }
+  if (i >= m) return;
+  const uint32_t here = fs + nk - 1u, next_base = base + m;
+  uint32_t j = blk_off[blockIdx.x] + s[t] - ni;
+  WNode<W>& w = wide[(size_t)base + i];
+  for (int k = 0; k < W; ++k) {
+    const uint32_t kid = wkids[((size_t)base + i) * W + k];
+    uint32_t ref;
+    if (kid == 0xFFFFFFFFu) {
+      ref = REF_EMPTY;
+    } else if (bn[kid].count) {
+      ref = make_leaf_ref(bn[kid].first, bn[kid].count);
+    } else {
+      ref = next_base + j;
+      nfront[j] = kid;
+      nstack[j] = here;
+      ++j;
+    }
+    w.child[k] = ref;
+  }
+}
+
+// quantize_node (bvh_host.cpp) for every wide node: the padded child boxes on
+// the node's power-of-two grid, rounded outwards and checked in double.
+template <int W>
+__global__ __launch_bounds__(COL_BLOCK) void k_col_quant(const Node2* __restrict__ bn,
+                                                         const uint32_t* __restrict__ wkids, uint32_t n_wide,
+                                                         float pad, WNode<W>* __restrict__ wide) {
+  const uint32_t i = blockIdx.x * COL_BLOCK + threadIdx.x;
+  if (i >= n_wide) return;
+  const uint32_t* kids = wkids + (size_t)i * W;
+  int nk = 0;
+  while (nk < W && kids[nk] != 0xFFFFFFFFu) ++nk;
+  float clo[W][3], chi[W][3], nlo[3] = {INFINITY, INFINITY, INFINITY};
+  float nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = 0; k < nk; ++k) {
+    const Node2& c = bn[kids[k]];
+    for (int a = 0; a < 3; ++a) {
+      clo[k][a] = c.lo[a] - pad;
+      chi[k][a] = c.hi[a] + pad;
+      nlo[a] = smin(nlo[a], clo[k][a]);
+      nhi[a] = smax(nhi[a], chi[k][a]);
+    }
+  }
+  WNode<W>& w = wide[i];
+  for (int a = 0; a < 3; ++a) {
+    const double lo = nk ? (double)nlo[a] : 0.0, ext = nk ? (double)nhi[a] - lo : 0.0;
+    int e = -126;
+    if (ext > 0.0) {
+      (void)frexp(ext / 255.0, &e);  // 2^(e-1) <= ext/255 < 2^e
+      e = e < -126 ? -126 : (e > 127 ? 127 : e);
+      while (e > -126 && ext <= 255.0 * ldexp(1.0, e - 1)) --e;
+      while (e < 127 && ext > 255.0 * ldexp(1.0, e)) ++e;
+    }
+    if (a == 0) {
+      w.ox = (float)lo;
+      w.ex = (uint8_t)(e + 127);
+    } else if (a == 1) {
+      w.oy = (float)lo;
+      w.ey = (uint8_t)(e + 127);
+    } else {
+      w.oz = (float)lo;
+      w.ez = (uint8_t)(e + 127);
+    }
+    const double step = ldexp(1.0, e);
+    for (int k = 0; k < W; ++k) {
+      if (k >= nk) {
+        w.q[a][k] = 255;
+        w.q[a + 3][k] = 0;
+        continue;
+      }
+      double ql = floor(((double)clo[k][a] - lo) / step);
+      double qh = ceil(((double)chi[k][a] - lo) / step);
+      ql = fmin(255.0, fmax(0.0, ql));
+      qh = fmin(255.0, fmax(0.0, qh));
+      while (ql > 0.0 && lo + ql * step > (double)clo[k][a]) ql -= 1.0;
+      while (qh < 255.0 && lo + qh * step < (double)chi[k][a]) qh += 1.0;
+      w.q[a][k] = (uint8_t)ql;
+      w.q[a + 3][k] = (uint8_t)qh;
+    }
+  }
+}
+
+// Per-block partial sums of the binary tree's SAH cost terms (relative area,
+// times the triangle count for a leaf), each block's nodes in index order.
+__global__ __launch_bounds__(COL_BLOCK) void k_col_sah(const Node2* __restrict__ bn, uint32_t n_nodes,
+                                                       double root_area, double* __restrict__ part) {
+  __shared__ double s[COL_BLOCK];
+  const uint32_t i = blockIdx.x * COL_BLOCK + threadIdx.x;
+  double v = 0.0;
+  if (i < n_nodes) {
+    const Node2& nd = bn[i];
+    const double a = (double)col_area(nd.lo, nd.hi) / root_area;
+    v = nd.count ? a * nd.count : a;
+  }
+  s[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (uint32_t k = 0; k < COL_BLOCK; ++k) t += s[k];
+    part[blockIdx.x] = t;
+  }
+}
+
 template <typename T>
 struct DBuf {
   T* p = nullptr;
@@ -687,22 +920,145 @@ struct DBuf {
 
 }  // namespace
 
-apt_status gather_tris_device(const float* d_tri_xyz, const uint32_t* h_order, uint32_t n, float4* d_tris,
-                              hipStream_t s) {
-  if (n == 0) return APT_OK;
-  DBuf<uint32_t> order;
-  APT_HIP(order.alloc(n));
-  APT_HIP(hipMemcpyAsync(order.p, h_order, (size_t)n * 4, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_gather_tris, dim3((n + 255) / 256), dim3(256), 0, s, d_tri_xyz, order.p, n, d_tris);
-  APT_HIP(hipGetLastError());
+namespace {
+
+// Collapses the binary tree bn[0, n_bin) (device) into W-wide quantized nodes
+// and fills `out` like the host path's upload_wide_bvh; d_order is the leaf
+// order of the triangles d_tri.
+template <int W>
+apt_status collapse_device(const Node2* bn, uint32_t n_bin, const uint32_t* d_order, const float* d_tri,
+                           uint32_t n, hipStream_t s, apt_bvh* out) {
+  Node2 root;
+  APT_HIP(hipMemcpyAsync(&root, bn, sizeof(Node2), hipMemcpyDeviceToHost, s));
   APT_HIP(hipStreamSynchronize(s));
+  // SAH cost relative to the root area (bvh2_sah_cost), summed per block of
+  // 256 nodes and then over the blocks in order
+  double sah_cost = 0.0;
+  {
+    const float dx = root.hi[0] - root.lo[0], dy = root.hi[1] - root.lo[1], dz = root.hi[2] - root.lo[2];
+    const double root_area = std::max(1e-30f, 2.0f * (dx * dy + dy * dz + dz * dx));
+    const uint32_t nb = (n_bin + COL_BLOCK - 1) / COL_BLOCK;
+    DBuf<double> part;
+    APT_HIP(part.alloc(nb));
+    hipLaunchKernelGGL(k_col_sah, dim3(nb), dim3(COL_BLOCK), 0, s, bn, n_bin, root_area, part.p);
+    std::vector<double> h(nb);
+    APT_HIP(hipMemcpyAsync(h.data(), part.p, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, s));
+    APT_HIP(hipStreamSynchronize(s));
+    for (double v : h) sah_cost += v;
+  }
+  // scene_pad: 2^-14 of the largest coordinate magnitude, which the root box holds
+  float m_abs = 0.0f;
+  for (int a = 0; a < 3; ++a) m_abs = std::max(m_abs, std::max(std::fabs(root.lo[a]), std::fabs(root.hi[a])));
+  const float pad = std::max(m_abs * (1.0f / 16384.0f), 1e-30f);
+
+  uint32_t root_ref = 0, n_wide = 0, depth = 0, stats_h[2] = {1u, 0u};
+  DBuf<WNode<W>> wide;
+  DBuf<uint32_t> wkids;
+  if (root.count) {
+    root_ref = make_leaf_ref(root.first, root.count);
+    stats_h[1] = 1;
+  } else {
+    const uint32_t cap_w = n_bin;  // wide nodes <= internal binary nodes
+    const uint32_t cap_f = n + 1;  // nodes of one level
+    DBuf<uint32_t> front[2], fstack[2], nkni, blk, stats, total;
+    APT_HIP(wide.alloc(cap_w));
+    APT_HIP(wkids.alloc((size_t)cap_w * W));
+    for (int k = 0; k < 2; ++k) {
+      APT_HIP(front[k].alloc(cap_f));
+      APT_HIP(fstack[k].alloc(cap_f));
+    }
+    APT_HIP(nkni.alloc(cap_f));
+    APT_HIP(blk.alloc((cap_f + COL_BLOCK - 1) / COL_BLOCK));
+    APT_HIP(stats.alloc(2));
+    APT_HIP(total.alloc(1));
+    APT_HIP(hipMemsetAsync(wide.p, 0, (size_t)cap_w * sizeof(WNode<W>), s));
+    const uint32_t zero = 0u;
+    APT_HIP(hipMemcpyAsync(front[0].p, &zero, 4, hipMemcpyHostToDevice, s));
+    APT_HIP(hipMemcpyAsync(fstack[0].p, &zero, 4, hipMemcpyHostToDevice, s));
+    APT_HIP(hipMemcpyAsync(stats.p, stats_h, sizeof(stats_h), hipMemcpyHostToDevice, s));
+    uint32_t m = 1, base = 0, cur = 0;
+    depth = 1;
+    while (true) {
+      APT_REQUIRE(base + m <= cap_w, APT_ERR_DEVICE, "bvh: device collapse ran out of nodes");
+      const uint32_t nb = (m + COL_BLOCK - 1) / COL_BLOCK;
+      hipLaunchKernelGGL((k_col_open<W>), dim3(nb), dim3(COL_BLOCK), 0, s, bn, front[cur].p, m, base, wkids.p,
+                         nkni.p, blk.p);
+      hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(SCAN_BLOCK), 0, s, blk.p, nb, total.p);
+      hipLaunchKernelGGL((k_col_write<W>), dim3(nb), dim3(COL_BLOCK), 0, s, bn, fstack[cur].p, m, base, wkids.p,
+                         nkni.p, blk.p, wide.p, front[cur ^ 1].p, fstack[cur ^ 1].p, stats.p);
+      uint32_t next = 0;
+      APT_HIP(hipMemcpyAsync(&next, total.p, 4, hipMemcpyDeviceToHost, s));
+      APT_HIP(hipStreamSynchronize(s));
+      base += m;
+      if (next == 0) break;
+      APT_REQUIRE(next <= cap_f, APT_ERR_DEVICE, "bvh: device collapse level overflow");
+      m = next;
+      cur ^= 1;
+      ++depth;
+    }
+    n_wide = base;
+    hipLaunchKernelGGL((k_col_quant<W>), dim3((n_wide + COL_BLOCK - 1) / COL_BLOCK), dim3(COL_BLOCK), 0, s, bn,
+                       wkids.p, n_wide, pad, wide.p);
+    APT_HIP(hipMemcpyAsync(stats_h, stats.p, sizeof(stats_h), hipMemcpyDeviceToHost, s));
+  }
+
+  const size_t node_bytes = std::max<size_t>(1, n_wide) * sizeof(WNode<W>);
+  const size_t tri_bytes = std::max<size_t>(1, (size_t)n * 3) * sizeof(float4);
+  void* d_nodes = nullptr;
+  void* d_tris = nullptr;
+  APT_HIP(hipMalloc(&d_nodes, node_bytes));
+  hipError_t e = hipMalloc(&d_tris, tri_bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(d_nodes);
+    set_error("hipMalloc of %zu triangle bytes failed: %s", tri_bytes, hipGetErrorString(e));
+    return APT_ERR_OOM;
+  }
+  e = n_wide ? hipMemcpyAsync(d_nodes, wide.p, (size_t)n_wide * sizeof(WNode<W>), hipMemcpyDeviceToDevice, s)
+             : hipSuccess;
+  if (e == hipSuccess && n) {
+    hipLaunchKernelGGL(k_gather_tris, dim3((n + 255) / 256), dim3(256), 0, s, d_tri, d_order, n,
+                       static_cast<float4*>(d_tris));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(d_nodes);
+    (void)hipFree(d_tris);
+    set_error("bvh: device collapse failed: %s", hipGetErrorString(e));
+    return APT_ERR_DEVICE;
+  }
+  out->width = W;
+  out->n_tris = n;
+  out->n_nodes = n_wide;
+  out->root = root_ref;
+  out->d_nodes = d_nodes;
+  out->d_tris = (float4*)d_tris;
+  e = hipMalloc(&out->d_counters, APT_BVH_COUNTERS * POOL_WORDS * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    out->d_counters = nullptr;
+    set_error("hipMalloc of the ray-pool counters failed: %s", hipGetErrorString(e));
+    return APT_ERR_OOM;
+  }
+  out->stats.n_tris = n;
+  out->stats.n_nodes = n_wide;
+  out->stats.n_leaves = stats_h[1];
+  out->stats.width = W;
+  out->stats.depth = n_wide ? depth : 0;
+  out->stats.max_stack = stats_h[0];
+  out->stats.device_bytes = node_bytes + tri_bytes;
+  out->stats.sah_cost = sah_cost;
   return APT_OK;
 }
 
-// Builds the binary tree on the device from device-resident triangles and
-// returns it (nodes, triangle order, SAH cost) like build_bvh2_host.
-apt_status build_bvh2_device(const float* d_tri, uint32_t n, uint32_t max_leaf, uint32_t bins,
-                             hipStream_t s, BvhHostResult& out) {
+}  // namespace
+
+
+// Builds the binary tree on the device from device-resident triangles, then
+// collapses it there (collapse_device): only level sizes and statistics come
+// back to the host.
+apt_status build_wide_device(const float* d_tri, uint32_t n, uint32_t max_leaf, uint32_t bins, uint32_t width,
+                             hipStream_t s, apt_bvh* out) {
+  APT_REQUIRE(width == 2 || width == 4 || width == 8, APT_ERR_INVALID_ARG, "bvh: width %u", width);
   APT_REQUIRE(bins <= (uint32_t)MAX_BINS, APT_ERR_INVALID_ARG, "bvh: bins %u above %d", bins, MAX_BINS);
   DBuf<float> prim, box[2], child_box;
   DBuf<uint32_t> order[2], node_ids[2], range[2], split, mid, counter, bad;
@@ -835,13 +1191,11 @@ apt_status build_bvh2_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
     depth++;
   }
   APT_HIP(hipGetLastError());
-  out.nodes.resize(node_base);
-  out.order.resize(n);
-  APT_HIP(hipMemcpyAsync(out.nodes.data(), nodes.p, (size_t)node_base * sizeof(Node2), hipMemcpyDeviceToHost, s));
-  APT_HIP(hipMemcpyAsync(out.order.data(), order[ord].p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
-  APT_HIP(hipStreamSynchronize(s));
-  out.sah_cost = bvh2_sah_cost(out.nodes);
-  return APT_OK;
+  switch (width) {
+    case 2: return collapse_device<2>(nodes.p, node_base, order[ord].p, d_tri, n, s, out);
+    case 4: return collapse_device<4>(nodes.p, node_base, order[ord].p, d_tri, n, s, out);
+    default: return collapse_device<8>(nodes.p, node_base, order[ord].p, d_tri, n, s, out);
+  }
 }
 
 }  // namespace apt

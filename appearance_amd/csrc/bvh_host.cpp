@@ -324,8 +324,7 @@ void quantize_node(WNode<W>& w, int nk, const float (&clo)[W][3], const float (&
 }
 
 template <int W>
-apt_status collapse_upload(const float* tri_xyz, const float* d_tri_xyz, uint32_t n, const BvhHostResult& b2,
-                           float pad, apt_bvh* out) {
+apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad, apt_bvh* out) {
   const std::vector<Node2>& bn = b2.nodes;
   std::vector<WNode<W>> wide;
   std::vector<uint32_t> wkids;  // the W BVH2 nodes under each wide node (~0u: empty slot)
@@ -443,21 +442,17 @@ apt_status collapse_upload(const float* tri_xyz, const float* d_tri_xyz, uint32_
     }
   });
 
-  // Triangles in leaf order: v0 | prim, v1, v2 as float4; gathered on the
-  // device when the caller's triangles are there.
-  std::vector<float4> tris;
-  if (!d_tri_xyz) {
-    tris.resize((size_t)n * 3);
-    parallel_for(n, [&](uint32_t i0, uint32_t i1) {
-      for (uint32_t i = i0; i < i1; ++i) {
-        const uint32_t p = b2.order[i];
-        const float* t = tri_xyz + (uint64_t)p * 9;
-        tris[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], u2f(p));
-        tris[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], 0.0f);
-        tris[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
-      }
-    });
-  }
+  // Triangles in leaf order: v0 | prim, v1, v2 as float4.
+  std::vector<float4> tris((size_t)n * 3);
+  parallel_for(n, [&](uint32_t i0, uint32_t i1) {
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t p = b2.order[i];
+      const float* t = tri_xyz + (uint64_t)p * 9;
+      tris[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], u2f(p));
+      tris[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], 0.0f);
+      tris[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+    }
+  });
 
   size_t node_bytes = std::max<size_t>(1, wide.size()) * sizeof(WNode<W>);
   size_t tri_bytes = std::max<size_t>(1, (size_t)n * 3) * sizeof(float4);
@@ -472,14 +467,6 @@ apt_status collapse_upload(const float* tri_xyz, const float* d_tri_xyz, uint32_
   }
   if (!wide.empty()) APT_HIP(hipMemcpy(d_nodes, wide.data(), wide.size() * sizeof(WNode<W>), hipMemcpyHostToDevice));
   if (!tris.empty()) APT_HIP(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
-  if (d_tri_xyz) {
-    apt_status gs = gather_tris_device(d_tri_xyz, b2.order.data(), n, static_cast<float4*>(d_tris), lib_stream());
-    if (gs != APT_OK) {
-      (void)hipFree(d_nodes);
-      (void)hipFree(d_tris);
-      return gs;
-    }
-  }
 
   out->width = W;
   out->n_tris = n;
@@ -506,12 +493,12 @@ apt_status collapse_upload(const float* tri_xyz, const float* d_tri_xyz, uint32_
 
 }  // namespace
 
-apt_status upload_wide_bvh(const float* tri_xyz, const float* d_tri_xyz, uint32_t n, const BvhHostResult& b2,
-                           uint32_t width, float pad, apt_bvh* out) {
+apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, uint32_t width, float pad,
+                           apt_bvh* out) {
   switch (width) {
-    case 2: return collapse_upload<2>(tri_xyz, d_tri_xyz, n, b2, pad, out);
-    case 4: return collapse_upload<4>(tri_xyz, d_tri_xyz, n, b2, pad, out);
-    case 8: return collapse_upload<8>(tri_xyz, d_tri_xyz, n, b2, pad, out);
+    case 2: return collapse_upload<2>(tri_xyz, n, b2, pad, out);
+    case 4: return collapse_upload<4>(tri_xyz, n, b2, pad, out);
+    case 8: return collapse_upload<8>(tri_xyz, n, b2, pad, out);
     default:
       set_error("BVH width %u is not one of 2, 4, 8", width);
       return APT_ERR_INVALID_ARG;
