@@ -15,8 +15,9 @@
 // steps over many blocks instead of one wave.  A single-block scan then
 // numbers the children of the splitting nodes and forms the next level.  Every choice depends only on the set of triangles in
 // a node and their input order, so the binary tree, its leaves and their
-// triangle order equal the host builder's (which uses a stable partition too);
-// the breadth-first collapse to 2/4/8-wide nodes is shared with it.
+// triangle order equal the host builder's (which uses a stable partition too).
+// The breadth-first collapse to 2/4/8-wide quantized nodes then runs on the
+// device as well (collapse_device), byte-identical to the host collapse.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
