@@ -189,6 +189,78 @@ __device__ void sah_sweep(const int (*blo)[MAX_BINS][3], const int (*bhi)[MAX_BI
   nleft_out = best_left;
 }
 
+// The same sweep with one lane per bin, for the one-wave-per-node kernel:
+// prefix and suffix unions and counts of the bins by wave scans (min, max and
+// integer adds are exact, so every boundary sees the boxes and counts of the
+// sequential sweep), each boundary's cost by the same float expression, and
+// the first strictly smallest cost in (axis, boundary) order by a wave
+// reduction.  Every lane of the wave calls it and gets the result.
+__device__ void sah_sweep_wave(const int (*blo)[MAX_BINS][3], const int (*bhi)[MAX_BINS][3],
+                               const uint32_t (*bcnt)[MAX_BINS], const bool* use, int bins, uint32_t lane,
+                               int& axis_out, int& split_out, uint32_t& nleft_out) {
+  float best_cost = INFINITY;
+  int best_key = 0x7FFFFFFF;  // axis * MAX_BINS + boundary
+  uint32_t best_left = 0;
+  const bool in = (int)lane < bins;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!use[ax]) continue;
+    float plo[3], phi[3], slo[3], shi[3];
+    uint32_t pc = in ? bcnt[ax][lane] : 0u, sc = pc;
+    for (int k = 0; k < 3; ++k) {
+      plo[k] = slo[k] = in ? ofloat(blo[ax][lane][k]) : INFINITY;
+      phi[k] = shi[k] = in ? ofloat(bhi[ax][lane][k]) : -INFINITY;
+    }
+    // inclusive prefix (bins 0..lane) and suffix (bins lane..) scans
+    for (uint32_t d = 1; d < (uint32_t)WAVE; d <<= 1) {
+      const bool up = lane >= d, down = lane + d < (uint32_t)WAVE;
+      for (int k = 0; k < 3; ++k) {
+        const float ul = __shfl_up(plo[k], d), uh = __shfl_up(phi[k], d);
+        const float dl = __shfl_down(slo[k], d), dh = __shfl_down(shi[k], d);
+        if (up) {
+          plo[k] = smin(ul, plo[k]);
+          phi[k] = smax(uh, phi[k]);
+        }
+        if (down) {
+          slo[k] = smin(slo[k], dl);
+          shi[k] = smax(shi[k], dh);
+        }
+      }
+      const uint32_t uc = __shfl_up(pc, d), dc = __shfl_down(sc, d);
+      if (up) pc += uc;
+      if (down) sc += dc;
+    }
+    // boundary b = lane: left = bins 0..b-1 (the prefix one lane down), right = bins b..
+    float llo[3], lhi[3];
+    for (int k = 0; k < 3; ++k) {
+      llo[k] = __shfl_up(plo[k], 1);
+      lhi[k] = __shfl_up(phi[k], 1);
+    }
+    const uint32_t lc = __shfl_up(pc, 1);
+    if (lane >= 1 && in && lc != 0 && sc != 0) {
+      const float cost = box_area(llo, lhi) * (float)lc + box_area(slo, shi) * (float)sc;
+      if (cost < best_cost) {  // a lane's earlier axis wins a tie
+        best_cost = cost;
+        best_key = ax * MAX_BINS + (int)lane;
+        best_left = lc;
+      }
+    }
+  }
+  for (int m = WAVE / 2; m >= 1; m >>= 1) {
+    const float oc = __shfl_xor(best_cost, m);
+    const int ok = __shfl_xor(best_key, m);
+    const uint32_t ol = __shfl_xor(best_left, m);
+    if (oc < best_cost || (oc == best_cost && ok < best_key)) {
+      best_cost = oc;
+      best_key = ok;
+      best_left = ol;
+    }
+  }
+  const bool found = best_cost < INFINITY;
+  axis_out = found ? best_key / MAX_BINS : -1;
+  split_out = found ? best_key % MAX_BINS : 0;
+  nleft_out = found ? best_left : 0u;
+}
+
 // Binning set-up of a node from its centroid box: the axes that can split
 // (non-flat, above the depth limit) and their bin scale.
 __device__ __forceinline__ void bin_setup(const float* box, uint32_t depth, int bins, float* cblo, float* scale,
@@ -261,7 +333,14 @@ __global__ __launch_bounds__(WAVE) void k_level(Level L) {
       }
     }
     __syncthreads();
-    if (lane == 0) sah_sweep(s_lo, s_hi, s_cnt, use, bins, s_axis, s_split, s_nleft);
+    int ax_w, split_w;
+    uint32_t nleft_w;
+    sah_sweep_wave(s_lo, s_hi, s_cnt, use, bins, lane, ax_w, split_w, nleft_w);
+    if (lane == 0) {
+      s_axis = ax_w;
+      s_split = split_w;
+      s_nleft = nleft_w;
+    }
   }
   __syncthreads();
   const int axis = s_axis, split = s_split;
