@@ -72,8 +72,26 @@ __global__ void k_prims(const float* __restrict__ tri, uint32_t n, float* __rest
   if (!ok) atomicMin(bad, i);
 }
 
-// Root triangle and centroid bounds: keys[0..5] box lo/hi, keys[6..11] centroid lo/hi.
-__global__ void k_root_bounds(const float* __restrict__ prim, uint32_t n, int* __restrict__ keys) {
+// Root triangle and centroid bounds as order-preserving keys ([0..5] box
+// lo/hi, [6..11] centroid lo/hi): every work-group reduces its share to one
+// set of 12 keys in `part` (integer min / max: exact), and k_root_init
+// reduces those.  (Atomics on 12 global words from every wave would queue
+// thousands of adds on each word.)
+constexpr int ROOT_BLOCK = 256;
+constexpr uint32_t ROOT_GRID = 1024;
+
+__device__ __forceinline__ void reduce_keys_wave(int (&key)[12]) {
+  for (int m = WAVE / 2; m >= 1; m >>= 1) {
+    for (int k = 0; k < 12; ++k) {
+      const int o = __shfl_xor(key[k], m);
+      key[k] = (k % 6) < 3 ? min(key[k], o) : max(key[k], o);
+    }
+  }
+}
+
+__global__ __launch_bounds__(ROOT_BLOCK) void k_root_bounds(const float* __restrict__ prim, uint32_t n,
+                                                             int* __restrict__ part) {
+  __shared__ int s_key[ROOT_BLOCK / WAVE][12];
   float v[12];
   for (int k = 0; k < 3; ++k) {
     v[k] = v[6 + k] = INFINITY;
@@ -88,20 +106,34 @@ __global__ void k_root_bounds(const float* __restrict__ prim, uint32_t n, int* _
       v[9 + k] = smax(v[9 + k], p[6 + k]);
     }
   }
-  for (int k = 0; k < 3; ++k) {
-    atomicMin(&keys[k], okey(v[k]));
-    atomicMax(&keys[3 + k], okey(v[3 + k]));
-    atomicMin(&keys[6 + k], okey(v[6 + k]));
-    atomicMax(&keys[9 + k], okey(v[9 + k]));
+  int key[12];
+  for (int k = 0; k < 12; ++k) key[k] = okey(v[k]);
+  reduce_keys_wave(key);
+  if ((threadIdx.x & (WAVE - 1)) == 0)
+    for (int k = 0; k < 12; ++k) s_key[threadIdx.x / WAVE][k] = key[k];
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int k = (int)threadIdx.x;
+    int r = s_key[0][k];
+    for (int w = 1; w < ROOT_BLOCK / WAVE; ++w) r = (k % 6) < 3 ? min(r, s_key[w][k]) : max(r, s_key[w][k]);
+    part[(size_t)blockIdx.x * 12 + k] = r;
   }
 }
 
-__global__ void k_root_init(const int* __restrict__ keys, uint32_t n, uint32_t* act_node, uint32_t* act_range,
-                            float* act_box) {
-  act_node[0] = 0;
-  act_range[0] = 0;
-  act_range[1] = n;
-  for (int k = 0; k < 12; ++k) act_box[k] = ofloat(keys[k]);
+// One wave: reduces the work-groups' keys and sets up the root's range and box.
+__global__ __launch_bounds__(WAVE) void k_root_init(const int* __restrict__ part, uint32_t n_part, uint32_t n,
+                                                    uint32_t* act_node, uint32_t* act_range, float* act_box) {
+  int key[12];
+  for (int k = 0; k < 12; ++k) key[k] = (k % 6) < 3 ? 0x7FFFFFFF : (int)0x80000000;
+  for (uint32_t j = threadIdx.x; j < n_part; j += WAVE)
+    for (int k = 0; k < 12; ++k) key[k] = (k % 6) < 3 ? min(key[k], part[j * 12 + k]) : max(key[k], part[j * 12 + k]);
+  reduce_keys_wave(key);
+  if (threadIdx.x == 0) {
+    act_node[0] = 0;
+    act_range[0] = 0;
+    act_range[1] = n;
+    for (int k = 0; k < 12; ++k) act_box[k] = ofloat(key[k]);
+  }
 }
 
 struct Level {
@@ -135,66 +167,15 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// SAH sweep of the host builder over three axes of bins (lo/hi box keys and
-// counts): right-to-left areas and counts, then left-to-right costs; the
-// first strictly smaller cost wins.  One thread.
-__device__ void sah_sweep(const int (*blo)[MAX_BINS][3], const int (*bhi)[MAX_BINS][3],
-                          const uint32_t (*bcnt)[MAX_BINS], const bool* use, int bins, int& axis_out,
-                          int& split_out, uint32_t& nleft_out) {
-  float best_cost = INFINITY;
-  int best_axis = -1, best_split = 0;
-  uint32_t best_left = 0;
-  float right_area[MAX_BINS];
-  uint32_t right_cnt[MAX_BINS];
-  for (int ax = 0; ax < 3; ++ax) {
-    if (!use[ax]) continue;
-    GBox acc;
-    for (int k = 0; k < 3; ++k) {
-      acc.lo[k] = INFINITY;
-      acc.hi[k] = -INFINITY;
-    }
-    uint32_t cnt = 0;
-    for (int b = bins - 1; b > 0; --b) {
-      for (int k = 0; k < 3; ++k) {
-        acc.lo[k] = smin(acc.lo[k], ofloat(blo[ax][b][k]));
-        acc.hi[k] = smax(acc.hi[k], ofloat(bhi[ax][b][k]));
-      }
-      cnt += bcnt[ax][b];
-      right_area[b] = box_area(acc.lo, acc.hi);
-      right_cnt[b] = cnt;
-    }
-    for (int k = 0; k < 3; ++k) {
-      acc.lo[k] = INFINITY;
-      acc.hi[k] = -INFINITY;
-    }
-    cnt = 0;
-    for (int b = 1; b < bins; ++b) {
-      for (int k = 0; k < 3; ++k) {
-        acc.lo[k] = smin(acc.lo[k], ofloat(blo[ax][b - 1][k]));
-        acc.hi[k] = smax(acc.hi[k], ofloat(bhi[ax][b - 1][k]));
-      }
-      cnt += bcnt[ax][b - 1];
-      if (cnt == 0 || right_cnt[b] == 0) continue;
-      const float cost = box_area(acc.lo, acc.hi) * (float)cnt + right_area[b] * (float)right_cnt[b];
-      if (cost < best_cost) {
-        best_cost = cost;
-        best_axis = ax;
-        best_split = b;
-        best_left = cnt;
-      }
-    }
-  }
-  axis_out = best_axis;
-  split_out = best_split;
-  nleft_out = best_left;
-}
-
-// The same sweep with one lane per bin, for the one-wave-per-node kernel:
-// prefix and suffix unions and counts of the bins by wave scans (min, max and
+// The host builder's SAH sweep over three axes of bins (lo/hi box keys and
+// counts) with one lane per bin: for every boundary b the cost is
+// area(bins < b) * count(bins < b) + area(bins >= b) * count(bins >= b), and
+// the first strictly smallest cost in (axis, boundary) order wins.  The
+// prefix and suffix unions and counts come from wave scans (min, max and
 // integer adds are exact, so every boundary sees the boxes and counts of the
-// sequential sweep), each boundary's cost by the same float expression, and
-// the first strictly smallest cost in (axis, boundary) order by a wave
-// reduction.  Every lane of the wave calls it and gets the result.
+// host's sequential sweep), each cost from the same float expression, and the
+// winner from a wave reduction.  Every lane of the wave calls it and gets the
+// result.
 __device__ void sah_sweep_wave(const int (*blo)[MAX_BINS][3], const int (*bhi)[MAX_BINS][3],
                                const uint32_t (*bcnt)[MAX_BINS], const bool* use, int bins, uint32_t lane,
                                int& axis_out, int& split_out, uint32_t& nleft_out) {
@@ -495,7 +476,8 @@ __global__ __launch_bounds__(LBLOCK) void k_large_bin(Level L, Large G) {
   }
 }
 
-__global__ void k_large_sweep(Level L, Large G) {
+// One wave per large node (the bins are merged in global memory).
+__global__ __launch_bounds__(WAVE) void k_large_sweep(Level L, Large G) {
   const uint32_t j = blockIdx.x, a = G.act[j];
   const uint32_t n = L.act_range[2 * a + 1] - L.act_range[2 * a];
   float cblo[3], scale[3];
@@ -505,12 +487,15 @@ __global__ void k_large_sweep(Level L, Large G) {
   uint32_t nleft = 0;
   int* gb = G.bins + (size_t)j * BIN_WORDS;
   if (use[0] || use[1] || use[2])
-    sah_sweep(reinterpret_cast<const int (*)[MAX_BINS][3]>(bins_lo(gb)),
-              reinterpret_cast<const int (*)[MAX_BINS][3]>(bins_hi(gb)),
-              reinterpret_cast<const uint32_t (*)[MAX_BINS]>(bins_cnt(gb)), use, (int)L.bins, axis, split, nleft);
-  G.info[4 * j + 0] = axis;
-  G.info[4 * j + 1] = split;
-  G.info[4 * j + 2] = (int)(axis >= 0 ? nleft : n / 2);  // object median when nothing splits
+    sah_sweep_wave(reinterpret_cast<const int (*)[MAX_BINS][3]>(bins_lo(gb)),
+                   reinterpret_cast<const int (*)[MAX_BINS][3]>(bins_hi(gb)),
+                   reinterpret_cast<const uint32_t (*)[MAX_BINS]>(bins_cnt(gb)), use, (int)L.bins, threadIdx.x, axis,
+                   split, nleft);
+  if (threadIdx.x == 0) {
+    G.info[4 * j + 0] = axis;
+    G.info[4 * j + 1] = split;
+    G.info[4 * j + 2] = (int)(axis >= 0 ? nleft : n / 2);  // object median when nothing splits
+  }
 }
 
 // The split test of large node j (axis, split bin, or the object median).
@@ -1161,17 +1146,16 @@ apt_status build_wide_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
 
   APT_HIP(bad.alloc(1));
   APT_HIP(nodes.alloc(cap_nodes));
-  APT_HIP(keys.alloc(12));
+  APT_HIP(keys.alloc(12 * (size_t)ROOT_GRID));  // per-work-group root keys
 
   const uint32_t nb = (n + 255) / 256;
   const uint32_t bad_init = 0xFFFFFFFFu;
   APT_HIP(hipMemcpyAsync(bad.p, &bad_init, 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_prims, dim3(nb), dim3(256), 0, s, d_tri, n, prim.p, order[0].p, bad.p);
-  int kinit[12];
-  for (int k = 0; k < 12; ++k) kinit[k] = ((k % 6) < 3) ? 0x7FFFFFFF : (int)0x80000000;
-  APT_HIP(hipMemcpyAsync(keys.p, kinit, sizeof(kinit), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_root_bounds, dim3(std::min(nb, 1024u)), dim3(256), 0, s, prim.p, n, keys.p);
-  hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, s, keys.p, n, node_ids[0].p, range[0].p, box[0].p);
+  const uint32_t n_part = std::min(nb, ROOT_GRID);
+  hipLaunchKernelGGL(k_root_bounds, dim3(n_part), dim3(ROOT_BLOCK), 0, s, prim.p, n, keys.p);
+  hipLaunchKernelGGL(k_root_init, dim3(1), dim3(WAVE), 0, s, keys.p, n_part, n, node_ids[0].p, range[0].p,
+                     box[0].p);
   uint32_t bad_h = 0;
   APT_HIP(hipMemcpyAsync(&bad_h, bad.p, 4, hipMemcpyDeviceToHost, s));
   APT_HIP(hipStreamSynchronize(s));
@@ -1247,7 +1231,7 @@ apt_status build_wide_device(const float* d_tri, uint32_t n, uint32_t max_leaf, 
       G.blocks = blocks;
       hipLaunchKernelGGL(k_large_init, dim3(n_large), dim3(LBLOCK), 0, s, G);
       hipLaunchKernelGGL(k_large_bin, dim3(blocks), dim3(LBLOCK), 0, s, L, G);
-      hipLaunchKernelGGL(k_large_sweep, dim3(n_large), dim3(1), 0, s, L, G);
+      hipLaunchKernelGGL(k_large_sweep, dim3(n_large), dim3(WAVE), 0, s, L, G);
       hipLaunchKernelGGL(k_large_count, dim3(blocks), dim3(LBLOCK), 0, s, L, G);
       hipLaunchKernelGGL(k_large_scan, dim3(1), dim3(1), 0, s, G);
       hipLaunchKernelGGL(k_large_scatter, dim3(blocks), dim3(LBLOCK), 0, s, L, G);
