@@ -133,8 +133,8 @@ struct apt_bvh {
 constexpr uint32_t APT_BVH_COUNTERS = 64;  // ray pools in a BVH's ring
 
 // A persistent launch's ray pool: POOL_PARTS counters POOL_STRIDE words apart
-// (one range of rays per XCD, see pool_take in trace_device.h).
-constexpr uint32_t POOL_PARTS = 8;
+// (two ranges of rays per XCD, see pool_take in trace_device.h).
+constexpr uint32_t POOL_PARTS = 16;
 constexpr uint32_t POOL_STRIDE = 16;
 constexpr uint32_t POOL_WORDS = POOL_PARTS * POOL_STRIDE;
 
