@@ -87,7 +87,7 @@ typedef struct apt_hit {
 
 typedef enum apt_builder {
   APT_BUILDER_HOST = 0,  /* binned SAH on the host threads of libapt */
-  APT_BUILDER_GPU = 1    /* binned SAH on the device */
+  APT_BUILDER_GPU = 1    /* binned SAH, collapse and quantization on the device; the same bytes */
 } apt_builder;
 
 typedef struct apt_bvh_opts {
@@ -109,7 +109,8 @@ typedef struct apt_bvh_stats {
   uint32_t max_stack;   /* worst-case traversal stack entries */
   uint64_t device_bytes;
   double build_ms;
-  double sah_cost;      /* SAH cost of the binary tree before collapse */
+  double sah_cost;      /* SAH cost of the binary tree before collapse (the device builder sums
+                           it per 256 nodes, so it may differ from the host's in the last bits) */
 } apt_bvh_stats;
 
 typedef struct apt_bvh apt_bvh;
@@ -119,7 +120,8 @@ typedef struct apt_bvh apt_bvh;
  * indices. */
 apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
                          apt_bvh** out);
-/* Same, with tri_xyz already resident in device memory. */
+/* Same, with tri_xyz already resident in device memory (with APT_BUILDER_GPU
+ * nothing but level sizes and statistics crosses to the host). */
 apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris,
                                 const apt_bvh_opts* opts, apt_bvh** out);
 void apt_bvh_destroy(apt_bvh* bvh);

@@ -43,6 +43,26 @@ def test_device_build_hits(gpu_available):
     assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+def test_device_build_small_scenes(gpu_available):
+    """Tiny triangle counts (a single-leaf root, one or two wide levels, a
+    level narrower than its wide nodes) and a scene of exact duplicates:
+    the device build and collapse give the host's bytes and statistics."""
+    rng = np.random.default_rng(31)
+    cases = {n: rng.uniform(-1, 1, (n, 9)).astype(np.float32) for n in (1, 2, 3, 5, 17, 64, 257)}
+    cases["dup"] = np.repeat(rng.uniform(-1, 1, (1, 9)).astype(np.float32), 40, axis=0)
+    for name, tris in cases.items():
+        for w in (2, 4, 8):
+            print(f"[bvh] small scene {name} W={w}", flush=True)
+            h = Bvh(tris, width=w, builder=BUILDER_HOST)
+            g = Bvh(tris, width=w, builder=BUILDER_GPU)
+            sh, sg = h.stats(), g.stats()
+            for k in ("n_nodes", "n_leaves", "depth", "max_stack"):
+                assert sh[k] == sg[k], (name, w, k, sh[k], sg[k])
+            nh, th = h.export()
+            ng, tg = g.export()
+            assert np.array_equal(nh, ng) and np.array_equal(th.view(np.uint32), tg.view(np.uint32))
+
+
 def test_device_build_rejects_bad_input(gpu_available):
     tris = np.zeros((4, 9), np.float32)
     tris[2, 4] = np.nan
