@@ -418,9 +418,10 @@ __global__ __launch_bounds__(BLOCK) void k_rs_raygen(RestirParams p, SceneView s
 // memory latency.  Left to itself the compiler uses ~210 registers and runs
 // two waves, 15% slower end to end on the 1080p ReSTIR frame.
 constexpr int RS_TRACE_WAVES = 4;
-// The DI RIS (33 BSDF evaluations, its rays traced by the pool) runs best at
-// three waves, where it needs no spills (1.70 ms against 1.85 ms at four).
-constexpr int RS_RIS_WAVES = 3;
+// The DI RIS (33 BSDF evaluations, its rays traced by the pool) fits four
+// waves per SIMD without spills once its first round (area candidate 0 and
+// the BSDF candidate) is peeled off the loop; five spill.
+constexpr int RS_RIS_WAVES = 4;
 
 // ---- ray stages --------------------------------------------------------------
 // The rays that dominate the frame (the path's extension ray, the BSDF
@@ -550,68 +551,72 @@ __global__ __launch_bounds__(BLOCK, RS_RIS_WAVES) void k_rs_di_ris(RestirParams 
   uint32_t rng = b.rng[pix];
   const float p_sun = sun_pick(sc);
   const float na = (float)p.di_candidates, nb = 1.0f;
-  const uint32_t iters = p.di_candidates > 1u ? p.di_candidates : 1u;
   DiRes r = di_empty();
-  for (uint32_t i = 0; i < iters; ++i) {
-    uint32_t a_light = LIGHT_EMPTY;
-    float a_u = 0.0f, a_v = 0.0f, a_phat = 0.0f, a_pl = 0.0f, a_pb = 0.0f;
-    if (i < p.di_candidates) {
-      const float r0 = rng_next(rng);
-      rng_next(rng);  // r1: instance pick
-      const float r2 = rng_next(rng), r3 = rng_next(rng), r4 = rng_next(rng);
-      sample_light(sc, r0, r2, r3, r4, a_light, a_u, a_v);
+  // One area-light candidate (five draws): light, (u, v), target and the two
+  // densities of its direction.
+  const auto light_candidate = [&](uint32_t& light, float& u, float& w, float& phat, float& pl, float& pb) {
+    light = LIGHT_EMPTY;
+    u = w = phat = pl = pb = 0.0f;
+    const float r0 = rng_next(rng);
+    rng_next(rng);  // r1: instance pick
+    const float r2 = rng_next(rng), r3 = rng_next(rng), r4 = rng_next(rng);
+    sample_light(sc, r0, r2, r3, r4, light, u, w);
+    LightHit e;
+    if (eval_light(sc, light, u, w, v.x, e)) {
+      pl = e.pdf;
+      const float c = dot(e.wi, v.n);
+      if (c > 0.0f && pl > 0.0f) {
+        const v3 f = bsdf_eval(bs, v.wo, e.wi, pb);
+        if (pb > 0.0f) phat = luma(f * e.Le * c);
+      }
+    }
+  };
+  const auto update = [&](uint32_t light, float u, float w_, float phat, float pl, float pb) {
+    const float w = pl > 0.0f ? phat / (na * pl + nb * pb) : 0.0f;
+    if (res_update(r.wsum, r.M, w, rng)) {
+      r.light = light; r.u = u; r.v = w_; r.phat = phat;
+    }
+  };
+  // First round: area candidate 0, then the BSDF candidate (three draws),
+  // then their two reservoir updates; later rounds: one area candidate each.
+  uint32_t a_light = LIGHT_EMPTY;
+  float a_u = 0.0f, a_v = 0.0f, a_phat = 0.0f, a_pl = 0.0f, a_pb = 0.0f;
+  if (p.di_candidates > 0u) light_candidate(a_light, a_u, a_v, a_phat, a_pl, a_pb);
+  uint32_t b_light = LIGHT_EMPTY;
+  float b_u = 0.0f, b_v = 0.0f, b_phat = 0.0f, b_pl = 0.0f, b_pb = 0.0f;
+  {
+    const float r0 = rng_next(rng), r1 = rng_next(rng), r2 = rng_next(rng);
+    const BsdfSample s = bsdf_sample(bs, v.wo, r0, r1, r2);
+    b_pb = s.pdf;
+    const v3 contrib = s.f * fabsf(dot(s.wi, v.n));
+    if (s.pdf > 0.0f && dot(contrib, contrib) > 0.0f) {
+      const float4 hq = b.rq_hit[pix];  // the ray k_rs_di_bsdf_ray queued for this sample
+      HitRec h;
+      h.u = hq.y;
+      h.v = hq.z;
+      h.prim = f2u(hq.w);
+      if (h.prim != INVALID) {
+        if (has_emission(sc.mats[sc.tri_mat[h.prim]])) {
+          b_light = h.prim;
+          b_u = h.u;
+          b_v = h.v;
+        }
+      } else if (p_sun > 0.0f) {
+        b_light = LIGHT_SUN;
+        sun_uv(sc, s.wi, b_u, b_v);
+      }
       LightHit e;
-      if (eval_light(sc, a_light, a_u, a_v, v.x, e)) {
-        a_pl = e.pdf;
-        const float c = dot(e.wi, v.n);
-        if (c > 0.0f && a_pl > 0.0f) {
-          const v3 f = bsdf_eval(bs, v.wo, e.wi, a_pb);
-          if (a_pb > 0.0f) a_phat = luma(f * e.Le * c);
-        }
+      if (b_light != LIGHT_EMPTY && eval_light(sc, b_light, b_u, b_v, v.x, e)) {
+        b_pl = e.pdf;
+        b_phat = luma(contrib * e.Le);
       }
     }
-    uint32_t b_light = LIGHT_EMPTY;
-    float b_u = 0.0f, b_v = 0.0f, b_phat = 0.0f, b_pl = 0.0f, b_pb = 0.0f;
-    if (i == 0) {
-      const float r0 = rng_next(rng), r1 = rng_next(rng), r2 = rng_next(rng);
-      const BsdfSample s = bsdf_sample(bs, v.wo, r0, r1, r2);
-      b_pb = s.pdf;
-      const v3 contrib = s.f * fabsf(dot(s.wi, v.n));
-      if (s.pdf > 0.0f && dot(contrib, contrib) > 0.0f) {
-        const float4 hq = b.rq_hit[pix];  // the ray k_rs_di_bsdf_ray queued for this sample
-        HitRec h;
-        h.u = hq.y;
-        h.v = hq.z;
-        h.prim = f2u(hq.w);
-        if (h.prim != INVALID) {
-          if (has_emission(sc.mats[sc.tri_mat[h.prim]])) {
-            b_light = h.prim;
-            b_u = h.u;
-            b_v = h.v;
-          }
-        } else if (p_sun > 0.0f) {
-          b_light = LIGHT_SUN;
-          sun_uv(sc, s.wi, b_u, b_v);
-        }
-        LightHit e;
-        if (b_light != LIGHT_EMPTY && eval_light(sc, b_light, b_u, b_v, v.x, e)) {
-          b_pl = e.pdf;
-          b_phat = luma(contrib * e.Le);
-        }
-      }
-    }
-    if (i < p.di_candidates) {
-      const float w = a_pl > 0.0f ? a_phat / (na * a_pl + nb * a_pb) : 0.0f;
-      if (res_update(r.wsum, r.M, w, rng)) {
-        r.light = a_light; r.u = a_u; r.v = a_v; r.phat = a_phat;
-      }
-    }
-    if (i == 0) {
-      const float w = b_pb > 0.0f ? b_phat / (na * b_pl + nb * b_pb) : 0.0f;
-      if (res_update(r.wsum, r.M, w, rng)) {
-        r.light = b_light; r.u = b_u; r.v = b_v; r.phat = b_phat;
-      }
-    }
+  }
+  if (p.di_candidates > 0u) update(a_light, a_u, a_v, a_phat, a_pl, a_pb);
+  update(b_light, b_u, b_v, b_phat, b_pl, b_pb);
+  for (uint32_t i = 1; i < p.di_candidates; ++i) {
+    light_candidate(a_light, a_u, a_v, a_phat, a_pl, a_pb);
+    update(a_light, a_u, a_v, a_phat, a_pl, a_pb);
   }
   // W = wsum / phat once the winner's shadow ray (queued here) finds it
   // visible: k_rs_di_vis
