@@ -90,7 +90,7 @@ struct RestirState {
   DevBuf<GiRes> gi, gi_hist[2];
   DevBuf<GTexel> gbuf[2];
   DevBuf<float2> mom[2];
-  DevBuf<float> var[2];
+  DevBuf<uint2> guide;
   size_t npix = 0;
 
   template <typename F>
@@ -98,7 +98,7 @@ struct RestirState {
     f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(demod[0]); f(demod[1]);
     f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
     f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
-    f(var[0]); f(var[1]); f(rq_o); f(rq_d); f(rq_hit);
+    f(guide); f(rq_o); f(rq_d); f(rq_hit);
   }
   // Sizes every buffer for n pixels and clears them (no history).
   apt_status reset(size_t n, hipStream_t s) {
@@ -130,7 +130,7 @@ struct RestirState {
     b.gbuf[0] = gbuf[0].p; b.gbuf[1] = gbuf[1].p;
     b.demod[0] = demod[0].p; b.demod[1] = demod[1].p; b.work = work.p;
     b.svgf_col[0] = col[0].p; b.svgf_col[1] = col[1].p; b.svgf_mom[0] = mom[0].p; b.svgf_mom[1] = mom[1].p;
-    b.svgf_var[0] = var[0].p; b.svgf_var[1] = var[1].p; b.svgf_count = count.p; b.svgf_hist = hist.p;
+    b.svgf_guide = guide.p; b.svgf_count = count.p; b.svgf_hist = hist.p;
     b.counters = counters.p;
     b.rq_o = rq_o.p; b.rq_d = rq_d.p; b.rq_hit = rq_hit.p; b.rq_ctr = rq_ctr.p;
     return b;

@@ -84,9 +84,10 @@ struct RestirBuffers {
   // post chain
   float4* demod[2];     // demodulated radiance: [cur] this frame, [cur ^ 1] the previous final
   float4* work;         // scratch image (firefly / TAA output)
-  float4* svgf_col[2];  // temporal / a-trous ping-pong
+  float4* svgf_col[2];  // temporal / a-trous ping-pong: colour, variance in w
   float2* svgf_mom[2];
-  float* svgf_var[2];
+  uint2* svgf_guide;    // the a-trous edge stops of this frame's texels: the G-buffer's FP16
+                        // depth and normal words, x = ~0u for sky
   uint32_t* svgf_count;
   float4* svgf_hist;    // a-trous pass-0 output kept for the next frame
   uint32_t* counters;   // closest-hit rays, then shadow rays traced: RS_CTR_SLOTS partial sums each

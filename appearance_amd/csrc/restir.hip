@@ -1006,6 +1006,8 @@ __global__ __launch_bounds__(BLOCK) void k_rs_firefly(RestirParams p, RestirBuff
   out[pix] = v4(rec, 0.0f);
 }
 
+constexpr uint32_t SVGF_SKY = 0xFFFFFFFFu;  // guide word of a sky texel (an FP16 NaN depth)
+
 // svgf_temporal.wgsl:87-139.
 __global__ __launch_bounds__(BLOCK) void k_svgf_temporal(RestirParams p, RestirBuffers b, const float4* in) {
   uint32_t x, y, pix;
@@ -1013,11 +1015,12 @@ __global__ __launch_bounds__(BLOCK) void k_svgf_temporal(RestirParams p, RestirB
   const float4 rad4 = in[pix];
   const GTexel cg = b.gbuf[p.cur][pix];
   if (cg.pos_depth.w == 0.0f) {
-    b.svgf_col[0][pix] = rad4;
-    b.svgf_var[0][pix] = 0.0f;
+    b.svgf_col[0][pix] = v4(f4v(rad4), 0.0f);  // variance 0
+    b.svgf_guide[pix] = make_uint2(SVGF_SKY, 0u);
     b.svgf_mom[p.cur][pix] = make_float2(0.0f, 0.0f);
     return;
   }
+  b.svgf_guide[pix] = make_uint2(cg.packed.x, cg.packed.y);
   const Tex cur = unpack_texel(cg);
   const v3 rad = f4v(rad4);
   const float l = luma(rad);
@@ -1044,59 +1047,66 @@ __global__ __launch_bounds__(BLOCK) void k_svgf_temporal(RestirParams p, RestirB
   m1 = lerpf(m1, l, alpha);
   m2 = lerpf(m2, l * l, alpha);
   tc = lerp3(tc, rad, alpha);
-  b.svgf_var[0][pix] = fmaxf(m2 - m1 * m1, 0.0f);
-  b.svgf_col[0][pix] = v4(tc, 0.0f);
+  // the variance rides in w for the a-trous passes (0 when there are none:
+  // the image goes on as it is)
+  b.svgf_col[0][pix] = v4(tc, p.atrous_passes ? fmaxf(m2 - m1 * m1, 0.0f) : 0.0f);
   b.svgf_mom[p.cur][pix] = make_float2(m1, m2);
   b.svgf_count[pix] = count;
 }
 
 // svgf_atrous.wgsl:48-117: pass `pass` reads ping-pong slot pass & 1.  Sky
-// pixels pass through (the reference leaves them unwritten).
+// pixels pass through (the reference leaves them unwritten).  The edge stops
+// read the 8-byte guide k_svgf_temporal wrote (the G-buffer's FP16 depth and
+// normal, the values unpack_texel decodes) and the variance rides in the
+// colour's w, so a tap reads 24 bytes; the last pass writes w = 0.
 __global__ __launch_bounds__(BLOCK) void k_svgf_atrous(RestirParams p, RestirBuffers b, uint32_t pass) {
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   const uint32_t src = pass & 1u, dst = src ^ 1u;
+  const bool last = pass + 1u == p.atrous_passes;
   const float4 c4 = b.svgf_col[src][pix];
-  const GTexel cg = b.gbuf[p.cur][pix];
-  if (cg.pos_depth.w == 0.0f) {
-    b.svgf_col[dst][pix] = c4;
-    b.svgf_var[dst][pix] = 0.0f;
-    if (pass == 0) b.svgf_hist[pix] = c4;
+  const uint2 cg = b.svgf_guide[pix];
+  if (cg.x == SVGF_SKY) {
+    b.svgf_col[dst][pix] = v4(f4v(c4), 0.0f);
+    if (pass == 0) b.svgf_hist[pix] = v4(f4v(c4), 0.0f);
     return;
   }
-  const Tex ct = unpack_texel(cg);
+  const float cdepth = f16lo(cg.x);
+  const v3 cn = mk(f16hi(cg.x), f16lo(cg.y), f16hi(cg.y));
   const float KW[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};  // svgf_atrous.wgsl:11-15
   const int step = 1 << pass;
   v3 sum = f4v(c4);
-  float vsum = b.svgf_var[src][pix];
+  float vsum = c4.w;
   float wsum = 1.0f;
   const float cl = luma(sum);
-  const float cv = vsum;
+  const float inv_l = 1.0f / fmaxf(4.0f * sqrtf(c4.w), 1e-8f);  // PHI_LUMA
   for (int dy = -2; dy <= 2; ++dy) {
     for (int dx = -2; dx <= 2; ++dx) {
       if (dx == 0 && dy == 0) continue;
       const int sx = (int)x + dx * step, sy = (int)y + dy * step;
       if (sx < 0 || sy < 0 || sx >= (int)p.width || sy >= (int)p.rows) continue;
       const uint32_t q = (uint32_t)sy * p.width + (uint32_t)sx;
-      const GTexel g = b.gbuf[p.cur][q];
-      if (g.pos_depth.w == 0.0f) continue;
-      const Tex tq = unpack_texel(g);
-      const v3 cq = f4v(b.svgf_col[src][q]);
-      const float wn = powf(fmaxf(dot(ct.n, tq.n), 0.0f), 128.0f);  // PHI_NORMAL
-      const float dd = fabsf(ct.depth - tq.depth);
-      const float wd = expf(-(dd * dd) / (2.0f * (0.1f * 0.1f)));  // PHI_DEPTH
-      const float wl = expf(-fabsf(cl - luma(cq)) / fmaxf(4.0f * sqrtf(cv), 1e-8f));  // PHI_LUMA
-      float w = wn * wd * wl;
+      const uint2 g = b.svgf_guide[q];
+      if (g.x == SVGF_SKY) continue;
+      const float4 cq4 = b.svgf_col[src][q];
+      const v3 cq = f4v(cq4);
+      const v3 nq = mk(f16hi(g.x), f16lo(g.y), f16hi(g.y));
+      // PHI_NORMAL: max(n . nq, 0)^128 by seven squarings
+      float wn = fmaxf(dot(cn, nq), 0.0f);
+      for (int k = 0; k < 7; ++k) wn *= wn;
+      const float dd = fabsf(cdepth - f16lo(g.x));
+      // PHI_DEPTH and PHI_LUMA in one exponential
+      const float e = -(dd * dd) * (1.0f / (2.0f * (0.1f * 0.1f))) - fabsf(cl - luma(cq)) * inv_l;
+      float w = wn * expf(e);
       w *= KW[dx < 0 ? -dx : dx] * KW[dy < 0 ? -dy : dy];
       sum += cq * w;
-      vsum += b.svgf_var[src][q] * (w * w);
+      vsum += cq4.w * (w * w);
       wsum += w;
     }
   }
   sum = sum / wsum;
   vsum = vsum / (wsum * wsum);
-  b.svgf_col[dst][pix] = v4(sum, 0.0f);
-  b.svgf_var[dst][pix] = vsum;
+  b.svgf_col[dst][pix] = v4(sum, last ? 0.0f : vsum);
   if (pass == 0) b.svgf_hist[pix] = v4(sum, 0.0f);
 }
 
