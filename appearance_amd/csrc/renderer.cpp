@@ -91,6 +91,7 @@ struct RestirState {
   DevBuf<GTexel> gbuf[2];
   DevBuf<float2> mom[2];
   DevBuf<uint2> guide;
+  DevBuf<float> di_self;
   size_t npix = 0;
 
   template <typename F>
@@ -98,7 +99,7 @@ struct RestirState {
     f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(demod[0]); f(demod[1]);
     f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
     f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
-    f(guide); f(rq_o); f(rq_d); f(rq_hit);
+    f(guide); f(di_self); f(rq_o); f(rq_d); f(rq_hit);
   }
   // Sizes every buffer for n pixels and clears them (no history).
   apt_status reset(size_t n, hipStream_t s) {
@@ -126,6 +127,7 @@ struct RestirState {
     b.ray_o = ray_o.p; b.ray_d = ray_d.p; b.thr = thr.p; b.rng = rng.p; b.radiance = radiance.p;
     b.vtx_p = vtx_p.p; b.vtx_n = vtx_n.p; b.vtx_t = vtx_t.p;
     b.di = di.p; b.di_tmp = di_tmp.p; b.di_hist[0] = di_hist[0].p; b.di_hist[1] = di_hist[1].p;
+    b.di_self = di_self.p;
     b.gi = gi.p; b.gi_hist[0] = gi_hist[0].p; b.gi_hist[1] = gi_hist[1].p;
     b.gbuf[0] = gbuf[0].p; b.gbuf[1] = gbuf[1].p;
     b.demod[0] = demod[0].p; b.demod[1] = demod[1].p; b.work = work.p;

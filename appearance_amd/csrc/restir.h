@@ -78,6 +78,8 @@ struct RestirBuffers {
   DiRes* di;       // RIS output, then the reused reservoir that is shaded
   DiRes* di_tmp;   // temporal output
   DiRes* di_hist[2];
+  float* di_self;       // target of each pixel's temporal DI sample at the pixel itself, before
+                        // its shadow ray (queued in rq_*, traced before the spatial pass)
   GiRes* gi;
   GiRes* gi_hist[2];
   GTexel* gbuf[2];

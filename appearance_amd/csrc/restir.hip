@@ -755,6 +755,46 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_temporal(Restir
   b.rng[pix] = rng;
 }
 
+// Before the spatial pass: each pixel's temporal sample at the pixel itself.
+// combine_unbiased needs that sample's visibility-tested target at the pixel
+// (while the pixel keeps its own sample) and at every pixel whose reservoir
+// takes it from this one (as a neighbour): the same number, so it is
+// evaluated once here, the BSDF part into di_self and the shadow ray (the
+// one visible() would trace) queued for the pool kernel.
+__global__ __launch_bounds__(BLOCK) void k_rs_di_self(RestirParams p, SceneView sc, RestirBuffers b) {
+  uint32_t x, y, pix;
+  if (!tile_pixel(p, x, y, pix)) return;
+  Vtx v;
+  float ph = 0.0f;
+  bool shoot = false;
+  v3 o = mk(0.0f, 0.0f, 0.0f), wi = mk(0.0f, 0.0f, 1.0f);
+  float tmax = 0.0f;
+  if (load_vtx(b, pix, v)) {
+    const DiRes r = b.di_tmp[pix];
+    LightHit e;
+    if (eval_light(sc, r.light, r.u, r.v, v.x, e)) {
+      const float c = dot(e.wi, v.n);
+      if (c > 0.0f) {
+        const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+        float pdf;
+        ph = luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * c);
+        shoot = ph != 0.0f;
+        o = v.x + v.n * TRACE_EPSILON;
+        wi = e.wi;
+        tmax = e.dist - 2.0f * TRACE_EPSILON;
+      }
+    }
+  }
+  b.di_self[pix] = ph;
+  count_rays(shoot, b.counters + RS_CTR_SLOTS);
+  rq_put(b, shoot, pix, o, wi, tmax);
+}
+
+// di_phat(.., vis = true) > 0 of pixel i's own temporal sample at i.
+__device__ __forceinline__ bool di_self_visible(const RestirBuffers& b, uint32_t i) {
+  return b.di_self[i] > 0.0f && b.rq_hit[i].w == 0.0f;
+}
+
 // interleaved_gradient_noise_animated (shared/sampling.wgsl:64-74).
 __device__ __forceinline__ float fractf(float a) { return a - floorf(a); }
 __device__ __forceinline__ float ign_animated(uint32_t x, uint32_t y, uint32_t frame) {
@@ -800,12 +840,12 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
   const uint32_t aseed = hash_combine(x >> 2, hash_combine(y >> 2, pass_seed));
   const float a_off = (float)aseed * (1.0f / (float)0xFFFFFFFFu) * TWO_PI;
   const uint32_t n = p.neighbours;
-  // Whether r's sample has a visibility-tested target > 0 at this pixel:
-  // -1 not known yet, else 0 / 1.  It is a function of the sample alone, so
-  // it carries over while r keeps its sample, and a kept neighbour sample
-  // brings its own (nr.phat below): each shadow ray at this pixel is traced
-  // once.
-  int r_vis = -1;
+  // Whether r's sample has a visibility-tested target > 0 at this pixel (0 /
+  // 1).  It is a function of the sample alone, so it carries over while r
+  // keeps its sample, and a kept neighbour sample brings its own (nr.phat
+  // below): each shadow ray at this pixel is traced once.  The pixel's own
+  // sample's was evaluated by k_rs_di_self.
+  int r_vis = di_self_visible(b, pix) ? 1 : 0;
   for (uint32_t i = 0; i < n; ++i) {
     const float ang = (float)i * GOLDEN_ANGLE + a_off;
     const float rad = powf((float)i / (float)n, 0.5f) * radius + r_off;
@@ -832,15 +872,22 @@ __global__ __launch_bounds__(BLOCK, RS_TRACE_WAVES) void k_rs_di_spatial(RestirP
       if (kept == 2) {
         c_vis = nr.phat > 0.0f ? 1 : 0;
       } else if (kept == 1) {
-        if (r_vis < 0) r_vis = di_phat<W, S>(sc, st, v, pixel_bsdf(), r.light, r.u, r.v, true, b.counters) > 0.0f ? 1 : 0;
         c_vis = r_vis;
       }
       if (c_vis) z += r.M;
-      Vtx vn;
-      if (load_vtx(b, q, vn)) {
-        vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
-        const Bsdf bn = make_bsdf(sc, vn.mat, vn.n, vn.back, vn.t);
-        if (di_phat<W, S>(sc, st, vn, bn, c.light, c.u, c.v, true, b.counters) > 0.0f) z += nr.M;
+      // The merged sample's visibility-tested target at the neighbour: the
+      // neighbour's own sample's was evaluated by k_rs_di_self (the
+      // neighbour's vertex is its G-buffer texel's position), another sample
+      // is evaluated here; an empty merge has no target.
+      if (kept == 2) {
+        if (di_self_visible(b, q)) z += nr.M;
+      } else if (kept == 1) {
+        Vtx vn;
+        if (load_vtx(b, q, vn)) {
+          vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
+          const Bsdf bn = make_bsdf(sc, vn.mat, vn.n, vn.back, vn.t);
+          if (di_phat<W, S>(sc, st, vn, bn, c.light, c.u, c.v, true, b.counters) > 0.0f) z += nr.M;
+        }
       }
       if (c.phat > RESTIR_DI_EPSILON && z * c.wsum > RESTIR_DI_EPSILON) c.W = (1.0f / c.phat) * (1.0f / z * c.wsum);
       r = c;
@@ -1230,7 +1277,12 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
     if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S>), grid, block, 0, s, p, sc, b, bounce);
     if (bounce == 0 && p.restir_di) {
       hipLaunchKernelGGL((k_rs_di_temporal<W, S>), grid, block, 0, s, p, sc, b);
-      if (p.neighbours > 0) hipLaunchKernelGGL((k_rs_di_spatial<W, S>), grid, block, 0, s, p, sc, b);
+      if (p.neighbours > 0) {
+        hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
+        hipLaunchKernelGGL(k_rs_di_self, grid, block, 0, s, p, sc, b);
+        hipLaunchKernelGGL((k_rs_trace<W, S, true, false>), pool, block, 0, s, sc, b, npix);
+        hipLaunchKernelGGL((k_rs_di_spatial<W, S>), grid, block, 0, s, p, sc, b);
+      }
     }
     if (bounce == 0 && more && p.restir_gi)
       hipLaunchKernelGGL((k_rs_gi_temporal<W, S>), grid, block, 0, s, p, sc, b);
