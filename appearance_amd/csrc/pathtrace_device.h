@@ -13,19 +13,6 @@ constexpr float SUN_DISTANCE = 1e6f;    // shared/sky_bindings.wgsl:4
 
 constexpr uint32_t FLAG_NEE_SIDE = 1u;  // previous vertex: light sampling could reach wi
 
-// Wave-level compaction: returns this lane's slot in `queue`, or -1.
-__device__ __forceinline__ int wave_append(bool want, uint32_t* counter) {
-  const uint64_t mask = __ballot(want);
-  if (mask == 0) return -1;
-  const uint32_t lane = __lane_id();
-  const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  return want ? (int)(base + below) : -1;
-}
-
 // Work-group-level compaction into two queues at once: one atomic per queue
 // per work-group instead of one per wave (a single counter that every wave of
 // a launch hits serialises those atomics).  Every thread of the work-group
