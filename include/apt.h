@@ -185,6 +185,10 @@ typedef struct apt_material {
   uint32_t clearcoat_texture;
   float clearcoat_roughness;
   uint32_t clearcoat_roughness_texture;
+  /* Carried for layout only: every hit is opaque, as in the reference, whose
+   * MAX_NON_OPAQUE_DEPTH and MAX_NON_OPAQUE_SHADOW_DEPTH are 1
+   * (helpers/trace.wgsl:7-8), so its alpha test (trace.wgsl:124-136) always
+   * takes the opaque branch and shadow rays use the opaque query. */
   float alpha_cutoff;
   uint32_t sheen_tint_texture;
   uint32_t clearcoat_normal_texture;
