@@ -102,7 +102,7 @@ class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("segments", C.c_uint64),
                 ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
-                ("ms_shadow", C.c_double), ("ms_other", C.c_double)]
+                ("ms_shadow", C.c_double), ("ms_other", C.c_double), ("ms_scene", C.c_double)]
 
 
 assert C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
@@ -140,6 +140,7 @@ _SIGS = {
     "apt_render_cb": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, RESULT_CB,
                                 _P]),
     "apt_get_render_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
+    "apt_set_frame_index": (C.c_int, [_P, C.c_uint32]),
 }
 
 _lib = None

@@ -249,6 +249,7 @@ apt_status validate_config(const apt_config& c) {
 
 apt_status upload_scene(apt_renderer* r) {
   if (!r->scene_dirty) return APT_OK;
+  const auto t0 = std::chrono::steady_clock::now();
   if (!r->explicit_scene) {
     // Flatten the spawned models into one triangle soup (world space).
     r->pos.clear();
@@ -300,6 +301,7 @@ apt_status upload_scene(apt_renderer* r) {
   }
   r->scene_dirty = false;
   r->scene_version++;
+  r->stats.ms_scene = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return APT_OK;
 }
 
@@ -396,7 +398,9 @@ apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint3
   if ((st = finish_frame(r, npix, out_rgba, out_accum)) != APT_OK) return st;
   float ms = 0.0f;
   APT_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  const double ms_scene = r->stats.ms_scene;  // kept until the next rebuild
   r->stats = apt_render_stats{};
+  r->stats.ms_scene = ms_scene;
   r->stats.paths = (uint64_t)npix * spp;
   uint64_t cnt[2] = {0, 0};
   for (uint32_t k = 0; k < 2 * RS_CTR_SLOTS; ++k) cnt[k / RS_CTR_SLOTS] += part[k];
@@ -490,7 +494,9 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
   if ((st = finish_frame(r, npix, out_rgba, out_accum)) != APT_OK) return st;
   float ms = 0.0f;
   APT_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  const double ms_scene = r->stats.ms_scene;  // kept until the next rebuild
   r->stats = apt_render_stats{};
+  r->stats.ms_scene = ms_scene;
   r->stats.paths = (uint64_t)npix * spp;
   r->stats.extension_rays = cnt[CNT_STAT_EXT];
   r->stats.shadow_rays = cnt[CNT_STAT_SHADOW];

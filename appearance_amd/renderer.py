@@ -120,6 +120,10 @@ class PathTracerGpu:
               "apt_render")
         return rgba
 
+    def set_frame_index(self, frame_idx: int) -> None:
+        """Restart the frame sequence at frame_idx (seeds frame * spp + sample)."""
+        check(lib().apt_set_frame_index(self._h, frame_idx), "apt_set_frame_index")
+
     def stats(self) -> dict:
         s = RenderStats()
         check(lib().apt_get_render_stats(self._h, C.byref(s)), "apt_get_render_stats")

@@ -356,6 +356,8 @@ typedef struct apt_render_stats {
   double ms_shade;
   double ms_shadow;
   double ms_other;
+  double ms_scene;           /* wall time of the last scene rebuild after a world action or
+                              * apt_scene_set: flatten, device BVH build, upload (0 if none yet) */
 } apt_render_stats;
 apt_status apt_get_render_stats(const apt_renderer* r, apt_render_stats* out);
 

@@ -132,6 +132,52 @@ def test_world_actions_match_scene_set(gpu_available):
         b.render_accum((64, 64))
 
 
+def _model_xform(angle_deg, tx):
+    """Column-major-friendly 4x4 (row i = basis row): rotation about y, then a move along x."""
+    a = np.deg2rad(angle_deg)
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
+    m[0, 3] = tx
+    return m
+
+
+def test_model_actions_equal_direct_spawn(gpu_available):
+    """TransformModel and DestroyModel (scene_resources/mod.rs:159-276 rebuilds
+    the TLAS after them) leave the renderer as if the surviving models had
+    been spawned at their final transforms: the image is bit-identical, and
+    each action's scene rebuild (device BVH build) is timed in the stats."""
+    import uuid
+    cam = world.CameraUpdate(world.look_at((0.0, 1.0, 6.5), (0.0, 1.0, 0.0)), 50.0)
+    ua, ub = uuid.UUID(int=1), uuid.UUID(int=2)
+    t_b0, t_b1 = _model_xform(0.0, 2.5), _model_xform(30.0, -2.2)
+
+    def renderer(*spawns):
+        pt = PathTracerGpu(plain_config(sample_count=2, max_bounces=4))
+        for u, t in spawns:
+            pt.handle_visible_world_action(world.SpawnModel("procedural://cornell?seed=0", t, u))
+        pt.handle_visible_world_action(cam)
+        return pt
+
+    a = renderer((ua, np.eye(4, dtype=np.float32)), (ub, t_b0))
+    _, before = a.render_accum((128, 64))
+    assert a.stats()["ms_scene"] > 0.0
+    a.handle_visible_world_action(world.TransformModel(ub, t_b1))
+    a.set_frame_index(0)
+    _, moved = a.render_accum((128, 64))
+    _, ref = renderer((ua, np.eye(4, dtype=np.float32)), (ub, t_b1)).render_accum((128, 64))
+    np.testing.assert_array_equal(moved, ref)
+    assert not np.array_equal(moved, before)
+    a.handle_visible_world_action(world.DestroyModel(ua))
+    a.set_frame_index(0)
+    _, left = a.render_accum((128, 64))
+    _, ref = renderer((ub, t_b1)).render_accum((128, 64))
+    np.testing.assert_array_equal(left, ref)
+    with pytest.raises(Exception):
+        a.handle_visible_world_action(world.DestroyModel(ua))
+    with pytest.raises(Exception):
+        a.handle_visible_world_action(world.TransformModel(ua, t_b0))
+
+
 def test_progressive_accumulation(gpu_available):
     s = generate_scene("cornell")
     one = PathTracerGpu(plain_config(sample_count=2, max_bounces=4, accum_frames=1))
