@@ -257,13 +257,17 @@ apt_status upload_scene(apt_renderer* r) {
     r->mat_ids.clear();
     r->mats.clear();
     bool all_normals = !r->models.empty();
+    size_t total = 0;
     for (const Model& m : r->models) {
       apt_mesh_desc md;
-      apt_camera cam;
-      apt_sky sky;
-      apt_scene_data_view(m.data, &md, &cam, &sky);
+      apt_scene_data_view(m.data, &md, nullptr, nullptr);
       all_normals = all_normals && md.normals != nullptr;
+      total += md.n_tris;
     }
+    r->pos.resize(9 * total);
+    r->mat_ids.resize(total);
+    if (all_normals) r->nrm.resize(9 * total);
+    size_t base = 0;
     for (const Model& m : r->models) {
       apt_mesh_desc md;
       apt_scene_data_view(m.data, &md, nullptr, nullptr);
@@ -271,23 +275,24 @@ apt_status upload_scene(apt_renderer* r) {
       r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
       float nm[9];
       const bool nm_ok = normal_matrix(m.xform, nm);
+      float* pos = r->pos.data() + 9 * base;
+      uint32_t* ids = r->mat_ids.data() + base;
       for (uint32_t t = 0; t < md.n_tris; ++t) {
-        float v[9];
-        for (int k = 0; k < 3; ++k) mat_point(m.xform, md.positions + 9 * (size_t)t + 3 * k, v + 3 * k);
-        r->pos.insert(r->pos.end(), v, v + 9);
-        r->mat_ids.push_back(mat_base + (md.material_ids ? md.material_ids[t] : 0u));
-        if (all_normals) {
-          for (int k = 0; k < 3; ++k) {
-            const float* n = md.normals + 9 * (size_t)t + 3 * k;
-            float o[3];
-            for (int q = 0; q < 3; ++q)
-              o[q] = nm_ok ? nm[3 * q] * n[0] + nm[3 * q + 1] * n[1] + nm[3 * q + 2] * n[2] : n[q];
-            float l = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
-            if (l > 0.0f) for (int q = 0; q < 3; ++q) o[q] /= l;
-            r->nrm.insert(r->nrm.end(), o, o + 3);
-          }
+        for (int k = 0; k < 3; ++k) mat_point(m.xform, md.positions + 9 * (size_t)t + 3 * k, pos + 9 * (size_t)t + 3 * k);
+        ids[t] = mat_base + (md.material_ids ? md.material_ids[t] : 0u);
+      }
+      if (all_normals) {
+        float* nrm = r->nrm.data() + 9 * base;
+        for (size_t j = 0; j < 3 * (size_t)md.n_tris; ++j) {
+          const float* n = md.normals + 3 * j;
+          float* o = nrm + 3 * j;
+          for (int q = 0; q < 3; ++q)
+            o[q] = nm_ok ? nm[3 * q] * n[0] + nm[3 * q + 1] * n[1] + nm[3 * q + 2] * n[2] : n[q];
+          const float l = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+          if (l > 0.0f) for (int q = 0; q < 3; ++q) o[q] /= l;
         }
       }
+      base += md.n_tris;
     }
     r->has_normals = all_normals;
   }

@@ -23,6 +23,44 @@ apt_status upload(T*& dst, const T* src, size_t n, size_t& bytes) {
   return APT_OK;
 }
 
+// xyz triples (3 per triangle) -> float4 with w = 0.
+__global__ void k_widen3(const float* __restrict__ xyz, float4* __restrict__ out, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
+}
+
+// Uploads n vertices (xyz triples) once and widens them to float4 on the
+// device.  Returns the device copy of the triples in *keep_xyz if asked
+// (the caller frees it), else frees it.
+apt_status upload_widened(float4*& dst, const float* src, size_t n, size_t& bytes, float** keep_xyz) {
+  float* d_xyz = nullptr;
+  apt_status st = upload(d_xyz, src, 3 * n, bytes);
+  if (st != APT_OK) return st;
+  bytes -= std::max<size_t>(1, 3 * n) * sizeof(float);  // staging, not resident
+  hipError_t e = hipMalloc(&dst, std::max<size_t>(1, n) * sizeof(float4));
+  if (e != hipSuccess) {
+    dst = nullptr;
+    (void)hipFree(d_xyz);
+    set_error("scene upload: hipMalloc(%zu B) failed: %s", n * sizeof(float4), hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
+  }
+  bytes += std::max<size_t>(1, n) * sizeof(float4);
+  if (n) {
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_widen3, dim3(blocks), dim3(256), 0, nullptr, d_xyz, dst, (uint32_t)n);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      (void)hipFree(d_xyz);
+      set_error("scene upload: widening failed: %s", hipGetErrorString(e));
+      return APT_ERR_DEVICE;
+    }
+  }
+  if (keep_xyz) *keep_xyz = d_xyz;
+  else (void)hipFree(d_xyz);
+  return APT_OK;
+}
+
 }  // namespace
 
 apt_status scene_gpu_upload(const float* pos, const float* nrm, const uint32_t* mat_ids, uint32_t n,
@@ -42,15 +80,17 @@ apt_status scene_gpu_upload(const float* pos, const float* nrm, const uint32_t* 
   // (1M triangles: 13 ms against 175 ms), which every spawn, move or removal
   // of a model pays.
   apt_bvh_opts o{bvh_width, 4, 32, APT_BUILDER_GPU, lds_stack};
-  apt_status st = apt_bvh_build(pos, n, &o, &s.bvh);
-  if (st != APT_OK) return st;
-  s.bytes = s.bvh->stats.device_bytes;
-  std::vector<float4> p4((size_t)n * 3);
-  for (size_t i = 0; i < (size_t)n * 3; ++i) p4[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0f);
-  if ((st = upload(s.d_pos, p4.data(), p4.size(), s.bytes)) != APT_OK) { scene_gpu_free(s); return st; }
-  if (nrm) {
-    for (size_t i = 0; i < (size_t)n * 3; ++i) p4[i] = make_float4(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.0f);
-    if ((st = upload(s.d_nrm, p4.data(), p4.size(), s.bytes)) != APT_OK) { scene_gpu_free(s); return st; }
+  // The positions cross PCIe once: the builder reads the uploaded triples.
+  float* d_xyz = nullptr;
+  apt_status st = upload_widened(s.d_pos, pos, (size_t)n * 3, s.bytes, &d_xyz);
+  if (st != APT_OK) { scene_gpu_free(s); return st; }
+  st = apt_bvh_build_device(d_xyz, n, &o, &s.bvh);
+  (void)hipFree(d_xyz);
+  if (st != APT_OK) { scene_gpu_free(s); return st; }
+  s.bytes += s.bvh->stats.device_bytes;
+  if (nrm && (st = upload_widened(s.d_nrm, nrm, (size_t)n * 3, s.bytes, nullptr)) != APT_OK) {
+    scene_gpu_free(s);
+    return st;
   }
   std::vector<uint32_t> ids(mat_ids ? mat_ids : nullptr, mat_ids ? mat_ids + n : nullptr);
   if (!mat_ids) ids.assign(n, 0u);
