@@ -82,6 +82,36 @@ def test_disney_vs_oracle(gpu_available, oracle):
     assert acc.mean() > 0.05
 
 
+def test_disney_full_frame(gpu_available, oracle):
+    """The disney workload at its full size (1920x1088, 4 spp, depth 8, RR
+    from bounce 3).  Size-independent properties: one 2^24-path pass, four
+    one-sample passes (2^20-path budget) and two slabs give the same frame
+    bit for bit (per-pixel seeds, samples summed in order); and 16384
+    sampled pixels match the oracle's same pixels, same seeds, to relMSE
+    <= 1e-3."""
+    s = generate_scene("disney", seed=3)
+    w, h = 1920, 1088
+    print("[render] disney 1920x1088 4 spp, one pass", flush=True)
+    _, _, full = gpu_render(s, w, h, 4, 8, 3)
+    assert np.isfinite(full).all() and (full >= 0).all() and full.mean() > 0.05
+    print("[render] disney 1920x1088 4 spp, one sample per pass", flush=True)
+    _, _, passes = gpu_render(s, w, h, 4, 8, 3, queue_chunk=1 << 20)
+    np.testing.assert_array_equal(passes, full)
+    for r0, r1 in ((0, 512), (512, 1088)):
+        print(f"[render] disney slab rows {r0}..{r1}", flush=True)
+        pt = PathTracerGpu(plain_config(sample_count=4, max_bounces=8, rr_start_bounce=3))
+        pt.set_scene(s)
+        pt.set_camera(s.camera)
+        _, slab = pt.render_accum((w, h), r0, r1)
+        np.testing.assert_array_equal(slab, full[r0:r1])
+    print("[render] disney oracle on 16384 sampled pixels", flush=True)
+    pix = np.random.default_rng(7).choice(w * h, 16384, replace=False).astype(np.uint32)
+    ref, _ = oracle_render(oracle, s, w, h, spp=4, max_bounces=8, rr_start=3, pixels=pix)
+    r = relmse(full.reshape(-1, 3)[pix], ref)
+    print(f"[render] disney full frame relMSE on sampled pixels = {r:.3e}", flush=True)
+    assert r < 1e-3
+
+
 def test_slab_and_determinism(gpu_available):
     s = generate_scene("cornell")
     pt = PathTracerGpu(plain_config(sample_count=4, max_bounces=4))
