@@ -27,7 +27,55 @@ struct Model {
   uint8_t uuid[16];
   float xform[16];  // column-major
   apt_scene_data* data;
+  // The model's own triangles, resident from its first flatten on: a move
+  // re-places them on the device instead of crossing PCIe again.
+  float* d_xyz = nullptr;
+  float* d_nrm = nullptr;
+  uint32_t* d_mat = nullptr;
 };
+
+void model_free(Model& m) {
+  apt_scene_data_destroy(m.data);
+  for (void* p : {(void*)m.d_xyz, (void*)m.d_nrm, (void*)m.d_mat})
+    if (p) (void)hipFree(p);
+  m.data = nullptr;
+  m.d_xyz = m.d_nrm = nullptr;
+  m.d_mat = nullptr;
+}
+
+// Copies a model's triangles to the device once (checking its material ids).
+apt_status model_resident(Model& m) {
+  if (m.d_xyz) return APT_OK;
+  apt_mesh_desc md;
+  apt_scene_data_view(m.data, &md, nullptr, nullptr);
+  const size_t n = md.n_tris;
+  std::vector<uint32_t> ids(n, 0u);
+  for (size_t t = 0; t < n; ++t) {
+    const uint32_t id = md.material_ids ? md.material_ids[t] : 0u;
+    APT_REQUIRE(id < md.n_materials, APT_ERR_INVALID_ARG, "model: triangle %zu uses material %u of %u", t, id,
+                md.n_materials);
+    ids[t] = id;
+  }
+  const auto put = [](auto*& dst, const auto* src, size_t count) -> apt_status {
+    hipError_t e = hipMalloc(&dst, std::max<size_t>(1, count) * sizeof(*src));
+    if (e == hipSuccess && count) e = hipMemcpy(dst, src, count * sizeof(*src), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_error("model upload: %s", hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
+    }
+    return APT_OK;
+  };
+  apt_status st = put(m.d_xyz, md.positions, 9 * n);
+  if (st == APT_OK && md.normals) st = put(m.d_nrm, md.normals, 9 * n);
+  if (st == APT_OK) st = put(m.d_mat, ids.data(), n);
+  if (st != APT_OK) {
+    for (void* p : {(void*)m.d_xyz, (void*)m.d_nrm, (void*)m.d_mat})
+      if (p) (void)hipFree(p);
+    m.d_xyz = m.d_nrm = nullptr;
+    m.d_mat = nullptr;
+  }
+  return st;
+}
 
 // Page-locked host memory, so the frame's readback runs at the link's rate
 // with no staging copy.
@@ -143,10 +191,6 @@ CamView cam_view(const SceneView& sc) {
   return CamView{sc.cam_pos, sc.cam_right, sc.cam_up, sc.cam_back, sc.cam_tan_half, sc.cam_aspect};
 }
 
-void mat_point(const float* m, const float* p, float* out) {
-  for (int r = 0; r < 3; ++r) out[r] = m[r] * p[0] + m[4 + r] * p[1] + m[8 + r] * p[2] + m[12 + r];
-}
-
 // Normal transform: inverse transpose of the upper 3x3.
 bool normal_matrix(const float* m, float* n) {
   const double a = m[0], b = m[4], c = m[8], d = m[1], e = m[5], f = m[9], g = m[2], h = m[6], i = m[10];
@@ -203,7 +247,7 @@ struct apt_renderer {
   uint32_t rs_w = 0, rs_h = 0, rs_row0 = 0;
 
   ~apt_renderer() {
-    for (Model& m : models) apt_scene_data_destroy(m.data);
+    for (Model& m : models) model_free(m);
     scene_gpu_free(gpu);
     d_frame_sum.release(); d_accum_tm.release(); d_accum_lin.release(); d_rgba.release();
     p_o.release(); p_d.release(); p_t.release(); p_l.release(); p_h.release(); p_so.release();
@@ -250,59 +294,63 @@ apt_status validate_config(const apt_config& c) {
 apt_status upload_scene(apt_renderer* r) {
   if (!r->scene_dirty) return APT_OK;
   const auto t0 = std::chrono::steady_clock::now();
+  scene_gpu_free(r->gpu);
   if (!r->explicit_scene) {
-    // Flatten the spawned models into one triangle soup (world space).
-    r->pos.clear();
-    r->nrm.clear();
-    r->mat_ids.clear();
+    // Flatten the spawned models into one triangle soup in world space, on
+    // the device, from each model's resident copy.
     r->mats.clear();
     bool all_normals = !r->models.empty();
     size_t total = 0;
-    for (const Model& m : r->models) {
+    for (Model& m : r->models) {
+      apt_status st = model_resident(m);
+      if (st != APT_OK) return st;
+      all_normals = all_normals && m.d_nrm != nullptr;
       apt_mesh_desc md;
       apt_scene_data_view(m.data, &md, nullptr, nullptr);
-      all_normals = all_normals && md.normals != nullptr;
       total += md.n_tris;
     }
-    r->pos.resize(9 * total);
-    r->mat_ids.resize(total);
-    if (all_normals) r->nrm.resize(9 * total);
-    size_t base = 0;
-    for (const Model& m : r->models) {
-      apt_mesh_desc md;
-      apt_scene_data_view(m.data, &md, nullptr, nullptr);
-      const uint32_t mat_base = (uint32_t)r->mats.size();
-      r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
-      float nm[9];
-      const bool nm_ok = normal_matrix(m.xform, nm);
-      float* pos = r->pos.data() + 9 * base;
-      uint32_t* ids = r->mat_ids.data() + base;
-      for (uint32_t t = 0; t < md.n_tris; ++t) {
-        for (int k = 0; k < 3; ++k) mat_point(m.xform, md.positions + 9 * (size_t)t + 3 * k, pos + 9 * (size_t)t + 3 * k);
-        ids[t] = mat_base + (md.material_ids ? md.material_ids[t] : 0u);
+    APT_REQUIRE(total < (1u << 28), APT_ERR_INVALID_ARG, "scene: %zu triangles exceed 2^28", total);
+    if (total > 0) {
+      float *d_xyz = nullptr, *d_nrm = nullptr;
+      uint32_t* d_mat = nullptr;
+      hipError_t e = hipMalloc(&d_xyz, 9 * total * sizeof(float));
+      if (e == hipSuccess && all_normals) e = hipMalloc(&d_nrm, 9 * total * sizeof(float));
+      if (e == hipSuccess) e = hipMalloc(&d_mat, total * sizeof(uint32_t));
+      apt_status st = APT_OK;
+      if (e != hipSuccess) {
+        set_error("scene flatten: hipMalloc failed: %s", hipGetErrorString(e));
+        st = e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
       }
-      if (all_normals) {
-        float* nrm = r->nrm.data() + 9 * base;
-        for (size_t j = 0; j < 3 * (size_t)md.n_tris; ++j) {
-          const float* n = md.normals + 3 * j;
-          float* o = nrm + 3 * j;
-          for (int q = 0; q < 3; ++q)
-            o[q] = nm_ok ? nm[3 * q] * n[0] + nm[3 * q + 1] * n[1] + nm[3 * q + 2] * n[2] : n[q];
-          const float l = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
-          if (l > 0.0f) for (int q = 0; q < 3; ++q) o[q] /= l;
-        }
+      size_t base = 0;
+      for (const Model& m : r->models) {
+        if (st != APT_OK) break;
+        apt_mesh_desc md;
+        apt_scene_data_view(m.data, &md, nullptr, nullptr);
+        ModelXform x{};
+        std::memcpy(x.m, m.xform, sizeof(x.m));
+        x.nm_ok = normal_matrix(m.xform, x.nm) ? 1u : 0u;
+        x.mat_base = (uint32_t)r->mats.size();
+        r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
+        st = scene_flatten_model(m.d_xyz, m.d_nrm, m.d_mat, md.n_tris, x, d_xyz + 9 * base,
+                                 d_nrm ? d_nrm + 9 * base : nullptr, d_mat + base);
+        base += md.n_tris;
       }
-      base += md.n_tris;
+      if (st == APT_OK)
+        st = scene_gpu_upload_device(d_xyz, d_nrm, d_mat, (uint32_t)total, r->mats.data(),
+                                     (uint32_t)r->mats.size(), &r->sky, r->cfg.bvh_width,
+                                     r->cfg.lds_stack_depth, r->gpu);
+      for (void* p : {(void*)d_xyz, (void*)d_nrm, (void*)d_mat})
+        if (p) (void)hipFree(p);
+      if (st != APT_OK) return st;
     }
-    r->has_normals = all_normals;
-  }
-  const uint32_t n = (uint32_t)r->mat_ids.size();
-  scene_gpu_free(r->gpu);
-  if (n > 0) {
-    apt_status st = scene_gpu_upload(r->pos.data(), r->has_normals ? r->nrm.data() : nullptr,
-                                     r->mat_ids.data(), n, r->mats.data(), (uint32_t)r->mats.size(),
-                                     &r->sky, r->cfg.bvh_width, r->cfg.lds_stack_depth, r->gpu);
-    if (st != APT_OK) return st;
+  } else {
+    const uint32_t n = (uint32_t)r->mat_ids.size();
+    if (n > 0) {
+      apt_status st = scene_gpu_upload(r->pos.data(), r->has_normals ? r->nrm.data() : nullptr,
+                                       r->mat_ids.data(), n, r->mats.data(), (uint32_t)r->mats.size(),
+                                       &r->sky, r->cfg.bvh_width, r->cfg.lds_stack_depth, r->gpu);
+      if (st != APT_OK) return st;
+    }
   }
   r->scene_dirty = false;
   r->scene_version++;
@@ -658,7 +706,7 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
       APT_REQUIRE(len == 16, APT_ERR_INVALID_ARG, "DestroyModel carries 16 bytes, got %zu", len);
       for (size_t i = 0; i < r->models.size(); ++i) {
         if (std::memcmp(r->models[i].uuid, b, 16) == 0) {
-          apt_scene_data_destroy(r->models[i].data);
+          model_free(r->models[i]);
           r->models.erase(r->models.begin() + (long)i);
           r->scene_dirty = true;
           return APT_OK;
@@ -669,7 +717,7 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
     }
     case 4: {  // Clear
       APT_REQUIRE(len == 4, APT_ERR_INVALID_ARG, "Clear carries 4 bytes, got %zu", len);
-      for (Model& m : r->models) apt_scene_data_destroy(m.data);
+      for (Model& m : r->models) model_free(m);
       r->models.clear();
       r->explicit_scene = false;
       r->scene_dirty = true;

@@ -29,39 +29,130 @@ __global__ void k_widen3(const float* __restrict__ xyz, float4* __restrict__ out
     out[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
 }
 
-// Uploads n vertices (xyz triples) once and widens them to float4 on the
-// device.  Returns the device copy of the triples in *keep_xyz if asked
-// (the caller frees it), else frees it.
-apt_status upload_widened(float4*& dst, const float* src, size_t n, size_t& bytes, float** keep_xyz) {
-  float* d_xyz = nullptr;
-  apt_status st = upload(d_xyz, src, 3 * n, bytes);
-  if (st != APT_OK) return st;
-  bytes -= std::max<size_t>(1, 3 * n) * sizeof(float);  // staging, not resident
-  hipError_t e = hipMalloc(&dst, std::max<size_t>(1, n) * sizeof(float4));
-  if (e != hipSuccess) {
-    dst = nullptr;
-    (void)hipFree(d_xyz);
-    set_error("scene upload: hipMalloc(%zu B) failed: %s", n * sizeof(float4), hipGetErrorString(e));
-    return e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
-  }
-  bytes += std::max<size_t>(1, n) * sizeof(float4);
-  if (n) {
-    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_widen3, dim3(blocks), dim3(256), 0, nullptr, d_xyz, dst, (uint32_t)n);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
-    if (e != hipSuccess) {
-      (void)hipFree(d_xyz);
-      set_error("scene upload: widening failed: %s", hipGetErrorString(e));
-      return APT_ERR_DEVICE;
+// Emitters: the material emits and the edges' cross product is not zero (a
+// degenerate emitter emits nothing).  The same float operations as a host
+// loop (no contraction), so the list does not depend on where it is made.
+__global__ void k_emit_flags(const float* __restrict__ xyz, const uint32_t* __restrict__ mat,
+                             const apt_material* __restrict__ mats, uint8_t* __restrict__ flag, uint32_t n) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const apt_material& m = mats[mat[t]];
+    bool e = m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f;
+    if (e) {
+      const float* p = xyz + 9 * (size_t)t;
+      const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+      const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      e = c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 0.0f;
     }
+    flag[t] = e ? 1 : 0;
   }
-  if (keep_xyz) *keep_xyz = d_xyz;
-  else (void)hipFree(d_xyz);
+}
+
+__global__ void k_flatten(const float* __restrict__ lxyz, const float* __restrict__ lnrm,
+                          const uint32_t* __restrict__ lmat, uint32_t n, ModelXform x,
+                          float* __restrict__ oxyz, float* __restrict__ onrm, uint32_t* __restrict__ omat) {
+  const uint32_t nv = 3 * n;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += gridDim.x * blockDim.x) {
+    const float* p = lxyz + 3 * (size_t)j;
+    float* o = oxyz + 3 * (size_t)j;
+    for (int r = 0; r < 3; ++r) o[r] = x.m[r] * p[0] + x.m[4 + r] * p[1] + x.m[8 + r] * p[2] + x.m[12 + r];
+    if (onrm) {
+      const float* q = lnrm + 3 * (size_t)j;
+      float v[3];
+      for (int r = 0; r < 3; ++r)
+        v[r] = x.nm_ok ? x.nm[3 * r] * q[0] + x.nm[3 * r + 1] * q[1] + x.nm[3 * r + 2] * q[2] : q[r];
+      const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      if (l > 0.0f) for (int r = 0; r < 3; ++r) v[r] /= l;
+      float* w = onrm + 3 * (size_t)j;
+      w[0] = v[0];
+      w[1] = v[1];
+      w[2] = v[2];
+    }
+    if (j < n) omat[j] = x.mat_base + (lmat ? lmat[j] : 0u);
+  }
+}
+
+uint32_t grid_for(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 4096)); }
+
+apt_status sync_null(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    set_error("scene upload: %s failed: %s", what, hipGetErrorString(e));
+    return APT_ERR_DEVICE;
+  }
   return APT_OK;
 }
 
+apt_status alloc_dev(void** p, size_t b) {
+  hipError_t e = hipMalloc(p, std::max<size_t>(1, b));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error("scene upload: hipMalloc(%zu B) failed: %s", b, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
+  }
+  return APT_OK;
+}
+
+// n vertices (xyz triples in device memory) -> a new float4 buffer.
+apt_status widen(const float* d_xyz, size_t n, float4*& dst, size_t& bytes) {
+  apt_status st = alloc_dev((void**)&dst, n * sizeof(float4));
+  if (st != APT_OK) return st;
+  bytes += std::max<size_t>(1, n * sizeof(float4));
+  if (n) hipLaunchKernelGGL(k_widen3, dim3(grid_for(n)), dim3(256), 0, nullptr, d_xyz, dst, (uint32_t)n);
+  return sync_null("widening");
+}
+
 }  // namespace
+
+apt_status scene_gpu_upload_device(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat_ids,
+                                   uint32_t n, const apt_material* mats, uint32_t n_mats,
+                                   const apt_sky* sky, uint32_t bvh_width, uint32_t lds_stack,
+                                   SceneGpu& out) {
+  APT_REQUIRE(n > 0 && d_xyz && d_mat_ids && mats && n_mats > 0, APT_ERR_INVALID_ARG,
+              "scene: needs triangles and at least one material");
+  SceneGpu s;
+  s.n_tris = n;
+  s.n_mats = n_mats;
+  if (sky) s.sky = *sky;
+  // The device builder gives the host builder's bytes about 13x sooner
+  // (1M triangles: 13 ms against 175 ms), which every spawn, move or removal
+  // of a model pays.
+  apt_bvh_opts o{bvh_width, 4, 32, APT_BUILDER_GPU, lds_stack};
+  apt_status st = widen(d_xyz, (size_t)n * 3, s.d_pos, s.bytes);
+  if (st == APT_OK && (st = apt_bvh_build_device(d_xyz, n, &o, &s.bvh)) == APT_OK)
+    s.bytes += s.bvh->stats.device_bytes;
+  if (st == APT_OK && d_nrm) st = widen(d_nrm, (size_t)n * 3, s.d_nrm, s.bytes);
+  if (st == APT_OK && (st = alloc_dev((void**)&s.d_mat, (size_t)n * sizeof(uint32_t))) == APT_OK) {
+    s.bytes += (size_t)n * sizeof(uint32_t);
+    if (hipMemcpy(s.d_mat, d_mat_ids, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice) != hipSuccess)
+      st = sync_null("material id copy");
+  }
+  if (st == APT_OK) st = upload(s.d_mats, mats, n_mats, s.bytes);
+  std::vector<uint8_t> flag(n);
+  if (st == APT_OK) {
+    uint8_t* d_flag = nullptr;
+    st = alloc_dev((void**)&d_flag, n);
+    if (st == APT_OK) {
+      hipLaunchKernelGGL(k_emit_flags, dim3(grid_for(n)), dim3(256), 0, nullptr, d_xyz, s.d_mat, s.d_mats,
+                         d_flag, n);
+      st = sync_null("emitter flags");
+      if (st == APT_OK && hipMemcpy(flag.data(), d_flag, n, hipMemcpyDeviceToHost) != hipSuccess)
+        st = sync_null("emitter flag readback");
+      (void)hipFree(d_flag);
+    }
+  }
+  if (st != APT_OK) {
+    scene_gpu_free(s);
+    return st;
+  }
+  std::vector<uint32_t> em;
+  for (uint32_t t = 0; t < n; ++t)
+    if (flag[t]) em.push_back(t);
+  s.n_emissive = (uint32_t)em.size();
+  if ((st = upload(s.d_emissive, em.data(), em.size(), s.bytes)) != APT_OK) { scene_gpu_free(s); return st; }
+  out = s;
+  return APT_OK;
+}
 
 apt_status scene_gpu_upload(const float* pos, const float* nrm, const uint32_t* mat_ids, uint32_t n,
                             const apt_material* mats, uint32_t n_mats, const apt_sky* sky,
@@ -72,43 +163,29 @@ apt_status scene_gpu_upload(const float* pos, const float* nrm, const uint32_t* 
     const uint32_t m = mat_ids ? mat_ids[t] : 0u;
     APT_REQUIRE(m < n_mats, APT_ERR_INVALID_ARG, "scene: triangle %u uses material %u of %u", t, m, n_mats);
   }
-  SceneGpu s;
-  s.n_tris = n;
-  s.n_mats = n_mats;
-  if (sky) s.sky = *sky;
-  // The device builder gives the host builder's bytes about 13x sooner
-  // (1M triangles: 13 ms against 175 ms), which every spawn, move or removal
-  // of a model pays.
-  apt_bvh_opts o{bvh_width, 4, 32, APT_BUILDER_GPU, lds_stack};
-  // The positions cross PCIe once: the builder reads the uploaded triples.
-  float* d_xyz = nullptr;
-  apt_status st = upload_widened(s.d_pos, pos, (size_t)n * 3, s.bytes, &d_xyz);
-  if (st != APT_OK) { scene_gpu_free(s); return st; }
-  st = apt_bvh_build_device(d_xyz, n, &o, &s.bvh);
-  (void)hipFree(d_xyz);
-  if (st != APT_OK) { scene_gpu_free(s); return st; }
-  s.bytes += s.bvh->stats.device_bytes;
-  if (nrm && (st = upload_widened(s.d_nrm, nrm, (size_t)n * 3, s.bytes, nullptr)) != APT_OK) {
-    scene_gpu_free(s);
-    return st;
-  }
-  std::vector<uint32_t> ids(mat_ids ? mat_ids : nullptr, mat_ids ? mat_ids + n : nullptr);
+  // Each array crosses PCIe once; the device copy feeds the build and the
+  // attribute buffers.
+  std::vector<uint32_t> ids;
   if (!mat_ids) ids.assign(n, 0u);
-  if ((st = upload(s.d_mat, ids.data(), ids.size(), s.bytes)) != APT_OK) { scene_gpu_free(s); return st; }
-  if ((st = upload(s.d_mats, mats, n_mats, s.bytes)) != APT_OK) { scene_gpu_free(s); return st; }
-  std::vector<uint32_t> em;
-  for (uint32_t t = 0; t < n; ++t) {
-    const apt_material& m = mats[ids[t]];
-    if (!(m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f)) continue;
-    const float* p = pos + 9 * (size_t)t;
-    const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
-    const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-    if (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] > 0.0f) em.push_back(t);  // degenerate emitters emit nothing
-  }
-  s.n_emissive = (uint32_t)em.size();
-  if ((st = upload(s.d_emissive, em.data(), em.size(), s.bytes)) != APT_OK) { scene_gpu_free(s); return st; }
-  out = s;
-  return APT_OK;
+  float *d_xyz = nullptr, *d_nrm = nullptr;
+  uint32_t* d_ids = nullptr;
+  size_t staged = 0;
+  apt_status st = upload(d_xyz, pos, (size_t)n * 9, staged);
+  if (st == APT_OK && nrm) st = upload(d_nrm, nrm, (size_t)n * 9, staged);
+  if (st == APT_OK) st = upload(d_ids, mat_ids ? mat_ids : ids.data(), (size_t)n, staged);
+  if (st == APT_OK)
+    st = scene_gpu_upload_device(d_xyz, d_nrm, d_ids, n, mats, n_mats, sky, bvh_width, lds_stack, out);
+  for (void* p : {(void*)d_xyz, (void*)d_nrm, (void*)d_ids})
+    if (p) (void)hipFree(p);
+  return st;
+}
+
+apt_status scene_flatten_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
+                               const ModelXform& x, float* out_xyz, float* out_nrm, uint32_t* out_mat) {
+  if (n == 0) return APT_OK;
+  hipLaunchKernelGGL(k_flatten, dim3(grid_for((size_t)n * 3)), dim3(256), 0, nullptr, d_xyz,
+                     out_nrm ? d_nrm : nullptr, d_mat, n, x, out_xyz, out_nrm, out_mat);
+  return sync_null("model flatten");
 }
 
 void scene_gpu_free(SceneGpu& s) {
