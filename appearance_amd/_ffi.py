@@ -117,6 +117,8 @@ _SIGS = {
     "apt_device_query": (C.c_int, [C.POINTER(DeviceInfo)]),
     "apt_bvh_build": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(BvhOpts), C.POINTER(_P)]),
     "apt_bvh_build_device": (C.c_int, [_P, C.c_uint32, C.POINTER(BvhOpts), C.POINTER(_P)]),
+    "apt_bvh_plan": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(BvhOpts), C.POINTER(BvhStats)]),
+    "apt_trace_stack_capacity": (C.c_uint32, []),
     "apt_bvh_destroy": (None, [_P]),
     "apt_bvh_get_stats": (C.c_int, [_P, C.POINTER(BvhStats)]),
     "apt_bvh_export": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t]),

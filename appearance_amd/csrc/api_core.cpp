@@ -145,6 +145,27 @@ apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_op
   return build_impl(tri_xyz, nullptr, n_tris, opts, out);
 }
 
+apt_status apt_bvh_plan(const float* tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts, apt_bvh_stats* out) {
+  clear_error();
+  APT_REQUIRE(out && tri_xyz && n_tris > 0, APT_ERR_INVALID_ARG, "apt_bvh_plan: null or empty argument");
+  APT_REQUIRE(n_tris < (1u << 28), APT_ERR_INVALID_ARG, "apt_bvh_plan: %u triangles exceed 2^28", n_tris);
+  apt_bvh_opts o;
+  apt_status st = resolve_opts(opts, o);
+  if (st != APT_OK) return st;
+  auto t0 = std::chrono::steady_clock::now();
+  BvhHostResult b2;
+  st = build_bvh2_host(tri_xyz, n_tris, o.max_leaf, o.bins, b2);
+  apt_bvh plan;
+  if (st == APT_OK)
+    st = upload_wide_bvh(tri_xyz, n_tris, b2, o.width, scene_pad(tri_xyz, n_tris), &plan, false);
+  if (st != APT_OK) return st;
+  plan.stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = plan.stats;
+  return check_trace_stack(plan.stats, "apt_bvh_plan");
+}
+
+uint32_t apt_trace_stack_capacity(void) { return trace_stack_capacity(); }
+
 apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
                                 apt_bvh** out) {
   clear_error();

@@ -141,14 +141,22 @@ constexpr uint32_t POOL_WORDS = POOL_PARTS * POOL_STRIDE;
 namespace apt {
 // Collapse a binary BVH into a wide one and upload it with the leaf-order
 // triangle records (bvh_host.cpp).
+// With upload = false only the wide tree's statistics are filled in (no
+// device memory is touched).
 apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, uint32_t width, float pad,
-                           apt_bvh* out);
+                           apt_bvh* out, bool upload = true);
 // The same binary tree built on the device from device-resident triangles
 // and collapsed there into `out` (bvh_gpu.hip): byte-identical to
 // build_bvh2_host + upload_wide_bvh (the SAH cost statistic is summed in
 // another order).  Synchronous on stream s.
 apt_status build_wide_device(const float* d_tri_xyz, uint32_t n, uint32_t max_leaf, uint32_t bins,
                              uint32_t width, hipStream_t s, apt_bvh* out);
+// Stack entries every traversal kernel holds whatever its LDS share (the
+// smallest LDS stack, 8, plus the private overflow array), and the check
+// every path that traces a BVH makes first: a deeper tree would lose subtrees
+// when the stack is full, so it is refused with APT_ERR_UNSUPPORTED (trace.hip).
+uint32_t trace_stack_capacity();
+apt_status check_trace_stack(const apt_bvh_stats& st, const char* who);
 // Launch the traversal kernels (trace.hip).
 apt_status launch_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
                                 apt_hit* d_hits, hipStream_t s);

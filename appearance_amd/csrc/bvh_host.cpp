@@ -324,7 +324,8 @@ void quantize_node(WNode<W>& w, int nk, const float (&clo)[W][3], const float (&
 }
 
 template <int W>
-apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad, apt_bvh* out) {
+apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad, apt_bvh* out,
+                           bool upload) {
   const std::vector<Node2>& bn = b2.nodes;
   std::vector<WNode<W>> wide;
   std::vector<uint32_t> wkids;  // the W BVH2 nodes under each wide node (~0u: empty slot)
@@ -456,6 +457,19 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
 
   size_t node_bytes = std::max<size_t>(1, wide.size()) * sizeof(WNode<W>);
   size_t tri_bytes = std::max<size_t>(1, (size_t)n * 3) * sizeof(float4);
+  out->stats.n_tris = n;
+  out->stats.n_nodes = (uint32_t)wide.size();
+  out->stats.n_leaves = n_leaves;
+  out->stats.width = W;
+  out->stats.depth = wide.empty() ? 0 : max_depth;
+  out->stats.max_stack = max_stack;
+  out->stats.device_bytes = node_bytes + tri_bytes;
+  out->stats.sah_cost = b2.sah_cost;
+  out->width = W;
+  out->n_tris = n;
+  out->n_nodes = (uint32_t)wide.size();
+  out->root = root;
+  if (!upload) return APT_OK;
   void* d_nodes = nullptr;
   void* d_tris = nullptr;
   APT_HIP(hipMalloc(&d_nodes, node_bytes));
@@ -468,10 +482,6 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
   if (!wide.empty()) APT_HIP(hipMemcpy(d_nodes, wide.data(), wide.size() * sizeof(WNode<W>), hipMemcpyHostToDevice));
   if (!tris.empty()) APT_HIP(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
 
-  out->width = W;
-  out->n_tris = n;
-  out->n_nodes = (uint32_t)wide.size();
-  out->root = root;
   out->d_nodes = d_nodes;
   out->d_tris = (float4*)d_tris;
   e = hipMalloc(&out->d_counters, APT_BVH_COUNTERS * POOL_WORDS * sizeof(uint32_t));
@@ -480,25 +490,17 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
     set_error("hipMalloc of the ray-pool counters failed: %s", hipGetErrorString(e));
     return APT_ERR_OOM;
   }
-  out->stats.n_tris = n;
-  out->stats.n_nodes = (uint32_t)wide.size();
-  out->stats.n_leaves = n_leaves;
-  out->stats.width = W;
-  out->stats.depth = wide.empty() ? 0 : max_depth;
-  out->stats.max_stack = max_stack;
-  out->stats.device_bytes = node_bytes + tri_bytes;
-  out->stats.sah_cost = b2.sah_cost;
   return APT_OK;
 }
 
 }  // namespace
 
 apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, uint32_t width, float pad,
-                           apt_bvh* out) {
+                           apt_bvh* out, bool upload) {
   switch (width) {
-    case 2: return collapse_upload<2>(tri_xyz, n, b2, pad, out);
-    case 4: return collapse_upload<4>(tri_xyz, n, b2, pad, out);
-    case 8: return collapse_upload<8>(tri_xyz, n, b2, pad, out);
+    case 2: return collapse_upload<2>(tri_xyz, n, b2, pad, out, upload);
+    case 4: return collapse_upload<4>(tri_xyz, n, b2, pad, out, upload);
+    case 8: return collapse_upload<8>(tri_xyz, n, b2, pad, out, upload);
     default:
       set_error("BVH width %u is not one of 2, 4, 8", width);
       return APT_ERR_INVALID_ARG;

@@ -290,6 +290,20 @@ __device__ __forceinline__ bool eval_light(const SceneView& sc, uint32_t light, 
   return true;
 }
 
+// A DI reservoir's contribution weight is in solid angle at the shading point
+// that made it: its light pdfs are p_A d^2 / cos_l there (solid-angle light
+// pdfs, a documented deviation from helpers/nee.wgsl:83-85, whose area-measure
+// pdfs need no conversion).  Reused at another shading point, the weight is
+// scaled by the ratio of the two points' solid-angle pdfs of the same light
+// point; a sun sample is a direction and keeps its weight.
+__device__ __forceinline__ float di_jacobian(const SceneView& sc, uint32_t light, float u, float v, v3 here,
+                                             v3 there) {
+  if (light == LIGHT_SUN || light == LIGHT_EMPTY) return 1.0f;
+  LightHit a, b;
+  if (!eval_light(sc, light, u, v, here, a) || !eval_light(sc, light, u, v, there, b)) return 0.0f;
+  return a.pdf > 0.0f ? b.pdf / a.pdf : 0.0f;
+}
+
 // trace_shadow_ray (helpers/trace.wgsl:20-84): opaque any-hit from the point
 // offset along the normal, tmax = distance - 2 eps.
 template <int W, int S>
@@ -745,6 +759,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(Resti
       const uint32_t qx = min((uint32_t)floorf(px + ux), p.width - 1);
       const uint32_t qy = min((uint32_t)floorf(py + uy), p.rows - 1);
       DiRes pr = b.di_hist[p.cur ^ 1][qy * p.width + qx];
+      pr.W *= di_jacobian(sc, pr.light, pr.u, pr.v, v.x, f4v(b.gbuf[p.cur ^ 1][qy * p.width + qx].pos_depth));
       pr.M = fminf(pr.M, (float)p.di_mcap * r.M);
       const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
       pr.phat = di_phat<W, S>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
@@ -869,6 +884,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
     }
     if (valid) {
       DiRes nr = b.di_tmp[q];
+      nr.W *= di_jacobian(sc, nr.light, nr.u, nr.v, v.x, f4v(ng.pos_depth));
       nr.phat = di_phat<W, S>(sc, st, v, pixel_bsdf(), nr.light, nr.u, nr.v, true, b.counters);
       valid = nr.phat > 0.0f;
       // DiReservoir::combine_unbiased: Z counts the M of the domains where the

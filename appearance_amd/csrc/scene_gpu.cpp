@@ -121,6 +121,8 @@ apt_status scene_gpu_upload_device(const float* d_xyz, const float* d_nrm, const
   apt_status st = widen(d_xyz, (size_t)n * 3, s.d_pos, s.bytes);
   if (st == APT_OK && (st = apt_bvh_build_device(d_xyz, n, &o, &s.bvh)) == APT_OK)
     s.bytes += s.bvh->stats.device_bytes;
+  // the renderer, the server and the autotuner trace every scene through here
+  if (st == APT_OK) st = check_trace_stack(s.bvh->stats, "scene");
   if (st == APT_OK && d_nrm) st = widen(d_nrm, (size_t)n * 3, s.d_nrm, s.bytes);
   if (st == APT_OK && (st = alloc_dev((void**)&s.d_mat, (size_t)n * sizeof(uint32_t))) == APT_OK) {
     s.bytes += (size_t)n * sizeof(uint32_t);

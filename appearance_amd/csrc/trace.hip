@@ -152,9 +152,8 @@ apt_status dispatch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_h
                     uint32_t* d_occ, bool any, hipStream_t s) {
   APT_REQUIRE(bvh && d_rays && (any ? d_occ != nullptr : d_hits != nullptr),
               APT_ERR_INVALID_ARG, "trace: null argument");
-  APT_REQUIRE(bvh->stats.max_stack <= 8 + TRACE_OVF_STACK, APT_ERR_UNSUPPORTED,
-              "trace: BVH needs a %u-entry stack, kernel holds %d", bvh->stats.max_stack,
-              8 + TRACE_OVF_STACK);
+  apt_status st = check_trace_stack(bvh->stats, "trace");
+  if (st != APT_OK) return st;
   switch (bvh->width) {
     case 2: return launch<2>(bvh, d_rays, n, d_hits, d_occ, any, s);
     case 4: return launch<4>(bvh, d_rays, n, d_hits, d_occ, any, s);
@@ -164,6 +163,15 @@ apt_status dispatch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_h
 }
 
 }  // namespace
+
+uint32_t trace_stack_capacity() { return 8u + (uint32_t)TRACE_OVF_STACK; }
+
+apt_status check_trace_stack(const apt_bvh_stats& st, const char* who) {
+  APT_REQUIRE(st.max_stack <= trace_stack_capacity(), APT_ERR_UNSUPPORTED,
+              "%s: the BVH needs a %u-entry traversal stack; the kernels hold %u (8 in LDS at the least, "
+              "%d private)", who, st.max_stack, trace_stack_capacity(), TRACE_OVF_STACK);
+  return APT_OK;
+}
 
 apt_status launch_trace_closest(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
                                 apt_hit* d_hits, hipStream_t s) {

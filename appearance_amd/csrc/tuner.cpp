@@ -35,7 +35,7 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
   std::vector<float> ref, img((size_t)npix * 3);
   double best = 1e30;
   apt_tune_result res{};
-  uint32_t trials = 0;
+  uint32_t trials = 0, rejected = 0;
   for (uint32_t w : widths) {
     apt_config cfg;
     apt_config_default(&cfg);
@@ -49,6 +49,16 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
     apt_status st = apt_create(&cfg, &r);
     if (st != APT_OK) return st;
     st = apt_scene_set(r, mesh, sky);
+    if (st == APT_ERR_UNSUPPORTED) {
+      // this width's tree is deeper than the traversal stack: not a candidate
+      char buf[96];
+      std::snprintf(buf, sizeof(buf), "%s{\"bvh_width\": %u, \"rejected\": \"traversal stack\"}",
+                    trials || rejected ? ", " : "", w);
+      js += buf;
+      rejected++;
+      apt_destroy(r);
+      continue;
+    }
     if (st == APT_OK) st = apt_world_action(r, 0, camera, sizeof(apt_camera));
     for (uint32_t s : stacks) {
       for (uint32_t q : chunks) {
@@ -75,7 +85,7 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
         const bool same = std::memcmp(ref.data(), img.data(), img.size() * sizeof(float)) == 0;
         char buf[160];
         std::snprintf(buf, sizeof(buf), "%s{\"bvh_width\": %u, \"lds_stack_depth\": %u, \"queue_chunk\": %u, \"ms\": %.4f, \"identical\": %s}",
-                      trials ? ", " : "", w, s, q, ms_best, same ? "true" : "false");
+                      trials || rejected ? ", " : "", w, s, q, ms_best, same ? "true" : "false");
         js += buf;
         trials++;
         if (!same) {
@@ -102,6 +112,9 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
     apt_destroy(r);
     if (st != APT_OK) return st;
   }
+  APT_REQUIRE(trials > 0, APT_ERR_UNSUPPORTED,
+              "apt_autotune: the scene's tree needs a deeper traversal stack than %u entries at every width",
+              apt::trace_stack_capacity());
   res.best_ms = best;
   res.trials = trials;
   char tail[200];

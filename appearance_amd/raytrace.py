@@ -23,6 +23,21 @@ BUILDER_HOST = 0
 BUILDER_GPU = 1
 
 
+def plan(tris: np.ndarray, width: int = 4, max_leaf: int = 4, bins: int = 32):
+    """Host build without the device (apt_bvh_plan): (status, statistics dict).
+    A nonzero status means the tree needs a deeper traversal stack than
+    stack_capacity() entries and every tracing entry point refuses it."""
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    st = BvhStats()
+    opts = BvhOpts(width, max_leaf, bins, BUILDER_HOST, 8, 0)
+    rc = lib().apt_bvh_plan(_ffi.ptr(tris), tris.shape[0], C.byref(opts), C.byref(st))
+    return rc, {f: getattr(st, f) for f, _ in BvhStats._fields_}
+
+
+def stack_capacity() -> int:
+    return int(lib().apt_trace_stack_capacity())
+
+
 class Bvh:
     """A wide BVH resident in device memory (libapt's apt_bvh)."""
 

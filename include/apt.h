@@ -124,6 +124,17 @@ apt_status apt_bvh_build(const float* tri_xyz, uint32_t n_tris, const apt_bvh_op
  * nothing but level sizes and statistics crosses to the host). */
 apt_status apt_bvh_build_device(const float* d_tri_xyz, uint32_t n_tris,
                                 const apt_bvh_opts* opts, apt_bvh** out);
+/* Builds on the host threads and reports the tree's statistics without
+ * touching the device, for checking a scene before it is uploaded.  Returns
+ * APT_ERR_UNSUPPORTED, with *out filled in, when the tree needs a deeper
+ * traversal stack (out->max_stack) than the kernels hold
+ * (apt_trace_stack_capacity()); every entry point that traces a BVH (the ray
+ * queries, apt_scene_set and world actions, apt_server_add_scene,
+ * apt_autotune) refuses such a tree the same way instead of losing subtrees.
+ * No counterpart in the reference (the driver builds and traverses there). */
+apt_status apt_bvh_plan(const float* tri_xyz, uint32_t n_tris, const apt_bvh_opts* opts,
+                        apt_bvh_stats* out);
+uint32_t apt_trace_stack_capacity(void);
 void apt_bvh_destroy(apt_bvh* bvh);
 apt_status apt_bvh_get_stats(const apt_bvh* bvh, apt_bvh_stats* out);
 /* Copies the device layout out: n_nodes wide nodes of 48 / 64 / 96 bytes for

@@ -35,6 +35,9 @@ void or_restir_destroy(void* st);
 int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
                     uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg20,
                     int threads, float* out, uint64_t* rays);
+void or_set_plant(int kind);
+int or_reservoir_probe(uint32_t kind, const float* a, const float* b, const float* extra, uint32_t* rng,
+                       float* out);
 void or_f16_round(const float* in, uint32_t n, float* out);
 void or_atrous_probe(uint32_t width, uint32_t height, uint32_t pass, const float* col, const float* var,
                      const float* depth, const float* nrm, float* out);

@@ -6,6 +6,37 @@
 // (shared/material/disney_bsdf.wgsl, ggxmdf.wgsl), sky and sun
 // (shared/sky_bindings.wgsl), the scene view and the traversal entry points
 // of oracle_core.cpp.
+// Licence notices.  The Disney BSDF below restates the reference's
+// shared/material/disney_bsdf.wgsl, which carries this notice:
+//
+//   This code has been adapted from AppleSeed: https://appleseedhq.net
+//   The AppleSeed software is released under the MIT license.
+//   Copyright (c) 2014-2018 Esteban Tovagliari, The appleseedhq Organization.
+//
+// and the reference itself (Appearance) is:
+//
+//   MIT License. Copyright (c) 2025 Temporal Interactive
+//
+// Both under these terms:
+//
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions:
+//
+//   The above copyright notice and this permission notice shall be included
+//   in all copies or substantial portions of the Software.
+//
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS
+//   OR IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF
+//   MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN
+//   NO EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM,
+//   DAMAGES OR OTHER LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR
+//   OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE
+//   USE OR OTHER DEALINGS IN THE SOFTWARE.
 #pragma once
 
 #include <algorithm>

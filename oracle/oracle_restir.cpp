@@ -371,6 +371,34 @@ bool eval_light(const Scene& sc, uint32_t light, F u, F v, const V3& x, LightHit
 
 F draw(uint32_t& st) { return ocr::draw<F>(st); }
 
+// Planted mistakes (tests only): or_set_plant selects one, 0 = none.  Each
+// one is a plausible slip in the reuse maths that a pin must catch.
+enum Plant : int {
+  PLANT_NONE = 0,
+  PLANT_Z_IS_M = 1,       // combine_unbiased: Z = M1 + M2 whatever the targets
+  PLANT_NO_DI_MCAP = 2,   // DI temporal: previous M not capped
+  PLANT_NO_INV_M = 3,     // combine: W = wsum / p^ (the 1/M dropped)
+  PLANT_NO_GI_MCAP = 4,   // GI temporal: previous M not capped
+  PLANT_Z_SWAP = 5,       // combine_unbiased: each domain's test adds the other's M
+  PLANT_NO_JACOBIAN = 6   // DI reuse: a reused weight keeps the other point's measure
+};
+int g_plant = PLANT_NONE;
+
+// A DI reservoir's contribution weight is in solid angle at the shading point
+// that made it: its light pdfs are p_A d^2 / cos_l there (solid-angle light
+// pdfs, a documented deviation from helpers/nee.wgsl:83-85, whose area-measure
+// pdfs need no conversion).  Reused at another shading point, the weight is
+// scaled by the ratio of the two points' solid-angle pdfs of the same light
+// point; a sun sample is a direction and keeps its weight.
+F di_jacobian(const Scene& sc, uint32_t light, F u, F v, const V3& here, const V3& there) {
+  if (light == LIGHT_SUN || light == LIGHT_EMPTY) return 1.0f;
+  LightHit a, b;
+  if (!eval_light(sc, light, u, v, here, a) || !eval_light(sc, light, u, v, there, b)) return 0.0f;
+  return a.pdf > 0.0f ? b.pdf / a.pdf : 0.0f;
+}
+
+
+
 bool res_update(F& wsum, F& M, F w, uint32_t& rng) {
   wsum += w;
   M += 1.0f;
@@ -387,6 +415,9 @@ F di_phat(const Scene& sc, const Vtx& v, const Bsdf& bs, uint32_t light, F u, F 
   return luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * cs);
 }
 
+
+// The two WRS updates of DiReservoir::combine (di_reservoir.wgsl:99-103):
+// weights p^ W M of each input, M = M1 + M2.
 DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng) {
   DiRes c;
   if (res_update(c.wsum, c.M, r1.phat * r1.W * r1.M, rng)) {
@@ -397,6 +428,64 @@ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint32_t& rng) {
   }
   c.M = r1.M + r2.M;
   return c;
+}
+
+// DiReservoir::combine (di_reservoir.wgsl:99-110): W = wsum / (M p^) of the
+// kept sample, 0 below RESTIR_DI_EPSILON.
+DiRes di_combine(const DiRes& r1, const DiRes& r2, uint32_t& rng) {
+  DiRes m = di_merge(r1, r2, rng);
+  const F inv_m = g_plant == PLANT_NO_INV_M ? 1.0f : 1.0f / m.M;
+  if (m.phat > RESTIR_DI_EPSILON && m.M * m.wsum > RESTIR_DI_EPSILON) m.W = (1.0f / m.phat) * (inv_m * m.wsum);
+  return m;
+}
+
+// DiReservoir::combine_unbiased (di_reservoir.wgsl:112-133): Z sums the M of
+// each input whose domain gives the kept sample a nonzero visibility-tested
+// target; at1 / at2 evaluate that target at the first / second domain.
+template <typename A1, typename A2>
+DiRes di_combine_unbiased(const DiRes& r1, const DiRes& r2, uint32_t& rng, A1&& at1, A2&& at2) {
+  DiRes m = di_merge(r1, r2, rng);
+  F z = 0.0f;
+  const bool in1 = at1(m) > 0.0f, in2 = at2(m) > 0.0f;
+  if (g_plant == PLANT_Z_IS_M) z = r1.M + r2.M;
+  else if (g_plant == PLANT_Z_SWAP) z = (in1 ? r2.M : 0.0f) + (in2 ? r1.M : 0.0f);
+  else z = (in1 ? r1.M : 0.0f) + (in2 ? r2.M : 0.0f);
+  if (m.phat > RESTIR_DI_EPSILON && z * m.wsum > RESTIR_DI_EPSILON) m.W = (1.0f / m.phat) * (1.0f / z * m.wsum);
+  return m;
+}
+
+// The temporal step of restir_di_temporal.wgsl:159-165 (stochastic branch):
+// the previous reservoir's M capped at mcap times the current M, its target
+// re-evaluated at this pixel (phat_here), then combine.
+DiRes di_temporal_merge(const DiRes& r, DiRes pr, F mcap, F phat_here, uint32_t& rng) {
+  if (g_plant != PLANT_NO_DI_MCAP) pr.M = std::min(pr.M, mcap * r.M);
+  pr.phat = phat_here;
+  return di_combine(r, pr, rng);
+}
+
+// GiReservoir::combine (gi_reservoir.wgsl:65-76) after the temporal pass's
+// cap of the previous M at mcap (restir_gi_temporal.wgsl:100-105);
+// phat_here is the previous sample's target re-evaluated at this pixel.
+GiRes gi_temporal_merge(const GiRes& r, const GiRes& pr, F mcap, F phat_here, uint32_t& rng) {
+  const F pM = g_plant == PLANT_NO_GI_MCAP ? pr.M : std::min(pr.M, mcap);
+  F wsum = 0, M = 0, phat = 0;
+  GiRes m;
+  if (res_update(wsum, M, r.phat * r.W * r.M, rng)) {
+    m.x2 = r.x2;
+    m.rng = r.rng;
+    phat = r.phat;
+  }
+  if (res_update(wsum, M, phat_here * pr.W * pM, rng)) {
+    m.x2 = pr.x2;
+    m.rng = pr.rng;
+    phat = phat_here;
+  }
+  M = r.M + pM;
+  m.M = M;
+  m.W = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
+  m.wsum = wsum;
+  m.phat = phat;
+  return m;
 }
 
 V3 approx_trace(const Scene& sc, const V3& x, const V3& wi, const V3& Tf, V3& first, uint32_t& rng,
@@ -720,13 +809,11 @@ void pass_di_temporal(Ctx& c) {
         const uint32_t qx = std::min((uint32_t)std::floor(px + ux), p.width - 1);
         const uint32_t qy = std::min((uint32_t)std::floor(py + uy), p.rows - 1);
         DiRes pr = s.di_hist[p.cur ^ 1][qy * p.width + qx];
-        pr.M = std::min(pr.M, (F)p.di_mcap * r.M);
+        if (g_plant != PLANT_NO_JACOBIAN)
+          pr.W *= di_jacobian(c.sc, pr.light, pr.u, pr.v, v.x, s.gbuf[p.cur ^ 1][qy * p.width + qx].pos);
         const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
-        pr.phat = di_phat(c.sc, v, bs, pr.light, pr.u, pr.v, true, c.cnt);
-        DiRes m = di_merge(r, pr, rng);
-        if (m.phat > RESTIR_DI_EPSILON && m.M * m.wsum > RESTIR_DI_EPSILON)
-          m.W = (1.0f / m.phat) * (1.0f / m.M * m.wsum);
-        r = m;
+        const F ph = di_phat(c.sc, v, bs, pr.light, pr.u, pr.v, true, c.cnt);
+        r = di_temporal_merge(r, pr, (F)p.di_mcap, ph, rng);
       }
     }
     s.di_tmp[pix] = r;
@@ -771,19 +858,20 @@ void pass_di_spatial(Ctx& c) {
       if (valid) valid = std::fabs(cd - depth_cs(nt.depth)) / cd < 0.1f && dotp(ct.n, nt.n) > 0.906f;
       if (valid) {
         DiRes nr = s.di_tmp[q];
+        if (g_plant != PLANT_NO_JACOBIAN) nr.W *= di_jacobian(c.sc, nr.light, nr.u, nr.v, v.x, nt.pos);
         nr.phat = di_phat(c.sc, v, bs, nr.light, nr.u, nr.v, true, c.cnt);
         valid = nr.phat > 0.0f;
-        DiRes m = di_merge(r, nr, rng);
-        F z = 0.0f;
-        if (di_phat(c.sc, v, bs, m.light, m.u, m.v, true, c.cnt) > 0.0f) z += r.M;
         Vtx vn;
-        if (load_vtx(s, q, vn)) {
-          vn.x = nt.pos;
-          const Bsdf bn = make_bsdf(c.sc, vn.mat, vn.n, vn.back, vn.t);
-          if (di_phat(c.sc, vn, bn, m.light, m.u, m.v, true, c.cnt) > 0.0f) z += nr.M;
-        }
-        if (m.phat > RESTIR_DI_EPSILON && z * m.wsum > RESTIR_DI_EPSILON) m.W = (1.0f / m.phat) * (1.0f / z * m.wsum);
-        r = m;
+        const bool has_vn = load_vtx(s, q, vn);
+        vn.x = nt.pos;
+        r = di_combine_unbiased(
+            r, nr, rng,
+            [&](const DiRes& m) { return di_phat(c.sc, v, bs, m.light, m.u, m.v, true, c.cnt); },
+            [&](const DiRes& m) {
+              if (!has_vn) return 0.0f;
+              const Bsdf bn = make_bsdf(c.sc, vn.mat, vn.n, vn.back, vn.t);
+              return di_phat(c.sc, vn, bn, m.light, m.u, m.v, true, c.cnt);
+            });
       }
       if (!valid) radius = std::max(radius * 0.5f, 3.0f);
     }
@@ -814,27 +902,9 @@ void pass_gi_temporal(Ctx& c) {
         const Tex prev = sample_prev(s.gbuf[p.cur ^ 1], p, px, py);
         if (texel_similar(cur, prev)) {
           const GiRes pr = s.gi_hist[p.cur ^ 1][qy * p.width + qx];
-          const F pM = std::min(pr.M, (F)p.gi_mcap);
           const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
           const F pp = gi_phat(c.sc, v, bs, s.thr_T[pix], pr, c.cnt);
-          F wsum = 0, M = 0, phat = 0;
-          GiRes m;
-          if (res_update(wsum, M, r.phat * r.W * r.M, rng)) {
-            m.x2 = r.x2;
-            m.rng = r.rng;
-            phat = r.phat;
-          }
-          if (res_update(wsum, M, pp * pr.W * pM, rng)) {
-            m.x2 = pr.x2;
-            m.rng = pr.rng;
-            phat = pp;
-          }
-          M = r.M + pM;
-          m.M = M;
-          m.W = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
-          m.wsum = wsum;
-          m.phat = phat;
-          r = m;
+          r = gi_temporal_merge(r, pr, (F)p.gi_mcap, pp, rng);
         }
       }
       s.rng[pix] = rng;
@@ -1234,6 +1304,62 @@ int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t
   if (rays) {
     rays[0] = s.rays[0];
     rays[1] = s.rays[1];
+  }
+  return 0;
+}
+
+// Plants one of the oracle's reuse mistakes (ors::Plant; 0 = none) so the
+// tests can show that their pins catch it.  Process-wide; tests only.
+void or_set_plant(int kind) { ors::g_plant = kind; }
+
+// Reservoir merges on caller-made reservoirs, the functions the reuse passes
+// run.  DI reservoirs are 7 floats (light id bits, u, v, wsum, M, W, p^), GI
+// reservoirs 8 (sample point xyz, p^ rng bits, M, W, wsum, p^).
+//   kind 0: DiReservoir::combine(a, b)
+//   kind 1: DiReservoir::combine_unbiased(a, b); extra[0..3] = the target of
+//           a's sample at domain 1 and at domain 2, then of b's sample at
+//           domain 1 and at domain 2
+//   kind 2: the DI temporal step; extra[0] = M cap factor, extra[1] = the
+//           previous sample's target at this pixel
+//   kind 3: the GI temporal step; extra[0] = M cap, extra[1] = as kind 2
+// *rng is the WRS state in and out; out receives the merged reservoir.
+int or_reservoir_probe(uint32_t kind, const float* a, const float* b, const float* extra, uint32_t* rng,
+                       float* out) {
+  using namespace ors;
+  auto di = [](const float* f) {
+    DiRes r;
+    std::memcpy(&r.light, f, 4);
+    r.u = f[1]; r.v = f[2]; r.wsum = f[3]; r.M = f[4]; r.W = f[5]; r.phat = f[6];
+    return r;
+  };
+  auto gi = [](const float* f) {
+    GiRes r;
+    r.x2 = V3(f[0], f[1], f[2]);
+    std::memcpy(&r.rng, f + 3, 4);
+    r.M = f[4]; r.W = f[5]; r.wsum = f[6]; r.phat = f[7];
+    return r;
+  };
+  auto put_di = [&](const DiRes& r) {
+    std::memcpy(out, &r.light, 4);
+    out[1] = r.u; out[2] = r.v; out[3] = r.wsum; out[4] = r.M; out[5] = r.W; out[6] = r.phat;
+  };
+  if (kind == 0) {
+    put_di(di_combine(di(a), di(b), *rng));
+  } else if (kind == 1) {
+    const DiRes ra = di(a), rb = di(b);
+    auto is_b = [&](const DiRes& m) { return m.light == rb.light && m.u == rb.u && m.v == rb.v; };
+    put_di(di_combine_unbiased(
+        ra, rb, *rng, [&](const DiRes& m) { return is_b(m) ? extra[2] : extra[0]; },
+        [&](const DiRes& m) { return is_b(m) ? extra[3] : extra[1]; }));
+  } else if (kind == 2) {
+    put_di(di_temporal_merge(di(a), di(b), extra[0], extra[1], *rng));
+  } else if (kind == 3) {
+    const GiRes m = gi_temporal_merge(gi(a), gi(b), extra[0], extra[1], *rng);
+    out[0] = m.x2.x; out[1] = m.x2.y; out[2] = m.x2.z;
+    std::memcpy(out + 3, &m.rng, 4);
+    out[4] = m.M; out[5] = m.W; out[6] = m.wsum; out[7] = m.phat;
+  } else {
+    return -1;
   }
   return 0;
 }

@@ -89,3 +89,25 @@ def sun_plane_radiance(albedo, sky) -> float:
     Ls = sky.sun_intensity * max(0.0, 1 - np.exp(-((cut - np.arccos(l[1])) / 1.4)))
     alpha = sky.sun_size * 0.1
     return albedo * Ls * np.sin(alpha) ** 2 * l[1]
+
+
+def deep_tree_triangles(n: int = 600, base: float = 1.5) -> np.ndarray:
+    """Tiny triangles at x = base^-k, k = 0..n-1: each binned-SAH split peels a
+    few triangles off the far end, so the tree is a long caterpillar whose
+    4- and 8-wide collapses need more traversal stack than the kernels hold
+    (the 2-wide one fits)."""
+    x = base ** (-np.arange(n, dtype=np.float64))
+    s = x * 0.01
+    t = np.zeros((n, 9), np.float64)
+    t[:, 0] = x
+    t[:, 3] = x + s
+    t[:, 6] = x
+    t[:, 7] = s
+    return np.ascontiguousarray(t, np.float32)
+
+
+def deep_tree_scene() -> SceneData:
+    pos = deep_tree_triangles()
+    m = material(color=(0.5,) * 3, bsdf_model=1)
+    return SceneData(pos, None, np.zeros(len(pos), np.uint32), [m],
+                     camera((0.5, 0.0, 2.0), (0.5, 0.0, 0.0)), Sky())

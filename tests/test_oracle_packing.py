@@ -66,3 +66,21 @@ def test_oct30_axes_and_layout(oracle):
     # +x maps to (1, 0.5)
     assert int(p[2]) & 0x7FFF == 32767
     assert int(p[0]) >> 30 == 0
+
+
+def test_oct30_ties_round_to_even(oracle):
+    """WGSL round() is ties-to-even (packing.wgsl:33-34); Rust's f32::round
+    in the reference's host twin is ties-away (crates/appearance-packing/src/lib.rs:55-56).
+    These directions put the x field exactly on k + 0.5 with k even (the
+    float32 chain |x|+|y|+|z|, x/s, *0.5+0.5, *32767 is exact for them), where
+    the two rules differ: the codec follows WGSL, k, not k + 1."""
+    f = np.float32
+    for k, a in [(16386, 0.00018310546875), (16388, 0.00030517578125), (16384, 6.103515625e-05)]:
+        a = f(a)
+        b = f(f(1.0) - a)
+        s = f(a + b)
+        ex = f(f(f(a / s) * f(0.5)) + f(0.5))
+        assert f(ex * f(32767.0)) == f(k + 0.5)      # an exact tie
+        p = int(oracle.oct30_pack(np.array([[a, b, 0.0]], np.float32))[0])
+        assert p & 0x7FFF == k, (k, p & 0x7FFF)
+        assert k % 2 == 0

@@ -3,6 +3,12 @@
 - With reuse off the ReSTIR estimator is unbiased: its mean over many frames
   equals the plain oracle path tracer at pixel centres, pixel by pixel within
   a z-score bound (the plain tracer is pinned by the furnace and FP64 checks).
+- With reuse on, on a static camera, it stays unbiased: the mean over many
+  independent frame sequences of the last frame's image equals the plain
+  tracer for DI temporal reuse (M cap engaged), DI spatial reuse (unbiased
+  combination with its Z count and the solid-angle Jacobian of reused light
+  samples), GI temporal reuse, and all of them together; planted mistakes in
+  Z, 1/M and the Jacobian make the check fail.
 - The a-trous pass keeps a constant image constant, reproduces a linear ramp
   inside a flat region, and spreads an impulse with the closed-form weights
   h(dx) h(dy) / (1 + 55/64), h = (3/8, 1/4, 1/16), at the pass's step size.
@@ -67,6 +73,104 @@ def test_restir_without_reuse_is_unbiased(oracle, name, seed, w, h, bounces):
     # the check has the power to see a 2% bias
     zb = (m * 1.02 - rm) / sd
     assert np.sqrt((zb ** 2).mean()) > 1.6
+
+
+def _plain_reference(oracle, s, w, h, bounces, bvh, batches=8, spp=256):
+    refs = np.array([oracle_render(oracle, s, w, h, spp=spp, max_bounces=bounces, rr_start=3,
+                                   jitter=False, frame_idx=1000 + b, bvh=bvh)[0] for b in range(batches)],
+                    np.float64)
+    return refs.mean(0), refs.var(0, ddof=1) / batches
+
+
+def _reuse_sequences(oracle, s, w, h, bounces, cfg, sequences, frames, bvh):
+    """Mean and variance of the mean of the last frame of `sequences`
+    independent frame sequences (a fresh reservoir history each, seeds far
+    apart) on the scene's static camera."""
+    acc = np.zeros((h, w, 3))
+    acc2 = np.zeros((h, w, 3))
+    for k in range(sequences):
+        r = OracleRestir(oracle, s, w, h, bvh=bvh)
+        for f in range(frames):
+            img = r.frame(cfg, frame_idx=f, seed=7919 * k + 13, max_bounces=bounces, threads=1)
+        img = img.astype(np.float64)
+        acc += img
+        acc2 += img * img
+    m = acc / sequences
+    return m, np.maximum(acc2 / sequences - m * m, 0.0) / sequences
+
+
+def _reuse_z(m, var, rm, rv):
+    sd = np.sqrt(var + rv + (1e-6 * np.abs(rm)) ** 2 + 1e-20)
+    z = (m - rm) / sd
+    return float(np.sqrt((z ** 2).mean())), float((np.abs(z) > 5).mean()), \
+        float(np.sqrt((((m * 1.02 - rm) / sd) ** 2).mean()))
+
+
+# Reuse modes: config overrides and frames per sequence.  The caps are small
+# so they engage within the sequence (M grows by the 5 candidates of a frame).
+REUSE_MODES = {
+    "di_temporal": (dict(restir_di=1, restir_gi=0, di_spatial_neighbours=0, di_mcap=2), 4),
+    "di_spatial": (dict(restir_di=1, restir_gi=0, di_spatial_neighbours=5, di_spatial_radius=6.0), 1),
+    "gi_temporal": (dict(restir_di=0, restir_gi=1, gi_mcap=2), 4),
+    "all": (dict(restir_di=1, restir_gi=1, di_spatial_neighbours=5, di_spatial_radius=6.0, di_mcap=2,
+                 gi_mcap=3), 4),
+}
+
+
+@pytest.fixture(scope="module")
+def cornell_reuse(oracle):
+    s = generate_scene("cornell")
+    bvh = oracle.bvh(np.ascontiguousarray(s.positions, np.float32))
+    return s, bvh, _plain_reference(oracle, s, 24, 23, 2, bvh)
+
+
+@pytest.mark.parametrize("mode", list(REUSE_MODES))
+def test_restir_with_reuse_is_unbiased(oracle, cornell_reuse, mode):
+    s, bvh, (rm, rv) = cornell_reuse
+    kw, frames = REUSE_MODES[mode]
+    cfg = default_config(firefly_filter=0, **kw)
+    m, var = _reuse_sequences(oracle, s, 24, 23, 2, cfg, 512, frames, bvh)
+    zrms, tail, power = _reuse_z(m, var, rm, rv)
+    rel = abs(m.mean() - rm.mean()) / rm.mean()
+    print(f"[restir] cornell {mode}: mean {m.mean():.6f} plain {rm.mean():.6f} rel {rel:.2e} z rms {zrms:.2f}")
+    assert rel < 3e-3
+    assert zrms < 1.5
+    assert tail < 5e-3
+    assert power > 1.6
+
+
+@pytest.mark.parametrize("mode,plant_kind", [("di_spatial", 1), ("di_spatial", 5), ("di_spatial", 6),
+                                             ("di_temporal", 3)])
+def test_restir_reuse_planted_errors_are_caught(oracle, cornell_reuse, mode, plant_kind):
+    """Z = M1 + M2, the two domains' M swapped in Z, no Jacobian on reused
+    light samples, no 1/M: each biases the reuse check beyond its bound."""
+    from tests.test_oracle_reservoir import plant
+    s, bvh, (rm, rv) = cornell_reuse
+    kw, frames = REUSE_MODES[mode]
+    cfg = default_config(firefly_filter=0, **kw)
+    with plant(oracle, plant_kind):
+        m, var = _reuse_sequences(oracle, s, 24, 23, 2, cfg, 512, frames, bvh)
+    zrms, _, _ = _reuse_z(m, var, rm, rv)
+    print(f"[restir] cornell {mode} with plant {plant_kind}: z rms {zrms:.2f}")
+    assert zrms > 2.5
+
+
+def test_restir_with_all_reuse_is_unbiased_on_the_many_light_scene(oracle):
+    """The Disney spheres with 4096 emissive triangles and the sun: every
+    reuse pass on, three frames per sequence."""
+    s = generate_scene("restir", seed=4)
+    bvh = oracle.bvh(np.ascontiguousarray(s.positions, np.float32))
+    rm, rv = _plain_reference(oracle, s, 24, 23, 2, bvh)
+    kw, _ = REUSE_MODES["all"]
+    m, var = _reuse_sequences(oracle, s, 24, 23, 2, default_config(firefly_filter=0, **kw), 512, 3, bvh)
+    zrms, tail, power = _reuse_z(m, var, rm, rv)
+    rel = abs(m.mean() - rm.mean()) / rm.mean()
+    print(f"[restir] many-light all reuse: mean {m.mean():.6f} plain {rm.mean():.6f} rel {rel:.2e} "
+          f"z rms {zrms:.2f}")
+    assert rel < 6e-3
+    assert zrms < 1.5
+    assert tail < 5e-3
+    assert power > 1.6
 
 
 def _flat(h, w):
