@@ -65,9 +65,11 @@ struct PathView {
   uint32_t* counters;  // Counter
 };
 
+// stage_ev: NULL, or 6 * fv.max_bounces events recorded around each bounce's
+// extend, shade and shadow launches (start, end per stage).
 apt_status integrate_pass(const FrameView& fv, const SceneView& sc, const PathView& ps,
                           uint32_t n_paths, uint32_t sample_base, uint32_t s_count,
-                          float4* frame_sum, hipStream_t s);
+                          float4* frame_sum, hipStream_t s, hipEvent_t* stage_ev = nullptr);
 apt_status resolve_frame(const FrameView& fv, const float4* frame_sum, float4* accum_tm,
                          float4* accum_lin, uint32_t accum_frames, uint32_t* out_rgba,
                          hipStream_t s);

@@ -197,16 +197,25 @@ __global__ __launch_bounds__(BLOCK) void k_resolve(FrameView fv, const float4* _
 
 template <int W>
 void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_paths, int qin,
-                           const FrameView& fv, uint32_t bounce, hipStream_t s) {
+                           const FrameView& fv, uint32_t bounce, hipStream_t s, hipEvent_t* ev) {
   // the traversal stages pull from their queue and the shade stage strides
   // over it: grids that fill the device
   const dim3 grid(std::min<uint32_t>(pool_blocks(), (n_paths + BLOCK - 1) / BLOCK));
   const dim3 pool(std::min<uint32_t>(pool_blocks(), (n_paths + 63) / 64));
   const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
+  const auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev[k], s);
+  };
   APT_FOR_STACK_DEPTH(sc.lds_stack, {
+    mark(0);
     hipLaunchKernelGGL((k_extend<W, S_>), pool, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
+    mark(1);
+    mark(2);
     hipLaunchKernelGGL(k_shade, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
+    mark(3);
+    mark(4);
     hipLaunchKernelGGL((k_shadow<W, S_>), pool, dim3(BLOCK), 0, s, sc, ps, nodes);
+    mark(5);
   })
 }
 
@@ -214,15 +223,16 @@ void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_p
 
 apt_status integrate_pass(const FrameView& fv, const SceneView& sc, const PathView& ps,
                           uint32_t n_paths, uint32_t sample_base, uint32_t s_count,
-                          float4* frame_sum, hipStream_t s) {
+                          float4* frame_sum, hipStream_t s, hipEvent_t* stage_ev) {
   const dim3 grid((n_paths + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL(k_raygen, grid, dim3(BLOCK), 0, s, fv, sc, ps, n_paths, sample_base);
   for (uint32_t b = 0; b < fv.max_bounces; ++b) {
     const int qin = (int)(b & 1u);
+    hipEvent_t* ev = stage_ev ? stage_ev + 6 * b : nullptr;
     switch (sc.width) {
-      case 2: launch_bounce_kernels<2>(sc, ps, n_paths, qin, fv, b, s); break;
-      case 4: launch_bounce_kernels<4>(sc, ps, n_paths, qin, fv, b, s); break;
-      case 8: launch_bounce_kernels<8>(sc, ps, n_paths, qin, fv, b, s); break;
+      case 2: launch_bounce_kernels<2>(sc, ps, n_paths, qin, fv, b, s, ev); break;
+      case 4: launch_bounce_kernels<4>(sc, ps, n_paths, qin, fv, b, s, ev); break;
+      case 8: launch_bounce_kernels<8>(sc, ps, n_paths, qin, fv, b, s, ev); break;
       default: set_error("integrator: unsupported BVH width %u", sc.width); return APT_ERR_INVALID_ARG;
     }
   }

@@ -235,6 +235,9 @@ struct apt_renderer {
   DevBuf<uint32_t> p_rng, p_q0, p_q1, p_qs, p_cnt;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // start/end events around each extend, shade and shadow launch of a frame
+  // (6 per bounce per pass), for the per-stage device times
+  std::vector<hipEvent_t> stage_ev;
   apt_render_stats stats{};
   uint32_t frame_idx = 0;
   PinnedBytes cb_pixels;  // the callback's borrowed RGBA8 slab
@@ -257,6 +260,7 @@ struct apt_renderer {
     cb_pixels.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : stage_ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -532,10 +536,21 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
   ps.rng = r->p_rng.p; ps.queue[0] = r->p_q0.p; ps.queue[1] = r->p_q1.p;
   ps.shadow_queue = r->p_qs.p; ps.counters = r->p_cnt.p;
 
+  // Stage events for every launch of the frame, up to a bounded count (a
+  // frame of many passes is timed as a whole only).
+  const uint32_t passes = (spp + per_pass - 1) / per_pass;
+  const size_t n_ev = (size_t)passes * 6 * r->cfg.max_bounces;
+  const bool staged = n_ev <= 1536;
+  while (staged && r->stage_ev.size() < n_ev) {
+    hipEvent_t e;
+    APT_HIP(hipEventCreate(&e));
+    r->stage_ev.push_back(e);
+  }
   APT_HIP(hipEventRecord(r->ev0, s));
-  for (uint32_t base = 0; base < spp; base += per_pass) {
+  for (uint32_t base = 0, pass = 0; base < spp; base += per_pass, ++pass) {
     const uint32_t cnt = std::min(per_pass, spp - base);
-    st = integrate_pass(fv, sc, ps, (uint32_t)(cnt * npix), base, cnt, r->d_frame_sum.p, s);
+    hipEvent_t* ev = staged ? r->stage_ev.data() + (size_t)pass * 6 * r->cfg.max_bounces : nullptr;
+    st = integrate_pass(fv, sc, ps, (uint32_t)(cnt * npix), base, cnt, r->d_frame_sum.p, s, ev);
     if (st != APT_OK) return st;
   }
   st = resolve_frame(fv, r->d_frame_sum.p, r->d_accum_tm.p, r->d_accum_lin.p, r->accum_frames,
@@ -555,6 +570,18 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
   r->stats.shadow_rays = cnt[CNT_STAT_SHADOW];
   r->stats.segments = cnt[CNT_STAT_EXT];
   r->stats.ms_total = ms;
+  if (staged) {
+    double stage_ms[3] = {0.0, 0.0, 0.0};
+    for (size_t k = 0; k < n_ev; k += 2) {
+      float e = 0.0f;
+      APT_HIP(hipEventElapsedTime(&e, r->stage_ev[k], r->stage_ev[k + 1]));
+      stage_ms[(k / 2) % 3] += e;
+    }
+    r->stats.ms_extend = stage_ms[0];
+    r->stats.ms_shade = stage_ms[1];
+    r->stats.ms_shadow = stage_ms[2];
+    r->stats.ms_other = std::max(0.0, ms - stage_ms[0] - stage_ms[1] - stage_ms[2]);
+  }
   r->accum_frames++;
   r->frame_idx++;
   return APT_OK;

@@ -49,8 +49,12 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
     apt_status st = apt_create(&cfg, &r);
     if (st != APT_OK) return st;
     st = apt_scene_set(r, mesh, sky);
+    if (st == APT_OK) st = apt_world_action(r, 0, camera, sizeof(apt_camera));
+    // The scene is built at the first render: a tree deeper than the
+    // traversal stack at this width is refused there and is not a candidate.
+    if (st == APT_OK) st = apt_render(r, o.probe_width, o.probe_height, 0, o.probe_height, nullptr, 0, img.data(),
+                                      img.size());
     if (st == APT_ERR_UNSUPPORTED) {
-      // this width's tree is deeper than the traversal stack: not a candidate
       char buf[96];
       std::snprintf(buf, sizeof(buf), "%s{\"bvh_width\": %u, \"rejected\": \"traversal stack\"}",
                     trials || rejected ? ", " : "", w);
@@ -59,7 +63,6 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
       apt_destroy(r);
       continue;
     }
-    if (st == APT_OK) st = apt_world_action(r, 0, camera, sizeof(apt_camera));
     for (uint32_t s : stacks) {
       for (uint32_t q : chunks) {
         if (st != APT_OK) break;

@@ -120,6 +120,13 @@ class PathTracerGpu:
               "apt_render")
         return rgba
 
+    def render_device(self, resolution, start_row: int = 0, end_row: int | None = None) -> None:
+        """Render into the renderer's device buffers only (both outputs NULL):
+        the frame is resolved in HBM and nothing crosses PCIe."""
+        w, h = int(resolution[0]), int(resolution[1])
+        end_row = h if end_row is None else end_row
+        check(lib().apt_render(self._h, w, h, start_row, end_row, None, 0, None, 0), "apt_render")
+
     def set_frame_index(self, frame_idx: int) -> None:
         """Restart the frame sequence at frame_idx (seeds frame * spp + sample)."""
         check(lib().apt_set_frame_index(self._h, frame_idx), "apt_set_frame_index")

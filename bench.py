@@ -1,16 +1,27 @@
 """Benchmark of the path-tracing hot path on one MI355X.
 
-Default workload (bvh_stress): 1,048,576 random triangles (seed 1) and
-2,097,152 rays (1M coherent primary rays of a 1024x1024 pinhole camera at
-(0,0,-25), 1M incoherent rays, seed 2).  One step is one closest-hit pass
-over all rays (libapt's apt_trace_closest) with rays and hits resident in
-HBM.  Prints one JSON line.
+Default workload (disney): the Disney-BSDF scene (256 UV spheres of 2048
+triangles + a 2-triangle ground, 524,290 triangles, seed 3) with the analytic
+sky and sun, 1920x1088 (1080 rounded up to the 64-row block layout), 4 spp,
+depth 8, Russian roulette from bounce 3, FP32, the wavefront path tracer.
+One step is one frame (apt_render) with the scene resident in HBM and the
+frame resolved into device buffers; `value` is Msamples/s over the timed
+frames, Mrays/s (extension + shadow rays) beside it.  The same frame through
+the render-node entry (RGBA8 block layout read back to host memory) is
+reported as msamples_s_render_node.
 
---workload disney: the Disney-BSDF scene (256 UV spheres, 524,290 triangles)
-at 1920x1080, 4 spp, depth 8; one step is one apt_render call (one frame);
-value is Msamples/s, Mrays/s is reported beside it.
+roofline: the dominant kernel, the extend stage (k_extend, closest-hit
+traversal of the extension rays): its algorithmic bytes per frame (the
+extension rays per sample times the bytes per closest-hit ray of the canonical
+BVH2, tests/golden/disney_sample_counts.json) over its device time per frame
+(HIP events around each launch, on the launch stream).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bvh_stress|disney]
+Beside it: the BVH-stress traversal (1,048,576 random triangles, 2,097,152
+rays, closest and any hit) as `bvh_stress`, and the CPU baseline: the FP32
+oracle path tracer on a bounded pixel sample of the same frame.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--workload disney|bvh_stress|restir|serving|bvh_build]
                   [--no-cpu-baseline]
 """
 from __future__ import annotations
@@ -28,20 +39,22 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Msamples/s & Mrays/s at 1080p; served p50/p99 latency vs one-at-a-time"
 HBM_PEAK_GBS = 8000.0
 COUNTS = ROOT / "tests" / "golden" / "bvh_stress_counts.json"
+DISNEY_COUNTS = ROOT / "tests" / "golden" / "disney_sample_counts.json"
 
 
 COPY_BYTES = 256 << 20
 TRAFFIC = ROOT / "profiles" / "trace_closest_bvh_stress_traffic.md"
+EXTEND_TRAFFIC = ROOT / "profiles" / "extend_disney_traffic.md"
 
 
-def measured_traffic():
-    """HBM bytes per closest-hit launch from the committed FETCH_SIZE/WRITE_SIZE
-    counter summary (already scaled by the copy-kernel calibration), or None."""
+def measured_traffic(path=TRAFFIC, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch from a committed FETCH_SIZE/WRITE_SIZE counter
+    summary (already scaled by the copy-kernel calibration), or None."""
     import re
 
-    if not TRAFFIC.exists():
+    if not path.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC.read_text())
+    m = re.search(key + r"\s*=\s*([0-9.eE+]+)", path.read_text())
     return float(m.group(1)) if m else None
 
 
@@ -49,83 +62,61 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline_trace(tris, rays, closest_bytes):
-    """The oracle's BVH2 traversal on the host cores, on a bounded sample."""
-    import numpy as np
-
-    from tests._oracle import load_oracle
-
-    o = load_oracle()
-    threads = max(1, os.cpu_count() or 1)
-    t0 = time.time()
-    b = o.bvh(tris)
-    build_s = time.time() - t0
-    n_half = 1 << 20
-    rng = np.random.default_rng(0)
-    # 1/4 of each ray set, so the sample has the workload's coherent/incoherent mix
-    k = n_half // 4
-    idx = np.concatenate([rng.choice(n_half, k, replace=False),
-                          n_half + rng.choice(n_half, k, replace=False)])
-    sample = np.ascontiguousarray(rays.view(np.float32).reshape(-1, 8)[np.sort(idx)])
-    b.trace(sample[:4096], threads=threads)  # warm-up
-    times = []
-    for _ in range(3):
-        t0 = time.time()
-        b.trace(sample, threads=threads)
-        times.append(time.time() - t0)
-    t = sorted(times)[1]
-    return {"value": sample.shape[0] / t / 1e6, "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{sample.shape[0]} rays (1/4 of the coherent and incoherent sets), "
-                      f"closest hit, oracle BVH2 built in {build_s:.1f} s, median of 3"}
+def host_cores() -> int:
+    """Threads for the CPU baseline: the cores this process may run on, at
+    most 16 (a one-GPU share of the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--width", type=int, default=4)
-    ap.add_argument("--lds-stack", type=int, default=8)
-    ap.add_argument("--workload", default="bvh_stress",
-                    choices=["bvh_stress", "disney", "restir", "serving", "bvh_build"])
-    ap.add_argument("--requests", type=int, default=96)
-    ap.add_argument("--rates", default="20,200")
-    ap.add_argument("--policy", type=int, default=0,
-                    help="serving: 0 = one-batch requests shortest first, then the rest in "
-                         "arrival order; 1 = FIFO; 2 = shortest remaining work first")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="run one 256 MiB device-to-device copy before timing (the known-size "
-                         "kernel that scales the HBM counters of a --pmc run)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              f"(slabs, rays and requests would shard per GPU with no exchange)", file=sys.stderr)
-        return 2
+def cpu_model() -> tuple[str, set]:
+    name, flags = "unknown", set()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name") and name == "unknown":
+                name = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return name, flags
 
-    if args.workload == "disney":
-        return bench_disney(args)
-    if args.workload == "serving":
-        return bench_serving(args)
-    if args.workload == "restir":
-        return bench_restir(args)
-    if args.workload == "bvh_build":
-        return bench_bvh_build(args)
 
+def baseline_oracle():
+    """The oracle build the CPU baseline times: -O3 -march=x86-64-v3 where
+    the host has AVX2, FMA and BMI2 (the two builds give the same bits), else
+    the generic -O3 build.  Returns (oracle, build flags, cpu model)."""
+    import ctypes as C
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import build as oracle_build  # noqa: E402  (oracle/build.py: the flag lists only)
+    sys.path.pop(0)
+    from tests._oracle import LIB, Oracle, load_oracle
+
+    name, flags = cpu_model()
+    v3 = oracle_build.OUT_V3
+    if {"avx2", "fma", "bmi2"} <= flags and v3.exists():
+        return Oracle(C.CDLL(str(v3))), oracle_build.baseline_flags(), name
+    return load_oracle(), " ".join(oracle_build.FLAGS[:1] + ["-ffp-contract=off"]) + " (generic x86-64)", name
+
+
+def bench_bvh_stress(args, steps: int | None = None):
+    """The BVH-stress traversal: closest and any hit over all 2,097,152 rays
+    with rays and hits resident in HBM.  Returns its result dict."""
     import numpy as np
     import torch
 
     from appearance_amd.raytrace import BUILDER_GPU, Bvh, rays_bvh_stress
     from appearance_amd.scene import generate_scene
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
-    log("generating bvh_stress scene and rays")
+    steps = args.steps if steps is None else steps
+    log("bvh_stress: generating the scene and rays")
     scene = generate_scene("bvh_stress", seed=1)
     rays = rays_bvh_stress(seed=2, side=1024, n_incoherent=1 << 20)
     n = rays.shape[0]
-    log(f"building BVH (width {args.width})")
     # Built twice: the first build in a process also loads the builder's code
     # objects; the second one's time is the one reported.
     bvh = Bvh(scene.positions, width=args.width, lds_stack=args.lds_stack, builder=BUILDER_GPU)
@@ -147,49 +138,126 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-
     for _ in range(args.warmup):
         bvh.trace_closest_device(d_rays, n, d_hits, stream)
     torch.cuda.synchronize()
-
-    log(f"timing {args.steps} steps")
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    log(f"bvh_stress: timing {steps} closest-hit launches")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         starts[k].record(stream)
         bvh.trace_closest_device(d_rays, n, d_hits, stream)
         ends[k].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    launch_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
     avg_ms = sum(launch_ms) / len(launch_ms)
     hits = d_hits.view(torch.int32).view(n, 4)[:, 3].cpu().numpy().view(np.uint32)
     n_hit = int((hits != 0xFFFFFFFF).sum())
-
-    # any-hit pass (reported beside the headline)
     for _ in range(2):
         bvh.trace_any_device(d_rays, n, d_occ, stream)
     ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ea.record(stream)
-    for _ in range(args.steps):
+    for _ in range(steps):
         bvh.trace_any_device(d_rays, n, d_occ, stream)
     eb.record(stream)
     torch.cuda.synchronize()
-    any_ms = ea.elapsed_time(eb) / args.steps
-
+    any_ms = ea.elapsed_time(eb) / steps
     counts = json.loads(COUNTS.read_text())
     bpr = counts["closest_all"]["alg_bytes_per_ray"]
     achieved = bpr * n / (avg_ms * 1e-3) / 1e9
+    return {
+        "scene": scene, "rays": rays, "wall": wall, "steps": steps, "n": n, "bpr": bpr,
+        "mrays_s_closest": n * steps / wall / 1e6,
+        "mrays_s_any": n / (any_ms * 1e-3) / 1e6,
+        "closest_hits": n_hit,
+        "config": {"n_tris": scene.n_tris, "n_rays": n, "bvh_width": args.width, "lds_stack": args.lds_stack,
+                   "bvh_nodes": st["n_nodes"], "bvh_build_ms": round(st["build_ms"], 1)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "alg_bytes_per_ray": bpr, "kernel_avg_ms": avg_ms},
+    }
+
+
+def cpu_baseline_trace(tris, rays):
+    """The oracle's BVH2 traversal on the host cores, on a bounded sample."""
+    import numpy as np
+
+    o, flags, model = baseline_oracle()
+    threads = host_cores()
+    t0 = time.time()
+    b = o.bvh(tris)
+    build_s = time.time() - t0
+    n_half = 1 << 20
+    rng = np.random.default_rng(0)
+    # 1/4 of each ray set, so the sample has the workload's coherent/incoherent mix
+    k = n_half // 4
+    idx = np.concatenate([rng.choice(n_half, k, replace=False),
+                          n_half + rng.choice(n_half, k, replace=False)])
+    sample = np.ascontiguousarray(rays.view(np.float32).reshape(-1, 8)[np.sort(idx)])
+    b.trace(sample[:4096], threads=threads)  # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.time()
+        b.trace(sample, threads=threads)
+        times.append(time.time() - t0)
+    t = sorted(times)[2]
+    return {"value": sample.shape[0] / t / 1e6, "unit": "Mrays/s", "cores": threads,
+            "kind": "port", "cpu_model": model, "build_flags": flags,
+            "sample": f"{sample.shape[0]} rays (1/4 of the coherent and incoherent sets), "
+                      f"closest hit, oracle BVH2 built in {build_s:.1f} s, warm-up then median of 5"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=4)
+    ap.add_argument("--lds-stack", type=int, default=8)
+    ap.add_argument("--workload", default="disney",
+                    choices=["disney", "bvh_stress", "restir", "serving", "bvh_build"])
+    ap.add_argument("--requests", type=int, default=96)
+    ap.add_argument("--rates", default="20,200")
+    ap.add_argument("--policy", type=int, default=0,
+                    help="serving: 0 = one-batch requests shortest first, then the rest in "
+                         "arrival order; 1 = FIFO; 2 = shortest remaining work first")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bvh-stress", action="store_true",
+                    help="disney: skip the traversal leg reported beside the headline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device-to-device copy before timing (the known-size "
+                         "kernel that scales the HBM counters of a --pmc run)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(slabs, rays and requests would shard per GPU with no exchange)", file=sys.stderr)
+        return 2
+
+    if args.workload == "disney":
+        return bench_disney(args)
+    if args.workload == "serving":
+        return bench_serving(args)
+    if args.workload == "restir":
+        return bench_restir(args)
+    if args.workload == "bvh_build":
+        return bench_bvh_build(args)
+
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    r = bench_bvh_stress(args)
     out = {
         "metric": METRIC,
-        "value": n * args.steps / wall / 1e6,
+        "value": r["mrays_s_closest"],
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": r["steps"],
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": r["wall"] * 1e3 / r["steps"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -197,18 +265,14 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "bvh_stress: 1,048,576 triangles, 2,097,152 rays "
                                "(1M coherent 1024x1024 primary + 1M incoherent), closest hit",
-                   "n_tris": scene.n_tris, "n_rays": n, "bvh_width": args.width,
-                   "lds_stack": args.lds_stack,
-                   "bvh_nodes": st["n_nodes"], "bvh_build_ms": round(st["build_ms"], 1)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "alg_bytes_per_ray": bpr, "kernel_avg_ms": avg_ms},
-        "closest_hits": n_hit,
-        "any_hit_mrays_s": n / (any_ms * 1e-3) / 1e6,
+                   **r["config"]},
+        "roofline": r["roofline"],
+        "closest_hits": r["closest_hits"],
+        "any_hit_mrays_s": r["mrays_s_any"],
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle BVH2 on host cores)")
-        out["cpu_baseline"] = cpu_baseline_trace(scene.positions, rays, bpr)
+        out["cpu_baseline"] = cpu_baseline_trace(r["scene"].positions, r["rays"])
     print(json.dumps(out), flush=True)
     return 0
 
@@ -267,26 +331,40 @@ def bench_bvh_build(args) -> int:
     return 0
 
 
-def cpu_baseline_render(scene, width, height, spp, bounces, rr):
-    """The oracle path tracer on the host cores, on a bounded pixel sample."""
+def cpu_baseline_render(scene, width, height, spp, bounces, rr, target_s=3.0):
+    """The FP32 oracle path tracer on the host cores, on a bounded random pixel
+    sample of the same frame (same scene, camera and per-pixel seeds): one
+    warm-up run sizes the sample to about target_s per run, then the median of
+    five runs."""
     import numpy as np
 
-    from tests._oracle import load_oracle, oracle_render
+    from tests._oracle import oracle_render
 
-    o = load_oracle()
-    threads = max(1, os.cpu_count() or 1)
-    b = o.bvh(scene.positions)
-    pix = np.random.default_rng(0).choice(width * height, 16384, replace=False).astype(np.uint32)
-    oracle_render(o, scene, width, height, spp=spp, max_bounces=bounces, rr_start=rr,
-                  pixels=pix[:256], bvh=b, threads=threads)
+    o, flags, model = baseline_oracle()
+    threads = host_cores()
     t0 = time.time()
-    oracle_render(o, scene, width, height, spp=spp, max_bounces=bounces, rr_start=rr, pixels=pix,
+    b = o.bvh(scene.positions)
+    build_s = time.time() - t0
+    order = np.random.default_rng(0).permutation(width * height).astype(np.uint32)
+    probe = order[:8192]
+    t0 = time.time()
+    oracle_render(o, scene, width, height, spp=spp, max_bounces=bounces, rr_start=rr, pixels=probe,
                   bvh=b, threads=threads)
-    t = time.time() - t0
-    return {"value": pix.shape[0] * spp / t / 1e6, "unit": "Msamples/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{pix.shape[0]} random pixels of the {width}x{height} frame x {spp} spp, "
-                      f"FP32 oracle path tracer, same scene/camera/seeds"}
+    rate = probe.shape[0] / max(time.time() - t0, 1e-3)
+    n = int(min(width * height, max(8192, rate * target_s)))
+    pix = np.sort(order[:n])
+    times = []
+    for _ in range(5):
+        t0 = time.time()
+        oracle_render(o, scene, width, height, spp=spp, max_bounces=bounces, rr_start=rr, pixels=pix,
+                      bvh=b, threads=threads)
+        times.append(time.time() - t0)
+    t = sorted(times)[2]
+    return {"value": n * spp / t / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "build_flags": flags,
+            "sample": f"{n} random pixels of the {width}x{height} frame x {spp} spp (same scene, camera and "
+                      f"seeds), FP32 oracle path tracer, BVH2 built in {build_s:.1f} s; warm-up, then median "
+                      f"of 5 runs ({', '.join(f'{x:.2f}' for x in times)} s)"}
 
 
 def bench_disney(args) -> int:
@@ -298,45 +376,82 @@ def bench_disney(args) -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     W, H, SPP, DEPTH, RR = 1920, 1088, 4, 8, 3  # 1088: slab heights are multiples of 64
+    counts = json.loads(DISNEY_COUNTS.read_text())
     log("generating disney scene")
     scene = generate_scene("disney", seed=3)
     pt = PathTracerGpu(plain_config(sample_count=SPP, max_bounces=DEPTH, rr_start_bounce=RR,
                                       bvh_width=args.width, lds_stack_depth=args.lds_stack))
     pt.set_scene(scene)
     pt.set_camera(scene.camera)
-    # render_rgba is the render-node entry: the frame resolved to RGBA8 in the
-    # 64x64 block layout and read back to host memory (the FP32 accumulator
-    # export that the parity tests read is not part of it)
     for _ in range(args.warmup):
-        pt.render_rgba((W, H))
+        pt.render_device((W, H))
     torch.cuda.synchronize()
     log(f"timing {args.steps} frames")
-    dev_ms, rays = [], 0
+    frames = []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        pt.render_rgba((W, H))
-        st = pt.stats()
-        dev_ms.append(st["ms_total"])
-        rays += st["extension_rays"] + st["shadow_rays"]
+        pt.render_device((W, H))
+        frames.append(pt.stats())
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    samples = W * H * SPP * args.steps
+    samples = W * H * SPP
+    rays = sum(f["extension_rays"] + f["shadow_rays"] for f in frames)
+    ext = sum(f["extension_rays"] for f in frames) / len(frames)
+    dev_ms = sum(f["ms_total"] for f in frames) / len(frames)
+    ext_ms = sum(f["ms_extend"] for f in frames) / len(frames)
+    shade_ms = sum(f["ms_shade"] for f in frames) / len(frames)
+    shadow_ms = sum(f["ms_shadow"] for f in frames) / len(frames)
+    # the render-node entry: the same frames resolved to RGBA8 in the block
+    # layout and read back to host memory
+    k_rn = max(3, args.steps // 4)
+    t1 = time.perf_counter()
+    for _ in range(k_rn):
+        pt.render_rgba((W, H))
+    wall_rn = time.perf_counter() - t1
+    # extend stage: bytes per frame over its device time per frame (8 launches)
+    ext_bytes = counts["alg_bytes"]["extend_stage_per_sample"] * samples
+    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
+    frame_bytes = counts["alg_bytes"]["sample"] * samples
     out = {
         "metric": METRIC,
-        "value": samples / wall / 1e6,
+        "value": samples * args.steps / wall / 1e6,
         "unit": "Msamples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"disney: 256 UV spheres + ground (524,290 triangles), "
-                               f"{W}x{H}, {SPP} spp, depth {DEPTH}, RR from {RR}",
+        "config": {"workload": f"disney: 256 UV spheres x 2048 triangles + ground (524,290 triangles, seed 3), "
+                               f"analytic sky + sun, {W}x{H} (1080 rounded up to 64-row blocks), {SPP} spp, "
+                               f"depth {DEPTH}, RR from bounce {RR}, wavefront path tracer",
+                   "width": W, "height": H, "spp": SPP, "max_bounces": DEPTH,
                    "bvh_width": args.width, "lds_stack": args.lds_stack},
         "mrays_s": rays / wall / 1e6,
-        "device_ms_per_frame": sum(dev_ms) / len(dev_ms),
+        "device_ms_per_frame": dev_ms,
+        "stage_ms_per_frame": {"extend": ext_ms, "shade": shade_ms, "shadow": shadow_ms,
+                               "other": max(0.0, dev_ms - ext_ms - shade_ms - shadow_ms)},
+        "extension_rays_per_sample": {"device": ext / samples,
+                                      "oracle": counts["per_sample"]["extension_rays"]},
+        "msamples_s_render_node": samples * k_rn / wall_rn / 1e6,
+        "roofline": {"bound": "hbm", "kernel": "k_extend (closest-hit traversal of the extension rays)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(EXTEND_TRAFFIC, "traffic_bytes_per_launch"),
+                     "alg_bytes_per_launch": ext_bytes / DEPTH,
+                     "alg_bytes_per_ray": counts["alg_bytes"]["closest_ray"],
+                     "kernel_avg_ms": ext_ms / DEPTH, "launches_per_frame": DEPTH},
+        "frame_roofline": {"alg_bytes_per_sample": counts["alg_bytes"]["sample"],
+                           "achieved": frame_bytes / (dev_ms * 1e-3) / 1e9,
+                           "frac": frame_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    if not args.no_bvh_stress:
+        r = bench_bvh_stress(args, steps=10)
+        out["bvh_stress"] = {"mrays_s_closest": r["mrays_s_closest"], "mrays_s_any": r["mrays_s_any"],
+                             "closest_hits": r["closest_hits"], "roofline": r["roofline"], **r["config"]}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle path tracer on host cores)")
         out["cpu_baseline"] = cpu_baseline_render(scene, W, H, SPP, DEPTH, RR)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0
 
@@ -361,10 +476,10 @@ def restir_config(**kw):
 def cpu_baseline_restir(scene, frames=3, w=480, h=270):
     """The oracle's ReSTIR integrator on the host cores on a bounded sample:
     a few frames of the orbit at 1/16 of the 1080p pixel count."""
-    from tests._oracle import OracleRestir, load_oracle
+    from tests._oracle import OracleRestir
 
-    o = load_oracle()
-    threads = max(1, os.cpu_count() or 1)
+    o, flags, model = baseline_oracle()
+    threads = host_cores()
     r = OracleRestir(o, scene, w, h)
     cfg = restir_config()
     r.frame(cfg, camera=restir_orbit(0), frame_idx=0, threads=threads)  # warm-up, builds history
@@ -373,6 +488,7 @@ def cpu_baseline_restir(scene, frames=3, w=480, h=270):
         r.frame(cfg, camera=restir_orbit(f), frame_idx=f, threads=threads)
     t = time.time() - t0
     return {"value": w * h * frames / t / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "build_flags": flags,
             "sample": f"{frames} frames of the orbit at {w}x{h} (1/16 of the 1080p pixels), 1 spp, "
                       f"same settings, FP32 oracle ReSTIR integrator"}
 

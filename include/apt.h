@@ -363,7 +363,10 @@ typedef struct apt_render_stats {
   uint64_t shadow_rays;      /* any-hit rays traced */
   uint64_t segments;         /* shaded path vertices */
   double ms_total;           /* device time of the last render (hipEvents) */
-  double ms_extend;
+  double ms_extend;          /* APT_INTEGRATOR_PLAIN: device time of the frame's extend (closest
+                              * hit), shade and shadow (any hit) launches, summed over bounces and
+                              * passes (hipEvents around each launch; 0 when a frame has more than
+                              * 256 launches of a stage); ms_other is the rest of ms_total */
   double ms_shade;
   double ms_shadow;
   double ms_other;

@@ -30,6 +30,10 @@ int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const
               uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
               const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
               uint64_t* counts);
+int or_render_counted(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
+                      uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
+                      const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
+                      uint64_t* counts, uint32_t n_counts);
 void* or_restir_create(uint32_t width, uint32_t rows);
 void or_restir_destroy(void* st);
 int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,

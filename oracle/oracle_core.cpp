@@ -397,9 +397,18 @@ done:
 
 
 // ---- hooks for the integrator (oracle_render.cpp) --------------------------
+// Per-thread tally of the FP32 queries' box and triangle tests (closest hit
+// at [0], [1], any hit at [2], [3]) while a caller has set it.
+thread_local uint64_t* g_tally = nullptr;
+void oracle_set_tally(uint64_t* t4) { g_tally = t4; }
+
 bool oracle_closest_f(const Bvh* b, const float* ray8, float* hit4) {
   uint64_t nb = 0, nt = 0;
   trace1(*b, ray8, false, hit4, nb, nt);
+  if (g_tally) {
+    g_tally[0] += nb;
+    g_tally[1] += nt;
+  }
   uint32_t p;
   std::memcpy(&p, &hit4[3], 4);
   return p != 0xFFFFFFFFu;
@@ -409,6 +418,10 @@ bool oracle_any_f(const Bvh* b, const float* ray8) {
   uint64_t nb = 0, nt = 0;
   float h[4];
   trace1(*b, ray8, true, h, nb, nt);
+  if (g_tally) {
+    g_tally[2] += nb;
+    g_tally[3] += nt;
+  }
   return h[0] != 0.0f;
 }
 

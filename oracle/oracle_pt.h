@@ -610,6 +610,7 @@ struct ClosestResult {
   uint32_t prim;
 };
 bool oracle_closest_f(const Bvh* b, const float* ray8, float* hit4);
+void oracle_set_tally(uint64_t* t4);
 bool oracle_any_f(const Bvh* b, const float* ray8);
 bool oracle_closest_d(const Bvh* b, const double* o, const double* d, double tmin, double tmax,
                       double* t, double* u, double* v, uint32_t* prim);

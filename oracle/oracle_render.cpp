@@ -175,7 +175,7 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
 
 template <typename R>
 void render(const Scene& sc, const Cfg& cf, const uint32_t* pixels, uint32_t n_pixels, float* out,
-            int threads, uint64_t* counts) {
+            int threads, uint64_t* counts, size_t n_counts) {
   // Work items are 64x64 tiles of the slab (the reference CPU tracer's block
   // loop, crates/appearance-path-tracer/src/lib.rs:107-190), or chunks of an
   // explicit pixel list.
@@ -183,7 +183,10 @@ void render(const Scene& sc, const Cfg& cf, const uint32_t* pixels, uint32_t n_p
   const uint32_t tiles_x = (cf.width + 63) / 64, tiles_y = (cf.rows + 63) / 64;
   const uint32_t items = list ? (n_pixels + 255) / 256 : tiles_x * tiles_y;
   std::atomic<uint32_t> next{0};
-  std::vector<uint64_t> cnt((size_t)threads * 2, 0);
+  // per thread: extension rays, shadow rays, then the box and triangle tests
+  // of the closest-hit and of the any-hit queries (FP32 only)
+  constexpr size_t NC = 6;
+  std::vector<uint64_t> cnt((size_t)threads * NC, 0);
   auto shade_px = [&](uint32_t x, uint32_t yl, float* o, uint64_t* nr) {
     V<R> acc;
     for (uint32_t s = 0; s < cf.spp; ++s) acc = acc + path<R>(sc, cf, x, cf.row_start + yl, s, nr);
@@ -193,7 +196,8 @@ void render(const Scene& sc, const Cfg& cf, const uint32_t* pixels, uint32_t n_p
     o[2] = (float)(acc.z * inv);
   };
   auto work = [&](int tid) {
-    uint64_t* nr = &cnt[2 * (size_t)tid];
+    uint64_t* nr = &cnt[NC * (size_t)tid];
+    orc::oracle_set_tally(nr + 2);
     while (true) {
       uint32_t it = next.fetch_add(1);
       if (it >= items) break;
@@ -214,11 +218,11 @@ void render(const Scene& sc, const Cfg& cf, const uint32_t* pixels, uint32_t n_p
   for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
   work(0);
   for (auto& th : pool) th.join();
+  orc::oracle_set_tally(nullptr);
   if (counts) {
-    counts[0] = counts[1] = 0;
-    for (int t = 0; t < threads; ++t) {
-      counts[0] += cnt[2 * (size_t)t];
-      counts[1] += cnt[2 * (size_t)t + 1];
+    for (size_t k = 0; k < n_counts; ++k) {
+      counts[k] = 0;
+      for (int t = 0; t < threads; ++t) counts[k] += cnt[NC * (size_t)t + k];
     }
   }
 }
@@ -237,6 +241,19 @@ int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const
               uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
               const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
               uint64_t* counts) {
+  return or_render_counted(pos, nrm, mat_ids, mats, n_tris, bvh, sky, cam, cfg10, pixels, n_pixels, precision,
+                           threads, out, counts, 2);
+}
+
+// or_render with up to six counters: extension rays, shadow rays, box tests
+// and triangle tests of the extension (closest-hit) rays, box tests and
+// triangle tests of the shadow (any-hit) rays; the tests are counted in FP32
+// only.  The per-sample figures of the algorithmic-bytes roofline.
+int or_render_counted(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
+                      uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
+                      const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
+                      uint64_t* counts, uint32_t n_counts) {
+  if (n_counts > 6) return -1;
   ocr::Scene sc;
   sc.pos = pos;
   sc.nrm = nrm;
@@ -257,8 +274,8 @@ int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const
   }
   ocr::Cfg cf{cfg10[0], cfg10[1], cfg10[2], cfg10[3], cfg10[4], cfg10[5], cfg10[6], cfg10[7], cfg10[8], cfg10[9]};
   if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
-  if (precision == 64) ocr::render<double>(sc, cf, pixels, n_pixels, out, threads, counts);
-  else ocr::render<float>(sc, cf, pixels, n_pixels, out, threads, counts);
+  if (precision == 64) ocr::render<double>(sc, cf, pixels, n_pixels, out, threads, counts, n_counts);
+  else ocr::render<float>(sc, cf, pixels, n_pixels, out, threads, counts, n_counts);
   return 0;
 }
 

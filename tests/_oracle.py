@@ -133,8 +133,10 @@ def load_oracle() -> Oracle:
 def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounces: int,
                   rr_start: int, seed: int = 0, frame_idx: int = 0, row_start: int = 0,
                   rows: int | None = None, pixels=None, precision: int = 32, threads: int = 0,
-                  bvh=None, camera=None, sky=None, jitter: bool = True):
-    """Run the oracle path tracer on a SceneData; returns (mean radiance, counts)."""
+                  bvh=None, camera=None, sky=None, jitter: bool = True, tally: bool = False):
+    """Run the oracle path tracer on a SceneData; returns (mean radiance, counts):
+    counts = (extension rays, shadow rays), or with tally=True a dict that
+    adds the box and triangle tests of both kinds of query (FP32)."""
     import ctypes as C
     from appearance_amd._ffi import Material
     rows = height - row_start if rows is None else rows
@@ -152,18 +154,21 @@ def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounc
         out = np.empty((pixels.shape[0], 3), np.float32)
     else:
         out = np.empty((rows, width, 3), np.float32)
-    counts = np.zeros(2, np.uint64)
-    fn = o.lib.or_render
+    counts = np.zeros(6, np.uint64)
+    fn = o.lib.or_render_counted
     fn.restype = C.c_int
-    fn.argtypes = [C.c_void_p] * 6 + [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
-                                      C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
-                   C.c_void_p, C.c_void_p]
-    fn(_vp(pos), _vp(nrm) if nrm is not None else None, _vp(mids), C.cast(mats, C.c_void_p),
-       pos.shape[0], b.h, C.cast(C.pointer(sk), C.c_void_p), C.cast(C.pointer(cam), C.c_void_p),
-       _vp(cfg), _vp(pixels) if pixels is not None else None,
-       0 if pixels is None else pixels.shape[0], precision, threads, _vp(out), _vp(counts))
+                   C.c_void_p, C.c_void_p, C.c_uint32]
+    rc = fn(_vp(pos), _vp(nrm) if nrm is not None else None, _vp(mids), C.cast(mats, C.c_void_p),
+            pos.shape[0], b.h, C.cast(C.pointer(sk), C.c_void_p), C.cast(C.pointer(cam), C.c_void_p),
+            _vp(cfg), _vp(pixels) if pixels is not None else None,
+            0 if pixels is None else pixels.shape[0], precision, threads, _vp(out), _vp(counts), 6)
+    assert rc == 0
+    if tally:
+        keys = ["extension_rays", "shadow_rays", "closest_box_tests", "closest_tri_tests", "any_box_tests",
+                "any_tri_tests"]
+        return out, {k: int(v) for k, v in zip(keys, counts)}
     return out, (int(counts[0]), int(counts[1]))
 
 

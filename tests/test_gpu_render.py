@@ -92,8 +92,20 @@ def test_disney_full_frame(gpu_available, oracle):
     s = generate_scene("disney", seed=3)
     w, h = 1920, 1088
     print("[render] disney 1920x1088 4 spp, one pass", flush=True)
-    _, _, full = gpu_render(s, w, h, 4, 8, 3)
+    pt_full, _, full = gpu_render(s, w, h, 4, 8, 3)
     assert np.isfinite(full).all() and (full >= 0).all() and full.mean() > 0.05
+    # the frame's ray counts equal the oracle's on the same seeds (the
+    # committed per-sample figures the bench's algorithmic bytes come from)
+    import json
+    from pathlib import Path
+    fix = json.loads((Path(__file__).parent / "golden" / "disney_sample_counts.json").read_text())
+    st = pt_full.stats()
+    for k in ("extension_rays", "shadow_rays"):
+        got, want = st[k], fix["counts"][k]
+        print(f"[render] disney {k}: device {got}, oracle {want}", flush=True)
+        assert abs(got - want) <= 2e-4 * want
+    assert st["ms_extend"] > 0 and st["ms_shade"] > 0 and st["ms_shadow"] > 0
+    assert st["ms_extend"] + st["ms_shade"] + st["ms_shadow"] <= st["ms_total"] * 1.001
     print("[render] disney 1920x1088 4 spp, one sample per pass", flush=True)
     _, _, passes = gpu_render(s, w, h, 4, 8, 3, queue_chunk=1 << 20)
     np.testing.assert_array_equal(passes, full)

@@ -119,17 +119,39 @@ def test_restir_slab_and_reset(gpu_available):
     assert st["paths"] == 128 * 128 and st["extension_rays"] > 128 * 128
 
 
+SEQUENCE = dict(di_candidates=32, di_mcap=20, di_spatial_neighbours=5, di_spatial_radius=30.0, svgf=1,
+                svgf_iterations=5, svgf_history=4)
+
+
+def test_restir_steady_state_vs_oracle(gpu_available, oracle):
+    """The 1080p sequence's exact settings at 128x72 over 24 frames of its
+    orbit, so every history saturates: the DI temporal M reaches its cap
+    (20 x the 33 candidates of a frame), the GI M its cap of 30 and the
+    SVGF moments their 4-frame history.  Every frame against the oracle's."""
+    s = generate_scene("restir", seed=3)
+    cams = [orbit((0.0, 0.0, 0.5), 13.5, 7.5, 2.0 * math.pi * k / 240.0, 40.0) for k in range(24)]
+    print("[restir] restir scene 128x72, 24 orbit frames", flush=True)
+    worst = 0.0
+    for f, (g, o, st, rays) in enumerate(run_pair(oracle, s, 128, 72, 24, cams, **SEQUENCE)):
+        r = relmse(g, o)
+        worst = max(worst, r)
+        print(f"[restir] steady state frame {f}: relMSE {r:.3e}, mean {g.mean():.4f} vs {o.mean():.4f}",
+              flush=True)
+        assert np.isfinite(g).all()
+        assert r < 1e-3
+    print(f"[restir] steady state worst relMSE {worst:.3e}", flush=True)
+
+
 def test_restir_full_size_vs_oracle(gpu_available, oracle):
     """The 1080p sequence at its full size (1920x1088, the many-light scene,
     32 candidates, M cap 20, 5 taps within 30 px, GI reuse, SVGF 5 passes /
-    4 frames, firefly filter): the first two orbit frames against the
-    oracle's, frame by frame."""
+    4 frames, firefly filter): the first five orbit frames against the
+    oracle's, frame by frame (the SVGF history saturates at frame 4)."""
     s = generate_scene("restir", seed=3)
-    cams = [orbit((0.0, 0.0, 0.5), 13.5, 7.5, 2.0 * math.pi * k / 240.0, 40.0) for k in range(2)]
-    kw = dict(di_candidates=32, di_mcap=20, di_spatial_neighbours=5, di_spatial_radius=30.0, svgf=1,
-              svgf_iterations=5, svgf_history=4)
-    print("[restir] restir scene 1920x1088 2 frames", flush=True)
-    for f, (g, o, st, rays) in enumerate(run_pair(oracle, s, 1920, 1088, 2, cams, **kw)):
+    cams = [orbit((0.0, 0.0, 0.5), 13.5, 7.5, 2.0 * math.pi * k / 240.0, 40.0) for k in range(5)]
+    kw = SEQUENCE
+    print("[restir] restir scene 1920x1088 5 frames", flush=True)
+    for f, (g, o, st, rays) in enumerate(run_pair(oracle, s, 1920, 1088, 5, cams, **kw)):
         r = relmse(g, o)
         print(f"[restir] full size frame {f}: relMSE {r:.3e}, mean {g.mean():.4f} vs {o.mean():.4f}",
               flush=True)

@@ -58,8 +58,10 @@ def test_ray_queries(gpu_available, oracle):
 def test_renderer_refuses_then_renders_at_width_2(gpu_available):
     s = deep_tree_scene()
     pt = PathTracerGpu(plain_config(sample_count=1, max_bounces=2))
+    pt.set_scene(s)  # the scene is built at the next render
+    pt.set_camera(s.camera)
     with pytest.raises(AptError) as e:
-        pt.set_scene(s)
+        pt.render_accum((64, 64), with_rgba=False)
     assert e.value.status == APT_ERR_UNSUPPORTED
     pt2 = PathTracerGpu(plain_config(sample_count=1, max_bounces=2, bvh_width=2))
     pt2.set_scene(s)
