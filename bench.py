@@ -62,6 +62,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def calibrate_copy() -> None:
+    """One 256 MiB fill and one 256 MiB device-to-device copy: the known-size
+    kernels whose HBM counter readings scale a --pmc run's."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def host_cores() -> int:
     """Threads for the CPU baseline: the cores this process may run on, at
     most 16 (a one-GPU share of the box)."""
@@ -132,12 +145,7 @@ def bench_bvh_stress(args, steps: int | None = None):
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
     if args.calibrate_copy:
-        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy()
     for _ in range(args.warmup):
         bvh.trace_closest_device(d_rays, n, d_hits, stream)
     torch.cuda.synchronize()
@@ -383,6 +391,8 @@ def bench_disney(args) -> int:
                                       bvh_width=args.width, lds_stack_depth=args.lds_stack))
     pt.set_scene(scene)
     pt.set_camera(scene.camera)
+    if args.calibrate_copy:
+        calibrate_copy()
     for _ in range(args.warmup):
         pt.render_device((W, H))
     torch.cuda.synchronize()
