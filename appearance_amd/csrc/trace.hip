@@ -19,6 +19,7 @@
 // schedules return the same hits: every ray visits children nearest first
 // and the closest hit is the lexicographic (t, prim) minimum.
 #include <algorithm>
+#include <mutex>
 
 #include "apt_internal.h"
 #include "trace_device.h"
@@ -112,9 +113,11 @@ uint32_t pool_grid() {
   return g;
 }
 
+// counter_override: the ray-pool counters to use instead of one of the BVH's
+// ring (NULL: the ring).
 template <int W>
 apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
-                  uint32_t* d_occ, bool any, hipStream_t s) {
+                  uint32_t* d_occ, bool any, hipStream_t s, uint32_t* counter_override) {
   if (n == 0) return APT_OK;
   const WNode<W>* nodes = static_cast<const WNode<W>*>(bvh->d_nodes);
   if (bvh->refill > 64) {
@@ -130,8 +133,11 @@ apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit
   } else {
     // one pool of the BVH's ring per call: calls in flight on different
     // streams (up to the ring size) do not share counters
-    const uint32_t slot = __atomic_fetch_add(&bvh->counter_next, 1u, __ATOMIC_RELAXED) % APT_BVH_COUNTERS;
-    uint32_t* counter = bvh->d_counters + (size_t)slot * POOL_WORDS;
+    uint32_t* counter = counter_override;
+    if (!counter) {
+      const uint32_t slot = __atomic_fetch_add(&bvh->counter_next, 1u, __ATOMIC_RELAXED) % APT_BVH_COUNTERS;
+      counter = bvh->d_counters + (size_t)slot * POOL_WORDS;
+    }
     hipLaunchKernelGGL(k_zero_pool, dim3(1), dim3(POOL_PARTS), 0, s, counter);
     APT_FOR_STACK_DEPTH(bvh->lds_stack, {
       if (any)
@@ -149,15 +155,15 @@ apt_status launch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit
 }
 
 apt_status dispatch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
-                    uint32_t* d_occ, bool any, hipStream_t s) {
+                    uint32_t* d_occ, bool any, hipStream_t s, uint32_t* counter_override = nullptr) {
   APT_REQUIRE(bvh && d_rays && (any ? d_occ != nullptr : d_hits != nullptr),
               APT_ERR_INVALID_ARG, "trace: null argument");
   apt_status st = check_trace_stack(bvh->stats, "trace");
   if (st != APT_OK) return st;
   switch (bvh->width) {
-    case 2: return launch<2>(bvh, d_rays, n, d_hits, d_occ, any, s);
-    case 4: return launch<4>(bvh, d_rays, n, d_hits, d_occ, any, s);
-    case 8: return launch<8>(bvh, d_rays, n, d_hits, d_occ, any, s);
+    case 2: return launch<2>(bvh, d_rays, n, d_hits, d_occ, any, s, counter_override);
+    case 4: return launch<4>(bvh, d_rays, n, d_hits, d_occ, any, s, counter_override);
+    case 8: return launch<8>(bvh, d_rays, n, d_hits, d_occ, any, s, counter_override);
     default: set_error("trace: unsupported width %u", bvh->width); return APT_ERR_INVALID_ARG;
   }
 }
@@ -184,3 +190,77 @@ apt_status launch_trace_any(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t 
 }
 
 }  // namespace apt
+
+// ---------------------------------------------------------------------------
+// apt_trace_closest_host_noalloc: apt_trace_closest_host without allocation.
+// The rays, the hits and the ray-pool counters are staged in `scratch`, a
+// fixed device array of this library (each GPU the library runs on holds its
+// own copy until the process ends): APT_TRACE_NOALLOC_MAX_RAYS rays of 32 B,
+// as many 16 B hits, then the counters, 3 MiB + 1 KiB.  Host code never
+// passes `scratch` itself (its name is a host-side stand-in, not a device
+// address): its address on each device comes from hipGetSymbolAddress on the
+// first call with that device current, and is kept per device.
+// ---------------------------------------------------------------------------
+namespace apt {
+namespace {
+
+constexpr size_t NOALLOC_RAY_BYTES = (size_t)APT_TRACE_NOALLOC_MAX_RAYS * sizeof(apt_ray);
+constexpr size_t NOALLOC_HIT_BYTES = (size_t)APT_TRACE_NOALLOC_MAX_RAYS * sizeof(apt_hit);
+constexpr size_t NOALLOC_CTR_OFFSET = NOALLOC_RAY_BYTES + NOALLOC_HIT_BYTES;
+constexpr size_t NOALLOC_BYTES = NOALLOC_CTR_OFFSET + (size_t)POOL_WORDS * sizeof(uint32_t);
+static_assert(NOALLOC_CTR_OFFSET % 256 == 0, "counters stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[MAX_DEVICES] = {};
+
+apt_status scratch_for_current_device(unsigned char** out) {
+  int dev = 0;
+  APT_HIP(hipGetDevice(&dev));
+  APT_REQUIRE(dev >= 0 && dev < MAX_DEVICES, APT_ERR_DEVICE, "noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("noalloc: hipGetSymbolAddress of the scratch array failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return APT_ERR_DEVICE;
+    }
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[dev];
+  return APT_OK;
+}
+
+}  // namespace
+}  // namespace apt
+
+extern "C" apt_status apt_trace_closest_host_noalloc(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
+                                                     apt_hit* hits) {
+  using namespace apt;
+  clear_error();
+  APT_REQUIRE(bvh && rays && hits, APT_ERR_INVALID_ARG, "apt_trace_closest_host_noalloc: null argument");
+  APT_REQUIRE(n <= APT_TRACE_NOALLOC_MAX_RAYS, APT_ERR_INVALID_ARG,
+              "apt_trace_closest_host_noalloc: %u rays exceed the %u of its fixed device array", n,
+              APT_TRACE_NOALLOC_MAX_RAYS);
+  if (n == 0) return APT_OK;
+  unsigned char* base = nullptr;
+  apt_status st = scratch_for_current_device(&base);
+  if (st != APT_OK) return st;
+  apt_ray* d_rays = reinterpret_cast<apt_ray*>(base);
+  apt_hit* d_hits = reinterpret_cast<apt_hit*>(base + NOALLOC_RAY_BYTES);
+  uint32_t* d_ctr = reinterpret_cast<uint32_t*>(base + NOALLOC_CTR_OFFSET);
+  hipStream_t s = lib_stream();
+  APT_REQUIRE(s != nullptr, APT_ERR_DEVICE, "apt_trace_closest_host_noalloc: no library stream");
+  // The pool counters are cleared by the launch's own k_zero_pool (a kernel,
+  // not a memset); rays and hits are written before they are read.
+  APT_HIP(hipMemcpyAsync(d_rays, rays, (size_t)n * sizeof(apt_ray), hipMemcpyHostToDevice, s));
+  st = dispatch(bvh, d_rays, n, d_hits, nullptr, false, s, d_ctr);
+  if (st != APT_OK) return st;
+  APT_HIP(hipMemcpyAsync(hits, d_hits, (size_t)n * sizeof(apt_hit), hipMemcpyDeviceToHost, s));
+  APT_HIP(hipStreamSynchronize(s));
+  return APT_OK;
+}

@@ -163,6 +163,19 @@ apt_status apt_trace_closest_host(const apt_bvh* bvh, const apt_ray* rays, uint3
 apt_status apt_trace_any_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
                               uint32_t* occluded);
 
+/* apt_trace_closest_host that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * rays, hits and ray-pool counters are staged in a fixed device array of the
+ * library, 3 MiB + 1 KiB per GPU, held until the process ends; n above
+ * APT_TRACE_NOALLOC_MAX_RAYS returns APT_ERR_INVALID_ARG.  Every call on a
+ * GPU shares that GPU's array (it keeps the last call's rays and hits), so
+ * calls that overlap on one GPU race: serialise them.  The BVH must already
+ * exist (apt_bvh_build allocates). */
+#define APT_TRACE_NOALLOC_MAX_RAYS 65536u
+apt_status apt_trace_closest_host_noalloc(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
+                                          apt_hit* hits);
+
 /* ------------------------------------------------------------------------
  * Scene data.  Layouts follow the reference's GPU pools.
  * ------------------------------------------------------------------------ */
