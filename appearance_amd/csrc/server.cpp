@@ -14,6 +14,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -46,6 +47,7 @@ struct Req {
 
 struct Batch {
   int buf = 0;
+  uint32_t depth = 0;  // deepest request of the batch
   std::vector<std::pair<Req*, uint32_t>> parts;
   hipEvent_t t0 = nullptr, t1 = nullptr;
 };
@@ -69,6 +71,13 @@ struct apt_server {
   apt_server_stats stats{};
   std::thread worker;
 
+  // Per-request accumulators and results live in one arena of accum_budget
+  // bytes allocated at creation: admission and retirement take and return
+  // ranges of it (first fit, neighbours merged) instead of calling hipMalloc
+  // and hipFree, which can stall the device, for every request.
+  unsigned char* arena = nullptr;
+  std::map<size_t, size_t> arena_free;  // offset -> bytes
+
   // device
   hipStream_t stream = nullptr, side = nullptr;
   float4 *p_o = nullptr, *p_d = nullptr, *p_t = nullptr, *p_l = nullptr, *p_h = nullptr;
@@ -79,7 +88,9 @@ struct apt_server {
   SrvReq* h_reqs = nullptr;           // pinned staging, one per slot
   SrvBatch* d_batch[2] = {nullptr, nullptr};
   SrvBatch* h_batch[2] = {nullptr, nullptr};  // pinned staging
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // per staging buffer, one captured batch per depth 1..max_bounces: a batch
+  // runs as many bounces as its deepest request asks for
+  std::vector<hipGraphExec_t> graph[2];
   SrvDevice dev[2];
   bool device_failed = false;
 
@@ -91,9 +102,49 @@ struct apt_server {
 
 namespace {
 
-void release_request_memory(Req* q) {
-  if (q->d_accum) (void)hipFree(q->d_accum);
-  if (q->d_out) (void)hipFree(q->d_out);
+constexpr size_t ARENA_ALIGN = 256;
+
+size_t arena_bytes(const Req* q) {
+  const size_t npix = (size_t)q->r.width * q->r.height;
+  const size_t acc = (npix * 3 * sizeof(unsigned long long) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
+  return acc + (npix * 3 * sizeof(float) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
+}
+
+// Under the lock: a range of the arena for q's accumulator and result.
+bool arena_take(apt_server* s, Req* q) {
+  const size_t need = arena_bytes(q);
+  for (auto it = s->arena_free.begin(); it != s->arena_free.end(); ++it) {
+    if (it->second < need) continue;
+    const size_t off = it->first, left = it->second - need;
+    s->arena_free.erase(it);
+    if (left) s->arena_free[off + need] = left;
+    const size_t npix = (size_t)q->r.width * q->r.height;
+    const size_t acc = (npix * 3 * sizeof(unsigned long long) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
+    q->d_accum = reinterpret_cast<unsigned long long*>(s->arena + off);
+    q->d_out = reinterpret_cast<float*>(s->arena + off + acc);
+    return true;
+  }
+  return false;
+}
+
+// Under the lock: return q's range to the arena, merged with free neighbours.
+void release_request_memory(apt_server* s, Req* q) {
+  if (!q->d_accum) return;
+  size_t off = (size_t)(reinterpret_cast<unsigned char*>(q->d_accum) - s->arena), len = arena_bytes(q);
+  auto next = s->arena_free.lower_bound(off);
+  if (next != s->arena_free.end() && next->first == off + len) {
+    len += next->second;
+    next = s->arena_free.erase(next);
+  }
+  if (next != s->arena_free.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      off = prev->first;
+      len += prev->second;
+      s->arena_free.erase(prev);
+    }
+  }
+  s->arena_free[off] = len;
   q->d_accum = nullptr;
   q->d_out = nullptr;
 }
@@ -106,21 +157,12 @@ void admit(apt_server* s) {
     for (size_t i = 0; i < s->slots.size(); ++i)
       if (!s->slots[i]) { slot = (int)i; break; }
     if (slot < 0) return;
-    if (s->accum_used + q->footprint > s->cfg.accum_budget && s->accum_used > 0) return;
+    // footprint admission: wait until the arena has a range for it
+    if (!arena_take(s, q)) return;
     s->queued.pop_front();
     const size_t npix = (size_t)q->r.width * q->r.height;
-    hipError_t e = hipMalloc(&q->d_accum, npix * 3 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&q->d_out, npix * 3 * sizeof(float));
-    if (e != hipSuccess) {
-      release_request_memory(q);
-      q->state = APT_REQ_FAILED;
-      q->status = APT_ERR_OOM;
-      q->t_done = s->now_ms();
-      s->cv_done.notify_all();
-      continue;
-    }
     if (srv_zero(q->d_accum, npix * 3, s->stream) != APT_OK) {
-      release_request_memory(q);
+      release_request_memory(s, q);
       q->state = APT_REQ_FAILED;
       q->status = APT_ERR_DEVICE;
       q->t_done = s->now_ms();
@@ -141,7 +183,7 @@ void admit(apt_server* s) {
     q->state = APT_REQ_ACTIVE;
     q->t_admit = s->now_ms();
     s->slots[slot] = q;
-    s->accum_used += q->footprint;
+    s->accum_used += arena_bytes(q);
   }
 }
 
@@ -191,6 +233,7 @@ void build_batch(apt_server* s, Batch& b, SrvBatch& hb) {
     if (taken) {
       q->inflight++;
       b.parts.push_back({q, taken});
+      b.depth = std::max(b.depth, q->r.max_bounces);
     }
   }
 }
@@ -200,10 +243,14 @@ apt_status launch_batch(apt_server* s, Batch& b) {
   const size_t bytes = offsetof(SrvBatch, tiles) + (size_t)hb.n_tiles * sizeof(SrvTile);
   APT_HIP(hipMemcpyAsync(s->d_batch[b.buf], &hb, bytes, hipMemcpyHostToDevice, s->stream));
   APT_HIP(hipEventRecord(b.t0, s->stream));
-  if (s->graph[b.buf]) {
-    APT_HIP(hipGraphLaunch(s->graph[b.buf], s->stream));
+  const uint32_t depth = std::max<uint32_t>(1u, std::min(b.depth, s->cfg.max_bounces));
+  hipGraphExec_t g = s->graph[b.buf].size() >= depth ? s->graph[b.buf][depth - 1] : nullptr;
+  if (g) {
+    APT_HIP(hipGraphLaunch(g, s->stream));
   } else {
-    apt_status st = srv_enqueue_batch(s->dev[b.buf], s->stream);
+    SrvDevice d = s->dev[b.buf];
+    d.max_bounces = depth;
+    apt_status st = srv_enqueue_batch(d, s->stream);
     if (st != APT_OK) return st;
   }
   APT_HIP(hipEventRecord(b.t1, s->stream));
@@ -248,15 +295,18 @@ void finalize(apt_server* s, Batch& b) {
       q->state = APT_REQ_CANCELLED;
       q->t_done = s->now_ms();
     }
-    release_request_memory(q);
+    if (q->d_accum) s->accum_used -= arena_bytes(q);
+    release_request_memory(s, q);
     if (q->slot >= 0) s->slots[q->slot] = nullptr;
     q->slot = -1;
-    s->accum_used -= q->footprint;
   }
   if (!dropped.empty()) s->cv_done.notify_all();
   s->cv_work.notify_all();
 }
 
+// After a device failure: every queued and active request fails, and the
+// active ones give back their slots and arena ranges (no batch that could
+// still write them is awaited any more: the caller drops its in-flight list).
 void fail_all(apt_server* s, apt_status st) {
   std::lock_guard<std::mutex> lock(s->mu);
   s->device_failed = true;
@@ -267,6 +317,14 @@ void fail_all(apt_server* s, apt_status st) {
       q->status = st;
       q->t_done = s->now_ms();
     }
+  }
+  for (Req*& q : s->slots) {
+    if (!q) continue;
+    if (q->d_accum) s->accum_used -= arena_bytes(q);
+    release_request_memory(s, q);
+    q->slot = -1;
+    q->inflight = 0;
+    q = nullptr;
   }
   s->queued.clear();
   s->cv_done.notify_all();
@@ -289,8 +347,8 @@ void worker_loop(apt_server* s) {
         if (q && q->cancel && q->inflight == 0) {
           q->state = APT_REQ_CANCELLED;
           q->t_done = s->now_ms();
-          release_request_memory(q);
-          s->accum_used -= q->footprint;
+          if (q->d_accum) s->accum_used -= arena_bytes(q);
+          release_request_memory(s, q);
           q->slot = -1;
           q = nullptr;
           s->cv_done.notify_all();
@@ -312,7 +370,9 @@ void worker_loop(apt_server* s) {
       ev_next = (ev_next + 2) % 4;
       apt_status st = launch_batch(s, b);
       if (st != APT_OK) {
+        (void)hipStreamSynchronize(s->stream);
         fail_all(s, st);
+        inflight.clear();
         continue;
       }
       inflight.push_back(b);
@@ -352,6 +412,8 @@ apt_status create_device(apt_server* s) {
     if ((st = dmalloc(*b, P)) != APT_OK) return st;
   for (uint32_t** b : {&s->p_rng, &s->p_q0, &s->p_q1, &s->p_qs, &s->path_req, &s->path_pix})
     if ((st = dmalloc(*b, P)) != APT_OK) return st;
+  if ((st = dmalloc(s->arena, s->cfg.accum_budget)) != APT_OK) return st;
+  s->arena_free[0] = s->cfg.accum_budget;
   if ((st = dmalloc(s->p_cnt, CNT_COUNT)) != APT_OK) return st;
   APT_HIP(hipMemset(s->p_cnt, 0, CNT_COUNT * sizeof(uint32_t)));
   if ((st = dmalloc(s->d_reqs, s->cfg.max_active)) != APT_OK) return st;
@@ -378,15 +440,20 @@ apt_status create_device(apt_server* s) {
     d.ps.shadow_queue = s->p_qs; d.ps.counters = s->p_cnt;
     d.path_req = s->path_req;
     d.path_pix = s->path_pix;
-    // Capture the batch once per staging buffer; replays only read device
-    // memory the host updates between batches.  If capture is unavailable
-    // the same kernels are launched directly.
-    hipGraph_t g = nullptr;
-    if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
-      apt_status est = srv_enqueue_batch(d, s->stream);
+    // Capture the batch once per staging buffer and depth; replays only read
+    // device memory the host updates between batches.  If capture is
+    // unavailable the same kernels are launched directly.
+    s->graph[i].assign(s->cfg.max_bounces, nullptr);
+    for (uint32_t depth = 1; depth <= s->cfg.max_bounces; ++depth) {
+      SrvDevice dd = d;
+      dd.max_bounces = depth;
+      hipGraph_t g = nullptr;
+      if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) != hipSuccess) break;
+      apt_status est = srv_enqueue_batch(dd, s->stream);
       hipError_t e = hipStreamEndCapture(s->stream, &g);
       if (est == APT_OK && e == hipSuccess && g) {
-        if (hipGraphInstantiate(&s->graph[i], g, nullptr, nullptr, 0) != hipSuccess) s->graph[i] = nullptr;
+        if (hipGraphInstantiate(&s->graph[i][depth - 1], g, nullptr, nullptr, 0) != hipSuccess)
+          s->graph[i][depth - 1] = nullptr;
       }
       if (g) (void)hipGraphDestroy(g);
       (void)hipGetLastError();
@@ -413,13 +480,12 @@ apt_server::~apt_server() {
   cv_work.notify_all();
   if (worker.joinable()) worker.join();
   if (stream) (void)hipStreamSynchronize(stream);
-  for (auto& kv : all) {
-    release_request_memory(kv.second);
-    delete kv.second;
-  }
+  for (auto& kv : all) delete kv.second;
+  if (arena) (void)hipFree(arena);
   for (SceneGpu& g : scenes) scene_gpu_free(g);
   for (int i = 0; i < 2; ++i) {
-    if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
+    for (hipGraphExec_t g : graph[i])
+      if (g) (void)hipGraphExecDestroy(g);
     if (d_batch[i]) (void)hipFree(d_batch[i]);
     if (h_batch[i]) (void)hipHostFree(h_batch[i]);
   }
@@ -501,7 +567,7 @@ apt_status apt_submit(apt_server* s, const apt_request* req, float* out_rgb, uin
   q->r = *req;
   q->out = out_rgb;
   q->total = npix * req->spp;
-  q->footprint = (size_t)npix * 3 * (sizeof(unsigned long long) + sizeof(float));
+  q->footprint = arena_bytes(q);
   {
     std::lock_guard<std::mutex> lock(s->mu);
     if (req->scene_id >= s->scenes.size()) {
@@ -516,7 +582,8 @@ apt_status apt_submit(apt_server* s, const apt_request* req, float* out_rgb, uin
     }
     if (q->footprint > s->cfg.accum_budget) {
       delete q;
-      set_error("apt_submit: request footprint %zu B exceeds the budget", (size_t)npix * 28);
+      set_error("apt_submit: request footprint %zu B exceeds the budget of %llu B", q->footprint,
+                (unsigned long long)s->cfg.accum_budget);
       return APT_ERR_OOM;
     }
     q->id = s->next_id++;
@@ -565,6 +632,21 @@ apt_status apt_wait(apt_server* s, uint64_t id, uint32_t timeout_ms, apt_result*
     set_error("apt_wait: request %llu still running", (unsigned long long)id);
     return APT_ERR_BUSY;
   }
+  return APT_OK;
+}
+
+apt_status apt_release(apt_server* s, uint64_t id) {
+  clear_error();
+  APT_REQUIRE(s, APT_ERR_INVALID_ARG, "apt_release: null server");
+  std::lock_guard<std::mutex> lock(s->mu);
+  auto it = s->all.find(id);
+  APT_REQUIRE(it != s->all.end(), APT_ERR_NOT_FOUND, "apt_release: unknown request %llu",
+              (unsigned long long)id);
+  Req* q = it->second;
+  APT_REQUIRE(q->state >= APT_REQ_DONE && q->slot < 0 && q->inflight == 0, APT_ERR_BUSY,
+              "apt_release: request %llu is still queued or running", (unsigned long long)id);
+  delete q;
+  s->all.erase(it);
   return APT_OK;
 }
 

@@ -47,6 +47,7 @@ _SIGS = {
     "apt_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(Result)]),
     "apt_wait": (C.c_int, [_P, C.c_uint64, C.c_uint32, C.POINTER(Result)]),
     "apt_cancel": (C.c_int, [_P, C.c_uint64]),
+    "apt_release": (C.c_int, [_P, C.c_uint64]),
     "apt_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
 }
 
@@ -116,6 +117,11 @@ class Server:
     def cancel(self, rid: int) -> None:
         check(_lib().apt_cancel(self._h, rid), "apt_cancel")
 
+    def release(self, rid: int) -> np.ndarray:
+        """Forget a finished request (apt_release); returns its result."""
+        check(_lib().apt_release(self._h, rid), "apt_release")
+        return self._out.pop(rid)
+
     def stats(self) -> dict:
         s = ServerStats()
         check(_lib().apt_server_get_stats(self._h, C.byref(s)), "apt_server_get_stats")
@@ -131,3 +137,46 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- the serving workload ---------------------------------------------------
+
+RESOLUTIONS = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+SPPS = [1, 4, 16, 64]
+
+
+def mixed_requests(n: int, rate: float, seed: int = 5) -> list[dict]:
+    """The serving configuration's request stream: resolution drawn from
+    {256^2, 512^2, 1280x720, 1920x1080}, spp from {1, 4, 16, 64}, depth 4..8,
+    one of four resident scenes, a seeded orbit angle for the camera and
+    Poisson arrivals at `rate` requests/s (t in seconds)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        w, h = RESOLUTIONS[rng.integers(4)]
+        out.append({"t": t, "scene": int(rng.integers(4)), "w": w, "h": h,
+                    "spp": SPPS[rng.integers(4)], "depth": int(rng.integers(4, 9)),
+                    "angle": float(rng.uniform(0, 2 * np.pi)), "seed": i})
+    return out
+
+
+def orbit_camera(scene, angle: float) -> Camera:
+    """The scene's camera turned by angle/4 - 1/8 radians about a point 5
+    units ahead of it (a seeded pose per request)."""
+    from .world import look_at
+
+    m = np.array(scene.camera.transform[:], np.float32).reshape(4, 4)  # rows are columns
+    eye = m[3, :3].astype(np.float64)
+    back = m[2, :3].astype(np.float64)
+    target = eye - back * 5.0
+    off = eye - target
+    c, s_ = np.cos(angle * 0.25 - 0.125), np.sin(angle * 0.25 - 0.125)
+    rot = np.array([off[0] * c + off[2] * s_, off[1], -off[0] * s_ + off[2] * c])
+    cam = Camera()
+    cam.transform[:] = look_at(tuple(target + rot), tuple(target)).T.ravel().tolist()
+    cam.fov_deg = scene.camera.fov_deg
+    cam.near_plane = 0.01
+    cam.far_plane = 1000.0
+    return cam

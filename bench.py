@@ -555,40 +555,19 @@ def bench_restir(args) -> int:
     return 0
 
 
-def make_requests(n, rate, seed=5):
-    """Requests drawn as in the serving configuration: resolution from
-    {256^2, 512^2, 1280x720, 1920x1080}, spp from {1, 4, 16, 64}, depth 4..8,
-    scene 0..3, a seeded orbit camera; Poisson arrivals at `rate` req/s."""
+def size_classes(reqs, lat_ms, span_s):
+    """Requests per size class (samples <= 1 M, <= 16 M, more): count,
+    requests/s over the run, p50 and p99 latency."""
     import numpy as np
 
-    rng = np.random.default_rng(seed)
-    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
-    t = 0.0
-    out = []
-    for i in range(n):
-        t += rng.exponential(1.0 / rate)
-        w, h = res[rng.integers(4)]
-        out.append({"t": t, "scene": int(rng.integers(4)), "w": w, "h": h,
-                    "spp": [1, 4, 16, 64][rng.integers(4)], "depth": int(rng.integers(4, 9)),
-                    "angle": float(rng.uniform(0, 2 * np.pi)), "seed": i})
+    out = {}
+    for name, lo, hi in (("small <=1M samples", 0, 1 << 20), ("medium <=16M", 1 << 20, 1 << 24),
+                         ("large >16M", 1 << 24, 1 << 62)):
+        ls = [l for r, l in zip(reqs, lat_ms) if lo < r["w"] * r["h"] * r["spp"] <= hi]
+        if ls:
+            out[name] = {"requests": len(ls), "req_s": len(ls) / span_s,
+                         "p50_ms": float(np.percentile(ls, 50)), "p99_ms": float(np.percentile(ls, 99))}
     return out
-
-
-def orbit_camera(scene, angle):
-    import numpy as np
-
-    from tests.scenes_util import camera
-
-    cam = scene.camera
-    m = np.array(cam.transform[:], np.float32).reshape(4, 4)  # column-major rows = columns
-    eye = m[3, :3].astype(np.float64)
-    back = m[2, :3].astype(np.float64)
-    dist = 5.0
-    target = eye - back * dist
-    off = eye - target
-    c, s_ = np.cos(angle * 0.25 - 0.125), np.sin(angle * 0.25 - 0.125)
-    rot = np.array([off[0] * c + off[2] * s_, off[1], -off[0] * s_ + off[2] * c])
-    return camera(tuple(target + rot), tuple(target), cam.fov_deg)
 
 
 def bench_serving(args) -> int:
@@ -599,7 +578,7 @@ def bench_serving(args) -> int:
 
     from appearance_amd.renderer import PathTracerGpu, plain_config
     from appearance_amd.scene import generate_scene
-    from appearance_amd.serving import DONE, Server
+    from appearance_amd.serving import DONE, Server, mixed_requests, orbit_camera
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -619,7 +598,7 @@ def bench_serving(args) -> int:
     total_samples = None
     rates = [float(r) for r in args.rates.split(",")]
     for rate in rates:
-        reqs = make_requests(args.requests, rate)
+        reqs = mixed_requests(args.requests, rate)
         for r in reqs:
             r["cam"] = orbit_camera(scenes[r["scene"]], r["angle"])
         samples = sum(r["w"] * r["h"] * r["spp"] for r in reqs)
@@ -639,11 +618,12 @@ def bench_serving(args) -> int:
                 rids[i] = srv.submit(ids[r["scene"]], r["w"], r["h"], r["spp"], r["depth"], r["cam"],
                                      seed=r["seed"])
 
+        busy0 = srv.stats()["gpu_busy_ms"]
         th = threading.Thread(target=client)
         th.start()
         th.join()
         lat = []
-        last = 0.0
+        last, first = 0.0, float("inf")
         for k, rid in enumerate(rids):
             if k % 128 == 127:
                 log(f"served: {k + 1} of {len(rids)} requests done")
@@ -651,13 +631,18 @@ def bench_serving(args) -> int:
             assert res.state == DONE
             lat.append(res.done_ms - res.submit_ms)
             last = max(last, res.done_ms)
-        first = min(srv.poll(r).submit_ms for r in rids)
+            first = min(first, res.submit_ms)
+            srv.release(rid)
         span_s = (last - first) / 1e3
+        busy_s = (srv.stats()["gpu_busy_ms"] - busy0) / 1e3
         served = {"msamples_s": samples / span_s / 1e6, "p50_ms": float(np.percentile(lat, 50)),
-                  "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span_s}
+                  "p99_ms": float(np.percentile(lat, 99)), "makespan_s": span_s,
+                  "gpu_busy_s": busy_s, "gpu_busy_frac": busy_s / span_s,
+                  "by_size": size_classes(reqs, lat, span_s)}
         # one at a time: FIFO through the plain renderer
         log("one-at-a-time baseline")
         lat_b = []
+        dev_b = 0.0
         t0 = time.perf_counter()
         for k, r in enumerate(reqs):
             if k % 128 == 127:
@@ -670,9 +655,12 @@ def bench_serving(args) -> int:
             pt.set_camera(r["cam"])
             pt.render_accum((r["w"], r["h"]), with_rgba=False)
             lat_b.append((time.perf_counter() - t0 - r["t"]) * 1e3)
+            dev_b += pt.stats()["ms_total"] / 1e3
         span_b = time.perf_counter() - t0 - reqs[0]["t"]
         base = {"msamples_s": samples / span_b / 1e6, "p50_ms": float(np.percentile(lat_b, 50)),
-                "p99_ms": float(np.percentile(lat_b, 99)), "makespan_s": span_b}
+                "p99_ms": float(np.percentile(lat_b, 99)), "makespan_s": span_b,
+                "gpu_busy_s": dev_b, "gpu_busy_frac": dev_b / span_b,
+                "by_size": size_classes(reqs, lat_b, span_b)}
         report["rates"][str(rate)] = {"served": served, "one_at_a_time": base,
                                       "throughput_ratio": served["msamples_s"] / base["msamples_s"]}
         log(f"rate {rate}: served {served}, baseline {base}")

@@ -433,7 +433,9 @@ typedef struct apt_server apt_server;
 typedef struct apt_server_config {
   uint32_t path_slots;        /* paths per wavefront batch (0 = 2^24) */
   uint32_t max_active;        /* admitted requests at a time (0 = 64) */
-  uint64_t accum_budget;      /* bytes of per-request accumulators admitted (0 = 4 GiB) */
+  uint64_t accum_budget;      /* bytes of the accumulator arena allocated at creation, from which each
+                                 admitted request takes 36 B per pixel (0 = 4 GiB); a request waits in
+                                 the queue until a range of that size is free */
   uint32_t max_bounces;       /* deepest path any request may ask for (0 = 8) */
   uint32_t tile_paths;        /* paths per scheduling tile (0 = 65536) */
   uint32_t bvh_width;         /* 2, 4, 8 (0 = 4) */
@@ -486,6 +488,11 @@ apt_status apt_poll(apt_server* s, uint64_t id, apt_result* out);
  * (APT_ERR_BUSY). */
 apt_status apt_wait(apt_server* s, uint64_t id, uint32_t timeout_ms, apt_result* out);
 apt_status apt_cancel(apt_server* s, uint64_t id);
+/* Forgets a finished (done, cancelled or failed) request: its record is
+ * freed and later calls with its id return APT_ERR_NOT_FOUND; APT_ERR_BUSY
+ * while it is queued or running.  A long-running caller releases every
+ * request once it has read the result. */
+apt_status apt_release(apt_server* s, uint64_t id);
 
 typedef struct apt_server_stats {
   uint64_t batches;

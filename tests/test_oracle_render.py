@@ -44,3 +44,44 @@ def test_cornell_fp32_vs_fp64(oracle):
     rel = np.sum((f32 - f64) ** 2) / np.sum(f64 ** 2)
     assert rel < 1e-3, rel
     assert f32.mean() > 0.05
+
+
+def cornell_fp64_fixture():
+    """The FP64 2^16-spp Cornell radiance on 4096 seeded pixels
+    (tests/golden/make_cornell_fp64.py)."""
+    from pathlib import Path
+    f = np.load(Path(__file__).parent / "golden" / "cornell_fp64_4096px.npz")
+    return f["pixels"], f["mean"], f["batch_var"], int(f["batches"])
+
+
+def fp64_zscores(est, mean, batch_var, batches, spp):
+    """Per-pixel z of a spp estimate against the fixture: the difference over
+    the standard deviation of the two estimates (the estimate's variance is
+    batch_var x 4096 / spp, the fixture's batch_var / batches)."""
+    sd = np.sqrt(batch_var * (4096.0 / spp) + batch_var / batches + (1e-6 * np.abs(mean)) ** 2 + 1e-30)
+    return (est - mean) / sd
+
+
+def relmse(a, b):
+    return float(np.sum((a - b) ** 2) / np.sum(b ** 2))
+
+
+def test_cornell_fp32_against_fp64_high_spp(oracle):
+    """The FP32 oracle at 1024 spp (frame 0, its own seeds) against the FP64
+    2^16-spp mean on the fixture's pixels: the per-pixel errors are the noise
+    of the two estimates (z rms 1: no bias between the precisions), and a 1%
+    planted bias stands out.  (The summed relMSE is dominated by a few
+    firefly pixels and varies 6x between seeds, so z is the statistic.)"""
+    pix, mean, bvar, nb = cornell_fp64_fixture()
+    s = generate_scene("cornell")
+    f32, _ = oracle_render(oracle, s, 512, 512, spp=1024, max_bounces=6, rr_start=3, pixels=pix)
+    f32 = f32.astype(np.float64)
+    z = fp64_zscores(f32, mean, bvar, nb, 1024)
+    zb = fp64_zscores(f32 * 1.01, mean, bvar, nb, 1024)
+    zrms, zbrms = float(np.sqrt((z ** 2).mean())), float(np.sqrt((zb ** 2).mean()))
+    print(f"[oracle] cornell fp32 1024 spp vs fp64 2^16 spp: relMSE {relmse(f32, mean):.3e}, z rms {zrms:.3f}, "
+          f"with a 1% bias {zbrms:.3f}")
+    assert 0.9 < zrms < 1.1
+    assert (np.abs(z) > 5).mean() < 5e-3
+    assert zbrms > 1.3
+    assert abs(f32.mean() - mean.mean()) / mean.mean() < 2e-3
