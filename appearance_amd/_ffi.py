@@ -109,6 +109,8 @@ assert C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
 assert C.sizeof(Material) == 160 and C.sizeof(Camera) == 80
 
 RESULT_CB = C.CFUNCTYPE(None, C.POINTER(C.c_uint8), C.c_size_t, C.c_void_p)
+BLOCK_CB = C.CFUNCTYPE(None, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8), C.c_size_t,
+                       C.c_void_p)
 
 _P = C.c_void_p
 _SIGS = {
@@ -143,6 +145,7 @@ _SIGS = {
                                 _P]),
     "apt_get_render_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "apt_set_frame_index": (C.c_int, [_P, C.c_uint32]),
+    "apt_start_render": (C.c_int, [_P, _P, C.c_size_t, BLOCK_CB, _P]),
 }
 
 _lib = None

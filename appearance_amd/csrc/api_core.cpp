@@ -46,6 +46,7 @@ static_assert(sizeof(apt_ray) == 32, "apt_ray must be 32 B");
 static_assert(sizeof(apt_hit) == 16, "apt_hit must be 16 B");
 static_assert(sizeof(apt_material) == 160, "apt_material must match MaterialDescriptor (160 B)");
 static_assert(sizeof(apt_camera) == 80, "apt_camera must match CameraUpdateData (80 B)");
+static_assert(sizeof(apt_start_render_data) == 20, "apt_start_render_data must match StartRenderData (20 B)");
 
 extern "C" {
 

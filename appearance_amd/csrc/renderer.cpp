@@ -799,6 +799,35 @@ apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint3
   return APT_OK;
 }
 
+apt_status apt_start_render(apt_renderer* r, const void* start_render_data, size_t len, apt_block_cb cb,
+                            void* user) {
+  clear_error();
+  APT_REQUIRE(r && start_render_data && cb, APT_ERR_INVALID_ARG, "apt_start_render: null argument");
+  APT_REQUIRE(len == sizeof(apt_start_render_data), APT_ERR_INVALID_ARG,
+              "apt_start_render: StartRenderData carries %zu bytes, got %zu", sizeof(apt_start_render_data), len);
+  apt_start_render_data d;
+  std::memcpy(&d, start_render_data, sizeof(d));
+  constexpr uint32_t B = 64;  // RENDER_BLOCK_SIZE
+  APT_REQUIRE(d.row_end > d.row_start && d.row_end <= d.height && d.width % B == 0 &&
+                  (d.row_end - d.row_start) % B == 0,
+              APT_ERR_INVALID_ARG, "apt_start_render: %ux%u rows [%u,%u) is not a slab of whole 64x64 blocks",
+              d.width, d.height, d.row_start, d.row_end);
+  const size_t bytes = (size_t)d.width * (d.row_end - d.row_start) * 4;
+  apt_status st = r->cb_pixels.ensure(bytes);
+  if (st != APT_OK) return st;
+  st = render_impl(r, d.width, d.height, d.row_start, d.row_end, r->cb_pixels.p, bytes, nullptr, 0);
+  if (st != APT_OK) return st;
+  // Node::start_render's slicing (node.rs:55-75): block (x, y) of the slab
+  // starts at pixel (y * nx + x) * 64 * 64 of the block-major frame.
+  const uint32_t nx = d.width / B, ny = (d.row_end - d.row_start) / B;
+  for (uint32_t by = 0; by < ny; ++by)
+    for (uint32_t bx = 0; bx < nx; ++bx) {
+      const size_t first = ((size_t)by * nx + bx) * B * B;
+      cb(by * B + d.row_start, bx, d.frame_idx, r->cb_pixels.p + first * 4, (size_t)B * B * 4, user);
+    }
+  return APT_OK;
+}
+
 apt_status apt_set_frame_index(apt_renderer* r, uint32_t frame_idx) {
   clear_error();
   APT_REQUIRE(r, APT_ERR_INVALID_ARG, "apt_set_frame_index: renderer is NULL");

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import world
-from ._ffi import RESULT_CB, Config, MeshDesc, RenderStats, Sky, check, lib, vptr
+from ._ffi import BLOCK_CB, RESULT_CB, Config, MeshDesc, RenderStats, Sky, check, lib, vptr
 
 
 def default_config(**overrides) -> Config:
@@ -126,6 +126,19 @@ class PathTracerGpu:
         w, h = int(resolution[0]), int(resolution[1])
         end_row = h if end_row is None else end_row
         check(lib().apt_render(self._h, w, h, start_row, end_row, None, 0, None, 0), "apt_render")
+
+    def start_render(self, data, block_callback) -> None:
+        """Node::start_render (node.rs:44-105) without the transport: data is a
+        world.StartRenderData or its 20 bytes; block_callback(row, column_block,
+        frame_idx, pixels) receives each 64x64 RGBA8 block of the slab."""
+        raw = data.to_bytes() if hasattr(data, "to_bytes") else bytes(data)
+
+        def _cb(row, col, frame, ptr, n, _user):
+            block_callback(row, col, frame, C.string_at(ptr, n))
+
+        cb = BLOCK_CB(_cb)
+        buf = C.create_string_buffer(raw, len(raw))
+        check(lib().apt_start_render(self._h, buf, len(raw), cb, None), "apt_start_render")
 
     def set_frame_index(self, frame_idx: int) -> None:
         """Restart the frame sequence at frame_idx (seeds frame * spp + sample)."""

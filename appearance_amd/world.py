@@ -107,3 +107,23 @@ def from_ty_and_bytes(ty: int, data: bytes):
     if ty == DESTROY_MODEL:
         return DestroyModel(_uuid.UUID(bytes=data))
     return Clear(struct.unpack("<I", data)[0])
+
+
+@dataclass
+class StartRenderData:
+    """The render-node job (crates/appearance-render-loop/src/host.rs:85-93),
+    20 bytes: width, height, row_start, row_end, frame_idx (u32 each)."""
+    width: int
+    height: int
+    row_start: int
+    row_end: int
+    frame_idx: int = 0
+
+    def to_bytes(self) -> bytes:
+        return struct.pack("<5I", self.width, self.height, self.row_start, self.row_end, self.frame_idx)
+
+    @classmethod
+    def from_bytes(cls, b: bytes) -> "StartRenderData":
+        if len(b) != 20:
+            raise ValueError(f"StartRenderData carries 20 bytes, got {len(b)}")
+        return cls(*struct.unpack("<5I", b))

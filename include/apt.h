@@ -367,6 +367,33 @@ typedef void (*apt_result_cb)(const uint8_t* pixels, size_t len, void* user);
 apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint32_t row_start,
                          uint32_t row_end, apt_result_cb cb, void* user);
 
+/* StartRenderData (crates/appearance-render-loop/src/host.rs:85-93), 20 B:
+ * the render-node job the host sends (HostToNodeMessage::StartRender). */
+typedef struct apt_start_render_data {
+  uint32_t width;
+  uint32_t height;
+  uint32_t row_start;
+  uint32_t row_end;
+  uint32_t frame_idx;
+} apt_start_render_data;
+
+/* One 64x64 block of a rendered slab, the unit Node::start_render sends as a
+ * RenderPartialFinished message (node.rs:55-93): the block's first frame row,
+ * its column, the job's frame_idx and 64*64*4 RGBA8 bytes, valid only during
+ * the call. */
+typedef void (*apt_block_cb)(uint32_t row, uint32_t column_block, uint32_t frame_idx, const uint8_t* pixels,
+                             size_t len, void* user);
+
+/* Node::start_render (crates/appearance-render-loop/src/node.rs:44-105)
+ * without the transport: start_render_data holds the 20 bytes of
+ * StartRenderData as they arrive; the slab is rendered (NodeRenderer::render)
+ * and cb receives its blocks in the order the node sends them (block rows top
+ * to bottom, columns left to right).  The JPEG compression and the UDP
+ * messages stay with the caller.  Width and slab height must be multiples of
+ * 64 (host.rs:476). */
+apt_status apt_start_render(apt_renderer* r, const void* start_render_data, size_t len, apt_block_cb cb,
+                            void* user);
+
 /* Restarts the frame sequence at frame_idx (seeds are frame*spp + sample). */
 apt_status apt_set_frame_index(apt_renderer* r, uint32_t frame_idx);
 

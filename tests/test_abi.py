@@ -76,3 +76,11 @@ def test_rays_bvh_stress():
     assert np.all(r["org"][:4096] == np.array([0, 0, -25], np.float32))
     assert np.all(np.abs(r["org"][4096:]) <= 10.0)
     assert np.all(r["tmax"] == np.float32(1e30))
+
+
+def test_start_render_data_bytes():
+    from appearance_amd import world
+    d = world.StartRenderData(1920, 1088, 64, 512, 9)
+    b = d.to_bytes()
+    assert len(b) == 20 and b[:4] == (1920).to_bytes(4, "little")
+    assert world.StartRenderData.from_bytes(b) == d

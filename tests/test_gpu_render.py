@@ -275,3 +275,77 @@ def test_autotune_cornell(gpu_available, tmp_path):
     best = rep["best"]
     assert best["bvh_width"] in (2, 4, 8) and best["lds_stack_depth"] in (8, 16, 24)
     assert (tmp_path / "t.json").exists()
+
+
+def test_spawn_transform_matches_numpy_placement(gpu_available):
+    """SpawnModel with a rotation, a translation and a non-uniform scale
+    (the device placement kernel and the inverse-transpose normal matrix)
+    against apt_scene_set fed the same model placed in numpy: positions
+    M p in float32 with the kernel's order of operations, vertex normals
+    through the inverse transpose of M's 3x3 part and renormalised.  The
+    scenes are the Disney spheres (vertex normals) and the Cornell box
+    (geometric normals)."""
+    from appearance_amd.scene import SceneData
+    a = np.radians(35.0)
+    rot = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    b = np.radians(-12.0)
+    rot = rot @ np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    m = np.eye(4)
+    m[:3, :3] = rot @ np.diag([1.3, 0.7, 1.15])
+    m[:3, 3] = [0.4, -0.25, 0.6]
+    m = m.astype(np.float32)
+    f = np.float32
+    for name, seed in (("disney", 3), ("cornell", 0)):
+        s = generate_scene(name, seed=seed)
+        p = s.positions.reshape(-1, 3).astype(np.float32)
+        world_p = np.empty_like(p)
+        for r in range(3):
+            world_p[:, r] = (m[r, 0] * p[:, 0] + m[r, 1] * p[:, 1]) + m[r, 2] * p[:, 2] + m[r, 3]
+        nrm = None
+        if s.normals is not None:
+            nm = np.linalg.inv(m[:3, :3].astype(np.float64)).T.astype(np.float32)
+            q = s.normals.reshape(-1, 3).astype(np.float32)
+            v = np.empty_like(q)
+            for r in range(3):
+                v[:, r] = (nm[r, 0] * q[:, 0] + nm[r, 1] * q[:, 1]) + nm[r, 2] * q[:, 2]
+            ln = np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+            nrm = np.where(ln[:, None] > f(0), v / ln[:, None], v).reshape(-1, 9).astype(np.float32)
+        placed = SceneData(world_p.reshape(-1, 9), nrm, s.material_ids, s.materials, s.camera, s.sky)
+        spawn = PathTracerGpu(plain_config(sample_count=2, max_bounces=4))
+        spawn.handle_visible_world_action(world.SpawnModel(f"procedural://{name}?seed={seed}", m))
+        spawn.set_camera(s.camera)
+        _, got = spawn.render_accum((256, 128), with_rgba=False)
+        direct = PathTracerGpu(plain_config(sample_count=2, max_bounces=4))
+        direct.set_scene(placed, sky=s.sky)
+        direct.set_camera(s.camera)
+        _, want = direct.render_accum((256, 128), with_rgba=False)
+        r = relmse(got, want)
+        print(f"[render] {name} spawned with a transform vs numpy placement: relMSE {r:.3e}, "
+              f"bit-identical {np.array_equal(got, want)}", flush=True)
+        assert want.mean() > 0.01
+        assert r < 1e-6
+
+
+def test_start_render_job(gpu_available):
+    """apt_start_render: the 20-byte StartRenderData job renders the slab and
+    hands over its 64x64 blocks in Node::start_render's order (node.rs:55-93),
+    each tagged with its row, column block and the job's frame_idx; put
+    together they are the slab that apt_render returns."""
+    s = generate_scene("cornell")
+    job = world.StartRenderData(192, 256, 64, 192, frame_idx=7)
+    a = PathTracerGpu(plain_config(sample_count=2, max_bounces=3))
+    a.set_scene(s)
+    a.set_camera(s.camera)
+    blocks = []
+    a.start_render(job.to_bytes(), lambda row, col, frame, px: blocks.append((row, col, frame, px)))
+    assert [(r, c) for r, c, _, _ in blocks] == [(64 + 64 * by, bx) for by in range(2) for bx in range(3)]
+    assert all(f == 7 and len(px) == 64 * 64 * 4 for _, _, f, px in blocks)
+    b = PathTracerGpu(plain_config(sample_count=2, max_bounces=3))
+    b.set_scene(s)
+    b.set_camera(s.camera)
+    rgba = b.render_rgba((192, 256), 64, 192)
+    assert b"".join(px for _, _, _, px in blocks) == bytes(rgba)
+    with pytest.raises(Exception):
+        a.start_render(world.StartRenderData(100, 256, 0, 64).to_bytes(), lambda *x: None)
+    with pytest.raises(Exception):
+        a.start_render(job.to_bytes()[:16], lambda *x: None)
