@@ -28,6 +28,7 @@ SOURCES = [
     "tuner.cpp",
     "restir.hip",
     "bvh_gpu.hip",
+    "gltf.cpp",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

@@ -116,6 +116,21 @@ struct apt_bvh_impl;
 
 }  // namespace apt
 
+// A triangle soup with per-triangle materials (procedural scenes, glTF files).
+struct apt_scene_data {
+  std::vector<float> positions;   // n_tris * 9
+  std::vector<float> normals;     // n_tris * 9 or empty
+  std::vector<uint32_t> mat_ids;  // n_tris
+  std::vector<apt_material> materials;
+  apt_camera camera{};
+  apt_sky sky{};
+};
+
+namespace apt {
+// glTF ingestion (gltf.cpp): a .glb / .gltf file into a scene.
+apt_status gltf_load(const char* path, apt_scene_data** out);
+}  // namespace apt
+
 // The opaque handle of the C ABI.
 struct apt_bvh {
   uint32_t width = 4;

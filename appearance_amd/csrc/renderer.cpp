@@ -701,12 +701,11 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
       path[256] = 0;
       std::string name;
       uint32_t seed;
-      APT_REQUIRE(parse_procedural(path, name, seed), APT_ERR_UNSUPPORTED,
-                  "SpawnModel: '%s' is not a procedural:// asset (glTF ingestion is not available)", path);
       Model m;
       std::memcpy(m.xform, b, 64);
       std::memcpy(m.uuid, b + 64, 16);
-      apt_status st = apt_scene_generate(name.c_str(), seed, &m.data);
+      apt_status st = parse_procedural(path, name, seed) ? apt_scene_generate(name.c_str(), seed, &m.data)
+                                                         : gltf_load(path, &m.data);
       if (st != APT_OK) return st;
       apt_sky sky;
       apt_scene_data_view(m.data, nullptr, nullptr, &sky);

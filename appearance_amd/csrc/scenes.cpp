@@ -12,14 +12,6 @@
 
 #include "apt_internal.h"
 
-struct apt_scene_data {
-  std::vector<float> positions;   // n_tris * 9
-  std::vector<float> normals;     // n_tris * 9 or empty
-  std::vector<uint32_t> mat_ids;  // n_tris
-  std::vector<apt_material> materials;
-  apt_camera camera{};
-  apt_sky sky{};
-};
 
 namespace apt {
 namespace {

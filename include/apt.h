@@ -276,6 +276,16 @@ typedef struct apt_camera {
  * SpawnModel world action. */
 typedef struct apt_scene_data apt_scene_data;
 apt_status apt_scene_generate(const char* name, uint32_t seed, apt_scene_data** out);
+/* Reads a glTF 2.0 model (.glb, or .gltf with external or base64 buffers)
+ * into a triangle soup with materials, as the reference's asset loader
+ * (crates/appearance-model/src/asset.rs:19-380) builds a Model: the default
+ * scene's nodes with their transforms applied, triangle primitives with
+ * POSITION, NORMAL (generated when absent, mesh.rs:58-79) and indices, and
+ * each material's factors mapped as asset.rs:199-339 maps them (emissive
+ * strength, volume absorption, transmission, eta = 1/ior, specular,
+ * clearcoat, sheen).  Textures are not decoded: every material renders with
+ * its factors.  The same paths work as SpawnModel asset paths. */
+apt_status apt_scene_load_gltf(const char* path, apt_scene_data** out);
 /* Views into the library-owned arrays; valid until apt_scene_data_destroy. */
 apt_status apt_scene_data_view(const apt_scene_data* s, apt_mesh_desc* mesh, apt_camera* camera,
                                apt_sky* sky);
@@ -345,8 +355,8 @@ apt_status apt_set_config(apt_renderer* r, const apt_config* cfg);
 /* VisibleWorldActionType::from_ty_and_bytes (visible_world_action.rs:87-96):
  * ty 0 CameraUpdate (80 B), 1 SpawnModel (336 B), 2 TransformModel (80 B),
  * 3 DestroyModel (16 B), 4 Clear (4 B).  SpawnModel accepts
- * "procedural://<name>?seed=<n>" asset paths; glTF files return
- * APT_ERR_UNSUPPORTED. */
+ * "procedural://<name>?seed=<n>" asset paths and glTF files (.glb, .gltf;
+ * apt_scene_load_gltf). */
 apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, size_t len);
 
 /* Replace the whole scene with a triangle soup (procedural or caller-made). */

@@ -369,6 +369,7 @@ extern "C" {
     // scene data
     pub fn apt_material_default(m: *mut apt_material);
     pub fn apt_scene_generate(name: *const c_char, seed: u32, out: *mut *mut apt_scene_data) -> apt_status;
+    pub fn apt_scene_load_gltf(path: *const c_char, out: *mut *mut apt_scene_data) -> apt_status;
     pub fn apt_scene_data_view(s: *const apt_scene_data, mesh: *mut apt_mesh_desc, camera: *mut apt_camera,
                                sky: *mut apt_sky) -> apt_status;
     pub fn apt_scene_data_destroy(s: *mut apt_scene_data);

@@ -1,0 +1,223 @@
+"""glTF 2.0 helpers for the tests, independent of libapt's C++ reader:
+write_glb makes small binary glTF files; read_glb flattens one the way the
+reference's loader does (default scene, node TRS / matrix, triangle
+primitives, per-vertex normals or generated ones) with numpy."""
+from __future__ import annotations
+
+import json
+import struct
+
+import numpy as np
+
+CTYPE = {5126: np.float32, 5125: np.uint32, 5123: np.uint16, 5121: np.uint8}
+NCOMP = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4}
+
+
+def quat_matrix(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def node_matrix(n):
+    if "matrix" in n:
+        return np.array(n["matrix"], np.float64).reshape(4, 4).T
+    m = np.eye(4)
+    m[:3, :3] = quat_matrix(n.get("rotation", [0, 0, 0, 1])) @ np.diag(n.get("scale", [1, 1, 1]))
+    m[:3, 3] = n.get("translation", [0, 0, 0])
+    return m
+
+
+def read_glb(path):
+    b = open(path, "rb").read()
+    assert b[:4] == b"glTF"
+    jlen = struct.unpack("<I", b[12:16])[0]
+    doc = json.loads(b[20:20 + jlen])
+    off = 20 + jlen
+    binc = b""
+    if off + 8 <= len(b):
+        blen = struct.unpack("<I", b[off:off + 4])[0]
+        binc = b[off + 8:off + 8 + blen]
+
+    def acc(i):
+        a = doc["accessors"][i]
+        bv = doc["bufferViews"][a["bufferView"]]
+        dt = np.dtype(CTYPE[a["componentType"]])
+        nc = NCOMP[a["type"]]
+        start = bv.get("byteOffset", 0) + a.get("byteOffset", 0)
+        stride = bv.get("byteStride", dt.itemsize * nc)
+        raw = np.frombuffer(binc, np.uint8)
+        view = np.lib.stride_tricks.as_strided(raw[start:], shape=(a["count"], nc * dt.itemsize),
+                                               strides=(stride, 1))
+        return np.ascontiguousarray(view).view(dt).reshape(a["count"], nc)
+
+    pos, nrm, mids = [], [], []
+
+    def visit(i, parent):
+        n = doc["nodes"][i]
+        w = parent @ node_matrix(n)
+        if "mesh" in n:
+            for p in doc["meshes"][n["mesh"]]["primitives"]:
+                P = acc(p["attributes"]["POSITION"]).astype(np.float32).astype(np.float64)
+                I = acc(p["indices"]).reshape(-1).astype(np.int64)
+                if "NORMAL" in p["attributes"]:
+                    N = acc(p["attributes"]["NORMAL"]).astype(np.float32).astype(np.float64)
+                else:
+                    N = np.zeros_like(P)
+                    T = I.reshape(-1, 3)
+                    f = np.cross(P[T[:, 1]] - P[T[:, 0]], P[T[:, 2]] - P[T[:, 0]])
+                    ln = np.linalg.norm(f, axis=1, keepdims=True)
+                    f = np.where(ln > 0, f / np.where(ln > 0, ln, 1), 0)
+                    for j in range(3):
+                        np.add.at(N, T[:, j], f)
+                    ln = np.linalg.norm(N, axis=1, keepdims=True)
+                    N = np.where(ln > 0, N / np.where(ln > 0, ln, 1), 0).astype(np.float32).astype(np.float64)
+                wp = (w[:3, :3] @ P.T).T + w[:3, 3]
+                nm = np.linalg.inv(w[:3, :3]).T
+                wn = (nm @ N.T).T
+                wn /= np.linalg.norm(wn, axis=1, keepdims=True)
+                pos.append(wp[I].reshape(-1, 9))
+                nrm.append(wn[I].reshape(-1, 9))
+                mids.append(np.full(len(I) // 3, p.get("material", 0)))
+        for c in n.get("children", []):
+            visit(c, w)
+
+    sc = doc["scenes"][doc.get("scene", 0)]
+    for r in sc["nodes"]:
+        visit(r, np.eye(4))
+    return (np.concatenate(pos), np.concatenate(nrm), np.concatenate(mids), doc)
+
+
+def write_glb(path, nodes, meshes, materials, scene_roots, extensions=()):
+    """meshes: list of lists of primitives, each a dict with positions (v,3),
+    indices (t*3,), optional normals (v,3), material, index_type (5125 or
+    5123), interleaved (positions and normals in one strided view)."""
+    binc = bytearray()
+    accessors, views = [], []
+
+    def add_view(data: bytes, stride=None):
+        while len(binc) % 4:
+            binc.append(0)
+        v = {"buffer": 0, "byteOffset": len(binc), "byteLength": len(data)}
+        if stride:
+            v["byteStride"] = stride
+        binc.extend(data)
+        views.append(v)
+        return len(views) - 1
+
+    def add_acc(view, ctype, count, typ, offset=0, mn=None, mx=None):
+        a = {"bufferView": view, "componentType": ctype, "count": count, "type": typ}
+        if offset:
+            a["byteOffset"] = offset
+        if mn is not None:
+            a["min"], a["max"] = mn, mx
+        accessors.append(a)
+        return len(accessors) - 1
+
+    gl_meshes = []
+    for prims in meshes:
+        gp = []
+        for p in prims:
+            P = np.asarray(p["positions"], np.float32)
+            attrs = {}
+            if p.get("interleaved") and "normals" in p:
+                N = np.asarray(p["normals"], np.float32)
+                inter = np.concatenate([P, N], axis=1).astype(np.float32)
+                v = add_view(inter.tobytes(), stride=24)
+                attrs["POSITION"] = add_acc(v, 5126, len(P), "VEC3", 0, P.min(0).tolist(), P.max(0).tolist())
+                attrs["NORMAL"] = add_acc(v, 5126, len(P), "VEC3", 12)
+            else:
+                attrs["POSITION"] = add_acc(add_view(P.tobytes()), 5126, len(P), "VEC3", 0, P.min(0).tolist(),
+                                            P.max(0).tolist())
+                if "normals" in p:
+                    attrs["NORMAL"] = add_acc(add_view(np.asarray(p["normals"], np.float32).tobytes()), 5126,
+                                              len(P), "VEC3")
+            it = p.get("index_type", 5125)
+            I = np.asarray(p["indices"], np.uint32 if it == 5125 else np.uint16)
+            prim = {"attributes": attrs, "indices": add_acc(add_view(I.tobytes()), it, len(I), "SCALAR")}
+            if "material" in p:
+                prim["material"] = p["material"]
+            if "mode" in p:
+                prim["mode"] = p["mode"]
+            gp.append(prim)
+        gl_meshes.append({"primitives": gp})
+    while len(binc) % 4:
+        binc.append(0)
+    doc = {"asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": list(scene_roots)}], "nodes": nodes,
+           "meshes": gl_meshes, "materials": materials, "accessors": accessors, "bufferViews": views,
+           "buffers": [{"byteLength": len(binc)}]}
+    if extensions:
+        doc["extensionsUsed"] = list(extensions)
+    js = json.dumps(doc).encode()
+    while len(js) % 4:
+        js += b" "
+    out = struct.pack("<4sII", b"glTF", 2, 12 + 8 + len(js) + 8 + len(binc))
+    out += struct.pack("<I4s", len(js), b"JSON") + js + struct.pack("<I4s", len(binc), b"BIN\0") + bytes(binc)
+    open(path, "wb").write(out)
+
+
+def box(center, half):
+    """A closed box (12 triangles) wound outwards."""
+    c = np.asarray(center, np.float64)
+    h = np.asarray(half, np.float64)
+    v = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float64) * h + c
+    f = [(0, 1, 3), (0, 3, 2), (4, 6, 7), (4, 7, 5), (0, 4, 5), (0, 5, 1), (2, 3, 7), (2, 7, 6), (0, 2, 6),
+         (0, 6, 4), (1, 5, 7), (1, 7, 3)]
+    idx = np.array(f, np.uint32).reshape(-1)
+    return v.astype(np.float32), idx
+
+
+def demo_scene(path):
+    """A small scene exercising the reader: a floor, two boxes under a node
+    hierarchy with TRS and matrix transforms, an emissive quad, a glass
+    panel; 16-bit and 32-bit indices, an interleaved view, generated normals;
+    materials with the emissive-strength, transmission, ior, volume,
+    clearcoat, sheen and specular extensions."""
+    floor_v = np.array([[-3, 0, -3], [3, 0, -3], [3, 0, 3], [-3, 0, 3]], np.float32)
+    floor_i = np.array([0, 2, 1, 0, 3, 2], np.uint32)
+    floor_n = np.tile([0, 1, 0], (4, 1)).astype(np.float32)
+    bv, bi = box((0, 0.5, 0), (0.5, 0.5, 0.5))
+    light_v = np.array([[-0.6, 2.5, -0.6], [0.6, 2.5, -0.6], [0.6, 2.5, 0.6], [-0.6, 2.5, 0.6]], np.float32)
+    light_i = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    glass_v = np.array([[-1, 0, 0], [1, 0, 0], [1, 1.5, 0], [-1, 1.5, 0]], np.float32)
+    glass_i = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    meshes = [
+        [{"positions": floor_v, "indices": floor_i, "normals": floor_n, "material": 0, "interleaved": True}],
+        [{"positions": bv, "indices": bi, "material": 1, "index_type": 5123}],
+        [{"positions": light_v, "indices": light_i, "material": 2}],
+        [{"positions": glass_v, "indices": glass_i, "normals": np.tile([0, 0, 1], (4, 1)), "material": 3}],
+    ]
+    materials = [
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.7, 0.7, 0.65, 1], "metallicFactor": 0.0,
+                                  "roughnessFactor": 0.8}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.2, 0.1, 1], "metallicFactor": 0.3,
+                                  "roughnessFactor": 0.4},
+         "extensions": {"KHR_materials_clearcoat": {"clearcoatFactor": 0.7, "clearcoatRoughnessFactor": 0.1},
+                        "KHR_materials_sheen": {"sheenColorFactor": [0.2, 0.3, 0.4], "sheenRoughnessFactor": 0.6},
+                        "KHR_materials_specular": {"specularFactor": 0.5,
+                                                   "specularColorFactor": [1.0, 0.9, 0.8]}}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [1, 1, 1, 1], "metallicFactor": 0.0},
+         "emissiveFactor": [1.0, 0.8, 0.6],
+         "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 12.0}}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.9, 1.0, 0.95, 1], "metallicFactor": 0.0,
+                                  "roughnessFactor": 0.05},
+         "extensions": {"KHR_materials_transmission": {"transmissionFactor": 1.0},
+                        "KHR_materials_ior": {"ior": 1.45},
+                        "KHR_materials_volume": {"attenuationColor": [0.9, 0.8, 0.7],
+                                                 "attenuationDistance": 2.0}}},
+    ]
+    a = np.radians(25.0)
+    nodes = [
+        {"mesh": 0},
+        {"translation": [0.0, 0.0, -0.5], "rotation": [0, np.sin(a / 2), 0, np.cos(a / 2)],
+         "children": [2, 3]},
+        {"mesh": 1, "translation": [-1.2, 0, 0.3], "scale": [1.0, 1.6, 1.0]},
+        {"mesh": 1, "matrix": [0.9, 0, 0.2, 0, 0, 0.9, 0, 0, -0.2, 0, 0.9, 0, 1.1, 0, 0.2, 1]},
+        {"mesh": 2},
+        {"mesh": 3, "translation": [0.0, 0.0, 1.6]},
+    ]
+    write_glb(path, nodes, meshes, materials, [0, 1, 4, 5],
+              extensions=("KHR_materials_clearcoat", "KHR_materials_sheen", "KHR_materials_specular",
+                          "KHR_materials_emissive_strength", "KHR_materials_transmission", "KHR_materials_ior",
+                          "KHR_materials_volume"))
