@@ -93,3 +93,33 @@ def test_errors(tmp_path):
     write_glb(r, [{"mesh": 0}], [[{"positions": v, "indices": [0, 1, 2]}]], [], [0])
     s = load_gltf(r)
     assert s.n_tris == 1 and len(s.materials) == 1 and s.materials[0].metallic == 1.0
+
+
+CONOR = Path(__file__).resolve().parent / "golden" / "cornell_conor.glb"
+
+
+def test_cornell_conor_fixture_loads():
+    """The committed re-encoding of the reference's CornellBoxConor.glb
+    (tests/golden/make_cornell_conor.py): 36 triangles in a 2 m box, eight
+    materials, the light at emissiveFactor 1 x emissive strength 5."""
+    s, doc = compare(CONOR)
+    assert s.n_tris == 36 and len(s.materials) == 8
+    lo, hi = s.positions.reshape(-1, 3).min(0), s.positions.reshape(-1, 3).max(0)
+    np.testing.assert_allclose(lo, [-1, 0, -1], atol=1e-5)
+    np.testing.assert_allclose(hi, [1, 2, 1], atol=1e-5)
+    light = [i for i, m in enumerate(s.materials) if max(m.emission) > 0]
+    assert light == [5] and list(s.materials[5].emission) == [5.0, 5.0, 5.0]
+    assert (s.material_ids == 5).sum() == 2
+
+
+@pytest.mark.skipif(not (ASSETS / "CornellBoxConor.glb").exists(), reason="the reference's assets are not on this host")
+def test_cornell_conor_fixture_equals_original():
+    """libapt reads the original asset (textures and all) and the fixture into
+    the same triangles and materials, up to the order of the triangles."""
+    a = load_gltf(ASSETS / "CornellBoxConor.glb")
+    b = load_gltf(CONOR)
+    order = np.argsort(a.material_ids, kind="stable")
+    np.testing.assert_allclose(b.positions, a.positions[order], atol=1e-6)
+    np.testing.assert_allclose(b.normals, a.normals[order], atol=1e-6)
+    np.testing.assert_array_equal(b.material_ids, a.material_ids[order])
+    assert [bytes(m) for m in a.materials] == [bytes(m) for m in b.materials]

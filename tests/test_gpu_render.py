@@ -379,3 +379,26 @@ def test_gltf_spawn_matches_oracle(gpu_available, oracle, tmp_path):
     print(f"[render] glTF demo scene spawned: mean {spawned.mean():.4f}, relMSE vs oracle {r:.3e}", flush=True)
     assert spawned.mean() > 0.01
     assert r < 1e-3
+
+
+def test_cornell_conor_spawn_matches_oracle(gpu_available, oracle):
+    """The reference render host's Cornell box (assets/CornellBoxConor.glb,
+    apps/render-host/src/lib.rs:138), re-encoded without its texture as
+    tests/golden/cornell_conor.glb, spawned through SpawnModel and rendered
+    against the FP32 oracle on the same triangles (relMSE <= 1e-3)."""
+    from pathlib import Path
+    from appearance_amd._ffi import Sky
+    from appearance_amd.scene import load_gltf
+    from tests.scenes_util import camera
+    p = Path(__file__).resolve().parent / "golden" / "cornell_conor.glb"
+    s = load_gltf(p)
+    cam = camera((0.0, 1.0, 3.6), (0.0, 1.0, 0.0), 45.0)
+    pt = PathTracerGpu(plain_config(sample_count=32, max_bounces=8))
+    pt.handle_visible_world_action(world.SpawnModel(str(p)))
+    pt.set_camera(cam)
+    _, got = pt.render_accum((128, 128), with_rgba=False)
+    ref, _ = oracle_render(oracle, s, 128, 128, spp=32, max_bounces=8, rr_start=3, camera=cam, sky=Sky())
+    r = relmse(got, ref)
+    print(f"[render] CornellBoxConor spawned: mean {got.mean():.4f}, relMSE vs oracle {r:.3e}", flush=True)
+    assert got.mean() > 0.05
+    assert r < 1e-3
