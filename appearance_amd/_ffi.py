@@ -44,6 +44,12 @@ class BvhStats(C.Structure):
                 ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("sah_cost", C.c_double)]
 
 
+class TlasStats(C.Structure):
+    _fields_ = [("n_blas", C.c_uint32), ("n_instances", C.c_uint32), ("n_tlas_nodes", C.c_uint32),
+                ("width", C.c_uint32), ("max_stack", C.c_uint32), ("_pad", C.c_uint32),
+                ("blas_tris", C.c_uint64), ("device_bytes", C.c_uint64), ("ms_last_update", C.c_double)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_uint32),
                 ("wavefront_size", C.c_uint32), ("hbm_bytes", C.c_uint64),
@@ -128,6 +134,15 @@ _SIGS = {
     "apt_trace_any": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "apt_trace_closest_host": (C.c_int, [_P, _P, C.c_uint32, _P]),
     "apt_trace_any_host": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "apt_tlas_create": (C.c_int, [C.POINTER(BvhOpts), C.POINTER(_P)]),
+    "apt_tlas_add_blas": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    "apt_tlas_set_instances": (C.c_int, [_P, _P, C.c_uint32]),
+    "apt_tlas_destroy": (None, [_P]),
+    "apt_tlas_get_stats": (C.c_int, [_P, C.POINTER(TlasStats)]),
+    "apt_trace_closest_tlas": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "apt_trace_any_tlas": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "apt_trace_closest_tlas_host": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "apt_trace_any_tlas_host": (C.c_int, [_P, _P, C.c_uint32, _P]),
     "apt_material_default": (None, [C.POINTER(Material)]),
     "apt_scene_generate": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     "apt_scene_data_view": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(Camera), C.POINTER(Sky)]),

@@ -29,6 +29,7 @@ SOURCES = [
     "restir.hip",
     "bvh_gpu.hip",
     "gltf.cpp",
+    "tlas.hip",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "apt_internal.h"
+#include "tlas.h"
 
 namespace apt {
 
@@ -216,10 +217,25 @@ apt_status apt_trace_any(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n,
   return launch_trace_any(bvh, d_rays, n, d_occluded, s);
 }
 
-static apt_status trace_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n, void* out,
-                             bool any) {
+apt_status apt_trace_closest_tlas(const apt_tlas* t, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
+                                  void* stream) {
   clear_error();
-  APT_REQUIRE(bvh && rays && out, APT_ERR_INVALID_ARG, "trace_host: null argument");
+  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  return launch_trace_tlas(t, d_rays, n, d_hits, nullptr, false, s);
+}
+
+apt_status apt_trace_any_tlas(const apt_tlas* t, const apt_ray* d_rays, uint32_t n, uint32_t* d_occluded,
+                              void* stream) {
+  clear_error();
+  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  return launch_trace_tlas(t, d_rays, n, nullptr, d_occluded, true, s);
+}
+
+// One BVH or one two-level scene (the other NULL), host arrays in and out.
+static apt_status trace_host(const apt_bvh* bvh, const apt_tlas* tlas, const apt_ray* rays, uint32_t n,
+                             void* out, bool any) {
+  clear_error();
+  APT_REQUIRE((bvh || tlas) && rays && out, APT_ERR_INVALID_ARG, "trace_host: null argument");
   if (n == 0) return APT_OK;
   const size_t out_bytes = (size_t)n * (any ? sizeof(uint32_t) : sizeof(apt_hit));
   void* d_rays = nullptr;
@@ -235,8 +251,12 @@ static apt_status trace_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n
   apt_status st = APT_OK;
   e = hipMemcpyAsync(d_rays, rays, (size_t)n * sizeof(apt_ray), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
-    st = any ? launch_trace_any(bvh, (const apt_ray*)d_rays, n, (uint32_t*)d_out, s)
-             : launch_trace_closest(bvh, (const apt_ray*)d_rays, n, (apt_hit*)d_out, s);
+    if (tlas)
+      st = launch_trace_tlas(tlas, (const apt_ray*)d_rays, n, any ? nullptr : (apt_hit*)d_out,
+                             any ? (uint32_t*)d_out : nullptr, any, s);
+    else
+      st = any ? launch_trace_any(bvh, (const apt_ray*)d_rays, n, (uint32_t*)d_out, s)
+               : launch_trace_closest(bvh, (const apt_ray*)d_rays, n, (apt_hit*)d_out, s);
     if (st == APT_OK) e = hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, s);
     if (st == APT_OK && e == hipSuccess) e = hipStreamSynchronize(s);
   }
@@ -252,12 +272,22 @@ static apt_status trace_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n
 
 apt_status apt_trace_closest_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
                                   apt_hit* hits) {
-  return trace_host(bvh, rays, n, hits, false);
+  return trace_host(bvh, nullptr, rays, n, hits, false);
 }
 
 apt_status apt_trace_any_host(const apt_bvh* bvh, const apt_ray* rays, uint32_t n,
                               uint32_t* occluded) {
-  return trace_host(bvh, rays, n, occluded, true);
+  return trace_host(bvh, nullptr, rays, n, occluded, true);
+}
+
+apt_status apt_trace_closest_tlas_host(const apt_tlas* t, const apt_ray* rays, uint32_t n, apt_hit* hits) {
+  APT_REQUIRE(t, APT_ERR_INVALID_ARG, "apt_trace_closest_tlas_host: null scene");
+  return trace_host(nullptr, t, rays, n, hits, false);
+}
+
+apt_status apt_trace_any_tlas_host(const apt_tlas* t, const apt_ray* rays, uint32_t n, uint32_t* occluded) {
+  APT_REQUIRE(t, APT_ERR_INVALID_ARG, "apt_trace_any_tlas_host: null scene");
+  return trace_host(nullptr, t, rays, n, occluded, true);
 }
 
 }  // extern "C"

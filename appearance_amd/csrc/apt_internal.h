@@ -91,6 +91,52 @@ APT_HD float qstep(uint32_t e) {
   return c.f;
 }
 
+// ---------------------------------------------------------------------------
+// Two-level scenes.
+//
+// Node arena: [TLAS, TLAS_NODE_CAP slots][BLAS 0][BLAS 1]...  Every BLAS is
+// relocated into it (internal refs + its node base, leaf first + its
+// triangle base), so all BLASes share one node array and one triangle array
+// and the TLAS's top levels sit at the front, where the pool kernels stage
+// them in LDS.  The TLAS is a wide BVH over instance boxes whose leaves hold
+// one instance each: leaf first = the index of the instance record.
+//
+// Instance record, 64 B, in TLAS leaf order: rows of the world-to-object
+// 3x4 matrix, then the BLAS root ref (relocated), the prim base added to
+// every BLAS-local prim id, and flags.  A ray enters the instance as
+// o' = ((m0 x + m1 y) + m2 z) + m3 per row (d' without m3), no fused
+// multiply-adds, or unchanged with INST_IDENTITY.
+// ---------------------------------------------------------------------------
+struct alignas(16) InstRec {
+  float w2o[12];
+  uint32_t blas_root;
+  uint32_t prim_base;
+  uint32_t flags;
+  uint32_t pad;
+};
+static_assert(sizeof(InstRec) == 64, "InstRec size");
+constexpr uint32_t INST_IDENTITY = 1u;
+constexpr uint32_t TLAS_MAX_INSTANCES = APT_TLAS_MAX_INSTANCES;  // MAX_TLAS_INSTANCES, scene_resources/mod.rs:20
+// A wide tree over n one-instance leaves has at most n - 1 internal nodes.
+constexpr uint32_t TLAS_NODE_CAP = TLAS_MAX_INSTANCES;
+
+// A point and a direction mapped by a 3x4 matrix held as three float4 rows
+// (the evaluation order of an instance record).
+APT_HD v3 xform_point(const float4& r0, const float4& r1, const float4& r2, v3 p) {
+  return mk(((r0.x * p.x + r0.y * p.y) + r0.z * p.z) + r0.w, ((r1.x * p.x + r1.y * p.y) + r1.z * p.z) + r1.w,
+            ((r2.x * p.x + r2.y * p.y) + r2.z * p.z) + r2.w);
+}
+APT_HD v3 xform_dir(const float4& r0, const float4& r1, const float4& r2, v3 d) {
+  return mk((r0.x * d.x + r0.y * d.y) + r0.z * d.z, (r1.x * d.x + r1.y * d.y) + r1.z * d.z,
+            (r2.x * d.x + r2.y * d.y) + r2.z * d.z);
+}
+
+// Inverse of an object-to-world affine 3x4 (row major) in double, rounded to
+// float: the world-to-object rows of an instance record.  False when the 3x3
+// part is singular.  *cond receives ||A||_F ||A^-1||_F (>= the 2-norm
+// condition number), which sizes the instance box padding.
+bool affine_inverse(const float* o2w, float* w2o, double* cond);
+
 // Host-side binary BVH produced by the builders before collapse.
 struct Node2 {
   float lo[3], hi[3];
@@ -109,6 +155,13 @@ apt_status build_bvh2_host(const float* tri_xyz, uint32_t n, uint32_t max_leaf, 
                            BvhHostResult& out);
 
 double bvh2_sah_cost(const std::vector<Node2>& nodes);
+
+// Collapse of a binary tree into the wide layout on the host only: the wide
+// nodes (breadth first, wide_node_bytes(width) bytes each), the root ref and
+// the statistics (n_nodes, n_leaves, depth, max_stack).  Child boxes are
+// padded by `pad`.
+apt_status collapse_wide_host(const BvhHostResult& b2, uint32_t width, float pad, std::vector<uint8_t>& nodes,
+                              uint32_t& root, apt_bvh_stats& stats);
 // Scene-scale padding used for every node box (see box_pad in bvh_host.cpp).
 float scene_pad(const float* tri_xyz, uint32_t n);
 

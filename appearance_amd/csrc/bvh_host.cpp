@@ -323,11 +323,17 @@ void quantize_node(WNode<W>& w, int nk, const float (&clo)[W][3], const float (&
   }
 }
 
+// The wide tree of a binary one (breadth first), its root ref and statistics.
 template <int W>
-apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad, apt_bvh* out,
-                           bool upload) {
-  const std::vector<Node2>& bn = b2.nodes;
+struct WideTree {
   std::vector<WNode<W>> wide;
+  uint32_t root = 0, max_depth = 1, max_stack = 1, n_leaves = 0;
+};
+
+template <int W>
+void collapse_tree(const BvhHostResult& b2, float pad, WideTree<W>& out) {
+  const std::vector<Node2>& bn = b2.nodes;
+  std::vector<WNode<W>>& wide = out.wide;
   std::vector<uint32_t> wkids;  // the W BVH2 nodes under each wide node (~0u: empty slot)
   uint32_t max_depth = 1, max_stack = 1, n_leaves = 0;
 
@@ -442,6 +448,19 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
       quantize_node<W>(wide[i], nk, clo, chi, nlo, nhi);
     }
   });
+  out.root = root;
+  out.max_depth = max_depth;
+  out.max_stack = max_stack;
+  out.n_leaves = n_leaves;
+}
+
+template <int W>
+apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, float pad, apt_bvh* out,
+                           bool upload) {
+  WideTree<W> wt;
+  collapse_tree<W>(b2, pad, wt);
+  const std::vector<WNode<W>>& wide = wt.wide;
+  const uint32_t root = wt.root, max_depth = wt.max_depth, max_stack = wt.max_stack, n_leaves = wt.n_leaves;
 
   // Triangles in leaf order: v0 | prim, v1, v2 as float4.
   std::vector<float4> tris((size_t)n * 3);
@@ -493,7 +512,34 @@ apt_status collapse_upload(const float* tri_xyz, uint32_t n, const BvhHostResult
   return APT_OK;
 }
 
+template <int W>
+void collapse_host_bytes(const BvhHostResult& b2, float pad, std::vector<uint8_t>& nodes, uint32_t& root,
+                         apt_bvh_stats& stats) {
+  WideTree<W> wt;
+  collapse_tree<W>(b2, pad, wt);
+  nodes.resize(wt.wide.size() * sizeof(WNode<W>));
+  if (!wt.wide.empty()) std::memcpy(nodes.data(), wt.wide.data(), nodes.size());
+  root = wt.root;
+  stats.n_nodes = (uint32_t)wt.wide.size();
+  stats.n_leaves = wt.n_leaves;
+  stats.width = W;
+  stats.depth = wt.wide.empty() ? 0 : wt.max_depth;
+  stats.max_stack = wt.max_stack;
+}
+
 }  // namespace
+
+apt_status collapse_wide_host(const BvhHostResult& b2, uint32_t width, float pad, std::vector<uint8_t>& nodes,
+                              uint32_t& root, apt_bvh_stats& stats) {
+  switch (width) {
+    case 2: collapse_host_bytes<2>(b2, pad, nodes, root, stats); return APT_OK;
+    case 4: collapse_host_bytes<4>(b2, pad, nodes, root, stats); return APT_OK;
+    case 8: collapse_host_bytes<8>(b2, pad, nodes, root, stats); return APT_OK;
+    default:
+      set_error("BVH width %u is not one of 2, 4, 8", width);
+      return APT_ERR_INVALID_ARG;
+  }
+}
 
 apt_status upload_wide_bvh(const float* tri_xyz, uint32_t n, const BvhHostResult& b2, uint32_t width, float pad,
                            apt_bvh* out, bool upload) {

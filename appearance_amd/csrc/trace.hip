@@ -22,6 +22,7 @@
 #include <mutex>
 
 #include "apt_internal.h"
+#include "tlas.h"
 #include "trace_device.h"
 
 namespace apt {
@@ -30,12 +31,13 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-template <int W, int S, bool ANY>
+template <int W, int S, bool ANY, bool INST = false>
 __global__ __launch_bounds__(BLOCK) void k_trace(const WNode<W>* __restrict__ nodes,
                                                  const float4* __restrict__ tris, uint32_t root,
                                                  const apt_ray* __restrict__ rays, uint32_t n,
                                                  apt_hit* __restrict__ hits,
-                                                 uint32_t* __restrict__ occluded) {
+                                                 uint32_t* __restrict__ occluded,
+                                                 const InstRec* __restrict__ insts = nullptr) {
   __shared__ uint2 s_stack[S * BLOCK];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t i = blk * BLOCK + threadIdx.x;
@@ -46,9 +48,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const WNode<W>* __restrict__ no
   TraceRay ray = make_trace_ray(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w);
   LdsStack st{s_stack + threadIdx.x, BLOCK};
   if (ANY) {
-    occluded[i] = trace_any<W, S>(nodes, tris, root, ray, st) ? 1u : 0u;
+    occluded[i] = trace_any<W, S, INST>(nodes, tris, root, ray, st, insts) ? 1u : 0u;
   } else {
-    HitRec h = trace_closest<W, S>(nodes, tris, root, ray, st);
+    HitRec h = trace_closest<W, S, INST>(nodes, tris, root, ray, st, insts);
     float4 out = make_float4(h.prim == INVALID ? INFINITY : h.t, h.u, h.v, u2f(h.prim));
     reinterpret_cast<float4*>(hits)[i] = out;
   }
@@ -81,30 +83,44 @@ struct RaySrc {
   __device__ __forceinline__ void any(uint32_t i, bool hit) const { occluded[i] = hit ? 1u : 0u; }
 };
 
-template <int W, int S, bool ANY>
-__global__ __launch_bounds__(BLOCK, 8) void k_trace_pool(const WNode<W>* __restrict__ nodes,
+// INST: a two-level scene (nodes/tris its arenas, root its TLAS root,
+// n_nodes its TLAS node count for the LDS copy).
+template <int W, int S, bool ANY, bool INST = false>
+__global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_trace_pool(const WNode<W>* __restrict__ nodes,
                                                          const float4* __restrict__ tris, uint32_t root,
                                                          const apt_ray* __restrict__ rays, uint32_t n,
                                                          apt_hit* __restrict__ hits,
                                                          uint32_t* __restrict__ occluded,
                                                          uint32_t* __restrict__ counter, uint32_t refill,
-                                                         uint32_t n_nodes) {
+                                                         uint32_t n_nodes,
+                                                         const InstRec* __restrict__ insts = nullptr) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   __shared__ float4 s_top[TOP_BYTES / 16];
   const uint32_t ktop = stage_top<W>(nodes, n_nodes, s_top);
   RaySrc<W> src{nodes, tris, root, rays, hits, occluded};
-  trace_pool<W, S, ANY>(src, n, counter, refill, s_stack + threadIdx.x, BLOCK, s_top, ktop);
+  trace_pool<W, S, ANY, RaySrc<W>, INST>(src, n, counter, refill, s_stack + threadIdx.x, BLOCK, s_top, ktop,
+                                         insts);
+}
+
+// A closest-hit miss or an unoccluded ray for every entry (a two-level scene
+// without instances).
+__global__ void k_fill_miss(uint32_t n, apt_hit* __restrict__ hits, uint32_t* __restrict__ occluded) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (hits) reinterpret_cast<float4*>(hits)[i] = make_float4(INFINITY, 0.0f, 0.0f, u2f(INVALID));
+  if (occluded) occluded[i] = 0u;
 }
 
 // Blocks of k_trace_pool that fit on the device at once.
-template <int W, int S, bool ANY>
+template <int W, int S, bool ANY, bool INST = false>
 uint32_t pool_grid() {
   static uint32_t g = 0;
   if (!g) {
     int per_cu = 0, dev = 0;
     hipDeviceProp_t p;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_pool<W, S, ANY>, BLOCK, 0) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_pool<W, S, ANY, INST>, BLOCK, 0) ==
+            hipSuccess &&
         per_cu > 0)
       g = (uint32_t)per_cu * (uint32_t)p.multiProcessorCount;
     else
@@ -168,7 +184,64 @@ apt_status dispatch(const apt_bvh* bvh, const apt_ray* d_rays, uint32_t n, apt_h
   }
 }
 
+template <int W>
+apt_status launch_tlas(const apt_tlas* t, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits, uint32_t* d_occ,
+                       bool any, hipStream_t s) {
+  const WNode<W>* nodes = static_cast<const WNode<W>*>(t->d_nodes);
+  const InstRec* insts = t->d_inst;
+  if (t->refill > 64) {
+    dim3 grid((n + BLOCK - 1) / BLOCK);
+    APT_FOR_STACK_DEPTH(t->lds_stack, {
+      if (any)
+        hipLaunchKernelGGL((k_trace<W, S_, true, true>), grid, dim3(BLOCK), 0, s, nodes, t->d_tris, t->root,
+                           d_rays, n, d_hits, d_occ, insts);
+      else
+        hipLaunchKernelGGL((k_trace<W, S_, false, true>), grid, dim3(BLOCK), 0, s, nodes, t->d_tris, t->root,
+                           d_rays, n, d_hits, d_occ, insts);
+    })
+  } else {
+    const uint32_t slot = __atomic_fetch_add(&t->counter_next, 1u, __ATOMIC_RELAXED) % APT_BVH_COUNTERS;
+    uint32_t* counter = t->d_counters + (size_t)slot * POOL_WORDS;
+    hipLaunchKernelGGL(k_zero_pool, dim3(1), dim3(POOL_PARTS), 0, s, counter);
+    APT_FOR_STACK_DEPTH(t->lds_stack, {
+      if (any)
+        hipLaunchKernelGGL((k_trace_pool<W, S_, true, true>),
+                           dim3(std::min(pool_grid<W, S_, true, true>(), (n + 63) / 64)), dim3(BLOCK), 0, s, nodes,
+                           t->d_tris, t->root, d_rays, n, d_hits, d_occ, counter, t->refill, t->n_tlas_nodes,
+                           insts);
+      else
+        hipLaunchKernelGGL((k_trace_pool<W, S_, false, true>),
+                           dim3(std::min(pool_grid<W, S_, false, true>(), (n + 63) / 64)), dim3(BLOCK), 0, s,
+                           nodes, t->d_tris, t->root, d_rays, n, d_hits, d_occ, counter, t->refill,
+                           t->n_tlas_nodes, insts);
+    })
+  }
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
 }  // namespace
+
+apt_status launch_trace_tlas(const apt_tlas* t, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
+                             uint32_t* d_occ, bool any, hipStream_t s) {
+  APT_REQUIRE(t && d_rays && (any ? d_occ != nullptr : d_hits != nullptr), APT_ERR_INVALID_ARG,
+              "trace (two-level): null argument");
+  if (n == 0) return APT_OK;
+  if (t->n_inst == 0) {
+    hipLaunchKernelGGL(k_fill_miss, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n, any ? nullptr : d_hits,
+                       any ? d_occ : nullptr);
+    APT_HIP(hipGetLastError());
+    return APT_OK;
+  }
+  APT_REQUIRE(t->max_stack <= trace_stack_capacity(), APT_ERR_UNSUPPORTED,
+              "trace (two-level): the scene needs a %u-entry stack", t->max_stack);
+  switch (t->width) {
+    case 2: return launch_tlas<2>(t, d_rays, n, d_hits, d_occ, any, s);
+    case 4: return launch_tlas<4>(t, d_rays, n, d_hits, d_occ, any, s);
+    case 8: return launch_tlas<8>(t, d_rays, n, d_hits, d_occ, any, s);
+    default: set_error("trace (two-level): unsupported width %u", t->width); return APT_ERR_INVALID_ARG;
+  }
+}
 
 uint32_t trace_stack_capacity() { return 8u + (uint32_t)TRACE_OVF_STACK; }
 

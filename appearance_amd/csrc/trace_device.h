@@ -26,14 +26,35 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t G) {
 struct TraceRay {
   RayPre pre;
   v3 inv;
+  v3 d;  // the direction as given (read only by the two-level traversal)
 };
 
 __device__ __forceinline__ TraceRay make_trace_ray(v3 o, v3 d, float tmin, float tmax) {
   TraceRay r;
   r.pre = ray_pre(o, d, tmin, tmax);
   r.inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+  r.d = d;
   return r;
 }
+
+// Per-lane state of the two-level traversal: the world-space ray, the prim
+// base of the instance being traversed and the stack depth at which it was
+// entered (-1 in the top level).  When the stack falls back to that depth the
+// instance is finished and the world ray is restored.
+struct InstLane {
+  v3 wo, wd;
+  uint32_t prim_base;
+  int sp_inst;
+};
+
+// Two-level traversal widens every box's entry and exit distances by 2^-18 of
+// their magnitude.  An object-space origin can lie far from a small BLAS (a
+// small instance seen from afar), where the slab test's rounding, about
+// 2^-22 of the distance, outgrows the BLAS's scene-relative padding; and a
+// hit found in object space maps back onto the world ray only up to the
+// rounding of the instance transform.  A relative margin covers both at any
+// distance.
+constexpr float INST_WIDEN = 3.814697265625e-06f;  // 2^-18
 
 struct HitRec {
   float t, u, v;
@@ -53,7 +74,7 @@ struct LdsStack {
 // (origin - o) * inv and of the FMA is about 2^-23 of the node's distance
 // from the origin, far below the boxes' padding of 2^-14 max|coord| for
 // origins within ~100x the scene extent, so the test stays conservative.
-template <int W>
+template <int W, bool ROBUST = false>
 __device__ __forceinline__ void child_dists(const WNode<W>& nd, const TraceRay& ray, float tcap,
                                             float (&dist)[W]) {
   const v3 inv = ray.inv, o = ray.pre.o;
@@ -81,8 +102,15 @@ __device__ __forceinline__ void child_dists(const WNode<W>& nd, const TraceRay& 
       const float tfy = __builtin_fmaf((float)((fy >> sh) & 0xFFu), ay, by);
       const float tnz = __builtin_fmaf((float)((nz >> sh) & 0xFFu), az, bz);
       const float tfz = __builtin_fmaf((float)((fz >> sh) & 0xFFu), az, bz);
-      const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, ray.pre.tmin));
-      const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tcap));
+      float tn, tf;
+      if (ROBUST) {
+        const float bn = fmaxf(fmaxf(tnx, tny), tnz), bf = fminf(fminf(tfx, tfy), tfz);
+        tn = fmaxf(__builtin_fmaf(-fabsf(bn), INST_WIDEN, bn), ray.pre.tmin);
+        tf = fminf(__builtin_fmaf(fabsf(bf), INST_WIDEN, bf), tcap);
+      } else {
+        tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, ray.pre.tmin));
+        tf = fminf(fminf(tfx, tfy), fminf(tfz, tcap));
+      }
       dist[k] = (tn <= tf && nd.child[k] != REF_EMPTY) ? tn : INFINITY;
     }
   } else {
@@ -91,8 +119,16 @@ __device__ __forceinline__ void child_dists(const WNode<W>& nd, const TraceRay& 
       float t0x = __builtin_fmaf((float)nd.q[QLO_X][k], ax, bx), t1x = __builtin_fmaf((float)nd.q[QHI_X][k], ax, bx);
       float t0y = __builtin_fmaf((float)nd.q[QLO_Y][k], ay, by), t1y = __builtin_fmaf((float)nd.q[QHI_Y][k], ay, by);
       float t0z = __builtin_fmaf((float)nd.q[QLO_Z][k], az, bz), t1z = __builtin_fmaf((float)nd.q[QHI_Z][k], az, bz);
-      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), ray.pre.tmin));
-      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcap));
+      float tn, tf;
+      if (ROBUST) {
+        const float bn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+        const float bf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+        tn = fmaxf(__builtin_fmaf(-fabsf(bn), INST_WIDEN, bn), ray.pre.tmin);
+        tf = fminf(__builtin_fmaf(fabsf(bf), INST_WIDEN, bf), tcap);
+      } else {
+        tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), ray.pre.tmin));
+        tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcap));
+      }
       dist[k] = (tn <= tf && nd.child[k] != REF_EMPTY) ? tn : INFINITY;
     }
   }
@@ -160,16 +196,20 @@ __device__ __forceinline__ uint2 stack_pop(int& sp, const uint2* lds, int stride
 // padded box starts no later than the best t so far, so equal-t candidates are
 // still visited and the tie goes to the lower prim.  One ray per lane, in the
 // single-step schedule (step_fetch / step_item, defined below).
-template <int W, int S>
+// With INST the BVH is a two-level scene: nodes and tris are its arenas,
+// root the TLAS root and insts its instance records.
+template <int W, int S, bool INST = false>
 __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
                                                 const float4* __restrict__ tris, uint32_t root,
-                                                const TraceRay& ray, LdsStack st);
+                                                const TraceRay& ray, LdsStack st,
+                                                const InstRec* __restrict__ insts = nullptr);
 // Any hit: true as soon as one triangle passes with tmin < t < tmax
 // (the reference's terminate-on-first-hit shadow query, helpers/trace.wgsl:20-26).
-template <int W, int S>
+template <int W, int S, bool INST = false>
 __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
                                           const float4* __restrict__ tris, uint32_t root,
-                                          const TraceRay& ray, LdsStack st);
+                                          const TraceRay& ray, LdsStack st,
+                                          const InstRec* __restrict__ insts = nullptr);
 
 // Host-side dispatch over the supported LDS stack depths.
 #define APT_FOR_STACK_DEPTH(S, ...) \
@@ -297,13 +337,23 @@ constexpr int pool_lds_entries(int S) { return S + 1; }
 // leaf is consumed).
 constexpr int STEP_NQ = 6;  // float4 per fetch: a node or two triangles
 
-template <int W>
+// With INST (a two-level scene) a leaf ref met in the top level (in_top) is an
+// instance: its 64-byte record is the item fetched.
+template <int W, bool INST = false>
 __device__ __forceinline__ void step_fetch(const WNode<W>* __restrict__ nodes, const float4* __restrict__ tris,
                                            const float4* top, uint32_t ktop, uint32_t cur,
-                                           float4 (&q)[STEP_NQ]) {
+                                           float4 (&q)[STEP_NQ], const InstRec* __restrict__ insts = nullptr,
+                                           bool in_top = false) {
   constexpr int N4 = (int)(sizeof(WNode<W>) / 16);  // float4 per node: 3, 4, 6
   static_assert(N4 <= STEP_NQ, "a node must fit the fetch registers");
   const bool in_leaf = ref_is_leaf(cur);
+  if (INST && in_leaf && in_top) {
+    const float4* p = reinterpret_cast<const float4*>(insts + ref_leaf_first(cur));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = p[k];
+    asm volatile("" ::: "memory");
+    return;
+  }
   const bool two = in_leaf && ref_leaf_count(cur) >= 2u;
   if (!in_leaf && cur < ktop) {
 #pragma unroll
@@ -326,13 +376,30 @@ __device__ __forceinline__ void step_fetch(const WNode<W>* __restrict__ nodes, c
 // Processes the item step_fetch loaded for `cur` and moves `cur` on; returns
 // true when the ray is finished (stack empty, or an any-hit occluder found).
 // The (u, v) of an improved closest hit go to *uv when UV_LDS, else to hu, hv.
-template <int W, int S, bool ANY, bool UV_LDS>
-__device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], const TraceRay& ray, uint32_t& cur,
+// With INST: an instance record maps the ray into the instance's object space
+// and continues at its BLAS root; triangle prims are offset by the instance's
+// prim base; when the stack falls back to the depth at which the instance was
+// entered, the world ray is restored.
+template <int W, int S, bool ANY, bool UV_LDS, bool INST = false>
+__device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], TraceRay& ray, uint32_t& cur,
                                           float& best_t, uint32_t& best_prim, float& hu, float& hv, uint2* uv,
                                           bool& occluded, uint2* const lds, const int stride,
-                                          uint2 (&ovf)[TRACE_OVF_STACK], int& sp) {
+                                          uint2 (&ovf)[TRACE_OVF_STACK], int& sp, InstLane& il) {
   constexpr int N4 = (int)(sizeof(WNode<W>) / 16);
   const bool in_leaf = ref_is_leaf(cur);
+  if (INST && in_leaf && il.sp_inst < 0) {
+    const uint32_t flags = f2u(q[3].z);
+    v3 o = il.wo, d = il.wd;
+    if (!(flags & INST_IDENTITY)) {
+      o = xform_point(q[0], q[1], q[2], o);
+      d = xform_dir(q[0], q[1], q[2], d);
+    }
+    ray = make_trace_ray(o, d, ray.pre.tmin, ray.pre.tmax);
+    il.prim_base = f2u(q[3].y);
+    il.sp_inst = sp;
+    cur = f2u(q[3].x);
+    return false;
+  }
   const uint32_t left = ref_leaf_count(cur);
   const bool two = in_leaf && left >= 2u;
   uint32_t next = REF_EMPTY;  // ref to enter after this item; REF_EMPTY: pop
@@ -341,7 +408,7 @@ __device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], const Trac
     WNode<W> nd;
     __builtin_memcpy(&nd, q, sizeof(float4) * N4);
     float dist[W];
-    child_dists<W>(nd, ray, ANY ? ray.pre.tmax : best_t, dist);
+    child_dists<W, INST>(nd, ray, ANY ? ray.pre.tmax : best_t, dist);
     if (ANY) {
 #pragma unroll
       for (int k = 0; k < W; ++k) {
@@ -371,7 +438,7 @@ __device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], const Trac
         if (ANY) {
           occluded = true;
         } else {
-          const uint32_t prim = f2u(a.w);
+          const uint32_t prim = INST ? il.prim_base + f2u(a.w) : f2u(a.w);
           if (hit_less(h.t, prim, best_t, best_prim)) {
             best_t = h.t;
             best_prim = prim;
@@ -391,7 +458,13 @@ __device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], const Trac
   if (occluded) return true;
   if (stay) return false;
   if (next == REF_EMPTY) {
-    while (sp > 0) {
+    while (true) {
+      if (INST && sp == il.sp_inst) {
+        // the instance is done: back to the world ray and the top level
+        ray = make_trace_ray(il.wo, il.wd, ray.pre.tmin, ray.pre.tmax);
+        il.sp_inst = -1;
+      }
+      if (sp <= 0) break;
       const uint2 e = APT_STACK_POP();
       if (ANY || u2f(e.y) <= best_t) {
         next = e.x;
@@ -420,10 +493,13 @@ __device__ __forceinline__ bool step_item(const float4 (&q)[STEP_NQ], const Trac
 // (rays of different scenes in one launch); and `void closest(uint32_t i,
 // const HitRec&)` or `void any(uint32_t i, bool hit)`.  Nodes below `ktop`
 // are read from the LDS copy `top` (stage_top; ktop = 0 with several scenes).
-template <int W, int S, bool ANY, class Src>
+// With INST (one two-level scene, kOneScene) nodes/tris are the scene's
+// arenas, root its TLAS root and insts its instance records.
+template <int W, int S, bool ANY, class Src, bool INST = false>
 __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* counter, uint32_t refill,
                                            uint2* const lds, const int stride, const float4* top,
-                                           uint32_t ktop) {
+                                           uint32_t ktop, const InstRec* __restrict__ insts = nullptr) {
+  static_assert(!INST || Src::kOneScene, "a two-level launch traces one scene");
   // The first ceil(n / 64) waves take every ray on their first visits (64
   // each); the rest of the grid would only add to the queue on the range
   // counters and find them drained, which over a few thousand rays costs
@@ -445,6 +521,7 @@ __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* count
     root = src.root;
   }
   TraceRay ray;
+  InstLane il{};
   float best_t = 0.0f;
   uint32_t best_prim = INVALID;
   uint2* const uv = lds + S * stride;  // (u, v) of the best hit
@@ -459,15 +536,20 @@ __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* count
       sp = 0;
       cur = root;
       active = true;
+      if (INST) {
+        il.wo = ray.pre.o;
+        il.wd = ray.d;
+        il.sp_inst = -1;
+      }
     }
     if (__ballot(active) == 0ull) break;
     if (!active) continue;
     float4 q[STEP_NQ];
-    step_fetch<W>(nodes, tris, top, ktop, cur, q);
+    step_fetch<W, INST>(nodes, tris, top, ktop, cur, q, insts, il.sp_inst < 0);
     bool occluded = false;
     float hu, hv;
-    const bool done = step_item<W, S, ANY, true>(q, ray, cur, best_t, best_prim, hu, hv, uv, occluded, lds, stride,
-                                                 ovf, sp);
+    const bool done = step_item<W, S, ANY, true, INST>(q, ray, cur, best_t, best_prim, hu, hv, uv, occluded, lds,
+                                                       stride, ovf, sp, il);
     if (done) {
       if constexpr (ANY) {
         src.any(rid, occluded);
@@ -481,45 +563,60 @@ __device__ __forceinline__ void trace_pool(Src& src, uint32_t n, uint32_t* count
   }
 }
 
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ HitRec trace_closest(const WNode<W>* __restrict__ nodes,
                                                 const float4* __restrict__ tris, uint32_t root,
-                                                const TraceRay& ray, LdsStack st) {
+                                                const TraceRay& ray0, LdsStack st,
+                                                const InstRec* __restrict__ insts) {
   uint2* const lds = st.base;
   const int stride = st.stride;
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
   uint32_t cur = root;
+  TraceRay ray = ray0;
+  InstLane il{};
+  if (INST) {
+    il.wo = ray0.pre.o;
+    il.wd = ray0.d;
+    il.sp_inst = -1;
+  }
   float best_t = ray.pre.tmax, hu = 0.0f, hv = 0.0f;
   uint32_t best_prim = INVALID;
   bool occluded = false;
   while (true) {
     float4 q[STEP_NQ];
-    step_fetch<W>(nodes, tris, nullptr, 0u, cur, q);
-    if (step_item<W, S, false, false>(q, ray, cur, best_t, best_prim, hu, hv, nullptr, occluded, lds, stride,
-                                      ovf, sp))
+    step_fetch<W, INST>(nodes, tris, nullptr, 0u, cur, q, insts, il.sp_inst < 0);
+    if (step_item<W, S, false, false, INST>(q, ray, cur, best_t, best_prim, hu, hv, nullptr, occluded, lds,
+                                            stride, ovf, sp, il))
       break;
   }
   return HitRec{best_t, hu, hv, best_prim};
 }
 
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
                                           const float4* __restrict__ tris, uint32_t root,
-                                          const TraceRay& ray, LdsStack st) {
+                                          const TraceRay& ray0, LdsStack st, const InstRec* __restrict__ insts) {
   uint2* const lds = st.base;
   const int stride = st.stride;
   uint2 ovf[TRACE_OVF_STACK];
   int sp = 0;
   uint32_t cur = root;
+  TraceRay ray = ray0;
+  InstLane il{};
+  if (INST) {
+    il.wo = ray0.pre.o;
+    il.wd = ray0.d;
+    il.sp_inst = -1;
+  }
   float best_t = ray.pre.tmax, hu, hv;
   uint32_t best_prim = INVALID;
   bool occluded = false;
   while (true) {
     float4 q[STEP_NQ];
-    step_fetch<W>(nodes, tris, nullptr, 0u, cur, q);
-    if (step_item<W, S, true, false>(q, ray, cur, best_t, best_prim, hu, hv, nullptr, occluded, lds, stride,
-                                     ovf, sp))
+    step_fetch<W, INST>(nodes, tris, nullptr, 0u, cur, q, insts, il.sp_inst < 0);
+    if (step_item<W, S, true, false, INST>(q, ray, cur, best_t, best_prim, hu, hv, nullptr, occluded, lds, stride,
+                                           ovf, sp, il))
       break;
   }
   return occluded;

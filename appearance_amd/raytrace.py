@@ -12,10 +12,15 @@ import ctypes as C
 import numpy as np
 
 from . import _ffi
-from ._ffi import BvhOpts, BvhStats, check, lib, vptr
+from ._ffi import BvhOpts, BvhStats, TlasStats, check, lib, vptr
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4")])
+# apt_instance (64 B): object-to-world 3x4 row major, BLAS index, custom index
+INSTANCE_DTYPE = np.dtype([("transform", "<f4", (3, 4)), ("blas", "<u4"), ("custom_index", "<u4"),
+                           ("_pad", "<u4", 2)])
+assert INSTANCE_DTYPE.itemsize == 64
+MAX_INSTANCES = 8192
 # bytes of one wide node (quantized child boxes, include/apt.h apt_bvh_export)
 NODE_BYTES = {2: 48, 4: 64, 8: 96}
 
@@ -118,6 +123,77 @@ class Bvh:
             self.close()
         except Exception:
             pass
+
+
+class Tlas:
+    """A two-level scene (libapt's apt_tlas): BLASes copied into one device
+    arena and a TLAS over instances of them, the reference's per-mesh BLAS and
+    per-frame TLAS (src/scene_resources/scene_model.rs:53-88,
+    scene_resources/mod.rs:159-276).  Hits carry custom_index + BLAS prim."""
+
+    def __init__(self, width: int = 4, lds_stack: int = 8, refill: int = 0):
+        opts = BvhOpts(width, 0, 0, 0, lds_stack, refill)
+        h = C.c_void_p()
+        check(lib().apt_tlas_create(C.byref(opts), C.byref(h)), "apt_tlas_create")
+        self._h = h
+        self.width = width
+
+    def add_blas(self, bvh: Bvh) -> int:
+        idx = C.c_uint32()
+        check(lib().apt_tlas_add_blas(self._h, bvh.handle, C.byref(idx)), "apt_tlas_add_blas")
+        return int(idx.value)
+
+    def set_instances(self, inst: np.ndarray) -> None:
+        inst = np.ascontiguousarray(inst, dtype=INSTANCE_DTYPE).reshape(-1)
+        check(lib().apt_tlas_set_instances(self._h, vptr(inst), inst.shape[0]), "apt_tlas_set_instances")
+
+    def stats(self) -> dict:
+        s = TlasStats()
+        check(lib().apt_tlas_get_stats(self._h, C.byref(s)), "apt_tlas_get_stats")
+        return {f: getattr(s, f) for f, _ in TlasStats._fields_}
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)
+        hits = np.empty(rays.shape[0], dtype=HIT_DTYPE)
+        check(lib().apt_trace_closest_tlas_host(self._h, vptr(rays), rays.shape[0], vptr(hits)),
+              "apt_trace_closest_tlas_host")
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE).reshape(-1)
+        occ = np.empty(rays.shape[0], dtype=np.uint32)
+        check(lib().apt_trace_any_tlas_host(self._h, vptr(rays), rays.shape[0], vptr(occ)),
+              "apt_trace_any_tlas_host")
+        return occ
+
+    def trace_closest_device(self, d_rays, n: int, d_hits, stream=None) -> None:
+        check(lib().apt_trace_closest_tlas(self._h, _dp(d_rays), n, _dp(d_hits), _dp(stream)),
+              "apt_trace_closest_tlas")
+
+    def trace_any_device(self, d_rays, n: int, d_occ, stream=None) -> None:
+        check(lib().apt_trace_any_tlas(self._h, _dp(d_rays), n, _dp(d_occ), _dp(stream)),
+              "apt_trace_any_tlas")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().apt_tlas_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def instances(transforms, blas, custom) -> np.ndarray:
+    """An apt_instance array from (n, 3, 4) transforms, BLAS indices and custom indices."""
+    t = np.asarray(transforms, np.float32).reshape(-1, 3, 4)
+    out = np.zeros(t.shape[0], INSTANCE_DTYPE)
+    out["transform"] = t
+    out["blas"] = blas
+    out["custom_index"] = custom
+    return out
 
 
 def _dp(x):

@@ -177,6 +177,75 @@ apt_status apt_trace_closest_host_noalloc(const apt_bvh* bvh, const apt_ray* ray
                                           apt_hit* hits);
 
 /* ------------------------------------------------------------------------
+ * Two-level scenes: a top-level BVH (TLAS) over instances of bottom-level
+ * BVHs (BLAS), the reference's acceleration structures
+ * (src/scene_resources/scene_model.rs:53-88: one BLAS per mesh, built once;
+ * scene_resources/mod.rs:159-276: a TLAS of at most MAX_TLAS_INSTANCES = 8192
+ * instances rebuilt from the instance transforms every frame).
+ *
+ * A hit is traced in the instance's object space: the ray is mapped by the
+ * inverse of the instance transform (computed in double, rounded to float)
+ * as o' = ((m0 x + m1 y) + m2 z) + m3 per row, without fused multiply-adds,
+ * and tested against the BLAS triangles with the watertight test; t is the
+ * same ray parameter in both spaces (affine map) and u, v the same
+ * barycentrics.  An instance whose transform is exactly the identity uses
+ * the ray as it is.  The closest hit is the lexicographic minimum of
+ * (t, custom_index + BLAS primitive) over all instances.
+ * ------------------------------------------------------------------------ */
+
+#define APT_TLAS_MAX_INSTANCES 8192u
+
+/* 64 B.  transform: object-to-world, row-major 3x4 (the reference's
+ * TlasInstance transform, scene_resources/mod.rs:177-191).  blas: index
+ * returned by apt_tlas_add_blas.  custom_index: added to the BLAS primitive
+ * index in every hit.  The reference's instance custom data selects a
+ * vertex-pool slice whose first triangle the shader adds to primitive_index
+ * (trace.wgsl:106-120); custom_index is that first triangle. */
+typedef struct apt_instance {
+  float transform[12];
+  uint32_t blas;
+  uint32_t custom_index;
+  uint32_t _pad[2];
+} apt_instance;
+
+typedef struct apt_tlas apt_tlas;
+
+typedef struct apt_tlas_stats {
+  uint32_t n_blas;
+  uint32_t n_instances;
+  uint32_t n_tlas_nodes;
+  uint32_t width;
+  uint32_t max_stack;       /* worst-case traversal stack: TLAS path + deepest BLAS */
+  uint32_t _pad;
+  uint64_t blas_tris;       /* triangles of all BLASes */
+  uint64_t device_bytes;
+  double ms_last_update;    /* wall time of the last apt_tlas_set_instances */
+} apt_tlas_stats;
+
+/* An empty two-level scene; opts->width (0 = 4) is the width of every BLAS
+ * it takes and of its TLAS; lds_stack and refill as for apt_bvh_opts. */
+apt_status apt_tlas_create(const apt_bvh_opts* opts, apt_tlas** out);
+/* Copies a BLAS (built with the same width) into the scene's device arena
+ * and returns its index; the apt_bvh may be destroyed afterwards. */
+apt_status apt_tlas_add_blas(apt_tlas* t, const apt_bvh* blas, uint32_t* blas_index);
+/* Rebuilds the top level over n instances (n <= APT_TLAS_MAX_INSTANCES;
+ * n = 0 makes every ray miss).  Synchronous: on return the device holds the
+ * new TLAS.  APT_ERR_UNSUPPORTED when a transform is singular or the two
+ * levels need a deeper traversal stack than apt_trace_stack_capacity(). */
+apt_status apt_tlas_set_instances(apt_tlas* t, const apt_instance* instances, uint32_t n);
+void apt_tlas_destroy(apt_tlas* t);
+apt_status apt_tlas_get_stats(const apt_tlas* t, apt_tlas_stats* out);
+/* The ray queries of apt_trace_closest / apt_trace_any over a two-level
+ * scene (device arrays, asynchronous on stream), and their host-array
+ * variants. */
+apt_status apt_trace_closest_tlas(const apt_tlas* t, const apt_ray* d_rays, uint32_t n, apt_hit* d_hits,
+                                  void* stream);
+apt_status apt_trace_any_tlas(const apt_tlas* t, const apt_ray* d_rays, uint32_t n, uint32_t* d_occluded,
+                              void* stream);
+apt_status apt_trace_closest_tlas_host(const apt_tlas* t, const apt_ray* rays, uint32_t n, apt_hit* hits);
+apt_status apt_trace_any_tlas_host(const apt_tlas* t, const apt_ray* rays, uint32_t n, uint32_t* occluded);
+
+/* ------------------------------------------------------------------------
  * Scene data.  Layouts follow the reference's GPU pools.
  * ------------------------------------------------------------------------ */
 

@@ -24,6 +24,11 @@ void or_trace(void* b, const float* rays, uint32_t n, int any, float* out, uint6
               int threads);
 void or_brute(const float* tris, uint32_t ntris, const float* rays, uint32_t n, int any,
               float* out);
+void* or_inst_create(const float* tris, const uint32_t* blas_first, uint32_t n_blas);
+void or_inst_destroy(void* sc);
+void or_inst_trace(void* sc, const void* instances, uint32_t n_inst, const float* rays, uint32_t n, int any,
+                   int brute, float* out, int threads);
+int or_affine_inverse(const float* o2w, float* w2o);
 uint32_t or_block_index(uint32_t x, uint32_t y, uint32_t width);
 float or_aces(float x);
 int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,

@@ -22,6 +22,10 @@
 
 #include "oracle.h"
 
+namespace ors {
+extern int g_plant;  // planted mistakes, oracle_restir.cpp (tests only)
+}
+
 namespace orc {
 
 // ---- RNG: assets/shaders/random.wgsl:1-68 ---------------------------------
@@ -305,14 +309,23 @@ Bvh* build(const float* tris, uint32_t n) {
   return b;
 }
 
+// With WIDE (the object-space rays of instances) the box's entry and exit
+// distances move out by 2^-18 of their magnitude before the clamps, so a box
+// far from the ray origin is still entered when rounding would just miss it.
+template <bool WIDE = false>
 static inline bool box_enter(const BNode& nd, const float* o, const float* inv, float tmin,
                              float tcap, float& tnear) {
-  float tn = tmin, tf = tcap;
+  float bn = -INFINITY, bf = INFINITY;
   for (int k = 0; k < 3; ++k) {
     float a = (nd.lo[k] - o[k]) * inv[k], c = (nd.hi[k] - o[k]) * inv[k];
-    tn = std::max(tn, std::min(a, c));
-    tf = std::min(tf, std::max(a, c));
+    bn = std::max(bn, std::min(a, c));
+    bf = std::min(bf, std::max(a, c));
   }
+  if (WIDE) {
+    bn -= std::fabs(bn) * 0x1p-18f;
+    bf += std::fabs(bf) * 0x1p-18f;
+  }
+  const float tn = std::max(tmin, bn), tf = std::min(tcap, bf);
   tnear = tn;
   return tn <= tf;
 }
@@ -322,6 +335,7 @@ static inline float rcp_safe(float d) {
 }
 
 // Closest (any == false) or any hit of one ray; counts box and triangle tests.
+template <bool WIDE = false>
 void trace1(const Bvh& b, const float* ray, bool any, float* hit, uint64_t& nbox, uint64_t& ntri) {
   const float* o = ray;
   const float* d = ray + 4;
@@ -334,7 +348,7 @@ void trace1(const Bvh& b, const float* ray, bool any, float* hit, uint64_t& nbox
   uint32_t cur = 0;
   float tn0;
   nbox++;
-  if (!box_enter(b.nodes[0], o, inv, s.tmin, bt, tn0)) goto done;
+  if (!box_enter<WIDE>(b.nodes[0], o, inv, s.tmin, bt, tn0)) goto done;
   while (true) {
     const BNode& nd = b.nodes[cur];
     if (nd.count) {
@@ -358,8 +372,8 @@ void trace1(const Bvh& b, const float* ray, bool any, float* hit, uint64_t& nbox
     } else {
       float t0, t1;
       nbox += 2;
-      bool h0 = box_enter(b.nodes[nd.a], o, inv, s.tmin, bt, t0);
-      bool h1 = box_enter(b.nodes[nd.a + 1], o, inv, s.tmin, bt, t1);
+      bool h0 = box_enter<WIDE>(b.nodes[nd.a], o, inv, s.tmin, bt, t0);
+      bool h1 = box_enter<WIDE>(b.nodes[nd.a + 1], o, inv, s.tmin, bt, t1);
       if (h0 && h1) {
         uint32_t nearer = (t1 < t0) ? nd.a + 1 : nd.a, farther = (t1 < t0) ? nd.a : nd.a + 1;
         if (sp < 256) stack[sp++] = farther;
@@ -376,7 +390,7 @@ void trace1(const Bvh& b, const float* ray, bool any, float* hit, uint64_t& nbox
     while (sp > 0) {
       uint32_t c = stack[--sp];
       float tn;
-      if (any || box_enter(b.nodes[c], o, inv, s.tmin, bt, tn)) {
+      if (any || box_enter<WIDE>(b.nodes[c], o, inv, s.tmin, bt, tn)) {
         cur = c;
         found = true;
         break;
@@ -515,6 +529,141 @@ bool oracle_any_d(const Bvh* b, const double* o, const double* d, double tmin, d
   return trace_d(b, o, d, tmin, tmax, true, &t, &u, &v, &p);
 }
 
+// ---- two-level scenes (instances of BLASes) ------------------------------
+// The reference traces rays against a TLAS of BLAS instances
+// (scene_resources/mod.rs:159-276); the driver maps each ray into the
+// instance's object space.  The restatement: the world-to-object matrix is
+// the inverse of the instance's object-to-world 3x4, evaluated in double by
+// cofactors and rounded to float; an instance whose transform is exactly the
+// identity keeps the ray; otherwise o' = ((m0 x + m1 y) + m2 z) + m3 and
+// d' = (m0 x + m1 y) + m2 z per row in float.  The closest hit is the
+// lexicographic minimum of (t, custom + BLAS prim) over all instances.
+
+struct Inst {  // apt_instance: 12 floats, blas, custom, 2 pad (64 B)
+  float xf[12];
+  uint32_t blas, custom, pad[2];
+};
+
+bool affine_inverse(const float* o2w, float* w2o) {
+  const double m00 = o2w[0], m01 = o2w[1], m02 = o2w[2], t0 = o2w[3];
+  const double m10 = o2w[4], m11 = o2w[5], m12 = o2w[6], t1 = o2w[7];
+  const double m20 = o2w[8], m21 = o2w[9], m22 = o2w[10], t2 = o2w[11];
+  // cofactors of the first row, then the determinant
+  const double c00 = m11 * m22 - m12 * m21, c01 = m12 * m20 - m10 * m22, c02 = m10 * m21 - m11 * m20;
+  const double det = m00 * c00 + m01 * c01 + m02 * c02;
+  if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return false;
+  const double r[3][3] = {{c00 / det, (m02 * m21 - m01 * m22) / det, (m01 * m12 - m02 * m11) / det},
+                          {c01 / det, (m00 * m22 - m02 * m20) / det, (m02 * m10 - m00 * m12) / det},
+                          {c02 / det, (m01 * m20 - m00 * m21) / det, (m00 * m11 - m01 * m10) / det}};
+  for (int i = 0; i < 3; ++i) {
+    for (int k = 0; k < 3; ++k) w2o[4 * i + k] = (float)r[i][k];
+    w2o[4 * i + 3] = (float)(-((r[i][0] * t0 + r[i][1] * t1) + r[i][2] * t2));
+  }
+  for (int k = 0; k < 12; ++k)
+    if (!std::isfinite(w2o[k])) return false;
+  return true;
+}
+
+static bool is_identity(const float* m) {
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 4; ++k) {
+      const float want = (k == i) ? 1.0f : 0.0f;
+      if (m[4 * i + k] != want || std::signbit(m[4 * i + k])) return false;
+    }
+  return true;
+}
+
+// The ray (8 floats) of instance `in` in its object space.
+static bool inst_ray(const Inst& in, const float* ray, float* out) {
+  std::memcpy(out, ray, 8 * sizeof(float));
+  if (is_identity(in.xf)) return true;
+  float w[12];
+  if (!affine_inverse(in.xf, w)) return false;
+  if (ors::g_plant == 7) {
+    std::swap(w[1], w[4]);
+    std::swap(w[2], w[8]);
+    std::swap(w[6], w[9]);
+  } else if (ors::g_plant == 8) {
+    w[3] = -w[3];
+    w[7] = -w[7];
+    w[11] = -w[11];
+  }
+  for (int i = 0; i < 3; ++i) {
+    const float* m = w + 4 * i;
+    out[i] = ((m[0] * ray[0] + m[1] * ray[1]) + m[2] * ray[2]) + m[3];
+    out[4 + i] = (m[0] * ray[4] + m[1] * ray[5]) + m[2] * ray[6];
+  }
+  return true;
+}
+
+struct InstScene {
+  std::vector<float> tris;          // all BLASes, 9 floats per triangle
+  std::vector<uint32_t> first;      // n_blas + 1 triangle offsets
+  std::vector<Bvh*> bvh;            // per BLAS
+  ~InstScene() {
+    for (Bvh* b : bvh) delete b;
+  }
+};
+
+// One ray against all instances: brute force over every BLAS triangle, or
+// each BLAS's BVH2 with the widened box test.
+static void inst_trace1(const InstScene& sc, const Inst* inst, uint32_t n_inst, const float* ray, bool any,
+                        bool brute, float* hit) {
+  float bt = ray[7], bu = 0.0f, bv = 0.0f;
+  uint32_t bp = 0xFFFFFFFFu;
+  bool occ = false;
+  for (uint32_t k = 0; k < n_inst && !occ; ++k) {
+    const Inst& in = inst[k];
+    float r[8];
+    if (!inst_ray(in, ray, r)) continue;
+    const uint32_t f0 = sc.first[in.blas], f1 = sc.first[in.blas + 1];
+    if (brute) {
+      Shear s = shear_of(r, r + 4, r[3], r[7]);
+      for (uint32_t p = f0; p < f1 && !occ; ++p) {
+        float t, u, v;
+        if (!tri_test(s, &sc.tris[9 * (size_t)p], t, u, v)) continue;
+        if (any) {
+          occ = true;
+          break;
+        }
+        const uint32_t g = in.custom + (p - f0);
+        if (before(t, g, bt, bp)) {
+          bt = t;
+          bu = u;
+          bv = v;
+          bp = g;
+        }
+      }
+    } else {
+      float h[4];
+      uint64_t nb = 0, nt = 0;
+      trace1<true>(*sc.bvh[in.blas], r, any, h, nb, nt);
+      if (any) {
+        occ = h[0] != 0.0f;
+        continue;
+      }
+      uint32_t lp;
+      std::memcpy(&lp, &h[3], 4);
+      if (lp == 0xFFFFFFFFu) continue;
+      const uint32_t g = in.custom + lp;
+      if (before(h[0], g, bt, bp)) {
+        bt = h[0];
+        bu = h[1];
+        bv = h[2];
+        bp = g;
+      }
+    }
+  }
+  if (any) {
+    hit[0] = occ ? 1.0f : 0.0f;
+    return;
+  }
+  hit[0] = bp == 0xFFFFFFFFu ? INFINITY : bt;
+  hit[1] = bu;
+  hit[2] = bv;
+  std::memcpy(&hit[3], &bp, 4);
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -628,6 +777,48 @@ uint32_t or_block_index(uint32_t x, uint32_t y, uint32_t width) {
   uint32_t per_row = (width + B - 1) / B;
   return (by * per_row + bx) * (B * B) + oy * B + ox;
 }
+
+// Two-level scenes.  tris: the BLASes' triangles one after another (9 floats
+// each); blas_first: n_blas + 1 offsets into them (in triangles).
+void* or_inst_create(const float* tris, const uint32_t* blas_first, uint32_t n_blas) {
+  InstScene* sc = new InstScene();
+  sc->first.assign(blas_first, blas_first + n_blas + 1);
+  sc->tris.assign(tris, tris + 9 * (size_t)blas_first[n_blas]);
+  for (uint32_t b = 0; b < n_blas; ++b)
+    sc->bvh.push_back(build(tris + 9 * (size_t)blas_first[b], blas_first[b + 1] - blas_first[b]));
+  return sc;
+}
+void or_inst_destroy(void* sc) { delete static_cast<InstScene*>(sc); }
+
+// instances: n_inst apt_instance records (64 B each); rays: n x 8 floats;
+// out: n x 4 floats (closest: t, u, v, prim bits) or n floats (any: 0/1).
+// brute != 0: every triangle of every instance; else the BLASes' BVH2s.
+void or_inst_trace(void* scp, const void* instances, uint32_t n_inst, const float* rays, uint32_t n, int any,
+                   int brute, float* out, int threads) {
+  const InstScene& sc = *static_cast<InstScene*>(scp);
+  const Inst* inst = static_cast<const Inst*>(instances);
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  std::atomic<uint32_t> next{0};
+  auto work = [&]() {
+    while (true) {
+      const uint32_t s0 = next.fetch_add(256);
+      if (s0 >= n) break;
+      const uint32_t e = std::min(n, s0 + 256);
+      for (uint32_t i = s0; i < e; ++i) {
+        float h[4];
+        inst_trace1(sc, inst, n_inst, rays + 8 * (size_t)i, any != 0, brute != 0, h);
+        if (any) out[i] = h[0];
+        else std::memcpy(out + 4 * (size_t)i, h, 16);
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+}
+
+int or_affine_inverse(const float* o2w, float* w2o) { return affine_inverse(o2w, w2o) ? 1 : 0; }
 
 // resolve.wgsl:29-38 ACES fit, clamped to [0, 1].
 float or_aces(float x) {

@@ -380,7 +380,9 @@ enum Plant : int {
   PLANT_NO_INV_M = 3,     // combine: W = wsum / p^ (the 1/M dropped)
   PLANT_NO_GI_MCAP = 4,   // GI temporal: previous M not capped
   PLANT_Z_SWAP = 5,       // combine_unbiased: each domain's test adds the other's M
-  PLANT_NO_JACOBIAN = 6   // DI reuse: a reused weight keeps the other point's measure
+  PLANT_NO_JACOBIAN = 6,  // DI reuse: a reused weight keeps the other point's measure
+  PLANT_INST_TRANSPOSE = 7,   // instances: the world-to-object 3x3 transposed (oracle_core.cpp)
+  PLANT_INST_TRANSLATION = 8  // instances: the world-to-object translation negated (oracle_core.cpp)
 };
 int g_plant = PLANT_NONE;
 

@@ -9,7 +9,8 @@
 //!   (crates/appearance-render-loop/src/node.rs:14-25).
 //! * `apt_start_render`: `Node::start_render` (node.rs:44-105) without the
 //!   transport.
-//! * `apt_bvh_*` / `apt_trace_*`: the driver's BLAS build and ray query.
+//! * `apt_bvh_*` / `apt_trace_*`: the driver's BLAS build and ray query;
+//!   `apt_tlas_*`: its TLAS over BLAS instances.
 //! * `apt_server_*`, `apt_submit`, ...: the serving loop (no counterpart).
 #![allow(non_camel_case_types)]
 
@@ -31,6 +32,7 @@ pub const APT_ERR_NOT_FOUND: apt_status = 6;
 pub const APT_ABI_VERSION: u32 = 1;
 pub const APT_MISS: u32 = 0xFFFF_FFFF;
 pub const APT_TRACE_NOALLOC_MAX_RAYS: u32 = 65536;
+pub const APT_TLAS_MAX_INSTANCES: u32 = 8192;
 pub const APT_INVALID_TEXTURE: u32 = 0xFFFF_FFFF;
 pub const APT_BSDF_DISNEY: u32 = 0;
 pub const APT_BSDF_LAMBERT: u32 = 1;
@@ -62,6 +64,10 @@ pub struct apt_scene_data {
 }
 #[repr(C)]
 pub struct apt_renderer {
+    _p: [u8; 0],
+}
+#[repr(C)]
+pub struct apt_tlas {
     _p: [u8; 0],
 }
 #[repr(C)]
@@ -123,6 +129,31 @@ pub struct apt_bvh_stats {
     pub device_bytes: u64,
     pub build_ms: f64,
     pub sah_cost: f64,
+}
+
+/// One instance of a two-level scene (the reference's TlasInstance,
+/// scene_resources/mod.rs:177-191), 64 B: object-to-world 3x4 row major.
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct apt_instance {
+    pub transform: [f32; 12],
+    pub blas: u32,
+    pub custom_index: u32,
+    pub _pad: [u32; 2],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Default, Debug)]
+pub struct apt_tlas_stats {
+    pub n_blas: u32,
+    pub n_instances: u32,
+    pub n_tlas_nodes: u32,
+    pub width: u32,
+    pub max_stack: u32,
+    pub _pad: u32,
+    pub blas_tris: u64,
+    pub device_bytes: u64,
+    pub ms_last_update: f64,
 }
 
 /// The reference's MaterialDescriptor (src/scene_resources/material_pool.rs:13-46), 160 B.
@@ -365,6 +396,21 @@ extern "C" {
         -> apt_status;
     pub fn apt_trace_closest_host_noalloc(bvh: *const apt_bvh, rays: *const apt_ray, n: u32,
                                           hits: *mut apt_hit) -> apt_status;
+
+    // two-level scenes
+    pub fn apt_tlas_create(opts: *const apt_bvh_opts, out: *mut *mut apt_tlas) -> apt_status;
+    pub fn apt_tlas_add_blas(t: *mut apt_tlas, blas: *const apt_bvh, blas_index: *mut u32) -> apt_status;
+    pub fn apt_tlas_set_instances(t: *mut apt_tlas, instances: *const apt_instance, n: u32) -> apt_status;
+    pub fn apt_tlas_destroy(t: *mut apt_tlas);
+    pub fn apt_tlas_get_stats(t: *const apt_tlas, out: *mut apt_tlas_stats) -> apt_status;
+    pub fn apt_trace_closest_tlas(t: *const apt_tlas, d_rays: *const apt_ray, n: u32, d_hits: *mut apt_hit,
+                                  stream: *mut c_void) -> apt_status;
+    pub fn apt_trace_any_tlas(t: *const apt_tlas, d_rays: *const apt_ray, n: u32, d_occluded: *mut u32,
+                              stream: *mut c_void) -> apt_status;
+    pub fn apt_trace_closest_tlas_host(t: *const apt_tlas, rays: *const apt_ray, n: u32, hits: *mut apt_hit)
+        -> apt_status;
+    pub fn apt_trace_any_tlas_host(t: *const apt_tlas, rays: *const apt_ray, n: u32, occluded: *mut u32)
+        -> apt_status;
 
     // scene data
     pub fn apt_material_default(m: *mut apt_material);

@@ -26,6 +26,10 @@ _SIGS = {
     "or_bvh_node_count": (C.c_uint32, [_P]),
     "or_trace": (None, [_P, _P, C.c_uint32, C.c_int, _P, _P, C.c_int]),
     "or_brute": (None, [_P, C.c_uint32, _P, C.c_uint32, C.c_int, _P]),
+    "or_inst_create": (_P, [_P, _P, C.c_uint32]),
+    "or_inst_destroy": (None, [_P]),
+    "or_inst_trace": (None, [_P, _P, C.c_uint32, _P, C.c_uint32, C.c_int, C.c_int, _P, C.c_int]),
+    "or_affine_inverse": (C.c_int, [_P, _P]),
     "or_block_index": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "or_aces": (C.c_float, [C.c_float]),
 }
@@ -91,6 +95,45 @@ class Oracle:
         out = np.empty(n if any_hit else (n, 4), np.float32)
         self.lib.or_brute(_vp(tris), tris.shape[0], _vp(rays), n, int(any_hit), _vp(out))
         return out
+
+
+    def affine_inverse(self, o2w: np.ndarray):
+        """World-to-object 3x4 of an object-to-world 3x4 (float32), or None if singular."""
+        o2w = np.ascontiguousarray(o2w, np.float32).reshape(12)
+        out = np.empty(12, np.float32)
+        return out.reshape(3, 4) if self.lib.or_affine_inverse(_vp(o2w), _vp(out)) else None
+
+    def instances(self, blas_tris) -> "OracleInstances":
+        return OracleInstances(self, blas_tris)
+
+
+class OracleInstances:
+    """BLASes (a list of (n, 9) triangle arrays) traced through instances."""
+
+    def __init__(self, o: Oracle, blas_tris):
+        self.o = o
+        mats = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t in blas_tris]
+        self.first = np.zeros(len(mats) + 1, np.uint32)
+        self.first[1:] = np.cumsum([m.shape[0] for m in mats])
+        self.tris = np.ascontiguousarray(np.concatenate(mats, 0))
+        self.h = C.c_void_p(o.lib.or_inst_create(_vp(self.tris), _vp(self.first), len(mats)))
+
+    def trace(self, instances: np.ndarray, rays: np.ndarray, any_hit: bool = False, brute: bool = False,
+              threads: int = 0) -> np.ndarray:
+        inst = np.ascontiguousarray(instances)
+        assert inst.dtype.itemsize == 64
+        rays = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        out = np.empty(n if any_hit else (n, 4), np.float32)
+        self.o.lib.or_inst_trace(self.h, _vp(inst), inst.shape[0], _vp(rays), n, int(any_hit), int(brute),
+                                 _vp(out), threads)
+        return out
+
+    def __del__(self):
+        try:
+            self.o.lib.or_inst_destroy(self.h)
+        except Exception:
+            pass
 
 
 class OracleBvh:
