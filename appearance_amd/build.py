@@ -27,6 +27,7 @@ SOURCES = [
     "server.hip",
     "tuner.cpp",
     "restir.hip",
+    "restir_two_level.hip",
     "bvh_gpu.hip",
     "gltf.cpp",
     "tlas.hip",

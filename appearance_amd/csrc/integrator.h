@@ -31,10 +31,11 @@ struct FrameView {
 // Scene resident in HBM.  Triangle attributes are in input (prim) order; the
 // BVH keeps its own copy of the positions in leaf order.
 struct SceneView {
-  const void* nodes;            // WNode<width>[]
-  const float4* bvh_tris;
-  uint32_t root;
-  uint32_t n_nodes;
+  const void* nodes;            // WNode<width>[] (a two-level scene: its node arena)
+  const float4* bvh_tris;       // (a two-level scene: its triangle arena)
+  uint32_t root;                // (a two-level scene: the TLAS root)
+  uint32_t n_nodes;             // nodes the pool kernels may stage in LDS (a two-level scene: the TLAS's)
+  const InstRec* insts;         // instance records of a two-level scene, or null
   uint32_t width;
   uint32_t lds_stack;           // traversal stack entries in LDS (8, 16, 24)
   uint32_t refill;              // persistent traversal refill threshold (65: one ray per thread)

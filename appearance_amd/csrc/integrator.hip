@@ -57,9 +57,11 @@ struct ExtendSrc {
   }
 };
 
-template <int W, int S>
-__global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, int qin,
-                                                     const WNode<W>* __restrict__ nodes) {
+// INST: a two-level scene (sc.insts); its kernels trade occupancy for the
+// instance state's registers.
+template <int W, int S, bool INST>
+__global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_extend(SceneView sc, PathView ps, int qin,
+                                                                const WNode<W>* __restrict__ nodes) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   __shared__ float4 s_top[TOP_BYTES / 16];
   const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
@@ -71,8 +73,8 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(SceneView sc, PathView ps, 
   }
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_SHADOW], threadIdx.x);
   ExtendSrc<W> src{nodes, sc.bvh_tris, sc.root, ps, ps.queue[qin]};
-  trace_pool<W, S, false>(src, count, &ps.counters[CNT_POOL_EXT], min(sc.refill, 64u), s_stack + threadIdx.x,
-                          BLOCK, s_top, ktop);
+  trace_pool<W, S, false, ExtendSrc<W>, INST>(src, count, &ps.counters[CNT_POOL_EXT], min(sc.refill, 64u),
+                                              s_stack + threadIdx.x, BLOCK, s_top, ktop, sc.insts);
 }
 
 // Four waves per SIMD: the compiler's own choice (136 VGPRs) allows three.
@@ -121,9 +123,9 @@ struct ShadowSrc {
   }
 };
 
-template <int W, int S>
-__global__ __launch_bounds__(BLOCK, 8) void k_shadow(SceneView sc, PathView ps,
-                                                     const WNode<W>* __restrict__ nodes) {
+template <int W, int S, bool INST>
+__global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_shadow(SceneView sc, PathView ps,
+                                                                const WNode<W>* __restrict__ nodes) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   __shared__ float4 s_top[TOP_BYTES / 16];
   const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
@@ -133,8 +135,8 @@ __global__ __launch_bounds__(BLOCK, 8) void k_shadow(SceneView sc, PathView ps,
   }
   if (blockIdx.x == 0) pool_zero(&ps.counters[CNT_POOL_EXT], threadIdx.x);  // next bounce's extend
   ShadowSrc<W> src{nodes, sc.bvh_tris, sc.root, ps};
-  trace_pool<W, S, true>(src, count, &ps.counters[CNT_POOL_SHADOW], min(sc.refill, 64u), s_stack + threadIdx.x,
-                         BLOCK, s_top, ktop);
+  trace_pool<W, S, true, ShadowSrc<W>, INST>(src, count, &ps.counters[CNT_POOL_SHADOW], min(sc.refill, 64u),
+                                             s_stack + threadIdx.x, BLOCK, s_top, ktop, sc.insts);
 }
 
 // Adds this pass's samples of every pixel to its accumulator, one sample at a
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_resolve(FrameView fv, const float4* _
   if (out_rgba) out_rgba[dst] = unorm8(tm.x * k) | (unorm8(tm.y * k) << 8) | (unorm8(tm.z * k) << 16) | (255u << 24);
 }
 
-template <int W>
+template <int W, bool INST>
 void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_paths, int qin,
                            const FrameView& fv, uint32_t bounce, hipStream_t s, hipEvent_t* ev) {
   // the traversal stages pull from their queue and the shade stage strides
@@ -208,13 +210,13 @@ void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_p
   };
   APT_FOR_STACK_DEPTH(sc.lds_stack, {
     mark(0);
-    hipLaunchKernelGGL((k_extend<W, S_>), pool, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
+    hipLaunchKernelGGL((k_extend<W, S_, INST>), pool, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
     mark(1);
     mark(2);
     hipLaunchKernelGGL(k_shade, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
     mark(3);
     mark(4);
-    hipLaunchKernelGGL((k_shadow<W, S_>), pool, dim3(BLOCK), 0, s, sc, ps, nodes);
+    hipLaunchKernelGGL((k_shadow<W, S_, INST>), pool, dim3(BLOCK), 0, s, sc, ps, nodes);
     mark(5);
   })
 }
@@ -229,10 +231,20 @@ apt_status integrate_pass(const FrameView& fv, const SceneView& sc, const PathVi
   for (uint32_t b = 0; b < fv.max_bounces; ++b) {
     const int qin = (int)(b & 1u);
     hipEvent_t* ev = stage_ev ? stage_ev + 6 * b : nullptr;
+    const bool inst = sc.insts != nullptr;
     switch (sc.width) {
-      case 2: launch_bounce_kernels<2>(sc, ps, n_paths, qin, fv, b, s, ev); break;
-      case 4: launch_bounce_kernels<4>(sc, ps, n_paths, qin, fv, b, s, ev); break;
-      case 8: launch_bounce_kernels<8>(sc, ps, n_paths, qin, fv, b, s, ev); break;
+      case 2:
+        if (inst) launch_bounce_kernels<2, true>(sc, ps, n_paths, qin, fv, b, s, ev);
+        else launch_bounce_kernels<2, false>(sc, ps, n_paths, qin, fv, b, s, ev);
+        break;
+      case 4:
+        if (inst) launch_bounce_kernels<4, true>(sc, ps, n_paths, qin, fv, b, s, ev);
+        else launch_bounce_kernels<4, false>(sc, ps, n_paths, qin, fv, b, s, ev);
+        break;
+      case 8:
+        if (inst) launch_bounce_kernels<8, true>(sc, ps, n_paths, qin, fv, b, s, ev);
+        else launch_bounce_kernels<8, false>(sc, ps, n_paths, qin, fv, b, s, ev);
+        break;
       default: set_error("integrator: unsupported BVH width %u", sc.width); return APT_ERR_INVALID_ARG;
     }
   }

@@ -2,12 +2,15 @@
 // (crates/appearance-path-tracer-gpu/src/lib.rs:250-615) and of the
 // NodeRenderer job interface (crates/appearance-render-loop/src/node.rs:14-25).
 //
-// Host responsibilities: world actions (camera, procedural models), scene
-// flattening and upload (BVH build, triangle attributes, emitter list), frame
+// Host responsibilities: world actions (camera, models spawned from procedural
+// or glTF assets as instances of per-asset BLASes, placed by a TLAS rebuilt on
+// every action), scene upload (BVH or TLAS, triangle attributes, emitter list), frame
 // buffers for the requested slab, the sample/bounce schedule of the wavefront
 // integrator, progressive accumulation and the blocking readback.
 #include <chrono>
 #include <cstddef>
+#include <map>
+#include <memory>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -18,36 +21,42 @@
 #include "integrator.h"
 #include "restir.h"
 #include "scene_gpu.h"
+#include "tlas.h"
 
 namespace {
 
 using namespace apt;
 
-struct Model {
-  uint8_t uuid[16];
-  float xform[16];  // column-major
-  apt_scene_data* data;
-  // The model's own triangles, resident from its first flatten on: a move
-  // re-places them on the device instead of crossing PCIe again.
+// A loaded model asset: its triangles resident in HBM in object space and its
+// BLAS in the renderer's two-level scene.  Spawns of the same asset path share
+// it, as the reference shares one SceneModel per asset path
+// (scene_resources/mod.rs:115-133); it lives until Clear.
+struct Asset {
+  apt_scene_data* data = nullptr;
+  uint32_t n_tris = 0, n_mats = 0;
+  bool emissive = false;
   float* d_xyz = nullptr;
   float* d_nrm = nullptr;
   uint32_t* d_mat = nullptr;
+  uint32_t blas = INVALID;  // BLAS index in the renderer's two-level scene
+  ~Asset() {
+    apt_scene_data_destroy(data);
+    for (void* p : {(void*)d_xyz, (void*)d_nrm, (void*)d_mat})
+      if (p) (void)hipFree(p);
+  }
 };
 
-void model_free(Model& m) {
-  apt_scene_data_destroy(m.data);
-  for (void* p : {(void*)m.d_xyz, (void*)m.d_nrm, (void*)m.d_mat})
-    if (p) (void)hipFree(p);
-  m.data = nullptr;
-  m.d_xyz = m.d_nrm = nullptr;
-  m.d_mat = nullptr;
-}
+struct Model {
+  uint8_t uuid[16];
+  float xform[16];  // column-major
+  std::shared_ptr<Asset> asset;
+};
 
-// Copies a model's triangles to the device once (checking its material ids).
-apt_status model_resident(Model& m) {
-  if (m.d_xyz) return APT_OK;
+// Copies an asset's triangles to the device once (checking its material ids).
+apt_status asset_resident(Asset& a) {
+  if (a.d_xyz) return APT_OK;
   apt_mesh_desc md;
-  apt_scene_data_view(m.data, &md, nullptr, nullptr);
+  apt_scene_data_view(a.data, &md, nullptr, nullptr);
   const size_t n = md.n_tris;
   std::vector<uint32_t> ids(n, 0u);
   for (size_t t = 0; t < n; ++t) {
@@ -65,14 +74,14 @@ apt_status model_resident(Model& m) {
     }
     return APT_OK;
   };
-  apt_status st = put(m.d_xyz, md.positions, 9 * n);
-  if (st == APT_OK && md.normals) st = put(m.d_nrm, md.normals, 9 * n);
-  if (st == APT_OK) st = put(m.d_mat, ids.data(), n);
+  apt_status st = put(a.d_xyz, md.positions, 9 * n);
+  if (st == APT_OK && md.normals) st = put(a.d_nrm, md.normals, 9 * n);
+  if (st == APT_OK) st = put(a.d_mat, ids.data(), n);
   if (st != APT_OK) {
-    for (void* p : {(void*)m.d_xyz, (void*)m.d_nrm, (void*)m.d_mat})
+    for (void* p : {(void*)a.d_xyz, (void*)a.d_nrm, (void*)a.d_mat})
       if (p) (void)hipFree(p);
-    m.d_xyz = m.d_nrm = nullptr;
-    m.d_mat = nullptr;
+    a.d_xyz = a.d_nrm = nullptr;
+    a.d_mat = nullptr;
   }
   return st;
 }
@@ -213,8 +222,12 @@ struct apt_renderer {
   // camera (CameraUpdate); the reference's default camera until one arrives
   apt_camera cam{};
   uint64_t cam_version = 1, accum_cam_version = 0;
-  // scene
+  // scene: spawned models (instances of assets in a two-level scene) or an
+  // explicit triangle soup (apt_scene_set, one BVH)
   std::vector<Model> models;
+  std::map<std::string, std::shared_ptr<Asset>> assets;
+  apt_tlas* tlas = nullptr;
+  bool emitters_dirty = true;
   std::vector<float> pos, nrm;
   std::vector<uint32_t> mat_ids;
   std::vector<apt_material> mats;
@@ -250,8 +263,10 @@ struct apt_renderer {
   uint32_t rs_w = 0, rs_h = 0, rs_row0 = 0;
 
   ~apt_renderer() {
-    for (Model& m : models) model_free(m);
+    models.clear();
+    assets.clear();
     scene_gpu_free(gpu);
+    apt_tlas_destroy(tlas);
     d_frame_sum.release(); d_accum_tm.release(); d_accum_lin.release(); d_rgba.release();
     p_o.release(); p_d.release(); p_t.release(); p_l.release(); p_h.release(); p_so.release();
     p_sd.release(); p_sc.release(); p_rng.release(); p_q0.release(); p_q1.release();
@@ -295,59 +310,107 @@ apt_status validate_config(const apt_config& c) {
   return APT_OK;
 }
 
+// Row-major 3x4 of a column-major 4x4 (the instance layout).
+void xform_rows(const float* m, float* out) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 4; ++c) out[4 * r + c] = m[4 * c + r];
+}
+
+// The spawned models as a two-level scene: each asset's BLAS is built once in
+// object space (the device builder) and copied into the renderer's scene; a
+// world action then rebuilds the TLAS over the instances and re-places the
+// world-space attributes the shading reads (positions, normals, material ids)
+// from the resident object-space triangles.  The emitter list is rebuilt only
+// when the set of models changes or an emissive model moves.
+apt_status upload_instances(apt_renderer* r) {
+  const apt_config& c = r->cfg;
+  hipStream_t s = r->stream;
+  apt_status st;
+  if (r->tlas && r->tlas->width != c.bvh_width) {
+    r->gpu.tlas = nullptr;
+    apt_tlas_destroy(r->tlas);
+    r->tlas = nullptr;
+    for (auto& kv : r->assets) kv.second->blas = INVALID;
+  }
+  if (!r->tlas) {
+    apt_bvh_opts o{};
+    o.width = c.bvh_width;
+    o.lds_stack = c.lds_stack_depth;
+    if ((st = apt_tlas_create(&o, &r->tlas)) != APT_OK) return st;
+  }
+  r->tlas->lds_stack = c.lds_stack_depth;
+  size_t total = 0;
+  bool all_normals = !r->models.empty();
+  for (Model& m : r->models) {
+    Asset& a = *m.asset;
+    if ((st = asset_resident(a)) != APT_OK) return st;
+    if (a.blas == INVALID) {
+      apt_bvh* b = nullptr;
+      const apt_bvh_opts o{c.bvh_width, 4, 32, APT_BUILDER_GPU, c.lds_stack_depth, 0};
+      st = apt_bvh_build_device(a.d_xyz, a.n_tris, &o, &b);
+      if (st == APT_OK) st = check_trace_stack(b->stats, "model");
+      if (st == APT_OK) st = apt_tlas_add_blas(r->tlas, b, &a.blas);
+      apt_bvh_destroy(b);
+      if (st != APT_OK) return st;
+    }
+    all_normals = all_normals && a.d_nrm != nullptr;
+    total += a.n_tris;
+  }
+  APT_REQUIRE(total < (1u << 28), APT_ERR_INVALID_ARG, "scene: %zu triangles exceed 2^28", total);
+  APT_REQUIRE(r->models.size() <= TLAS_MAX_INSTANCES, APT_ERR_INVALID_ARG, "scene: %zu models exceed %u",
+              r->models.size(), TLAS_MAX_INSTANCES);
+  r->gpu.tlas = nullptr;
+  if (total == 0) return APT_OK;
+  std::vector<apt_instance> inst(r->models.size());
+  std::vector<ModelXform> place(r->models.size());
+  r->mats.clear();
+  uint32_t base = 0;
+  for (size_t k = 0; k < r->models.size(); ++k) {
+    const Model& m = r->models[k];
+    std::memset(&inst[k], 0, sizeof(apt_instance));
+    xform_rows(m.xform, inst[k].transform);
+    inst[k].blas = m.asset->blas;
+    inst[k].custom_index = base;
+    ModelXform& x = place[k];
+    std::memcpy(x.m, m.xform, sizeof(x.m));
+    x.nm_ok = normal_matrix(m.xform, x.nm) ? 1u : 0u;
+    x.mat_base = (uint32_t)r->mats.size();
+    apt_mesh_desc md;
+    apt_scene_data_view(m.asset->data, &md, nullptr, nullptr);
+    r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
+    base += m.asset->n_tris;
+  }
+  if ((st = apt_tlas_set_instances(r->tlas, inst.data(), (uint32_t)inst.size())) != APT_OK) return st;
+  if ((st = scene_reserve(r->gpu, (uint32_t)total, all_normals, r->mats.data(), (uint32_t)r->mats.size(), s)) !=
+      APT_OK)
+    return st;
+  base = 0;
+  for (size_t k = 0; k < r->models.size(); ++k) {
+    const Asset& a = *r->models[k].asset;
+    if ((st = scene_place_model(a.d_xyz, a.d_nrm, a.d_mat, a.n_tris, place[k], r->gpu, base, s)) != APT_OK)
+      return st;
+    base += a.n_tris;
+  }
+  if (r->emitters_dirty) {
+    if ((st = scene_emitters(r->gpu, s)) != APT_OK) return st;
+    r->emitters_dirty = false;
+  } else {
+    APT_HIP(hipStreamSynchronize(s));
+  }
+  r->gpu.sky = r->sky;
+  r->gpu.tlas = r->tlas;
+  return APT_OK;
+}
+
 apt_status upload_scene(apt_renderer* r) {
   if (!r->scene_dirty) return APT_OK;
   const auto t0 = std::chrono::steady_clock::now();
-  scene_gpu_free(r->gpu);
   if (!r->explicit_scene) {
-    // Flatten the spawned models into one triangle soup in world space, on
-    // the device, from each model's resident copy.
-    r->mats.clear();
-    bool all_normals = !r->models.empty();
-    size_t total = 0;
-    for (Model& m : r->models) {
-      apt_status st = model_resident(m);
-      if (st != APT_OK) return st;
-      all_normals = all_normals && m.d_nrm != nullptr;
-      apt_mesh_desc md;
-      apt_scene_data_view(m.data, &md, nullptr, nullptr);
-      total += md.n_tris;
-    }
-    APT_REQUIRE(total < (1u << 28), APT_ERR_INVALID_ARG, "scene: %zu triangles exceed 2^28", total);
-    if (total > 0) {
-      float *d_xyz = nullptr, *d_nrm = nullptr;
-      uint32_t* d_mat = nullptr;
-      hipError_t e = hipMalloc(&d_xyz, 9 * total * sizeof(float));
-      if (e == hipSuccess && all_normals) e = hipMalloc(&d_nrm, 9 * total * sizeof(float));
-      if (e == hipSuccess) e = hipMalloc(&d_mat, total * sizeof(uint32_t));
-      apt_status st = APT_OK;
-      if (e != hipSuccess) {
-        set_error("scene flatten: hipMalloc failed: %s", hipGetErrorString(e));
-        st = e == hipErrorOutOfMemory ? APT_ERR_OOM : APT_ERR_DEVICE;
-      }
-      size_t base = 0;
-      for (const Model& m : r->models) {
-        if (st != APT_OK) break;
-        apt_mesh_desc md;
-        apt_scene_data_view(m.data, &md, nullptr, nullptr);
-        ModelXform x{};
-        std::memcpy(x.m, m.xform, sizeof(x.m));
-        x.nm_ok = normal_matrix(m.xform, x.nm) ? 1u : 0u;
-        x.mat_base = (uint32_t)r->mats.size();
-        r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
-        st = scene_flatten_model(m.d_xyz, m.d_nrm, m.d_mat, md.n_tris, x, d_xyz + 9 * base,
-                                 d_nrm ? d_nrm + 9 * base : nullptr, d_mat + base);
-        base += md.n_tris;
-      }
-      if (st == APT_OK)
-        st = scene_gpu_upload_device(d_xyz, d_nrm, d_mat, (uint32_t)total, r->mats.data(),
-                                     (uint32_t)r->mats.size(), &r->sky, r->cfg.bvh_width,
-                                     r->cfg.lds_stack_depth, r->gpu);
-      for (void* p : {(void*)d_xyz, (void*)d_nrm, (void*)d_mat})
-        if (p) (void)hipFree(p);
-      if (st != APT_OK) return st;
-    }
+    if (r->gpu.bvh) scene_gpu_free(r->gpu);  // an explicit scene before
+    apt_status st = upload_instances(r);
+    if (st != APT_OK) return st;
   } else {
+    scene_gpu_free(r->gpu);
     const uint32_t n = (uint32_t)r->mat_ids.size();
     if (n > 0) {
       apt_status st = scene_gpu_upload(r->pos.data(), r->has_normals ? r->nrm.data() : nullptr,
@@ -490,7 +553,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
               "render: out_accum_rgb holds %zu floats, needs %zu", accum_len, npix * 3);
   apt_status st = upload_scene(r);
   if (st != APT_OK) return st;
-  APT_REQUIRE(r->gpu.bvh != nullptr, APT_ERR_INVALID_ARG, "render: the scene is empty");
+  APT_REQUIRE(r->gpu.bvh != nullptr || r->gpu.tlas != nullptr, APT_ERR_INVALID_ARG, "render: the scene is empty");
 
   hipStream_t s = r->stream;
   const bool resized = r->fw != width || r->fh != height || r->frow0 != row_start || r->frows != rows;
@@ -673,6 +736,7 @@ apt_status apt_set_config(apt_renderer* r, const apt_config* cfg) {
   if (st != APT_OK) return st;
   if (cfg->bvh_width != r->cfg.bvh_width) r->scene_dirty = true;
   if (r->gpu.bvh) r->gpu.bvh->lds_stack = cfg->lds_stack_depth;
+  if (r->tlas) r->tlas->lds_stack = cfg->lds_stack_depth;
   if (cfg->sample_count != r->cfg.sample_count || cfg->max_bounces != r->cfg.max_bounces ||
       cfg->rr_start_bounce != r->cfg.rr_start_bounce || cfg->seed != r->cfg.seed ||
       std::memcmp(&cfg->integrator, &r->cfg.integrator,
@@ -699,21 +763,39 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
       char path[257];
       std::memcpy(path, b + 80, 256);
       path[256] = 0;
-      std::string name;
-      uint32_t seed;
+      std::shared_ptr<Asset>& slot = r->assets[path];
+      if (!slot) {
+        auto a = std::make_shared<Asset>();
+        std::string name;
+        uint32_t seed;
+        apt_status st = parse_procedural(path, name, seed) ? apt_scene_generate(name.c_str(), seed, &a->data)
+                                                           : gltf_load(path, &a->data);
+        if (st != APT_OK) {
+          r->assets.erase(path);
+          return st;
+        }
+        apt_mesh_desc md;
+        apt_scene_data_view(a->data, &md, nullptr, nullptr);
+        a->n_tris = md.n_tris;
+        a->n_mats = md.n_materials;
+        for (uint32_t k = 0; k < md.n_materials; ++k) {
+          const float* e = md.materials[k].emission;
+          a->emissive = a->emissive || e[0] > 0.0f || e[1] > 0.0f || e[2] > 0.0f;
+        }
+        slot = a;
+      }
       Model m;
       std::memcpy(m.xform, b, 64);
       std::memcpy(m.uuid, b + 64, 16);
-      apt_status st = parse_procedural(path, name, seed) ? apt_scene_generate(name.c_str(), seed, &m.data)
-                                                         : gltf_load(path, &m.data);
-      if (st != APT_OK) return st;
+      m.asset = slot;
       apt_sky sky;
-      apt_scene_data_view(m.data, nullptr, nullptr, &sky);
+      apt_scene_data_view(m.asset->data, nullptr, nullptr, &sky);
       if (sky.sun_enabled > 0.0f || sky.zenith[0] > 0.0f || sky.zenith[1] > 0.0f || sky.zenith[2] > 0.0f)
         r->sky = sky;
       r->models.push_back(m);
       r->explicit_scene = false;
       r->scene_dirty = true;
+      r->emitters_dirty = true;
       return APT_OK;
     }
     case 2: {  // TransformModel
@@ -722,6 +804,7 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
         if (std::memcmp(m.uuid, b + 64, 16) == 0) {
           std::memcpy(m.xform, b, 64);
           r->scene_dirty = true;
+          if (m.asset->emissive) r->emitters_dirty = true;
           return APT_OK;
         }
       }
@@ -732,21 +815,25 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
       APT_REQUIRE(len == 16, APT_ERR_INVALID_ARG, "DestroyModel carries 16 bytes, got %zu", len);
       for (size_t i = 0; i < r->models.size(); ++i) {
         if (std::memcmp(r->models[i].uuid, b, 16) == 0) {
-          model_free(r->models[i]);
           r->models.erase(r->models.begin() + (long)i);
           r->scene_dirty = true;
+          r->emitters_dirty = true;
           return APT_OK;
         }
       }
       set_error("DestroyModel: unknown entity");
       return APT_ERR_NOT_FOUND;
     }
-    case 4: {  // Clear
+    case 4: {  // Clear: the models and their assets (the reference drops its SceneModels, mod.rs:151-153)
       APT_REQUIRE(len == 4, APT_ERR_INVALID_ARG, "Clear carries 4 bytes, got %zu", len);
-      for (Model& m : r->models) model_free(m);
       r->models.clear();
+      r->assets.clear();
+      r->gpu.tlas = nullptr;
+      apt_tlas_destroy(r->tlas);
+      r->tlas = nullptr;
       r->explicit_scene = false;
       r->scene_dirty = true;
+      r->emitters_dirty = true;
       return APT_OK;
     }
     default:

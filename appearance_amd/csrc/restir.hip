@@ -306,24 +306,25 @@ __device__ __forceinline__ float di_jacobian(const SceneView& sc, uint32_t light
 
 // trace_shadow_ray (helpers/trace.wgsl:20-84): opaque any-hit from the point
 // offset along the normal, tmax = distance - 2 eps.
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ bool visible(const SceneView& sc, const LdsStack& st, v3 x, v3 n, v3 wi, float dist,
                                         uint32_t* ctr) {
   TraceRay ray = make_trace_ray(x + n * TRACE_EPSILON, wi, 0.0f, dist - 2.0f * TRACE_EPSILON);
   count_rays(true, ctr + RS_CTR_SLOTS);
-  return !trace_any<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st);
+  return !trace_any<W, S, INST>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st, sc.insts);
 }
 
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ HitRec closest(const SceneView& sc, const LdsStack& st, v3 o, v3 d, uint32_t* ctr) {
   TraceRay ray = make_trace_ray(o, d, 0.0f, 1e30f);
   count_rays(true, ctr);
-  return trace_closest<W, S>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st);
+  return trace_closest<W, S, INST>(static_cast<const WNode<W>*>(sc.nodes), sc.bvh_tris, sc.root, ray, st,
+                                   sc.insts);
 }
 
 // LightSample::phat (di_reservoir.wgsl:58-97): luma(f Le cos), with the
 // visibility test when `vis`.
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ float di_phat(const SceneView& sc, const LdsStack& st, const Vtx& v, const Bsdf& bs,
                                          uint32_t light, float u, float w, bool vis, uint32_t* ctr) {
   LightHit e;
@@ -334,7 +335,7 @@ __device__ __forceinline__ float di_phat(const SceneView& sc, const LdsStack& st
   // zero target needs no ray
   float pdf;
   const float ph = luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * c);
-  if (vis && ph != 0.0f && !visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, ctr)) return 0.0f;
+  if (vis && ph != 0.0f && !visible<W, S, INST>(sc, st, v.x, v.n, e.wi, e.dist, ctr)) return 0.0f;
   return ph;
 }
 
@@ -368,11 +369,11 @@ __device__ __forceinline__ DiRes di_merge(const DiRes& r1, const DiRes& r2, uint
 // radiance toward x carried by direction wi with path throughput Tf, and the
 // first hit (or a far point on an escaped ray).  Next-event estimation at the
 // hit takes one light sample (5 draws), visibility tested.
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& st, v3 x, v3 wi, v3 Tf, v3& first,
                                            uint32_t& rng, uint32_t* ctr) {
   const v3 o = x + wi * TRACE_EPSILON;
-  const HitRec h = closest<W, S>(sc, st, o, wi, ctr);
+  const HitRec h = closest<W, S, INST>(sc, st, o, wi, ctr);
   if (h.prim == INVALID) {
     first = x + wi * GI_SKY_DISTANCE;
     return Tf * sky_gradient(sc.sky, wi);
@@ -398,7 +399,7 @@ __device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& 
   if (!eval_light(sc, light, lu, lv, x2, e) || !(e.pdf > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
   const float c = dot(e.wi, n2);
   if (!(c > 0.0f) || !(maxc(e.Le) > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
-  if (!visible<W, S>(sc, st, x2, n2, e.wi, e.dist, ctr)) return mk(0.0f, 0.0f, 0.0f);
+  if (!visible<W, S, INST>(sc, st, x2, n2, e.wi, e.dist, ctr)) return mk(0.0f, 0.0f, 0.0f);
   const Bsdf b2 = make_bsdf(sc, sc.tri_mat[h.prim], n2, back, h.t);
   float pdf;
   return Tf * bsdf_eval(b2, wo2, e.wi, pdf) * e.Le * (c / e.pdf);
@@ -406,7 +407,7 @@ __device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& 
 
 // GiReservoir::phat (inline_path_tracer.wgsl:299-331): replays the approximate
 // estimate of sample x2 from shading point v with the stored RNG state.
-template <int W, int S>
+template <int W, int S, bool INST>
 __device__ __forceinline__ float gi_phat(const SceneView& sc, const LdsStack& st, const Vtx& v, const Bsdf& bs,
                                          v3 T, const GiRes& g, uint32_t* ctr) {
   const v3 wi = normalize(f4v(g.x2_rng) - v.x);
@@ -416,7 +417,7 @@ __device__ __forceinline__ float gi_phat(const SceneView& sc, const LdsStack& st
   const v3 Tf = T * f * c;
   uint32_t r = f2u(g.x2_rng.w);
   v3 first;
-  const v3 L = approx_trace<W, S>(sc, st, v.x + wi * GI_OFFSET, wi, Tf, first, r, ctr);
+  const v3 L = approx_trace<W, S, INST>(sc, st, v.x + wi * GI_OFFSET, wi, Tf, first, r, ctr);
   return luma(L) + GI_PHAT_FLOOR * luma(Tf);
 }
 
@@ -489,15 +490,15 @@ struct RqSrc {
   }
 };
 
-template <int W, int S, bool ANY, bool EXT>
-__global__ __launch_bounds__(BLOCK, 8) void k_rs_trace(SceneView sc, RestirBuffers b, uint32_t n) {
+template <int W, int S, bool ANY, bool EXT, bool INST>
+__global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_rs_trace(SceneView sc, RestirBuffers b, uint32_t n) {
   __shared__ uint2 s_stack[pool_lds_entries(S) * BLOCK];
   __shared__ float4 s_top[TOP_BYTES / 16];
   const WNode<W>* nodes = static_cast<const WNode<W>*>(sc.nodes);
   const uint32_t ktop = stage_top<W>(nodes, sc.n_nodes, s_top);
   RqSrc<W, EXT> src{nodes, sc.bvh_tris, sc.root, b};
-  trace_pool<W, S, ANY>(src, n, b.rq_ctr + RQ_POOL, min(sc.refill, 64u), s_stack + threadIdx.x, BLOCK, s_top,
-                        ktop);
+  trace_pool<W, S, ANY, RqSrc<W, EXT>, INST>(src, n, b.rq_ctr + RQ_POOL, min(sc.refill, 64u), s_stack + threadIdx.x,
+                                             BLOCK, s_top, ktop, sc.insts);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_vis(RestirBuffers b, uint32_t n
 // one BSDF candidate whose target is the approximate one-bounce estimate.
 // Roulette keeps the path with probability min(1, max T) (the reference
 // divides by max T even above 1), and a path it ends keeps this vertex's DI.
-template <int W, int S>
+template <int W, int S, bool INST>
 __global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
@@ -719,7 +720,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView 
       const v3 Tf = T * s.f * c;
       const uint32_t phat_rng = rng;
       v3 first;
-      const v3 L = approx_trace<W, S>(sc, st, v.x, s.wi, Tf, first, rng, b.counters);
+      const v3 L = approx_trace<W, S, INST>(sc, st, v.x, s.wi, Tf, first, rng, b.counters);
       const float ph = luma(L) + GI_PHAT_FLOOR * luma(Tf);
       if (res_update(wsum, M, ph / s.pdf, rng)) {
         g.x2_rng = v4(first, u2f(phat_rng));
@@ -736,7 +737,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView 
 
 // restir_di_temporal.wgsl:68-176 (nearest previous pixel, stochastic
 // rounding), with the visibility-tested target.
-template <int W, int S>
+template <int W, int S, bool INST>
 __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(RestirParams p, SceneView sc, RestirBuffers b) {
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
@@ -762,7 +763,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(Resti
       pr.W *= di_jacobian(sc, pr.light, pr.u, pr.v, v.x, f4v(b.gbuf[p.cur ^ 1][qy * p.width + qx].pos_depth));
       pr.M = fminf(pr.M, (float)p.di_mcap * r.M);
       const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
-      pr.phat = di_phat<W, S>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
+      pr.phat = di_phat<W, S, INST>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
       int kept;
       DiRes c = di_merge(r, pr, rng, kept);
       if (c.phat > RESTIR_DI_EPSILON && c.M * c.wsum > RESTIR_DI_EPSILON)
@@ -835,7 +836,7 @@ __device__ __forceinline__ uint32_t mirror_px(int x, int m) {
 }
 
 // restir_di_spatial.wgsl:70-162 (one pass, unbiased combination).
-template <int W, int S>
+template <int W, int S, bool INST>
 __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(RestirParams p, SceneView sc, RestirBuffers b) {
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
@@ -885,7 +886,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
     if (valid) {
       DiRes nr = b.di_tmp[q];
       nr.W *= di_jacobian(sc, nr.light, nr.u, nr.v, v.x, f4v(ng.pos_depth));
-      nr.phat = di_phat<W, S>(sc, st, v, pixel_bsdf(), nr.light, nr.u, nr.v, true, b.counters);
+      nr.phat = di_phat<W, S, INST>(sc, st, v, pixel_bsdf(), nr.light, nr.u, nr.v, true, b.counters);
       valid = nr.phat > 0.0f;
       // DiReservoir::combine_unbiased: Z counts the M of the domains where the
       // chosen sample has a (visibility-tested) target > 0
@@ -910,7 +911,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
         if (load_vtx(b, q, vn)) {
           vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
           const Bsdf bn = make_bsdf(sc, vn.mat, vn.n, vn.back, vn.t);
-          if (di_phat<W, S>(sc, st, vn, bn, c.light, c.u, c.v, true, b.counters) > 0.0f) z += nr.M;
+          if (di_phat<W, S, INST>(sc, st, vn, bn, c.light, c.u, c.v, true, b.counters) > 0.0f) z += nr.M;
         }
       }
       if (c.phat > RESTIR_DI_EPSILON && z * c.wsum > RESTIR_DI_EPSILON) c.W = (1.0f / c.phat) * (1.0f / z * c.wsum);
@@ -925,7 +926,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
 }
 
 // restir_gi_temporal.wgsl:69-115 (M capped at gi_mcap, biased combination).
-template <int W, int S>
+template <int W, int S, bool INST>
 __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, SceneView sc, RestirBuffers b) {
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
@@ -954,7 +955,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
         const GiRes pr = b.gi_hist[p.cur ^ 1][qy * p.width + qx];
         const float pM = fminf(pr.mwsp.x, (float)p.gi_mcap);
         const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
-        const float pp = gi_phat<W, S>(sc, st, v, bs, f4v(t4), pr, b.counters);
+        const float pp = gi_phat<W, S, INST>(sc, st, v, bs, f4v(t4), pr, b.counters);
         // GiReservoir::combine (gi_reservoir.wgsl:66-78)
         float wsum = 0.0f, M = 0.0f, phat = 0.0f;
         GiRes c;
@@ -981,7 +982,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
 
 // apply_di.wgsl:48-111: the reservoir's sample, visibility tested, weighted by
 // W, with the throughput at the vertex.
-template <int W, int S>
+template <int W, int S, bool INST>
 __global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneView sc, RestirBuffers b) {
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
@@ -995,7 +996,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneVie
   const float c = dot(e.wi, v.n);
   if (!(c > 0.0f)) return;
   const LdsStack st{s_stack + threadIdx.x, BLOCK8};
-  if (!visible<W, S>(sc, st, v.x, v.n, e.wi, e.dist, b.counters)) return;
+  if (!visible<W, S, INST>(sc, st, v.x, v.n, e.wi, e.dist, b.counters)) return;
   const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
   float pdf;
   const v3 add = v.T * bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * (c * r.W);
@@ -1280,7 +1281,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_remod(RestirParams p, RestirBuffer
   if (fin != b.demod[p.cur]) b.demod[p.cur][pix] = c;
 }
 
-template <int W, int S>
+template <int W, int S, bool INST>
 apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
   const uint32_t tiles = ((p.width + TILE - 1) / TILE) * ((p.rows + TILE - 1) / TILE);
   const uint32_t tiles8 = ((p.width + TILE8 - 1) / TILE8) * ((p.rows + TILE8 - 1) / TILE8);
@@ -1291,27 +1292,27 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
     const uint32_t npix = p.width * p.rows;
     const dim3 pool(std::min<uint32_t>(pool_blocks(), (npix + 63) / 64));
     hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
-    hipLaunchKernelGGL((k_rs_trace<W, S, false, true>), pool, block, 0, s, sc, b, npix);
+    hipLaunchKernelGGL((k_rs_trace<W, S, false, true, INST>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rs_extend, grid, block, 0, s, p, sc, b, bounce);
     hipLaunchKernelGGL(k_rs_di_bsdf_ray, grid, block, 0, s, p, sc, b);  // also resets the pool counters
-    hipLaunchKernelGGL((k_rs_trace<W, S, false, false>), pool, block, 0, s, sc, b, npix);
+    hipLaunchKernelGGL((k_rs_trace<W, S, false, false, INST>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
     hipLaunchKernelGGL(k_rs_di_ris, grid8, block8, 0, s, p, sc, b);
-    hipLaunchKernelGGL((k_rs_trace<W, S, true, false>), pool, block, 0, s, sc, b, npix);
+    hipLaunchKernelGGL((k_rs_trace<W, S, true, false, INST>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rs_di_vis, dim3((npix + BLOCK - 1) / BLOCK), block, 0, s, b, npix);
-    if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S>), grid8, block8, 0, s, p, sc, b, bounce);
+    if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S, INST>), grid8, block8, 0, s, p, sc, b, bounce);
     if (bounce == 0 && p.restir_di) {
-      hipLaunchKernelGGL((k_rs_di_temporal<W, S>), grid8, block8, 0, s, p, sc, b);
+      hipLaunchKernelGGL((k_rs_di_temporal<W, S, INST>), grid8, block8, 0, s, p, sc, b);
       if (p.neighbours > 0) {
         hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
         hipLaunchKernelGGL(k_rs_di_self, grid, block, 0, s, p, sc, b);
-        hipLaunchKernelGGL((k_rs_trace<W, S, true, false>), pool, block, 0, s, sc, b, npix);
-        hipLaunchKernelGGL((k_rs_di_spatial<W, S>), grid8, block8, 0, s, p, sc, b);
+        hipLaunchKernelGGL((k_rs_trace<W, S, true, false, INST>), pool, block, 0, s, sc, b, npix);
+        hipLaunchKernelGGL((k_rs_di_spatial<W, S, INST>), grid8, block8, 0, s, p, sc, b);
       }
     }
     if (bounce == 0 && more && p.restir_gi)
-      hipLaunchKernelGGL((k_rs_gi_temporal<W, S>), grid8, block8, 0, s, p, sc, b);
-    hipLaunchKernelGGL((k_rs_apply_di<W, S>), grid8, block8, 0, s, p, sc, b);
+      hipLaunchKernelGGL((k_rs_gi_temporal<W, S, INST>), grid8, block8, 0, s, p, sc, b);
+    hipLaunchKernelGGL((k_rs_apply_di<W, S, INST>), grid8, block8, 0, s, p, sc, b);
     if (more) hipLaunchKernelGGL(k_rs_apply_gi, grid, block, 0, s, p, sc, b);
   }
   APT_HIP(hipGetLastError());
@@ -1320,17 +1321,33 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
 
 }  // namespace
 
+#ifndef APT_RESTIR_TWO_LEVEL
 apt_status restir_sample(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
+  if (sc.insts) return restir_sample_two_level(p, sc, b, s);
   switch (sc.width) {
-    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_>(p, sc, b, s)));
-    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_>(p, sc, b, s)));
-    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_>(p, sc, b, s)));
+    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_, false>(p, sc, b, s)));
+    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_, false>(p, sc, b, s)));
+    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_, false>(p, sc, b, s)));
     default: break;
   }
   set_error("restir: unsupported BVH width %u", sc.width);
   return APT_ERR_INVALID_ARG;
 }
+#else
+apt_status restir_sample_two_level(const RestirParams& p, const SceneView& sc, const RestirBuffers& b,
+                                   hipStream_t s) {
+  switch (sc.width) {
+    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_, true>(p, sc, b, s)));
+    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_, true>(p, sc, b, s)));
+    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_, true>(p, sc, b, s)));
+    default: break;
+  }
+  set_error("restir: unsupported BVH width %u", sc.width);
+  return APT_ERR_INVALID_ARG;
+}
+#endif
 
+#ifndef APT_RESTIR_TWO_LEVEL
 apt_status restir_post(const RestirParams& p, const RestirBuffers& b, bool firefly, bool svgf, bool taa,
                        float4* frame_sum, hipStream_t s) {
   const uint32_t tiles = ((p.width + TILE - 1) / TILE) * ((p.rows + TILE - 1) / TILE);
@@ -1355,5 +1372,7 @@ apt_status restir_post(const RestirParams& p, const RestirBuffers& b, bool firef
   APT_HIP(hipGetLastError());
   return APT_OK;
 }
+
+#endif  // APT_RESTIR_TWO_LEVEL
 
 }  // namespace apt

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "scene_gpu.h"
+#include "tlas.h"
 
 namespace apt {
 
@@ -47,14 +48,18 @@ __global__ void k_emit_flags(const float* __restrict__ xyz, const uint32_t* __re
   }
 }
 
-__global__ void k_flatten(const float* __restrict__ lxyz, const float* __restrict__ lnrm,
-                          const uint32_t* __restrict__ lmat, uint32_t n, ModelXform x,
-                          float* __restrict__ oxyz, float* __restrict__ onrm, uint32_t* __restrict__ omat) {
+// A model's vertices in world space: positions M p (((m0 x + m4 y) + m8 z) +
+// m12 per row, column-major M), normals through the inverse transpose of its
+// 3x3 part and renormalised, material ids offset by the model's material base.
+__global__ void k_place(const float* __restrict__ lxyz, const float* __restrict__ lnrm,
+                        const uint32_t* __restrict__ lmat, uint32_t n, ModelXform x, float4* __restrict__ opos,
+                        float4* __restrict__ onrm, uint32_t* __restrict__ omat) {
   const uint32_t nv = 3 * n;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += gridDim.x * blockDim.x) {
     const float* p = lxyz + 3 * (size_t)j;
-    float* o = oxyz + 3 * (size_t)j;
+    float o[3];
     for (int r = 0; r < 3; ++r) o[r] = x.m[r] * p[0] + x.m[4 + r] * p[1] + x.m[8 + r] * p[2] + x.m[12 + r];
+    opos[j] = make_float4(o[0], o[1], o[2], 0.0f);
     if (onrm) {
       const float* q = lnrm + 3 * (size_t)j;
       float v[3];
@@ -62,12 +67,26 @@ __global__ void k_flatten(const float* __restrict__ lxyz, const float* __restric
         v[r] = x.nm_ok ? x.nm[3 * r] * q[0] + x.nm[3 * r + 1] * q[1] + x.nm[3 * r + 2] * q[2] : q[r];
       const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       if (l > 0.0f) for (int r = 0; r < 3; ++r) v[r] /= l;
-      float* w = onrm + 3 * (size_t)j;
-      w[0] = v[0];
-      w[1] = v[1];
-      w[2] = v[2];
+      onrm[j] = make_float4(v[0], v[1], v[2], 0.0f);
     }
     if (j < n) omat[j] = x.mat_base + (lmat ? lmat[j] : 0u);
+  }
+}
+
+// k_emit_flags on float4 positions (the same float operations).
+__global__ void k_emit_flags4(const float4* __restrict__ pos, const uint32_t* __restrict__ mat,
+                              const apt_material* __restrict__ mats, uint8_t* __restrict__ flag, uint32_t n) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const apt_material& m = mats[mat[t]];
+    bool e = m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f;
+    if (e) {
+      const float4 a = pos[3 * (size_t)t], b = pos[3 * (size_t)t + 1], c = pos[3 * (size_t)t + 2];
+      const float e1[3] = {b.x - a.x, b.y - a.y, b.z - a.z}, e2[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
+      const float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                           e1[0] * e2[1] - e1[1] * e2[0]};
+      e = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] > 0.0f;
+    }
+    flag[t] = e ? 1 : 0;
   }
 }
 
@@ -182,12 +201,85 @@ apt_status scene_gpu_upload(const float* pos, const float* nrm, const uint32_t* 
   return st;
 }
 
-apt_status scene_flatten_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
-                               const ModelXform& x, float* out_xyz, float* out_nrm, uint32_t* out_mat) {
+apt_status scene_place_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
+                             const ModelXform& x, SceneGpu& s, uint32_t base, hipStream_t st) {
   if (n == 0) return APT_OK;
-  hipLaunchKernelGGL(k_flatten, dim3(grid_for((size_t)n * 3)), dim3(256), 0, nullptr, d_xyz,
-                     out_nrm ? d_nrm : nullptr, d_mat, n, x, out_xyz, out_nrm, out_mat);
-  return sync_null("model flatten");
+  APT_REQUIRE((size_t)base + n <= s.cap_tris, APT_ERR_INVALID_ARG, "scene placement: prims [%u, %u) past %zu",
+              base, base + n, s.cap_tris);
+  APT_REQUIRE(!s.d_nrm || d_nrm, APT_ERR_INVALID_ARG, "scene placement: the model has no normals");
+  hipLaunchKernelGGL(k_place, dim3(grid_for((size_t)n * 3)), dim3(256), 0, st, d_xyz, s.d_nrm ? d_nrm : nullptr,
+                     d_mat, n, x, s.d_pos + 3 * (size_t)base, s.d_nrm ? s.d_nrm + 3 * (size_t)base : nullptr,
+                     s.d_mat + base);
+  APT_HIP(hipGetLastError());
+  return APT_OK;
+}
+
+apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, const apt_material* mats, uint32_t n_mats,
+                         hipStream_t st) {
+  if (n > s.cap_tris || (with_normals && !s.d_nrm)) {
+    for (void* p : {(void*)s.d_pos, (void*)s.d_nrm, (void*)s.d_mat})
+      if (p) (void)hipFree(p);
+    s.d_pos = s.d_nrm = nullptr;
+    s.d_mat = nullptr;
+    s.cap_tris = 0;
+    apt_status r = alloc_dev((void**)&s.d_pos, (size_t)n * 3 * sizeof(float4));
+    if (r == APT_OK && with_normals) r = alloc_dev((void**)&s.d_nrm, (size_t)n * 3 * sizeof(float4));
+    if (r == APT_OK) r = alloc_dev((void**)&s.d_mat, (size_t)n * sizeof(uint32_t));
+    if (r != APT_OK) return r;
+    s.cap_tris = n;
+  } else if (!with_normals && s.d_nrm) {
+    (void)hipFree(s.d_nrm);
+    s.d_nrm = nullptr;
+  }
+  if (n_mats > s.cap_mats) {
+    if (s.d_mats) (void)hipFree(s.d_mats);
+    s.d_mats = nullptr;
+    s.cap_mats = 0;
+    apt_status r = alloc_dev((void**)&s.d_mats, (size_t)n_mats * sizeof(apt_material));
+    if (r != APT_OK) return r;
+    s.cap_mats = n_mats;
+  }
+  if (n_mats) APT_HIP(hipMemcpyAsync(s.d_mats, mats, (size_t)n_mats * sizeof(apt_material), hipMemcpyHostToDevice, st));
+  s.n_tris = n;
+  s.n_mats = n_mats;
+  s.bytes = s.cap_tris * (3 * sizeof(float4) * (s.d_nrm ? 2 : 1) + sizeof(uint32_t)) +
+            s.cap_mats * sizeof(apt_material) + s.cap_emissive * sizeof(uint32_t);
+  return APT_OK;
+}
+
+apt_status scene_emitters(SceneGpu& s, hipStream_t st) {
+  const uint32_t n = s.n_tris;
+  std::vector<uint8_t> flag(n);
+  if (n) {
+    uint8_t* d_flag = nullptr;
+    apt_status r = alloc_dev((void**)&d_flag, n);
+    if (r != APT_OK) return r;
+    hipLaunchKernelGGL(k_emit_flags4, dim3(grid_for(n)), dim3(256), 0, st, s.d_pos, s.d_mat, s.d_mats, d_flag, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(flag.data(), d_flag, n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_flag);
+    if (e != hipSuccess) {
+      set_error("scene emitters: %s", hipGetErrorString(e));
+      return APT_ERR_DEVICE;
+    }
+  }
+  std::vector<uint32_t> em;
+  for (uint32_t t = 0; t < n; ++t)
+    if (flag[t]) em.push_back(t);
+  if (em.size() > s.cap_emissive || !s.d_emissive) {
+    if (s.d_emissive) (void)hipFree(s.d_emissive);
+    s.d_emissive = nullptr;
+    s.cap_emissive = 0;
+    apt_status r = alloc_dev((void**)&s.d_emissive, std::max<size_t>(1, em.size()) * sizeof(uint32_t));
+    if (r != APT_OK) return r;
+    s.cap_emissive = std::max<size_t>(1, em.size());
+  }
+  if (!em.empty())
+    APT_HIP(hipMemcpyAsync(s.d_emissive, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  APT_HIP(hipStreamSynchronize(st));
+  s.n_emissive = (uint32_t)em.size();
+  return APT_OK;
 }
 
 void scene_gpu_free(SceneGpu& s) {
@@ -199,13 +291,25 @@ void scene_gpu_free(SceneGpu& s) {
 
 SceneView scene_gpu_view(const SceneGpu& s, const apt_camera& cam, uint32_t width, uint32_t height) {
   SceneView sc{};
-  sc.nodes = s.bvh ? s.bvh->d_nodes : nullptr;
-  sc.bvh_tris = s.bvh ? s.bvh->d_tris : nullptr;
-  sc.root = s.bvh ? s.bvh->root : 0;
-  sc.n_nodes = s.bvh ? s.bvh->n_nodes : 0;
-  sc.width = s.bvh ? s.bvh->width : 4;
-  sc.lds_stack = s.bvh ? s.bvh->lds_stack : 8;
-  sc.refill = s.bvh ? s.bvh->refill : 32;
+  if (s.tlas) {
+    sc.nodes = s.tlas->d_nodes;
+    sc.bvh_tris = s.tlas->d_tris;
+    sc.root = s.tlas->root;
+    sc.n_nodes = s.tlas->n_tlas_nodes;
+    sc.width = s.tlas->width;
+    sc.lds_stack = s.tlas->lds_stack;
+    sc.refill = s.tlas->refill;
+    sc.insts = s.tlas->d_inst;
+  } else {
+    sc.nodes = s.bvh ? s.bvh->d_nodes : nullptr;
+    sc.bvh_tris = s.bvh ? s.bvh->d_tris : nullptr;
+    sc.root = s.bvh ? s.bvh->root : 0;
+    sc.n_nodes = s.bvh ? s.bvh->n_nodes : 0;
+    sc.width = s.bvh ? s.bvh->width : 4;
+    sc.lds_stack = s.bvh ? s.bvh->lds_stack : 8;
+    sc.refill = s.bvh ? s.bvh->refill : 32;
+    sc.insts = nullptr;
+  }
   sc.tri_pos = s.d_pos;
   sc.tri_nrm = s.d_nrm;
   sc.tri_mat = s.d_mat;

@@ -8,6 +8,11 @@ namespace apt {
 
 struct SceneGpu {
   apt_bvh* bvh = nullptr;
+  // A two-level scene (spawned models) traces this instead of bvh; owned by
+  // the caller, and the attribute arrays below keep their capacity across
+  // world actions (cap_tris, cap_mats, cap_emissive).
+  apt_tlas* tlas = nullptr;
+  size_t cap_tris = 0, cap_mats = 0, cap_emissive = 0;
   float4* d_pos = nullptr;      // 3 per triangle
   float4* d_nrm = nullptr;      // 3 per triangle or null
   uint32_t* d_mat = nullptr;
@@ -41,10 +46,20 @@ struct ModelXform {
   uint32_t mat_base;
 };
 
-// Writes a model's n triangles in world space (positions, unit normals if
-// out_nrm, material ids + mat_base) on the device; synchronous.
-apt_status scene_flatten_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
-                               const ModelXform& x, float* out_xyz, float* out_nrm, uint32_t* out_mat);
+// Places a model's n resident triangles in world space in a two-level
+// scene's attribute arrays from prim `base` on: float4 positions, unit
+// normals (when the scene keeps normals), material ids + mat_base.
+// Asynchronous on stream st.
+apt_status scene_place_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
+                             const ModelXform& x, SceneGpu& s, uint32_t base, hipStream_t st);
+// Sizes a two-level scene's attribute arrays for n triangles (normals when
+// with_normals) and n_mats materials, keeping them when they are big enough,
+// and uploads the materials.
+apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, const apt_material* mats, uint32_t n_mats,
+                         hipStream_t st);
+// Rebuilds the emitter list of a two-level scene from its world positions
+// and materials; synchronous on st.
+apt_status scene_emitters(SceneGpu& s, hipStream_t st);
 
 // Kernel view of a scene seen through a CameraUpdate camera (rotation and
 // translation of the world transform, vertical fov, aspect = width/height).

@@ -425,7 +425,13 @@ apt_status apt_set_config(apt_renderer* r, const apt_config* cfg);
  * ty 0 CameraUpdate (80 B), 1 SpawnModel (336 B), 2 TransformModel (80 B),
  * 3 DestroyModel (16 B), 4 Clear (4 B).  SpawnModel accepts
  * "procedural://<name>?seed=<n>" asset paths and glTF files (.glb, .gltf;
- * apt_scene_load_gltf). */
+ * apt_scene_load_gltf).  Spawned models form a two-level scene as in the
+ * reference (scene_resources/mod.rs:115-276): an asset path is loaded once
+ * and its BLAS built once in object space (later spawns of the path share
+ * them, until Clear); every world action rebuilds the TLAS over the models
+ * (at most APT_TLAS_MAX_INSTANCES) and re-places their world-space
+ * attributes.  A singular model transform makes the next render return
+ * APT_ERR_UNSUPPORTED. */
 apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, size_t len);
 
 /* Replace the whole scene with a triangle soup (procedural or caller-made). */
@@ -489,8 +495,9 @@ typedef struct apt_render_stats {
   double ms_shade;
   double ms_shadow;
   double ms_other;
-  double ms_scene;           /* wall time of the last scene rebuild after a world action or
-                              * apt_scene_set: flatten, device BVH build, upload (0 if none yet) */
+  double ms_scene;           /* wall time of the last scene rebuild after a world action (BLAS builds
+                              * of newly loaded assets, TLAS, placement, emitter list) or apt_scene_set
+                              * (upload and device BVH build); 0 if none yet */
 } apt_render_stats;
 apt_status apt_get_render_stats(const apt_renderer* r, apt_render_stats* out);
 

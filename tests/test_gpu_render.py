@@ -210,7 +210,7 @@ def test_model_actions_equal_direct_spawn(gpu_available):
     """TransformModel and DestroyModel (scene_resources/mod.rs:159-276 rebuilds
     the TLAS after them) leave the renderer as if the surviving models had
     been spawned at their final transforms: the image is bit-identical, and
-    each action's scene rebuild (device BVH build) is timed in the stats."""
+    each action's scene rebuild (TLAS and placement) is timed in the stats."""
     import uuid
     cam = world.CameraUpdate(world.look_at((0.0, 1.0, 6.5), (0.0, 1.0, 0.0)), 50.0)
     ua, ub = uuid.UUID(int=1), uuid.UUID(int=2)
@@ -284,7 +284,10 @@ def test_spawn_transform_matches_numpy_placement(gpu_available):
     M p in float32 with the kernel's order of operations, vertex normals
     through the inverse transpose of M's 3x3 part and renormalised.  The
     scenes are the Disney spheres (vertex normals) and the Cornell box
-    (geometric normals)."""
+    (geometric normals).  The spawned model is traced in its object space
+    (a two-level scene) and the placed one in world space, so hit distances
+    differ in their last bits: the frames agree to relMSE 1e-6 (measured
+    about 3e-7), not bit for bit."""
     from appearance_amd.scene import SceneData
     a = np.radians(35.0)
     rot = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
