@@ -90,6 +90,16 @@ class MeshDesc(C.Structure):
                 ("n_materials", C.c_uint32), ("materials", C.POINTER(Material))]
 
 
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba8", C.POINTER(C.c_uint32))]
+
+
+class MeshAttribs(C.Structure):
+    _fields_ = [("uvs", C.POINTER(C.c_float)), ("tangents", C.POINTER(C.c_float)),
+                ("bitangents", C.POINTER(C.c_float)), ("textures", C.POINTER(Texture)),
+                ("n_textures", C.c_uint32), ("_pad", C.c_uint32)]
+
+
 class Camera(C.Structure):
     _fields_ = [("transform", C.c_float * 16), ("fov_deg", C.c_float), ("near_plane", C.c_float),
                 ("far_plane", C.c_float), ("_padding", C.c_uint32)]
@@ -147,6 +157,8 @@ _SIGS = {
     "apt_scene_generate": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     "apt_scene_data_view": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(Camera), C.POINTER(Sky)]),
     "apt_scene_data_destroy": (None, [_P]),
+    "apt_scene_data_attribs": (C.c_int, [_P, C.POINTER(MeshAttribs)]),
+    "apt_scene_set_attribs": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(MeshAttribs), C.POINTER(Sky)]),
     "apt_rays_bvh_stress": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "apt_config_default": (None, [C.POINTER(Config)]),
     "apt_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),

@@ -30,6 +30,7 @@ SOURCES = [
     "restir_two_level.hip",
     "bvh_gpu.hip",
     "gltf.cpp",
+    "image.cpp",
     "tlas.hip",
 ]
 

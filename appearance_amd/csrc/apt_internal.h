@@ -170,11 +170,21 @@ struct apt_bvh_impl;
 }  // namespace apt
 
 // A triangle soup with per-triangle materials (procedural scenes, glTF files).
+struct apt_scene_texture {
+  uint32_t width = 0, height = 0;
+  std::vector<uint32_t> rgba;  // RGBA8, row major
+};
+
 struct apt_scene_data {
   std::vector<float> positions;   // n_tris * 9
   std::vector<float> normals;     // n_tris * 9 or empty
   std::vector<uint32_t> mat_ids;  // n_tris
   std::vector<apt_material> materials;
+  std::vector<float> uvs;         // n_tris * 6 or empty
+  std::vector<float> tangents;    // n_tris * 12 (xyz, handedness) or empty
+  std::vector<float> bitangents;  // n_tris * 9 or empty (then cross(n, t) * w)
+  std::vector<apt_scene_texture> textures;
+  mutable std::vector<apt_texture> texture_views;  // views of `textures` for apt_scene_data_attribs
   apt_camera camera{};
   apt_sky sky{};
 };
@@ -182,6 +192,14 @@ struct apt_scene_data {
 namespace apt {
 // glTF ingestion (gltf.cpp): a .glb / .gltf file into a scene.
 apt_status gltf_load(const char* path, apt_scene_data** out);
+// PNG image -> RGBA8 (image.cpp); false with a reason in err.
+bool png_decode(const uint8_t* data, size_t n, uint32_t& w, uint32_t& h, std::vector<uint32_t>& rgba,
+                std::string& err);
+// Per-vertex tangents of an indexed mesh (Lengyel's method, the reference's
+// generate_tangents, crates/appearance-model/src/mesh.rs:82-160): 4 floats per
+// vertex (unit tangent orthogonalised against the normal, handedness +-1).
+void generate_tangents(const std::vector<float>& pos, const std::vector<float>& nrm, const std::vector<float>& uv,
+                       const std::vector<uint32_t>& idx, std::vector<float>& out);
 }  // namespace apt
 
 // The opaque handle of the C ABI.

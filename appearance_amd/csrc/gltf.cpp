@@ -5,16 +5,22 @@
 //
 // Per primitive (triangles only, indices required, as asset.rs:140-160):
 // POSITION, NORMAL (generated per vertex from the area-independent sum of
-// face normals when absent, mesh.rs:58-79), TEXCOORD_0 is read but not kept
-// (textures are not decoded: every material uses its factors).  Materials
-// follow asset.rs:199-339: base colour, metallic, roughness, emission times
+// face normals when absent, mesh.rs:58-79), TEXCOORD_0, TANGENT (generated
+// per vertex from positions, normals and uvs when absent, mesh.rs:82-160,
+// once some material has a normal map).  Materials follow asset.rs:199-339:
+// base colour, metallic, roughness, emission times
 // KHR_materials_emissive_strength, KHR_materials_volume absorption
 // (1 - attenuation colour) / attenuation distance, KHR_materials_transmission,
 // eta = 1 / KHR_materials_ior (1.5), KHR_materials_specular,
 // KHR_materials_clearcoat, KHR_materials_sheen (its roughness into `sheen`,
-// its colour into `sheen_tint`), alpha cutoff 0.5.  Node transforms are
-// matrix or translation * rotation * scale, composed from the default scene's
-// roots down; normals go through the inverse transpose and are renormalised.
+// its colour into `sheen_tint`), alpha cutoff 0.5, and their textures (base
+// colour, metallic-roughness, normal with its scale, emission, transmission,
+// clearcoat, clearcoat roughness, clearcoat normal, sheen roughness, sheen
+// colour), each image decoded once (PNG; other formats leave the slot empty).
+// Node transforms are matrix or translation * rotation * scale, composed from
+// the default scene's roots down; normals, tangents and bitangents (cross(n,
+// t) * w in the mesh's space) go through the inverse transpose and are
+// renormalised.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -235,6 +241,60 @@ struct Loader {
   std::string dir;  // for external buffer URIs
   std::string err;
   apt_scene_data* out = nullptr;
+  std::map<size_t, uint32_t> image_tex;  // image index -> scene texture (APT_INVALID_TEXTURE: not decoded)
+  bool need_tangents = false;            // some material has a normal or clearcoat normal map
+  bool any_uv = false;
+
+  // Bytes of image `idx`: a buffer view, a data URI or an external file.
+  bool image_bytes(size_t idx, std::vector<uint8_t>& bytes) {
+    const Json* im = doc.get("images") ? doc.get("images")->at(idx) : nullptr;
+    if (!im) return false;
+    if (const Json* bvj = im->get("bufferView")) {
+      const Json* bv = doc.get("bufferViews") ? doc.get("bufferViews")->at((size_t)bvj->num) : nullptr;
+      if (!bv) return false;
+      const size_t buf = (size_t)bv->number("buffer", 0), off = (size_t)bv->number("byteOffset", 0);
+      const size_t len = (size_t)bv->number("byteLength", 0);
+      if (buf >= buffers.size() || off + len > buffers[buf].size()) return false;
+      bytes.assign(buffers[buf].begin() + (long)off, buffers[buf].begin() + (long)(off + len));
+      return true;
+    }
+    const Json* uri = im->get("uri");
+    if (!uri) return false;
+    const std::string& u = uri->str;
+    if (u.rfind("data:", 0) == 0) {
+      const size_t comma = u.find(',');
+      if (comma == std::string::npos || u.find(";base64", 0) > comma) return false;
+      bytes = base64(u.substr(comma + 1));
+      return true;
+    }
+    std::ifstream f(dir + u, std::ios::binary);
+    if (!f) return false;
+    bytes.assign((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    return true;
+  }
+
+  // The scene texture of a textureInfo object ({"index": texture}), decoding
+  // its image on first use; APT_INVALID_TEXTURE when absent or undecodable.
+  uint32_t texture(const Json* info) {
+    if (!info) return APT_INVALID_TEXTURE;
+    const Json* ti = info->get("index");
+    const Json* tex = (ti && doc.get("textures")) ? doc.get("textures")->at((size_t)ti->num) : nullptr;
+    const Json* src = tex ? tex->get("source") : nullptr;
+    if (!src) return APT_INVALID_TEXTURE;
+    const size_t img = (size_t)src->num;
+    auto it = image_tex.find(img);
+    if (it != image_tex.end()) return it->second;
+    uint32_t id = APT_INVALID_TEXTURE;
+    std::vector<uint8_t> bytes;
+    apt_scene_texture t;
+    std::string e;
+    if (image_bytes(img, bytes) && png_decode(bytes.data(), bytes.size(), t.width, t.height, t.rgba, e)) {
+      id = (uint32_t)out->textures.size();
+      out->textures.push_back(std::move(t));
+    }
+    image_tex[img] = id;
+    return id;
+  }
 
   bool fail(const std::string& m) {
     if (err.empty()) err = m;
@@ -357,6 +417,8 @@ struct Loader {
         for (int i = 0; i < 3; ++i) r.color[i] = (float)(c->at(i) ? c->at(i)->num : 1.0);
       r.metallic = (float)pbr->number("metallicFactor", 1.0);
       r.roughness = (float)pbr->number("roughnessFactor", 1.0);
+      r.color_texture = texture(pbr->get("baseColorTexture"));
+      r.metallic_roughness_texture = texture(pbr->get("metallicRoughnessTexture"));
     } else {
       r.metallic = 1.0f;  // the glTF default of an absent pbrMetallicRoughness
     }
@@ -367,6 +429,11 @@ struct Loader {
         strength = (float)es->number("emissiveStrength", 1.0);
     if (const Json* e = m->get("emissiveFactor"))
       for (int i = 0; i < 3; ++i) r.emission[i] = (float)(e->at(i) ? e->at(i)->num : 0.0) * strength;
+    r.emission_texture = texture(m->get("emissiveTexture"));
+    if (const Json* nt = m->get("normalTexture")) {
+      r.normal_texture = texture(nt);
+      r.normal_scale = (float)nt->number("scale", 1.0);
+    }
     double ior = 1.5;
     if (ext) {
       if (const Json* v = ext->get("KHR_materials_volume")) {
@@ -377,8 +444,10 @@ struct Loader {
           r.absorption[i] = (float)((1.0 - c) / dist);
         }
       }
-      if (const Json* t = ext->get("KHR_materials_transmission"))
+      if (const Json* t = ext->get("KHR_materials_transmission")) {
         r.transmission = (float)t->number("transmissionFactor", 0.0);
+        r.transmission_texture = texture(t->get("transmissionTexture"));
+      }
       if (const Json* i = ext->get("KHR_materials_ior")) ior = i->number("ior", 1.5);
       if (const Json* sp = ext->get("KHR_materials_specular")) {
         r.specular = (float)sp->number("specularFactor", 1.0);
@@ -390,11 +459,16 @@ struct Loader {
       if (const Json* cc = ext->get("KHR_materials_clearcoat")) {
         r.clearcoat = (float)cc->number("clearcoatFactor", 0.0);
         r.clearcoat_roughness = (float)cc->number("clearcoatRoughnessFactor", 0.0);
+        r.clearcoat_texture = texture(cc->get("clearcoatTexture"));
+        r.clearcoat_roughness_texture = texture(cc->get("clearcoatRoughnessTexture"));
+        r.clearcoat_normal_texture = texture(cc->get("clearcoatNormalTexture"));
       }
       if (const Json* sh = ext->get("KHR_materials_sheen")) {
         r.sheen = (float)sh->number("sheenRoughnessFactor", 0.0);
         if (const Json* c = sh->get("sheenColorFactor"))
           for (int k = 0; k < 3; ++k) r.sheen_tint[k] = (float)(c->at(k) ? c->at(k)->num : 0.0);
+        r.sheen_texture = texture(sh->get("sheenRoughnessTexture"));
+        r.sheen_tint_texture = texture(sh->get("sheenColorTexture"));
       }
     }
     r.eta = (float)(1.0 / ior);
@@ -420,16 +494,21 @@ struct Loader {
       if (!pos) return fail("a primitive without POSITION");
       const Json* ind = p.get("indices");
       if (!ind) return fail("a primitive without indices (the reference requires them)");
-      std::vector<float> P, N, UV;
+      std::vector<float> P, N, UV, TG;
       std::vector<uint32_t> I;
       if (!read_floats((int)pos->num, 3, P) || !read_indices((int)ind->num, I)) return false;
       if (const Json* n = attrs->get("NORMAL"))
         if (!read_floats((int)n->num, 3, N)) return false;
       if (const Json* t = attrs->get("TEXCOORD_0"))
         if (!read_floats((int)t->num, 2, UV)) return false;
+      if (const Json* t = attrs->get("TANGENT"))
+        if (need_tangents && !read_floats((int)t->num, 4, TG)) return false;
       const size_t nv = P.size() / 3;
       for (uint32_t i : I)
         if (i >= nv) return fail("index out of range");
+      if (!UV.empty() && UV.size() != nv * 2) return fail("TEXCOORD_0 count differs from POSITION");
+      if (!TG.empty() && TG.size() != nv * 4) return fail("TANGENT count differs from POSITION");
+      any_uv = any_uv || !UV.empty();
       if (N.empty()) {  // mesh.rs:58-79: sum of unit face normals per vertex
         std::vector<double> acc(nv * 3, 0.0);
         for (size_t t = 0; t + 2 < I.size(); t += 3) {
@@ -451,20 +530,42 @@ struct Loader {
           for (int k = 0; k < 3; ++k) N[3 * v + k] = l > 0.0 ? (float)(acc[3 * v + k] / l) : 0.0f;
         }
       }
+      if (need_tangents && TG.empty()) {
+        std::vector<float> uv0 = UV;
+        if (uv0.empty()) uv0.assign(nv * 2, 0.0f);
+        generate_tangents(P, N, uv0, I, TG);
+      }
       const uint32_t mat = (uint32_t)p.number("material", 0);
       if (mat >= out->materials.size()) return fail("material index out of range");
+      // a vector through the inverse transpose, renormalised
+      const auto itx = [&](const double* n, std::vector<float>& dst) {
+        const double w[3] = {sg * (c00 * n[0] + c01 * n[1] + c02 * n[2]), sg * (c10 * n[0] + c11 * n[1] + c12 * n[2]),
+                             sg * (c20 * n[0] + c21 * n[1] + c22 * n[2])};
+        const double l = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        for (int r = 0; r < 3; ++r) dst.push_back(l > 0.0 ? (float)(w[r] / l) : 0.0f);
+      };
       for (size_t t = 0; t + 2 < I.size(); t += 3) {
         for (int j = 0; j < 3; ++j) {
-          const float* v = &P[3 * I[t + j]];
+          const uint32_t vi = I[t + j];
+          const float* v = &P[3 * vi];
           for (int r = 0; r < 3; ++r)
             out->positions.push_back(
                 (float)(a[r] * v[0] + a[4 + r] * v[1] + a[8 + r] * v[2] + a[12 + r]));
-          const float* n = &N[3 * I[t + j]];
-          const double w[3] = {sg * (c00 * n[0] + c01 * n[1] + c02 * n[2]),
-                               sg * (c10 * n[0] + c11 * n[1] + c12 * n[2]),
-                               sg * (c20 * n[0] + c21 * n[1] + c22 * n[2])};
-          const double l = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-          for (int r = 0; r < 3; ++r) out->normals.push_back(l > 0.0 ? (float)(w[r] / l) : 0.0f);
+          const float* n = &N[3 * vi];
+          const double nd[3] = {n[0], n[1], n[2]};
+          itx(nd, out->normals);
+          out->uvs.push_back(UV.empty() ? 0.0f : UV[2 * vi]);
+          out->uvs.push_back(UV.empty() ? 0.0f : UV[2 * vi + 1]);
+          if (need_tangents) {
+            const float* tg = &TG[4 * vi];
+            const double td[3] = {tg[0], tg[1], tg[2]};
+            // bitangent in the mesh's space (vertex_pool_bindings.wgsl:43-46)
+            const double bd[3] = {(nd[1] * td[2] - nd[2] * td[1]) * tg[3], (nd[2] * td[0] - nd[0] * td[2]) * tg[3],
+                                  (nd[0] * td[1] - nd[1] * td[0]) * tg[3]};
+            itx(td, out->tangents);
+            out->tangents.push_back(tg[3]);
+            itx(bd, out->bitangents);
+          }
         }
         out->mat_ids.push_back(mat);
       }
@@ -494,6 +595,9 @@ struct Loader {
     if (mats)
       for (const Json& m : mats->arr) out->materials.push_back(material(&m));
     if (out->materials.empty()) out->materials.push_back(material(nullptr));
+    for (const apt_material& m : out->materials)
+      need_tangents = need_tangents || m.normal_texture != APT_INVALID_TEXTURE ||
+                      m.clearcoat_normal_texture != APT_INVALID_TEXTURE;
     const Json* scenes = doc.get("scenes");
     const size_t si = (size_t)doc.number("scene", 0);
     const Json* sc = scenes ? scenes->at(si) : nullptr;
@@ -502,9 +606,18 @@ struct Loader {
       for (const Json& r : roots->arr)
         if (!node((size_t)r.num, identity(), 0)) return false;
     if (out->positions.size() / 9 >= (1u << 28)) return fail("more than 2^28 triangles");
+    if (!any_uv) out->uvs.clear();
     return true;
   }
 };
+
+float3 f3at(const std::vector<float>& v, size_t i) { return make_float3(v[3 * i], v[3 * i + 1], v[3 * i + 2]); }
+float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+float3 scale3(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+float3 cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
 
 bool read_file(const char* path, std::vector<uint8_t>& data) {
   std::ifstream f(path, std::ios::binary);
@@ -514,6 +627,49 @@ bool read_file(const char* path, std::vector<uint8_t>& data) {
 }
 
 }  // namespace
+
+void generate_tangents(const std::vector<float>& pos, const std::vector<float>& nrm, const std::vector<float>& uv,
+                       const std::vector<uint32_t>& idx, std::vector<float>& out) {
+  const size_t nv = pos.size() / 3;
+  std::vector<float3> tan1(nv, make_float3(0.0f, 0.0f, 0.0f)), tan2(nv, make_float3(0.0f, 0.0f, 0.0f));
+  for (size_t i = 0; i + 2 < idx.size(); i += 3) {
+    const uint32_t i1 = idx[i], i2 = idx[i + 1], i3 = idx[i + 2];
+    const float3 v1 = f3at(pos, i1), v2 = f3at(pos, i2), v3 = f3at(pos, i3);
+    const float x1 = v2.x - v1.x, x2 = v3.x - v1.x, y1 = v2.y - v1.y, y2 = v3.y - v1.y;
+    const float z1 = v2.z - v1.z, z2 = v3.z - v1.z;
+    const float s1 = uv[2 * i2] - uv[2 * i1], s2 = uv[2 * i3] - uv[2 * i1];
+    const float t1 = uv[2 * i2 + 1] - uv[2 * i1 + 1], t2 = uv[2 * i3 + 1] - uv[2 * i1 + 1];
+    const float rdiv = s1 * t2 - s2 * t1;
+    const float r = rdiv == 0.0f ? 0.0f : 1.0f / rdiv;
+    const float3 sdir = make_float3((t2 * x1 - t1 * x2) * r, (t2 * y1 - t1 * y2) * r, (t2 * z1 - t1 * z2) * r);
+    const float3 tdir = make_float3((s1 * x2 - s2 * x1) * r, (s1 * y2 - s2 * y1) * r, (s1 * z2 - s2 * z1) * r);
+    for (uint32_t k : {i1, i2, i3}) {
+      tan1[k] = make_float3(tan1[k].x + sdir.x, tan1[k].y + sdir.y, tan1[k].z + sdir.z);
+      tan2[k] = make_float3(tan2[k].x + tdir.x, tan2[k].y + tdir.y, tan2[k].z + tdir.z);
+    }
+  }
+  out.assign(nv * 4, 0.0f);
+  for (size_t i = 0; i < nv; ++i) {
+    const float3 n = f3at(nrm, i), t = tan1[i];
+    float3 xyz = sub3(t, scale3(n, dot3(n, t)));
+    const float l = std::sqrt(dot3(xyz, xyz));
+    if (l > 0.0f && std::isfinite(l)) {
+      xyz = scale3(xyz, 1.0f / l);
+    } else {
+      // degenerate uvs: any unit vector orthogonal to the normal (the
+      // reference's normalize of a zero vector gives NaN here)
+      const float3 a = std::fabs(n.x) > 0.9f ? make_float3(0.0f, 1.0f, 0.0f) : make_float3(1.0f, 0.0f, 0.0f);
+      xyz = cross3(n, a);
+      const float la = std::sqrt(dot3(xyz, xyz));
+      xyz = la > 0.0f ? scale3(xyz, 1.0f / la) : make_float3(1.0f, 0.0f, 0.0f);
+    }
+    const float w = dot3(cross3(n, t), tan2[i]) < 0.0f ? -1.0f : 1.0f;
+    out[4 * i] = xyz.x;
+    out[4 * i + 1] = xyz.y;
+    out[4 * i + 2] = xyz.z;
+    out[4 * i + 3] = w;
+  }
+}
 
 apt_status gltf_load(const char* path, apt_scene_data** out) {
   std::vector<uint8_t> data;

@@ -28,6 +28,12 @@ struct FrameView {
   uint32_t rr_start;
 };
 
+// A texture of a scene: RGBA8 texels (row major, R in the low byte).
+struct TexDesc {
+  const uint32_t* texels;
+  uint32_t width, height;
+};
+
 // Scene resident in HBM.  Triangle attributes are in input (prim) order; the
 // BVH keeps its own copy of the positions in leaf order.
 struct SceneView {
@@ -41,6 +47,10 @@ struct SceneView {
   uint32_t refill;              // persistent traversal refill threshold (65: one ray per thread)
   const float4* tri_pos;        // 3 per triangle
   const float4* tri_nrm;        // 3 per triangle, or null (geometric normals)
+  const float2* tri_uv;         // 3 per triangle, or null (uv 0)
+  const float4* tri_tan;        // 3 per triangle (tangent), or null (no normal maps)
+  const float4* tri_bit;        // 3 per triangle (bitangent), or null
+  const TexDesc* tex;           // the scene's textures, or null (no textured material)
   const uint32_t* tri_mat;
   const apt_material* mats;
   const uint32_t* emissive;     // prims with emission

@@ -69,6 +69,167 @@ __device__ __forceinline__ bool has_emission(const apt_material& m) {
   return m.emission[0] > 0.0f || m.emission[1] > 0.0f || m.emission[2] > 0.0f;
 }
 
+// ---------------------------------------------------------------------------
+// Textured materials (shared/material/material_pool_bindings.wgsl:19-154).
+// ---------------------------------------------------------------------------
+
+// Bilinear sample with repeat addressing at level 0 (the reference's sampler,
+// src/scene_resources/material_pool.rs:68-74): texel centres at (i + 1/2) / W,
+// RGBA8 texels as c / 255.
+__device__ __forceinline__ float4 tex_sample(const SceneView& sc, uint32_t id, float u, float v) {
+  const TexDesc d = sc.tex[id];
+  const float x = (u == u ? u : 0.0f) * (float)d.width - 0.5f, y = (v == v ? v : 0.0f) * (float)d.height - 0.5f;
+  const float fx = fminf(fmaxf(floorf(x), -1099511627776.0f), 1099511627776.0f);  // +-2^40
+  const float fy = fminf(fmaxf(floorf(y), -1099511627776.0f), 1099511627776.0f);
+  const float ax = fminf(fmaxf(x - fx, 0.0f), 1.0f), ay = fminf(fmaxf(y - fy, 0.0f), 1.0f);
+  const int64_t W = d.width, H = d.height;
+  const int64_t ix = (int64_t)fx, iy = (int64_t)fy;
+  const uint32_t x0 = (uint32_t)(((ix % W) + W) % W), x1 = (uint32_t)((((ix + 1) % W) + W) % W);
+  const uint32_t y0 = (uint32_t)(((iy % H) + H) % H), y1 = (uint32_t)((((iy + 1) % H) + H) % H);
+  const auto texel = [&](uint32_t tx, uint32_t ty) {
+    const uint32_t c = d.texels[(size_t)ty * d.width + tx];
+    return make_float4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
+                       (float)((c >> 16) & 0xFFu) / 255.0f, (float)(c >> 24) / 255.0f);
+  };
+  const float4 c00 = texel(x0, y0), c10 = texel(x1, y0), c01 = texel(x0, y1), c11 = texel(x1, y1);
+  const float bx = 1.0f - ax, by = 1.0f - ay;
+  return make_float4((c00.x * bx + c10.x * ax) * by + (c01.x * bx + c11.x * ax) * ay,
+                     (c00.y * bx + c10.y * ax) * by + (c01.y * bx + c11.y * ax) * ay,
+                     (c00.z * bx + c10.z * ax) * by + (c01.z * bx + c11.z * ax) * ay,
+                     (c00.w * bx + c10.w * ax) * by + (c01.w * bx + c11.w * ax) * ay);
+}
+
+// srgb_to_linear (assets/shaders/color.wgsl:9-15), per channel.
+__device__ __forceinline__ float srgb_linear(float c) {
+  return c < 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+}
+
+// A material's factors at a surface point: the descriptor's values scaled by
+// its textures where it has them (MaterialDescriptor::color, emission,
+// metallic_roughness, clearcoat, clearcoat_roughness, transmission, sheen,
+// sheen_tint).
+struct MatEval {
+  v3 color;
+  float luminance;  // the colour's alpha
+  float metallic, roughness;
+  v3 emission;
+  float transmission, sheen;
+  v3 sheen_tint;
+  float clearcoat, clearcoat_roughness;
+};
+
+__device__ __forceinline__ MatEval mat_eval(const SceneView& sc, const apt_material& m, float2 uv) {
+  MatEval e;
+  e.color = mk(m.color[0], m.color[1], m.color[2]);
+  e.luminance = 1.0f;
+  e.metallic = m.metallic;
+  e.roughness = m.roughness;
+  e.emission = mk(m.emission[0], m.emission[1], m.emission[2]);
+  e.transmission = m.transmission;
+  e.sheen = m.sheen;
+  e.sheen_tint = mk(m.sheen_tint[0], m.sheen_tint[1], m.sheen_tint[2]);
+  e.clearcoat = m.clearcoat;
+  e.clearcoat_roughness = m.clearcoat_roughness;
+  if (!sc.tex) return e;
+  if (m.color_texture != APT_INVALID_TEXTURE) {
+    const float4 t = tex_sample(sc, m.color_texture, uv.x, uv.y);
+    e.color = e.color * mk(srgb_linear(t.x), srgb_linear(t.y), srgb_linear(t.z));
+    e.luminance = t.w;
+  }
+  if (m.emission_texture != APT_INVALID_TEXTURE && dot(e.emission, e.emission) > 0.0f) {
+    const float4 t = tex_sample(sc, m.emission_texture, uv.x, uv.y);
+    e.emission = e.emission * mk(t.x, t.y, t.z);
+  }
+  if (m.metallic_roughness_texture != APT_INVALID_TEXTURE && (e.metallic > 0.0f || e.roughness > 0.0f)) {
+    const float4 t = tex_sample(sc, m.metallic_roughness_texture, uv.x, uv.y);
+    e.metallic *= t.z;
+    e.roughness *= t.y;
+  }
+  if (m.clearcoat_texture != APT_INVALID_TEXTURE && e.clearcoat > 0.0f)
+    e.clearcoat *= tex_sample(sc, m.clearcoat_texture, uv.x, uv.y).x;
+  if (m.clearcoat_roughness_texture != APT_INVALID_TEXTURE && e.clearcoat_roughness > 0.0f)
+    e.clearcoat_roughness *= tex_sample(sc, m.clearcoat_roughness_texture, uv.x, uv.y).y;
+  if (m.transmission_texture != APT_INVALID_TEXTURE && e.transmission > 0.0f)
+    e.transmission *= tex_sample(sc, m.transmission_texture, uv.x, uv.y).x;
+  if (m.sheen_texture != APT_INVALID_TEXTURE && e.sheen > 0.0f)
+    e.sheen *= tex_sample(sc, m.sheen_texture, uv.x, uv.y).x;
+  if (m.sheen_tint_texture != APT_INVALID_TEXTURE && dot(e.sheen_tint, e.sheen_tint) > 0.0f) {
+    const float4 t = tex_sample(sc, m.sheen_tint_texture, uv.x, uv.y);
+    e.sheen_tint = e.sheen_tint * mk(srgb_linear(t.x), srgb_linear(t.y), srgb_linear(t.z));
+  }
+  return e;
+}
+
+// Emission of a material at a surface point (MaterialDescriptor::emission).
+__device__ __forceinline__ v3 mat_emission(const SceneView& sc, const apt_material& m, float2 uv) {
+  v3 e = mk(m.emission[0], m.emission[1], m.emission[2]);
+  if (sc.tex && m.emission_texture != APT_INVALID_TEXTURE && dot(e, e) > 0.0f) {
+    const float4 t = tex_sample(sc, m.emission_texture, uv.x, uv.y);
+    e = e * mk(t.x, t.y, t.z);
+  }
+  return e;
+}
+
+// DisneyBsdf::from_material (disney_bsdf.wgsl:226-244) of the evaluated factors.
+__device__ __forceinline__ Disney disney_from_eval(const apt_material& m, const MatEval& e) {
+  Disney d = disney_from_material(m);
+  d.color = e.color;
+  d.metallic = e.metallic;
+  d.luminance = e.luminance;
+  d.specular = fmaxf(m.specular, 1.0f - e.roughness);
+  d.roughness = e.roughness;
+  d.sheen = e.sheen;
+  d.sheen_tint = e.sheen_tint;
+  d.clearcoat = e.clearcoat;
+  d.clearcoat_gloss = 1.0f - e.clearcoat_roughness;
+  d.transmission = e.transmission;
+  return d;
+}
+
+// Interpolated uv of a hit (0 without uvs); bu, bv weight vertices 1 and 2.
+__device__ __forceinline__ float2 hit_uv(const SceneView& sc, uint32_t prim, float bu, float bv) {
+  if (!sc.tri_uv) return make_float2(0.0f, 0.0f);
+  const float2* t = sc.tri_uv + 3 * (size_t)prim;
+  const float b0 = 1.0f - bu - bv;
+  return make_float2(t[0].x * b0 + t[1].x * bu + t[2].x * bv, t[0].y * b0 + t[1].y * bu + t[2].y * bv);
+}
+
+// Normal mapping (MaterialDescriptor::apply_normal_mapping and
+// apply_clearcoat_normal_mapping, material_pool_bindings.wgsl:89-123, in the
+// frame of VertexPoolBindings::load_tbn, vertex_pool_bindings.wgsl:48-58):
+// n is the interpolated vertex normal before the face flip.  n_map bends it
+// by the normal texture; n_cc is the clearcoat lobe's normal -- the clearcoat
+// normal texture's, n_map when both textures are the same, else n itself
+// (trace.wgsl:176-191).
+__device__ __forceinline__ void map_normals(const SceneView& sc, const apt_material& m, uint32_t prim, float bu,
+                                            float bv, float2 uv, v3 n, v3& n_map, v3& n_cc) {
+  n_map = n;
+  n_cc = n;
+  if (!sc.tex || !sc.tri_tan) return;
+  const bool nt = m.normal_texture != APT_INVALID_TEXTURE && m.normal_scale > 0.0f;
+  const bool ct = m.clearcoat_normal_texture != APT_INVALID_TEXTURE;
+  if (!nt && !ct) return;
+  const float b0 = 1.0f - bu - bv;
+  const float4* tt = sc.tri_tan + 3 * (size_t)prim;
+  const float4* tb = sc.tri_bit + 3 * (size_t)prim;
+  const v3 T = normalize(ld3(tt[0]) * b0 + ld3(tt[1]) * bu + ld3(tt[2]) * bv);
+  const v3 B = normalize(ld3(tb[0]) * b0 + ld3(tb[1]) * bu + ld3(tb[2]) * bv);
+  if (nt) {
+    const float4 t = tex_sample(sc, m.normal_texture, uv.x, uv.y);
+    v3 nm = normalize(T * (t.x * 2.0f - 1.0f) + B * (t.y * 2.0f - 1.0f) + n * (t.z * 2.0f - 1.0f));
+    if (m.normal_scale < 1.0f) nm = normalize(n * (1.0f - m.normal_scale) + nm * m.normal_scale);
+    n_map = nm;
+  }
+  if (ct) {
+    if (m.clearcoat_normal_texture == m.normal_texture) {
+      n_cc = n_map;
+    } else {
+      const float4 t = tex_sample(sc, m.clearcoat_normal_texture, uv.x, uv.y);
+      n_cc = normalize(T * (t.x * 2.0f - 1.0f) + B * (t.y * 2.0f - 1.0f) + n * (t.z * 2.0f - 1.0f));
+    }
+  }
+}
+
 __device__ __forceinline__ float sun_pick(const SceneView& sc) {
   if (!(sc.sky.sun_enabled > 0.0f)) return 0.0f;
   return sc.n_emissive > 0 ? 0.5f : 1.0f;  // Nee::sample_uniform (nee.wgsl:102-115)
@@ -141,11 +302,21 @@ __device__ __forceinline__ void shade_path(const SceneView& sc, const PathView& 
       ns = normalize(ld3(tn[0]) * (1.0f - bu - bv) + ld3(tn[1]) * bu + ld3(tn[2]) * bv);
     }
     const v3 wo = -rd;
+    // shading normals: normal mapping on the interpolated normal, then the
+    // face flip of all of them (trace.wgsl:143-191)
+    const float2 uv = hit_uv(sc, prim, bu, bv);
+    v3 n_map, n_cc;
+    map_normals(sc, mat, prim, bu, bv, uv, ns, n_map, n_cc);
     const bool back_face = dot(wo, ns) < 0.0f;
-    if (back_face) ns = -ns;
+    if (back_face) {
+      n_map = -n_map;
+      n_cc = -n_cc;
+    }
+    ns = n_map;
+    const MatEval me = mat_eval(sc, mat, uv);
     // Emission (one-sided: the emitter's front is its winding normal).
     if (has_emission(mat) && dot(ng, rd) < 0.0f) {
-      v3 Le = mk(mat.emission[0], mat.emission[1], mat.emission[2]);
+      v3 Le = me.emission;
       float w = 1.0f;
       if (pdf_prev > 0.0f && (flags & FLAG_NEE_SIDE) && sc.n_emissive > 0) {
         float cos_l = -dot(ng, rd);
@@ -155,11 +326,11 @@ __device__ __forceinline__ void shade_path(const SceneView& sc, const PathView& 
       L += T * Le * w;
     }
     const bool lambert = mat.bsdf_model == APT_BSDF_LAMBERT;
-    const v3 albedo = mk(mat.color[0], mat.color[1], mat.color[2]);
-    const Disney bd = disney_from_material(mat);
+    const v3 albedo = me.color;
+    const Disney bd = disney_from_eval(mat, me);
     ShadeFrame sf;
     sf.f = onb(ns);
-    sf.cc = sf.f;
+    sf.cc = sc.tri_tan ? onb(n_cc) : sf.f;
 
     // Next-event estimation: one light sample (sun with probability p_sun,
     // else a uniformly chosen emissive triangle, uniform in area), pdf in
@@ -190,7 +361,7 @@ __device__ __forceinline__ void shade_path(const SceneView& sc, const PathView& 
         wi = dv / dist;
         const float cos_l = -dot(nl, wi);
         const apt_material& lm = sc.mats[sc.tri_mat[lp]];
-        Le = mk(lm.emission[0], lm.emission[1], lm.emission[2]);
+        Le = mat_emission(sc, lm, hit_uv(sc, lp, su * (1.0f - u_b), su * u_b));
         p_l = cos_l > 0.0f ? (1.0f - p_sun) / (float)sc.n_emissive / tri_area(lg) * (dist * dist) / cos_l : 0.0f;
       }
       const float cos_i = dot(wi, ns);

@@ -38,10 +38,16 @@ struct Asset {
   float* d_xyz = nullptr;
   float* d_nrm = nullptr;
   uint32_t* d_mat = nullptr;
+  float2* d_uv = nullptr;      // per vertex, or null
+  float4* d_tan = nullptr;     // per vertex (xyz, handedness), or null
+  float4* d_bit = nullptr;     // per vertex, or null
+  uint32_t* d_texels = nullptr;       // the asset's textures, one after another
+  std::vector<TexDesc> tex;           // their descriptors (into d_texels)
   uint32_t blas = INVALID;  // BLAS index in the renderer's two-level scene
   ~Asset() {
     apt_scene_data_destroy(data);
-    for (void* p : {(void*)d_xyz, (void*)d_nrm, (void*)d_mat})
+    for (void* p : {(void*)d_xyz, (void*)d_nrm, (void*)d_mat, (void*)d_uv, (void*)d_tan, (void*)d_bit,
+                    (void*)d_texels})
       if (p) (void)hipFree(p);
   }
 };
@@ -77,13 +83,50 @@ apt_status asset_resident(Asset& a) {
   apt_status st = put(a.d_xyz, md.positions, 9 * n);
   if (st == APT_OK && md.normals) st = put(a.d_nrm, md.normals, 9 * n);
   if (st == APT_OK) st = put(a.d_mat, ids.data(), n);
+  // uvs, tangent frames and textures (glTF assets)
+  const apt_scene_data& d = *a.data;
+  if (st == APT_OK && !d.uvs.empty()) st = put(a.d_uv, reinterpret_cast<const float2*>(d.uvs.data()), 3 * n);
+  if (st == APT_OK && !d.tangents.empty())
+    st = put(a.d_tan, reinterpret_cast<const float4*>(d.tangents.data()), 3 * n);
+  if (st == APT_OK && !d.bitangents.empty()) {
+    std::vector<float4> b(3 * n);
+    for (size_t j = 0; j < 3 * n; ++j)
+      b[j] = make_float4(d.bitangents[3 * j], d.bitangents[3 * j + 1], d.bitangents[3 * j + 2], 0.0f);
+    st = put(a.d_bit, b.data(), 3 * n);
+  }
+  if (st == APT_OK && !d.textures.empty()) {
+    std::vector<uint32_t> texels;
+    std::vector<size_t> off;
+    for (const apt_scene_texture& t : d.textures) {
+      off.push_back(texels.size());
+      texels.insert(texels.end(), t.rgba.begin(), t.rgba.end());
+    }
+    st = put(a.d_texels, texels.data(), texels.size());
+    a.tex.clear();
+    for (size_t k = 0; st == APT_OK && k < d.textures.size(); ++k)
+      a.tex.push_back(TexDesc{a.d_texels + off[k], d.textures[k].width, d.textures[k].height});
+  }
   if (st != APT_OK) {
-    for (void* p : {(void*)a.d_xyz, (void*)a.d_nrm, (void*)a.d_mat})
+    for (void* p : {(void*)a.d_xyz, (void*)a.d_nrm, (void*)a.d_mat, (void*)a.d_uv, (void*)a.d_tan, (void*)a.d_bit,
+                    (void*)a.d_texels})
       if (p) (void)hipFree(p);
     a.d_xyz = a.d_nrm = nullptr;
     a.d_mat = nullptr;
+    a.d_uv = nullptr;
+    a.d_tan = a.d_bit = nullptr;
+    a.d_texels = nullptr;
+    a.tex.clear();
   }
   return st;
+}
+
+// A material's texture indices offset by the model's first texture in the
+// scene's texture table.
+void offset_textures(apt_material& m, uint32_t base) {
+  for (uint32_t* t : {&m.color_texture, &m.metallic_roughness_texture, &m.normal_texture, &m.emission_texture,
+                      &m.sheen_texture, &m.clearcoat_texture, &m.clearcoat_roughness_texture,
+                      &m.sheen_tint_texture, &m.clearcoat_normal_texture, &m.transmission_texture})
+    if (*t != APT_INVALID_TEXTURE) *t += base;
 }
 
 // Page-locked host memory, so the frame's readback runs at the link's rate
@@ -140,7 +183,7 @@ struct DevBuf {
 
 // Per-pixel state of the ReSTIR integrator for one slab.
 struct RestirState {
-  DevBuf<float4> ray_o, ray_d, thr, radiance, vtx_p, vtx_n, vtx_t, demod[2], work, col[2], hist;
+  DevBuf<float4> ray_o, ray_d, thr, radiance, vtx_p, vtx_n, vtx_t, vtx_m, demod[2], work, col[2], hist;
   DevBuf<float4> rq_o, rq_d, rq_hit;
   DevBuf<uint32_t> rng, count, counters, rq_ctr;
   DevBuf<DiRes> di, di_tmp, di_hist[2];
@@ -153,7 +196,7 @@ struct RestirState {
 
   template <typename F>
   void each(F&& f) {
-    f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(demod[0]); f(demod[1]);
+    f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(vtx_m); f(demod[0]); f(demod[1]);
     f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
     f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
     f(guide); f(di_self); f(rq_o); f(rq_d); f(rq_hit);
@@ -182,7 +225,7 @@ struct RestirState {
   RestirBuffers view() {
     RestirBuffers b{};
     b.ray_o = ray_o.p; b.ray_d = ray_d.p; b.thr = thr.p; b.rng = rng.p; b.radiance = radiance.p;
-    b.vtx_p = vtx_p.p; b.vtx_n = vtx_n.p; b.vtx_t = vtx_t.p;
+    b.vtx_p = vtx_p.p; b.vtx_n = vtx_n.p; b.vtx_t = vtx_t.p; b.vtx_m = vtx_m.p;
     b.di = di.p; b.di_tmp = di_tmp.p; b.di_hist[0] = di_hist[0].p; b.di_hist[1] = di_hist[1].p;
     b.di_self = di_self.p;
     b.gi = gi.p; b.gi_hist[0] = gi_hist[0].p; b.gi_hist[1] = gi_hist[1].p;
@@ -228,7 +271,8 @@ struct apt_renderer {
   std::map<std::string, std::shared_ptr<Asset>> assets;
   apt_tlas* tlas = nullptr;
   bool emitters_dirty = true;
-  std::vector<float> pos, nrm;
+  std::vector<float> pos, nrm, uvs, tangents, bitangents;
+  std::vector<apt_scene_texture> textures;
   std::vector<uint32_t> mat_ids;
   std::vector<apt_material> mats;
   apt_sky sky{};
@@ -340,7 +384,7 @@ apt_status upload_instances(apt_renderer* r) {
   }
   r->tlas->lds_stack = c.lds_stack_depth;
   size_t total = 0;
-  bool all_normals = !r->models.empty();
+  bool all_normals = !r->models.empty(), any_uv = false, any_tan = false;
   for (Model& m : r->models) {
     Asset& a = *m.asset;
     if ((st = asset_resident(a)) != APT_OK) return st;
@@ -354,6 +398,8 @@ apt_status upload_instances(apt_renderer* r) {
       if (st != APT_OK) return st;
     }
     all_normals = all_normals && a.d_nrm != nullptr;
+    any_uv = any_uv || a.d_uv != nullptr;
+    any_tan = any_tan || a.d_tan != nullptr;
     total += a.n_tris;
   }
   APT_REQUIRE(total < (1u << 28), APT_ERR_INVALID_ARG, "scene: %zu triangles exceed 2^28", total);
@@ -364,6 +410,12 @@ apt_status upload_instances(apt_renderer* r) {
   std::vector<apt_instance> inst(r->models.size());
   std::vector<ModelXform> place(r->models.size());
   r->mats.clear();
+  // texture table: each asset's textures once, in the order of first use
+  std::vector<TexDesc> tex;
+  std::map<const Asset*, uint32_t> tex_base;
+  for (const Model& m : r->models)
+    if (!m.asset->tex.empty() && tex_base.emplace(m.asset.get(), (uint32_t)tex.size()).second)
+      tex.insert(tex.end(), m.asset->tex.begin(), m.asset->tex.end());
   uint32_t base = 0;
   for (size_t k = 0; k < r->models.size(); ++k) {
     const Model& m = r->models[k];
@@ -377,17 +429,28 @@ apt_status upload_instances(apt_renderer* r) {
     x.mat_base = (uint32_t)r->mats.size();
     apt_mesh_desc md;
     apt_scene_data_view(m.asset->data, &md, nullptr, nullptr);
-    r->mats.insert(r->mats.end(), md.materials, md.materials + md.n_materials);
+    const auto tb = tex_base.find(m.asset.get());
+    for (uint32_t k = 0; k < md.n_materials; ++k) {
+      apt_material mt = md.materials[k];
+      if (tb != tex_base.end()) offset_textures(mt, tb->second);
+      r->mats.push_back(mt);
+    }
     base += m.asset->n_tris;
   }
   if ((st = apt_tlas_set_instances(r->tlas, inst.data(), (uint32_t)inst.size())) != APT_OK) return st;
-  if ((st = scene_reserve(r->gpu, (uint32_t)total, all_normals, r->mats.data(), (uint32_t)r->mats.size(), s)) !=
-      APT_OK)
+  // tangent frames need every model's normals
+  any_tan = any_tan && all_normals;
+  if ((st = scene_reserve(r->gpu, (uint32_t)total, all_normals, any_uv, any_tan, r->mats.data(),
+                          (uint32_t)r->mats.size(), tex.data(), (uint32_t)tex.size(), s)) != APT_OK)
     return st;
   base = 0;
   for (size_t k = 0; k < r->models.size(); ++k) {
     const Asset& a = *r->models[k].asset;
-    if ((st = scene_place_model(a.d_xyz, a.d_nrm, a.d_mat, a.n_tris, place[k], r->gpu, base, s)) != APT_OK)
+    ModelAttribs at;
+    at.uv = a.d_uv;
+    at.tan = a.d_tan;
+    at.bit = a.d_bit;
+    if ((st = scene_place_model(a.d_xyz, a.d_nrm, a.d_mat, a.n_tris, place[k], at, r->gpu, base, s)) != APT_OK)
       return st;
     base += a.n_tris;
   }
@@ -416,6 +479,16 @@ apt_status upload_scene(apt_renderer* r) {
       apt_status st = scene_gpu_upload(r->pos.data(), r->has_normals ? r->nrm.data() : nullptr,
                                        r->mat_ids.data(), n, r->mats.data(), (uint32_t)r->mats.size(),
                                        &r->sky, r->cfg.bvh_width, r->cfg.lds_stack_depth, r->gpu);
+      if (st != APT_OK) return st;
+      std::vector<apt_texture> tv;
+      for (const apt_scene_texture& t : r->textures) tv.push_back(apt_texture{t.width, t.height, t.rgba.data()});
+      apt_mesh_attribs at{};
+      at.uvs = r->uvs.empty() ? nullptr : r->uvs.data();
+      at.tangents = r->tangents.empty() ? nullptr : r->tangents.data();
+      at.bitangents = r->bitangents.empty() ? nullptr : r->bitangents.data();
+      at.textures = tv.empty() ? nullptr : tv.data();
+      at.n_textures = (uint32_t)tv.size();
+      st = scene_gpu_attribs(r->gpu, r->pos.data(), r->has_normals ? r->nrm.data() : nullptr, n, at);
       if (st != APT_OK) return st;
     }
   }
@@ -843,6 +916,11 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
 }
 
 apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_sky* sky) {
+  return apt_scene_set_attribs(r, mesh, nullptr, sky);
+}
+
+apt_status apt_scene_set_attribs(apt_renderer* r, const apt_mesh_desc* mesh, const apt_mesh_attribs* attribs,
+                                 const apt_sky* sky) {
   clear_error();
   APT_REQUIRE(r && mesh, APT_ERR_INVALID_ARG, "apt_scene_set: null argument");
   APT_REQUIRE(mesh->n_tris == 0 || mesh->positions, APT_ERR_INVALID_ARG, "apt_scene_set: no positions");
@@ -856,6 +934,28 @@ apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_s
   if (mesh->material_ids) r->mat_ids.assign(mesh->material_ids, mesh->material_ids + n);
   else r->mat_ids.assign(n, 0u);
   r->mats.assign(mesh->materials, mesh->materials + mesh->n_materials);
+  apt_mesh_attribs at{};
+  if (attribs) at = *attribs;
+  APT_REQUIRE(at.n_textures == 0 || at.textures, APT_ERR_INVALID_ARG, "apt_scene_set: n_textures without textures");
+  for (const apt_material& m : r->mats)
+    for (uint32_t t : {m.color_texture, m.metallic_roughness_texture, m.normal_texture, m.emission_texture,
+                       m.sheen_texture, m.clearcoat_texture, m.clearcoat_roughness_texture, m.sheen_tint_texture,
+                       m.clearcoat_normal_texture, m.transmission_texture})
+      APT_REQUIRE(t == APT_INVALID_TEXTURE || t < at.n_textures, APT_ERR_INVALID_ARG,
+                  "apt_scene_set: a material uses texture %u of %u", t, at.n_textures);
+  r->uvs.assign(at.uvs ? at.uvs : nullptr, at.uvs ? at.uvs + 6 * n : nullptr);
+  r->tangents.assign(at.tangents ? at.tangents : nullptr, at.tangents ? at.tangents + 12 * n : nullptr);
+  r->bitangents.assign(at.bitangents ? at.bitangents : nullptr, at.bitangents ? at.bitangents + 9 * n : nullptr);
+  r->textures.clear();
+  for (uint32_t k = 0; k < at.n_textures; ++k) {
+    const apt_texture& t = at.textures[k];
+    APT_REQUIRE(t.width > 0 && t.height > 0 && t.rgba8, APT_ERR_INVALID_ARG, "apt_scene_set: texture %u is empty", k);
+    apt_scene_texture c;
+    c.width = t.width;
+    c.height = t.height;
+    c.rgba.assign(t.rgba8, t.rgba8 + (size_t)t.width * t.height);
+    r->textures.push_back(std::move(c));
+  }
   if (sky) r->sky = *sky;
   else std::memset(&r->sky, 0, sizeof(r->sky));
   r->explicit_scene = true;

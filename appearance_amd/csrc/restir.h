@@ -74,6 +74,7 @@ struct RestirBuffers {
   float4* vtx_p;   // hit point, t (< 0: no vertex)
   float4* vtx_n;   // front-facing shading normal, material | back << 31 (bits)
   float4* vtx_t;   // throughput at the vertex
+  float4* vtx_m;   // textured scenes: uv and the clearcoat normal (oct30 bits) of the vertex
   // reservoirs
   DiRes* di;       // RIS output, then the reused reservoir that is shaded
   DiRes* di_tmp;   // temporal output

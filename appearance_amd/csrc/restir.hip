@@ -161,12 +161,14 @@ __device__ __forceinline__ bool reproject(const RestirParams& p, v3 x, float& px
 // ---------------------------------------------------------------------------
 struct Vtx {
   v3 x, n, wo, T;
+  v3 ncc;     // clearcoat lobe normal (n without clearcoat normal mapping)
+  float2 uv;  // texture coordinates (textured scenes)
   float t;
   uint32_t mat;
   bool back;
 };
 
-__device__ __forceinline__ bool load_vtx(const RestirBuffers& b, uint32_t i, Vtx& v) {
+__device__ __forceinline__ bool load_vtx(const SceneView& sc, const RestirBuffers& b, uint32_t i, Vtx& v) {
   const float4 p4 = b.vtx_p[i];
   if (!(p4.w > 0.0f)) return false;
   const float4 n4 = b.vtx_n[i];
@@ -178,6 +180,14 @@ __device__ __forceinline__ bool load_vtx(const RestirBuffers& b, uint32_t i, Vtx
   v.back = (bits >> 31) != 0u;
   v.wo = -f4v(b.ray_d[i]);
   v.T = f4v(b.vtx_t[i]);
+  if (sc.tex) {
+    const float4 m4 = b.vtx_m[i];
+    v.uv = make_float2(m4.x, m4.y);
+    v.ncc = sc.tri_tan ? unpack_oct30(f2u(m4.z)) : v.n;
+  } else {
+    v.uv = make_float2(0.0f, 0.0f);
+    v.ncc = v.n;
+  }
   return true;
 }
 
@@ -189,17 +199,30 @@ struct Bsdf {
   float t;
 };
 
-__device__ __forceinline__ Bsdf make_bsdf(const SceneView& sc, uint32_t mat, v3 n, bool back, float t) {
+// The BSDF at a shading point: material `mat` with its textures at uv, the
+// shading frame of n and the clearcoat frame of ncc.
+__device__ __forceinline__ Bsdf make_bsdf_at(const SceneView& sc, uint32_t mat, v3 n, v3 ncc, float2 uv, bool back,
+                                             float t) {
   const apt_material& m = sc.mats[mat];
   Bsdf b;
   b.lambert = m.bsdf_model == APT_BSDF_LAMBERT;
-  b.albedo = mk(m.color[0], m.color[1], m.color[2]);
-  b.d = disney_from_material(m);
+  if (sc.tex) {
+    const MatEval e = mat_eval(sc, m, uv);
+    b.albedo = e.color;
+    b.d = disney_from_eval(m, e);
+  } else {
+    b.albedo = mk(m.color[0], m.color[1], m.color[2]);
+    b.d = disney_from_material(m);
+  }
   b.sf.f = onb(n);
-  b.sf.cc = b.sf.f;
+  b.sf.cc = sc.tri_tan ? onb(ncc) : b.sf.f;
   b.back = back;
   b.t = t;
   return b;
+}
+
+__device__ __forceinline__ Bsdf make_bsdf(const SceneView& sc, const Vtx& v) {
+  return make_bsdf_at(sc, v.mat, v.n, v.ncc, v.uv, v.back, v.t);
 }
 
 __device__ __forceinline__ v3 bsdf_eval(const Bsdf& b, v3 wo, v3 wi, float& pdf) {
@@ -281,7 +304,7 @@ __device__ __forceinline__ bool eval_light(const SceneView& sc, uint32_t light, 
   const float cos_l = -dot(nl, e.wi);
   const apt_material& lm = sc.mats[sc.tri_mat[light]];
   if (cos_l > 0.0f) {
-    e.Le = mk(lm.emission[0], lm.emission[1], lm.emission[2]);
+    e.Le = mat_emission(sc, lm, hit_uv(sc, light, u, v));  // at the light point (nee.wgsl:39-51)
     e.pdf = (1.0f - p_sun) / (float)sc.n_emissive / tri_area(g) * (e.dist * e.dist) / cos_l;
   } else {
     e.Le = mk(0.0f, 0.0f, 0.0f);
@@ -387,8 +410,13 @@ __device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& 
     n2 = normalize(ld3(tn[0]) * (1.0f - h.u - h.v) + ld3(tn[1]) * h.u + ld3(tn[2]) * h.v);
   }
   const v3 wo2 = -wi;
+  const uint32_t mat2 = sc.tri_mat[h.prim];
+  const float2 uv2 = hit_uv(sc, h.prim, h.u, h.v);
+  v3 nm2, ncc2;
+  map_normals(sc, sc.mats[mat2], h.prim, h.u, h.v, uv2, n2, nm2, ncc2);
   const bool back = dot(wo2, n2) < 0.0f;
-  if (back) n2 = -n2;
+  n2 = back ? -nm2 : nm2;
+  ncc2 = back ? -ncc2 : ncc2;
   const float r0 = rng_next(rng);
   rng_next(rng);  // r1: instance pick
   const float r2 = rng_next(rng), r3 = rng_next(rng), r4 = rng_next(rng);
@@ -400,7 +428,7 @@ __device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& 
   const float c = dot(e.wi, n2);
   if (!(c > 0.0f) || !(maxc(e.Le) > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
   if (!visible<W, S, INST>(sc, st, x2, n2, e.wi, e.dist, ctr)) return mk(0.0f, 0.0f, 0.0f);
-  const Bsdf b2 = make_bsdf(sc, sc.tri_mat[h.prim], n2, back, h.t);
+  const Bsdf b2 = make_bsdf_at(sc, mat2, n2, ncc2, uv2, back, h.t);
   float pdf;
   return Tf * bsdf_eval(b2, wo2, e.wi, pdf) * e.Le * (c / e.pdf);
 }
@@ -538,22 +566,34 @@ __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView s
     const float4* tn = sc.tri_nrm + 3 * (size_t)h.prim;
     ns = normalize(ld3(tn[0]) * (1.0f - h.u - h.v) + ld3(tn[1]) * h.u + ld3(tn[2]) * h.v);
   }
-  const bool back = dot(-d, ns) < 0.0f;
-  if (back) ns = -ns;
   const uint32_t mid = sc.tri_mat[h.prim];
   const apt_material& m = sc.mats[mid];
+  const float2 uv = hit_uv(sc, h.prim, h.u, h.v);
+  // normal mapping on the interpolated normal, then the face flip (trace.wgsl:143-191)
+  v3 n_map, n_cc;
+  map_normals(sc, m, h.prim, h.u, h.v, uv, ns, n_map, n_cc);
+  const bool back = dot(-d, ns) < 0.0f;
+  ns = back ? -n_map : n_map;
+  n_cc = back ? -n_cc : n_cc;
   // emission at the first vertex only (trace.wgsl:139-141), front side only
   if (bounce == 0 && has_emission(m) && dot(ng, d) < 0.0f) {
-    const v3 e = T * mk(m.emission[0], m.emission[1], m.emission[2]);
+    const v3 e = T * mat_emission(sc, m, uv);
     b.radiance[pix] = make_float4(L.x + e.x, L.y + e.y, L.z + e.z, 0.0f);
   }
   const uint32_t bits = mid | (back ? 0x80000000u : 0u);
   b.vtx_p[pix] = v4(hp, h.t);
   b.vtx_n[pix] = v4(ns, u2f(bits));
   b.vtx_t[pix] = v4(T, 0.0f);
+  if (sc.tex) b.vtx_m[pix] = make_float4(uv.x, uv.y, u2f(pack_oct30(n_cc)), 0.0f);
   if (bounce == 0) {
     // albedo = colour + emission (trace.wgsl:218-223)
-    const v3 alb = mk(m.color[0] + m.emission[0], m.color[1] + m.emission[1], m.color[2] + m.emission[2]);
+    v3 alb;
+    if (sc.tex) {
+      const MatEval e = mat_eval(sc, m, uv);
+      alb = e.color + e.emission;
+    } else {
+      alb = mk(m.color[0] + m.emission[0], m.color[1] + m.emission[1], m.color[2] + m.emission[2]);
+    }
     b.gbuf[p.cur][pix] = pack_texel(hp, h.t, ns, alb, bits);
   }
 }
@@ -566,11 +606,11 @@ __global__ __launch_bounds__(BLOCK8, RS_RIS_WAVES) void k_rs_di_ris(RestirParams
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) {
+  if (!load_vtx(sc, b, pix, v)) {
     rq_put(b, false, pix, mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 1.0f), 0.0f);
     return;
   }
-  const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+  const Bsdf bs = make_bsdf(sc, v);
   uint32_t rng = b.rng[pix];
   const float p_sun = sun_pick(sc);
   const float na = (float)p.di_candidates, nb = 1.0f;
@@ -659,11 +699,11 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_bsdf_ray(RestirParams p, SceneV
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
-  const bool have = load_vtx(b, pix, v);
+  const bool have = load_vtx(sc, b, pix, v);
   bool want = false;
   v3 wi = mk(0.0f, 0.0f, 0.0f);
   if (have) {
-    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    const Bsdf bs = make_bsdf(sc, v);
     uint32_t rng = b.rng[pix];
     if (p.di_candidates > 0u)
       for (int k = 0; k < 5; ++k) rng_next(rng);
@@ -697,7 +737,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView 
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) return;
+  if (!load_vtx(sc, b, pix, v)) return;
   const LdsStack st{s_stack + threadIdx.x, BLOCK8};
   uint32_t rng = b.rng[pix];
   v3 T = v.T;
@@ -711,7 +751,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView 
     else T = T * (1.0f / q);
   }
   if (alive) {
-    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    const Bsdf bs = make_bsdf(sc, v);
     const float r0 = rng_next(rng), r1 = rng_next(rng), r2 = rng_next(rng);
     const BsdfSample s = bsdf_sample(bs, v.wo, r0, r1, r2);
     float M = 0.0f, wsum = 0.0f, phat = 0.0f;
@@ -743,7 +783,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(Resti
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) {
+  if (!load_vtx(sc, b, pix, v)) {
     b.di_tmp[pix] = di_empty();
     b.di_hist[p.cur][pix] = di_empty();
     return;
@@ -762,7 +802,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(Resti
       DiRes pr = b.di_hist[p.cur ^ 1][qy * p.width + qx];
       pr.W *= di_jacobian(sc, pr.light, pr.u, pr.v, v.x, f4v(b.gbuf[p.cur ^ 1][qy * p.width + qx].pos_depth));
       pr.M = fminf(pr.M, (float)p.di_mcap * r.M);
-      const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+      const Bsdf bs = make_bsdf(sc, v);
       pr.phat = di_phat<W, S, INST>(sc, st, v, bs, pr.light, pr.u, pr.v, true, b.counters);
       int kept;
       DiRes c = di_merge(r, pr, rng, kept);
@@ -793,13 +833,13 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_self(RestirParams p, SceneView 
   bool shoot = false;
   v3 o = mk(0.0f, 0.0f, 0.0f), wi = mk(0.0f, 0.0f, 1.0f);
   float tmax = 0.0f;
-  if (load_vtx(b, pix, v)) {
+  if (load_vtx(sc, b, pix, v)) {
     const DiRes r = b.di_tmp[pix];
     LightHit e;
     if (eval_light(sc, r.light, r.u, r.v, v.x, e)) {
       const float c = dot(e.wi, v.n);
       if (c > 0.0f) {
-        const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+        const Bsdf bs = make_bsdf(sc, v);
         float pdf;
         ph = luma(bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * c);
         shoot = ph != 0.0f;
@@ -842,7 +882,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) {
+  if (!load_vtx(sc, b, pix, v)) {
     b.di[pix] = di_empty();
     return;
   }
@@ -854,7 +894,7 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
   const auto pixel_bsdf = [&]() {
     uint32_t m = v.mat;
     asm volatile("" : "+v"(m));
-    return make_bsdf(sc, m, v.n, v.back, v.t);
+    return make_bsdf_at(sc, m, v.n, v.ncc, v.uv, v.back, v.t);
   };
   const Tex ct = unpack_texel(b.gbuf[p.cur][pix]);
   const float cd = depth_cs(ct.depth);
@@ -908,9 +948,9 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
         if (di_self_visible(b, q)) z += nr.M;
       } else if (kept == 1) {
         Vtx vn;
-        if (load_vtx(b, q, vn)) {
+        if (load_vtx(sc, b, q, vn)) {
           vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
-          const Bsdf bn = make_bsdf(sc, vn.mat, vn.n, vn.back, vn.t);
+          const Bsdf bn = make_bsdf(sc, vn);
           if (di_phat<W, S, INST>(sc, st, vn, bn, c.light, c.u, c.v, true, b.counters) > 0.0f) z += nr.M;
         }
       }
@@ -935,7 +975,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
   GiRes r;
   r.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   r.mwsp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (!load_vtx(b, pix, v)) {
+  if (!load_vtx(sc, b, pix, v)) {
     b.gi_hist[p.cur][pix] = r;
     return;
   }
@@ -954,7 +994,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
       if (texel_similar(cur, prev)) {
         const GiRes pr = b.gi_hist[p.cur ^ 1][qy * p.width + qx];
         const float pM = fminf(pr.mwsp.x, (float)p.gi_mcap);
-        const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+        const Bsdf bs = make_bsdf(sc, v);
         const float pp = gi_phat<W, S, INST>(sc, st, v, bs, f4v(t4), pr, b.counters);
         // GiReservoir::combine (gi_reservoir.wgsl:66-78)
         float wsum = 0.0f, M = 0.0f, phat = 0.0f;
@@ -988,7 +1028,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneVie
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) return;
+  if (!load_vtx(sc, b, pix, v)) return;
   const DiRes r = b.di[pix];
   if (r.W == 0.0f) return;
   LightHit e;
@@ -997,7 +1037,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneVie
   if (!(c > 0.0f)) return;
   const LdsStack st{s_stack + threadIdx.x, BLOCK8};
   if (!visible<W, S, INST>(sc, st, v.x, v.n, e.wi, e.dist, b.counters)) return;
-  const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+  const Bsdf bs = make_bsdf(sc, v);
   float pdf;
   const v3 add = v.T * bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * (c * r.W);
   if (!finite3(add)) return;
@@ -1010,13 +1050,13 @@ __global__ __launch_bounds__(BLOCK) void k_rs_apply_gi(RestirParams p, SceneView
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
-  if (!load_vtx(b, pix, v)) return;
+  if (!load_vtx(sc, b, pix, v)) return;
   const float4 t4 = b.thr[pix];
   if (!(t4.w > 0.0f)) return;
   const GiRes g = b.gi[pix];
   if (g.mwsp.y > 0.0f) {
     const v3 wi = normalize(f4v(g.x2_rng) - v.x);
-    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    const Bsdf bs = make_bsdf(sc, v);
     float pdf;
     const v3 T = f4v(t4) * bsdf_eval(bs, v.wo, wi, pdf) * (fabsf(dot(v.n, wi)) * g.mwsp.y);
     const bool ok = finite3(T) && maxc(T) > 0.0f;

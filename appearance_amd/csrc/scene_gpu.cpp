@@ -51,9 +51,26 @@ __global__ void k_emit_flags(const float* __restrict__ xyz, const uint32_t* __re
 // A model's vertices in world space: positions M p (((m0 x + m4 y) + m8 z) +
 // m12 per row, column-major M), normals through the inverse transpose of its
 // 3x3 part and renormalised, material ids offset by the model's material base.
+// A vector through the normal matrix (or kept), renormalised.
+__device__ __forceinline__ float4 place_dir(const ModelXform& x, float q0, float q1, float q2) {
+  float v[3] = {q0, q1, q2};
+  if (x.nm_ok)
+    for (int r = 0; r < 3; ++r) v[r] = x.nm[3 * r] * q0 + x.nm[3 * r + 1] * q1 + x.nm[3 * r + 2] * q2;
+  const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (l > 0.0f) for (int r = 0; r < 3; ++r) v[r] /= l;
+  return make_float4(v[0], v[1], v[2], 0.0f);
+}
+
+// A model's vertices in world space: positions M p (((m0 x + m4 y) + m8 z) +
+// m12 per row, column-major M), normals, tangents and bitangents through the
+// inverse transpose of its 3x3 part and renormalised (as the reference maps
+// the interpolated frame, trace.wgsl:143-156), uvs as they are, material ids
+// offset by the model's material base.  Arrays the scene does not keep are
+// null; a model without an attribute the scene keeps writes zeros.
 __global__ void k_place(const float* __restrict__ lxyz, const float* __restrict__ lnrm,
-                        const uint32_t* __restrict__ lmat, uint32_t n, ModelXform x, float4* __restrict__ opos,
-                        float4* __restrict__ onrm, uint32_t* __restrict__ omat) {
+                        const uint32_t* __restrict__ lmat, uint32_t n, ModelXform x, ModelAttribs at,
+                        float4* __restrict__ opos, float4* __restrict__ onrm, float2* __restrict__ ouv,
+                        float4* __restrict__ otan, float4* __restrict__ obit, uint32_t* __restrict__ omat) {
   const uint32_t nv = 3 * n;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += gridDim.x * blockDim.x) {
     const float* p = lxyz + 3 * (size_t)j;
@@ -62,12 +79,26 @@ __global__ void k_place(const float* __restrict__ lxyz, const float* __restrict_
     opos[j] = make_float4(o[0], o[1], o[2], 0.0f);
     if (onrm) {
       const float* q = lnrm + 3 * (size_t)j;
-      float v[3];
-      for (int r = 0; r < 3; ++r)
-        v[r] = x.nm_ok ? x.nm[3 * r] * q[0] + x.nm[3 * r + 1] * q[1] + x.nm[3 * r + 2] * q[2] : q[r];
-      const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-      if (l > 0.0f) for (int r = 0; r < 3; ++r) v[r] /= l;
-      onrm[j] = make_float4(v[0], v[1], v[2], 0.0f);
+      onrm[j] = place_dir(x, q[0], q[1], q[2]);
+    }
+    if (ouv) ouv[j] = at.uv ? at.uv[j] : make_float2(0.0f, 0.0f);
+    if (otan) {
+      if (at.tan && lnrm) {
+        const float4 t = at.tan[j];
+        const float* q = lnrm + 3 * (size_t)j;
+        float4 b;
+        if (at.bit) {
+          b = at.bit[j];
+        } else {  // cross(n, t) * w in object space (vertex_pool_bindings.wgsl:43-46)
+          b = make_float4((q[1] * t.z - q[2] * t.y) * t.w, (q[2] * t.x - q[0] * t.z) * t.w,
+                          (q[0] * t.y - q[1] * t.x) * t.w, 0.0f);
+        }
+        otan[j] = place_dir(x, t.x, t.y, t.z);
+        obit[j] = place_dir(x, b.x, b.y, b.z);
+      } else {
+        otan[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        obit[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
     }
     if (j < n) omat[j] = x.mat_base + (lmat ? lmat[j] : 0u);
   }
@@ -202,31 +233,40 @@ apt_status scene_gpu_upload(const float* pos, const float* nrm, const uint32_t* 
 }
 
 apt_status scene_place_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
-                             const ModelXform& x, SceneGpu& s, uint32_t base, hipStream_t st) {
+                             const ModelXform& x, const ModelAttribs& at, SceneGpu& s, uint32_t base,
+                             hipStream_t st) {
   if (n == 0) return APT_OK;
   APT_REQUIRE((size_t)base + n <= s.cap_tris, APT_ERR_INVALID_ARG, "scene placement: prims [%u, %u) past %zu",
               base, base + n, s.cap_tris);
   APT_REQUIRE(!s.d_nrm || d_nrm, APT_ERR_INVALID_ARG, "scene placement: the model has no normals");
-  hipLaunchKernelGGL(k_place, dim3(grid_for((size_t)n * 3)), dim3(256), 0, st, d_xyz, s.d_nrm ? d_nrm : nullptr,
-                     d_mat, n, x, s.d_pos + 3 * (size_t)base, s.d_nrm ? s.d_nrm + 3 * (size_t)base : nullptr,
-                     s.d_mat + base);
+  const size_t v0 = 3 * (size_t)base;
+  hipLaunchKernelGGL(k_place, dim3(grid_for((size_t)n * 3)), dim3(256), 0, st, d_xyz, d_nrm, d_mat, n, x, at,
+                     s.d_pos + v0, s.d_nrm ? s.d_nrm + v0 : nullptr, s.d_uv ? s.d_uv + v0 : nullptr,
+                     s.d_tan ? s.d_tan + v0 : nullptr, s.d_tan ? s.d_bit + v0 : nullptr, s.d_mat + base);
   APT_HIP(hipGetLastError());
   return APT_OK;
 }
 
-apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, const apt_material* mats, uint32_t n_mats,
+apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, bool with_uv, bool with_tangents,
+                         const apt_material* mats, uint32_t n_mats, const TexDesc* tex, uint32_t n_tex,
                          hipStream_t st) {
-  if (n > s.cap_tris || (with_normals && !s.d_nrm)) {
-    for (void* p : {(void*)s.d_pos, (void*)s.d_nrm, (void*)s.d_mat})
+  const bool have_uv = s.d_uv != nullptr, have_tan = s.d_tan != nullptr;
+  if (n > s.cap_tris || (with_normals && !s.d_nrm) || with_uv != have_uv || with_tangents != have_tan) {
+    for (void* p : {(void*)s.d_pos, (void*)s.d_nrm, (void*)s.d_mat, (void*)s.d_uv, (void*)s.d_tan, (void*)s.d_bit})
       if (p) (void)hipFree(p);
-    s.d_pos = s.d_nrm = nullptr;
+    s.d_pos = s.d_nrm = s.d_tan = s.d_bit = nullptr;
+    s.d_uv = nullptr;
     s.d_mat = nullptr;
     s.cap_tris = 0;
-    apt_status r = alloc_dev((void**)&s.d_pos, (size_t)n * 3 * sizeof(float4));
-    if (r == APT_OK && with_normals) r = alloc_dev((void**)&s.d_nrm, (size_t)n * 3 * sizeof(float4));
-    if (r == APT_OK) r = alloc_dev((void**)&s.d_mat, (size_t)n * sizeof(uint32_t));
+    const size_t cap = std::max<size_t>(n, 1);
+    apt_status r = alloc_dev((void**)&s.d_pos, cap * 3 * sizeof(float4));
+    if (r == APT_OK && with_normals) r = alloc_dev((void**)&s.d_nrm, cap * 3 * sizeof(float4));
+    if (r == APT_OK && with_uv) r = alloc_dev((void**)&s.d_uv, cap * 3 * sizeof(float2));
+    if (r == APT_OK && with_tangents) r = alloc_dev((void**)&s.d_tan, cap * 3 * sizeof(float4));
+    if (r == APT_OK && with_tangents) r = alloc_dev((void**)&s.d_bit, cap * 3 * sizeof(float4));
+    if (r == APT_OK) r = alloc_dev((void**)&s.d_mat, cap * sizeof(uint32_t));
     if (r != APT_OK) return r;
-    s.cap_tris = n;
+    s.cap_tris = cap;
   } else if (!with_normals && s.d_nrm) {
     (void)hipFree(s.d_nrm);
     s.d_nrm = nullptr;
@@ -240,10 +280,91 @@ apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, const apt_m
     s.cap_mats = n_mats;
   }
   if (n_mats) APT_HIP(hipMemcpyAsync(s.d_mats, mats, (size_t)n_mats * sizeof(apt_material), hipMemcpyHostToDevice, st));
+  if (n_tex > s.cap_tex) {
+    if (s.d_tex) (void)hipFree(s.d_tex);
+    s.d_tex = nullptr;
+    s.cap_tex = 0;
+    apt_status r = alloc_dev((void**)&s.d_tex, (size_t)n_tex * sizeof(TexDesc));
+    if (r != APT_OK) return r;
+    s.cap_tex = n_tex;
+  }
+  if (n_tex) APT_HIP(hipMemcpyAsync(s.d_tex, tex, (size_t)n_tex * sizeof(TexDesc), hipMemcpyHostToDevice, st));
+  s.n_tex = n_tex;
   s.n_tris = n;
   s.n_mats = n_mats;
-  s.bytes = s.cap_tris * (3 * sizeof(float4) * (s.d_nrm ? 2 : 1) + sizeof(uint32_t)) +
-            s.cap_mats * sizeof(apt_material) + s.cap_emissive * sizeof(uint32_t);
+  s.bytes = s.cap_tris * (3 * sizeof(float4) * (1 + (s.d_nrm ? 1 : 0) + (s.d_tan ? 2 : 0)) +
+                          (s.d_uv ? 3 * sizeof(float2) : 0) + sizeof(uint32_t)) +
+            s.cap_mats * sizeof(apt_material) + s.cap_emissive * sizeof(uint32_t) + s.cap_tex * sizeof(TexDesc);
+  return APT_OK;
+}
+
+apt_status scene_gpu_attribs(SceneGpu& s, const float* pos, const float* nrm, uint32_t n,
+                             const apt_mesh_attribs& a) {
+  const size_t nv = 3 * (size_t)n;
+  if (a.uvs) {
+    std::vector<float2> uv(nv);
+    for (size_t j = 0; j < nv; ++j) uv[j] = make_float2(a.uvs[2 * j], a.uvs[2 * j + 1]);
+    apt_status r = upload(s.d_uv, uv.data(), nv, s.bytes);
+    if (r != APT_OK) return r;
+  }
+  if (a.tangents) {
+    std::vector<float4> t(nv), b(nv);
+    for (size_t j = 0; j < nv; ++j) {
+      const float* q = a.tangents + 4 * j;
+      float n3[3];
+      if (nrm) {
+        for (int k = 0; k < 3; ++k) n3[k] = nrm[3 * j + k];
+      } else {  // the face normal
+        const float* p = pos + 9 * (j / 3);
+        const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+        n3[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        n3[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        n3[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      }
+      float bx, by, bz;
+      if (a.bitangents) {
+        bx = a.bitangents[3 * j];
+        by = a.bitangents[3 * j + 1];
+        bz = a.bitangents[3 * j + 2];
+      } else {
+        bx = (n3[1] * q[2] - n3[2] * q[1]) * q[3];
+        by = (n3[2] * q[0] - n3[0] * q[2]) * q[3];
+        bz = (n3[0] * q[1] - n3[1] * q[0]) * q[3];
+      }
+      const float lt = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+      const float lb = std::sqrt(bx * bx + by * by + bz * bz);
+      t[j] = lt > 0.0f ? make_float4(q[0] / lt, q[1] / lt, q[2] / lt, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      b[j] = lb > 0.0f ? make_float4(bx / lb, by / lb, bz / lb, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    apt_status r = upload(s.d_tan, t.data(), nv, s.bytes);
+    if (r == APT_OK) r = upload(s.d_bit, b.data(), nv, s.bytes);
+    if (r != APT_OK) return r;
+  }
+  if (a.n_textures) {
+    APT_REQUIRE(a.textures, APT_ERR_INVALID_ARG, "scene: n_textures without textures");
+    size_t total = 0;
+    for (uint32_t k = 0; k < a.n_textures; ++k) {
+      const apt_texture& t = a.textures[k];
+      APT_REQUIRE(t.width > 0 && t.height > 0 && t.rgba8, APT_ERR_INVALID_ARG, "scene: texture %u is empty", k);
+      total += (size_t)t.width * t.height;
+    }
+    std::vector<uint32_t> texels;
+    texels.reserve(total);
+    std::vector<size_t> off(a.n_textures);
+    for (uint32_t k = 0; k < a.n_textures; ++k) {
+      const apt_texture& t = a.textures[k];
+      off[k] = texels.size();
+      texels.insert(texels.end(), t.rgba8, t.rgba8 + (size_t)t.width * t.height);
+    }
+    apt_status r = upload(s.d_texels, texels.data(), texels.size(), s.bytes);
+    if (r != APT_OK) return r;
+    std::vector<TexDesc> d(a.n_textures);
+    for (uint32_t k = 0; k < a.n_textures; ++k)
+      d[k] = TexDesc{s.d_texels + off[k], a.textures[k].width, a.textures[k].height};
+    r = upload(s.d_tex, d.data(), d.size(), s.bytes);
+    if (r != APT_OK) return r;
+    s.n_tex = a.n_textures;
+  }
   return APT_OK;
 }
 
@@ -284,7 +405,8 @@ apt_status scene_emitters(SceneGpu& s, hipStream_t st) {
 
 void scene_gpu_free(SceneGpu& s) {
   if (s.bvh) apt_bvh_destroy(s.bvh);
-  for (void* p : {(void*)s.d_pos, (void*)s.d_nrm, (void*)s.d_mat, (void*)s.d_mats, (void*)s.d_emissive})
+  for (void* p : {(void*)s.d_pos, (void*)s.d_nrm, (void*)s.d_mat, (void*)s.d_mats, (void*)s.d_emissive,
+                  (void*)s.d_uv, (void*)s.d_tan, (void*)s.d_bit, (void*)s.d_tex, (void*)s.d_texels})
     if (p) (void)hipFree(p);
   s = SceneGpu();
 }
@@ -312,6 +434,10 @@ SceneView scene_gpu_view(const SceneGpu& s, const apt_camera& cam, uint32_t widt
   }
   sc.tri_pos = s.d_pos;
   sc.tri_nrm = s.d_nrm;
+  sc.tri_uv = s.d_uv;
+  sc.tri_tan = s.d_tan;
+  sc.tri_bit = s.d_bit;
+  sc.tex = s.n_tex ? s.d_tex : nullptr;
   sc.tri_mat = s.d_mat;
   sc.mats = s.d_mats;
   sc.emissive = s.d_emissive;

@@ -15,6 +15,13 @@ struct SceneGpu {
   size_t cap_tris = 0, cap_mats = 0, cap_emissive = 0;
   float4* d_pos = nullptr;      // 3 per triangle
   float4* d_nrm = nullptr;      // 3 per triangle or null
+  float2* d_uv = nullptr;       // 3 per triangle or null
+  float4* d_tan = nullptr;      // 3 per triangle (world tangent) or null
+  float4* d_bit = nullptr;      // 3 per triangle (world bitangent) or null
+  TexDesc* d_tex = nullptr;     // texture table or null
+  uint32_t* d_texels = nullptr; // texels of an explicit scene's textures (spawned models keep their own)
+  uint32_t n_tex = 0;
+  size_t cap_tex = 0;
   uint32_t* d_mat = nullptr;
   apt_material* d_mats = nullptr;
   uint32_t* d_emissive = nullptr;
@@ -35,6 +42,11 @@ apt_status scene_gpu_upload_device(const float* d_xyz, const float* d_nrm, const
                                    const apt_sky* sky, uint32_t bvh_width, uint32_t lds_stack,
                                    SceneGpu& out);
 void scene_gpu_free(SceneGpu& s);
+// Uvs, tangent frames and textures of an explicit scene (apt_scene_set_attribs):
+// per vertex the unit tangent and the bitangent (given, or cross(n, t) * w with
+// the vertex normal, or the face normal without normals), in world space.
+apt_status scene_gpu_attribs(SceneGpu& s, const float* pos, const float* nrm, uint32_t n,
+                             const apt_mesh_attribs& a);
 
 // A model's placement: the column-major world transform, the normal matrix
 // (rows of the inverse transpose of its 3x3 part; nm_ok = 0 keeps normals as
@@ -46,16 +58,27 @@ struct ModelXform {
   uint32_t mat_base;
 };
 
+// A model's optional per-vertex attributes in object space: uvs (float2),
+// tangents (xyz, handedness) and bitangents (xyz; null: cross(n, t) * w).
+struct ModelAttribs {
+  const float2* uv = nullptr;
+  const float4* tan = nullptr;
+  const float4* bit = nullptr;
+};
+
 // Places a model's n resident triangles in world space in a two-level
 // scene's attribute arrays from prim `base` on: float4 positions, unit
 // normals (when the scene keeps normals), material ids + mat_base.
 // Asynchronous on stream st.
 apt_status scene_place_model(const float* d_xyz, const float* d_nrm, const uint32_t* d_mat, uint32_t n,
-                             const ModelXform& x, SceneGpu& s, uint32_t base, hipStream_t st);
-// Sizes a two-level scene's attribute arrays for n triangles (normals when
-// with_normals) and n_mats materials, keeping them when they are big enough,
-// and uploads the materials.
-apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, const apt_material* mats, uint32_t n_mats,
+                             const ModelXform& x, const ModelAttribs& at, SceneGpu& s, uint32_t base,
+                             hipStream_t st);
+// Sizes a two-level scene's attribute arrays for n triangles (normals, uvs,
+// tangent frames as asked), n_mats materials and n_tex texture descriptors,
+// keeping them when they are big enough, and uploads materials and
+// descriptors.
+apt_status scene_reserve(SceneGpu& s, uint32_t n, bool with_normals, bool with_uv, bool with_tangents,
+                         const apt_material* mats, uint32_t n_mats, const TexDesc* tex, uint32_t n_tex,
                          hipStream_t st);
 // Rebuilds the emitter list of a two-level scene from its world positions
 // and materials; synchronous on st.

@@ -318,6 +318,21 @@ apt_status apt_scene_data_view(const apt_scene_data* s, apt_mesh_desc* mesh, apt
   return APT_OK;
 }
 
+apt_status apt_scene_data_attribs(const apt_scene_data* s, apt_mesh_attribs* out) {
+  APT_REQUIRE(s && out, APT_ERR_INVALID_ARG, "apt_scene_data_attribs: null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->uvs = s->uvs.empty() ? nullptr : s->uvs.data();
+  out->tangents = s->tangents.empty() ? nullptr : s->tangents.data();
+  out->bitangents = s->bitangents.empty() ? nullptr : s->bitangents.data();
+  std::vector<apt_texture>& v = s->texture_views;
+  v.resize(s->textures.size());
+  for (size_t k = 0; k < v.size(); ++k) v[k] = apt_texture{s->textures[k].width, s->textures[k].height,
+                                                           s->textures[k].rgba.data()};
+  out->textures = v.empty() ? nullptr : v.data();
+  out->n_textures = (uint32_t)v.size();
+  return APT_OK;
+}
+
 void apt_scene_data_destroy(apt_scene_data* s) { delete s; }
 
 apt_status apt_rays_bvh_stress(uint32_t seed, uint32_t side, uint32_t n_incoherent, apt_ray* out) {

@@ -77,6 +77,11 @@ class PathTracerGpu:
         md.n_materials = len(scene.materials)
         md.materials = mats
         sk = sky if sky is not None else getattr(scene, "sky", None)
+        if getattr(scene, "textured", False):
+            at, _keep = scene.attribs()
+            check(lib().apt_scene_set_attribs(self._h, C.byref(md), C.byref(at),
+                                              C.byref(sk) if sk is not None else None), "apt_scene_set_attribs")
+            return
         check(lib().apt_scene_set(self._h, C.byref(md), C.byref(sk) if sk is not None else None),
               "apt_scene_set")
 

@@ -2,11 +2,11 @@
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._ffi import Camera, Material, MeshDesc, Sky, check, lib
+from ._ffi import Camera, Material, MeshAttribs, MeshDesc, Sky, Texture, check, lib
 
 
 @dataclass
@@ -17,6 +17,33 @@ class SceneData:
     materials: list            # list of Material (ctypes, 160 B)
     camera: Camera
     sky: Sky
+    uvs: np.ndarray | None = None         # (n, 6) float32
+    tangents: np.ndarray | None = None    # (n, 12) float32: xyz, handedness per vertex
+    bitangents: np.ndarray | None = None  # (n, 9) float32
+    textures: list = field(default_factory=list)  # (height, width) uint32 RGBA8 arrays
+
+    @property
+    def textured(self) -> bool:
+        return self.uvs is not None or self.tangents is not None or bool(self.textures)
+
+    def attribs(self):
+        """(MeshAttribs, keep-alive list) for apt_scene_set_attribs."""
+        keep = []
+        a = MeshAttribs()
+        for name, arr in (("uvs", self.uvs), ("tangents", self.tangents), ("bitangents", self.bitangents)):
+            if arr is not None:
+                arr = np.ascontiguousarray(arr, np.float32)
+                keep.append(arr)
+                setattr(a, name, arr.ctypes.data_as(C.POINTER(C.c_float)))
+        if self.textures:
+            texs = [np.ascontiguousarray(t, np.uint32) for t in self.textures]
+            keep += texs
+            tv = (Texture * len(texs))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint32)))
+                                         for t in texs])
+            keep.append(tv)
+            a.textures = tv
+            a.n_textures = len(texs)
+        return a, keep
 
     @property
     def n_tris(self) -> int:
@@ -59,6 +86,17 @@ def _view(h) -> SceneData:
                 nrm = np.ctypeslib.as_array(m.normals, shape=(n * 9,)).reshape(n, 9).copy()
             mids = np.ctypeslib.as_array(m.material_ids, shape=(n,)).copy()
         mats = [Material.from_buffer_copy(m.materials[i]) for i in range(m.n_materials)]
-        return SceneData(pos, nrm, mids, mats, cam, sky)
+        a = MeshAttribs()
+        check(lib().apt_scene_data_attribs(h, C.byref(a)), "apt_scene_data_attribs")
+
+        def arr(ptr, k):
+            return np.ctypeslib.as_array(ptr, shape=(n * k,)).reshape(n, k).copy() if ptr and n else None
+
+        texs = []
+        for i in range(a.n_textures):
+            t = a.textures[i]
+            texs.append(np.ctypeslib.as_array(t.rgba8, shape=(t.height * t.width,)).reshape(t.height, t.width).copy())
+        return SceneData(pos, nrm, mids, mats, cam, sky, arr(a.uvs, 6), arr(a.tangents, 12), arr(a.bitangents, 9),
+                         texs)
     finally:
         lib().apt_scene_data_destroy(h)

@@ -325,6 +325,43 @@ typedef struct apt_mesh_desc {
   const apt_material* materials; /* n_materials */
 } apt_mesh_desc;
 
+/* Texture sampling and vertex attributes of a textured scene.
+ *
+ * A texture is width x height RGBA8 texels (R in the low byte), row 0 at
+ * v = 0 (glTF's image origin).  Materials index the scene's texture list
+ * (APT_INVALID_TEXTURE = none) and the texture scales the material's factor as
+ * the reference's MaterialDescriptor accessors do
+ * (shared/material/material_pool_bindings.wgsl:19-107): colour x sRGB-to-
+ * linear(rgba) (alpha -> the Disney luminance), emission x rgb, metallic x b
+ * and roughness x g, clearcoat x r, clearcoat roughness x g, transmission x r,
+ * sheen x r, sheen tint x sRGB-to-linear(rgb), each only when the factor is
+ * nonzero; the normal texture (rgb * 2 - 1, in the interpolated tangent frame,
+ * blended by normal_scale below 1, :89-123) and the clearcoat normal texture
+ * bend the shading normals.  Sampling is bilinear with repeat addressing at
+ * level 0 (the reference's sampler, src/scene_resources/material_pool.rs:68-74),
+ * in FP32 on the decoded texels (the reference samples BC-compressed copies).
+ * Emission textures also weight light samples on emissive triangles
+ * (helpers/nee.wgsl:16-57). */
+typedef struct apt_texture {
+  uint32_t width;
+  uint32_t height;
+  const uint32_t* rgba8;  /* width * height texels, row major */
+} apt_texture;
+
+/* Per-vertex attributes beyond positions and normals (every pointer may be
+ * NULL): uvs = n_tris * 6 floats (TEXCOORD_0), tangents = n_tris * 12 floats
+ * (xyz and handedness w per vertex, glTF TANGENT), bitangents = n_tris * 9
+ * floats (NULL: cross(normal, tangent) * w per vertex,
+ * shared/vertex_pool_bindings.wgsl:43-58), and the scene's textures. */
+typedef struct apt_mesh_attribs {
+  const float* uvs;
+  const float* tangents;
+  const float* bitangents;
+  const apt_texture* textures;
+  uint32_t n_textures;
+  uint32_t _pad;
+} apt_mesh_attribs;
+
 /* The reference's CameraUpdateData (crates/appearance-world/src/visible_world_action.rs:9-15),
  * 80 B: column-major world transform of the camera (it looks down its -Z),
  * vertical fov in degrees, near, far. */
@@ -352,12 +389,18 @@ apt_status apt_scene_generate(const char* name, uint32_t seed, apt_scene_data** 
  * POSITION, NORMAL (generated when absent, mesh.rs:58-79) and indices, and
  * each material's factors mapped as asset.rs:199-339 maps them (emissive
  * strength, volume absorption, transmission, eta = 1/ior, specular,
- * clearcoat, sheen).  Textures are not decoded: every material renders with
- * its factors.  The same paths work as SpawnModel asset paths. */
+ * clearcoat, sheen), TEXCOORD_0, TANGENT (generated when absent with the
+ * reference's per-vertex method, crates/appearance-model/src/mesh.rs:82-160)
+ * and the textures the materials use (PNG images decoded to RGBA8; an image
+ * in another format leaves its texture slot empty, so the material uses its
+ * factor).  The same paths work as SpawnModel asset paths. */
 apt_status apt_scene_load_gltf(const char* path, apt_scene_data** out);
 /* Views into the library-owned arrays; valid until apt_scene_data_destroy. */
 apt_status apt_scene_data_view(const apt_scene_data* s, apt_mesh_desc* mesh, apt_camera* camera,
                                apt_sky* sky);
+/* The scene's uvs, tangents, bitangents and textures (NULL / 0 when it has
+ * none); valid until apt_scene_data_destroy. */
+apt_status apt_scene_data_attribs(const apt_scene_data* s, apt_mesh_attribs* out);
 void apt_scene_data_destroy(apt_scene_data* s);
 
 /* The bvh_stress ray sets: n_coherent primary rays of a side x side pinhole camera at
@@ -436,6 +479,9 @@ apt_status apt_world_action(apt_renderer* r, uint32_t ty, const void* bytes, siz
 
 /* Replace the whole scene with a triangle soup (procedural or caller-made). */
 apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_sky* sky);
+/* apt_scene_set with uvs, tangent frames and textures (attribs may be NULL). */
+apt_status apt_scene_set_attribs(apt_renderer* r, const apt_mesh_desc* mesh, const apt_mesh_attribs* attribs,
+                                 const apt_sky* sky);
 
 /* PathTracerGpu::render: renders rows [row_start, row_end) of a width x height
  * frame.  Writes RGBA8 in 64x64 block-major order (resolve.wgsl:56-57,

@@ -218,6 +218,27 @@ pub struct apt_mesh_desc {
     pub materials: *const apt_material,
 }
 
+/// One decoded texture: width * height RGBA8 texels, row major.
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct apt_texture {
+    pub width: u32,
+    pub height: u32,
+    pub rgba8: *const u32,
+}
+
+/// Per-vertex uvs, tangent frames and the scene's textures (NULL / 0 when absent).
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct apt_mesh_attribs {
+    pub uvs: *const f32,
+    pub tangents: *const f32,
+    pub bitangents: *const f32,
+    pub textures: *const apt_texture,
+    pub n_textures: u32,
+    pub _pad: u32,
+}
+
 /// CameraUpdateData (crates/appearance-world/src/visible_world_action.rs:9-15), 80 B.
 #[repr(C)]
 #[derive(Clone, Copy, Default, Debug)]
@@ -418,6 +439,7 @@ extern "C" {
     pub fn apt_scene_load_gltf(path: *const c_char, out: *mut *mut apt_scene_data) -> apt_status;
     pub fn apt_scene_data_view(s: *const apt_scene_data, mesh: *mut apt_mesh_desc, camera: *mut apt_camera,
                                sky: *mut apt_sky) -> apt_status;
+    pub fn apt_scene_data_attribs(s: *const apt_scene_data, out: *mut apt_mesh_attribs) -> apt_status;
     pub fn apt_scene_data_destroy(s: *mut apt_scene_data);
     pub fn apt_rays_bvh_stress(seed: u32, side: u32, n_incoherent: u32, out: *mut apt_ray) -> apt_status;
 
@@ -428,6 +450,8 @@ extern "C" {
     pub fn apt_set_config(r: *mut apt_renderer, cfg: *const apt_config) -> apt_status;
     pub fn apt_world_action(r: *mut apt_renderer, ty: u32, bytes: *const c_void, len: usize) -> apt_status;
     pub fn apt_scene_set(r: *mut apt_renderer, mesh: *const apt_mesh_desc, sky: *const apt_sky) -> apt_status;
+    pub fn apt_scene_set_attribs(r: *mut apt_renderer, mesh: *const apt_mesh_desc, attribs: *const apt_mesh_attribs,
+                                 sky: *const apt_sky) -> apt_status;
     pub fn apt_render(r: *mut apt_renderer, width: u32, height: u32, row_start: u32, row_end: u32,
                       out_rgba: *mut u8, out_len: usize, out_accum_rgb: *mut f32, accum_len: usize)
         -> apt_status;

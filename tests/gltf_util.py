@@ -89,10 +89,12 @@ def read_glb(path):
     return (np.concatenate(pos), np.concatenate(nrm), np.concatenate(mids), doc)
 
 
-def write_glb(path, nodes, meshes, materials, scene_roots, extensions=()):
+def write_glb(path, nodes, meshes, materials, scene_roots, extensions=(), images=(), textures=None):
     """meshes: list of lists of primitives, each a dict with positions (v,3),
-    indices (t*3,), optional normals (v,3), material, index_type (5125 or
-    5123), interleaved (positions and normals in one strided view)."""
+    indices (t*3,), optional normals (v,3), uvs (v,2), tangents (v,4),
+    material, index_type (5125 or 5123), interleaved (positions and normals
+    in one strided view).  images: (bytes, mime type) pairs stored in buffer
+    views; textures: list of image indices (default: one texture per image)."""
     binc = bytearray()
     accessors, views = [], []
 
@@ -133,6 +135,12 @@ def write_glb(path, nodes, meshes, materials, scene_roots, extensions=()):
                 if "normals" in p:
                     attrs["NORMAL"] = add_acc(add_view(np.asarray(p["normals"], np.float32).tobytes()), 5126,
                                               len(P), "VEC3")
+            if "uvs" in p:
+                attrs["TEXCOORD_0"] = add_acc(add_view(np.asarray(p["uvs"], np.float32).tobytes()), 5126, len(P),
+                                              "VEC2")
+            if "tangents" in p:
+                attrs["TANGENT"] = add_acc(add_view(np.asarray(p["tangents"], np.float32).tobytes()), 5126, len(P),
+                                           "VEC4")
             it = p.get("index_type", 5125)
             I = np.asarray(p["indices"], np.uint32 if it == 5125 else np.uint16)
             prim = {"attributes": attrs, "indices": add_acc(add_view(I.tobytes()), it, len(I), "SCALAR")}
@@ -142,11 +150,15 @@ def write_glb(path, nodes, meshes, materials, scene_roots, extensions=()):
                 prim["mode"] = p["mode"]
             gp.append(prim)
         gl_meshes.append({"primitives": gp})
+    gl_images = [{"bufferView": add_view(bytes(data)), "mimeType": mime} for data, mime in images]
     while len(binc) % 4:
         binc.append(0)
     doc = {"asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": list(scene_roots)}], "nodes": nodes,
            "meshes": gl_meshes, "materials": materials, "accessors": accessors, "bufferViews": views,
            "buffers": [{"byteLength": len(binc)}]}
+    if gl_images:
+        doc["images"] = gl_images
+        doc["textures"] = [{"source": i} for i in (textures if textures is not None else range(len(gl_images)))]
     if extensions:
         doc["extensionsUsed"] = list(extensions)
     js = json.dumps(doc).encode()

@@ -4,7 +4,8 @@ apps/render-host/src/lib.rs:138) re-encoded as data the GPU tests can load
 where the reference's assets are absent.  Geometry is flattened to world
 space by the numpy reader (tests/gltf_util.read_glb), one mesh per material,
 and the materials keep every factor and extension of the original; texture
-references and images are dropped (this build does not decode textures).
+references and images are dropped (the light's emission texture, a 956 KB
+screenshot, would outweigh the rest of the fixture a hundred times).
 
 Usage: python tests/golden/make_cornell_conor.py
 """
