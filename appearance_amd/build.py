@@ -28,6 +28,8 @@ SOURCES = [
     "tuner.cpp",
     "restir.hip",
     "restir_two_level.hip",
+    "restir_textured.hip",
+    "restir_textured_two_level.hip",
     "bvh_gpu.hip",
     "gltf.cpp",
     "image.cpp",

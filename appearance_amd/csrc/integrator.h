@@ -60,6 +60,22 @@ struct SceneView {
   float cam_tan_half, cam_aspect;
 };
 
+// A kernel's scene view with the textured-scene arrays known to be absent at
+// compile time when TEX is false: every textured branch of the shading code
+// (tested on these pointers) folds away, so kernels for untextured scenes
+// keep the registers and occupancy they have without texture support.
+template <bool TEX>
+__device__ __forceinline__ SceneView scene_view(const SceneView& s) {
+  SceneView v = s;
+  if constexpr (!TEX) {
+    v.tri_uv = nullptr;
+    v.tri_tan = nullptr;
+    v.tri_bit = nullptr;
+    v.tex = nullptr;
+  }
+  return v;
+}
+
 // Path state, structure of arrays over path slots.
 struct PathView {
   float4* o_pdf;    // origin, pdf of the BSDF sample that made the ray (0: camera)

@@ -78,8 +78,12 @@ __global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_extend(SceneView sc, Pa
 }
 
 // Four waves per SIMD: the compiler's own choice (136 VGPRs) allows three.
-__global__ __launch_bounds__(BLOCK, 4) void k_shade(FrameView fv, SceneView sc, PathView ps, int qin,
+// TEX: the scene has textures (the textured shading code is compiled in, at
+// three waves per SIMD: at four it spills).
+template <bool TEX>
+__global__ __launch_bounds__(BLOCK, TEX ? 3 : 4) void k_shade(FrameView fv, SceneView sc_, PathView ps, int qin,
                                                  uint32_t bounce) {
+  const SceneView sc = scene_view<TEX>(sc_);
   const uint32_t count = ps.counters[CNT_QUEUE0 + qin];
   // A grid of a few work-groups per CU strides over the queue: work-group b
   // takes tiles b, b + G, ..., so a short queue (a late bounce) still spreads
@@ -213,7 +217,8 @@ void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_p
     hipLaunchKernelGGL((k_extend<W, S_, INST>), pool, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
     mark(1);
     mark(2);
-    hipLaunchKernelGGL(k_shade, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
+    if (sc.tex) hipLaunchKernelGGL(k_shade<true>, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
+    else hipLaunchKernelGGL(k_shade<false>, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
     mark(3);
     mark(4);
     hipLaunchKernelGGL((k_shadow<W, S_, INST>), pool, dim3(BLOCK), 0, s, sc, ps, nodes);

@@ -112,11 +112,13 @@ constexpr uint32_t RQ_WORDS = RQ_POOL + POOL_WORDS;
 // b.radiance.
 apt_status restir_sample(const RestirParams& p, const SceneView& sc, const RestirBuffers& b,
                          hipStream_t s);
-// restir_sample for a two-level scene (sc.insts set), compiled in its own
-// translation unit (restir_two_level.hip) so the two variants build in
-// parallel.
-apt_status restir_sample_two_level(const RestirParams& p, const SceneView& sc, const RestirBuffers& b,
-                                   hipStream_t s);
+// restir_sample's four variants: two-level scene (sc.insts set) or not,
+// textured scene (sc.tex set) or not, each compiled in a translation unit of
+// its own (restir.hip, restir_two_level.hip, restir_textured.hip,
+// restir_textured_two_level.hip) so they build in parallel.
+template <bool INST, bool TEX>
+apt_status restir_sample_variant(const RestirParams& p, const SceneView& sc, const RestirBuffers& b,
+                                 hipStream_t s);
 // Post chain of the frame: demodulation, firefly filter, SVGF, TAA,
 // remodulation; adds the result to frame_sum.
 apt_status restir_post(const RestirParams& p, const RestirBuffers& b, bool firefly, bool svgf,

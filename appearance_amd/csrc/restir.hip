@@ -32,6 +32,13 @@
 #include "pathtrace_device.h"
 #include "restir.h"
 
+#ifndef APT_RESTIR_INST
+#define APT_RESTIR_INST 0
+#endif
+#ifndef APT_RESTIR_TEX
+#define APT_RESTIR_TEX 0
+#endif
+
 namespace apt {
 
 namespace {
@@ -473,6 +480,10 @@ constexpr int RS_TRACE_WAVES = 4;
 // waves per SIMD without spills once its first round (area candidate 0 and
 // the BSDF candidate) is peeled off the loop; five spill.
 constexpr int RS_RIS_WAVES = 4;
+// The textured (material textures, normal maps) and two-level (instance
+// traversal) variants hold more live state; at four waves they spill (28 to
+// 70 registers), so they run three.
+constexpr int RS_WIDE_WAVES = 3;
 
 // ---- ray stages --------------------------------------------------------------
 // The rays that dominate the frame (the path's extension ray, the BSDF
@@ -529,7 +540,9 @@ __global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_rs_trace(SceneView sc, 
                                              BLOCK, s_top, ktop, sc.insts);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
+template <bool TEX>
+__global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView sc_, RestirBuffers b, uint32_t bounce) {
+  const SceneView sc = scene_view<TEX>(sc_);
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   const float4 t4 = b.thr[pix];
@@ -602,7 +615,9 @@ __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView s
 // one BSDF candidate.  MIS is the balance heuristic over each candidate's own
 // densities under both strategies (the reference mixes the densities of two
 // different candidates), so W = wsum / p^ is unbiased.
-__global__ __launch_bounds__(BLOCK8, RS_RIS_WAVES) void k_rs_di_ris(RestirParams p, SceneView sc, RestirBuffers b) {
+template <bool TEX>
+__global__ __launch_bounds__(BLOCK8, TEX ? RS_WIDE_WAVES : RS_RIS_WAVES) void k_rs_di_ris(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
   Vtx v;
@@ -694,7 +709,9 @@ __global__ __launch_bounds__(BLOCK8, RS_RIS_WAVES) void k_rs_di_ris(RestirParams
 
 // The BSDF candidate of k_rs_di_ris: the same draws (after the first area
 // candidate's five), the same sample, its closest-hit ray queued.
-__global__ __launch_bounds__(BLOCK) void k_rs_di_bsdf_ray(RestirParams p, SceneView sc, RestirBuffers b) {
+template <bool TEX>
+__global__ __launch_bounds__(BLOCK) void k_rs_di_bsdf_ray(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   if (blockIdx.x == 0) pool_zero(b.rq_ctr + RQ_POOL, threadIdx.x);  // for the trace that follows
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
@@ -731,8 +748,9 @@ __global__ __launch_bounds__(BLOCK) void k_rs_di_vis(RestirBuffers b, uint32_t n
 // one BSDF candidate whose target is the approximate one-bounce estimate.
 // Roulette keeps the path with probability min(1, max T) (the reference
 // divides by max T even above 1), and a path it ends keeps this vertex's DI.
-template <int W, int S, bool INST>
-__global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView sc, RestirBuffers b, uint32_t bounce) {
+template <int W, int S, bool INST, bool TEX>
+__global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView sc_, RestirBuffers b, uint32_t bounce) {
+  const SceneView sc = scene_view<TEX>(sc_);
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
@@ -777,8 +795,9 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_ris(RestirParams p, SceneView 
 
 // restir_di_temporal.wgsl:68-176 (nearest previous pixel, stochastic
 // rounding), with the visibility-tested target.
-template <int W, int S, bool INST>
-__global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(RestirParams p, SceneView sc, RestirBuffers b) {
+template <int W, int S, bool INST, bool TEX>
+__global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
@@ -825,7 +844,9 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_temporal(Resti
 // takes it from this one (as a neighbour): the same number, so it is
 // evaluated once here, the BSDF part into di_self and the shadow ray (the
 // one visible() would trace) queued for the pool kernel.
-__global__ __launch_bounds__(BLOCK) void k_rs_di_self(RestirParams p, SceneView sc, RestirBuffers b) {
+template <bool TEX>
+__global__ __launch_bounds__(BLOCK) void k_rs_di_self(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
@@ -876,8 +897,9 @@ __device__ __forceinline__ uint32_t mirror_px(int x, int m) {
 }
 
 // restir_di_spatial.wgsl:70-162 (one pass, unbiased combination).
-template <int W, int S, bool INST>
-__global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(RestirParams p, SceneView sc, RestirBuffers b) {
+template <int W, int S, bool INST, bool TEX>
+__global__ __launch_bounds__(BLOCK8, TEX || INST ? RS_WIDE_WAVES : RS_TRACE_WAVES) void k_rs_di_spatial(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
@@ -966,8 +988,9 @@ __global__ __launch_bounds__(BLOCK8, RS_TRACE_WAVES) void k_rs_di_spatial(Restir
 }
 
 // restir_gi_temporal.wgsl:69-115 (M capped at gi_mcap, biased combination).
-template <int W, int S, bool INST>
-__global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, SceneView sc, RestirBuffers b) {
+template <int W, int S, bool INST, bool TEX>
+__global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
@@ -1022,8 +1045,9 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
 
 // apply_di.wgsl:48-111: the reservoir's sample, visibility tested, weighted by
 // W, with the throughput at the vertex.
-template <int W, int S, bool INST>
-__global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneView sc, RestirBuffers b) {
+template <int W, int S, bool INST, bool TEX>
+__global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   __shared__ uint2 s_stack[S * BLOCK8];
   uint32_t x, y, pix;
   if (!tile_pixel<TILE8>(p, x, y, pix)) return;
@@ -1046,7 +1070,9 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_apply_di(RestirParams p, SceneVie
 }
 
 // apply_gi.wgsl:44-99: the next ray heads for the GI sample point.
-__global__ __launch_bounds__(BLOCK) void k_rs_apply_gi(RestirParams p, SceneView sc, RestirBuffers b) {
+template <bool TEX>
+__global__ __launch_bounds__(BLOCK) void k_rs_apply_gi(RestirParams p, SceneView sc_, RestirBuffers b) {
+  const SceneView sc = scene_view<TEX>(sc_);
   uint32_t x, y, pix;
   if (!tile_pixel(p, x, y, pix)) return;
   Vtx v;
@@ -1321,7 +1347,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_remod(RestirParams p, RestirBuffer
   if (fin != b.demod[p.cur]) b.demod[p.cur][pix] = c;
 }
 
-template <int W, int S, bool INST>
+template <int W, int S, bool INST, bool TEX>
 apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
   const uint32_t tiles = ((p.width + TILE - 1) / TILE) * ((p.rows + TILE - 1) / TILE);
   const uint32_t tiles8 = ((p.width + TILE8 - 1) / TILE8) * ((p.rows + TILE8 - 1) / TILE8);
@@ -1333,27 +1359,27 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
     const dim3 pool(std::min<uint32_t>(pool_blocks(), (npix + 63) / 64));
     hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
     hipLaunchKernelGGL((k_rs_trace<W, S, false, true, INST>), pool, block, 0, s, sc, b, npix);
-    hipLaunchKernelGGL(k_rs_extend, grid, block, 0, s, p, sc, b, bounce);
-    hipLaunchKernelGGL(k_rs_di_bsdf_ray, grid, block, 0, s, p, sc, b);  // also resets the pool counters
+    hipLaunchKernelGGL(k_rs_extend<TEX>, grid, block, 0, s, p, sc, b, bounce);
+    hipLaunchKernelGGL(k_rs_di_bsdf_ray<TEX>, grid, block, 0, s, p, sc, b);  // also resets the pool counters
     hipLaunchKernelGGL((k_rs_trace<W, S, false, false, INST>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
-    hipLaunchKernelGGL(k_rs_di_ris, grid8, block8, 0, s, p, sc, b);
+    hipLaunchKernelGGL(k_rs_di_ris<TEX>, grid8, block8, 0, s, p, sc, b);
     hipLaunchKernelGGL((k_rs_trace<W, S, true, false, INST>), pool, block, 0, s, sc, b, npix);
     hipLaunchKernelGGL(k_rs_di_vis, dim3((npix + BLOCK - 1) / BLOCK), block, 0, s, b, npix);
-    if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S, INST>), grid8, block8, 0, s, p, sc, b, bounce);
+    if (more) hipLaunchKernelGGL((k_rs_gi_ris<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b, bounce);
     if (bounce == 0 && p.restir_di) {
-      hipLaunchKernelGGL((k_rs_di_temporal<W, S, INST>), grid8, block8, 0, s, p, sc, b);
+      hipLaunchKernelGGL((k_rs_di_temporal<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
       if (p.neighbours > 0) {
         hipLaunchKernelGGL(k_rq_reset, dim3(1), dim3(64), 0, s, b);
-        hipLaunchKernelGGL(k_rs_di_self, grid, block, 0, s, p, sc, b);
+        hipLaunchKernelGGL(k_rs_di_self<TEX>, grid, block, 0, s, p, sc, b);
         hipLaunchKernelGGL((k_rs_trace<W, S, true, false, INST>), pool, block, 0, s, sc, b, npix);
-        hipLaunchKernelGGL((k_rs_di_spatial<W, S, INST>), grid8, block8, 0, s, p, sc, b);
+        hipLaunchKernelGGL((k_rs_di_spatial<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
       }
     }
     if (bounce == 0 && more && p.restir_gi)
-      hipLaunchKernelGGL((k_rs_gi_temporal<W, S, INST>), grid8, block8, 0, s, p, sc, b);
-    hipLaunchKernelGGL((k_rs_apply_di<W, S, INST>), grid8, block8, 0, s, p, sc, b);
-    if (more) hipLaunchKernelGGL(k_rs_apply_gi, grid, block, 0, s, p, sc, b);
+      hipLaunchKernelGGL((k_rs_gi_temporal<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
+    hipLaunchKernelGGL((k_rs_apply_di<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
+    if (more) hipLaunchKernelGGL(k_rs_apply_gi<TEX>, grid, block, 0, s, p, sc, b);
   }
   APT_HIP(hipGetLastError());
   return APT_OK;
@@ -1361,33 +1387,34 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
 
 }  // namespace
 
-#ifndef APT_RESTIR_TWO_LEVEL
-apt_status restir_sample(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
-  if (sc.insts) return restir_sample_two_level(p, sc, b, s);
+// This file is compiled four times, once per variant: two-level scene or not
+// (APT_RESTIR_INST), textured scene or not (APT_RESTIR_TEX); each translation
+// unit defines its variant's restir_sample_variant, and the plain one also the
+// dispatch and the post chain.
+template <>
+apt_status restir_sample_variant<APT_RESTIR_INST != 0, APT_RESTIR_TEX != 0>(const RestirParams& p,
+                                                                          const SceneView& sc,
+                                                                          const RestirBuffers& b, hipStream_t s) {
+  constexpr bool INST = APT_RESTIR_INST != 0, TEX = APT_RESTIR_TEX != 0;
   switch (sc.width) {
-    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_, false>(p, sc, b, s)));
-    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_, false>(p, sc, b, s)));
-    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_, false>(p, sc, b, s)));
+    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_, INST, TEX>(p, sc, b, s)));
+    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_, INST, TEX>(p, sc, b, s)));
+    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_, INST, TEX>(p, sc, b, s)));
     default: break;
   }
   set_error("restir: unsupported BVH width %u", sc.width);
   return APT_ERR_INVALID_ARG;
 }
-#else
-apt_status restir_sample_two_level(const RestirParams& p, const SceneView& sc, const RestirBuffers& b,
-                                   hipStream_t s) {
-  switch (sc.width) {
-    case 2: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<2, S_, true>(p, sc, b, s)));
-    case 4: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<4, S_, true>(p, sc, b, s)));
-    case 8: APT_FOR_STACK_DEPTH(sc.lds_stack, return (sample_t<8, S_, true>(p, sc, b, s)));
-    default: break;
-  }
-  set_error("restir: unsupported BVH width %u", sc.width);
-  return APT_ERR_INVALID_ARG;
+
+#if !APT_RESTIR_INST && !APT_RESTIR_TEX
+apt_status restir_sample(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
+  if (sc.tex)
+    return sc.insts ? restir_sample_variant<true, true>(p, sc, b, s) : restir_sample_variant<false, true>(p, sc, b, s);
+  return sc.insts ? restir_sample_variant<true, false>(p, sc, b, s) : restir_sample_variant<false, false>(p, sc, b, s);
 }
 #endif
 
-#ifndef APT_RESTIR_TWO_LEVEL
+#if !APT_RESTIR_INST && !APT_RESTIR_TEX
 apt_status restir_post(const RestirParams& p, const RestirBuffers& b, bool firefly, bool svgf, bool taa,
                        float4* frame_sum, hipStream_t s) {
   const uint32_t tiles = ((p.width + TILE - 1) / TILE) * ((p.rows + TILE - 1) / TILE);
@@ -1413,6 +1440,6 @@ apt_status restir_post(const RestirParams& p, const RestirBuffers& b, bool firef
   return APT_OK;
 }
 
-#endif  // APT_RESTIR_TWO_LEVEL
+#endif  // the plain variant
 
 }  // namespace apt

@@ -113,7 +113,8 @@ __global__ __launch_bounds__(BLOCK) void k_srv_shade(const SrvReq* __restrict__ 
     if (i < count) {
       p = ps.queue[qin][i];
       const SrvReq& r = reqs[path_req[p]];
-      shade_path(r.sc, ps, p, bounce, r.max_bounces, r.rr_start, alive, want_shadow);
+      // resident scenes carry no textures
+      shade_path(scene_view<false>(r.sc), ps, p, bounce, r.max_bounces, r.rr_start, alive, want_shadow);
     }
     int slot, sslot;
     block_append2<BLOCK>(alive, &ps.counters[CNT_QUEUE0 + (qin ^ 1)], slot, want_shadow,

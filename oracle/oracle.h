@@ -34,16 +34,16 @@ float or_aces(float x);
 int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
               uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
               const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
-              uint64_t* counts);
+              uint64_t* counts, const void* attribs);
 int or_render_counted(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
                       uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
                       const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
-                      uint64_t* counts, uint32_t n_counts);
+                      uint64_t* counts, uint32_t n_counts, const void* attribs);
 void* or_restir_create(uint32_t width, uint32_t rows);
 void or_restir_destroy(void* st);
 int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
                     uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg20,
-                    int threads, float* out, uint64_t* rays);
+                    int threads, float* out, uint64_t* rays, const void* attribs);
 void or_set_plant(int kind);
 int or_reservoir_probe(uint32_t kind, const float* a, const float* b, const float* extra, uint32_t* rng,
                        float* out);
@@ -53,9 +53,12 @@ void or_atrous_probe(uint32_t width, uint32_t height, uint32_t pass, const float
 int or_post_probe(uint32_t kind, uint32_t width, uint32_t height, uint32_t frames, const float* imgs,
                   const uint8_t* sky, uint32_t svgf_history, float* out, float* out_var);
 double or_disney_eval(const void* mat, const double* n, const double* wo, const double* wi, int back,
-                      double* f);
+                      double* f, const double* ncc);
 double or_disney_sample(const void* mat, const double* n, const double* wo, int back, double r0, double r1,
-                        double r2, double* wi, double* f);
+                        double r2, double* wi, double* f, const double* ncc);
+void or_tex_sample(const void* tex, float u, float v, float* out4);
+void or_shading_probe(const void* mat, const void* attribs, uint32_t prim, float bu, float bv, const float* n,
+                      float* out);
 
 #ifdef __cplusplus
 }

@@ -301,6 +301,173 @@ Disney<R> from_mat(const Mat& m) {
   return d;
 }
 
+// ---- textured materials (shared/material/material_pool_bindings.wgsl:15-154) -----
+// A texture as the library receives it (apt_texture): width x height RGBA8
+// texels, row major, R in the low byte.
+struct Texture {
+  uint32_t width, height;
+  const uint32_t* rgba8;
+};
+// Per-vertex attributes of a textured scene (each NULL when absent), in world
+// space as the library places them: uv n*6 floats, tan and bit n*9 floats (the
+// unit tangent and bitangent of each vertex), and the scene's textures.
+struct Attribs {
+  const float* uv;
+  const float* tan;
+  const float* bit;
+  const Texture* tex;
+  uint32_t n_tex, pad;
+};
+constexpr uint32_t NO_TEX = 0xFFFFFFFFu;
+
+// textureSampleLevel(tex, sampler, uv, 0.0) (material_pool_bindings.wgsl:15-17)
+// with the reference's sampler (linear filtering, repeat addressing,
+// src/scene_resources/material_pool.rs:68-74) on an RGBA8 unorm texture: the
+// sample point in texel space is (u W - 1/2, v H - 1/2); the four texels around
+// it, their indices wrapped modulo the size, are weighted by the fractional
+// parts.  A texel channel c reads as c / 255.
+template <typename R>
+void tex_sample(const Texture& t, R u, R v, R out[4]) {
+  R x = u * (R)t.width - R(0.5), y = v * (R)t.height - R(0.5);
+  if (!std::isfinite(x)) x = R(-0.5);
+  if (!std::isfinite(y)) y = R(-0.5);
+  const R fx = std::floor(x), fy = std::floor(y);
+  const R ax = x - fx, ay = y - fy;
+  auto wrap = [](double i, uint32_t n) {
+    double m = std::fmod(i, (double)n);
+    if (m < 0) m += n;
+    return std::min((uint32_t)m, n - 1);
+  };
+  const uint32_t x0 = wrap(fx, t.width), x1 = wrap((double)fx + 1.0, t.width);
+  const uint32_t y0 = wrap(fy, t.height), y1 = wrap((double)fy + 1.0, t.height);
+  const uint32_t q[4] = {t.rgba8[(size_t)y0 * t.width + x0], t.rgba8[(size_t)y0 * t.width + x1],
+                         t.rgba8[(size_t)y1 * t.width + x0], t.rgba8[(size_t)y1 * t.width + x1]};
+  for (int c = 0; c < 4; ++c) {
+    R k[4];
+    for (int j = 0; j < 4; ++j) k[j] = (R)((q[j] >> (8 * c)) & 0xFFu) / R(255);
+    out[c] = (k[0] * (R(1) - ax) + k[1] * ax) * (R(1) - ay) + (k[2] * (R(1) - ax) + k[3] * ax) * ay;
+  }
+}
+
+// srgb_to_linear (assets/shaders/color.wgsl:9-15), one channel.
+template <typename R> R srgb_lin(R c) {
+  return c < R(0.04045) ? c / R(12.92) : std::pow((c + R(0.055)) / R(1.055), R(2.4));
+}
+
+// The material at a surface point (Material::from_material_descriptor,
+// material_pool_bindings.wgsl:126-154): each factor times its texture where the
+// slot holds one and the factor is nonzero (:20-92; the colour's test is on
+// vec4(color, 1), which always passes); colour and sheen tint through
+// srgb_to_linear; luminance = the colour texture's alpha.
+template <typename R>
+struct MatEval {
+  V<R> color, emission, sheen_tint;
+  R luminance, metallic, roughness, transmission, sheen, clearcoat, clearcoat_roughness;
+};
+
+template <typename R>
+MatEval<R> eval_material(const Attribs* at, const Mat& m, R u, R v) {
+  MatEval<R> e;
+  e.color = f3<R>(m.color);
+  e.emission = f3<R>(m.emission);
+  e.sheen_tint = f3<R>(m.sheen_tint);
+  e.luminance = R(1);
+  e.metallic = m.metallic;
+  e.roughness = m.roughness;
+  e.transmission = m.transmission;
+  e.sheen = m.sheen;
+  e.clearcoat = m.clearcoat;
+  e.clearcoat_roughness = m.clearcoat_roughness;
+  if (!at || !at->tex) return e;
+  R c[4];
+  auto tex = [&](uint32_t id) {
+    if (id == NO_TEX || id >= at->n_tex) return false;
+    tex_sample<R>(at->tex[id], u, v, c);
+    return true;
+  };
+  if (tex(m.color_texture)) {
+    e.color = V<R>(e.color.x * srgb_lin(c[0]), e.color.y * srgb_lin(c[1]), e.color.z * srgb_lin(c[2]));
+    e.luminance = c[3];
+  }
+  if (dotp(e.emission, e.emission) > R(0) && tex(m.emission_texture))
+    e.emission = V<R>(e.emission.x * c[0], e.emission.y * c[1], e.emission.z * c[2]);
+  if ((e.metallic > R(0) || e.roughness > R(0)) && tex(m.mr_tex)) {
+    e.metallic *= c[2];
+    e.roughness *= c[1];
+  }
+  if (e.clearcoat > R(0) && tex(m.clearcoat_texture)) e.clearcoat *= c[0];
+  if (e.clearcoat_roughness > R(0) && tex(m.cr_tex)) e.clearcoat_roughness *= c[1];
+  if (e.transmission > R(0) && tex(m.transmission_texture)) e.transmission *= c[0];
+  if (e.sheen > R(0) && tex(m.sheen_texture)) e.sheen *= c[0];
+  if (dotp(e.sheen_tint, e.sheen_tint) > R(0) && tex(m.sheen_tint_texture))
+    e.sheen_tint = V<R>(e.sheen_tint.x * srgb_lin(c[0]), e.sheen_tint.y * srgb_lin(c[1]),
+                        e.sheen_tint.z * srgb_lin(c[2]));
+  return e;
+}
+
+// DisneyBsdf::from_material (disney_bsdf.wgsl:226-244) of an evaluated material.
+template <typename R>
+Disney<R> from_eval(const Mat& m, const MatEval<R>& e) {
+  Disney<R> d = from_mat<R>(m);
+  d.color = e.color;
+  d.metallic = e.metallic;
+  d.luminance = e.luminance;
+  d.specular = std::max((R)m.specular, R(1) - e.roughness);
+  d.roughness = e.roughness;
+  d.sheen = e.sheen;
+  d.sheen_tint = e.sheen_tint;
+  d.clearcoat = e.clearcoat;
+  d.clearcoat_gloss = R(1) - e.clearcoat_roughness;
+  d.transmission = e.transmission;
+  return d;
+}
+
+// Texture coordinates of a hit (vertex-interpolated TEXCOORD_0,
+// helpers/inline_path_tracer.wgsl:82); bu, bv weight vertices 1 and 2.
+template <typename R>
+void hit_uv(const Attribs* at, uint32_t prim, R bu, R bv, R& u, R& v) {
+  u = v = R(0);
+  if (!at || !at->uv) return;
+  const float* t = at->uv + 6 * (size_t)prim;
+  const R b0 = R(1) - bu - bv;
+  u = (R)t[0] * b0 + (R)t[2] * bu + (R)t[4] * bv;
+  v = (R)t[1] * b0 + (R)t[3] * bu + (R)t[5] * bv;
+}
+
+// The shading and clearcoat normals of a hit before the face flip
+// (helpers/inline_path_tracer.wgsl:103-165): the tangent frame interpolated
+// from the vertices (VertexPoolBindings::load_tbn, vertex_pool_bindings.wgsl:48-58)
+// and normalised; MaterialDescriptor::apply_normal_mapping (:98-110) bends the
+// interpolated normal n by the normal texture, blended back towards n by
+// normal_scale < 1; the clearcoat lobe takes n without a clearcoat normal
+// texture, the mapped normal when both textures are one, else its own map
+// (:112-123, no scale).  Without tangents nothing is mapped.
+template <typename R>
+void mapped_normals(const Attribs* at, const Mat& m, uint32_t prim, R bu, R bv, R u, R v, const V<R>& n,
+                    V<R>& nmap, V<R>& ncc) {
+  nmap = ncc = n;
+  if (!at || !at->tex || !at->tan) return;
+  const bool has_n = m.normal_texture != NO_TEX && m.normal_texture < at->n_tex && m.normal_scale > 0.0f;
+  const bool has_c = m.clearcoat_normal_texture != NO_TEX && m.clearcoat_normal_texture < at->n_tex;
+  if (!has_n && !has_c) return;
+  const R b0 = R(1) - bu - bv;
+  const float* tt = at->tan + 9 * (size_t)prim;
+  const float* tb = at->bit + 9 * (size_t)prim;
+  const V<R> T = unit(f3<R>(tt) * b0 + f3<R>(tt + 3) * bu + f3<R>(tt + 6) * bv);
+  const V<R> B = unit(f3<R>(tb) * b0 + f3<R>(tb + 3) * bu + f3<R>(tb + 6) * bv);
+  R c[4];
+  auto bend = [&](uint32_t id) {
+    tex_sample<R>(at->tex[id], u, v, c);
+    return unit(T * (c[0] * R(2) - R(1)) + B * (c[1] * R(2) - R(1)) + n * (c[2] * R(2) - R(1)));
+  };
+  if (has_n) {
+    nmap = bend(m.normal_texture);
+    const R s = m.normal_scale;
+    if (s < R(1)) nmap = unit(n * (R(1) - s) + nmap * s);
+  }
+  if (has_c) ncc = m.clearcoat_normal_texture == m.normal_texture ? nmap : bend(m.clearcoat_normal_texture);
+}
+
 template <typename R> R schlick(R u) {
   R m = cl(R(1) - u, R(0), R(1));
   return sq(sq(m)) * m;
@@ -421,7 +588,8 @@ V<R> reflect_about(const V<R>& i, const V<R>& n) { return i - n * (R(2) * dotp(n
 
 // evaluate(): f and the density with which sample() produces wi.
 template <typename R>
-V<R> disney_f(const Disney<R>& b, const Frame<R>& fr, const V<R>& wo, const V<R>& wi, bool back, R& pdf) {
+V<R> disney_f(const Disney<R>& b, const Frame<R>& fr, const Frame<R>& fc, const V<R>& wo, const V<R>& wi, bool back,
+             R& pdf) {
   pdf = R(0);
   if (b.transmission > R(0.5)) {
     V<R> ol = fr.local(wo), il = fr.local(wi);
@@ -466,7 +634,7 @@ V<R> disney_f(const Disney<R>& b, const Frame<R>& fr, const V<R>& wo, const V<R>
     if (p > R(0)) { pdf += w[2] * p; v = v + c; }
   }
   if (w[3] > R(0)) {
-    V<R> ol = fr.local(wo), il = fr.local(wi);
+    V<R> ol = fc.local(wo), il = fc.local(wi);
     R a = cc_alpha(b);
     R p = mf_lobe(b, a, a, ol, il, unit(ol + il), true, c);
     if (p > R(0)) { pdf += w[3] * p; v = v + c; }
@@ -481,7 +649,8 @@ struct Smp {
 };
 
 template <typename R>
-Smp<R> disney_sample(const Disney<R>& b, const Frame<R>& fr, const V<R>& wo, R dist, bool back, R r0, R r1, R r2) {
+Smp<R> disney_sample(const Disney<R>& b, const Frame<R>& fr, const Frame<R>& fc, const V<R>& wo, R dist, bool back,
+                     R r0, R r1, R r2) {
   Smp<R> s;
   if (r0 < b.transmission) {
     R r3 = r0 / b.transmission;
@@ -538,7 +707,8 @@ Smp<R> disney_sample(const Disney<R>& b, const Frame<R>& fr, const V<R>& wo, R d
   } else {
     bool cc = !(r3 < cz);
     lobe = cc ? 3 : 2;
-    V<R> ol = fr.local(wo);
+    const Frame<R>& fl = cc ? fc : fr;  // the clearcoat lobe lives in its own frame
+    V<R> ol = fl.local(wo);
     if (ol.z == R(0)) return s;
     R ax, ay;
     if (cc) ax = ay = cc_alpha(b);
@@ -554,7 +724,7 @@ Smp<R> disney_sample(const Disney<R>& b, const Frame<R>& fr, const V<R>& wo, R d
              : spec_F(b, ol, m) * (D_ggx(ax, ay, m) * G_ggx(ax, ay, il, ol));
     val = val * (R(1) / std::fabs(R(4) * ol.z * il.z));
     prob = w[lobe] * cp;
-    wi = fr.world(il);
+    wi = fl.world(il);
   }
   // add the other lobes in the order diffuse, sheen, specular, clearcoat
   V<R> c;
@@ -569,7 +739,7 @@ Smp<R> disney_sample(const Disney<R>& b, const Frame<R>& fr, const V<R>& wo, R d
     val = val + c;
   }
   if (lobe != 3 && w[3] > R(0)) {
-    V<R> ol = fr.local(wo), il = fr.local(wi);
+    V<R> ol = fc.local(wo), il = fc.local(wi);
     R a = cc_alpha(b);
     prob += w[3] * mf_lobe(b, a, a, ol, il, unit(ol + il), true, c);
     val = val + c;
@@ -629,6 +799,7 @@ struct Scene {
   const Cam* cam;
   const orc::Bvh* bvh;
   std::vector<uint32_t> emissive;
+  const Attribs* at = nullptr;  // uvs, tangent frames, textures (NULL: untextured)
 };
 
 struct Cfg {

@@ -78,8 +78,16 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
       ns = unit(f3<R>(n) * (R(1) - bu - bv) + f3<R>(n + 3) * bu + f3<R>(n + 6) * bv);
     }
     V<R> wo = -rd;
+    // textures at the hit; normal mapping on the interpolated normal, then the
+    // face flip of the mapped and clearcoat normals (inline_path_tracer.wgsl:103-165)
+    R tu, tv;
+    hit_uv(sc.at, prim, bu, bv, tu, tv);
+    V<R> nmap, ncc;
+    mapped_normals(sc.at, mt, prim, bu, bv, tu, tv, ns, nmap, ncc);
     bool back = dotp(wo, ns) < R(0);
-    if (back) ns = -ns;
+    ns = back ? -nmap : nmap;
+    if (back) ncc = -ncc;
+    const MatEval<R> me = eval_material<R>(sc.at, mt, tu, tv);
     bool emits = mt.emission[0] > 0.0f || mt.emission[1] > 0.0f || mt.emission[2] > 0.0f;
     if (emits && dotp(ng, rd) < R(0)) {
       R w = R(1);
@@ -87,12 +95,12 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
         R p_l = (R(1) - p_sun) / (R)ne / area_of(a, b, c) * (t * t) / (-dotp(ng, rd));
         w = power_h(pdf_prev, p_l);
       }
-      L = L + T * f3<R>(mt.emission) * w;
+      L = L + T * me.emission * w;
     }
     const bool lambert = mt.bsdf_model == 1u;
-    const V<R> albedo = f3<R>(mt.color);
-    const Disney<R> bd = from_mat<R>(mt);
-    const Frame<R> fr = basis(ns);
+    const V<R> albedo = me.color;
+    const Disney<R> bd = from_eval<R>(mt, me);
+    const Frame<R> fr = basis(ns), fc = basis(ncc);
     // NEE
     R u_sel = draw<R>(rng), u_a = draw<R>(rng), u_b = draw<R>(rng);
     if (p_sun > R(0) || ne > 0) {
@@ -117,7 +125,9 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
         wi = dv * (R(1) / dist);
         R cos_l = -dotp(nl, wi);
         const Mat& lm = sc.mats[sc.mat_ids ? sc.mat_ids[lp] : 0];
-        Le = f3<R>(lm.emission);
+        R lu, lv;  // the emission texture at the light point (helpers/nee.wgsl:39-51)
+        hit_uv(sc.at, lp, su * (R(1) - u_b), su * u_b, lu, lv);
+        Le = eval_material<R>(sc.at, lm, lu, lv).emission;
         p_l = cos_l > R(0) ? (R(1) - p_sun) / (R)ne / area_of(la, lb, lc) * (dist * dist) / cos_l : R(0);
       }
       R cos_i = dotp(wi, ns);
@@ -131,7 +141,7 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
             p_b = std::fabs(cos_i) / C<R>::pi;
           }
         } else {
-          f = disney_f(bd, fr, wo, wi, back, p_b);
+          f = disney_f(bd, fr, fc, wo, wi, back, p_b);
         }
         R w = power_h(p_l, p_b);
         V<R> cc = T * f * Le * (cos_i * w / p_l);
@@ -152,7 +162,7 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
       bs.f = albedo * (R(1) / C<R>::pi);
       if (!(bs.pdf > R(0))) bs.pdf = R(0);
     } else {
-      bs = disney_sample(bd, fr, wo, t, back, r0, r1, r2);
+      bs = disney_sample(bd, fr, fc, wo, t, back, r0, r1, r2);
     }
     if (!(bs.pdf > R(0))) break;
     V<R> Tn = T * bs.f * (std::fabs(dotp(bs.wi, ns)) / bs.pdf);
@@ -236,13 +246,14 @@ extern "C" {
 // pixel, row order) to out; cfg10 ends with the jitter switch (0: pixel
 // centres).  If pixels != NULL only those slab-local pixel
 // indices are rendered, in list order.  precision: 32 or 64.
-// counts[0..1] receive extension and shadow rays traced.
+// counts[0..1] receive extension and shadow rays traced.  attribs: an
+// ocr::Attribs (uvs, tangent frames, textures) or NULL.
 int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
               uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
               const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
-              uint64_t* counts) {
+              uint64_t* counts, const void* attribs) {
   return or_render_counted(pos, nrm, mat_ids, mats, n_tris, bvh, sky, cam, cfg10, pixels, n_pixels, precision,
-                           threads, out, counts, 2);
+                           threads, out, counts, 2, attribs);
 }
 
 // or_render with up to six counters: extension rays, shadow rays, box tests
@@ -252,9 +263,10 @@ int or_render(const float* pos, const float* nrm, const uint32_t* mat_ids, const
 int or_render_counted(const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
                       uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg10,
                       const uint32_t* pixels, uint32_t n_pixels, int precision, int threads, float* out,
-                      uint64_t* counts, uint32_t n_counts) {
+                      uint64_t* counts, uint32_t n_counts, const void* attribs) {
   if (n_counts > 6) return -1;
   ocr::Scene sc;
+  sc.at = static_cast<const ocr::Attribs*>(attribs);
   sc.pos = pos;
   sc.nrm = nrm;
   sc.mat_ids = mat_ids;
@@ -279,15 +291,17 @@ int or_render_counted(const float* pos, const float* nrm, const uint32_t* mat_id
   return 0;
 }
 
-// Disney BSDF probes for the invariance tests (FP64).  frame = normal only.
+// Disney BSDF probes for the invariance tests (FP64).  frame = normal only;
+// ncc (NULL: n) is the clearcoat lobe's normal.
 // eval: returns pdf, writes f.  sample: writes wi, f, returns pdf.
 double or_disney_eval(const void* mat, const double* n, const double* wo, const double* wi, int back,
-                      double* f) {
+                      double* f, const double* ncc) {
   ocr::Disney<double> b = ocr::from_mat<double>(*static_cast<const ocr::Mat*>(mat));
   ocr::Frame<double> fr = ocr::basis(ocr::V<double>(n[0], n[1], n[2]));
+  ocr::Frame<double> fc = ncc ? ocr::basis(ocr::V<double>(ncc[0], ncc[1], ncc[2])) : fr;
   double pdf;
-  ocr::V<double> v = ocr::disney_f(b, fr, ocr::V<double>(wo[0], wo[1], wo[2]), ocr::V<double>(wi[0], wi[1], wi[2]),
-                                   back != 0, pdf);
+  ocr::V<double> v = ocr::disney_f(b, fr, fc, ocr::V<double>(wo[0], wo[1], wo[2]),
+                                   ocr::V<double>(wi[0], wi[1], wi[2]), back != 0, pdf);
   f[0] = v.x;
   f[1] = v.y;
   f[2] = v.z;
@@ -295,10 +309,12 @@ double or_disney_eval(const void* mat, const double* n, const double* wo, const 
 }
 
 double or_disney_sample(const void* mat, const double* n, const double* wo, int back, double r0, double r1,
-                        double r2, double* wi, double* f) {
+                        double r2, double* wi, double* f, const double* ncc) {
   ocr::Disney<double> b = ocr::from_mat<double>(*static_cast<const ocr::Mat*>(mat));
   ocr::Frame<double> fr = ocr::basis(ocr::V<double>(n[0], n[1], n[2]));
-  ocr::Smp<double> s = ocr::disney_sample(b, fr, ocr::V<double>(wo[0], wo[1], wo[2]), 0.0, back != 0, r0, r1, r2);
+  ocr::Frame<double> fc = ncc ? ocr::basis(ocr::V<double>(ncc[0], ncc[1], ncc[2])) : fr;
+  ocr::Smp<double> s = ocr::disney_sample(b, fr, fc, ocr::V<double>(wo[0], wo[1], wo[2]), 0.0, back != 0, r0, r1,
+                                          r2);
   wi[0] = s.wi.x;
   wi[1] = s.wi.y;
   wi[2] = s.wi.z;
@@ -306,6 +322,34 @@ double or_disney_sample(const void* mat, const double* n, const double* wo, int 
   f[1] = s.f.y;
   f[2] = s.f.z;
   return s.pdf;
+}
+
+// Texture probe: textureSampleLevel of one texture (apt_texture layout) at
+// (u, v) in FP32; out4 = RGBA.
+void or_tex_sample(const void* tex, float u, float v, float* out4) {
+  ocr::tex_sample<float>(*static_cast<const ocr::Texture*>(tex), u, v, out4);
+}
+
+// Shading-point probe (FP32): triangle prim of a scene with attributes
+// `attribs` (an ocr::Attribs), hit at barycentrics (bu, bv) of vertices 1 and
+// 2, interpolated normal n, material mat.  out (26 floats): uv, mapped normal,
+// clearcoat normal, then the evaluated material: colour, luminance, metallic,
+// roughness, emission, transmission, sheen, sheen tint, clearcoat, clearcoat
+// roughness.
+void or_shading_probe(const void* mat, const void* attribs, uint32_t prim, float bu, float bv, const float* n,
+                      float* out) {
+  const ocr::Mat& m = *static_cast<const ocr::Mat*>(mat);
+  const ocr::Attribs* at = static_cast<const ocr::Attribs*>(attribs);
+  float u, v;
+  ocr::hit_uv<float>(at, prim, bu, bv, u, v);
+  ocr::V<float> nmap, ncc;
+  ocr::mapped_normals<float>(at, m, prim, bu, bv, u, v, ocr::V<float>(n[0], n[1], n[2]), nmap, ncc);
+  const ocr::MatEval<float> e = ocr::eval_material<float>(at, m, u, v);
+  const float vals[26] = {u, v, nmap.x, nmap.y, nmap.z, ncc.x, ncc.y, ncc.z, e.color.x, e.color.y, e.color.z,
+                          e.luminance, e.metallic, e.roughness, e.emission.x, e.emission.y, e.emission.z,
+                          e.transmission, e.sheen, e.sheen_tint.x, e.sheen_tint.y, e.sheen_tint.z, e.clearcoat,
+                          e.clearcoat_roughness, 0.0f, 0.0f};
+  for (int k = 0; k < 26; ++k) out[k] = vals[k];
 }
 
 }  // extern "C"

@@ -41,6 +41,8 @@ namespace orc {
 uint32_t hash_combine(uint32_t x, uint32_t y);
 uint32_t rgb9e5_pack(const float* rgb);
 void rgb9e5_unpack(uint32_t d, float* rgb);
+uint32_t oct30_pack(const float* n);
+void oct30_unpack(uint32_t p, float* n);
 }  // namespace orc
 
 namespace ors {
@@ -174,6 +176,8 @@ struct GiRes {
 
 struct Vtx {
   V3 x, n, wo, T;
+  V3 ncc;          // clearcoat lobe normal (front-facing)
+  F tu = 0, tv = 0;  // texture coordinates
   F t = -1;
   uint32_t mat = 0;
   bool back = false;
@@ -258,21 +262,47 @@ bool emits(const Mat& m) { return m.emission[0] > 0.0f || m.emission[1] > 0.0f |
 struct Bsdf {
   ocr::Disney<F> d;
   V3 albedo;
-  ocr::Frame<F> fr;
+  ocr::Frame<F> fr, fc;  // shading frame, clearcoat frame
   bool lambert, back;
   F t;
 };
 
-Bsdf make_bsdf(const Scene& sc, uint32_t mat, const V3& n, bool back, F t) {
+// The BSDF of material `mat` at a point with texture coordinates (tu, tv),
+// front-facing shading normal n and clearcoat normal ncc.
+Bsdf make_bsdf_at(const Scene& sc, uint32_t mat, const V3& n, const V3& ncc, F tu, F tv, bool back, F t) {
   const Mat& m = sc.s.mats[mat];
+  const ocr::MatEval<F> e = ocr::eval_material<F>(sc.s.at, m, tu, tv);
   Bsdf b;
   b.lambert = m.bsdf_model == 1u;
-  b.albedo = f3<F>(m.color);
-  b.d = ocr::from_mat<F>(m);
+  b.albedo = e.color;
+  b.d = ocr::from_eval<F>(m, e);
   b.fr = ocr::basis(n);
+  b.fc = ocr::basis(ncc);
   b.back = back;
   b.t = t;
   return b;
+}
+
+Bsdf make_bsdf(const Scene& sc, const Vtx& v) { return make_bsdf_at(sc, v.mat, v.n, v.ncc, v.tu, v.tv, v.back, v.t); }
+
+// Emission of a light triangle at barycentrics (u, v) of its vertices 1 and 2
+// (the emission texture at the light point, helpers/nee.wgsl:39-51).
+V3 light_emission(const Scene& sc, uint32_t light, F u, F v) {
+  F tu, tv;
+  ocr::hit_uv<F>(sc.s.at, light, u, v, tu, tv);
+  return ocr::eval_material<F>(sc.s.at, mat_of(sc, light), tu, tv).emission;
+}
+
+// Texture coordinates and the front-facing shading and clearcoat normals of a
+// hit whose interpolated normal is n (helpers/inline_path_tracer.wgsl:103-165).
+void hit_shading(const Scene& sc, uint32_t prim, F bu, F bv, const V3& n, const V3& wo, F& tu, F& tv, V3& ns,
+                 V3& ncc, bool& back) {
+  ocr::hit_uv<F>(sc.s.at, prim, bu, bv, tu, tv);
+  V3 nmap;
+  ocr::mapped_normals<F>(sc.s.at, mat_of(sc, prim), prim, bu, bv, tu, tv, n, nmap, ncc);
+  back = dotp(wo, n) < 0.0f;
+  ns = back ? -nmap : nmap;
+  if (back) ncc = -ncc;
 }
 
 V3 bsdf_eval(const Bsdf& b, const V3& wo, const V3& wi, F& pdf) {
@@ -285,7 +315,7 @@ V3 bsdf_eval(const Bsdf& b, const V3& wo, const V3& wi, F& pdf) {
     pdf = std::fabs(ci) * (1.0f / PI);
     return b.albedo * (1.0f / PI);
   }
-  return ocr::disney_f(b.d, b.fr, wo, wi, b.back, pdf);
+  return ocr::disney_f(b.d, b.fr, b.fc, wo, wi, b.back, pdf);
 }
 
 ocr::Smp<F> bsdf_sample(const Bsdf& b, const V3& wo, F r0, F r1, F r2) {
@@ -299,7 +329,7 @@ ocr::Smp<F> bsdf_sample(const Bsdf& b, const V3& wo, F r0, F r1, F r2) {
     if (!(s.pdf > 0.0f)) s.pdf = 0.0f;
     return s;
   }
-  return ocr::disney_sample(b.d, b.fr, wo, b.t, b.back, r0, r1, r2);
+  return ocr::disney_sample(b.d, b.fr, b.fc, wo, b.t, b.back, r0, r1, r2);
 }
 
 V3 sun_to(const Sky& k) { return V3(-k.sun_direction[0], -k.sun_direction[1], -k.sun_direction[2]); }
@@ -360,7 +390,7 @@ bool eval_light(const Scene& sc, uint32_t light, F u, F v, const V3& x, LightHit
   e.wi = unit(dv);
   const F cos_l = -dotp(nl, e.wi);
   if (cos_l > 0.0f) {
-    e.Le = f3<F>(mat_of(sc, light).emission);
+    e.Le = light_emission(sc, light, u, v);
     e.pdf = (1.0f - sc.p_sun) / (F)sc.ne / ocr::area_of(a, b, c) * (e.dist * e.dist) / cos_l;
   } else {
     e.Le = V3();
@@ -502,10 +532,11 @@ V3 approx_trace(const Scene& sc, const V3& x, const V3& wi, const V3& Tf, V3& fi
   const V3 x2 = o + wi * t;
   first = x2;
   V3 ng;
-  V3 n2 = shading_normal(sc, prim, hu, hv, ng);
   const V3 wo2 = -wi;
-  const bool back = dotp(wo2, n2) < 0.0f;
-  if (back) n2 = -n2;
+  F tu, tv;
+  V3 n2, ncc2;
+  bool back;
+  hit_shading(sc, prim, hu, hv, shading_normal(sc, prim, hu, hv, ng), wo2, tu, tv, n2, ncc2, back);
   const F r0 = draw(rng);
   draw(rng);
   const F r2 = draw(rng), r3 = draw(rng), r4 = draw(rng);
@@ -517,7 +548,7 @@ V3 approx_trace(const Scene& sc, const V3& x, const V3& wi, const V3& Tf, V3& fi
   const F cs = dotp(e.wi, n2);
   if (!(cs > 0.0f) || !(mx(e.Le) > 0.0f)) return V3();
   if (!visible(sc, x2, n2, e.wi, e.dist, c[1])) return V3();
-  const Bsdf b2 = make_bsdf(sc, sc.s.mat_ids ? sc.s.mat_ids[prim] : 0, n2, back, t);
+  const Bsdf b2 = make_bsdf_at(sc, sc.s.mat_ids ? sc.s.mat_ids[prim] : 0, n2, ncc2, tu, tv, back, t);
   F pdf;
   return Tf * bsdf_eval(b2, wo2, e.wi, pdf) * e.Le * (cs / e.pdf);
 }
@@ -642,13 +673,14 @@ void pass_extend(Ctx& c, uint32_t bounce) {
       return;
     }
     const V3 hp = o + d * t;
-    V3 ng;
-    V3 ns = shading_normal(c.sc, prim, hu, hv, ng);
-    const bool back = dotp(-d, ns) < 0.0f;
-    if (back) ns = -ns;
+    V3 ng, ns, ncc;
+    F tu, tv;
+    bool back;
+    hit_shading(c.sc, prim, hu, hv, shading_normal(c.sc, prim, hu, hv, ng), -d, tu, tv, ns, ncc, back);
     const uint32_t mid = c.sc.s.mat_ids ? c.sc.s.mat_ids[prim] : 0;
     const Mat& m = c.sc.s.mats[mid];
-    if (bounce == 0 && emits(m) && dotp(ng, d) < 0.0f) s.radiance[pix] = s.radiance[pix] + T * f3<F>(m.emission);
+    const ocr::MatEval<F> me = ocr::eval_material<F>(c.sc.s.at, m, tu, tv);
+    if (bounce == 0 && emits(m) && dotp(ng, d) < 0.0f) s.radiance[pix] = s.radiance[pix] + T * me.emission;
     const uint32_t bits = mid | (back ? 0x80000000u : 0u);
     Vtx v;
     v.x = hp;
@@ -658,10 +690,21 @@ void pass_extend(Ctx& c, uint32_t bounce) {
     v.back = back;
     v.wo = -d;
     v.T = T;
+    v.tu = tu;
+    v.tv = tv;
+    // the clearcoat normal is kept as oct30 with a tangent frame, else it is n
+    v.ncc = ns;
+    const ocr::Attribs* at = c.sc.s.at;
+    if (at && at->tex && at->tan) {
+      const float q[3] = {ncc.x, ncc.y, ncc.z};
+      float r[3];
+      orc::oct30_unpack(orc::oct30_pack(q), r);
+      v.ncc = V3(r[0], r[1], r[2]);
+    }
     s.vtx[pix] = v;
     if (bounce == 0) {
-      const V3 alb(m.color[0] + m.emission[0], m.color[1] + m.emission[1], m.color[2] + m.emission[2]);
-      s.gbuf[c.p.cur][pix] = pack_texel(hp, t, ns, alb, bits);
+      // albedo = colour + emission (trace.wgsl:218-223)
+      s.gbuf[c.p.cur][pix] = pack_texel(hp, t, ns, me.color + me.emission, bits);
     }
   });
 }
@@ -672,7 +715,7 @@ void pass_di_ris(Ctx& c) {
     Vtx v;
     if (!load_vtx(s, pix, v)) return;
     const Scene& sc = c.sc;
-    const Bsdf bs = make_bsdf(sc, v.mat, v.n, v.back, v.t);
+    const Bsdf bs = make_bsdf(sc, v);
     uint32_t rng = s.rng[pix];
     const F na = (F)c.p.di_candidates, nb = 1.0f;
     const uint32_t iters = c.p.di_candidates > 1u ? c.p.di_candidates : 1u;
@@ -761,7 +804,7 @@ void pass_gi_ris(Ctx& c, uint32_t bounce) {
       else T = T * (1.0f / q);
     }
     if (alive) {
-      const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+      const Bsdf bs = make_bsdf(c.sc, v);
       const F r0 = draw(rng), r1 = draw(rng), r2 = draw(rng);
       const ocr::Smp<F> sm = bsdf_sample(bs, v.wo, r0, r1, r2);
       F M = 0, wsum = 0, phat = 0;
@@ -813,7 +856,7 @@ void pass_di_temporal(Ctx& c) {
         DiRes pr = s.di_hist[p.cur ^ 1][qy * p.width + qx];
         if (g_plant != PLANT_NO_JACOBIAN)
           pr.W *= di_jacobian(c.sc, pr.light, pr.u, pr.v, v.x, s.gbuf[p.cur ^ 1][qy * p.width + qx].pos);
-        const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+        const Bsdf bs = make_bsdf(c.sc, v);
         const F ph = di_phat(c.sc, v, bs, pr.light, pr.u, pr.v, true, c.cnt);
         r = di_temporal_merge(r, pr, (F)p.di_mcap, ph, rng);
       }
@@ -839,7 +882,7 @@ void pass_di_spatial(Ctx& c) {
     const uint32_t x = pix % p.width, y = pix / p.width;
     DiRes r = s.di_tmp[pix];
     uint32_t rng = s.rng[pix];
-    const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+    const Bsdf bs = make_bsdf(c.sc, v);
     const Tex& ct = s.gbuf[p.cur][pix];
     const F cd = depth_cs(ct.depth);
     F radius = p.radius > 0.0f ? p.radius : (F)(p.width + p.rows) / 2.0f * 0.05f;
@@ -871,7 +914,7 @@ void pass_di_spatial(Ctx& c) {
             [&](const DiRes& m) { return di_phat(c.sc, v, bs, m.light, m.u, m.v, true, c.cnt); },
             [&](const DiRes& m) {
               if (!has_vn) return 0.0f;
-              const Bsdf bn = make_bsdf(c.sc, vn.mat, vn.n, vn.back, vn.t);
+              const Bsdf bn = make_bsdf(c.sc, vn);
               return di_phat(c.sc, vn, bn, m.light, m.u, m.v, true, c.cnt);
             });
       }
@@ -904,7 +947,7 @@ void pass_gi_temporal(Ctx& c) {
         const Tex prev = sample_prev(s.gbuf[p.cur ^ 1], p, px, py);
         if (texel_similar(cur, prev)) {
           const GiRes pr = s.gi_hist[p.cur ^ 1][qy * p.width + qx];
-          const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+          const Bsdf bs = make_bsdf(c.sc, v);
           const F pp = gi_phat(c.sc, v, bs, s.thr_T[pix], pr, c.cnt);
           r = gi_temporal_merge(r, pr, (F)p.gi_mcap, pp, rng);
         }
@@ -928,7 +971,7 @@ void pass_apply_di(Ctx& c) {
     const F cs = dotp(e.wi, v.n);
     if (!(cs > 0.0f)) return;
     if (!visible(c.sc, v.x, v.n, e.wi, e.dist, c.cnt[1])) return;
-    const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+    const Bsdf bs = make_bsdf(c.sc, v);
     F pdf;
     const V3 add = v.T * bsdf_eval(bs, v.wo, e.wi, pdf) * e.Le * (cs * r.W);
     if (!finite3(add)) return;
@@ -945,7 +988,7 @@ void pass_apply_gi(Ctx& c) {
     const GiRes g = s.gi[pix];
     if (g.W > 0.0f) {
       const V3 wi = unit(g.x2 - v.x);
-      const Bsdf bs = make_bsdf(c.sc, v.mat, v.n, v.back, v.t);
+      const Bsdf bs = make_bsdf(c.sc, v);
       F pdf;
       const V3 T = s.thr_T[pix] * bsdf_eval(bs, v.wo, wi, pdf) * (std::fabs(dotp(v.n, wi)) * g.W);
       const bool ok = finite3(T) && mx(T) > 0.0f;
@@ -1253,12 +1296,13 @@ void or_restir_destroy(void* st) { delete static_cast<ors::State*>(st); }
 // frame_idx, di_candidates, di_mcap, neighbours, gi_mcap, restir_di,
 // restir_gi, firefly, svgf, taa, svgf_history, atrous passes, radius (float
 // bits).  out: the frame's linear mean radiance (rows x width x 3).
-// rays[0..1]: closest-hit and shadow rays traced.
+// rays[0..1]: closest-hit and shadow rays traced.  attribs: an ocr::Attribs or NULL.
 int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
                     uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg20,
-                    int threads, float* out, uint64_t* rays) {
+                    int threads, float* out, uint64_t* rays, const void* attribs) {
   ors::State& s = *static_cast<ors::State*>(st);
   ors::Scene sc;
+  sc.s.at = static_cast<const ocr::Attribs*>(attribs);
   sc.s.pos = pos;
   sc.s.nrm = nrm;
   sc.s.mat_ids = mat_ids;

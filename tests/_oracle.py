@@ -173,13 +173,119 @@ def load_oracle() -> Oracle:
     return Oracle(C.CDLL(str(LIB)))
 
 
+class _Attribs(C.Structure):
+    """ocr::Attribs (oracle/oracle_pt.h)."""
+    _fields_ = [("uv", C.c_void_p), ("tan", C.c_void_p), ("bit", C.c_void_p), ("tex", C.c_void_p),
+                ("n_tex", C.c_uint32), ("pad", C.c_uint32)]
+
+
+def _unit_rows(v: np.ndarray) -> np.ndarray:
+    v = np.asarray(v, np.float32)
+    ln = np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+    out = np.zeros_like(v)
+    nz = ln > 0
+    out[nz] = v[nz] / ln[nz, None]
+    return out
+
+
+def vertex_frames(scene):
+    """The per-vertex tangent frame a textured scene's triangles carry on the
+    device: unit tangent (glTF TANGENT xyz) and unit bitangent (given, else
+    cross(n, t) * w with the vertex normal, or the winding normal without
+    normals; vertex_pool_bindings.wgsl:43-46); (n, 9) float32 each."""
+    if scene.tangents is None:
+        return None, None
+    q = np.asarray(scene.tangents, np.float32).reshape(-1, 4)
+    if scene.normals is not None:
+        n3 = np.asarray(scene.normals, np.float32).reshape(-1, 3)
+    else:
+        p = np.asarray(scene.positions, np.float32).reshape(-1, 3, 3)
+        e1, e2 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]
+        fn = np.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1], e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                       e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], 1)
+        n3 = np.repeat(fn, 3, axis=0)
+    if scene.bitangents is not None:
+        b = np.asarray(scene.bitangents, np.float32).reshape(-1, 3)
+    else:
+        b = np.stack([(n3[:, 1] * q[:, 2] - n3[:, 2] * q[:, 1]) * q[:, 3],
+                      (n3[:, 2] * q[:, 0] - n3[:, 0] * q[:, 2]) * q[:, 3],
+                      (n3[:, 0] * q[:, 1] - n3[:, 1] * q[:, 0]) * q[:, 3]], 1).astype(np.float32)
+    return (np.ascontiguousarray(_unit_rows(q[:, :3]).reshape(-1, 9)),
+            np.ascontiguousarray(_unit_rows(b).reshape(-1, 9)))
+
+
+def oracle_attribs(scene, tan=None, bit=None):
+    """(ocr::Attribs pointer or None, keep-alive) for a SceneData: uvs, the
+    per-vertex frames (vertex_frames, or tan/bit given in world space) and the
+    textures."""
+    if not getattr(scene, "textured", False):
+        return None, []
+    from appearance_amd._ffi import Texture
+    keep = []
+    a = _Attribs()
+    if scene.uvs is not None:
+        uv = np.ascontiguousarray(scene.uvs, np.float32)
+        keep.append(uv)
+        a.uv = uv.ctypes.data
+    if tan is None and bit is None:
+        tan, bit = vertex_frames(scene)
+    if tan is not None:
+        tan = np.ascontiguousarray(tan, np.float32)
+        bit = np.ascontiguousarray(bit, np.float32)
+        keep += [tan, bit]
+        a.tan, a.bit = tan.ctypes.data, bit.ctypes.data
+    if scene.textures:
+        texs = [np.ascontiguousarray(t, np.uint32) for t in scene.textures]
+        tv = (Texture * len(texs))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint32)))
+                                     for t in texs])
+        keep += texs + [tv]
+        a.tex = C.cast(tv, C.c_void_p)
+        a.n_tex = len(texs)
+    keep.append(a)
+    return C.cast(C.pointer(a), C.c_void_p), keep
+
+
+def tex_sample(o: Oracle, tex: np.ndarray, u: float, v: float) -> np.ndarray:
+    """The oracle's bilinear repeat sample of an (h, w) uint32 RGBA8 texture."""
+    from appearance_amd._ffi import Texture
+    t = np.ascontiguousarray(tex, np.uint32)
+    d = Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint32)))
+    out = np.empty(4, np.float32)
+    fn = o.lib.or_tex_sample
+    fn.argtypes = [C.c_void_p, C.c_float, C.c_float, _P]
+    fn(C.cast(C.pointer(d), C.c_void_p), u, v, _vp(out))
+    return out
+
+
+SHADING_FIELDS = ["uv", "nmap", "ncc", "color", "luminance", "metallic", "roughness", "emission", "transmission",
+                  "sheen", "sheen_tint", "clearcoat", "clearcoat_roughness"]
+_SHADING_SIZES = [2, 3, 3, 3, 1, 1, 1, 3, 1, 1, 3, 1, 1]
+
+
+def shading_probe(o: Oracle, scene, mat, prim: int, bu: float, bv: float, n, tan=None, bit=None) -> dict:
+    """The oracle's shading inputs at a hit: uv, mapped and clearcoat normals, evaluated material."""
+    at, keep = oracle_attribs(scene, tan, bit)
+    out = np.empty(26, np.float32)
+    nn = np.ascontiguousarray(n, np.float32)
+    fn = o.lib.or_shading_probe
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, _P, _P]
+    fn(C.cast(C.pointer(mat), C.c_void_p), at, prim, bu, bv, _vp(nn), _vp(out))
+    res, k = {}, 0
+    for name, sz in zip(SHADING_FIELDS, _SHADING_SIZES):
+        res[name] = out[k] if sz == 1 else out[k:k + sz].copy()
+        k += sz
+    return res
+
+
 def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounces: int,
                   rr_start: int, seed: int = 0, frame_idx: int = 0, row_start: int = 0,
                   rows: int | None = None, pixels=None, precision: int = 32, threads: int = 0,
-                  bvh=None, camera=None, sky=None, jitter: bool = True, tally: bool = False):
+                  bvh=None, camera=None, sky=None, jitter: bool = True, tally: bool = False, frames=None):
     """Run the oracle path tracer on a SceneData; returns (mean radiance, counts):
     counts = (extension rays, shadow rays), or with tally=True a dict that
-    adds the box and triangle tests of both kinds of query (FP32)."""
+    adds the box and triangle tests of both kinds of query (FP32).  A textured
+    scene's uvs, tangent frames and textures go along; frames = (tan, bit)
+    gives the world-space per-vertex frames instead of vertex_frames(scene)."""
     import ctypes as C
     from appearance_amd._ffi import Material
     rows = height - row_start if rows is None else rows
@@ -198,15 +304,17 @@ def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounc
     else:
         out = np.empty((rows, width, 3), np.float32)
     counts = np.zeros(6, np.uint64)
+    at, keep = oracle_attribs(scene, *(frames or (None, None)))
     fn = o.lib.or_render_counted
     fn.restype = C.c_int
     fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
-                   C.c_void_p, C.c_void_p, C.c_uint32]
+                   C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
     rc = fn(_vp(pos), _vp(nrm) if nrm is not None else None, _vp(mids), C.cast(mats, C.c_void_p),
             pos.shape[0], b.h, C.cast(C.pointer(sk), C.c_void_p), C.cast(C.pointer(cam), C.c_void_p),
             _vp(cfg), _vp(pixels) if pixels is not None else None,
-            0 if pixels is None else pixels.shape[0], precision, threads, _vp(out), _vp(counts), 6)
+            0 if pixels is None else pixels.shape[0], precision, threads, _vp(out), _vp(counts), 6, at)
+    del keep
     assert rc == 0
     if tally:
         keys = ["extension_rays", "shadow_rays", "closest_box_tests", "closest_tri_tests", "any_box_tests",
@@ -223,9 +331,10 @@ class OracleRestir:
               "firefly_filter", "svgf", "taa", "svgf_history", "svgf_iterations"]
 
     def __init__(self, o: Oracle, scene, width: int, height: int, row_start: int = 0,
-                 rows: int | None = None, bvh=None):
+                 rows: int | None = None, bvh=None, frames=None):
         self.o = o
         self.scene = scene
+        self.attribs, self._keep = oracle_attribs(scene, *(frames or (None, None)))
         self.width, self.height, self.row_start = width, height, row_start
         self.rows = height - row_start if rows is None else rows
         self.pos = np.ascontiguousarray(scene.positions, np.float32)
@@ -254,10 +363,10 @@ class OracleRestir:
         rays = np.zeros(2, np.uint64)
         fn = self.o.lib.or_restir_frame
         fn.restype = C.c_int
-        fn.argtypes = [C.c_void_p, _P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P, C.c_int, _P, _P]
+        fn.argtypes = [C.c_void_p, _P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P, C.c_int, _P, _P, _P]
         r = fn(self.h, _vp(self.pos), _vp(self.nrm) if self.nrm is not None else None, _vp(self.mids),
                C.cast(self.mats, C.c_void_p), self.pos.shape[0], self.bvh.h, C.cast(C.pointer(sk), C.c_void_p),
-               C.cast(C.pointer(cam), C.c_void_p), _vp(c), threads, _vp(out), _vp(rays))
+               C.cast(C.pointer(cam), C.c_void_p), _vp(c), threads, _vp(out), _vp(rays), self.attribs)
         assert r == 0
         self.rays = (int(rays[0]), int(rays[1]))
         return out

@@ -233,3 +233,125 @@ def demo_scene(path):
               extensions=("KHR_materials_clearcoat", "KHR_materials_sheen", "KHR_materials_specular",
                           "KHR_materials_emissive_strength", "KHR_materials_transmission", "KHR_materials_ior",
                           "KHR_materials_volume"))
+
+
+def uv_box(center, half, uv_scale=1.0):
+    """A box with a vertex set per face (24 vertices, 12 triangles, wound
+    outwards): positions, face normals, uvs spanning each face."""
+    c = np.asarray(center, np.float64)
+    h = np.asarray(half, np.float64)
+    P, N, UV, I = [], [], [], []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[axis] = sign
+            a, b = [k for k in range(3) if k != axis]
+            if sign < 0:
+                a, b = b, a  # keep the winding outwards
+            base = len(P)
+            for (s, t) in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                p = np.zeros(3)
+                p[axis] = sign
+                p[a], p[b] = s, t
+                P.append(c + p * h)
+                N.append(n)
+                UV.append(((s + 1) * 0.5 * uv_scale, (t + 1) * 0.5 * uv_scale))
+            I += [base, base + 1, base + 2, base, base + 2, base + 3]
+    return (np.array(P, np.float32), np.array(N, np.float32), np.array(UV, np.float32),
+            np.array(I, np.uint32))
+
+
+def _quad(p0, p1, p2, p3, normal, uv):
+    P = np.array([p0, p1, p2, p3], np.float32)
+    N = np.tile(np.asarray(normal, np.float32), (4, 1))
+    return {"positions": P, "normals": N, "uvs": np.asarray(uv, np.float32),
+            "indices": np.array([0, 1, 2, 0, 2, 3], np.uint32)}
+
+
+def _png(img, ctype=2):
+    from tests import png_util
+    return (png_util.encode(np.asarray(img), ctype, 8), "image/png")
+
+
+def textured_room(path, seed=0):
+    """A closed-ish room (no front wall) of textured materials, written as GLB:
+    every material texture slot in use, uvs beyond [0, 1] (repeat addressing),
+    non-power-of-two images, tangents generated by the reader, a normal map
+    shared with a clearcoat normal map, an emission-textured ceiling light, a
+    transmission-textured glass box and a textured metal box.  Lit by the
+    light only (no sky)."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:10, 0:12]
+    checker = ((xx // 3 + yy // 2) % 2)[..., None]
+    color = np.where(checker, [200, 60, 40, 255], [40, 150, 210, 140]).astype(np.uint8)   # alpha: luminance
+    mr = np.stack([np.zeros((16, 16)), rng.integers(40, 256, (16, 16)), rng.integers(0, 256, (16, 16))], -1)
+    yy, xx = np.mgrid[0:16, 0:16] + 0.5
+
+    def normal_map(fx, fy, amp):
+        nx = amp * np.sin(2 * np.pi * fx * xx / 16)
+        ny = amp * np.cos(2 * np.pi * fy * yy / 16)
+        n = np.stack([nx, ny, np.ones_like(nx)], -1)
+        n /= np.linalg.norm(n, axis=-1, keepdims=True)
+        return np.round((n * 0.5 + 0.5) * 255).astype(np.uint8)
+
+    stripes = np.repeat(np.repeat(np.array([[255, 255, 255], [255, 170, 60], [90, 140, 255], [30, 30, 30]],
+                                           np.uint8)[None], 1, 0), 2, 1)
+    stripes = np.tile(stripes, (8, 1, 1))  # 8 x 8
+    grey = lambda lo, hi, s: np.repeat(rng.integers(lo, hi, (s, s))[..., None], 3, -1)
+    sheen_tint = rng.integers(60, 256, (6, 6, 3))
+    trans = np.repeat((((np.mgrid[0:4, 0:4].sum(0)) % 2) * 255)[..., None], 3, -1)
+    images = [_png(color, 6), _png(mr), _png(normal_map(1, 2, 0.6)), _png(stripes), _png(grey(80, 256, 8)),
+              _png(grey(0, 256, 8)), _png(normal_map(3, 1, 0.9)), _png(grey(50, 256, 5)), _png(sheen_tint),
+              _png(trans)]
+    W, H, D = 2.0, 3.0, 2.0
+    prims = [
+        dict(_quad((-W, 0, D), (W, 0, D), (W, 0, -D), (-W, 0, -D), (0, 1, 0),
+                   [(-1.5, 1.5), (1.5, 1.5), (1.5, -1.5), (-1.5, -1.5)]), material=0),   # floor
+        dict(_quad((-W, 0, -D), (W, 0, -D), (W, H, -D), (-W, H, -D), (0, 0, 1),
+                   [(0, 0), (2, 0), (2, 1.5), (0, 1.5)]), material=1),                   # back wall
+        dict(_quad((-W, 0, D), (-W, 0, -D), (-W, H, -D), (-W, H, D), (1, 0, 0),
+                   [(0, 0), (1.3, 0), (1.3, 1.1), (0, 1.1)]), material=2),               # left wall
+        dict(_quad((W, 0, -D), (W, 0, D), (W, H, D), (W, H, -D), (-1, 0, 0),
+                   [(0.2, 0.1), (2.2, 0.1), (2.2, 1.6), (0.2, 1.6)]), material=3),       # right wall
+        dict(_quad((-W, H, -D), (W, H, -D), (W, H, D), (-W, H, D), (0, -1, 0),
+                   [(0, 0), (1, 0), (1, 1), (0, 1)]), material=4),                       # ceiling
+        dict(_quad((-0.7, H - 0.01, -0.7), (0.7, H - 0.01, -0.7), (0.7, H - 0.01, 0.7), (-0.7, H - 0.01, 0.7),
+                   (0, -1, 0), [(0, 0), (1, 0), (1, 1), (0, 1)]), material=5),           # light
+    ]
+    for center, half, mat in (((-0.7, 0.5, -0.6), (0.45, 0.5, 0.45), 6), ((0.8, 0.35, 0.2), (0.35, 0.35, 0.5), 7)):
+        P, N, UV, I = uv_box(center, half, uv_scale=1.7)
+        prims.append({"positions": P, "normals": N, "uvs": UV, "indices": I, "material": mat})
+    materials = [
+        {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicRoughnessTexture": {"index": 1},
+                                  "metallicFactor": 0.5, "roughnessFactor": 0.9},
+         "normalTexture": {"index": 2, "scale": 0.7}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.7, 0.7, 0.7, 1], "metallicFactor": 0.0,
+                                  "roughnessFactor": 0.5},
+         "normalTexture": {"index": 2},
+         "extensions": {"KHR_materials_clearcoat": {"clearcoatFactor": 1.0, "clearcoatTexture": {"index": 4},
+                                                    "clearcoatRoughnessFactor": 0.6,
+                                                    "clearcoatRoughnessTexture": {"index": 5},
+                                                    "clearcoatNormalTexture": {"index": 6}}}},
+        {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicFactor": 0.0, "roughnessFactor": 0.7},
+         "extensions": {"KHR_materials_sheen": {"sheenColorFactor": [1, 1, 1], "sheenColorTexture": {"index": 8},
+                                                "sheenRoughnessFactor": 1.0,
+                                                "sheenRoughnessTexture": {"index": 7}}}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.3, 0.5, 0.8, 1], "metallicFactor": 0.0,
+                                  "roughnessFactor": 0.3},
+         "normalTexture": {"index": 2, "scale": 0.5},
+         "extensions": {"KHR_materials_clearcoat": {"clearcoatFactor": 0.8, "clearcoatNormalTexture": {"index": 2}}}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.8, 0.8, 1], "metallicFactor": 0.0}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.8, 0.8, 1], "metallicFactor": 0.0},
+         "emissiveFactor": [1.0, 1.0, 1.0], "emissiveTexture": {"index": 3},
+         "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 8.0}}},
+        {"pbrMetallicRoughness": {"baseColorFactor": [1, 1, 1, 1], "metallicFactor": 0.0, "roughnessFactor": 0.2},
+         "extensions": {"KHR_materials_transmission": {"transmissionFactor": 1.0, "transmissionTexture": {"index": 9}},
+                        "KHR_materials_ior": {"ior": 1.5}}},
+        {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicFactor": 1.0, "roughnessFactor": 0.4,
+                                  "metallicRoughnessTexture": {"index": 1}}},
+    ]
+    meshes = [[p] for p in prims]
+    nodes = [{"mesh": k} for k in range(len(meshes))]
+    write_glb(path, nodes, meshes, materials, list(range(len(nodes))), images=images,
+              extensions=("KHR_materials_clearcoat", "KHR_materials_sheen", "KHR_materials_emissive_strength",
+                          "KHR_materials_transmission", "KHR_materials_ior"))

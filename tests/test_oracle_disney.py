@@ -15,24 +15,28 @@ D3 = C.c_double * 3
 def _fns(oracle):
     ev = oracle.lib.or_disney_eval
     ev.restype = C.c_double
-    ev.argtypes = [C.c_void_p, D3, D3, D3, C.c_int, D3]
+    ev.argtypes = [C.c_void_p, D3, D3, D3, C.c_int, D3, C.c_void_p]
     sm = oracle.lib.or_disney_sample
     sm.restype = C.c_double
-    sm.argtypes = [C.c_void_p, D3, D3, C.c_int, C.c_double, C.c_double, C.c_double, D3, D3]
+    sm.argtypes = [C.c_void_p, D3, D3, C.c_int, C.c_double, C.c_double, C.c_double, D3, D3, C.c_void_p]
     return ev, sm
 
 
-def evaluate(oracle, m, n, wo, wi, back=False):
+def evaluate(oracle, m, n, wo, wi, back=False, ncc=None):
     ev, _ = _fns(oracle)
     f = D3()
-    pdf = ev(C.cast(C.pointer(m), C.c_void_p), D3(*n), D3(*wo), D3(*wi), int(back), f)
+    c = None if ncc is None else D3(*ncc)
+    pdf = ev(C.cast(C.pointer(m), C.c_void_p), D3(*n), D3(*wo), D3(*wi), int(back), f,
+             None if c is None else C.cast(c, C.c_void_p))
     return np.array(f[:]), pdf
 
 
-def sample(oracle, m, n, wo, r, back=False):
+def sample(oracle, m, n, wo, r, back=False, ncc=None):
     _, sm = _fns(oracle)
     wi, f = D3(), D3()
-    pdf = sm(C.cast(C.pointer(m), C.c_void_p), D3(*n), D3(*wo), int(back), r[0], r[1], r[2], wi, f)
+    c = None if ncc is None else D3(*ncc)
+    pdf = sm(C.cast(C.pointer(m), C.c_void_p), D3(*n), D3(*wo), int(back), r[0], r[1], r[2], wi, f,
+             None if c is None else C.cast(c, C.c_void_p))
     return np.array(wi[:]), np.array(f[:]), pdf
 
 
