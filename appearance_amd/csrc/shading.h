@@ -390,7 +390,7 @@ APT_HD bool lobe_weights(const Disney& b, LobeWeights& w) {  // :460-471
 
 struct ShadeFrame {
   Frame f;     // front-facing shading normal basis
-  Frame cc;    // clearcoat basis (geometric normal, no normal maps here)
+  Frame cc;    // clearcoat basis: the unmapped normal, or the clearcoat normal map's
 };
 
 // DisneyBsdf::evaluate (:567-682).  Returns f; pdf receives the density with
