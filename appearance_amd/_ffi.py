@@ -159,6 +159,10 @@ _SIGS = {
     "apt_scene_data_destroy": (None, [_P]),
     "apt_scene_data_attribs": (C.c_int, [_P, C.POINTER(MeshAttribs)]),
     "apt_scene_set_attribs": (C.c_int, [_P, C.POINTER(MeshDesc), C.POINTER(MeshAttribs), C.POINTER(Sky)]),
+    "apt_set_sky_panorama": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
+    "apt_image_decode_hdr": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.POINTER(C.c_float))]),
+    "apt_image_free": (None, [C.POINTER(C.c_float)]),
     "apt_rays_bvh_stress": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "apt_config_default": (None, [C.POINTER(Config)]),
     "apt_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),

@@ -1,9 +1,13 @@
 // Image decoding for glTF textures: PNG (zlib inflate, RFC 1950/1951; the
 // five scanline filters; grey, grey + alpha, RGB, RGBA and palette images at
-// 1-16 bits; Adam7 interlacing) into RGBA8.  The reference decodes textures
-// with the `image` crate and compresses them to BC formats for the GPU
-// (crates/appearance-texture/src/lib.rs:207-342); here texels stay RGBA8 so
-// the sampler sees the decoded values.
+// 1-16 bits; Adam7 interlacing) into RGBA8; and Radiance HDR (RGBE) sky
+// panoramas into linear RGB floats.  The reference decodes images with the
+// `image` crate and compresses them to BC formats for the GPU
+// (crates/appearance-texture/src/lib.rs:207-342); here texels stay as
+// decoded so the samplers see the decoded values.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -315,4 +319,118 @@ bool png_decode(const uint8_t* p, size_t n, uint32_t& w, uint32_t& h, std::vecto
   return true;
 }
 
+// ---- Radiance HDR (RGBE) ---------------------------------------------------------
+// Header lines up to an empty line (the magic "#?RADIANCE" or "#?RGBE", an
+// optional FORMAT=32-bit_rle_rgbe), then the resolution line; scanlines are
+// flat 4-byte texels, or, for widths 8..32767, either run-length encoded per
+// channel (a 2, 2, width-hi, width-lo marker, then for each of the four
+// channels runs of count > 128 repeating one byte count - 128 times and
+// literal runs of count bytes) or, in the old style, flat with (1, 1, 1, n)
+// markers repeating the previous texel n << shift times (shift grows by 8
+// over consecutive markers).
+bool hdr_decode(const uint8_t* p, size_t n, uint32_t& w, uint32_t& h, std::vector<float>& rgb, std::string& err) {
+  size_t pos = 0;
+  auto line = [&](std::string& out) {
+    out.clear();
+    while (pos < n && p[pos] != '\n') out.push_back((char)p[pos++]);
+    if (pos >= n) return false;
+    ++pos;
+    return true;
+  };
+  std::string l;
+  if (!line(l) || (l.rfind("#?RADIANCE", 0) != 0 && l.rfind("#?RGBE", 0) != 0))
+    return err = "not a Radiance HDR file", false;
+  while (true) {
+    if (!line(l)) return err = "HDR header not terminated", false;
+    if (l.empty()) break;
+    if (l.rfind("FORMAT=", 0) == 0 && l != "FORMAT=32-bit_rle_rgbe")
+      return err = "unsupported HDR pixel format " + l.substr(7), false;
+  }
+  if (!line(l)) return err = "HDR resolution line missing", false;
+  long hh = 0, ww = 0;
+  char ya[3] = {0}, xa[3] = {0};
+  if (std::sscanf(l.c_str(), "%2s %ld %2s %ld", ya, &hh, xa, &ww) != 4 || std::string(ya) != "-Y" ||
+      std::string(xa) != "+X")
+    return err = "unsupported HDR orientation '" + l + "' (only -Y h +X w)", false;
+  if (hh <= 0 || ww <= 0 || hh > 65536 || ww > 65536) return err = "bad HDR size", false;
+  w = (uint32_t)ww;
+  h = (uint32_t)hh;
+  rgb.assign((size_t)w * h * 3, 0.0f);
+  std::vector<uint8_t> sl((size_t)w * 4);
+  auto to_float = [](const uint8_t* e, float* out) {
+    if (e[3] == 0) {
+      out[0] = out[1] = out[2] = 0.0f;
+      return;
+    }
+    const float f = std::ldexp(1.0f, (int)e[3] - (128 + 8));
+    for (int c = 0; c < 3; ++c) out[c] = (float)e[c] * f;
+  };
+  for (uint32_t y = 0; y < h; ++y) {
+    const bool rle = w >= 8 && w <= 0x7fff && pos + 4 <= n && p[pos] == 2 && p[pos + 1] == 2 && !(p[pos + 2] & 0x80);
+    if (rle) {
+      if ((((uint32_t)p[pos + 2] << 8) | p[pos + 3]) != w) return err = "HDR scanline width mismatch", false;
+      pos += 4;
+      for (int c = 0; c < 4; ++c) {
+        uint32_t x = 0;
+        while (x < w) {
+          if (pos >= n) return err = "HDR data truncated", false;
+          uint32_t cnt = p[pos++];
+          if (cnt > 128) {
+            cnt -= 128;
+            if (cnt > w - x || pos >= n) return err = "bad HDR run", false;
+            const uint8_t v = p[pos++];
+            for (uint32_t k = 0; k < cnt; ++k) sl[(size_t)(x++) * 4 + c] = v;
+          } else {
+            if (cnt == 0 || cnt > w - x || pos + cnt > n) return err = "bad HDR run", false;
+            for (uint32_t k = 0; k < cnt; ++k) sl[(size_t)(x++) * 4 + c] = p[pos++];
+          }
+        }
+      }
+    } else {
+      uint32_t x = 0;
+      int shift = 0;
+      while (x < w) {
+        if (pos + 4 > n) return err = "HDR data truncated", false;
+        const uint8_t* e = p + pos;
+        pos += 4;
+        if (e[0] == 1 && e[1] == 1 && e[2] == 1) {  // old-style run of the previous texel
+          if (x == 0) return err = "HDR run at the start of a scanline", false;
+          if (shift > 24) return err = "bad HDR run", false;
+          const uint64_t cnt = (uint64_t)e[3] << shift;
+          if (cnt > w - x) return err = "bad HDR run", false;
+          for (uint64_t k = 0; k < cnt; ++k, ++x) std::memcpy(&sl[(size_t)x * 4], &sl[(size_t)(x - 1) * 4], 4);
+          shift += 8;
+        } else {
+          std::memcpy(&sl[(size_t)(x++) * 4], e, 4);
+          shift = 0;
+        }
+      }
+    }
+    for (uint32_t x = 0; x < w; ++x) to_float(&sl[(size_t)x * 4], &rgb[((size_t)y * w + x) * 3]);
+  }
+  return true;
+}
+
 }  // namespace apt
+
+using namespace apt;
+
+apt_status apt_image_decode_hdr(const uint8_t* bytes, size_t len, uint32_t* width, uint32_t* height,
+                                float** out_rgb) {
+  clear_error();
+  APT_REQUIRE(bytes && width && height && out_rgb, APT_ERR_INVALID_ARG, "apt_image_decode_hdr: null argument");
+  std::vector<float> rgb;
+  std::string err;
+  uint32_t w = 0, h = 0;
+  APT_REQUIRE(hdr_decode(bytes, len, w, h, rgb, err), APT_ERR_INVALID_ARG, "apt_image_decode_hdr: %s",
+              err.c_str());
+  float* out = static_cast<float*>(std::malloc(rgb.size() * sizeof(float)));
+  APT_REQUIRE(out, APT_ERR_OOM, "apt_image_decode_hdr: out of host memory");
+  std::memcpy(out, rgb.data(), rgb.size() * sizeof(float));
+  *width = w;
+  *height = h;
+  *out_rgb = out;
+  return APT_OK;
+}
+
+void apt_image_free(float* rgb) { std::free(rgb); }

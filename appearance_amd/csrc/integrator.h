@@ -51,6 +51,8 @@ struct SceneView {
   const float4* tri_tan;        // 3 per triangle (tangent), or null (no normal maps)
   const float4* tri_bit;        // 3 per triangle (bitangent), or null
   const TexDesc* tex;           // the scene's textures, or null (no textured material)
+  const float4* sky_tex;        // equirectangular sky panorama (RGB, w unused), or null (gradient)
+  uint32_t sky_w, sky_h;
   const uint32_t* tri_mat;
   const apt_material* mats;
   const uint32_t* emissive;     // prims with emission
@@ -59,6 +61,10 @@ struct SceneView {
   v3 cam_pos, cam_right, cam_up, cam_back;
   float cam_tan_half, cam_aspect;
 };
+
+// Whether a scene needs the textured kernel variants: material textures or a
+// sky panorama.
+__host__ __device__ inline bool scene_textured(const SceneView& s) { return s.tex || s.sky_tex; }
 
 // A kernel's scene view with the textured-scene arrays known to be absent at
 // compile time when TEX is false: every textured branch of the shading code
@@ -72,6 +78,7 @@ __device__ __forceinline__ SceneView scene_view(const SceneView& s) {
     v.tri_tan = nullptr;
     v.tri_bit = nullptr;
     v.tex = nullptr;
+    v.sky_tex = nullptr;
   }
   return v;
 }

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(BLOCK, INST ? 6 : 8) void k_extend(SceneView sc, Pa
 }
 
 // Four waves per SIMD: the compiler's own choice (136 VGPRs) allows three.
-// TEX: the scene has textures (the textured shading code is compiled in, at
+// TEX: the scene has material textures or a sky panorama (the textured code is compiled in, at
 // three waves per SIMD: at four it spills).
 template <bool TEX>
 __global__ __launch_bounds__(BLOCK, TEX ? 3 : 4) void k_shade(FrameView fv, SceneView sc_, PathView ps, int qin,
@@ -217,7 +217,7 @@ void launch_bounce_kernels(const SceneView& sc, const PathView& ps, uint32_t n_p
     hipLaunchKernelGGL((k_extend<W, S_, INST>), pool, dim3(BLOCK), 0, s, sc, ps, qin, nodes);
     mark(1);
     mark(2);
-    if (sc.tex) hipLaunchKernelGGL(k_shade<true>, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
+    if (scene_textured(sc)) hipLaunchKernelGGL(k_shade<true>, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
     else hipLaunchKernelGGL(k_shade<false>, grid, dim3(BLOCK), 0, s, fv, sc, ps, qin, bounce);
     mark(3);
     mark(4);

@@ -73,30 +73,63 @@ __device__ __forceinline__ bool has_emission(const apt_material& m) {
 // Textured materials (shared/material/material_pool_bindings.wgsl:19-154).
 // ---------------------------------------------------------------------------
 
-// Bilinear sample with repeat addressing at level 0 (the reference's sampler,
-// src/scene_resources/material_pool.rs:68-74): texel centres at (i + 1/2) / W,
-// RGBA8 texels as c / 255.
-__device__ __forceinline__ float4 tex_sample(const SceneView& sc, uint32_t id, float u, float v) {
-  const TexDesc d = sc.tex[id];
-  const float x = (u == u ? u : 0.0f) * (float)d.width - 0.5f, y = (v == v ? v : 0.0f) * (float)d.height - 0.5f;
+// Bilinear filtering with repeat addressing at level 0 (the reference's
+// samplers, src/scene_resources/material_pool.rs:68-74 and sky.rs:44-50) on a
+// width x height grid: texel centres at (i + 1/2) / W; the four texels around
+// (u W - 1/2, v H - 1/2), indices wrapped modulo the size, and the weights of
+// the second column and row.
+struct Bilinear {
+  uint32_t i00, i10, i01, i11;  // texel indices, row major
+  float ax, ay;
+};
+
+__device__ __forceinline__ Bilinear bilinear_repeat(uint32_t width, uint32_t height, float u, float v) {
+  const float x = (u == u ? u : 0.0f) * (float)width - 0.5f, y = (v == v ? v : 0.0f) * (float)height - 0.5f;
   const float fx = fminf(fmaxf(floorf(x), -1099511627776.0f), 1099511627776.0f);  // +-2^40
   const float fy = fminf(fmaxf(floorf(y), -1099511627776.0f), 1099511627776.0f);
-  const float ax = fminf(fmaxf(x - fx, 0.0f), 1.0f), ay = fminf(fmaxf(y - fy, 0.0f), 1.0f);
-  const int64_t W = d.width, H = d.height;
+  Bilinear b;
+  b.ax = fminf(fmaxf(x - fx, 0.0f), 1.0f);
+  b.ay = fminf(fmaxf(y - fy, 0.0f), 1.0f);
+  const int64_t W = width, H = height;
   const int64_t ix = (int64_t)fx, iy = (int64_t)fy;
   const uint32_t x0 = (uint32_t)(((ix % W) + W) % W), x1 = (uint32_t)((((ix + 1) % W) + W) % W);
   const uint32_t y0 = (uint32_t)(((iy % H) + H) % H), y1 = (uint32_t)((((iy + 1) % H) + H) % H);
-  const auto texel = [&](uint32_t tx, uint32_t ty) {
-    const uint32_t c = d.texels[(size_t)ty * d.width + tx];
+  b.i00 = y0 * width + x0;
+  b.i10 = y0 * width + x1;
+  b.i01 = y1 * width + x0;
+  b.i11 = y1 * width + x1;
+  return b;
+}
+
+__device__ __forceinline__ float bilerp(const Bilinear& b, float c00, float c10, float c01, float c11) {
+  return (c00 * (1.0f - b.ax) + c10 * b.ax) * (1.0f - b.ay) + (c01 * (1.0f - b.ax) + c11 * b.ax) * b.ay;
+}
+
+// A material texture (RGBA8 texels as c / 255).
+__device__ __forceinline__ float4 tex_sample(const SceneView& sc, uint32_t id, float u, float v) {
+  const TexDesc d = sc.tex[id];
+  const Bilinear b = bilinear_repeat(d.width, d.height, u, v);
+  const auto texel = [&](uint32_t i) {
+    const uint32_t c = d.texels[i];
     return make_float4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
                        (float)((c >> 16) & 0xFFu) / 255.0f, (float)(c >> 24) / 255.0f);
   };
-  const float4 c00 = texel(x0, y0), c10 = texel(x1, y0), c01 = texel(x0, y1), c11 = texel(x1, y1);
-  const float bx = 1.0f - ax, by = 1.0f - ay;
-  return make_float4((c00.x * bx + c10.x * ax) * by + (c01.x * bx + c11.x * ax) * ay,
-                     (c00.y * bx + c10.y * ax) * by + (c01.y * bx + c11.y * ax) * ay,
-                     (c00.z * bx + c10.z * ax) * by + (c01.z * bx + c11.z * ax) * ay,
-                     (c00.w * bx + c10.w * ax) * by + (c01.w * bx + c11.w * ax) * ay);
+  const float4 c00 = texel(b.i00), c10 = texel(b.i10), c01 = texel(b.i01), c11 = texel(b.i11);
+  return make_float4(bilerp(b, c00.x, c10.x, c01.x, c11.x), bilerp(b, c00.y, c10.y, c01.y, c11.y),
+                     bilerp(b, c00.z, c10.z, c01.z, c11.z), bilerp(b, c00.w, c10.w, c01.w, c11.w));
+}
+
+// The sky along direction d without the sun (Sky::sky, shared/sky_bindings.wgsl:68-70):
+// the panorama at unit_vector_to_panorama_coords(d) (assets/shaders/math.wgsl:101-106)
+// when the scene has one, else the analytic gradient.
+__device__ __forceinline__ v3 sky_radiance(const SceneView& sc, v3 d) {
+  if (!sc.sky_tex) return sky_gradient(sc.sky, d);
+  const float u = (atan2f(d.z, d.x) + PI) / TWO_PI;
+  const float v = acosf(clampf(d.y, -1.0f, 1.0f)) / PI;
+  const Bilinear b = bilinear_repeat(sc.sky_w, sc.sky_h, u, v);
+  const float4 c00 = sc.sky_tex[b.i00], c10 = sc.sky_tex[b.i10], c01 = sc.sky_tex[b.i01], c11 = sc.sky_tex[b.i11];
+  return mk(bilerp(b, c00.x, c10.x, c01.x, c11.x), bilerp(b, c00.y, c10.y, c01.y, c11.y),
+            bilerp(b, c00.z, c10.z, c01.z, c11.z));
 }
 
 // srgb_to_linear (assets/shaders/color.wgsl:9-15), per channel.
@@ -281,7 +314,7 @@ __device__ __forceinline__ void shade_path(const SceneView& sc, const PathView& 
   if (prim == INVALID) {
     // Miss: gradient sky (not light-sampled) plus the sun disk, MIS-weighted
     // against sun sampling at the previous vertex.
-    v3 Le = sky_gradient(sc.sky, rd);
+    v3 Le = sky_radiance(sc, rd);
     float si = sun_intensity(sc.sky, rd);
     if (si > 0.0f) {
       float w = 1.0f;

@@ -276,6 +276,9 @@ struct apt_renderer {
   std::vector<uint32_t> mat_ids;
   std::vector<apt_material> mats;
   apt_sky sky{};
+  // sky panorama (apt_set_sky_panorama): RGB texels as float4, or none
+  DevBuf<float4> d_sky_pano;
+  uint32_t pano_w = 0, pano_h = 0;
   bool scene_dirty = true, explicit_scene = false, has_normals = false;
   uint64_t scene_version = 1, accum_scene_version = 0;
   // device scene
@@ -312,6 +315,7 @@ struct apt_renderer {
     scene_gpu_free(gpu);
     apt_tlas_destroy(tlas);
     d_frame_sum.release(); d_accum_tm.release(); d_accum_lin.release(); d_rgba.release();
+    d_sky_pano.release();
     p_o.release(); p_d.release(); p_t.release(); p_l.release(); p_h.release(); p_so.release();
     p_sd.release(); p_sc.release(); p_rng.release(); p_q0.release(); p_q1.release();
     p_qs.release(); p_cnt.release();
@@ -465,6 +469,18 @@ apt_status upload_instances(apt_renderer* r) {
   return APT_OK;
 }
 
+// The scene as the kernels see it: the device scene, the camera and the sky
+// panorama.
+SceneView render_view(const apt_renderer* r, uint32_t width, uint32_t height) {
+  SceneView sc = scene_gpu_view(r->gpu, r->cam, width, height);
+  if (r->pano_w) {
+    sc.sky_tex = r->d_sky_pano.p;
+    sc.sky_w = r->pano_w;
+    sc.sky_h = r->pano_h;
+  }
+  return sc;
+}
+
 apt_status upload_scene(apt_renderer* r) {
   if (!r->scene_dirty) return APT_OK;
   const auto t0 = std::chrono::steady_clock::now();
@@ -551,7 +567,7 @@ apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint3
   }
   APT_HIP(hipMemsetAsync(r->rs.radiance.p, 0, npix * sizeof(float4), s));
   APT_HIP(hipMemsetAsync(r->rs.counters.p, 0, 2 * RS_CTR_SLOTS * sizeof(uint32_t), s));
-  const SceneView sc = scene_gpu_view(r->gpu, r->cam, width, height);
+  const SceneView sc = render_view(r, width, height);
   const CamView cur = cam_view(sc);
   const apt_config& c = r->cfg;
   RestirParams p{};
@@ -665,7 +681,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
 
   FrameView fv{width, height, row_start, rows, spp, r->frame_idx, r->cfg.seed, r->cfg.max_bounces,
                r->cfg.rr_start_bounce};
-  SceneView sc = scene_gpu_view(r->gpu, r->cam, width, height);
+  SceneView sc = render_view(r, width, height);
   PathView ps{};
   ps.o_pdf = r->p_o.p; ps.d_flags = r->p_d.p; ps.thr = r->p_t.p; ps.rad = r->p_l.p;
   ps.hit = r->p_h.p; ps.sh_o = r->p_so.p; ps.sh_d = r->p_sd.p; ps.sh_c = r->p_sc.p;
@@ -960,6 +976,35 @@ apt_status apt_scene_set_attribs(apt_renderer* r, const apt_mesh_desc* mesh, con
   else std::memset(&r->sky, 0, sizeof(r->sky));
   r->explicit_scene = true;
   r->scene_dirty = true;
+  return APT_OK;
+}
+
+apt_status apt_set_sky_panorama(apt_renderer* r, uint32_t width, uint32_t height, const float* rgb) {
+  clear_error();
+  APT_REQUIRE(r, APT_ERR_INVALID_ARG, "apt_set_sky_panorama: renderer is NULL");
+  if (!rgb || width == 0 || height == 0) {
+    r->d_sky_pano.release();
+    r->pano_w = r->pano_h = 0;
+    r->scene_version++;
+    return APT_OK;
+  }
+  APT_REQUIRE(width <= APT_SKY_PANORAMA_MAX && height <= APT_SKY_PANORAMA_MAX, APT_ERR_INVALID_ARG,
+              "apt_set_sky_panorama: %ux%u exceeds %u texels a side", width, height, APT_SKY_PANORAMA_MAX);
+  const size_t n = (size_t)width * height;
+  std::vector<float4> tex(n);
+  for (size_t i = 0; i < n; ++i) {
+    const float c[3] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
+    for (float v : c)
+      APT_REQUIRE(std::isfinite(v) && v >= 0.0f, APT_ERR_INVALID_ARG,
+                  "apt_set_sky_panorama: texel %zu is negative or not finite", i);
+    tex[i] = make_float4(c[0], c[1], c[2], 0.0f);
+  }
+  apt_status st = r->d_sky_pano.ensure(n);
+  if (st != APT_OK) return st;
+  APT_HIP(hipMemcpy(r->d_sky_pano.p, tex.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  r->pano_w = width;
+  r->pano_h = height;
+  r->scene_version++;  // a new sky: accumulation and reservoir history restart
   return APT_OK;
 }
 

@@ -406,7 +406,7 @@ __device__ __forceinline__ v3 approx_trace(const SceneView& sc, const LdsStack& 
   const HitRec h = closest<W, S, INST>(sc, st, o, wi, ctr);
   if (h.prim == INVALID) {
     first = x + wi * GI_SKY_DISTANCE;
-    return Tf * sky_gradient(sc.sky, wi);
+    return Tf * sky_radiance(sc, wi);
   }
   const TriGeom g = load_tri(sc, h.prim);
   const v3 x2 = o + wi * h.t;
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_extend(RestirParams p, SceneView s
   const float4 L = b.radiance[pix];
   if (h.prim == INVALID) {
     // sky without the sun disk, which the light samples cover (trace.wgsl:237-243)
-    const v3 sky = T * sky_gradient(sc.sky, d);
+    const v3 sky = T * sky_radiance(sc, d);
     b.radiance[pix] = make_float4(L.x + sky.x, L.y + sky.y, L.z + sky.z, 0.0f);
     b.thr[pix].w = 0.0f;
     b.vtx_p[pix].w = -1.0f;
@@ -1408,7 +1408,7 @@ apt_status restir_sample_variant<APT_RESTIR_INST != 0, APT_RESTIR_TEX != 0>(cons
 
 #if !APT_RESTIR_INST && !APT_RESTIR_TEX
 apt_status restir_sample(const RestirParams& p, const SceneView& sc, const RestirBuffers& b, hipStream_t s) {
-  if (sc.tex)
+  if (scene_textured(sc))
     return sc.insts ? restir_sample_variant<true, true>(p, sc, b, s) : restir_sample_variant<false, true>(p, sc, b, s);
   return sc.insts ? restir_sample_variant<true, false>(p, sc, b, s) : restir_sample_variant<false, false>(p, sc, b, s);
 }

@@ -85,6 +85,17 @@ class PathTracerGpu:
         check(lib().apt_scene_set(self._h, C.byref(md), C.byref(sk) if sk is not None else None),
               "apt_scene_set")
 
+    def set_sky_panorama(self, rgb) -> None:
+        """The equirectangular sky panorama ((height, width, 3) linear RGB,
+        row 0 at the zenith; None: the analytic gradient)."""
+        if rgb is None:
+            check(lib().apt_set_sky_panorama(self._h, 0, 0, None), "apt_set_sky_panorama")
+            return
+        a = np.ascontiguousarray(rgb, np.float32)
+        assert a.ndim == 3 and a.shape[2] == 3
+        check(lib().apt_set_sky_panorama(self._h, a.shape[1], a.shape[0], a.ctypes.data_as(C.POINTER(C.c_float))),
+              "apt_set_sky_panorama")
+
     def set_camera(self, camera) -> None:
         """Apply a CameraUpdate (world.CameraUpdate or the ABI Camera struct)."""
         if isinstance(camera, world.CameraUpdate):

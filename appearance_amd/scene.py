@@ -65,6 +65,23 @@ def load_gltf(path) -> SceneData:
     return _view(h)
 
 
+def decode_hdr(data: bytes) -> np.ndarray:
+    """A Radiance HDR (RGBE) image decoded by libapt: (height, width, 3) float32."""
+    w, h = C.c_uint32(), C.c_uint32()
+    out = C.POINTER(C.c_float)()
+    check(lib().apt_image_decode_hdr(data, len(data), C.byref(w), C.byref(h), C.byref(out)), "apt_image_decode_hdr")
+    try:
+        return np.ctypeslib.as_array(out, shape=(h.value * w.value * 3,)).reshape(h.value, w.value, 3).copy()
+    finally:
+        lib().apt_image_free(out)
+
+
+def load_hdr(path) -> np.ndarray:
+    """A Radiance .hdr file as a (height, width, 3) float32 sky panorama."""
+    with open(path, "rb") as f:
+        return decode_hdr(f.read())
+
+
 def generate_scene(name: str, seed: int = 0) -> SceneData:
     h = C.c_void_p()
     check(lib().apt_scene_generate(name.encode(), seed, C.byref(h)), "apt_scene_generate")

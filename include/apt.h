@@ -483,6 +483,32 @@ apt_status apt_scene_set(apt_renderer* r, const apt_mesh_desc* mesh, const apt_s
 apt_status apt_scene_set_attribs(apt_renderer* r, const apt_mesh_desc* mesh, const apt_mesh_attribs* attribs,
                                  const apt_sky* sky);
 
+/* The sky panorama (the reference's sky texture: an equirectangular HDR image,
+ * ::evening_road_01_puresky_4k.hdr, set in src/scene_resources/mod.rs:70-75 and
+ * sky.rs:93-105; its blob is missing from the reference, so by default the
+ * analytic gradient of apt_sky stands in).  width x height linear RGB texels,
+ * 3 floats each, row 0 at the zenith.  Rays that leave the scene read it
+ * (Sky::sky, shared/sky_bindings.wgsl:68-70) bilinearly with repeat addressing
+ * at u = (atan2(d.z, d.x) + pi) / 2 pi, v = acos(d.y) / pi
+ * (unit_vector_to_panorama_coords, assets/shaders/math.wgsl:101-106), in FP32
+ * (the reference samples a BC6H-compressed copy); the sun stays analytic.  The
+ * texels are copied to the device (16 B each).  rgb NULL or a zero size
+ * removes the panorama.  Negative or non-finite texels are refused.  A new
+ * sky restarts accumulation and the ReSTIR history. */
+#define APT_SKY_PANORAMA_MAX 16384u
+apt_status apt_set_sky_panorama(apt_renderer* r, uint32_t width, uint32_t height, const float* rgb);
+
+/* Radiance HDR (.hdr, RGBE) decoding: the "#?RADIANCE" / "#?RGBE" header with
+ * FORMAT=32-bit_rle_rgbe, the "-Y height +X width" orientation, and flat,
+ * run-length (new style) or repeat-marker (old style) scanlines.  A texel
+ * (r, g, b, e) is (r, g, b) * 2^(e - 136), zero when e = 0 (the image crate's
+ * decoder, which the reference loads HDR images with, crates/appearance-texture/src/asset.rs:10-14).
+ * *out_rgb receives width * height * 3 floats, row major, freed with
+ * apt_image_free. */
+apt_status apt_image_decode_hdr(const uint8_t* bytes, size_t len, uint32_t* width, uint32_t* height,
+                                float** out_rgb);
+void apt_image_free(float* rgb);
+
 /* PathTracerGpu::render: renders rows [row_start, row_end) of a width x height
  * frame.  Writes RGBA8 in 64x64 block-major order (resolve.wgsl:56-57,
  * block.wgsl:3-15) to out_rgba (width*(row_end-row_start)*4 bytes; width and

@@ -57,6 +57,7 @@ double or_disney_eval(const void* mat, const double* n, const double* wo, const 
 double or_disney_sample(const void* mat, const double* n, const double* wo, int back, double r0, double r1,
                         double r2, double* wi, double* f, const double* ncc);
 void or_tex_sample(const void* tex, float u, float v, float* out4);
+void or_sky_probe(const void* sky, const void* attribs, const float* d, float* out);
 void or_shading_probe(const void* mat, const void* attribs, uint32_t prim, float bu, float bv, const float* n,
                       float* out);
 

@@ -317,6 +317,8 @@ struct Attribs {
   const float* bit;
   const Texture* tex;
   uint32_t n_tex, pad;
+  const float* sky_rgb;  // sky panorama: sky_w x sky_h RGB floats, row 0 at the zenith (NULL: gradient)
+  uint32_t sky_w, sky_h;
 };
 constexpr uint32_t NO_TEX = 0xFFFFFFFFu;
 
@@ -347,6 +349,26 @@ void tex_sample(const Texture& t, R u, R v, R out[4]) {
     for (int j = 0; j < 4; ++j) k[j] = (R)((q[j] >> (8 * c)) & 0xFFu) / R(255);
     out[c] = (k[0] * (R(1) - ax) + k[1] * ax) * (R(1) - ay) + (k[2] * (R(1) - ax) + k[3] * ax) * ay;
   }
+}
+
+// The same filtering on an RGB float image (the sky panorama).
+template <typename R>
+void pano_sample(const float* rgb, uint32_t w, uint32_t h, R u, R v, R out[3]) {
+  R x = u * (R)w - R(0.5), y = v * (R)h - R(0.5);
+  if (!std::isfinite(x)) x = R(-0.5);
+  if (!std::isfinite(y)) y = R(-0.5);
+  const R fx = std::floor(x), fy = std::floor(y);
+  const R ax = x - fx, ay = y - fy;
+  auto wrap = [](double i, uint32_t n) {
+    double m = std::fmod(i, (double)n);
+    if (m < 0) m += n;
+    return std::min((uint32_t)m, n - 1);
+  };
+  const uint32_t x0 = wrap(fx, w), x1 = wrap((double)fx + 1.0, w), y0 = wrap(fy, h), y1 = wrap((double)fy + 1.0, h);
+  const float* q[4] = {rgb + 3 * ((size_t)y0 * w + x0), rgb + 3 * ((size_t)y0 * w + x1), rgb + 3 * ((size_t)y1 * w + x0),
+                       rgb + 3 * ((size_t)y1 * w + x1)};
+  for (int c = 0; c < 3; ++c)
+    out[c] = ((R)q[0][c] * (R(1) - ax) + (R)q[1][c] * ax) * (R(1) - ay) + ((R)q[2][c] * (R(1) - ax) + (R)q[3][c] * ax) * ay;
 }
 
 // srgb_to_linear (assets/shaders/color.wgsl:9-15), one channel.
@@ -760,6 +782,20 @@ V<R> sky_rad(const Sky& k, const V<R>& d) {
                 lerp((R)k.horizon[2], (R)k.zenith[2], t));
   }
   return f3<R>(k.ground);
+}
+
+// Sky::sky without the sun (shared/sky_bindings.wgsl:68-70): the panorama at
+// unit_vector_to_panorama_coords(d) = ((atan2(d.z, d.x) + pi) / 2 pi,
+// acos(d.y) / pi) (assets/shaders/math.wgsl:101-106) when the scene has one,
+// else the analytic gradient.
+template <typename R>
+V<R> sky_L(const Sky& k, const Attribs* at, const V<R>& d) {
+  if (!at || !at->sky_rgb) return sky_rad<R>(k, d);
+  const R u = (std::atan2(d.z, d.x) + C<R>::pi) / (R(2) * C<R>::pi);
+  const R v = std::acos(cl(d.y, R(-1), R(1))) / C<R>::pi;
+  R c[3];
+  pano_sample<R>(at->sky_rgb, at->sky_w, at->sky_h, u, v, c);
+  return V<R>(c[0], c[1], c[2]);
 }
 
 template <typename R>

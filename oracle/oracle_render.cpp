@@ -57,7 +57,7 @@ V<R> path(const Scene& sc, const Cfg& cf, uint32_t x, uint32_t y, uint32_t sampl
     uint32_t prim;
     if (nrays) nrays[0]++;
     if (!closest(sc, ro, rd, t, bu, bv, prim)) {
-      V<R> Le = sky_rad<R>(k, rd);
+      V<R> Le = sky_L<R>(k, sc.at, rd);
       R si = sun_I<R>(k, rd);
       if (si > R(0)) {
         R w = R(1);
@@ -350,6 +350,16 @@ void or_shading_probe(const void* mat, const void* attribs, uint32_t prim, float
                           e.transmission, e.sheen, e.sheen_tint.x, e.sheen_tint.y, e.sheen_tint.z, e.clearcoat,
                           e.clearcoat_roughness, 0.0f, 0.0f};
   for (int k = 0; k < 26; ++k) out[k] = vals[k];
+}
+
+// Sky probe: the sky (without the sun) seen along d, FP32; attribs may carry
+// a panorama.  out: RGB.
+void or_sky_probe(const void* sky, const void* attribs, const float* d, float* out) {
+  const ocr::V<float> v = ocr::sky_L<float>(*static_cast<const ocr::Sky*>(sky), static_cast<const ocr::Attribs*>(attribs),
+                                            ocr::V<float>(d[0], d[1], d[2]));
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
 }
 
 }  // extern "C"

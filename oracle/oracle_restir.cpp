@@ -527,7 +527,7 @@ V3 approx_trace(const Scene& sc, const V3& x, const V3& wi, const V3& Tf, V3& fi
   uint32_t prim;
   if (!closest(sc, o, wi, t, hu, hv, prim, c[0])) {
     first = x + wi * GI_SKY_DISTANCE;
-    return Tf * ocr::sky_rad<F>(*sc.s.sky, wi);
+    return Tf * ocr::sky_L<F>(*sc.s.sky, sc.s.at, wi);
   }
   const V3 x2 = o + wi * t;
   first = x2;
@@ -666,7 +666,7 @@ void pass_extend(Ctx& c, uint32_t bounce) {
     F t, hu, hv;
     uint32_t prim;
     if (!closest(c.sc, o, d, t, hu, hv, prim, c.cnt[0])) {
-      s.radiance[pix] = s.radiance[pix] + T * ocr::sky_rad<F>(*c.sc.s.sky, d);
+      s.radiance[pix] = s.radiance[pix] + T * ocr::sky_L<F>(*c.sc.s.sky, c.sc.s.at, d);
       s.alive[pix] = 0;
       s.vtx[pix].t = -1.0f;
       if (bounce == 0) s.gbuf[c.p.cur][pix] = pack_texel(V3(), 0.0f, V3(), V3(1, 1, 1), 0u);

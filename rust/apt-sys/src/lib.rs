@@ -32,6 +32,7 @@ pub const APT_ERR_NOT_FOUND: apt_status = 6;
 pub const APT_ABI_VERSION: u32 = 1;
 pub const APT_MISS: u32 = 0xFFFF_FFFF;
 pub const APT_TRACE_NOALLOC_MAX_RAYS: u32 = 65536;
+pub const APT_SKY_PANORAMA_MAX: u32 = 16384;
 pub const APT_TLAS_MAX_INSTANCES: u32 = 8192;
 pub const APT_INVALID_TEXTURE: u32 = 0xFFFF_FFFF;
 pub const APT_BSDF_DISNEY: u32 = 0;
@@ -452,6 +453,10 @@ extern "C" {
     pub fn apt_scene_set(r: *mut apt_renderer, mesh: *const apt_mesh_desc, sky: *const apt_sky) -> apt_status;
     pub fn apt_scene_set_attribs(r: *mut apt_renderer, mesh: *const apt_mesh_desc, attribs: *const apt_mesh_attribs,
                                  sky: *const apt_sky) -> apt_status;
+    pub fn apt_set_sky_panorama(r: *mut apt_renderer, width: u32, height: u32, rgb: *const f32) -> apt_status;
+    pub fn apt_image_decode_hdr(bytes: *const u8, len: usize, width: *mut u32, height: *mut u32,
+                                out_rgb: *mut *mut f32) -> apt_status;
+    pub fn apt_image_free(rgb: *mut f32);
     pub fn apt_render(r: *mut apt_renderer, width: u32, height: u32, row_start: u32, row_end: u32,
                       out_rgba: *mut u8, out_len: usize, out_accum_rgb: *mut f32, accum_len: usize)
         -> apt_status;

@@ -176,7 +176,8 @@ def load_oracle() -> Oracle:
 class _Attribs(C.Structure):
     """ocr::Attribs (oracle/oracle_pt.h)."""
     _fields_ = [("uv", C.c_void_p), ("tan", C.c_void_p), ("bit", C.c_void_p), ("tex", C.c_void_p),
-                ("n_tex", C.c_uint32), ("pad", C.c_uint32)]
+                ("n_tex", C.c_uint32), ("pad", C.c_uint32), ("sky_rgb", C.c_void_p), ("sky_w", C.c_uint32),
+                ("sky_h", C.c_uint32)]
 
 
 def _unit_rows(v: np.ndarray) -> np.ndarray:
@@ -214,15 +215,22 @@ def vertex_frames(scene):
             np.ascontiguousarray(_unit_rows(b).reshape(-1, 9)))
 
 
-def oracle_attribs(scene, tan=None, bit=None):
+def oracle_attribs(scene, tan=None, bit=None, sky_pano=None):
     """(ocr::Attribs pointer or None, keep-alive) for a SceneData: uvs, the
-    per-vertex frames (vertex_frames, or tan/bit given in world space) and the
-    textures."""
-    if not getattr(scene, "textured", False):
+    per-vertex frames (vertex_frames, or tan/bit given in world space), the
+    textures, and a sky panorama ((h, w, 3) float32) if given."""
+    if not getattr(scene, "textured", False) and sky_pano is None:
         return None, []
     from appearance_amd._ffi import Texture
     keep = []
     a = _Attribs()
+    if sky_pano is not None:
+        pano = np.ascontiguousarray(sky_pano, np.float32)
+        keep.append(pano)
+        a.sky_rgb, a.sky_w, a.sky_h = pano.ctypes.data, pano.shape[1], pano.shape[0]
+    if not getattr(scene, "textured", False):
+        keep.append(a)
+        return C.cast(C.pointer(a), C.c_void_p), keep
     if scene.uvs is not None:
         uv = np.ascontiguousarray(scene.uvs, np.float32)
         keep.append(uv)
@@ -243,6 +251,18 @@ def oracle_attribs(scene, tan=None, bit=None):
         a.n_tex = len(texs)
     keep.append(a)
     return C.cast(C.pointer(a), C.c_void_p), keep
+
+
+def sky_probe(o: Oracle, sky, d, pano=None) -> np.ndarray:
+    """The oracle's sky radiance (no sun) along direction d, with a panorama if given."""
+    at, keep = oracle_attribs(None, sky_pano=pano) if pano is not None else (None, [])
+    out = np.empty(3, np.float32)
+    dd = np.ascontiguousarray(d, np.float32)
+    fn = o.lib.or_sky_probe
+    fn.argtypes = [C.c_void_p, C.c_void_p, _P, _P]
+    fn(C.cast(C.pointer(sky), C.c_void_p), at, _vp(dd), _vp(out))
+    del keep
+    return out
 
 
 def tex_sample(o: Oracle, tex: np.ndarray, u: float, v: float) -> np.ndarray:
@@ -280,12 +300,14 @@ def shading_probe(o: Oracle, scene, mat, prim: int, bu: float, bv: float, n, tan
 def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounces: int,
                   rr_start: int, seed: int = 0, frame_idx: int = 0, row_start: int = 0,
                   rows: int | None = None, pixels=None, precision: int = 32, threads: int = 0,
-                  bvh=None, camera=None, sky=None, jitter: bool = True, tally: bool = False, frames=None):
+                  bvh=None, camera=None, sky=None, jitter: bool = True, tally: bool = False, frames=None,
+                  sky_pano=None):
     """Run the oracle path tracer on a SceneData; returns (mean radiance, counts):
     counts = (extension rays, shadow rays), or with tally=True a dict that
     adds the box and triangle tests of both kinds of query (FP32).  A textured
     scene's uvs, tangent frames and textures go along; frames = (tan, bit)
-    gives the world-space per-vertex frames instead of vertex_frames(scene)."""
+    gives the world-space per-vertex frames instead of vertex_frames(scene);
+    sky_pano an (h, w, 3) sky panorama."""
     import ctypes as C
     from appearance_amd._ffi import Material
     rows = height - row_start if rows is None else rows
@@ -304,7 +326,7 @@ def oracle_render(o: Oracle, scene, width: int, height: int, spp: int, max_bounc
     else:
         out = np.empty((rows, width, 3), np.float32)
     counts = np.zeros(6, np.uint64)
-    at, keep = oracle_attribs(scene, *(frames or (None, None)))
+    at, keep = oracle_attribs(scene, *(frames or (None, None)), sky_pano=sky_pano)
     fn = o.lib.or_render_counted
     fn.restype = C.c_int
     fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -331,10 +353,10 @@ class OracleRestir:
               "firefly_filter", "svgf", "taa", "svgf_history", "svgf_iterations"]
 
     def __init__(self, o: Oracle, scene, width: int, height: int, row_start: int = 0,
-                 rows: int | None = None, bvh=None, frames=None):
+                 rows: int | None = None, bvh=None, frames=None, sky_pano=None):
         self.o = o
         self.scene = scene
-        self.attribs, self._keep = oracle_attribs(scene, *(frames or (None, None)))
+        self.attribs, self._keep = oracle_attribs(scene, *(frames or (None, None)), sky_pano=sky_pano)
         self.width, self.height, self.row_start = width, height, row_start
         self.rows = height - row_start if rows is None else rows
         self.pos = np.ascontiguousarray(scene.positions, np.float32)
