@@ -432,6 +432,7 @@ apt_status create_device(apt_server* s) {
     d.max_bounces = s->cfg.max_bounces;
     d.width = s->cfg.bvh_width;
     d.lds_stack = s->cfg.lds_stack_depth;
+    d.persistent = s->cfg.persistent;
     d.batch = s->d_batch[i];
     d.reqs = s->d_reqs;
     d.ps.o_pdf = s->p_o; d.ps.d_flags = s->p_d; d.ps.thr = s->p_t; d.ps.rad = s->p_l;
@@ -522,6 +523,7 @@ apt_status apt_server_create(const apt_server_config* cfg, apt_server** out) {
   APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 16 || c.lds_stack_depth == 24,
               APT_ERR_INVALID_ARG, "server: lds_stack_depth %u is not 8, 16 or 24", c.lds_stack_depth);
   APT_REQUIRE(c.policy <= 2, APT_ERR_INVALID_ARG, "server: unknown policy %u", c.policy);
+  APT_REQUIRE(c.persistent <= 1, APT_ERR_INVALID_ARG, "server: persistent must be 0 or 1, not %u", c.persistent);
   apt_server* s = new apt_server();
   s->cfg = c;
   (void)hipGetDevice(&s->device);

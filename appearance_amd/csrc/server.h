@@ -41,6 +41,7 @@ struct SrvDevice {
   PathView ps;
   uint32_t* path_req;
   uint32_t* path_pix;
+  uint32_t persistent;  // 1: one persistent kernel per batch, 0: wavefront stages
 };
 
 apt_status srv_enqueue_batch(const SrvDevice& d, hipStream_t s);

@@ -62,6 +62,16 @@ __global__ __launch_bounds__(BLOCK) void k_srv_raygen(const SrvBatch* __restrict
 
 constexpr uint32_t SRV_REFILL = 32;  // persistent traversal refill threshold
 
+// Waves per SIMD of the persistent batch kernel (shading and both traversals
+// in one kernel), and its grid: every CU's share at that occupancy.
+constexpr int SRV_PERSIST_WAVES = 4;
+inline uint32_t srv_persist_blocks() {
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return (uint32_t)std::max(1, cus) * (uint32_t)SRV_PERSIST_WAVES;  // 4 SIMDs x waves / 4 waves per block
+}
+
 // Rays of paths from many requests (and scenes) in one launch.
 template <int W>
 struct SrvExtendSrc {
@@ -199,8 +209,104 @@ __global__ void k_srv_zero(unsigned long long* __restrict__ a, size_t n) {
   if (i < n) a[i] = 0ull;
 }
 
+// ---- one persistent kernel per batch ------------------------------------------
+// Every lane owns a path slot and pulls the batch's paths from a counter.  In
+// each round the lanes of a wave that hold no path (at the start, or whose
+// path just ended) take the next batch paths with one atomic add per wave;
+// then every lane with a path runs its next segment -- the extension ray,
+// the shading of its hit, the shadow ray -- so the paths of a wave advance
+// together whatever request, scene, resolution or depth they belong to.  A
+// finished path is added to its request's accumulator (64-bit fixed point,
+// as k_srv_film) before its lane takes the next.  A wave leaves once the
+// counter is past the batch and none of its lanes holds a path.  The
+// per-path arithmetic is the wavefront stages' (start_path, shade_path, the
+// bit-exact traversals, the same additions), so a request's image is the same
+// bits in either mode.
+__global__ void k_srv_persist_reset(PathView ps) {
+  if (threadIdx.x == 0) ps.counters[CNT_QUEUE0] = 0;
+}
+
+template <int W, int S>
+__global__ __launch_bounds__(BLOCK, SRV_PERSIST_WAVES) void k_srv_persistent(const SrvBatch* __restrict__ batch,
+                                                                      const SrvReq* __restrict__ reqs, PathView ps) {
+  __shared__ uint2 s_stack[S * BLOCK];
+  const LdsStack st{s_stack + threadIdx.x, BLOCK};
+  const uint32_t slot = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t n_paths = batch->n_paths;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* next = &ps.counters[CNT_QUEUE0];
+  bool active = false;
+  uint32_t bounce = 0, req = 0, pixel = 0;
+  while (true) {
+    const unsigned long long idle = __ballot(!active);
+    if (idle) {
+      const int leader = __ffsll((long long)idle) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) {
+        // once the batch is handed out no further adds: the counter cannot run
+        // past n_paths by more than the waves' idle lanes
+        base = __hip_atomic_load(next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (base < n_paths) base = atomicAdd(next, (uint32_t)__popcll(idle));
+      }
+      base = __shfl(base, leader);
+      if (!active) {
+        const uint32_t path = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (base < n_paths && path < n_paths) {
+          const SrvTile& t = batch->tiles[find_tile(batch->tiles, batch->n_tiles, path)];
+          const SrvReq& r = reqs[t.req_slot];
+          const uint32_t idx = t.first + (path - t.offset);  // request path index, sample-major
+          const uint32_t npix = r.width * r.height;
+          const uint32_t sample = idx / npix;
+          pixel = idx - sample * npix;
+          start_path(r.sc, ps, slot, pixel % r.width, pixel / r.width, r.width, r.height, sample + r.seed);
+          req = t.req_slot;
+          bounce = 0;
+          active = true;
+        }
+      }
+    }
+    if (!__any(active)) break;
+    if (active) {
+      const SrvReq& r = reqs[req];
+      const WNode<W>* nodes = static_cast<const WNode<W>*>(r.sc.nodes);
+      const float4 o = ps.o_pdf[slot], d = ps.d_flags[slot];
+      const HitRec h = trace_closest<W, S, false>(nodes, r.sc.bvh_tris, r.sc.root,
+                                                   make_trace_ray(ld3(o), ld3(d), 0.0f, 1e30f), st, nullptr);
+      ps.hit[slot] = make_float4(h.t, h.u, h.v, u2f(h.prim));
+      bool alive = false, want_shadow = false;
+      // resident scenes carry no textures
+      shade_path(scene_view<false>(r.sc), ps, slot, bounce, r.max_bounces, r.rr_start, alive, want_shadow);
+      if (want_shadow) {
+        const float4 so = ps.sh_o[slot], sd = ps.sh_d[slot];
+        if (!trace_any<W, S, false>(nodes, r.sc.bvh_tris, r.sc.root, make_trace_ray(ld3(so), ld3(sd), 0.0f, so.w),
+                                    st, nullptr)) {
+          const float4 c = ps.sh_c[slot], l = ps.rad[slot];
+          ps.rad[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+        }
+      }
+      ++bounce;
+      if (!alive) {
+        const float4 l = ps.rad[slot];
+        unsigned long long* a = r.accum + 3 * (size_t)pixel;
+        atomicAdd(a + 0, to_fixed(l.x));
+        atomicAdd(a + 1, to_fixed(l.y));
+        atomicAdd(a + 2, to_fixed(l.z));
+        active = false;
+      }
+    }
+  }
+}
+
 template <int W, int S>
 void enqueue_batch_t(const SrvDevice& d, hipStream_t s) {
+  if (d.persistent) {
+    // a grid that fills the device at the kernel's occupancy, within the path slots
+    const uint32_t blocks = std::min<uint32_t>(srv_persist_blocks(), d.path_slots / BLOCK);
+    hipLaunchKernelGGL(k_srv_persist_reset, dim3(1), dim3(64), 0, s, d.ps);
+    hipLaunchKernelGGL((k_srv_persistent<W, S>), dim3(std::max(1u, blocks)), dim3(BLOCK), 0, s, d.batch, d.reqs,
+                       d.ps);
+    return;
+  }
   // every stage strides over (or pulls from) the batch with a grid that fills the device
   const dim3 grid(std::min<uint32_t>(pool_blocks(), (d.path_slots + BLOCK - 1) / BLOCK)), block(BLOCK);
   const dim3 pool(std::min<uint32_t>(pool_blocks(), (d.path_slots + 63) / 64));

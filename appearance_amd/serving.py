@@ -16,7 +16,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("path_slots", C.c_uint32), ("max_active", C.c_uint32),
                 ("accum_budget", C.c_uint64), ("max_bounces", C.c_uint32),
                 ("tile_paths", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("lds_stack_depth", C.c_uint32), ("policy", C.c_uint32)]
+                ("lds_stack_depth", C.c_uint32), ("policy", C.c_uint32), ("persistent", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -66,9 +66,9 @@ def _lib():
 class Server:
     def __init__(self, path_slots: int = 0, max_active: int = 0, accum_budget: int = 0,
                  max_bounces: int = 0, tile_paths: int = 0, bvh_width: int = 0,
-                 lds_stack_depth: int = 0, policy: int = 0):
+                 lds_stack_depth: int = 0, policy: int = 0, persistent: int = 0):
         cfg = ServerConfig(path_slots, max_active, accum_budget, max_bounces, tile_paths, bvh_width,
-                           lds_stack_depth, policy)
+                           lds_stack_depth, policy, persistent)
         h = C.c_void_p()
         check(_lib().apt_server_create(C.byref(cfg), C.byref(h)), "apt_server_create")
         self._h = h

@@ -229,6 +229,8 @@ def main() -> int:
                     choices=["disney", "bvh_stress", "restir", "serving", "bvh_build"])
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--rates", default="20,200")
+    ap.add_argument("--persistent", type=int, default=0,
+                    help="serving: 1 = one persistent kernel per batch, 0 = wavefront stages")
     ap.add_argument("--policy", type=int, default=0,
                     help="serving: 0 = one-batch requests shortest first, then the rest in "
                          "arrival order; 1 = FIFO; 2 = shortest remaining work first")
@@ -587,7 +589,7 @@ def bench_serving(args) -> int:
     scenes = [generate_scene(n, seed=[0, 1, 3, 3][i]) for i, n in enumerate(names)]
     sky = scenes[2].sky
     scenes[1].sky = sky  # the triangle soup has no emitters: light it with the sky
-    srv = Server(max_active=64, policy=args.policy)
+    srv = Server(max_active=64, policy=args.policy, persistent=args.persistent)
     ids = [srv.add_scene(sc) for sc in scenes]
     plain = []
     for sc in scenes:
@@ -675,7 +677,8 @@ def bench_serving(args) -> int:
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"serving: {args.requests} requests, Poisson arrivals, 4 resident "
                                f"scenes, res in {{256^2,512^2,1280x720,1920x1080}}, spp in "
-                               f"{{1,4,16,64}}, depth 4..8", "total_samples": total_samples, "policy": args.policy},
+                               f"{{1,4,16,64}}, depth 4..8", "total_samples": total_samples, "policy": args.policy,
+                               "persistent": args.persistent},
         "serving": report,
     }
     print(json.dumps(out), flush=True)

@@ -628,6 +628,10 @@ typedef struct apt_server_config {
   uint32_t policy;            /* 0 = requests of at most path_slots samples shortest remaining work
                                  first, then the larger ones in arrival order; 1 = FIFO;
                                  2 = shortest remaining work first */
+  uint32_t persistent;        /* 0 = a batch is wavefront stages (raygen, extend/shade/shadow per
+                                 bounce, film); 1 = one persistent kernel per batch whose lanes
+                                 pull the batch's paths from a counter and run each through all
+                                 its bounces.  Both give the same bits. */
 } apt_server_config;
 
 /* A render request: scene, camera, frame size, samples, depth and seed. */

@@ -336,6 +336,7 @@ pub struct apt_server_config {
     pub bvh_width: u32,
     pub lds_stack_depth: u32,
     pub policy: u32,
+    pub persistent: u32,
 }
 
 #[repr(C)]

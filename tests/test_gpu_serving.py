@@ -156,3 +156,41 @@ def test_release(gpu_available, scenes):
     assert e.value.status == 6
     srv.release(big)
     assert srv.stats()["requests_done"] == 2
+
+
+def test_persistent_kernel_equals_wavefront(gpu_available):
+    """The same mixed stream served with a batch as one persistent kernel
+    (lanes pull paths from a counter and run them through every bounce) and
+    as wavefront stages: every request's image is the same bits."""
+    import time
+
+    from appearance_amd.serving import mixed_requests, orbit_camera
+
+    names = ["cornell", "bvh_stress", "disney", "restir"]
+    scenes = [generate_scene(n, seed=[0, 1, 3, 3][i]) for i, n in enumerate(names)]
+    scenes[1].sky = scenes[2].sky
+    reqs = [r for r in mixed_requests(48, 100.0, seed=11) if r["w"] * r["h"] * r["spp"] <= 16 << 20][:16]
+    assert len({r["scene"] for r in reqs}) >= 3
+    cams = [orbit_camera(scenes[r["scene"]], r["angle"]) for r in reqs]
+    out = {}
+    for persistent in (0, 1):
+        srv = Server(path_slots=1 << 22, persistent=persistent)
+        ids = [srv.add_scene(sc) for sc in scenes]
+        t0 = time.perf_counter()
+        rids = [srv.submit(ids[r["scene"]], r["w"], r["h"], r["spp"], r["depth"], c, seed=r["seed"])
+                for r, c in zip(reqs, cams)]
+        imgs = []
+        for rid in rids:
+            assert srv.wait(rid).state == DONE
+            imgs.append(srv.release(rid))
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        out[persistent] = imgs
+        ms = sum(r["w"] * r["h"] * r["spp"] for r in reqs) / 1e6
+        print(f"[serving] {'persistent kernel' if persistent else 'wavefront stages'}: {len(reqs)} requests, "
+              f"{ms:.0f} Msamples in {dt * 1e3:.0f} ms ({ms / dt:.0f} Msamples/s), GPU busy "
+              f"{st['gpu_busy_ms']:.0f} ms, {st['batches']} batches", flush=True)
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+    assert max(float(np.abs(a).max()) for a in out[1]) > 0
