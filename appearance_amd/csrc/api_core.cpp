@@ -79,8 +79,8 @@ static apt_status resolve_opts(const apt_bvh_opts* opts, apt_bvh_opts& o) {
   if (o.lds_stack == 0) o.lds_stack = 8;
   if (o.refill == 0) o.refill = 32;
   APT_REQUIRE(o.refill <= 65, APT_ERR_INVALID_ARG, "bvh: refill %u outside 1..65", o.refill);
-  APT_REQUIRE(o.lds_stack == 8 || o.lds_stack == 16 || o.lds_stack == 24, APT_ERR_INVALID_ARG,
-              "bvh: lds_stack %u is not 8, 16 or 24", o.lds_stack);
+  APT_REQUIRE(o.lds_stack == 8 || o.lds_stack == 12 || o.lds_stack == 16, APT_ERR_INVALID_ARG,
+              "bvh: lds_stack %u is not 8, 12 or 16", o.lds_stack);
   APT_REQUIRE(o.width == 2 || o.width == 4 || o.width == 8, APT_ERR_INVALID_ARG,
               "bvh: width %u is not 2, 4 or 8", o.width);
   APT_REQUIRE(o.max_leaf >= 1 && o.max_leaf <= 8, APT_ERR_INVALID_ARG,

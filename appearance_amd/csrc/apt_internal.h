@@ -208,7 +208,7 @@ struct apt_bvh {
   uint32_t n_tris = 0;
   uint32_t n_nodes = 0;
   uint32_t root = 0;        // root ref (an internal node or a single leaf)
-  uint32_t lds_stack = 16;  // traversal stack entries kept in LDS (8, 16, 24)
+  uint32_t lds_stack = 16;  // traversal stack entries kept in LDS (8, 12, 16)
   uint32_t refill = 32;     // persistent traversal refill threshold (65: one ray per thread)
   void* d_nodes = nullptr;  // WNode<width>[n_nodes]
   float4* d_tris = nullptr; // 3 * n_tris float4 in leaf order

@@ -43,7 +43,7 @@ struct SceneView {
   uint32_t n_nodes;             // nodes the pool kernels may stage in LDS (a two-level scene: the TLAS's)
   const InstRec* insts;         // instance records of a two-level scene, or null
   uint32_t width;
-  uint32_t lds_stack;           // traversal stack entries in LDS (8, 16, 24)
+  uint32_t lds_stack;           // traversal stack entries in LDS (8, 12, 16)
   uint32_t refill;              // persistent traversal refill threshold (65: one ray per thread)
   const float4* tri_pos;        // 3 per triangle
   const float4* tri_nrm;        // 3 per triangle, or null (geometric normals)

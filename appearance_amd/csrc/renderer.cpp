@@ -337,8 +337,8 @@ apt_status validate_config(const apt_config& c) {
               "config: sample_count %u outside 1..65536", c.sample_count);
   APT_REQUIRE(c.bvh_width == 2 || c.bvh_width == 4 || c.bvh_width == 8, APT_ERR_INVALID_ARG,
               "config: bvh_width %u is not 2, 4 or 8", c.bvh_width);
-  APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 16 || c.lds_stack_depth == 24,
-              APT_ERR_INVALID_ARG, "config: lds_stack_depth %u is not 8, 16 or 24", c.lds_stack_depth);
+  APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 12 || c.lds_stack_depth == 16,
+              APT_ERR_INVALID_ARG, "config: lds_stack_depth %u is not 8, 12 or 16", c.lds_stack_depth);
   APT_REQUIRE(c.integrator == APT_INTEGRATOR_PLAIN || c.integrator == APT_INTEGRATOR_RESTIR,
               APT_ERR_INVALID_ARG, "config: unknown integrator %u", c.integrator);
   if (c.integrator == APT_INTEGRATOR_PLAIN) {

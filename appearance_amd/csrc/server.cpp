@@ -520,8 +520,8 @@ apt_status apt_server_create(const apt_server_config* cfg, apt_server** out) {
               "server: max_bounces %u outside 1..64", c.max_bounces);
   APT_REQUIRE(c.bvh_width == 2 || c.bvh_width == 4 || c.bvh_width == 8, APT_ERR_INVALID_ARG,
               "server: bvh_width %u is not 2, 4 or 8", c.bvh_width);
-  APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 16 || c.lds_stack_depth == 24,
-              APT_ERR_INVALID_ARG, "server: lds_stack_depth %u is not 8, 16 or 24", c.lds_stack_depth);
+  APT_REQUIRE(c.lds_stack_depth == 8 || c.lds_stack_depth == 12 || c.lds_stack_depth == 16,
+              APT_ERR_INVALID_ARG, "server: lds_stack_depth %u is not 8, 12 or 16", c.lds_stack_depth);
   APT_REQUIRE(c.policy <= 2, APT_ERR_INVALID_ARG, "server: unknown policy %u", c.policy);
   APT_REQUIRE(c.persistent <= 1, APT_ERR_INVALID_ARG, "server: persistent must be 0 or 1, not %u", c.persistent);
   apt_server* s = new apt_server();

@@ -4,7 +4,7 @@
 // rayQueryProceed / rayQueryGetCommittedIntersection, trace.wgsl:98-102 and
 // helpers/trace.wgsl:20-26).  The traversal stack keeps its bottom S entries
 // in LDS (laid out [entry][lane] so lanes at the same depth hit distinct
-// banks; S = 8, 16 or 24 per BVH, trading LDS for occupancy) and spills
+// banks; S = 8, 12 or 16 per BVH, trading LDS for occupancy) and spills
 // deeper entries to a small private array; the host checks that the BVH's
 // worst-case stack fits.
 //

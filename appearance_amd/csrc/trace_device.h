@@ -215,8 +215,8 @@ __device__ __forceinline__ bool trace_any(const WNode<W>* __restrict__ nodes,
 #define APT_FOR_STACK_DEPTH(S, ...) \
   switch (S) {                      \
     case 8: { constexpr int S_ = 8; __VA_ARGS__; } break;   \
-    case 16: { constexpr int S_ = 16; __VA_ARGS__; } break; \
-    default: { constexpr int S_ = 24; __VA_ARGS__; } break; \
+    case 12: { constexpr int S_ = 12; __VA_ARGS__; } break; \
+    default: { constexpr int S_ = 16; __VA_ARGS__; } break; \
   }
 
 // 256-thread blocks that fill the device at 8 waves per SIMD (8 blocks per

@@ -27,7 +27,7 @@ apt_status apt_autotune(const apt_mesh_desc* mesh, const apt_sky* sky, const apt
   if (!o.probe_bounces) o.probe_bounces = 4;
   if (!o.repeats) o.repeats = 2;
   const uint32_t widths[3] = {2, 4, 8};
-  const uint32_t stacks[3] = {8, 16, 24};
+  const uint32_t stacks[3] = {8, 12, 16};
   const uint32_t npix = o.probe_width * o.probe_height;
   // whole frame per pass (0 = default budget) or one sample per pass
   const uint32_t chunks[2] = {0, npix};

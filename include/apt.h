@@ -95,7 +95,7 @@ typedef struct apt_bvh_opts {
   uint32_t max_leaf;  /* triangles per leaf, 1..8 (0 = 4) */
   uint32_t bins;      /* SAH bins per axis, 4..64 (0 = 32) */
   uint32_t builder;   /* apt_builder */
-  uint32_t lds_stack; /* traversal stack entries kept in LDS: 8, 16 or 24 (0 = 8) */
+  uint32_t lds_stack; /* traversal stack entries kept in LDS: 8, 12 or 16 (0 = 8) */
   uint32_t refill;    /* ray queries: persistent waves refetch rays once this many of their
                          64 lanes are idle, 1..64; 65 = one ray per thread (0 = 32) */
 } apt_bvh_opts;
@@ -446,7 +446,7 @@ typedef struct apt_config {
   uint32_t svgf_history;      /* RESTIR: max frames of SVGF moment history */
   uint32_t bvh_width;         /* 2, 4, 8 */
   uint32_t queue_chunk;       /* PLAIN: paths per wavefront pass (0 = 2^24) */
-  uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS: 8, 16, 24 */
+  uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS: 8, 12, 16 */
   uint32_t seed;              /* base seed mixed into every path seed */
   uint32_t frame_idx;         /* starting frame index */
 } apt_config;
@@ -575,7 +575,7 @@ apt_status apt_get_render_stats(const apt_renderer* r, apt_render_stats* out);
 
 /* ------------------------------------------------------------------------
  * Autotuner: per-scene choice of BVH width (2/4/8), traversal stack entries
- * kept in LDS (8/16/24) and wavefront path budget, by timing a probe frame of
+ * kept in LDS (8/12/16) and wavefront path budget, by timing a probe frame of
  * the integrator on the device for each candidate.  Every candidate must give
  * the identical image (checked); the fastest is returned with a JSON report.
  * ------------------------------------------------------------------------ */
@@ -624,7 +624,7 @@ typedef struct apt_server_config {
   uint32_t max_bounces;       /* deepest path any request may ask for (0 = 8) */
   uint32_t tile_paths;        /* paths per scheduling tile (0 = 65536) */
   uint32_t bvh_width;         /* 2, 4, 8 (0 = 4) */
-  uint32_t lds_stack_depth;   /* 8, 16, 24 (0 = 8) */
+  uint32_t lds_stack_depth;   /* 8, 12, 16 (0 = 8) */
   uint32_t policy;            /* 0 = requests of at most path_slots samples shortest remaining work
                                  first, then the larger ones in arrival order; 1 = FIFO;
                                  2 = shortest remaining work first */

@@ -7,7 +7,8 @@ texture slot, read by libapt's glTF reader (PNG images, generated tangents),
 * spawned twice as a model (SpawnModel of the GLB, one copy rotated and
   scaled non-uniformly): the two-level scene with per-asset textures, against
   the oracle on the same world-space triangles, uvs and tangent frames placed
-  in numpy.
+  in numpy, with the plain integrator and (under a sky panorama) with the
+  ReSTIR one.
 """
 import uuid
 
@@ -137,3 +138,39 @@ def test_spawned_textured_models_vs_oracle(gpu_available, oracle, room):
           f"relMSE {r:.3e}", flush=True)
     assert np.isfinite(got).all() and got.mean() > 0.05
     assert r < 1e-4
+
+
+def test_spawned_textured_models_restir_vs_oracle(gpu_available, oracle, room):
+    """The ReSTIR integrator on the two spawned rooms (the textured two-level
+    kernel variants) under a sky panorama."""
+    path, s = room
+    ms = [_xform(0.0, (0, 1, 0), (1, 1, 1), (0, 0, 0)),
+          _xform(30.0, (0.3, 1, 0.1), (0.25, 0.35, 0.3), (0.4, 0.8, 1.2))]
+    parts = [_place(s, m, k * len(s.materials)) for k, m in enumerate(ms)]
+    soup = SceneData(np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]),
+                     np.concatenate([p[5] for p in parts]),
+                     [Material.from_buffer_copy(m) for m in list(s.materials) * 2], s.camera, s.sky,
+                     uvs=np.concatenate([p[4] for p in parts]),
+                     tangents=np.zeros((sum(len(p[0]) for p in parts), 12), np.float32),
+                     textures=list(s.textures) * 2)
+    for m in soup.materials[len(s.materials):]:
+        for f, _ in m._fields_:
+            if f.endswith("_texture") and getattr(m, f) != 0xFFFFFFFF:
+                setattr(m, f, getattr(m, f) + len(s.textures))
+    frames = (np.concatenate([p[2] for p in parts]), np.concatenate([p[3] for p in parts]))
+    j, i = np.mgrid[0:16, 0:32].astype(np.float32)
+    pano = np.stack([0.2 + 0.02 * i, 0.3 + 0.03 * j, 0.6 + 0.0 * i], -1).astype(np.float32)
+    cfg = default_config()
+    pt = PathTracerGpu(cfg)
+    for k, m in enumerate(ms):
+        pt.handle_visible_world_action(world.SpawnModel(str(path), m, uuid.UUID(int=k + 1)))
+    pt.set_camera(_cam())
+    pt.set_sky_panorama(pano)
+    orr = OracleRestir(oracle, soup, 128, 96, frames=frames, sky_pano=pano)
+    for f in range(3):
+        _, got = pt.render_accum((128, 96), with_rgba=False)
+        ref = orr.frame(cfg, camera=_cam(), sky=s.sky, max_bounces=cfg.max_bounces, frame_idx=f)
+        r = relmse(got, ref)
+        print(f"[textures] two spawned rooms under a panorama, ReSTIR frame {f}: relMSE {r:.3e}", flush=True)
+        assert np.isfinite(got).all()
+        assert r < 1e-3
