@@ -177,6 +177,9 @@ _SIGS = {
     "apt_get_render_stats": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "apt_set_frame_index": (C.c_int, [_P, C.c_uint32]),
     "apt_start_render": (C.c_int, [_P, _P, C.c_size_t, BLOCK_CB, _P]),
+    "apt_start_render_jpeg": (C.c_int, [_P, _P, C.c_size_t, C.c_int, BLOCK_CB, _P]),
+    "apt_jpeg_compress_blocks": (C.c_int, [_P, C.c_uint32, C.c_int, _P, C.c_size_t, C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_float)]),
 }
 
 _lib = None

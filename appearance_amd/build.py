@@ -34,6 +34,7 @@ SOURCES = [
     "gltf.cpp",
     "image.cpp",
     "tlas.hip",
+    "jpeg.hip",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

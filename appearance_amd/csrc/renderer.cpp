@@ -19,6 +19,7 @@
 
 #include "apt_internal.h"
 #include "integrator.h"
+#include "jpeg.h"
 #include "restir.h"
 #include "scene_gpu.h"
 #include "tlas.h"
@@ -301,6 +302,10 @@ struct apt_renderer {
   apt_render_stats stats{};
   uint32_t frame_idx = 0;
   PinnedBytes cb_pixels;  // the callback's borrowed RGBA8 slab
+  // apt_start_render_jpeg: resolve the RGBA8 frame into d_rgba even with no
+  // host output, and compress its blocks there
+  bool rgba_on_device = false;
+  JpegEncoder jpeg;
   // ReSTIR integrator: per-pixel state, frame parity and the previous camera
   RestirState rs;
   uint32_t rs_cur = 0;
@@ -599,7 +604,7 @@ apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint3
     return st;
   FrameView fv{width, height, row_start, rows, spp, r->frame_idx, c.seed, c.max_bounces, 2};
   if ((st = resolve_frame(fv, r->d_frame_sum.p, r->d_accum_tm.p, r->d_accum_lin.p, r->accum_frames,
-                          out_rgba ? r->d_rgba.p : nullptr, s)) != APT_OK)
+                          out_rgba || r->rgba_on_device ? r->d_rgba.p : nullptr, s)) != APT_OK)
     return st;
   APT_HIP(hipEventRecord(r->ev1, s));
   uint32_t part[2 * RS_CTR_SLOTS];
@@ -631,7 +636,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
   APT_REQUIRE(width > 0 && height > 0 && row_start < row_end && row_end <= height,
               APT_ERR_INVALID_ARG, "render: bad frame %ux%u rows [%u,%u)", width, height, row_start, row_end);
   const uint32_t rows = row_end - row_start;
-  APT_REQUIRE(!out_rgba || (width % 64 == 0 && rows % 64 == 0), APT_ERR_INVALID_ARG,
+  APT_REQUIRE((!out_rgba && !r->rgba_on_device) || (width % 64 == 0 && rows % 64 == 0), APT_ERR_INVALID_ARG,
               "render: block-layout output needs width (%u) and slab height (%u) to be multiples "
               "of 64 (host.rs:476)", width, rows);
   const size_t npix = (size_t)width * rows;
@@ -706,7 +711,7 @@ apt_status render_impl(apt_renderer* r, uint32_t width, uint32_t height, uint32_
     if (st != APT_OK) return st;
   }
   st = resolve_frame(fv, r->d_frame_sum.p, r->d_accum_tm.p, r->d_accum_lin.p, r->accum_frames,
-                     out_rgba ? r->d_rgba.p : nullptr, s);
+                     out_rgba || r->rgba_on_device ? r->d_rgba.p : nullptr, s);
   if (st != APT_OK) return st;
   APT_HIP(hipEventRecord(r->ev1, s));
   uint32_t cnt[CNT_COUNT];
@@ -1030,21 +1035,62 @@ apt_status apt_render_cb(apt_renderer* r, uint32_t width, uint32_t height, uint3
   return APT_OK;
 }
 
+namespace {
+
+// StartRenderData as it arrives (host.rs:85-93), checked to be a slab of
+// whole 64x64 blocks (RENDER_BLOCK_SIZE, host.rs:17).
+apt_status parse_start(const void* data, size_t len, const char* who, apt_start_render_data& d) {
+  APT_REQUIRE(len == sizeof(apt_start_render_data), APT_ERR_INVALID_ARG,
+              "%s: StartRenderData carries %zu bytes, got %zu", who, sizeof(apt_start_render_data), len);
+  std::memcpy(&d, data, sizeof(d));
+  APT_REQUIRE(d.row_end > d.row_start && d.row_end <= d.height && d.width % 64 == 0 &&
+                  (d.row_end - d.row_start) % 64 == 0,
+              APT_ERR_INVALID_ARG, "%s: %ux%u rows [%u,%u) is not a slab of whole 64x64 blocks", who, d.width,
+              d.height, d.row_start, d.row_end);
+  return APT_OK;
+}
+
+}  // namespace
+
+apt_status apt_start_render_jpeg(apt_renderer* r, const void* start_render_data, size_t len, int quality,
+                                 apt_block_cb cb, void* user) {
+  clear_error();
+  APT_REQUIRE(r && start_render_data && cb, APT_ERR_INVALID_ARG, "apt_start_render_jpeg: null argument");
+  APT_REQUIRE(quality >= 1 && quality <= 100, APT_ERR_INVALID_ARG, "apt_start_render_jpeg: quality %d outside 1..100",
+              quality);
+  apt_start_render_data d;
+  apt_status st = parse_start(start_render_data, len, "apt_start_render_jpeg", d);
+  if (st != APT_OK) return st;
+  constexpr uint32_t B = 64;
+  const uint32_t nx = d.width / B, ny = (d.row_end - d.row_start) / B;
+  APT_REQUIRE((size_t)nx * ny <= JPEG_MAX_BLOCKS, APT_ERR_INVALID_ARG,
+              "apt_start_render_jpeg: %u blocks exceed %u", nx * ny, JPEG_MAX_BLOCKS);
+  // the frame stays in HBM: resolved to RGBA8 blocks there, compressed there,
+  // and only the files cross to the host
+  r->rgba_on_device = true;
+  st = render_impl(r, d.width, d.height, d.row_start, d.row_end, nullptr, 0, nullptr, 0);
+  r->rgba_on_device = false;
+  if (st != APT_OK) return st;
+  if ((st = r->jpeg.encode(r->d_rgba.p, nx * ny, quality, r->stream)) != APT_OK) return st;
+  for (uint32_t by = 0; by < ny; ++by)
+    for (uint32_t bx = 0; bx < nx; ++bx) {
+      const uint32_t b = by * nx + bx;
+      const uint32_t first = r->jpeg.offsets[b];
+      cb(by * B + d.row_start, bx, d.frame_idx, r->jpeg.h_files + first, r->jpeg.offsets[b + 1] - first, user);
+    }
+  return APT_OK;
+}
+
 apt_status apt_start_render(apt_renderer* r, const void* start_render_data, size_t len, apt_block_cb cb,
                             void* user) {
   clear_error();
   APT_REQUIRE(r && start_render_data && cb, APT_ERR_INVALID_ARG, "apt_start_render: null argument");
-  APT_REQUIRE(len == sizeof(apt_start_render_data), APT_ERR_INVALID_ARG,
-              "apt_start_render: StartRenderData carries %zu bytes, got %zu", sizeof(apt_start_render_data), len);
   apt_start_render_data d;
-  std::memcpy(&d, start_render_data, sizeof(d));
+  apt_status st = parse_start(start_render_data, len, "apt_start_render", d);
+  if (st != APT_OK) return st;
   constexpr uint32_t B = 64;  // RENDER_BLOCK_SIZE
-  APT_REQUIRE(d.row_end > d.row_start && d.row_end <= d.height && d.width % B == 0 &&
-                  (d.row_end - d.row_start) % B == 0,
-              APT_ERR_INVALID_ARG, "apt_start_render: %ux%u rows [%u,%u) is not a slab of whole 64x64 blocks",
-              d.width, d.height, d.row_start, d.row_end);
   const size_t bytes = (size_t)d.width * (d.row_end - d.row_start) * 4;
-  apt_status st = r->cb_pixels.ensure(bytes);
+  st = r->cb_pixels.ensure(bytes);
   if (st != APT_OK) return st;
   st = render_impl(r, d.width, d.height, d.row_start, d.row_end, r->cb_pixels.p, bytes, nullptr, 0);
   if (st != APT_OK) return st;

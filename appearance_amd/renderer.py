@@ -143,10 +143,12 @@ class PathTracerGpu:
         end_row = h if end_row is None else end_row
         check(lib().apt_render(self._h, w, h, start_row, end_row, None, 0, None, 0), "apt_render")
 
-    def start_render(self, data, block_callback) -> None:
+    def start_render(self, data, block_callback, jpeg_quality: int | None = None) -> None:
         """Node::start_render (node.rs:44-105) without the transport: data is a
         world.StartRenderData or its 20 bytes; block_callback(row, column_block,
-        frame_idx, pixels) receives each 64x64 RGBA8 block of the slab."""
+        frame_idx, pixels) receives each 64x64 RGBA8 block of the slab, or with
+        jpeg_quality (the node uses 95) the block's JPEG file, compressed on
+        the device (RenderPartialFinishedData::compressed_pixel_bytes)."""
         raw = data.to_bytes() if hasattr(data, "to_bytes") else bytes(data)
 
         def _cb(row, col, frame, ptr, n, _user):
@@ -154,7 +156,11 @@ class PathTracerGpu:
 
         cb = BLOCK_CB(_cb)
         buf = C.create_string_buffer(raw, len(raw))
-        check(lib().apt_start_render(self._h, buf, len(raw), cb, None), "apt_start_render")
+        if jpeg_quality is None:
+            check(lib().apt_start_render(self._h, buf, len(raw), cb, None), "apt_start_render")
+        else:
+            check(lib().apt_start_render_jpeg(self._h, buf, len(raw), int(jpeg_quality), cb, None),
+                  "apt_start_render_jpeg")
 
     def set_frame_index(self, frame_idx: int) -> None:
         """Restart the frame sequence at frame_idx (seeds frame * spp + sample)."""

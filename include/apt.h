@@ -545,11 +545,38 @@ typedef void (*apt_block_cb)(uint32_t row, uint32_t column_block, uint32_t frame
  * without the transport: start_render_data holds the 20 bytes of
  * StartRenderData as they arrive; the slab is rendered (NodeRenderer::render)
  * and cb receives its blocks in the order the node sends them (block rows top
- * to bottom, columns left to right).  The JPEG compression and the UDP
- * messages stay with the caller.  Width and slab height must be multiples of
- * 64 (host.rs:476). */
+ * to bottom, columns left to right).  The UDP messages stay with the
+ * caller; apt_start_render_jpeg below also does the node's compression.
+ * Width and slab height must be multiples of 64 (host.rs:476). */
 apt_status apt_start_render(apt_renderer* r, const void* start_render_data, size_t len, apt_block_cb cb,
                             void* user);
+
+/* JPEG compression of 64x64 RGBX blocks on the device, the node's
+ * turbojpeg::compress(image, quality, Subsamp::Sub2x2) of every block it sends
+ * (node.rs:65-79, quality 95; NODE_PIXEL_FORMAT RGBX, host.rs:22).  Each
+ * block becomes one complete baseline JFIF file, byte for byte what
+ * libjpeg-turbo's default compression (accurate integer DCT, the standard
+ * Huffman tables, 4:2:0) writes for it. */
+#define APT_JPEG_QUALITY 95
+#define APT_JPEG_FILE_MAX 41600u   /* bytes one block's file can take (623 B header + segment) */
+#define APT_JPEG_MAX_BLOCKS 16384u /* blocks per call */
+
+/* Node::start_render with the node's compression (node.rs:44-105): as
+ * apt_start_render, but each callback's pixels/len are the block's JPEG file
+ * (RenderPartialFinishedData::compressed_pixel_bytes, host.rs:24-29),
+ * compressed on the device at `quality` (1..100). */
+apt_status apt_start_render_jpeg(apt_renderer* r, const void* start_render_data, size_t len, int quality,
+                                 apt_block_cb cb, void* user);
+
+/* turbojpeg::compress(image, quality, Sub2x2) for n_blocks blocks of 64x64
+ * RGBX pixels (16384 bytes each, rows top to bottom, X ignored) in host
+ * memory.  Block b's file is out[offsets[b] .. offsets[b+1]); offsets holds
+ * n_blocks + 1 entries.  out_cap below the total fails with
+ * APT_ERR_INVALID_ARG and offsets[n_blocks] set to the bytes needed
+ * (n_blocks * APT_JPEG_FILE_MAX always suffices).  ms_device, when not NULL,
+ * receives the device time of the compression kernels. */
+apt_status apt_jpeg_compress_blocks(const uint8_t* rgbx, uint32_t n_blocks, int quality, uint8_t* out,
+                                    size_t out_cap, uint64_t* offsets, float* ms_device);
 
 /* Restarts the frame sequence at frame_idx (seeds are frame*spp + sample). */
 apt_status apt_set_frame_index(apt_renderer* r, uint32_t frame_idx);

@@ -60,6 +60,7 @@ void or_tex_sample(const void* tex, float u, float v, float* out4);
 void or_sky_probe(const void* sky, const void* attribs, const float* d, float* out);
 void or_shading_probe(const void* mat, const void* attribs, uint32_t prim, float bu, float bv, const float* n,
                       float* out);
+size_t or_jpeg_block(const uint8_t* rgbx, int quality, uint8_t* out, size_t cap);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,9 @@
 //!   handle_visible_world_action, render}` behind `NodeRenderer`
 //!   (crates/appearance-render-loop/src/node.rs:14-25).
 //! * `apt_start_render`: `Node::start_render` (node.rs:44-105) without the
-//!   transport.
+//!   transport; `apt_start_render_jpeg` with the node's per-block JPEG
+//!   compression (node.rs:65-79), `apt_jpeg_compress_blocks` that compression
+//!   alone.
 //! * `apt_bvh_*` / `apt_trace_*`: the driver's BLAS build and ray query;
 //!   `apt_tlas_*`: its TLAS over BLAS instances.
 //! * `apt_server_*`, `apt_submit`, ...: the serving loop (no counterpart).
@@ -32,6 +34,9 @@ pub const APT_ERR_NOT_FOUND: apt_status = 6;
 pub const APT_ABI_VERSION: u32 = 1;
 pub const APT_MISS: u32 = 0xFFFF_FFFF;
 pub const APT_TRACE_NOALLOC_MAX_RAYS: u32 = 65536;
+pub const APT_JPEG_QUALITY: u32 = 95;
+pub const APT_JPEG_FILE_MAX: u32 = 41600;
+pub const APT_JPEG_MAX_BLOCKS: u32 = 16384;
 pub const APT_SKY_PANORAMA_MAX: u32 = 16384;
 pub const APT_TLAS_MAX_INSTANCES: u32 = 8192;
 pub const APT_INVALID_TEXTURE: u32 = 0xFFFF_FFFF;
@@ -465,6 +470,10 @@ extern "C" {
                          cb: apt_result_cb, user: *mut c_void) -> apt_status;
     pub fn apt_start_render(r: *mut apt_renderer, start_render_data: *const c_void, len: usize,
                             cb: apt_block_cb, user: *mut c_void) -> apt_status;
+    pub fn apt_start_render_jpeg(r: *mut apt_renderer, start_render_data: *const c_void, len: usize,
+                                 quality: c_int, cb: apt_block_cb, user: *mut c_void) -> apt_status;
+    pub fn apt_jpeg_compress_blocks(rgbx: *const u8, n_blocks: u32, quality: c_int, out: *mut u8,
+                                    out_cap: usize, offsets: *mut u64, ms_device: *mut f32) -> apt_status;
     pub fn apt_set_frame_index(r: *mut apt_renderer, frame_idx: u32) -> apt_status;
     pub fn apt_get_render_stats(r: *const apt_renderer, out: *mut apt_render_stats) -> apt_status;
 

@@ -439,3 +439,16 @@ def post_probe(o: Oracle, kind: int, imgs, sky=None, svgf_history: int = 4):
     rc = fn(kind, w, h, f, _vp(imgs), _vp(skyb) if skyb is not None else None, svgf_history, _vp(out), _vp(var))
     assert rc == 0
     return out, var
+
+
+def jpeg_block(o: Oracle, rgbx, quality: int) -> bytes:
+    """The oracle's JFIF file of one (64, 64, 4) RGBX block."""
+    a = np.ascontiguousarray(rgbx, np.uint8)
+    assert a.shape == (64, 64, 4)
+    out = np.empty(1 << 17, np.uint8)
+    fn = o.lib.or_jpeg_block
+    fn.argtypes = [_P, C.c_int, _P, C.c_size_t]
+    fn.restype = C.c_size_t
+    n = fn(_vp(a), int(quality), _vp(out), out.size)
+    assert n > 0
+    return out[:n].tobytes()
