@@ -380,6 +380,95 @@ def cpu_baseline_render(scene, width, height, spp, bounces, rr, target_s=3.0):
                       f"of 5 runs ({', '.join(f'{x:.2f}' for x in times)} s)"}
 
 
+def bench_jpeg(pt, W, H, steps, cpu=True):
+    """The node's per-block JPEG compression (turbojpeg::compress(image, 95,
+    Sub2x2) of every 64x64 block, node.rs:65-79) of the rendered frame: the
+    device kernels on the frame's blocks resident in HBM (HIP events around
+    the three launches), the whole node step apt_start_render_jpeg (render,
+    compress, files to host), and on the host the same blocks through
+    libjpeg-turbo (Pillow's build, the library the reference's turbojpeg
+    crate wraps, one thread as the node's callback runs it) and through the
+    oracle port."""
+    import io
+
+    import numpy as np
+
+    from appearance_amd import jpeg, world
+
+    frame = pt.render_rgba((W, H))
+    blocks = np.frombuffer(frame, np.uint8).reshape(-1, 64, 64, 4)
+    n = blocks.shape[0]
+    jpeg.compress_blocks(blocks, 95)  # warm-up: buffers, tables
+    dev = []
+    for _ in range(steps):
+        files, ms = jpeg.compress_blocks(blocks, 95, timing=True)
+        dev.append(ms)
+    dev_ms = sorted(dev)[len(dev) // 2]
+    file_bytes = sum(len(f) for f in files)
+    # a full-chip batch: the frame's blocks repeated to 16384 (a 2^26-pixel slab)
+    big = blocks[np.arange(jpeg.MAX_BLOCKS) % n]
+    jpeg.compress_blocks(big, 95)
+    big_ms = sorted(jpeg.compress_blocks(big, 95, timing=True)[1] for _ in range(5))[2]
+    job = world.StartRenderData(width=W, height=H, row_start=0, row_end=H, frame_idx=0)
+    got = []
+    pt.start_render(job, lambda r, c, f, b: got.append(len(b)), jpeg_quality=95)
+    k = max(3, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        pt.start_render(job, lambda r, c, f, b: None, jpeg_quality=95)
+    node_ms = (time.perf_counter() - t0) * 1e3 / k
+    t0 = time.perf_counter()
+    for _ in range(k):
+        pt.start_render(job, lambda r, c, f, b: None)
+    node_raw_ms = (time.perf_counter() - t0) * 1e3 / k
+    in_bytes = n * 64 * 64 * 4
+    alg = in_bytes + file_bytes
+    out = {
+        "blocks": n, "quality": 95, "bytes_out": file_bytes,
+        "compression_ratio": in_bytes / file_bytes,
+        "device_ms": dev_ms, "mpix_s": n * 4096 / (dev_ms * 1e-3) / 1e6,
+        "device_ms_16384_blocks": big_ms, "mpix_s_16384_blocks": jpeg.MAX_BLOCKS * 4096 / (big_ms * 1e-3) / 1e6,
+        "node_step_ms": {"start_render_jpeg": node_ms, "start_render_rgba": node_raw_ms},
+        "roofline": {"bound": "hbm", "kernel": "k_jpeg_blocks + k_jpeg_offsets + k_jpeg_pack",
+                     "achieved": alg / (dev_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_launch": alg,
+                     "alg_bytes": "16384 B of RGBX read + the file written, per block"},
+    }
+    if cpu:
+        from PIL import Image, features
+
+        from tests._oracle import jpeg_block
+
+        o, flags, model = baseline_oracle()
+        imgs = [Image.fromarray(np.ascontiguousarray(b[..., :3])) for b in blocks]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 3.0:
+            for im in imgs:
+                im.save(io.BytesIO(), "JPEG", quality=95, subsampling=2)
+            reps += 1
+        lt_s = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 3.0:
+            for b in blocks:
+                jpeg_block(o, b, 95)
+            reps += 1
+        port_s = (time.perf_counter() - t0) / reps
+        out["cpu_libjpeg_turbo"] = {
+            "value": n * 4096 / lt_s / 1e6, "unit": "Mpix/s", "cores": 1, "kind": "reference library",
+            "ms_per_frame": lt_s * 1e3,
+            "sample": f"the {n} blocks of the frame, each compressed by libjpeg-turbo "
+                      f"{features.version_feature('libjpeg_turbo')} (Pillow, RGB, q95, 4:2:0, SIMD) as the "
+                      f"node's callback does, one thread, frames repeated for >= 3 s"}
+        out["cpu_baseline"] = {
+            "value": n * 4096 / port_s / 1e6, "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "ms_per_frame": port_s * 1e3, "cpu_model": model, "build_flags": flags,
+            "sample": f"the {n} blocks of the frame through the oracle (scalar C++), one thread, "
+                      f"frames repeated for >= 3 s"}
+    return out
+
+
 def bench_disney(args) -> int:
     import torch
 
@@ -459,6 +548,8 @@ def bench_disney(args) -> int:
                            "achieved": frame_bytes / (dev_ms * 1e-3) / 1e9,
                            "frac": frame_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    log("render-node JPEG blocks")
+    out["node_jpeg"] = bench_jpeg(pt, W, H, steps=10, cpu=not args.no_cpu_baseline)
     if not args.no_bvh_stress:
         r = bench_bvh_stress(args, steps=10)
         out["bvh_stress"] = {"mrays_s_closest": r["mrays_s_closest"], "mrays_s_any": r["mrays_s_any"],

@@ -57,6 +57,24 @@ impl AptRenderer {
         };
         assert_eq!(st, APT_OK, "apt_start_render: {}", last_error());
     }
+
+    /// Node::start_render with the node's compression: as start_render, but
+    /// each block arrives as its JPEG file (`turbojpeg::compress(image,
+    /// quality, Sub2x2)`, node.rs:65-79, compressed on the device), ready to
+    /// be RenderPartialFinishedData::compressed_pixel_bytes.
+    pub fn start_render_jpeg<F: FnMut(u32, u32, u32, &[u8])>(&mut self, start_render_data: &[u8], quality: i32,
+                                                             mut block: F) {
+        extern "C" fn tramp<F: FnMut(u32, u32, u32, &[u8])>(row: u32, col: u32, frame: u32, p: *const u8,
+                                                            n: usize, user: *mut c_void) {
+            let f = unsafe { &mut *(user as *mut F) };
+            f(row, col, frame, unsafe { std::slice::from_raw_parts(p, n) });
+        }
+        let st = unsafe {
+            apt_start_render_jpeg(self.raw, start_render_data.as_ptr().cast(), start_render_data.len(), quality,
+                                  tramp::<F>, (&mut block as *mut F).cast())
+        };
+        assert_eq!(st, APT_OK, "apt_start_render_jpeg: {}", last_error());
+    }
 }
 
 impl Default for AptRenderer {
