@@ -163,6 +163,8 @@ _SIGS = {
     "apt_image_decode_hdr": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                        C.POINTER(C.POINTER(C.c_float))]),
     "apt_image_free": (None, [C.POINTER(C.c_float)]),
+    "apt_image_decode_rgba8": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                         _P, C.c_size_t]),
     "apt_rays_bvh_stress": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "apt_config_default": (None, [C.POINTER(Config)]),
     "apt_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),

@@ -35,6 +35,7 @@ SOURCES = [
     "image.cpp",
     "tlas.hip",
     "jpeg.hip",
+    "jpeg_decode.cpp",
 ]
 
 ARCH = os.environ.get("APT_OFFLOAD_ARCH", "gfx950")

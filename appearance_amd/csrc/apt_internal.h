@@ -195,6 +195,9 @@ apt_status gltf_load(const char* path, apt_scene_data** out);
 // PNG image -> RGBA8 (image.cpp); false with a reason in err.
 bool png_decode(const uint8_t* data, size_t n, uint32_t& w, uint32_t& h, std::vector<uint32_t>& rgba,
                 std::string& err);
+// Baseline JPEG image -> RGBA8 (jpeg_decode.cpp); false with a reason in err.
+bool jpeg_decode(const uint8_t* data, size_t n, uint32_t& w, uint32_t& h, std::vector<uint32_t>& rgba,
+                 std::string& err);
 // Per-vertex tangents of an indexed mesh (Lengyel's method, the reference's
 // generate_tangents, crates/appearance-model/src/mesh.rs:82-160): 4 floats per
 // vertex (unit tangent orthogonalised against the normal, handedness +-1).

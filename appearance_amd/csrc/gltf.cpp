@@ -288,7 +288,8 @@ struct Loader {
     std::vector<uint8_t> bytes;
     apt_scene_texture t;
     std::string e;
-    if (image_bytes(img, bytes) && png_decode(bytes.data(), bytes.size(), t.width, t.height, t.rgba, e)) {
+    if (image_bytes(img, bytes) && (png_decode(bytes.data(), bytes.size(), t.width, t.height, t.rgba, e) ||
+                                    jpeg_decode(bytes.data(), bytes.size(), t.width, t.height, t.rgba, e))) {
       id = (uint32_t)out->textures.size();
       out->textures.push_back(std::move(t));
     }

@@ -1,6 +1,7 @@
 // Image decoding for glTF textures: PNG (zlib inflate, RFC 1950/1951; the
 // five scanline filters; grey, grey + alpha, RGB, RGBA and palette images at
-// 1-16 bits; Adam7 interlacing) into RGBA8; and Radiance HDR (RGBE) sky
+// 1-16 bits; Adam7 interlacing) into RGBA8 (baseline JPEG is in
+// jpeg_decode.cpp); and Radiance HDR (RGBE) sky
 // panoramas into linear RGB floats.  The reference decodes images with the
 // `image` crate and compresses them to BC formats for the GPU
 // (crates/appearance-texture/src/lib.rs:207-342); here texels stay as
@@ -434,3 +435,23 @@ apt_status apt_image_decode_hdr(const uint8_t* bytes, size_t len, uint32_t* widt
 }
 
 void apt_image_free(float* rgb) { std::free(rgb); }
+
+apt_status apt_image_decode_rgba8(const uint8_t* bytes, size_t len, uint32_t* width, uint32_t* height,
+                                  uint8_t* out_rgba, size_t out_len) {
+  clear_error();
+  APT_REQUIRE(bytes && width && height, APT_ERR_INVALID_ARG, "apt_image_decode_rgba8: null argument");
+  std::vector<uint32_t> rgba;
+  std::string err_png, err_jpeg;
+  uint32_t w = 0, h = 0;
+  const bool ok = png_decode(bytes, len, w, h, rgba, err_png) || jpeg_decode(bytes, len, w, h, rgba, err_jpeg);
+  APT_REQUIRE(ok, APT_ERR_INVALID_ARG, "apt_image_decode_rgba8: %s",
+              (len >= 2 && bytes[0] == 0xFF && bytes[1] == 0xD8 ? err_jpeg : err_png).c_str());
+  *width = w;
+  *height = h;
+  if (out_rgba) {
+    APT_REQUIRE(out_len >= rgba.size() * 4, APT_ERR_INVALID_ARG,
+                "apt_image_decode_rgba8: out_rgba holds %zu bytes, needs %zu", out_len, rgba.size() * 4);
+    std::memcpy(out_rgba, rgba.data(), rgba.size() * 4);
+  }
+  return APT_OK;
+}

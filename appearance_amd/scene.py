@@ -76,6 +76,16 @@ def decode_hdr(data: bytes) -> np.ndarray:
         lib().apt_image_free(out)
 
 
+def decode_image(data: bytes) -> np.ndarray:
+    """A PNG or baseline JPEG texture image decoded by libapt: (height, width, 4) uint8 RGBA."""
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().apt_image_decode_rgba8(data, len(data), C.byref(w), C.byref(h), None, 0), "apt_image_decode_rgba8")
+    out = np.empty((h.value, w.value, 4), np.uint8)
+    check(lib().apt_image_decode_rgba8(data, len(data), C.byref(w), C.byref(h), C.c_void_p(out.ctypes.data),
+                                       out.nbytes), "apt_image_decode_rgba8")
+    return out
+
+
 def load_hdr(path) -> np.ndarray:
     """A Radiance .hdr file as a (height, width, 3) float32 sky panorama."""
     with open(path, "rb") as f:

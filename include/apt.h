@@ -509,6 +509,15 @@ apt_status apt_image_decode_hdr(const uint8_t* bytes, size_t len, uint32_t* widt
                                 float** out_rgb);
 void apt_image_free(float* rgb);
 
+/* Texture image decoding, as glTF assets load their images (the reference
+ * decodes them with the image crate, crates/appearance-model/src/asset.rs:382-420):
+ * PNG, or baseline JPEG (8-bit sequential Huffman; the samples of
+ * libjpeg-turbo's default decompression).  width/height receive the size;
+ * out_rgba, when not NULL, receives width * height RGBA8 texels (out_len
+ * bytes at least), row major from the top. */
+apt_status apt_image_decode_rgba8(const uint8_t* bytes, size_t len, uint32_t* width, uint32_t* height,
+                                  uint8_t* out_rgba, size_t out_len);
+
 /* PathTracerGpu::render: renders rows [row_start, row_end) of a width x height
  * frame.  Writes RGBA8 in 64x64 block-major order (resolve.wgsl:56-57,
  * block.wgsl:3-15) to out_rgba (width*(row_end-row_start)*4 bytes; width and

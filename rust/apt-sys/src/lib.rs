@@ -463,6 +463,8 @@ extern "C" {
     pub fn apt_image_decode_hdr(bytes: *const u8, len: usize, width: *mut u32, height: *mut u32,
                                 out_rgb: *mut *mut f32) -> apt_status;
     pub fn apt_image_free(rgb: *mut f32);
+    pub fn apt_image_decode_rgba8(bytes: *const u8, len: usize, width: *mut u32, height: *mut u32,
+                                  out_rgba: *mut u8, out_len: usize) -> apt_status;
     pub fn apt_render(r: *mut apt_renderer, width: u32, height: u32, row_start: u32, row_end: u32,
                       out_rgba: *mut u8, out_len: usize, out_accum_rgb: *mut f32, accum_len: usize)
         -> apt_status;
