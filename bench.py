@@ -21,7 +21,7 @@ rays, closest and any hit) as `bvh_stress`, and the CPU baseline: the FP32
 oracle path tracer on a bounded pixel sample of the same frame.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--workload disney|bvh_stress|restir|serving|bvh_build]
+                  [--workload disney|bvh_stress|restir|serving|bvh_build|jpeg]
                   [--no-cpu-baseline]
 """
 from __future__ import annotations
@@ -45,6 +45,7 @@ DISNEY_COUNTS = ROOT / "tests" / "golden" / "disney_sample_counts.json"
 COPY_BYTES = 256 << 20
 TRAFFIC = ROOT / "profiles" / "trace_closest_bvh_stress_traffic.md"
 EXTEND_TRAFFIC = ROOT / "profiles" / "extend_disney_traffic.md"
+JPEG_TRAFFIC = ROOT / "profiles" / "jpeg_node_blocks_traffic.md"
 
 
 def measured_traffic(path=TRAFFIC, key="traffic_bytes_per_launch"):
@@ -229,7 +230,7 @@ def main() -> int:
                     help="LDS stack entries of the BVH-stress traversal (its incoherent rays into random "
                          "triangles overflow 8 entries; 12 is faster there and all but ends the overflow writes)")
     ap.add_argument("--workload", default="disney",
-                    choices=["disney", "bvh_stress", "restir", "serving", "bvh_build"])
+                    choices=["disney", "bvh_stress", "restir", "serving", "bvh_build", "jpeg"])
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--rates", default="20,200")
     ap.add_argument("--persistent", type=int, default=0,
@@ -257,6 +258,8 @@ def main() -> int:
         return bench_restir(args)
     if args.workload == "bvh_build":
         return bench_bvh_build(args)
+    if args.workload == "jpeg":
+        return bench_node_jpeg(args)
 
     import torch
 
@@ -432,6 +435,7 @@ def bench_jpeg(pt, W, H, steps, cpu=True):
         "roofline": {"bound": "hbm", "kernel": "k_jpeg_blocks + k_jpeg_offsets + k_jpeg_pack",
                      "achieved": alg / (dev_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes_per_launch": alg,
+                     "traffic": measured_traffic(JPEG_TRAFFIC, "traffic_bytes_per_frame"),
                      "alg_bytes": "16384 B of RGBX read + the file written, per block"},
     }
     if cpu:
@@ -467,6 +471,40 @@ def bench_jpeg(pt, W, H, steps, cpu=True):
             "sample": f"the {n} blocks of the frame through the oracle (scalar C++), one thread, "
                       f"frames repeated for >= 3 s"}
     return out
+
+
+def bench_node_jpeg(args) -> int:
+    """The render node's JPEG step alone (--workload jpeg): the disney frame
+    at 1920x1088 rendered once, then its 510 blocks compressed on the device
+    `--steps` times (the bench_jpeg leg)."""
+    import torch
+
+    from appearance_amd.renderer import PathTracerGpu, plain_config
+    from appearance_amd.scene import generate_scene
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    W, H = 1920, 1088
+    log("generating disney scene")
+    scene = generate_scene("disney", seed=3)
+    pt = PathTracerGpu(plain_config(sample_count=4, max_bounces=8, rr_start_bounce=3))
+    pt.set_scene(scene)
+    pt.set_camera(scene.camera)
+    if args.calibrate_copy:
+        calibrate_copy()
+    t0 = time.perf_counter()
+    r = bench_jpeg(pt, W, H, steps=args.steps, cpu=not args.no_cpu_baseline)
+    wall = time.perf_counter() - t0
+    out = {
+        "metric": "render-node JPEG blocks: Mpix/s compressed on the device", "value": r["mpix_s"],
+        "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": r["device_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"the disney frame {W}x{H} (4 spp, depth 8) as 510 64x64 RGBX blocks, "
+                               f"JPEG q95 4:2:0 per block", "blocks": r["blocks"], "quality": 95},
+        "wall_s": wall, **{k: v for k, v in r.items() if k not in ("mpix_s",)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def bench_disney(args) -> int:
