@@ -273,13 +273,24 @@ def _png(img, ctype=2):
     return (png_util.encode(np.asarray(img), ctype, 8), "image/png")
 
 
-def textured_room(path, seed=0):
+def _jpg(img, quality=90):
+    """A baseline 4:2:0 JPEG of an RGB image (Pillow's libjpeg-turbo)."""
+    import io
+
+    from PIL import Image
+    b = io.BytesIO()
+    Image.fromarray(np.ascontiguousarray(np.asarray(img, np.uint8))).save(b, "JPEG", quality=quality, subsampling=2)
+    return (b.getvalue(), "image/jpeg")
+
+
+def textured_room(path, seed=0, jpeg=False):
     """A closed-ish room (no front wall) of textured materials, written as GLB:
     every material texture slot in use, uvs beyond [0, 1] (repeat addressing),
     non-power-of-two images, tangents generated by the reader, a normal map
     shared with a clearcoat normal map, an emission-textured ceiling light, a
     transmission-textured glass box and a textured metal box.  Lit by the
-    light only (no sky)."""
+    light only (no sky).  With jpeg, the RGB images (all but the colour
+    texture, whose alpha is used) are stored as JPEG instead of PNG."""
     rng = np.random.default_rng(seed)
     yy, xx = np.mgrid[0:10, 0:12]
     checker = ((xx // 3 + yy // 2) % 2)[..., None]
@@ -300,9 +311,10 @@ def textured_room(path, seed=0):
     grey = lambda lo, hi, s: np.repeat(rng.integers(lo, hi, (s, s))[..., None], 3, -1)
     sheen_tint = rng.integers(60, 256, (6, 6, 3))
     trans = np.repeat((((np.mgrid[0:4, 0:4].sum(0)) % 2) * 255)[..., None], 3, -1)
-    images = [_png(color, 6), _png(mr), _png(normal_map(1, 2, 0.6)), _png(stripes), _png(grey(80, 256, 8)),
-              _png(grey(0, 256, 8)), _png(normal_map(3, 1, 0.9)), _png(grey(50, 256, 5)), _png(sheen_tint),
-              _png(trans)]
+    rgb = _jpg if jpeg else _png
+    images = [_png(color, 6), rgb(mr), rgb(normal_map(1, 2, 0.6)), rgb(stripes), rgb(grey(80, 256, 8)),
+              rgb(grey(0, 256, 8)), rgb(normal_map(3, 1, 0.9)), rgb(grey(50, 256, 5)), rgb(sheen_tint),
+              rgb(trans)]
     W, H, D = 2.0, 3.0, 2.0
     prims = [
         dict(_quad((-W, 0, D), (W, 0, D), (W, 0, -D), (-W, 0, -D), (0, 1, 0),

@@ -86,24 +86,29 @@ def test_errors(jpeg):
     assert offs[1] == len(jpeg.compress(b[0])) > out.size
 
 
-def test_start_render_jpeg_sends_the_blocks_compressed(oracle, gpu_available):
+@pytest.mark.parametrize("integrator", ["plain", "restir"])
+def test_start_render_jpeg_sends_the_blocks_compressed(oracle, gpu_available, integrator):
     """apt_start_render_jpeg's callbacks carry the oracle's JPEG of each
     RGBA block apt_start_render hands over for the same frame, in the same
-    order with the same row, column and frame fields."""
+    order with the same row, column and frame fields (the plain integrator,
+    and the default ReSTIR one with its post chain)."""
     from appearance_amd import world
-    from appearance_amd.renderer import PathTracerGpu, plain_config
+    from appearance_amd.renderer import PathTracerGpu, default_config, plain_config
     from appearance_amd.scene import generate_scene
     s = generate_scene("cornell")
-    pt = PathTracerGpu(plain_config(sample_count=2, max_bounces=3))
-    pt.set_scene(s)
-    pt.set_camera(s.camera)
+    cfg = plain_config(sample_count=2, max_bounces=3) if integrator == "plain" else default_config()
     job = world.StartRenderData(width=256, height=256, row_start=64, row_end=192, frame_idx=7)
     raw, comp = [], []
-    pt.set_frame_index(3)
-    pt.start_render(job, lambda r, c, f, px: raw.append((r, c, f, px)))
-    pt.set_frame_index(3)
-    print("start_render_jpeg 256x128 slab", flush=True)
-    pt.start_render(job, lambda r, c, f, px: comp.append((r, c, f, px)), jpeg_quality=95)
+    # two fresh renderers: the same first frame (ReSTIR keeps history across frames)
+    renderers = []
+    for out, q in ((raw, None), (comp, 95)):
+        pt = PathTracerGpu(cfg)
+        pt.set_scene(s)
+        pt.set_camera(s.camera)
+        pt.set_frame_index(3)
+        print(f"start_render{'_jpeg' if q else ''} 256x128 slab, {integrator}", flush=True)
+        pt.start_render(job, lambda r, c, f, px, out=out: out.append((r, c, f, px)), jpeg_quality=q)
+        renderers.append(pt)
     assert [x[:3] for x in raw] == [x[:3] for x in comp]
     assert len(comp) == 8
     for (r, c, f, px), (_, _, _, jp) in zip(raw, comp):
@@ -116,5 +121,6 @@ def test_start_render_jpeg_sends_the_blocks_compressed(oracle, gpu_available):
     # a slab that is not whole blocks is refused like apt_start_render's
     bad = world.StartRenderData(width=250, height=256, row_start=0, row_end=64, frame_idx=0)
     with pytest.raises(Exception):
-        pt.start_render(bad, lambda *a: None, jpeg_quality=95)
-    pt.close()
+        renderers[1].start_render(bad, lambda *a: None, jpeg_quality=95)
+    for pt in renderers:
+        pt.close()

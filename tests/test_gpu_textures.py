@@ -58,6 +58,25 @@ def test_textured_room_plain_vs_oracle(gpu_available, oracle, room):
     assert r < 1e-4
 
 
+def test_textured_room_jpeg_images_plain_vs_oracle(gpu_available, oracle, tmp_path):
+    """The same room with its RGB images stored as JPEG (decoded by libapt's
+    baseline decoder as the glTF is read) renders like the oracle on the
+    decoded texels."""
+    path = tmp_path / "room_jpeg.glb"
+    textured_room(path, jpeg=True)
+    s = load_gltf(path)
+    assert len(s.textures) == 10
+    pt = PathTracerGpu(plain_config(sample_count=16, max_bounces=5))
+    pt.set_scene(s)
+    pt.set_camera(_cam())
+    _, got = pt.render_accum((160, 120), with_rgba=False)
+    ref, _ = oracle_render(oracle, s, 160, 120, spp=16, max_bounces=5, rr_start=3, camera=_cam())
+    r = relmse(got, ref)
+    print(f"[textures] room with JPEG images, plain integrator: relMSE {r:.3e}", flush=True)
+    assert np.isfinite(got).all() and got.mean() > 0.05
+    assert r < 1e-4
+
+
 def test_textured_room_restir_vs_oracle(gpu_available, oracle, room):
     _, s = room
     cfg = default_config()
