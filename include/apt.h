@@ -583,7 +583,9 @@ apt_status apt_start_render_jpeg(apt_renderer* r, const void* start_render_data,
  * n_blocks + 1 entries.  out_cap below the total fails with
  * APT_ERR_INVALID_ARG and offsets[n_blocks] set to the bytes needed
  * (n_blocks * APT_JPEG_FILE_MAX always suffices).  ms_device, when not NULL,
- * receives the device time of the compression kernels. */
+ * receives the device time of the compression kernels.  Each call allocates
+ * and frees its device buffers (about 100 KB per block) and runs on the
+ * library stream; a renderer's apt_start_render_jpeg keeps its own. */
 apt_status apt_jpeg_compress_blocks(const uint8_t* rgbx, uint32_t n_blocks, int quality, uint8_t* out,
                                     size_t out_cap, uint64_t* offsets, float* ms_device);
 
