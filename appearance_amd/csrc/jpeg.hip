@@ -5,7 +5,7 @@
 // crates/appearance-render-loop/src/node.rs:65-79; ENABLE_COMPRESSION and
 // NODE_PIXEL_FORMAT = RGBX, host.rs:21-22), and the host decompresses it
 // (host.rs:341-350).  Here the frame is already in HBM in block-major order,
-// so one 256-thread work-group encodes one block, and the bytes are those of
+// so one 1024-thread work-group encodes one block, and the bytes are those of
 // libjpeg-turbo's default compression (the library the turbojpeg crate
 // wraps): integer JFIF colour conversion, 2x2 box downsampling with
 // alternating bias, the accurate integer DCT (Loeffler-Ligtenberg-Moschytz),
@@ -20,7 +20,7 @@
 //      run lengths come from a ballot of the nonzero lanes
 //   5. scan-order offsets of the 96 blocks, then every lane ORs its codes
 //      into the bit buffer at its own offset
-//   6. byte stuffing (0xFF -> 0xFF 0x00) by 256 chunks with a prefix sum
+//   6. byte stuffing (0xFF -> 0xFF 0x00) by 1024 chunks with a prefix sum
 // A second pass packs header + segment of every block back to back.
 #include <hip/hip_runtime.h>
 
@@ -34,7 +34,11 @@ namespace apt {
 
 namespace {
 
-constexpr uint32_t JB = 256;           // threads per 64x64 block
+// Threads per 64x64 block: 16 waves share the 96 8x8 blocks of the Huffman
+// phases (6 each); one frame's 510 blocks took 71 / 51 / 46 us at 256 / 512 /
+// 1024 threads (a 2^26-pixel slab 1.10 / 0.89 / 0.99 ms).
+constexpr uint32_t JB = 1024;
+constexpr uint32_t JP = 256;  // threads of the offset scan and of a block's copy
 constexpr uint32_t NB8 = 96;           // 8x8 blocks: 64 Y, 16 Cb, 16 Cr
 constexpr uint32_t PIX_WORDS = 6144;   // Y | Cb | Cr samples; then the bit buffer (<= 4980 words)
 
@@ -139,7 +143,8 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
   return v;
 }
 
-// Exclusive prefix sum over the work-group's 256 threads; `total` gets the sum.
+// Exclusive prefix sum over the work-group's T threads; `total` gets the sum.
+template <uint32_t T>
 __device__ __forceinline__ uint32_t group_scan_excl(uint32_t v, uint32_t* s_wsum, uint32_t& total) {
   const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
   const uint32_t incl = wave_scan_incl(v);
@@ -148,7 +153,7 @@ __device__ __forceinline__ uint32_t group_scan_excl(uint32_t v, uint32_t* s_wsum
   uint32_t base = 0;
   total = 0;
 #pragma unroll
-  for (uint32_t w = 0; w < JB / 64; ++w) {
+  for (uint32_t w = 0; w < T / 64; ++w) {
     const uint32_t x = s_wsum[w];
     base += w < wave ? x : 0u;
     total += x;
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(JB) void k_jpeg_blocks(const uint32_t* __restrict__
   uint32_t ff = 0;
   for (uint32_t j = b0; j < b1; ++j) ff += byte_at(j) == 0xFFu ? 1u : 0u;
   uint32_t ff_total;
-  const uint32_t ff_before = group_scan_excl(ff, s_wsum, ff_total);
+  const uint32_t ff_before = group_scan_excl<JB>(ff, s_wsum, ff_total);
   const uint32_t len = nbytes + ff_total + 2u;  // stuffed bytes, then EOI
   uint8_t* dst = seg + (size_t)blockIdx.x * JPEG_SEG_CAP;
   uint32_t o = b0 + ff_before;
@@ -418,15 +423,15 @@ __global__ __launch_bounds__(JB) void k_jpeg_blocks(const uint32_t* __restrict__
 
 // File offsets: block b's file (header + segment) starts at
 // b * hdr_len + the segments before it; off[n] is the total.
-__global__ __launch_bounds__(JB) void k_jpeg_offsets(const uint32_t* __restrict__ seg_len, uint32_t n,
+__global__ __launch_bounds__(JP) void k_jpeg_offsets(const uint32_t* __restrict__ seg_len, uint32_t n,
                                                     uint32_t hdr_len, uint32_t* __restrict__ off) {
-  __shared__ uint32_t s_wsum[JB / 64];
-  const uint32_t t = threadIdx.x, chunk = (n + JB - 1) / JB;
+  __shared__ uint32_t s_wsum[JP / 64];
+  const uint32_t t = threadIdx.x, chunk = (n + JP - 1) / JP;
   const uint32_t a = min(n, t * chunk), b = min(n, a + chunk);
   uint32_t sum = 0;
   for (uint32_t i = a; i < b; ++i) sum += seg_len[i] + hdr_len;
   uint32_t total;
-  uint32_t o = group_scan_excl(sum, s_wsum, total);
+  uint32_t o = group_scan_excl<JP>(sum, s_wsum, total);
   for (uint32_t i = a; i < b; ++i) {
     off[i] = o;
     o += seg_len[i] + hdr_len;
@@ -434,14 +439,14 @@ __global__ __launch_bounds__(JB) void k_jpeg_offsets(const uint32_t* __restrict_
   if (t == 0) off[n] = total;
 }
 
-__global__ __launch_bounds__(JB) void k_jpeg_pack(const uint8_t* __restrict__ seg, const uint32_t* __restrict__ seg_len,
+__global__ __launch_bounds__(JP) void k_jpeg_pack(const uint8_t* __restrict__ seg, const uint32_t* __restrict__ seg_len,
                                                  const uint32_t* __restrict__ off, const uint8_t* __restrict__ hdr,
                                                  uint32_t hdr_len, uint8_t* __restrict__ files) {
   const uint32_t b = blockIdx.x, n = seg_len[b];
   uint8_t* dst = files + off[b];
   const uint8_t* src = seg + (size_t)b * JPEG_SEG_CAP;
-  for (uint32_t i = threadIdx.x; i < hdr_len; i += JB) dst[i] = hdr[i];
-  for (uint32_t i = threadIdx.x; i < n; i += JB) dst[hdr_len + i] = src[i];
+  for (uint32_t i = threadIdx.x; i < hdr_len; i += JP) dst[i] = hdr[i];
+  for (uint32_t i = threadIdx.x; i < n; i += JP) dst[hdr_len + i] = src[i];
 }
 
 }  // namespace
@@ -508,8 +513,8 @@ apt_status JpegEncoder::encode(const uint32_t* d_rgbx, uint32_t n, int q, hipStr
   APT_HIP(hipEventRecord(ev0, s));
   hipLaunchKernelGGL(k_jpeg_blocks, dim3(n), dim3(JB), 0, s, d_rgbx, static_cast<const uint32_t*>(d_tab), d_seg,
                      d_len);
-  hipLaunchKernelGGL(k_jpeg_offsets, dim3(1), dim3(JB), 0, s, d_len, n, hdr_len, d_off);
-  hipLaunchKernelGGL(k_jpeg_pack, dim3(n), dim3(JB), 0, s, d_seg, d_len, d_off, d_hdr, hdr_len, d_files);
+  hipLaunchKernelGGL(k_jpeg_offsets, dim3(1), dim3(JP), 0, s, d_len, n, hdr_len, d_off);
+  hipLaunchKernelGGL(k_jpeg_pack, dim3(n), dim3(JP), 0, s, d_seg, d_len, d_off, d_hdr, hdr_len, d_files);
   APT_HIP(hipGetLastError());
   APT_HIP(hipEventRecord(ev1, s));
   offsets.resize((size_t)n + 1);
