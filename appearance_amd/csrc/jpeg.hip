@@ -19,7 +19,7 @@
 //   4. Huffman bit counts: a wave per 8x8 block, a lane per coefficient; the
 //      run lengths come from a ballot of the nonzero lanes
 //   5. scan-order offsets of the 96 blocks, then every lane ORs its codes
-//      into the bit buffer at its own offset
+//      into the bit buffer at its own offset (kept from step 4)
 //   6. byte stuffing (0xFF -> 0xFF 0x00) by 1024 chunks with a prefix sum
 // A second pass packs header + segment of every block back to back.
 #include <hip/hip_runtime.h>
@@ -291,6 +291,7 @@ __global__ __launch_bounds__(JB) void k_jpeg_blocks(const uint32_t* __restrict__
   __shared__ int32_t s_pix[PIX_WORDS];  // Y 64x64 | Cb 32x32 | Cr 32x32; later the bit buffer
   __shared__ int16_t s_coef[NB8 * 64];  // quantised, zig-zag order, storage block order
   __shared__ uint32_t s_tab[TAB_WORDS];
+  __shared__ uint16_t s_lpos[NB8 * 64];  // each lane's bit offset in its 8x8 block's code (scan order)
   __shared__ uint32_t s_off[NB8];
   __shared__ uint32_t s_wsum[JB / 64];
   __shared__ uint32_t s_total;
@@ -363,8 +364,10 @@ __global__ __launch_bounds__(JB) void k_jpeg_blocks(const uint32_t* __restrict__
     const int c = s_coef[sb * 64 + lane];
     const int pred_dc = pb < 0 ? 0 : (int)s_coef[pb * 64];
     const uint64_t nz = __ballot(lane > 0 && c != 0);
-    const uint32_t bits = wave_scan_incl(code_lane<false>(c, pred_dc, nz, lane, T.dc[comp], T.ac[comp], nullptr, 0));
-    if (lane == 63) s_off[k] = bits;
+    const uint32_t bits = code_lane<false>(c, pred_dc, nz, lane, T.dc[comp], T.ac[comp], nullptr, 0);
+    const uint32_t incl = wave_scan_incl(bits);
+    s_lpos[k * 64 + lane] = (uint16_t)(incl - bits);  // < 1660 bits per 8x8 block
+    if (lane == 63) s_off[k] = incl;
   }
   for (uint32_t i = tid; i < PIX_WORDS; i += JB) words[i] = 0u;
   __syncthreads();
@@ -387,9 +390,7 @@ __global__ __launch_bounds__(JB) void k_jpeg_blocks(const uint32_t* __restrict__
     const int c = s_coef[sb * 64 + lane];
     const int pred_dc = pb < 0 ? 0 : (int)s_coef[pb * 64];
     const uint64_t nz = __ballot(lane > 0 && c != 0);
-    const uint32_t bits = code_lane<false>(c, pred_dc, nz, lane, T.dc[comp], T.ac[comp], nullptr, 0);
-    const uint32_t pos = s_off[k] + wave_scan_incl(bits) - bits;
-    code_lane<true>(c, pred_dc, nz, lane, T.dc[comp], T.ac[comp], words, pos);
+    code_lane<true>(c, pred_dc, nz, lane, T.dc[comp], T.ac[comp], words, s_off[k] + s_lpos[k * 64 + lane]);
   }
   if (tid == 0 && (total_bits & 7u)) {  // pad the last byte with 1 bits
     uint32_t pos = total_bits;
