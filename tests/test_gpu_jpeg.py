@@ -35,6 +35,18 @@ def test_sample_blocks_bit_exact(oracle, jpeg, quality):
         assert f == jpeg_block(oracle, blocks[n], quality), (n, quality)
 
 
+def test_every_quality_bit_exact(oracle, jpeg):
+    """Qualities 1..100 (each scales the Annex K tables differently, with
+    the 8-bit clamp at the low end): three blocks per quality in one call."""
+    blocks = sample_blocks()
+    picks = np.stack([blocks["smooth_noise"], blocks["edge"], blocks["noise"]])
+    print("jpeg qualities 1..100", flush=True)
+    for q in range(1, 101):
+        got = jpeg.compress_blocks(picks, q)
+        for i in range(len(picks)):
+            assert got[i] == jpeg_block(oracle, picks[i], q), (q, i)
+
+
 def test_single_block_api(oracle, jpeg):
     b = sample_blocks()["edge"]
     assert jpeg.compress(b) == jpeg_block(oracle, b, 95)
