@@ -111,7 +111,7 @@ class Config(C.Structure):
         "restir_di", "restir_gi", "svgf", "firefly_filter", "taa", "di_candidates", "di_mcap",
         "di_spatial_neighbours")] + [("di_spatial_radius", C.c_float)] + [
         (n, C.c_uint32) for n in ("gi_mcap", "svgf_iterations", "svgf_history", "bvh_width",
-                                  "queue_chunk", "lds_stack_depth", "seed", "frame_idx")]
+                                  "queue_chunk", "lds_stack_depth", "seed", "frame_idx", "gi_spatial_passes")]
 
 
 class RenderStats(C.Structure):

@@ -188,7 +188,7 @@ struct RestirState {
   DevBuf<float4> rq_o, rq_d, rq_hit;
   DevBuf<uint32_t> rng, count, counters, rq_ctr;
   DevBuf<DiRes> di, di_tmp, di_hist[2];
-  DevBuf<GiRes> gi, gi_hist[2];
+  DevBuf<GiRes> gi, gi_tmp, gi_hist[2];
   DevBuf<GTexel> gbuf[2];
   DevBuf<float2> mom[2];
   DevBuf<uint2> guide;
@@ -199,7 +199,7 @@ struct RestirState {
   void each(F&& f) {
     f(ray_o); f(ray_d); f(thr); f(radiance); f(vtx_p); f(vtx_n); f(vtx_t); f(vtx_m); f(demod[0]); f(demod[1]);
     f(work); f(col[0]); f(col[1]); f(hist); f(rng); f(count); f(di); f(di_tmp); f(di_hist[0]);
-    f(di_hist[1]); f(gi); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
+    f(di_hist[1]); f(gi); f(gi_tmp); f(gi_hist[0]); f(gi_hist[1]); f(gbuf[0]); f(gbuf[1]); f(mom[0]); f(mom[1]);
     f(guide); f(di_self); f(rq_o); f(rq_d); f(rq_hit);
   }
   // Sizes every buffer for n pixels and clears them (no history).
@@ -229,7 +229,7 @@ struct RestirState {
     b.vtx_p = vtx_p.p; b.vtx_n = vtx_n.p; b.vtx_t = vtx_t.p; b.vtx_m = vtx_m.p;
     b.di = di.p; b.di_tmp = di_tmp.p; b.di_hist[0] = di_hist[0].p; b.di_hist[1] = di_hist[1].p;
     b.di_self = di_self.p;
-    b.gi = gi.p; b.gi_hist[0] = gi_hist[0].p; b.gi_hist[1] = gi_hist[1].p;
+    b.gi = gi.p; b.gi_tmp = gi_tmp.p; b.gi_hist[0] = gi_hist[0].p; b.gi_hist[1] = gi_hist[1].p;
     b.gbuf[0] = gbuf[0].p; b.gbuf[1] = gbuf[1].p;
     b.demod[0] = demod[0].p; b.demod[1] = demod[1].p; b.work = work.p;
     b.svgf_col[0] = col[0].p; b.svgf_col[1] = col[1].p; b.svgf_mom[0] = mom[0].p; b.svgf_mom[1] = mom[1].p;
@@ -360,6 +360,8 @@ apt_status validate_config(const apt_config& c) {
               "config: di_spatial_radius %g outside [0, 4096]", (double)c.di_spatial_radius);
   APT_REQUIRE(c.svgf_iterations <= 8, APT_ERR_INVALID_ARG, "config: svgf_iterations %u above 8", c.svgf_iterations);
   APT_REQUIRE(c.svgf_history >= 1, APT_ERR_INVALID_ARG, "config: svgf_history must be at least 1");
+  APT_REQUIRE(c.gi_spatial_passes <= 4, APT_ERR_INVALID_ARG, "config: gi_spatial_passes %u above 4",
+              c.gi_spatial_passes);
   return APT_OK;
 }
 
@@ -586,6 +588,7 @@ apt_status render_restir(apt_renderer* r, uint32_t width, uint32_t height, uint3
   p.neighbours = c.di_spatial_neighbours;
   p.radius = c.di_spatial_radius;
   p.gi_mcap = c.gi_mcap;
+  p.gi_spatial_passes = c.gi_spatial_passes;
   p.restir_di = c.restir_di ? 1u : 0u;
   p.restir_gi = c.restir_gi ? 1u : 0u;
   p.cur = r->rs_cur;
@@ -790,6 +793,7 @@ void apt_config_default(apt_config* c) {
   c->lds_stack_depth = 8;
   c->seed = 0;
   c->frame_idx = 0;
+  c->gi_spatial_passes = 0;  // spatial_pass_count: 0 (lib.rs:441)
 }
 
 apt_status apt_create(const apt_config* cfg, apt_renderer** out) {

@@ -54,6 +54,7 @@ struct RestirParams {
   uint32_t neighbours;                      // spatial taps
   float radius;                             // spatial radius in pixels
   uint32_t gi_mcap;                         // temporal M cap (absolute)
+  uint32_t gi_spatial_passes;               // GI spatial reuse passes (0: none)
   uint32_t restir_di, restir_gi;            // spatiotemporal reuse switches
   uint32_t cur;                             // parity of the frame: history buffers [cur] are written, [cur ^ 1] read
   uint32_t svgf_history;                    // max frames of SVGF moment history
@@ -82,6 +83,7 @@ struct RestirBuffers {
   float* di_self;       // target of each pixel's temporal DI sample at the pixel itself, before
                         // its shadow ray (queued in rq_*, traced before the spatial pass)
   GiRes* gi;
+  GiRes* gi_tmp;   // the GI spatial passes' other buffer
   GiRes* gi_hist[2];
   GTexel* gbuf[2];
   // post chain

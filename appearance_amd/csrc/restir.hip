@@ -999,7 +999,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
   r.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   r.mwsp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (!load_vtx(sc, b, pix, v)) {
-    b.gi_hist[p.cur][pix] = r;
+    if (p.gi_spatial_passes == 0) b.gi_hist[p.cur][pix] = r;
     return;
   }
   r = b.gi[pix];
@@ -1040,7 +1040,101 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_temporal(RestirParams p, Scene
     b.rng[pix] = rng;
   }
   b.gi[pix] = r;
-  b.gi_hist[p.cur][pix] = r;
+  if (p.gi_spatial_passes == 0) b.gi_hist[p.cur][pix] = r;  // else the last spatial pass writes it
+}
+
+// restir_gi_spatial.wgsl:70-165: pass `pass` of p.gi_spatial_passes, the
+// unbiased combination over five taps (NUM_SAMPLES); the radius is 5 % of
+// (width + rows) / 2 whatever the configured pixel radius, the seeds mix
+// seed * 3 + pass, and the angle hash uses pixel / 4 on the first pass,
+// pixel / 2 after.  Reads `in`, writes `out`; the last pass writes the
+// history.  A tap's targets are approximate one-bounce paths (gi_phat); Z
+// evaluates the chosen sample's at both domains only when W could be
+// nonzero (a sample chosen, a positive weight sum).
+template <int W, int S, bool INST, bool TEX>
+__global__ __launch_bounds__(BLOCK8) void k_rs_gi_spatial(RestirParams p, SceneView sc_, RestirBuffers b, uint32_t pass,
+                                                         const GiRes* __restrict__ in, GiRes* __restrict__ out) {
+  const SceneView sc = scene_view<TEX>(sc_);
+  __shared__ uint2 s_stack[S * BLOCK8];
+  uint32_t x, y, pix;
+  if (!tile_pixel<TILE8>(p, x, y, pix)) return;
+  const bool last = pass + 1 == p.gi_spatial_passes;
+  GiRes r = in[pix];
+  Vtx v;
+  if (!load_vtx(sc, b, pix, v)) {
+    out[pix] = r;
+    if (last) {
+      GiRes e;
+      e.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      e.mwsp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      b.gi_hist[p.cur][pix] = e;
+    }
+    return;
+  }
+  const float4 t4 = b.thr[pix];
+  if (t4.w > 0.0f) {
+    const LdsStack st{s_stack + threadIdx.x, BLOCK8};
+    uint32_t rng = b.rng[pix];
+    const v3 T = f4v(t4);
+    const Tex ct = unpack_texel(b.gbuf[p.cur][pix]);
+    const float cd = depth_cs(ct.depth);
+    float radius = (float)(p.width + p.rows) / 2.0f * 0.05f;
+    const uint32_t pass_seed = p.seed * 3u + pass;
+    const float r_off = ign_animated(x, y, pass_seed);
+    const uint32_t sh = pass == 0 ? 2u : 1u;
+    const uint32_t aseed = hash_combine(x >> sh, hash_combine(y >> sh, pass_seed));
+    const float a_off = (float)aseed * (1.0f / (float)0xFFFFFFFFu) * TWO_PI;
+    constexpr uint32_t n = 5;
+    for (uint32_t i = 0; i < n; ++i) {
+      const float ang = (float)i * GOLDEN_ANGLE + a_off;
+      const float rad = powf((float)i / (float)n, 0.5f) * radius + r_off;
+      const int ox = (int)(rad * cosf(ang)), oy = (int)(rad * sinf(ang));
+      const uint32_t nx = mirror_px((int)x + ox, (int)p.width), ny = mirror_px((int)y + oy, (int)p.rows);
+      const uint32_t q = ny * p.width + nx;
+      if (q == pix) continue;
+      const GTexel ng = b.gbuf[p.cur][q];
+      bool valid = ng.pos_depth.w != 0.0f;
+      if (valid) {
+        const Tex nt = unpack_texel(ng);
+        valid = fabsf(cd - depth_cs(nt.depth)) / cd < 0.1f && dot(ct.n, nt.n) > 0.906f;
+      }
+      if (valid) {
+        GiRes nr = in[q];
+        nr.mwsp.w = gi_phat<W, S, INST>(sc, st, v, make_bsdf(sc, v), T, nr, b.counters);
+        valid = nr.mwsp.w > 0.0f;
+        // GiReservoir::combine_unbiased (inline_path_tracer.wgsl:333-355)
+        float wsum = 0.0f, M = 0.0f, phat = 0.0f;
+        GiRes c;
+        c.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (res_update(wsum, M, r.mwsp.w * r.mwsp.y * r.mwsp.x, rng)) {
+          c.x2_rng = r.x2_rng;
+          phat = r.mwsp.w;
+        }
+        if (res_update(wsum, M, nr.mwsp.w * nr.mwsp.y * nr.mwsp.x, rng)) {
+          c.x2_rng = nr.x2_rng;
+          phat = nr.mwsp.w;
+        }
+        M = r.mwsp.x + nr.mwsp.x;
+        float z = 0.0f;
+        if (phat > 0.0f && wsum > 0.0f) {
+          if (gi_phat<W, S, INST>(sc, st, v, make_bsdf(sc, v), T, c, b.counters) > 0.0f) z += r.mwsp.x;
+          Vtx vn;
+          if (load_vtx(sc, b, q, vn)) {
+            vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position
+            if (gi_phat<W, S, INST>(sc, st, vn, make_bsdf(sc, vn), f4v(b.thr[q]), c, b.counters) > 0.0f)
+              z += nr.mwsp.x;
+          }
+        }
+        const float Wc = (phat > 0.0f && z * wsum > 0.0f) ? (1.0f / phat) * (1.0f / z * wsum) : 0.0f;
+        c.mwsp = make_float4(M, Wc, wsum, phat);
+        r = c;
+      }
+      if (!valid) radius = fmaxf(radius * 0.5f, 3.0f);
+    }
+    b.rng[pix] = rng;
+  }
+  out[pix] = r;
+  if (last) b.gi_hist[p.cur][pix] = r;
 }
 
 // apply_di.wgsl:48-111: the reservoir's sample, visibility tested, weighted by
@@ -1376,8 +1470,16 @@ apt_status sample_t(const RestirParams& p, const SceneView& sc, const RestirBuff
         hipLaunchKernelGGL((k_rs_di_spatial<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
       }
     }
-    if (bounce == 0 && more && p.restir_gi)
+    if (bounce == 0 && more && p.restir_gi) {
       hipLaunchKernelGGL((k_rs_gi_temporal<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
+      for (uint32_t pass = 0; pass < p.gi_spatial_passes; ++pass) {
+        const GiRes* in = pass % 2 == 0 ? b.gi : b.gi_tmp;
+        GiRes* out = pass % 2 == 0 ? b.gi_tmp : b.gi;
+        hipLaunchKernelGGL((k_rs_gi_spatial<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b, pass, in, out);
+      }
+      if (p.gi_spatial_passes % 2 == 1)
+        APT_HIP(hipMemcpyAsync(b.gi, b.gi_tmp, (size_t)npix * sizeof(GiRes), hipMemcpyDeviceToDevice, s));
+    }
     hipLaunchKernelGGL((k_rs_apply_di<W, S, INST, TEX>), grid8, block8, 0, s, p, sc, b);
     if (more) hipLaunchKernelGGL(k_rs_apply_gi<TEX>, grid, block, 0, s, p, sc, b);
   }

@@ -449,6 +449,8 @@ typedef struct apt_config {
   uint32_t lds_stack_depth;   /* traversal stack entries kept in LDS: 8, 12, 16 */
   uint32_t seed;              /* base seed mixed into every path seed */
   uint32_t frame_idx;         /* starting frame index */
+  uint32_t gi_spatial_passes; /* RESTIR: ReSTIR GI spatial reuse passes (restir_gi_spatial.wgsl), <= 4;
+                                 0 as the reference ships it (lib.rs:441) */
 } apt_config;
 
 /* PathTracerGpuConfig::default (src/lib.rs:230-245): the RESTIR integrator,

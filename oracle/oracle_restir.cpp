@@ -150,6 +150,7 @@ struct P {
   uint32_t di_candidates, di_mcap, neighbours, gi_mcap, restir_di, restir_gi;
   uint32_t firefly, svgf, taa, svgf_history, atrous;
   F radius;
+  uint32_t gi_spatial;  // GI spatial reuse passes (0: none, as the reference ships)
   uint32_t seed, cur, first_frame;
   CamV cam, prev_cam;
 };
@@ -190,7 +191,7 @@ struct State {
   std::vector<uint32_t> rng;
   std::vector<Vtx> vtx;
   std::vector<DiRes> di, di_tmp, di_hist[2];
-  std::vector<GiRes> gi, gi_hist[2];
+  std::vector<GiRes> gi, gi_tmp, gi_hist[2];
   std::vector<Tex> gbuf[2];
   std::vector<V3> demod[2], work, col[2], hist;
   std::vector<F> var[2];
@@ -206,6 +207,7 @@ struct State {
     alive.assign(n, 0); rng.assign(n, 0); vtx.assign(n, Vtx());
     di.assign(n, DiRes()); di_tmp.assign(n, DiRes());
     gi.assign(n, GiRes());
+    gi_tmp.assign(n, GiRes());
     for (int k = 0; k < 2; ++k) {
       di_hist[k].assign(n, DiRes());
       gi_hist[k].assign(n, GiRes());
@@ -412,7 +414,8 @@ enum Plant : int {
   PLANT_Z_SWAP = 5,       // combine_unbiased: each domain's test adds the other's M
   PLANT_NO_JACOBIAN = 6,  // DI reuse: a reused weight keeps the other point's measure
   PLANT_INST_TRANSPOSE = 7,   // instances: the world-to-object 3x3 transposed (oracle_core.cpp)
-  PLANT_INST_TRANSLATION = 8  // instances: the world-to-object translation negated (oracle_core.cpp)
+  PLANT_INST_TRANSLATION = 8,  // instances: the world-to-object translation negated (oracle_core.cpp)
+  PLANT_GI_Z_ONE_DOMAIN = 9   // GI spatial combine_unbiased: Z counts the pixel's domain only
 };
 int g_plant = PLANT_NONE;
 
@@ -517,6 +520,36 @@ GiRes gi_temporal_merge(const GiRes& r, const GiRes& pr, F mcap, F phat_here, ui
   m.W = (phat > 0.0f && M * wsum > 0.0f) ? (1.0f / phat) * (1.0f / M * wsum) : 0.0f;
   m.wsum = wsum;
   m.phat = phat;
+  return m;
+}
+
+// GiReservoir::combine_unbiased (inline_path_tracer.wgsl:333-355): Z counts
+// the M of the domains where the chosen sample's target (at1: r1's pixel,
+// at2: r2's) is positive.  The targets are not evaluated when W = 0 follows
+// anyway (nothing chosen, or a zero weight sum).
+template <typename A1, typename A2>
+GiRes gi_combine_unbiased(const GiRes& r1, const GiRes& r2, uint32_t& rng, A1&& at1, A2&& at2) {
+  F wsum = 0, M = 0, phat = 0;
+  GiRes m;
+  if (res_update(wsum, M, r1.phat * r1.W * r1.M, rng)) {
+    m.x2 = r1.x2;
+    m.rng = r1.rng;
+    phat = r1.phat;
+  }
+  if (res_update(wsum, M, r2.phat * r2.W * r2.M, rng)) {
+    m.x2 = r2.x2;
+    m.rng = r2.rng;
+    phat = r2.phat;
+  }
+  m.M = r1.M + r2.M;
+  m.wsum = wsum;
+  m.phat = phat;
+  F z = 0.0f;
+  if (phat > 0.0f && wsum > 0.0f) {
+    if (at1(m) > 0.0f) z += r1.M;
+    if (g_plant != PLANT_GI_Z_ONE_DOMAIN && at2(m) > 0.0f) z += r2.M;
+  }
+  m.W = (phat > 0.0f && z * wsum > 0.0f) ? (1.0f / phat) * (1.0f / z * wsum) : 0.0f;
   return m;
 }
 
@@ -955,8 +988,74 @@ void pass_gi_temporal(Ctx& c) {
       s.rng[pix] = rng;
     }
     s.gi[pix] = r;
-    s.gi_hist[p.cur][pix] = r;
+    if (p.gi_spatial == 0) s.gi_hist[p.cur][pix] = r;  // else the last spatial pass writes it
   });
+}
+
+// restir_gi_spatial.wgsl:70-165, pass `pass` of p.gi_spatial (unbiased
+// combination, five taps; the radius is 5 % of (width + rows) / 2 whatever the
+// configured pixel radius, and the seeds mix seed * 3 + pass).  Reads s.gi,
+// writes s.gi_tmp, then swaps them; the last pass writes the history.
+void pass_gi_spatial(Ctx& c, uint32_t pass) {
+  const bool last = pass + 1 == c.p.gi_spatial;
+  each_pixel(c.s.npix, c.threads, [&](uint32_t pix) {
+    State& s = c.s;
+    const P& p = c.p;
+    Vtx v;
+    if (!load_vtx(s, pix, v)) {
+      s.gi_tmp[pix] = s.gi[pix];
+      if (last) s.gi_hist[p.cur][pix] = GiRes();
+      return;
+    }
+    GiRes r = s.gi[pix];
+    if (s.alive[pix]) {
+      const uint32_t x = pix % p.width, y = pix / p.width;
+      uint32_t rng = s.rng[pix];
+      const Bsdf bs = make_bsdf(c.sc, v);
+      const V3 T = s.thr_T[pix];
+      const Tex& ct = s.gbuf[p.cur][pix];
+      const F cd = depth_cs(ct.depth);
+      F radius = (F)(p.width + p.rows) / 2.0f * 0.05f;
+      const uint32_t pass_seed = p.seed * 3u + pass;
+      const F r_off = ign_animated(x, y, pass_seed);
+      const uint32_t sh = pass == 0 ? 2u : 1u;
+      const uint32_t aseed = orc::hash_combine(x >> sh, orc::hash_combine(y >> sh, pass_seed));
+      const F a_off = (F)aseed * (1.0f / (F)0xFFFFFFFFu) * TWO_PI;
+      const uint32_t n = 5;  // NUM_SAMPLES
+      for (uint32_t i = 0; i < n; ++i) {
+        const F ang = (F)i * GOLDEN_ANGLE + a_off;
+        const F rad = std::pow((F)i / (F)n, 0.5f) * radius + r_off;
+        const int ox = (int)(rad * std::cos(ang)), oy = (int)(rad * std::sin(ang));
+        const uint32_t nx = mirror_px((int)x + ox, (int)p.width), ny = mirror_px((int)y + oy, (int)p.rows);
+        const uint32_t q = ny * p.width + nx;
+        if (q == pix) continue;
+        const Tex& nt = s.gbuf[p.cur][q];
+        bool valid = nt.depth32 != 0.0f;
+        if (valid) valid = std::fabs(cd - depth_cs(nt.depth)) / cd < 0.1f && dotp(ct.n, nt.n) > 0.906f;
+        if (valid) {
+          GiRes nr = s.gi[q];
+          nr.phat = gi_phat(c.sc, v, bs, T, nr, c.cnt);
+          valid = nr.phat > 0.0f;
+          Vtx vn;
+          const bool has_vn = load_vtx(s, q, vn);
+          vn.x = nt.pos;
+          const V3 Tn = s.thr_T[q];
+          r = gi_combine_unbiased(
+              r, nr, rng, [&](const GiRes& m) { return gi_phat(c.sc, v, bs, T, m, c.cnt); },
+              [&](const GiRes& m) {
+                if (!has_vn) return 0.0f;
+                const Bsdf bn = make_bsdf(c.sc, vn);
+                return gi_phat(c.sc, vn, bn, Tn, m, c.cnt);
+              });
+        }
+        if (!valid) radius = std::max(radius * 0.5f, 3.0f);
+      }
+      s.rng[pix] = rng;
+    }
+    s.gi_tmp[pix] = r;
+    if (last) s.gi_hist[p.cur][pix] = r;
+  });
+  std::swap(c.s.gi, c.s.gi_tmp);
 }
 
 void pass_apply_di(Ctx& c) {
@@ -1240,7 +1339,10 @@ void frame(const Scene& sc, State& s, P& p, int threads, float* out) {
         pass_di_temporal(c);
         if (p.neighbours > 0) pass_di_spatial(c);
       }
-      if (b == 0 && more && p.restir_gi) pass_gi_temporal(c);
+      if (b == 0 && more && p.restir_gi) {
+        pass_gi_temporal(c);
+        for (uint32_t k2 = 0; k2 < p.gi_spatial; ++k2) pass_gi_spatial(c, k2);
+      }
       pass_apply_di(c);
       if (more) pass_apply_gi(c);
     }
@@ -1292,10 +1394,10 @@ extern "C" {
 void* or_restir_create(uint32_t width, uint32_t rows) { return new ors::State(width * rows); }
 void or_restir_destroy(void* st) { delete static_cast<ors::State*>(st); }
 
-// One frame.  cfg20: width, height, row_start, rows, spp, max_bounces, seed,
-// frame_idx, di_candidates, di_mcap, neighbours, gi_mcap, restir_di,
-// restir_gi, firefly, svgf, taa, svgf_history, atrous passes, radius (float
-// bits).  out: the frame's linear mean radiance (rows x width x 3).
+// One frame.  cfg20 (21 words): width, height, row_start, rows, spp,
+// max_bounces, seed, frame_idx, di_candidates, di_mcap, neighbours, gi_mcap,
+// restir_di, restir_gi, firefly, svgf, taa, svgf_history, atrous passes,
+// radius (float bits), GI spatial passes.  out: the frame's linear mean radiance (rows x width x 3).
 // rays[0..1]: closest-hit and shadow rays traced.  attribs: an ocr::Attribs or NULL.
 int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t* mat_ids, const void* mats,
                     uint32_t n_tris, void* bvh, const void* sky, const void* cam, const uint32_t* cfg20,
@@ -1343,6 +1445,7 @@ int or_restir_frame(void* st, const float* pos, const float* nrm, const uint32_t
   p.svgf_history = cfg20[17];
   p.atrous = cfg20[18];
   std::memcpy(&p.radius, &cfg20[19], 4);
+  p.gi_spatial = cfg20[20];
   if (s.npix != p.width * p.rows) return -1;
   p.cam = ors::cam_view(*sc.s.cam, p.width, p.height);
   if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());

@@ -282,6 +282,7 @@ pub struct apt_config {
     pub lds_stack_depth: u32,
     pub seed: u32,
     pub frame_idx: u32,
+    pub gi_spatial_passes: u32,
 }
 
 /// StartRenderData (crates/appearance-render-loop/src/host.rs:85-93), 20 B.

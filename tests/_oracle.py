@@ -380,7 +380,8 @@ class OracleRestir:
         c = np.array([self.width, self.height, self.row_start, self.rows, spp, max_bounces, seed, frame_idx,
                       cfg.di_candidates, cfg.di_mcap, cfg.di_spatial_neighbours, cfg.gi_mcap,
                       cfg.restir_di, cfg.restir_gi, cfg.firefly_filter, cfg.svgf, cfg.taa,
-                      cfg.svgf_history, cfg.svgf_iterations, radius], np.uint32)
+                      cfg.svgf_history, cfg.svgf_iterations, radius, getattr(cfg, "gi_spatial_passes", 0)],
+                     np.uint32)
         out = np.empty((self.rows, self.width, 3), np.float32)
         rays = np.zeros(2, np.uint64)
         fn = self.o.lib.or_restir_frame

@@ -74,6 +74,26 @@ def test_disney_restir_scene_vs_oracle(gpu_available, oracle):
         assert r < 1e-3
 
 
+@pytest.mark.parametrize("scene,passes", [("cornell", 1), ("cornell", 2), ("restir", 3)])
+def test_gi_spatial_reuse_vs_oracle(gpu_available, oracle, scene, passes):
+    """GI spatial reuse (restir_gi_spatial.wgsl; off in the reference's
+    configuration) after GI temporal reuse, one to three passes, frame by
+    frame over an orbit."""
+    s = generate_scene(scene, seed=3)
+    if scene == "cornell":
+        cams = [orbit((0.0, 1.0, 0.0), 3.9, 1.0 + 0.05 * k, 0.04 * k, 40.0) for k in range(4)]
+        w, h, kw = 96, 71, dict()
+    else:
+        cams = [orbit((0.0, 0.0, 0.5), 13.5, 7.5, 0.02 * k, 40.0) for k in range(4)]
+        w, h, kw = 80, 45, dict(SEQUENCE, svgf=0)
+    print(f"[restir] gi spatial {scene} {w}x{h} {passes} passes, 4 frames", flush=True)
+    for f, (g, o, st, rays) in enumerate(run_pair(oracle, s, w, h, 4, cams, gi_spatial_passes=passes, **kw)):
+        r = relmse(g, o)
+        print(f"[restir] frame {f}: relMSE {r:.3e}, mean {g.mean():.4f} vs {o.mean():.4f}", flush=True)
+        assert np.isfinite(g).all()
+        assert r < 1e-3
+
+
 def test_gpu_restir_without_reuse_is_unbiased(gpu_available, oracle):
     s = generate_scene("cornell")
     w, h, bounces, frames = 32, 31, 3, 256

@@ -9,6 +9,9 @@
   combination with its Z count and the solid-angle Jacobian of reused light
   samples), GI temporal reuse, and all of them together; planted mistakes in
   Z, 1/M and the Jacobian make the check fail.
+- GI spatial reuse (restir_gi_spatial.wgsl, off in the reference's
+  configuration) is unbiased where the reconnection Jacobian it leaves out is
+  1: a floor lit by a panorama sky; Z counting one domain fails that check.
 - The a-trous pass keeps a constant image constant, reproduces a linear ramp
   inside a flat region, and spreads an impulse with the closed-form weights
   h(dx) h(dy) / (1 + 55/64), h = (3/8, 1/4, 1/16), at the pass's step size.
@@ -75,21 +78,22 @@ def test_restir_without_reuse_is_unbiased(oracle, name, seed, w, h, bounces):
     assert np.sqrt((zb ** 2).mean()) > 1.6
 
 
-def _plain_reference(oracle, s, w, h, bounces, bvh, batches=8, spp=256):
+def _plain_reference(oracle, s, w, h, bounces, bvh, batches=8, spp=256, sky_pano=None):
     refs = np.array([oracle_render(oracle, s, w, h, spp=spp, max_bounces=bounces, rr_start=3,
-                                   jitter=False, frame_idx=1000 + b, bvh=bvh)[0] for b in range(batches)],
+                                   jitter=False, frame_idx=1000 + b, bvh=bvh, sky_pano=sky_pano)[0]
+                     for b in range(batches)],
                     np.float64)
     return refs.mean(0), refs.var(0, ddof=1) / batches
 
 
-def _reuse_sequences(oracle, s, w, h, bounces, cfg, sequences, frames, bvh):
+def _reuse_sequences(oracle, s, w, h, bounces, cfg, sequences, frames, bvh, sky_pano=None):
     """Mean and variance of the mean of the last frame of `sequences`
     independent frame sequences (a fresh reservoir history each, seeds far
     apart) on the scene's static camera."""
     acc = np.zeros((h, w, 3))
     acc2 = np.zeros((h, w, 3))
     for k in range(sequences):
-        r = OracleRestir(oracle, s, w, h, bvh=bvh)
+        r = OracleRestir(oracle, s, w, h, bvh=bvh, sky_pano=sky_pano)
         for f in range(frames):
             img = r.frame(cfg, frame_idx=f, seed=7919 * k + 13, max_bounces=bounces, threads=1)
         img = img.astype(np.float64)
@@ -171,6 +175,71 @@ def test_restir_with_all_reuse_is_unbiased_on_the_many_light_scene(oracle):
     assert zrms < 1.5
     assert tail < 5e-3
     assert power > 1.6
+
+
+GI_SPATIAL = dict(restir_di=0, restir_gi=1, gi_mcap=2, gi_spatial_passes=2)
+
+
+@pytest.fixture(scope="module")
+def sky_plane_reuse(oracle):
+    """A grey Lambertian floor under a cosine-weighted panorama sky (no sun), seen from 2 m:
+    every indirect sample is a sky point 1000 units out, so a neighbour's
+    sample seen from this pixel is the same direction to within 1e-3 and the
+    reconnection Jacobian that GI spatial reuse leaves out
+    (inline_path_tracer.wgsl:321) is 1 to that accuracy."""
+    from tests.test_sky_panorama import cos_sky, plane_under_sky
+    s = plane_under_sky(0.5)
+    bvh = oracle.bvh(np.ascontiguousarray(s.positions, np.float32))
+    pano = cos_sky()
+    return s, bvh, pano, _plain_reference(oracle, s, 24, 23, 2, bvh, batches=8, spp=128, sky_pano=pano)
+
+
+def test_restir_gi_spatial_reuse_is_unbiased_under_a_sky(oracle, sky_plane_reuse):
+    """Two GI spatial passes after GI temporal reuse (M cap engaged), two
+    frames per sequence: the unbiased combination's Z counts both domains."""
+    s, bvh, pano, (rm, rv) = sky_plane_reuse
+    m, var = _reuse_sequences(oracle, s, 24, 23, 2, default_config(firefly_filter=0, **GI_SPATIAL), 256, 2,
+                              bvh, pano)
+    zrms, tail, power = _reuse_z(m, var, rm, rv)
+    rel = abs(m.mean() - rm.mean()) / rm.mean()
+    print(f"[restir] sky plane gi spatial: mean {m.mean():.6f} plain {rm.mean():.6f} rel {rel:.2e} "
+          f"z rms {zrms:.2f}")
+    assert rel < 3e-3
+    assert zrms < 1.5
+    assert tail < 5e-3
+    assert power > 1.6
+
+
+def test_restir_gi_spatial_planted_z_error_is_caught(oracle, sky_plane_reuse):
+    """Z counting the pixel's own domain only (W too large by (M1 + M2) / M1)."""
+    from tests.test_oracle_reservoir import plant
+    s, bvh, pano, (rm, rv) = sky_plane_reuse
+    with plant(oracle, 9):
+        m, var = _reuse_sequences(oracle, s, 24, 23, 2, default_config(firefly_filter=0, **GI_SPATIAL), 256, 2,
+                                  bvh, pano)
+    zrms, _, _ = _reuse_z(m, var, rm, rv)
+    print(f"[restir] sky plane gi spatial with plant 9: z rms {zrms:.2f}")
+    assert zrms > 2.5
+
+
+def test_restir_gi_spatial_passes_change_the_frame(oracle):
+    """On the Cornell box the passes run (the frame differs from GI temporal
+    alone, same seeds) and stay finite; zero passes is GI temporal exactly."""
+    s = generate_scene("cornell")
+    imgs = {}
+    for n in (0, 1, 3):
+        r = OracleRestir(oracle, s, 24, 23)
+        cfg = default_config(firefly_filter=0, **dict(GI_SPATIAL, gi_spatial_passes=n))
+        for f in range(2):
+            imgs[n] = r.frame(cfg, frame_idx=f, seed=5, max_bounces=2, threads=1)
+    r = OracleRestir(oracle, s, 24, 23)
+    cfg = default_config(firefly_filter=0, restir_di=0, restir_gi=1, gi_mcap=2)
+    for f in range(2):
+        base = r.frame(cfg, frame_idx=f, seed=5, max_bounces=2, threads=1)
+    np.testing.assert_array_equal(imgs[0], base)
+    for n in (1, 3):
+        assert np.isfinite(imgs[n]).all()
+        assert not np.array_equal(imgs[n], base)
 
 
 def _flat(h, w):
