@@ -1106,6 +1106,7 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_spatial(RestirParams p, SceneV
         float wsum = 0.0f, M = 0.0f, phat = 0.0f;
         GiRes c;
         c.x2_rng = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool took_n = false;
         if (res_update(wsum, M, r.mwsp.w * r.mwsp.y * r.mwsp.x, rng)) {
           c.x2_rng = r.x2_rng;
           phat = r.mwsp.w;
@@ -1113,11 +1114,13 @@ __global__ __launch_bounds__(BLOCK8) void k_rs_gi_spatial(RestirParams p, SceneV
         if (res_update(wsum, M, nr.mwsp.w * nr.mwsp.y * nr.mwsp.x, rng)) {
           c.x2_rng = nr.x2_rng;
           phat = nr.mwsp.w;
+          took_n = true;
         }
         M = r.mwsp.x + nr.mwsp.x;
         float z = 0.0f;
         if (phat > 0.0f && wsum > 0.0f) {
-          if (gi_phat<W, S, INST>(sc, st, v, make_bsdf(sc, v), T, c, b.counters) > 0.0f) z += r.mwsp.x;
+          // the neighbour's sample at this pixel is nr's phat, just evaluated
+          if (took_n || gi_phat<W, S, INST>(sc, st, v, make_bsdf(sc, v), T, c, b.counters) > 0.0f) z += r.mwsp.x;
           Vtx vn;
           if (load_vtx(sc, b, q, vn)) {
             vn.x = f4v(ng.pos_depth);  // the neighbour's G-buffer position

@@ -526,11 +526,13 @@ GiRes gi_temporal_merge(const GiRes& r, const GiRes& pr, F mcap, F phat_here, ui
 // GiReservoir::combine_unbiased (inline_path_tracer.wgsl:333-355): Z counts
 // the M of the domains where the chosen sample's target (at1: r1's pixel,
 // at2: r2's) is positive.  The targets are not evaluated when W = 0 follows
-// anyway (nothing chosen, or a zero weight sum).
+// anyway (nothing chosen, or a zero weight sum), and r2's phat must be its
+// target at r1's pixel (at1 of r2's sample is then known).
 template <typename A1, typename A2>
 GiRes gi_combine_unbiased(const GiRes& r1, const GiRes& r2, uint32_t& rng, A1&& at1, A2&& at2) {
   F wsum = 0, M = 0, phat = 0;
   GiRes m;
+  bool took2 = false;
   if (res_update(wsum, M, r1.phat * r1.W * r1.M, rng)) {
     m.x2 = r1.x2;
     m.rng = r1.rng;
@@ -540,13 +542,16 @@ GiRes gi_combine_unbiased(const GiRes& r1, const GiRes& r2, uint32_t& rng, A1&& 
     m.x2 = r2.x2;
     m.rng = r2.rng;
     phat = r2.phat;
+    took2 = true;
   }
   m.M = r1.M + r2.M;
   m.wsum = wsum;
   m.phat = phat;
   F z = 0.0f;
   if (phat > 0.0f && wsum > 0.0f) {
-    if (at1(m) > 0.0f) z += r1.M;
+    // r2's sample at r1's pixel is r2.phat (the caller evaluated it there,
+    // the same target of the same x2 and replay seed), positive here.
+    if (took2 || at1(m) > 0.0f) z += r1.M;
     if (g_plant != PLANT_GI_Z_ONE_DOMAIN && at2(m) > 0.0f) z += r2.M;
   }
   m.W = (phat > 0.0f && z * wsum > 0.0f) ? (1.0f / phat) * (1.0f / z * wsum) : 0.0f;
