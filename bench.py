@@ -238,6 +238,8 @@ def main() -> int:
     ap.add_argument("--policy", type=int, default=0,
                     help="serving: 0 = one-batch requests shortest first, then the rest in "
                          "arrival order; 1 = FIFO; 2 = shortest remaining work first")
+    ap.add_argument("--gi-spatial-passes", type=int, default=0,
+                    help="restir: ReSTIR GI spatial passes (0, as the reference ships its configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bvh-stress", action="store_true",
                     help="disney: skip the traversal leg reported beside the headline")
@@ -567,7 +569,8 @@ def bench_disney(args) -> int:
                                f"analytic sky + sun, {W}x{H} (1080 rounded up to 64-row blocks), {SPP} spp, "
                                f"depth {DEPTH}, RR from bounce {RR}, wavefront path tracer",
                    "width": W, "height": H, "spp": SPP, "max_bounces": DEPTH,
-                   "bvh_width": args.width, "lds_stack": args.lds_stack},
+                   "bvh_width": args.width, "lds_stack": args.lds_stack,
+                   "gi_spatial_passes": args.gi_spatial_passes},
         "mrays_s": rays / wall / 1e6,
         "device_ms_per_frame": dev_ms,
         "stage_ms_per_frame": {"extend": ext_ms, "shade": shade_ms, "shadow": shadow_ms,
@@ -648,7 +651,8 @@ def bench_restir(args) -> int:
     W, H = 1920, 1088  # RGBA8 block layout: slab heights are multiples of 64
     log("generating the many-light scene")
     scene = generate_scene("restir", seed=3)
-    pt = PathTracerGpu(restir_config(bvh_width=args.width, lds_stack_depth=args.lds_stack))
+    pt = PathTracerGpu(restir_config(bvh_width=args.width, lds_stack_depth=args.lds_stack,
+                                     gi_spatial_passes=args.gi_spatial_passes))
     pt.set_scene(scene)
     k = 0
     for _ in range(max(1, args.warmup)):
@@ -678,7 +682,8 @@ def bench_restir(args) -> int:
         "config": {"workload": f"restir: disney scene + 4096 emissive triangles (528,386 triangles), {W}x{H}, "
                                "1 spp per frame, camera orbit, ReSTIR DI (32 candidates, M cap 20, 5 taps in "
                                "30 px) + GI reuse, depth 2, firefly filter, SVGF 5 passes / 4 frames",
-                   "bvh_width": args.width, "lds_stack": args.lds_stack},
+                   "bvh_width": args.width, "lds_stack": args.lds_stack,
+                   "gi_spatial_passes": args.gi_spatial_passes},
         "mrays_s": rays / wall / 1e6,
         "device_ms_per_frame": sum(dev_ms) / len(dev_ms),
     }
